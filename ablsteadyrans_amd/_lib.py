@@ -1,0 +1,113 @@
+"""ctypes binding of ``libgust_amd.so`` (the C-ABI in ``include/gust_amd.h``).
+
+The library is required: importing this module raises if it is missing, and
+there is no fallback path of any kind.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgust_amd.so")
+
+
+class Array4(C.Structure):
+    """``gust_array4``: amrex::Array4 (i fastest, begin inclusive, end exclusive)."""
+
+    _fields_ = [
+        ("p", C.c_void_p),
+        ("jstride", C.c_int64),
+        ("kstride", C.c_int64),
+        ("nstride", C.c_int64),
+        ("begin", C.c_int32 * 3),
+        ("end", C.c_int32 * 3),
+        ("ncomp", C.c_int32),
+    ]
+
+
+class GeometryC(C.Structure):
+    """``gust_geometry``: cell-centred domain box, periodicity, spacing."""
+
+    _fields_ = [
+        ("lo", C.c_int32 * 3),
+        ("hi", C.c_int32 * 3),
+        ("periodic", C.c_int32 * 3),
+        ("dx", C.c_double * 3),
+    ]
+
+
+class LineOptionsC(C.Structure):
+    """``gust_line_options``."""
+
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("ghosts", C.c_int32),
+        ("overlap", C.c_int32),
+        ("order", C.c_int32),
+        ("alternate", C.c_int32),
+    ]
+
+
+class LineSystemC(C.Structure):
+    """``gust_line_system``."""
+
+    _fields_ = [
+        ("loc", C.c_int32),
+        ("geom", GeometryC),
+        ("A", Array4),
+        ("ap", Array4),
+        ("rhs", Array4),
+        ("delta", Array4),
+    ]
+
+
+def _load() -> C.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(there is no non-HIP fallback)")
+    return C.CDLL(LIB_PATH)
+
+
+lib = _load()
+
+_P = C.POINTER
+_sig = {
+    "gust_last_error": (C.c_char_p, []),
+    "gust_version": (C.c_char_p, []),
+    "gust_zline_jacobi": (C.c_int, [C.c_int, _P(Array4), _P(Array4), _P(Array4), C.c_int, _P(GeometryC),
+                                    _P(Array4), _P(C.c_double), C.c_void_p]),
+    "gust_zline_jacobi_f32": (C.c_int, [C.c_int, _P(Array4), _P(Array4), _P(Array4), C.c_int, _P(GeometryC),
+                                        _P(Array4), _P(C.c_double), C.c_void_p]),
+    "gust_zline_jacobi_batched_async": (C.c_int, [C.c_int, _P(LineSystemC), C.c_int, C.c_void_p, C.c_void_p]),
+    "gust_zline_jacobi_batched_async_f32": (C.c_int, [C.c_int, _P(LineSystemC), C.c_int, C.c_void_p,
+                                                      C.c_void_p]),
+    "gust_zline_lusgs": (C.c_int, [C.c_int, _P(Array4), _P(Array4), _P(Array4), C.c_int, _P(GeometryC),
+                                   _P(Array4), _P(LineOptionsC), _P(C.c_double), C.c_void_p]),
+    "gust_line_sweeps": (C.c_int, [_P(LineOptionsC), C.c_int, _P(Array4), _P(Array4), _P(Array4), C.c_int,
+                                   _P(GeometryC), _P(Array4), _P(C.c_double), C.c_void_p]),
+    "gust_check_line_options": (C.c_int, [_P(LineOptionsC)]),
+    "gust_jacobi_plan_create": (C.c_int, [C.c_int, _P(LineSystemC), C.c_int, _P(C.c_void_p), C.c_void_p]),
+    "gust_jacobi_plan_run": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "gust_jacobi_plan_sweep": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]),
+    "gust_jacobi_plan_info": (C.c_int, [C.c_void_p, _P(C.c_int64), _P(C.c_int32)]),
+    "gust_jacobi_plan_destroy": (None, [C.c_void_p]),
+}
+for _name, (_res, _args) in _sig.items():
+    _f = getattr(lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+
+class GustError(RuntimeError):
+    """A refused call; the message is the reference's "gust: <op>: ..." text."""
+
+
+def check(status: int) -> None:
+    if status != 0:
+        raise GustError(lib.gust_last_error().decode())
+
+
+def exported_symbols() -> list[str]:
+    return list(_sig)

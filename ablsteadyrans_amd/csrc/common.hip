@@ -1,0 +1,112 @@
+// common.hip -- host plumbing of libgust_amd (errors, workspaces, device info).
+#include "common.hpp"
+
+#include <map>
+#include <tuple>
+
+namespace gust_amd {
+
+namespace {
+thread_local std::string t_err;
+}
+
+void set_error(const char* fmt, ...)
+{
+    char buf[2048];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    t_err = buf;
+}
+
+void clear_error() { t_err.clear(); }
+
+const char* loc_words(int loc)
+{
+    switch (loc) {
+    case GUST_LOC_CELL: return "cell centres";
+    case GUST_LOC_XFACE: return "x-faces";
+    case GUST_LOC_YFACE: return "y-faces";
+    case GUST_LOC_ZFACE: return "z-faces";
+    default: return "?";
+    }
+}
+
+namespace {
+
+struct Buf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+std::mutex g_ws_mutex;
+std::map<std::tuple<int, hipStream_t, int>, Buf> g_ws;
+std::map<int, Buf> g_pinned;
+
+}   // namespace
+
+void* Workspace::get(hipStream_t s, int slot, size_t bytes)
+{
+    int dev = 0;
+    GUST_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_ws_mutex);
+    Buf& b = g_ws[std::make_tuple(dev, s, slot)];
+    if (b.cap < bytes) {
+        if (b.p) {
+            GUST_HIP(hipStreamSynchronize(s));
+            GUST_HIP(hipFree(b.p));
+            b.p = nullptr;
+            b.cap = 0;
+        }
+        const size_t want = bytes + bytes / 8 + 256;
+        GUST_HIP(hipMalloc(&b.p, want));
+        b.cap = want;
+    }
+    return b.p;
+}
+
+void* Workspace::get_host_pinned(int slot, size_t bytes)
+{
+    std::lock_guard<std::mutex> lk(g_ws_mutex);
+    Buf& b = g_pinned[slot];
+    if (b.cap < bytes) {
+        if (b.p) {
+            GUST_HIP(hipDeviceSynchronize());
+            GUST_HIP(hipHostFree(b.p));
+            b.p = nullptr;
+            b.cap = 0;
+        }
+        const size_t want = bytes + 256;
+        GUST_HIP(hipHostMalloc(&b.p, want, hipHostMallocDefault));
+        b.cap = want;
+    }
+    return b.p;
+}
+
+const DeviceInfo& device_info()
+{
+    static DeviceInfo info = [] {
+        DeviceInfo d;
+        int dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess) {
+            hipDeviceProp_t p;
+            if (hipGetDeviceProperties(&p, dev) == hipSuccess) {
+                d.cus = p.multiProcessorCount;
+                d.lds_per_block = p.sharedMemPerBlock;
+            }
+        }
+        return d;
+    }();
+    return info;
+}
+
+}   // namespace gust_amd
+
+extern "C" {
+
+const char* gust_last_error(void) { return gust_amd::t_err.c_str(); }
+
+const char* gust_version(void) { return "gust_amd 0.1 (gfx950)"; }
+
+}

@@ -1,0 +1,168 @@
+// common.hpp -- shared host/device plumbing of libgust_amd: error reporting,
+// view checks, the per-(device, stream) workspace, and small device helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "gust_amd.h"
+
+namespace gust_amd {
+
+// ---------------------------------------------------------------------------
+// Errors: the C-ABI never throws. A failing entry point stores the message
+// (the reference's "gust: <op>: ..." text) per thread and returns nonzero.
+
+void set_error(const char* fmt, ...);
+void clear_error();
+
+struct Fail {
+    int code;
+};
+
+#define GUST_FAIL(...)                        \
+    do {                                      \
+        ::gust_amd::set_error(__VA_ARGS__);   \
+        throw ::gust_amd::Fail{1};            \
+    } while (0)
+
+#define GUST_HIP(expr)                                                                                    \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess) {                                                                           \
+            ::gust_amd::set_error("gust: HIP error %s at %s:%d (%s)", hipGetErrorString(e_), __FILE__,     \
+                                  __LINE__, #expr);                                                       \
+            throw ::gust_amd::Fail{2};                                                                    \
+        }                                                                                                 \
+    } while (0)
+
+// Runs f; maps a Fail (or any C++ exception) to a nonzero status.
+template <class F>
+int guarded(F&& f) noexcept
+{
+    try {
+        clear_error();
+        f();
+        return 0;
+    } catch (const Fail& e) {
+        return e.code;
+    } catch (const std::exception& e) {
+        set_error("gust: internal error: %s", e.what());
+        return 3;
+    } catch (...) {
+        set_error("gust: internal error");
+        return 3;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Boxes and views.
+
+struct IBox {
+    int lo[3], hi[3];   // inclusive
+    int len(int d) const { return hi[d] - lo[d] + 1; }
+    int64_t npts() const { return int64_t(len(0)) * len(1) * len(2); }
+};
+
+// The domain box converted to a field location (bit d of loc = nodal in d).
+inline IBox loc_box(const gust_geometry& g, int loc)
+{
+    IBox b;
+    for (int d = 0; d < 3; ++d) {
+        b.lo[d] = g.lo[d];
+        b.hi[d] = g.hi[d] + ((loc >> d) & 1);
+    }
+    return b;
+}
+
+inline bool valid_loc(int loc)
+{
+    return loc == GUST_LOC_CELL || loc == GUST_LOC_XFACE || loc == GUST_LOC_YFACE || loc == GUST_LOC_ZFACE;
+}
+
+const char* loc_words(int loc);
+
+// True when the view's box [begin, end) covers b with at least ncomp comps.
+inline bool covers(const gust_array4* v, const IBox& b, int ncomp)
+{
+    if (!v || !v->p || v->ncomp < ncomp) { return false; }
+    for (int d = 0; d < 3; ++d) {
+        if (v->begin[d] > b.lo[d] || v->end[d] - 1 < b.hi[d]) { return false; }
+    }
+    return true;
+}
+
+inline IBox grow(IBox b, int gx, int gy, int gz)
+{
+    b.lo[0] -= gx; b.hi[0] += gx;
+    b.lo[1] -= gy; b.hi[1] += gy;
+    b.lo[2] -= gz; b.hi[2] += gz;
+    return b;
+}
+
+// Element offset of (i, j, k, n) in a view.
+inline int64_t offset_of(const gust_array4& v, int i, int j, int k, int n = 0)
+{
+    return int64_t(i - v.begin[0]) + int64_t(j - v.begin[1]) * v.jstride + int64_t(k - v.begin[2]) * v.kstride +
+           int64_t(n) * v.nstride;
+}
+
+// ---------------------------------------------------------------------------
+// Workspace: grow-only device buffers keyed by (device, stream, slot). A
+// buffer is reused by later calls on the same stream (stream order keeps
+// them apart); growing frees the old buffer after synchronising the stream.
+
+class Workspace {
+public:
+    static void* get(hipStream_t s, int slot, size_t bytes);
+    static void* get_host_pinned(int slot, size_t bytes);
+};
+
+// Device properties used for launch shapes.
+struct DeviceInfo {
+    int cus = 256;
+    size_t lds_per_block = 65536;
+};
+const DeviceInfo& device_info();
+
+}   // namespace gust_amd
+
+// ---------------------------------------------------------------------------
+// Device helpers.
+
+namespace gust_amd::dev {
+
+// Block index remapped so that the blocks the hardware places on one XCD
+// (round-robin, block b on XCD b % 8) cover one contiguous range of logical
+// tiles: neighbouring column rows then share an L2.
+__device__ __forceinline__ int xcd_remap(int b, int nblocks)
+{
+    constexpr int kXcd = 8;
+    const int q = nblocks / kXcd, r = nblocks % kXcd;
+    const int x = b % kXcd, idx = b / kXcd;
+    return x * q + (x < r ? x : r) + idx;
+}
+
+// |x| as ordered bits: max over these is max |x| with NaN winning.
+__device__ __forceinline__ unsigned long long abs_bits(double x)
+{
+    return static_cast<unsigned long long>(__double_as_longlong(x)) & 0x7fffffffffffffffull;
+}
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
+{
+    for (int off = 32; off >= 1; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+}   // namespace gust_amd::dev

@@ -1,0 +1,936 @@
+// line_sweeps.hip -- batched z-line Jacobi and line LU-SGS sweeps for gfx950.
+//
+// Reference behaviour (GUST):
+//   Source/LineSolve_K.H:22-45     the exact column solve (Thomas, x/y
+//                                  neighbours of the iterate on the rhs)
+//   Source/Predictor.cpp:87-127    zline_jacobi: sweeps of independent columns
+//   Source/LineSweeps.cpp:181-305  line LU-SGS, lagged / zero ghosts
+//   Source/LineSweeps.cpp:361-413  ownership, sweep region, checks
+//
+// Design (MI355X):
+//  * Jacobi: one 64-lane wavefront per 64 consecutive columns (flattened
+//    (i, j) with i fastest), lane = column, loop over k. In the Array4
+//    layout (i fastest) every k-step is one coalesced 512 B (FP64) load per
+//    operand across the wave. The Thomas factor c' lives in LDS (64 x nz
+//    reals per wave); e' is staged in the output array and overwritten by
+//    the back substitution (the re-read hits L2). Many systems (the five
+//    variables of an iteration, many tiles) share one launch through a
+//    device table; blocks are remapped so each XCD sweeps a contiguous range
+//    of column rows (the j +- 1 neighbour reads then hit its L2).
+//  * LU-SGS: one workgroup per system executes the diagonal wavefronts
+//    i + j = s (or i - j for the j-reversed pair) with a workgroup barrier
+//    between planes; the sequential order performs the same operations on
+//    the same values, so both orders return the same bits.
+//  * Arithmetic follows the reference expression by expression and the
+//    library is compiled without floating-point contraction, so FP64 results
+//    equal the CPU build bit for bit.
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+namespace gust_amd {
+
+namespace {
+
+constexpr int kWave = 64;
+
+// One system of a Jacobi launch. Pointers are at the valid box's low corner.
+template <typename T>
+struct JacobiSys {
+    const T* A;
+    int64_t a_j, a_k, a_n;
+    const T* ap;
+    int64_t p_j, p_k;
+    const T* r;
+    int64_t r_j, r_k;
+    T* d;            // the caller's delta (valid lo corner)
+    int64_t d_j, d_k;
+    T* s;            // scratch iterate (compact)
+    int64_t s_j, s_k;
+    int nx, ny, nz;  // valid extents at the field's location
+    int Lx, Ly;      // periodic shift in cells along x / y, 0 if not periodic
+    int tile0;       // first column tile of this system
+    int pad;
+};
+
+template <typename T>
+__device__ __forceinline__ T ldg(const T* p)
+{
+    return __builtin_nontemporal_load(p);
+}
+
+// Which system holds column tile t (tile0 ascending).
+template <typename S>
+__device__ __forceinline__ int find_sys(const S* __restrict__ sys, int nsys, int t)
+{
+    int lo = 0, hi = nsys - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sys[mid].tile0 <= t) { lo = mid; } else { hi = mid - 1; }
+    }
+    return lo;
+}
+
+// One z-line Jacobi sweep over every system of the table.
+//   kFirst: the input iterate is zero (the first sweep after the zeroing of
+//           check_line_system); its neighbour values are 0, not read.
+//   kLast:  record max |new - old| per system into changes (as ordered bits).
+//   dst_scratch: 1 = write the scratch iterate, read the caller's delta;
+//                0 = the reverse.
+//   cglob: c' in global memory ([tile][k][lane]) when the LDS cannot hold
+//          64 x nz reals (null: c' in dynamic LDS).
+template <typename T, bool kFirst, bool kLast>
+__global__ __launch_bounds__(kWave) void zjacobi_sweep_kernel(const JacobiSys<T>* __restrict__ table, int nsys,
+                                                              int total_tiles, int dst_scratch,
+                                                              unsigned long long* __restrict__ changes,
+                                                              T* __restrict__ cglob, int nzmax)
+{
+    extern __shared__ unsigned char lds_raw[];
+    const int lane = threadIdx.x;
+    const int tile = dev::xcd_remap(blockIdx.x, total_tiles);
+    T* cl = cglob ? cglob + size_t(tile) * kWave * size_t(nzmax) : reinterpret_cast<T*>(lds_raw);
+    const int si = find_sys(table, nsys, tile);
+    const JacobiSys<T> S = table[si];
+    const int c = (tile - S.tile0) * kWave + lane;
+    const int ncols = S.nx * S.ny;
+    unsigned long long chg = 0;
+    if (c < ncols) {
+        const int i = c % S.nx, j = c / S.nx;
+        const T* src = dst_scratch ? S.d : S.s;
+        T* dst = dst_scratch ? S.s : S.d;
+        const int64_t src_j = dst_scratch ? S.d_j : S.s_j, src_k = dst_scratch ? S.d_k : S.s_k;
+        const int64_t dst_j = dst_scratch ? S.s_j : S.d_j, dst_k = dst_scratch ? S.s_k : S.d_k;
+        // Neighbour columns of the input iterate: in the domain, across a
+        // periodic seam (the image; nodal or cell), or 0 outside.
+        int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1;
+        bool okm = true, okp = true, okjm = true, okjp = true;
+        if (im < 0) { if (S.Lx) { im += S.Lx; } else { okm = false; im = i; } }
+        if (ip > S.nx - 1) { if (S.Lx) { ip -= S.Lx; } else { okp = false; ip = i; } }
+        if (jm < 0) { if (S.Ly) { jm += S.Ly; } else { okjm = false; jm = j; } }
+        if (jp > S.ny - 1) { if (S.Ly) { jp -= S.Ly; } else { okjp = false; jp = j; } }
+        const T* a = S.A + i + int64_t(j) * S.a_j;
+        const T* pap = S.ap + i + int64_t(j) * S.p_j;
+        const T* pr = S.r + i + int64_t(j) * S.r_j;
+        const T* sm = src + im + int64_t(j) * src_j;
+        const T* sp = src + ip + int64_t(j) * src_j;
+        const T* sjm = src + i + int64_t(jm) * src_j;
+        const T* sjp = src + i + int64_t(jp) * src_j;
+        T* out = dst + i + int64_t(j) * dst_j;
+        const int64_t ak = S.a_k, an = S.a_n;
+        const int nz = S.nz;
+        T cprev = T(0), eprev = T(0);
+        // Forward elimination (Source/LineSolve_K.H:29-42).
+#pragma unroll 4
+        for (int k = 0; k < nz; ++k) {
+            const T* ak0 = a + k * ak;
+            const T axm = ldg(ak0 + 1 * an), axp = ldg(ak0 + 2 * an), aym = ldg(ak0 + 3 * an),
+                    ayp = ldg(ak0 + 4 * an), azm = ldg(ak0 + 5 * an), azp = ldg(ak0 + 6 * an);
+            const T dgk = ldg(pap + k * S.p_k), rr = ldg(pr + k * S.r_k);
+            T vm = T(0), vp = T(0), vjm = T(0), vjp = T(0);
+            if (!kFirst) {
+                const T tm = sm[k * src_k], tp = sp[k * src_k], tjm = sjm[k * src_k], tjp = sjp[k * src_k];
+                vm = okm ? tm : T(0);
+                vp = okp ? tp : T(0);
+                vjm = okjm ? tjm : T(0);
+                vjp = okjp ? tjp : T(0);
+            }
+            const T rk = rr + axm * vm + axp * vp + aym * vjm + ayp * vjp;
+            const T lower = -azm, upper = -azp;
+            T ck, ek;
+            if (k == 0) {
+                const T m = dgk;
+                ck = upper / m;
+                ek = rk / m;
+            } else {
+                const T m = dgk - lower * cprev;
+                ck = upper / m;
+                ek = (rk - lower * eprev) / m;
+            }
+            cl[k * kWave + lane] = ck;
+            out[k * dst_k] = ek;
+            cprev = ck;
+            eprev = ek;
+        }
+        // Back substitution (Source/LineSolve_K.H:43-44); e' is re-read from
+        // the output column, c' from LDS.
+        T o = eprev;
+        if (kLast) {
+            const T old = kFirst ? T(0) : src[i + int64_t(j) * src_j + (nz - 1) * src_k];
+            const unsigned long long b = dev::abs_bits(double(o - old));
+            chg = b > chg ? b : chg;
+        }
+#pragma unroll 4
+        for (int k = nz - 2; k >= 0; --k) {
+            const T ek = out[k * dst_k];
+            const T ck = cl[k * kWave + lane];
+            o = ek - ck * o;
+            out[k * dst_k] = o;
+            if (kLast) {
+                const T old = kFirst ? T(0) : src[i + int64_t(j) * src_j + k * src_k];
+                const unsigned long long b = dev::abs_bits(double(o - old));
+                chg = b > chg ? b : chg;
+            }
+        }
+    }
+    if (kLast) {
+        chg = dev::wave_max_u64(chg);
+        if (lane == 0 && chg) { atomicMax(changes + si, chg); }
+    }
+}
+
+// The copy of a periodic seam face OverrideSync keeps (Source/Predictor.cpp:125
+// with AMReX's owner mask: the low copy owns): delta(hi seam) = delta(lo seam).
+template <typename T>
+__global__ void seam_sync_kernel(T* d, int64_t d_j, int64_t d_k, int nx, int ny, int nz, int axis, int low_owner)
+{
+    const int64_t n = axis == 0 ? int64_t(ny) * nz : int64_t(nx) * nz;
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        int64_t lo, hi;
+        if (axis == 0) {
+            const int j = int(t % ny), k = int(t / ny);
+            lo = int64_t(j) * d_j + k * d_k;
+            hi = lo + (nx - 1);
+        } else {
+            const int i = int(t % nx), k = int(t / nx);
+            lo = i + int64_t(k) * d_k;
+            hi = lo + int64_t(ny - 1) * d_j;
+        }
+        if (low_owner) { d[hi] = d[lo]; } else { d[lo] = d[hi]; }
+    }
+}
+
+// Zero the points of a view's box outside its valid box [vlo, vhi] (local
+// indices): the ghosts of delta. The valid points are all overwritten by the
+// last sweep. A fill kernel, not a memset, so the call can live inside a
+// captured graph.
+template <typename T>
+__global__ void zero_ghosts_kernel(T* p, int64_t jstride, int64_t kstride, int nx, int ny, int nz, int vlo0,
+                                   int vhi0, int vlo1, int vhi1, int vlo2, int vhi2)
+{
+    const int64_t n = int64_t(nx) * ny * nz;
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        const int i = int(t % nx);
+        const int64_t r = t / nx;
+        const int j = int(r % ny), k = int(r / ny);
+        if (i >= vlo0 && i <= vhi0 && j >= vlo1 && j <= vhi1 && k >= vlo2 && k <= vhi2) { continue; }
+        p[i + j * jstride + k * kstride] = T(0);
+    }
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) { p[t] = 0ull; }
+}
+
+__global__ void bits_to_double_kernel(const unsigned long long* bits, double* out, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) { out[t] = __longlong_as_double(static_cast<long long>(bits[t])); }
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+
+// check_line_system (Source/LineSweeps.cpp:391-413) for a one-box domain.
+void check_line_system(const char* op, int loc, const gust_array4* A, const gust_array4* ap, const gust_array4* rhs,
+                       int sweeps, const gust_geometry* g, const gust_array4* delta, IBox& V)
+{
+    if (!g) { GUST_FAIL("gust: %s: geometry is null", op); }
+    if (sweeps < 1) { GUST_FAIL("gust: %s: sweeps must be >= 1 (got %d)", op, sweeps); }
+    if (!valid_loc(loc)) { GUST_FAIL("gust: %s: unknown location %d", op, loc); }
+    for (int d = 0; d < 3; ++d) {
+        if (g->hi[d] < g->lo[d]) { GUST_FAIL("gust: %s: empty domain", op); }
+    }
+    if (g->periodic[2]) { GUST_FAIL("gust: %s: the domain cannot be periodic in z", op); }
+    V = loc_box(*g, loc);
+    if (!A || A->ncomp != GUST_N_COEFS) { GUST_FAIL("gust: %s: A needs the implicit operator's components", op); }
+    if (!covers(A, V, GUST_N_COEFS)) { GUST_FAIL("gust: %s: A is not on the valid box at the %s", op, loc_words(loc)); }
+    if (!covers(ap, V, 1)) { GUST_FAIL("gust: %s: ap is not on the valid box at the %s", op, loc_words(loc)); }
+    if (!covers(rhs, V, 1)) { GUST_FAIL("gust: %s: rhs is not on the valid box at the %s", op, loc_words(loc)); }
+    if (!covers(delta, grow(V, 1, 1, 1), 1)) {
+        GUST_FAIL("gust: %s: delta needs at least 1 ghost cells at the %s", op, loc_words(loc));
+    }
+}
+
+// delta = 0 outside its valid box V (check_line_system's setVal; the valid
+// points are overwritten by the sweeps).
+template <typename T>
+void zero_ghosts(const gust_array4& v, const IBox& V, hipStream_t st)
+{
+    const int nx = v.end[0] - v.begin[0], ny = v.end[1] - v.begin[1], nz = v.end[2] - v.begin[2];
+    const int64_t n = int64_t(nx) * ny * nz;
+    if (n <= 0) { return; }
+    const int blocks = int(std::min<int64_t>((n + 255) / 256, 8192));
+    zero_ghosts_kernel<T><<<blocks, 256, 0, st>>>(static_cast<T*>(v.p), v.jstride, v.kstride, nx, ny, nz,
+                                                  V.lo[0] - v.begin[0], V.hi[0] - v.begin[0], V.lo[1] - v.begin[1],
+                                                  V.hi[1] - v.begin[1], V.lo[2] - v.begin[2], V.hi[2] - v.begin[2]);
+    GUST_HIP(hipGetLastError());
+}
+
+template <typename T>
+T* at_lo(const gust_array4& v, const IBox& V, int n = 0)
+{
+    return static_cast<T*>(v.p) + offset_of(v, V.lo[0], V.lo[1], V.lo[2], n);
+}
+
+// A batched Jacobi plan: the device table of systems, the scratch iterate
+// and the change accumulators, built once. Running it enqueues kernels only
+// (no host synchronisation, no copies, no memsets), so a caller can capture
+// it in a HIP graph and replay it every iteration.
+struct JacobiPlanBase {
+    virtual ~JacobiPlanBase() = default;
+    virtual void run(int sweeps, double* changes_dev, hipStream_t st) = 0;
+    virtual void sweep(bool first, bool last, int dst_scratch, hipStream_t st) = 0;
+    virtual void change_host(double* out, int n, hipStream_t st) = 0;
+    int nsys = 0, tiles = 0, nzmax = 1;
+    int64_t unknowns = 0;
+};
+
+template <typename T>
+struct JacobiPlan final : JacobiPlanBase {
+    std::vector<gust_line_system> sys;
+    std::vector<JacobiSys<T>> tab;
+    std::vector<IBox> valid;
+    JacobiSys<T>* dtab = nullptr;
+    T* scratch = nullptr;
+    T* cglob = nullptr;
+    unsigned long long* chg = nullptr;
+    bool owned = false;   // buffers from hipMalloc (a plan handle), not the stream workspace
+
+    ~JacobiPlan() override
+    {
+        if (owned) {
+            (void)hipFree(dtab);
+            (void)hipFree(scratch);
+            (void)hipFree(cglob);
+            (void)hipFree(chg);
+        }
+    }
+
+    void init(int n, const gust_line_system* in, bool own, hipStream_t st)
+    {
+        const char* op = "zline_jacobi";
+        if (n < 1 || !in) { GUST_FAIL("gust: %s: no systems", op); }
+        nsys = n;
+        owned = own;
+        sys.assign(in, in + n);
+        tab.resize(size_t(n));
+        valid.resize(size_t(n));
+        size_t scratch_elems = 0;
+        for (int s = 0; s < n; ++s) {
+            const gust_line_system& q = sys[size_t(s)];
+            IBox V;
+            check_line_system(op, q.loc, &q.A, &q.ap, &q.rhs, 1, &q.geom, &q.delta, V);
+            valid[size_t(s)] = V;
+            JacobiSys<T>& t = tab[size_t(s)];
+            t.A = at_lo<T>(q.A, V);
+            t.a_j = q.A.jstride; t.a_k = q.A.kstride; t.a_n = q.A.nstride;
+            t.ap = at_lo<T>(q.ap, V);
+            t.p_j = q.ap.jstride; t.p_k = q.ap.kstride;
+            t.r = at_lo<T>(q.rhs, V);
+            t.r_j = q.rhs.jstride; t.r_k = q.rhs.kstride;
+            t.d = at_lo<T>(q.delta, V);
+            t.d_j = q.delta.jstride; t.d_k = q.delta.kstride;
+            t.nx = V.len(0); t.ny = V.len(1); t.nz = V.len(2);
+            t.Lx = q.geom.periodic[0] ? (q.geom.hi[0] - q.geom.lo[0] + 1) : 0;
+            t.Ly = q.geom.periodic[1] ? (q.geom.hi[1] - q.geom.lo[1] + 1) : 0;
+            t.s_j = t.nx;
+            t.s_k = int64_t(t.nx) * t.ny;
+            t.s = reinterpret_cast<T*>(scratch_elems);   // offset, fixed up below
+            scratch_elems += size_t(V.npts());
+            unknowns += V.npts();
+            t.tile0 = tiles;
+            t.pad = 0;
+            const int64_t cols = int64_t(t.nx) * t.ny;
+            const int64_t nt = (cols + kWave - 1) / kWave;
+            if (int64_t(tiles) + nt > (int64_t(1) << 30)) { GUST_FAIL("gust: %s: too many columns in one batch", op); }
+            tiles += int(nt);
+            nzmax = std::max(nzmax, t.nz);
+        }
+        const size_t lds = size_t(kWave) * size_t(nzmax) * sizeof(T);
+        const bool use_lds = lds <= device_info().lds_per_block;
+        const size_t tab_bytes = tab.size() * sizeof(JacobiSys<T>);
+        const size_t cbytes = use_lds ? 0 : size_t(tiles) * lds;
+        if (own) {
+            GUST_HIP(hipMalloc(&scratch, scratch_elems * sizeof(T) + 256));
+            GUST_HIP(hipMalloc(&chg, size_t(n) * 8 + 8));
+            GUST_HIP(hipMalloc(&dtab, tab_bytes));
+            if (cbytes) { GUST_HIP(hipMalloc(&cglob, cbytes)); }
+        } else {
+            scratch = static_cast<T*>(Workspace::get(st, 0, scratch_elems * sizeof(T) + 256));
+            chg = static_cast<unsigned long long*>(Workspace::get(st, 1, size_t(n) * 8 + 8));
+            dtab = static_cast<JacobiSys<T>*>(Workspace::get(st, 2, tab_bytes));
+            if (cbytes) { cglob = static_cast<T*>(Workspace::get(st, 5, cbytes)); }
+        }
+        for (auto& t : tab) { t.s = scratch + reinterpret_cast<size_t>(t.s); }
+        // The table travels once; the synchronisation keeps the host copy's
+        // lifetime out of the picture.
+        GUST_HIP(hipMemcpyAsync(dtab, tab.data(), tab_bytes, hipMemcpyHostToDevice, st));
+        GUST_HIP(hipStreamSynchronize(st));
+    }
+
+    template <bool F, bool L>
+    void launch(int dst_scratch, hipStream_t st)
+    {
+        const size_t lds = cglob ? 0 : size_t(kWave) * size_t(nzmax) * sizeof(T);
+        if (lds > 65536) {
+            GUST_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&zjacobi_sweep_kernel<T, F, L>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+        }
+        zjacobi_sweep_kernel<T, F, L><<<tiles, kWave, lds, st>>>(dtab, nsys, tiles, dst_scratch, chg, cglob, nzmax);
+        GUST_HIP(hipGetLastError());
+    }
+
+    void sweep(bool first, bool last, int dst_scratch, hipStream_t st) override
+    {
+        if (first && last) { launch<true, true>(dst_scratch, st); }
+        else if (first) { launch<true, false>(dst_scratch, st); }
+        else if (last) { launch<false, true>(dst_scratch, st); }
+        else { launch<false, false>(dst_scratch, st); }
+    }
+
+    void run(int sweeps, double* changes_dev, hipStream_t st) override
+    {
+        if (sweeps < 1) { GUST_FAIL("gust: zline_jacobi: sweeps must be >= 1 (got %d)", sweeps); }
+        zero_u64_kernel<<<(nsys + 255) / 256, 256, 0, st>>>(chg, nsys);
+        GUST_HIP(hipGetLastError());
+        // delta = 0 outside the domain (check_line_system's setVal).
+        for (int s = 0; s < nsys; ++s) { zero_ghosts<T>(sys[size_t(s)].delta, valid[size_t(s)], st); }
+        // The sweeps ping-pong between delta and the scratch iterate, arranged
+        // so the last sweep writes delta; sweep 0 reads the zero iterate
+        // implicitly.
+        for (int s = 0; s < sweeps; ++s) {
+            const int dst_scratch = ((sweeps - 1 - s) % 2 == 1) ? 1 : 0;
+            sweep(s == 0, s == sweeps - 1, dst_scratch, st);
+        }
+        // OverrideSync of the periodic seam faces (the low copy owns).
+        for (int s = 0; s < nsys; ++s) {
+            const gust_line_system& q = sys[size_t(s)];
+            const JacobiSys<T>& t = tab[size_t(s)];
+            for (int d = 0; d < 2; ++d) {
+                if (!((q.loc >> d) & 1) || !q.geom.periodic[d]) { continue; }
+                const int64_t n = d == 0 ? int64_t(t.ny) * t.nz : int64_t(t.nx) * t.nz;
+                const int blocks = int(std::min<int64_t>((n + 255) / 256, 2048));
+                seam_sync_kernel<T><<<blocks, 256, 0, st>>>(t.d, t.d_j, t.d_k, t.nx, t.ny, t.nz, d, 1);
+                GUST_HIP(hipGetLastError());
+            }
+        }
+        if (changes_dev) {
+            bits_to_double_kernel<<<(nsys + 255) / 256, 256, 0, st>>>(chg, changes_dev, nsys);
+            GUST_HIP(hipGetLastError());
+        }
+    }
+
+    void change_host(double* out, int n, hipStream_t st) override
+    {
+        auto* h = static_cast<unsigned long long*>(Workspace::get_host_pinned(0, size_t(n) * 8));
+        GUST_HIP(hipMemcpyAsync(h, chg, size_t(n) * 8, hipMemcpyDeviceToHost, st));
+        GUST_HIP(hipStreamSynchronize(st));
+        std::memcpy(out, h, size_t(n) * 8);
+    }
+};
+
+JacobiPlanBase* make_plan(int dtype)
+{
+    if (dtype == 0) { return new JacobiPlan<double>(); }
+    if (dtype == 1) { return new JacobiPlan<float>(); }
+    GUST_FAIL("gust: zline_jacobi: unknown dtype %d (0 = f64, 1 = f32)", dtype);
+}
+
+template <typename T>
+void zjacobi_run(int nsys, const gust_line_system* sys, int sweeps, double* changes_dev, double* change_host,
+                 hipStream_t st)
+{
+    if (sweeps < 1) { GUST_FAIL("gust: zline_jacobi: sweeps must be >= 1 (got %d)", sweeps); }
+    JacobiPlan<T> plan;
+    plan.init(nsys, sys, false, st);
+    plan.run(sweeps, changes_dev, st);
+    if (change_host) { plan.change_host(change_host, 1, st); }
+}
+
+// ---------------------------------------------------------------------------
+// Line LU-SGS: one workgroup per system.
+
+template <typename T>
+struct LusgsSys {
+    const T* A;          // at the valid lo corner
+    int64_t a_j, a_k, a_n;
+    const T* ap;
+    int64_t p_j, p_k;
+    const T* r;
+    int64_t r_j, r_k;
+    T* d;                // the caller's delta at the valid lo corner (1 ghost)
+    int64_t d_j, d_k;
+    T* w;                // zero ghosts: work array at valid lo (ng + 1 ghosts)
+    int64_t w_j, w_k;
+    int w_gx, w_gy;      // work ghost widths
+    T* c;                // Thomas scratch over the sweep region
+    T* e;
+    int64_t c_j, c_k;
+    int nx, ny, nz;      // valid extents at loc (local index 0 = valid lo)
+    int Lx, Ly;          // periodic shifts (cells), 0 if not periodic
+    int s_lo[2], s_hi[2];// sweep region S (local indices)
+    int o_lo[2], o_hi[2];// owned region (local indices)
+    int fold[2];         // one-cell periodic axis folded into the diagonal
+    int zero_mode;       // 1: zero ghosts (work array), 0: lagged (in delta)
+    int loc;
+};
+
+// Coefficient/ap/rhs index (local) of a point of the grown region: the
+// periodic image inside the valid box.
+__device__ __forceinline__ int wrap_idx(int i, int n, int L)
+{
+    if (L) {
+        if (i < 0) { i += L; }
+        else if (i > n - 1) { i -= L; }
+    }
+    return i;
+}
+
+template <typename T>
+__device__ void lusgs_column(const LusgsSys<T>& S, T* W, int64_t wj, int64_t wk, int i, int j, bool track,
+                             unsigned long long& chg)
+{
+    const int ii = wrap_idx(i, S.nx, S.Lx), jj = wrap_idx(j, S.ny, S.Ly);
+    const T* a = S.A + ii + int64_t(jj) * S.a_j;
+    const T* pap = S.ap + ii + int64_t(jj) * S.p_j;
+    const T* pr = S.r + ii + int64_t(jj) * S.r_j;
+    T* col = W + i + int64_t(j) * wj;
+    const int64_t cb = int64_t(i - S.s_lo[0]) + int64_t(j - S.s_lo[1]) * S.c_j;
+    T* cc = S.c + cb;
+    T* ee = S.e + cb;
+    const int nz = S.nz;
+    const bool owned = i >= S.o_lo[0] && i <= S.o_hi[0] && j >= S.o_lo[1] && j <= S.o_hi[1];
+    T cprev = T(0), eprev = T(0);
+    for (int k = 0; k < nz; ++k) {
+        const T* ak = a + k * S.a_k;
+        T axm = ak[1 * S.a_n], axp = ak[2 * S.a_n], aym = ak[3 * S.a_n], ayp = ak[4 * S.a_n];
+        const T azm = ak[5 * S.a_n], azp = ak[6 * S.a_n];
+        T dg = pap[k * S.p_k];
+        // The fold of a periodic one-cell axis (Source/LineSweeps.cpp:242-256).
+        if (S.fold[0]) { dg -= axm + axp; axm = T(0); axp = T(0); }
+        if (S.fold[1]) { dg -= aym + ayp; aym = T(0); ayp = T(0); }
+        const T* wk0 = col + k * wk;
+        const T rk = pr[k * S.r_k] + axm * wk0[-1] + axp * wk0[1] + aym * wk0[-wj] + ayp * wk0[wj];
+        const T lower = -azm, upper = -azp;
+        T ck, ek;
+        if (k == 0) {
+            const T m = dg;
+            ck = upper / m;
+            ek = rk / m;
+        } else {
+            const T m = dg - lower * cprev;
+            ck = upper / m;
+            ek = (rk - lower * eprev) / m;
+        }
+        cc[k * S.c_k] = ck;
+        ee[k * S.c_k] = ek;
+        cprev = ck;
+        eprev = ek;
+    }
+    T o = eprev;
+    {
+        T* p = col + (nz - 1) * wk;
+        if (track && owned) {
+            const unsigned long long b = dev::abs_bits(double(*p - o));
+            chg = b > chg ? b : chg;
+        }
+        *p = o;
+    }
+    for (int k = nz - 2; k >= 0; --k) {
+        o = ee[k * S.c_k] - cc[k * S.c_k] * o;
+        T* p = col + k * wk;
+        if (track && owned) {
+            const unsigned long long b = dev::abs_bits(double(*p - o));
+            chg = b > chg ? b : chg;
+        }
+        *p = o;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void lusgs_kernel(const LusgsSys<T>* __restrict__ table, int sweeps, int alternate,
+                                                    unsigned long long* __restrict__ changes)
+{
+    const LusgsSys<T> S = table[blockIdx.x];
+    const int tid = threadIdx.x, nth = blockDim.x;
+    T* W;
+    int64_t wj, wk;
+    if (S.zero_mode) {
+        W = S.w; wj = S.w_j; wk = S.w_k;
+        // The box-local iterate: 0 everywhere, ghosts included.
+        const int gx = S.w_gx, gy = S.w_gy;
+        const int tx = S.nx + 2 * gx, ty = S.ny + 2 * gy;
+        const int64_t n = int64_t(tx) * ty * S.nz;
+        for (int64_t t = tid; t < n; t += nth) {
+            const int i = int(t % tx) - gx;
+            const int64_t rr = t / tx;
+            const int j = int(rr % ty) - gy, k = int(rr / ty);
+            W[i + j * wj + k * wk] = T(0);
+        }
+    } else {
+        W = S.d; wj = S.d_j; wk = S.d_k;
+    }
+    __syncthreads();
+    const int sx = S.s_hi[0] - S.s_lo[0] + 1, sy = S.s_hi[1] - S.s_lo[1] + 1;
+    const int last = sx + sy - 2;
+    unsigned long long chg = 0;
+    for (int s = 0; s < sweeps; ++s) {
+        const bool jrev = alternate && (s % 2 == 1);
+        for (int f = 1; f >= 0; --f) {
+            if (!S.zero_mode) {
+                // FillBoundary: the periodic images into delta's ghosts.
+                if (S.Lx) {
+                    const int64_t n = int64_t(S.ny) * S.nz;
+                    for (int64_t t = tid; t < n; t += nth) {
+                        const int j = int(t % S.ny), k = int(t / S.ny);
+                        T* row = W + j * wj + k * wk;
+                        row[-1] = row[S.Lx - 1];
+                        row[S.nx] = row[S.nx - S.Lx];
+                    }
+                }
+                if (S.Ly) {
+                    const int64_t n = int64_t(S.nx) * S.nz;
+                    for (int64_t t = tid; t < n; t += nth) {
+                        const int i = int(t % S.nx), k = int(t / S.nx);
+                        T* colp = W + i + k * wk;
+                        colp[-wj] = colp[(S.Ly - 1) * wj];
+                        colp[S.ny * wj] = colp[(S.ny - S.Ly) * wj];
+                    }
+                }
+                __syncthreads();
+            }
+            const bool track = (s == sweeps - 1) && f == 0;
+            for (int tt = 0; tt <= last; ++tt) {
+                const int p = f ? tt : last - tt;
+                const int amin = p - (sy - 1) > 0 ? p - (sy - 1) : 0;
+                const int amax = p < sx - 1 ? p : sx - 1;
+                for (int m = tid; m <= amax - amin; m += nth) {
+                    const int ia = amin + m;
+                    const int i = S.s_lo[0] + ia;
+                    const int j = jrev ? S.s_hi[1] - (p - ia) : S.s_lo[1] + p - ia;
+                    lusgs_column(S, W, wj, wk, i, j, track, chg);
+                }
+                __syncthreads();
+            }
+        }
+    }
+    // max |change| over the block.
+    __shared__ unsigned long long red[256 / 64];
+    chg = dev::wave_max_u64(chg);
+    if ((tid & 63) == 0) { red[tid >> 6] = chg; }
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long m = 0;
+        for (int w = 0; w < (nth + 63) / 64; ++w) { m = red[w] > m ? red[w] : m; }
+        changes[blockIdx.x] = m;
+    }
+    // The result into delta (zero ghosts) and the seam copies.
+    if (S.zero_mode) {
+        const int64_t n = int64_t(S.nx) * S.ny * S.nz;
+        for (int64_t t = tid; t < n; t += nth) {
+            const int i = int(t % S.nx);
+            const int64_t rr = t / S.nx;
+            const int j = int(rr % S.ny), k = int(rr / S.ny);
+            S.d[i + j * S.d_j + k * S.d_k] = W[i + j * wj + k * wk];
+        }
+        __syncthreads();
+    } else {
+        // lagged leaves stale periodic images in delta's ghosts: zero them
+        // (no caller reads them; zero outside the domain, as documented).
+        if (S.Lx) {
+            const int64_t n = int64_t(S.ny) * S.nz;
+            for (int64_t t = tid; t < n; t += nth) {
+                const int j = int(t % S.ny), k = int(t / S.ny);
+                T* row = S.d + j * S.d_j + k * S.d_k;
+                row[-1] = T(0);
+                row[S.nx] = T(0);
+            }
+        }
+        if (S.Ly) {
+            const int64_t n = int64_t(S.nx) * S.nz;
+            for (int64_t t = tid; t < n; t += nth) {
+                const int i = int(t % S.nx), k = int(t / S.nx);
+                T* colp = S.d + i + k * S.d_k;
+                colp[-S.d_j] = T(0);
+                colp[S.ny * S.d_j] = T(0);
+            }
+        }
+        __syncthreads();
+    }
+    for (int d = 0; d < 2; ++d) {
+        if (!((S.loc >> d) & 1) || !(d == 0 ? S.Lx : S.Ly)) { continue; }
+        if (S.zero_mode && !(S.loc == GUST_LOC_XFACE || S.loc == GUST_LOC_YFACE)) { continue; }
+        const int64_t n = d == 0 ? int64_t(S.ny) * S.nz : int64_t(S.nx) * S.nz;
+        for (int64_t t = tid; t < n; t += nth) {
+            int64_t lo, hi;
+            if (d == 0) {
+                const int j = int(t % S.ny), k = int(t / S.ny);
+                lo = j * S.d_j + k * S.d_k;
+                hi = lo + (S.nx - 1);
+            } else {
+                const int i = int(t % S.nx), k = int(t / S.nx);
+                lo = i + k * S.d_k;
+                hi = lo + (S.ny - 1) * S.d_j;
+            }
+            // lagged: AMReX's default owner (low copy); zero: line_owned_box
+            // (the high copy, Source/LineSweeps.cpp:361-374).
+            if (S.zero_mode) { S.d[lo] = S.d[hi]; } else { S.d[hi] = S.d[lo]; }
+        }
+    }
+}
+
+int check_line_options_impl(const gust_line_options* opt)
+{
+    if (!opt) { GUST_FAIL("gust: zline_lusgs: options are null"); }
+    if (opt->overlap < 0 || opt->overlap > GUST_MAX_LINE_OVERLAP) {
+        GUST_FAIL("gust: zline_lusgs: invalid options:\n  steady.line_overlap = %d: must be an integer >= 0 and <= %d",
+                  opt->overlap, GUST_MAX_LINE_OVERLAP);
+    }
+    if (opt->overlap > 0 && opt->ghosts != GUST_GHOSTS_ZERO) {
+        GUST_FAIL("gust: zline_lusgs: invalid options:\n  steady.line_overlap = %d: needs steady.line_ghosts = zero "
+                  "(the lagged ghosts sweep the valid box alone)", opt->overlap);
+    }
+    if (opt->alternate != 0 && opt->alternate != 1) {
+        GUST_FAIL("gust: zline_lusgs: invalid options:\n  steady.line_alternate = %d: must be 0 or 1", opt->alternate);
+    }
+    if (opt->ghosts != GUST_GHOSTS_LAGGED && opt->ghosts != GUST_GHOSTS_ZERO) {
+        GUST_FAIL("gust: zline_lusgs: invalid options:\n  steady.line_ghosts = %d: must be lagged or zero", opt->ghosts);
+    }
+    if (opt->order != GUST_ORDER_SEQUENTIAL && opt->order != GUST_ORDER_DIAGONAL) {
+        GUST_FAIL("gust: zline_lusgs: invalid options:\n  steady.line_order = %d: must be sequential or diagonal",
+                  opt->order);
+    }
+    return 0;
+}
+
+template <typename T>
+void lusgs_run(int loc, const gust_array4* A, const gust_array4* ap, const gust_array4* rhs, int sweeps,
+               const gust_geometry* g, gust_array4* delta, const gust_line_options* opt, double* change,
+               hipStream_t st)
+{
+    const char* op = "zline_lusgs";
+    IBox V;
+    check_line_system(op, loc, A, ap, rhs, sweeps, g, delta, V);
+    check_line_options_impl(opt);
+    // check_line_overlap_layout (Source/LineSweeps.cpp:343-359), one box.
+    const int ngx = (g->hi[0] - g->lo[0] + 1) > 1 ? opt->overlap : 0;
+    const int ngy = (g->hi[1] - g->lo[1] + 1) > 1 ? opt->overlap : 0;
+    int narrowest = 0x7fffffff;
+    if (ngx > 0) { narrowest = std::min(narrowest, g->hi[0] - g->lo[0] + 1); }
+    if (ngy > 0) { narrowest = std::min(narrowest, g->hi[1] - g->lo[1] + 1); }
+    if (opt->overlap > 0 && narrowest != 0x7fffffff && opt->overlap > narrowest) {
+        GUST_FAIL("gust: zline_lusgs: invalid options:\n  steady.line_overlap = %d: must be <= the narrowest box's "
+                  "width in x and y (%d cells; mesh.max_grid_size)", opt->overlap, narrowest);
+    }
+    LusgsSys<T> t{};
+    t.A = at_lo<T>(*A, V);
+    t.a_j = A->jstride; t.a_k = A->kstride; t.a_n = A->nstride;
+    t.ap = at_lo<T>(*ap, V);
+    t.p_j = ap->jstride; t.p_k = ap->kstride;
+    t.r = at_lo<T>(*rhs, V);
+    t.r_j = rhs->jstride; t.r_k = rhs->kstride;
+    t.d = at_lo<T>(*delta, V);
+    t.d_j = delta->jstride; t.d_k = delta->kstride;
+    t.nx = V.len(0); t.ny = V.len(1); t.nz = V.len(2);
+    t.Lx = g->periodic[0] ? (g->hi[0] - g->lo[0] + 1) : 0;
+    t.Ly = g->periodic[1] ? (g->hi[1] - g->lo[1] + 1) : 0;
+    t.loc = loc;
+    const bool zero = opt->ghosts == GUST_GHOSTS_ZERO;
+    t.zero_mode = zero ? 1 : 0;
+    // owned (line_owned_box) and the sweep region (line_sweep_box), local.
+    int olo[2] = {0, 0}, ohi[2] = {t.nx - 1, t.ny - 1};
+    if (zero) {
+        for (int d = 0; d < 2; ++d) {
+            if (((loc >> d) & 1) && g->periodic[d]) { olo[d] += 1; }
+        }
+    }
+    int slo[2] = {olo[0], olo[1]}, shi[2] = {ohi[0], ohi[1]};
+    if (zero) {
+        const int ng[2] = {ngx, ngy};
+        const int n[2] = {t.nx, t.ny};
+        for (int d = 0; d < 2; ++d) {
+            slo[d] -= ng[d];
+            shi[d] += ng[d];
+            if (!g->periodic[d]) {
+                slo[d] = std::max(slo[d], 0);
+                shi[d] = std::min(shi[d], n[d] - 1);
+            }
+        }
+        t.fold[0] = g->periodic[0] && (g->hi[0] - g->lo[0] + 1) == 1;
+        t.fold[1] = g->periodic[1] && (g->hi[1] - g->lo[1] + 1) == 1;
+    }
+    for (int d = 0; d < 2; ++d) {
+        t.s_lo[d] = slo[d]; t.s_hi[d] = shi[d];
+        t.o_lo[d] = olo[d]; t.o_hi[d] = ohi[d];
+    }
+    const int sx = shi[0] - slo[0] + 1, sy = shi[1] - slo[1] + 1;
+    t.c_j = sx;
+    t.c_k = int64_t(sx) * sy;
+    const size_t cbytes = size_t(sx) * sy * t.nz * sizeof(T);
+    size_t wbytes = 0;
+    if (zero) {
+        t.w_gx = ngx + 1;
+        t.w_gy = ngy + 1;
+        const int64_t tx = t.nx + 2 * t.w_gx, ty = t.ny + 2 * t.w_gy;
+        t.w_j = tx;
+        t.w_k = tx * ty;
+        wbytes = size_t(tx * ty * t.nz) * sizeof(T);
+    }
+    char* ws = static_cast<char*>(Workspace::get(st, 3, 2 * cbytes + wbytes + 1024));
+    t.c = reinterpret_cast<T*>(ws);
+    t.e = reinterpret_cast<T*>(ws + cbytes);
+    if (zero) { t.w = reinterpret_cast<T*>(ws + 2 * cbytes) + t.w_gx + t.w_gy * t.w_j; }
+    auto* chg = static_cast<unsigned long long*>(Workspace::get(st, 1, 16));
+    auto* dtab = static_cast<LusgsSys<T>*>(Workspace::get(st, 4, sizeof(LusgsSys<T>)));
+    GUST_HIP(hipMemcpyAsync(dtab, &t, sizeof t, hipMemcpyHostToDevice, st));
+    zero_ghosts<T>(*delta, V, st);
+    lusgs_kernel<T><<<1, 256, 0, st>>>(dtab, sweeps, opt->alternate, chg);
+    GUST_HIP(hipGetLastError());
+    if (change) {
+        auto* h = static_cast<unsigned long long*>(Workspace::get_host_pinned(0, 8));
+        GUST_HIP(hipMemcpyAsync(h, chg, 8, hipMemcpyDeviceToHost, st));
+        GUST_HIP(hipStreamSynchronize(st));
+        unsigned long long b = h[0];
+        double v;
+        std::memcpy(&v, &b, 8);
+        *change = v;
+    } else {
+        // The table copy source lives on this stack frame.
+        GUST_HIP(hipStreamSynchronize(st));
+    }
+}
+
+template <typename T>
+void zjacobi_one(int loc, const gust_array4* A, const gust_array4* ap, const gust_array4* rhs, int sweeps,
+                 const gust_geometry* geom, gust_array4* delta, double* change, hipStream_t st)
+{
+    if (!A || !ap || !rhs || !delta || !geom) { GUST_FAIL("gust: zline_jacobi: null argument"); }
+    gust_line_system q{};
+    q.loc = loc;
+    q.geom = *geom;
+    q.A = *A;
+    q.ap = *ap;
+    q.rhs = *rhs;
+    q.delta = *delta;
+    double c = 0;
+    zjacobi_run<T>(1, &q, sweeps, nullptr, &c, st);
+    if (change) { *change = c; }
+}
+
+}   // namespace
+
+}   // namespace gust_amd
+
+using namespace gust_amd;
+
+extern "C" {
+
+int gust_zline_jacobi(int loc, const gust_array4* A, const gust_array4* ap, const gust_array4* rhs, int sweeps,
+                      const gust_geometry* geom, gust_array4* delta, double* change, void* stream)
+{
+    return guarded([&] { zjacobi_one<double>(loc, A, ap, rhs, sweeps, geom, delta, change, (hipStream_t)stream); });
+}
+
+int gust_zline_jacobi_f32(int loc, const gust_array4* A, const gust_array4* ap, const gust_array4* rhs, int sweeps,
+                          const gust_geometry* geom, gust_array4* delta, double* change, void* stream)
+{
+    return guarded([&] { zjacobi_one<float>(loc, A, ap, rhs, sweeps, geom, delta, change, (hipStream_t)stream); });
+}
+
+int gust_zline_jacobi_batched_async(int nsys, const gust_line_system* sys, int sweeps, double* changes_dev,
+                                    void* stream)
+{
+    return guarded([&] { zjacobi_run<double>(nsys, sys, sweeps, changes_dev, nullptr, (hipStream_t)stream); });
+}
+
+int gust_zline_jacobi_batched_async_f32(int nsys, const gust_line_system* sys, int sweeps, double* changes_dev,
+                                        void* stream)
+{
+    return guarded([&] { zjacobi_run<float>(nsys, sys, sweeps, changes_dev, nullptr, (hipStream_t)stream); });
+}
+
+struct gust_jacobi_plan {
+    gust_amd::JacobiPlanBase* impl;
+};
+
+int gust_jacobi_plan_create(int nsys, const gust_line_system* sys, int dtype, gust_jacobi_plan** plan, void* stream)
+{
+    return guarded([&] {
+        if (!plan) { GUST_FAIL("gust: jacobi_plan_create: plan is null"); }
+        *plan = nullptr;
+        std::unique_ptr<JacobiPlanBase> p(make_plan(dtype));
+        if (dtype == 0) { static_cast<JacobiPlan<double>*>(p.get())->init(nsys, sys, true, (hipStream_t)stream); }
+        else { static_cast<JacobiPlan<float>*>(p.get())->init(nsys, sys, true, (hipStream_t)stream); }
+        *plan = new gust_jacobi_plan{p.release()};
+    });
+}
+
+int gust_jacobi_plan_run(gust_jacobi_plan* plan, int sweeps, double* changes_dev, void* stream)
+{
+    return guarded([&] {
+        if (!plan || !plan->impl) { GUST_FAIL("gust: jacobi_plan_run: plan is null"); }
+        plan->impl->run(sweeps, changes_dev, (hipStream_t)stream);
+    });
+}
+
+int gust_jacobi_plan_sweep(gust_jacobi_plan* plan, int first, int last, int into_scratch, void* stream)
+{
+    return guarded([&] {
+        if (!plan || !plan->impl) { GUST_FAIL("gust: jacobi_plan_sweep: plan is null"); }
+        plan->impl->sweep(first != 0, last != 0, into_scratch ? 1 : 0, (hipStream_t)stream);
+    });
+}
+
+int gust_jacobi_plan_info(const gust_jacobi_plan* plan, int64_t* unknowns, int32_t* tiles)
+{
+    return guarded([&] {
+        if (!plan || !plan->impl) { GUST_FAIL("gust: jacobi_plan_info: plan is null"); }
+        if (unknowns) { *unknowns = plan->impl->unknowns; }
+        if (tiles) { *tiles = plan->impl->tiles; }
+    });
+}
+
+void gust_jacobi_plan_destroy(gust_jacobi_plan* plan)
+{
+    if (plan) {
+        delete plan->impl;
+        delete plan;
+    }
+}
+
+int gust_zline_lusgs(int loc, const gust_array4* A, const gust_array4* ap, const gust_array4* rhs, int sweeps,
+                     const gust_geometry* geom, gust_array4* delta, const gust_line_options* opt, double* change,
+                     void* stream)
+{
+    return guarded([&] {
+        if (!A || !ap || !rhs || !delta || !geom || !opt) { GUST_FAIL("gust: zline_lusgs: null argument"); }
+        lusgs_run<double>(loc, A, ap, rhs, sweeps, geom, delta, opt, change, (hipStream_t)stream);
+    });
+}
+
+int gust_line_sweeps(const gust_line_options* opt, int loc, const gust_array4* A, const gust_array4* ap,
+                     const gust_array4* rhs, int sweeps, const gust_geometry* geom, gust_array4* delta,
+                     double* change, void* stream)
+{
+    return guarded([&] {
+        if (!opt) { GUST_FAIL("gust: line_sweeps: options are null"); }
+        if (opt->kind == GUST_SWEEP_LUSGS) {
+            if (!A || !ap || !rhs || !delta || !geom) { GUST_FAIL("gust: zline_lusgs: null argument"); }
+            lusgs_run<double>(loc, A, ap, rhs, sweeps, geom, delta, opt, change, (hipStream_t)stream);
+        } else {
+            zjacobi_one<double>(loc, A, ap, rhs, sweeps, geom, delta, change, (hipStream_t)stream);
+        }
+    });
+}
+
+int gust_check_line_options(const gust_line_options* opt)
+{
+    return guarded([&] { check_line_options_impl(opt); });
+}
+
+}

@@ -1,0 +1,161 @@
+/*
+ * gust_amd.h -- C-ABI of the MI355X-native SIMPLEC hot path of GUST
+ * (hgopalan/ablsteadyrans).
+ *
+ * Every entry point replaces one reference interface, cited per declaration
+ * as file:line in the reference tree. The AMReX-side shim that binds them
+ * (MultiFab/MFIter -> gust_array4 per box) is shown in INTEGRATION.md.
+ *
+ * Conventions
+ *  - Status: 0 = success, nonzero = failure; gust_last_error() then returns
+ *    the reference's message text ("gust: <op>: ..."), per calling thread.
+ *    Nothing here throws or aborts.
+ *  - Data: gust_array4 mirrors amrex::Array4 (i fastest, begin inclusive,
+ *    end exclusive, ghosts included in [begin, end)). Pointers are device
+ *    pointers valid on `stream`; element type is double unless the entry
+ *    point ends in _f32 (float).
+ *  - A tile is one box spanning its whole domain; gust_geometry names the
+ *    cell-centred domain box, the periodic axes (x/y) and the spacing.
+ *  - Streams: every entry point enqueues on `stream` (a hipStream_t). Entry
+ *    points returning a host scalar synchronise that stream (as the
+ *    reference's reductions do); the *_async variants do not.
+ *  - Handles and per-stream workspaces are not thread-safe: one caller per
+ *    stream at a time.
+ */
+#ifndef GUST_AMD_H_
+#define GUST_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* amrex::Array4<Real> (AMReX_Array4.H): p, jstride, kstride, nstride,
+ * begin, end, ncomp. */
+typedef struct gust_array4 {
+    void* p;
+    int64_t jstride, kstride, nstride;
+    int32_t begin[3], end[3];
+    int32_t ncomp;
+} gust_array4;
+
+/* The parts of amrex::Geometry the hot path reads: the cell-centred domain
+ * box (inclusive), periodicity in x and y, and dx, dy, dzeta. */
+typedef struct gust_geometry {
+    int32_t lo[3], hi[3];
+    int32_t periodic[3];
+    double dx[3];
+} gust_geometry;
+
+/* Field locations, Source/CVFluxes.H:22 (enum class Loc). */
+enum { GUST_LOC_CELL = 0, GUST_LOC_XFACE = 1, GUST_LOC_YFACE = 2, GUST_LOC_ZFACE = 4 };
+
+/* Coefficient components of A, Source/ImplicitOperator.H:57 (ImplicitCoef). */
+enum { GUST_COEF_P = 0, GUST_COEF_XM, GUST_COEF_XP, GUST_COEF_YM, GUST_COEF_YP, GUST_COEF_ZM, GUST_COEF_ZP,
+       GUST_N_COEFS };
+
+/* Source/LineSweeps.H:80-104 (SweepKind, LineGhosts, LineOrder,
+ * LineSweepOptions, max_line_overlap). */
+enum { GUST_SWEEP_ZJACOBI = 0, GUST_SWEEP_LUSGS = 1 };
+enum { GUST_GHOSTS_LAGGED = 0, GUST_GHOSTS_ZERO = 1 };
+enum { GUST_ORDER_SEQUENTIAL = 0, GUST_ORDER_DIAGONAL = 1 };
+#define GUST_MAX_LINE_OVERLAP 8
+
+typedef struct gust_line_options {
+    int32_t kind;       /* GUST_SWEEP_* */
+    int32_t ghosts;     /* GUST_GHOSTS_* */
+    int32_t overlap;    /* 0..GUST_MAX_LINE_OVERLAP, zero ghosts only */
+    int32_t order;      /* GUST_ORDER_*; both give bit-identical results */
+    int32_t alternate;  /* 0 or 1: odd sweeps take the j-reversed pair */
+} gust_line_options;
+
+/* One line system of a batch: a field at `loc` on a tile. */
+typedef struct gust_line_system {
+    int32_t loc;
+    gust_geometry geom;
+    gust_array4 A;      /* GUST_N_COEFS components at loc, no ghosts needed */
+    gust_array4 ap;     /* 1 component at loc */
+    gust_array4 rhs;    /* 1 component at loc */
+    gust_array4 delta;  /* 1 component at loc, >= 1 ghost cell; output */
+} gust_line_system;
+
+/* The message of the calling thread's last failure ("" if none). */
+const char* gust_last_error(void);
+
+/* Library version string. */
+const char* gust_version(void);
+
+/* z-line Jacobi sweeps of one system.
+ * Replaces: Real zline_jacobi(Loc, const MultiFab& A, const MultiFab& ap,
+ *   const MultiFab& rhs, int sweeps, const Mesh&, const Geometry&,
+ *   MultiFab& delta)  -- Source/Predictor.H:267-268, Source/Predictor.cpp:87-127.
+ * delta is zeroed (ghosts included) and receives the iterate; *change =
+ * max |change over the last sweep| (synchronises the stream). */
+int gust_zline_jacobi(int loc, const gust_array4* A, const gust_array4* ap, const gust_array4* rhs, int sweeps,
+                      const gust_geometry* geom, gust_array4* delta, double* change, void* stream);
+
+/* The same on float data (the mixed-precision preconditioner path);
+ * *change is returned as double. */
+int gust_zline_jacobi_f32(int loc, const gust_array4* A, const gust_array4* ap, const gust_array4* rhs, int sweeps,
+                          const gust_geometry* geom, gust_array4* delta, double* change, void* stream);
+
+/* Batched z-line Jacobi: nsys systems (any locations and tiles) in one
+ * launch per sweep. changes_dev (device, nsys doubles, may be NULL) receives
+ * each system's last-sweep change; nothing is synchronised.
+ * Replaces: the per-box ParallelFor loop of zline_jacobi
+ *   (Source/Predictor.cpp:97-114) over all boxes and the five variables of
+ *   an iteration (Source/Predictor.cpp:400, Source/Simplec.cpp:500-502). */
+int gust_zline_jacobi_batched_async(int nsys, const gust_line_system* sys, int sweeps, double* changes_dev,
+                                    void* stream);
+int gust_zline_jacobi_batched_async_f32(int nsys, const gust_line_system* sys, int sweeps, double* changes_dev,
+                                        void* stream);
+
+/* A batched Jacobi plan: the systems' device table, scratch iterate and
+ * change accumulators, built once (allocates; synchronises `stream` once).
+ * Running a plan enqueues kernels only -- no allocation, copy, memset or
+ * host synchronisation -- so it can be captured in a HIP graph and replayed
+ * every iteration. The views' memory must outlive the plan.
+ * Replaces: the same per-box sweep loop as gust_zline_jacobi_batched_async,
+ *   with the setup hoisted out of the iteration (the reference re-allocates
+ *   its three scratch MultiFabs per call, Source/Predictor.cpp:92-93).
+ * dtype: 0 = double data, 1 = float data. */
+typedef struct gust_jacobi_plan gust_jacobi_plan;
+int gust_jacobi_plan_create(int nsys, const gust_line_system* sys, int dtype, gust_jacobi_plan** plan,
+                            void* stream);
+/* Zero delta's ghosts, `sweeps` sweeps, the seam sync; changes_dev as in
+ * gust_zline_jacobi_batched_async. */
+int gust_jacobi_plan_run(gust_jacobi_plan* plan, int sweeps, double* changes_dev, void* stream);
+/* One sweep kernel alone (first: the input iterate is zero; last: record the
+ * change; into_scratch: read delta and write the scratch iterate, else the
+ * reverse). For timing one sweep and for callers composing their own loop. */
+int gust_jacobi_plan_sweep(gust_jacobi_plan* plan, int first, int last, int into_scratch, void* stream);
+/* Unknowns and 64-column tiles of the plan. */
+int gust_jacobi_plan_info(const gust_jacobi_plan* plan, int64_t* unknowns, int32_t* tiles);
+void gust_jacobi_plan_destroy(gust_jacobi_plan* plan);
+
+/* Line LU-SGS sweeps of one system with the given options.
+ * Replaces: Real zline_lusgs(Loc, A, ap, rhs, int sweeps, const Mesh&,
+ *   const Geometry&, MultiFab& delta, const LineSweepOptions&)
+ *   -- Source/LineSweeps.H:159-161, Source/LineSweeps.cpp:415-428.
+ * The sequential and diagonal orders give bit-identical results; on the
+ * GPU both run as diagonal wavefronts inside one workgroup per system. */
+int gust_zline_lusgs(int loc, const gust_array4* A, const gust_array4* ap, const gust_array4* rhs, int sweeps,
+                     const gust_geometry* geom, gust_array4* delta, const gust_line_options* opt, double* change,
+                     void* stream);
+
+/* Dispatcher. Replaces: Real line_sweeps(const LineSweepOptions&, Loc, ...)
+ *   -- Source/Predictor.H:257-259, Source/Predictor.cpp:79-85. */
+int gust_line_sweeps(const gust_line_options* opt, int loc, const gust_array4* A, const gust_array4* ap,
+                     const gust_array4* rhs, int sweeps, const gust_geometry* geom, gust_array4* delta,
+                     double* change, void* stream);
+
+/* The option checks, Source/LineSweeps.cpp:315-329 (check_line_options):
+ * 0 if valid, else nonzero with the message naming the steady.<key>. */
+int gust_check_line_options(const gust_line_options* opt);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,82 @@
+/*
+ * gust_oracle.h -- the FP64 CPU oracle of the SIMPLEC hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is a plain-C restatement of the reference
+ * solver's arithmetic (GUST, hgopalan/ablsteadyrans), written operation by
+ * operation in the reference's order so that the HIP library can be checked
+ * against it. Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * leg may load it; the product library never links or calls it.
+ *
+ * Every function cites the reference file:line it restates. Each one is
+ * pinned by checks in tests/test_oracle_*.py against closed forms, dense
+ * solves, invariants and the reference's own fixtures; a function without
+ * such a check says "parity unpinned" next to its declaration.
+ *
+ * Data: Array4-compatible views (AMReX Array4 semantics, i fastest):
+ *   element (i,j,k,n) at p[(i-begin[0]) + (j-begin[1])*jstride
+ *                           + (k-begin[2])*kstride + n*nstride],
+ *   begin inclusive, end exclusive (the box of the view, ghosts included).
+ * A tile is one box spanning its whole domain (x/y periodic or not).
+ */
+#ifndef GUST_ORACLE_H_
+#define GUST_ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    double* p;
+    int64_t jstride, kstride, nstride;
+    int32_t begin[3], end[3];
+    int32_t ncomp;
+} oa4;
+
+typedef struct {
+    int32_t lo[3], hi[3];    /* the cell-centred domain box (inclusive) */
+    int32_t periodic[3];     /* 1 = periodic along that axis (z never) */
+    double dx[3];            /* dx, dy, dzeta */
+} ogeom;
+
+/* Field locations (Source/CVFluxes.H:22): bit d set = nodal along d. */
+enum { OLOC_CELL = 0, OLOC_XFACE = 1, OLOC_YFACE = 2, OLOC_ZFACE = 4 };
+
+/* Coefficient components (Source/ImplicitOperator.H:57). */
+enum { OC_P = 0, OC_XM, OC_XP, OC_YM, OC_YP, OC_ZM, OC_ZP, OC_N };
+
+/* Line sweep options (Source/LineSweeps.H:80-104). */
+enum { OGHOSTS_LAGGED = 0, OGHOSTS_ZERO = 1 };
+enum { OORDER_SEQUENTIAL = 0, OORDER_DIAGONAL = 1 };
+#define O_MAX_LINE_OVERLAP 8
+
+const char* oracle_last_error(void);
+
+/* The exact column solve (Source/LineSolve_K.H:22-45). d is read at the
+ * x/y neighbours, c/e are Thomas scratch at loc, out receives the column. */
+void oracle_solve_column(const oa4* a, const oa4* dg, const oa4* r, const oa4* d, oa4* c, oa4* e, oa4* out,
+                         int i, int j, int klo, int khi);
+
+/* z-line Jacobi (Source/Predictor.cpp:87-127 with check_line_system,
+ * Source/LineSweeps.cpp:391-413). delta needs one ghost cell; it is zeroed,
+ * swept, and the periodic seam copies synchronised (the low copy wins).
+ * Returns 0 and the last sweep's max |change| in *change, or -1. */
+int oracle_zline_jacobi(int loc, const oa4* A, const oa4* ap, const oa4* rhs, int sweeps, const ogeom* g,
+                        oa4* delta, double* change);
+
+/* Line LU-SGS (Source/LineSweeps.cpp:181-305, 415-428): lagged or zero
+ * ghosts, overlap, sequential or diagonal order, alternate 0/1.
+ * The diagonal order is executed plane by plane (LineSweeps.cpp:66-94). */
+int oracle_zline_lusgs(int loc, const oa4* A, const oa4* ap, const oa4* rhs, int sweeps, const ogeom* g,
+                       oa4* delta, int ghosts, int overlap, int order, int alternate, double* change);
+
+/* Number of column-pass kernel launches the reference would make
+ * (line_sweep_launches, Source/LineSweeps.H:106-111) for the last call. */
+uint64_t oracle_last_lusgs_launches(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,91 @@
+"""The C-ABI library loads, exports every symbol include/*.h declares, and
+refuses bad input with the reference's messages -- all without a GPU (the
+refusals happen before any device call)."""
+import ctypes as C
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+from tests.conftest import ROOT
+
+
+def _declared():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(gust_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_every_declared_symbol_is_exported():
+    import ablsteadyrans_amd._lib as L
+    names = _declared()
+    assert len(names) >= 8
+    missing = [n for n in names if not hasattr(L.lib, n)]
+    assert not missing, missing
+    # and the Python binding covers each of them
+    assert set(names) <= set(L.exported_symbols()), set(names) - set(L.exported_symbols())
+
+
+def test_version_and_empty_error():
+    import ablsteadyrans_amd._lib as L
+    assert b"gfx950" in L.lib.gust_version()
+
+
+def test_option_refusals_name_their_keys():
+    import ablsteadyrans_amd as G
+    ok = G.LineSweepOptions(kind=G.LUSGS, ghosts=G.ZERO, overlap=2)
+    assert G.check_line_options(ok) == []
+    for bad, key in ((G.LineSweepOptions(kind=1, ghosts=1, overlap=-1), "steady.line_overlap"),
+                     (G.LineSweepOptions(kind=1, ghosts=1, overlap=9), "steady.line_overlap"),
+                     (G.LineSweepOptions(kind=1, ghosts=0, overlap=2), "steady.line_overlap"),
+                     (G.LineSweepOptions(kind=1, alternate=2), "steady.line_alternate")):
+        err = G.check_line_options(bad)
+        assert len(err) == 1 and key in err[0], err
+
+
+def _host_fab(geom, loc, ncomp, ngrow):
+    import ablsteadyrans_amd as G
+    return G.Fab.at_loc(geom, loc, ncomp, ngrow, backend="numpy")
+
+
+def test_system_refusals_before_any_device_call():
+    import ablsteadyrans_amd as G
+    geom = G.Geometry.of_size(4, 3, 5)
+    A = _host_fab(geom, G.CELL, G.N_COEFS, 0)
+    ap = _host_fab(geom, G.CELL, 1, 0)
+    rhs = _host_fab(geom, G.CELL, 1, 0)
+    d = _host_fab(geom, G.CELL, 1, 1)
+    with pytest.raises(G.GustError, match="gust: zline_jacobi: sweeps must be >= 1"):
+        G.zline_jacobi(G.CELL, A, ap, rhs, 0, geom, d, stream=0)
+    d0 = _host_fab(geom, G.CELL, 1, 0)
+    with pytest.raises(G.GustError, match="delta needs at least 1 ghost"):
+        G.zline_jacobi(G.CELL, A, ap, rhs, 1, geom, d0, stream=0)
+    A6 = _host_fab(geom, G.CELL, 6, 0)
+    with pytest.raises(G.GustError, match="implicit operator's components"):
+        G.zline_jacobi(G.CELL, A6, ap, rhs, 1, geom, d, stream=0)
+    # a face field on cell-centred data
+    with pytest.raises(G.GustError, match="x-faces"):
+        G.zline_jacobi(G.XFACE, A, ap, rhs, 1, geom, d, stream=0)
+    with pytest.raises(G.GustError, match="steady.line_overlap"):
+        G.zline_lusgs(G.CELL, A, ap, rhs, 1, geom, d, G.LineSweepOptions(kind=1, ghosts=1, overlap=5), stream=0)
+    # the message is per call: a later failure replaces it
+    with pytest.raises(G.GustError, match="sweeps"):
+        G.zline_lusgs(G.CELL, A, ap, rhs, -1, geom, d, stream=0)
+
+
+def test_array4_view_matches_amrex_indexing():
+    import ablsteadyrans_amd as G
+    f = G.Fab.alloc((2, -1, 0), (5, 3, 4), ncomp=3, ngrow=1, backend="numpy")
+    a = f.array4()
+    assert tuple(a.begin) == (1, -2, -1) and tuple(a.end) == (7, 5, 6)
+    f.data[...] = np.arange(f.data.size, dtype=np.float64).reshape(f.data.shape)
+    base = f.data.ctypes.data_as(C.POINTER(C.c_double))
+    for (i, j, k, n) in ((1, -2, -1, 0), (4, 2, 3, 2), (6, 4, 5, 1)):
+        off = (i - a.begin[0]) + (j - a.begin[1]) * a.jstride + (k - a.begin[2]) * a.kstride + n * a.nstride
+        assert base[off] == f.data[n, k - a.begin[2], j - a.begin[1], i - a.begin[0]]
