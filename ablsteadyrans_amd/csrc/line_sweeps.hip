@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 
@@ -56,10 +57,28 @@ struct JacobiSys {
     int pad;
 };
 
+// Global-memory accesses through address-space-1 pointers: the pointers
+// come out of a table in memory, so the compiler cannot infer their space
+// and would emit flat instructions (which also count against the LDS
+// counter and serialise the waits).
+#define GUST_GLOBAL __attribute__((address_space(1)))
+
 template <typename T>
 __device__ __forceinline__ T ldg(const T* p)
 {
-    return __builtin_nontemporal_load(p);
+    return __builtin_nontemporal_load((const GUST_GLOBAL T*)(p));
+}
+
+template <typename T>
+__device__ __forceinline__ T gld(const T* p)
+{
+    return *((const GUST_GLOBAL T*)(p));
+}
+
+template <typename T>
+__device__ __forceinline__ void gst(T* p, T v)
+{
+    *((GUST_GLOBAL T*)(p)) = v;
 }
 
 // Which system holds column tile t (tile0 ascending).
@@ -78,11 +97,47 @@ __device__ __forceinline__ int find_sys(const S* __restrict__ sys, int nsys, int
 //   kFirst: the input iterate is zero (the first sweep after the zeroing of
 //           check_line_system); its neighbour values are 0, not read.
 //   kLast:  record max |new - old| per system into changes (as ordered bits).
+//   kP:     prefetch depth: the operands of level k + kP are loaded while
+//           level k is eliminated (a register ring indexed at compile time),
+//           so every wave keeps ~kP levels of loads in flight instead of
+//           waiting one memory latency per level. kP x 12 loads stay under
+//           the 63 outstanding vector memory operations a wave can count.
 //   dst_scratch: 1 = write the scratch iterate, read the caller's delta;
 //                0 = the reverse.
 //   cglob: c' in global memory ([tile][k][lane]) when the LDS cannot hold
 //          64 x nz reals (null: c' in dynamic LDS).
-template <typename T, bool kFirst, bool kLast>
+template <typename T>
+struct LevelOps {
+    T axm, axp, aym, ayp, azm, azp, dg, rr, vm, vp, vjm, vjp;
+};
+
+template <typename T, bool kFirst, int kP>
+__device__ __forceinline__ void load_level(LevelOps<T>& o, const T* __restrict__ a, int64_t ak, int64_t an,
+                                           const T* __restrict__ pap, int64_t pk, const T* __restrict__ pr,
+                                           int64_t rk, const T* sm, const T* sp, const T* sjm, const T* sjp,
+                                           int64_t sk, bool okm, bool okp, bool okjm, bool okjp, int k)
+{
+    const T* ak0 = a + k * ak;
+    o.axm = ldg(ak0 + 1 * an);
+    o.axp = ldg(ak0 + 2 * an);
+    o.aym = ldg(ak0 + 3 * an);
+    o.ayp = ldg(ak0 + 4 * an);
+    o.azm = ldg(ak0 + 5 * an);
+    o.azp = ldg(ak0 + 6 * an);
+    o.dg = ldg(pap + k * pk);
+    o.rr = ldg(pr + k * rk);
+    if (!kFirst) {
+        const T tm = gld(sm + k * sk), tp = gld(sp + k * sk), tjm = gld(sjm + k * sk), tjp = gld(sjp + k * sk);
+        o.vm = okm ? tm : T(0);
+        o.vp = okp ? tp : T(0);
+        o.vjm = okjm ? tjm : T(0);
+        o.vjp = okjp ? tjp : T(0);
+    } else {
+        o.vm = o.vp = o.vjm = o.vjp = T(0);
+    }
+}
+
+template <typename T, bool kFirst, bool kLast, bool kLds, int kP = 4>
 __global__ __launch_bounds__(kWave) void zjacobi_sweep_kernel(const JacobiSys<T>* __restrict__ table, int nsys,
                                                               int total_tiles, int dst_scratch,
                                                               unsigned long long* __restrict__ changes,
@@ -91,7 +146,8 @@ __global__ __launch_bounds__(kWave) void zjacobi_sweep_kernel(const JacobiSys<T>
     extern __shared__ unsigned char lds_raw[];
     const int lane = threadIdx.x;
     const int tile = dev::xcd_remap(blockIdx.x, total_tiles);
-    T* cl = cglob ? cglob + size_t(tile) * kWave * size_t(nzmax) : reinterpret_cast<T*>(lds_raw);
+    T* cgl = cglob + (kLds ? 0 : size_t(tile) * kWave * size_t(nzmax));
+    T* cls = reinterpret_cast<T*>(lds_raw);
     const int si = find_sys(table, nsys, tile);
     const JacobiSys<T> S = table[si];
     const int c = (tile - S.tile0) * kWave + lane;
@@ -111,68 +167,104 @@ __global__ __launch_bounds__(kWave) void zjacobi_sweep_kernel(const JacobiSys<T>
         if (ip > S.nx - 1) { if (S.Lx) { ip -= S.Lx; } else { okp = false; ip = i; } }
         if (jm < 0) { if (S.Ly) { jm += S.Ly; } else { okjm = false; jm = j; } }
         if (jp > S.ny - 1) { if (S.Ly) { jp -= S.Ly; } else { okjp = false; jp = j; } }
-        const T* a = S.A + i + int64_t(j) * S.a_j;
-        const T* pap = S.ap + i + int64_t(j) * S.p_j;
-        const T* pr = S.r + i + int64_t(j) * S.r_j;
+        const T* __restrict__ a = S.A + i + int64_t(j) * S.a_j;
+        const T* __restrict__ pap = S.ap + i + int64_t(j) * S.p_j;
+        const T* __restrict__ pr = S.r + i + int64_t(j) * S.r_j;
         const T* sm = src + im + int64_t(j) * src_j;
         const T* sp = src + ip + int64_t(j) * src_j;
         const T* sjm = src + i + int64_t(jm) * src_j;
         const T* sjp = src + i + int64_t(jp) * src_j;
+        const T* sc = src + i + int64_t(j) * src_j;
         T* out = dst + i + int64_t(j) * dst_j;
         const int64_t ak = S.a_k, an = S.a_n;
         const int nz = S.nz;
+        // Forward elimination (Source/LineSolve_K.H:29-42) over a ring of kP
+        // prefetched levels. Full chunks of kP levels run branch-free (the
+        // prefetch level is clamped to the column, a harmless re-read near
+        // its top); the tail of nz % kP levels issues no loads.
+        LevelOps<T> ring[kP];
+#pragma unroll
+        for (int u = 0; u < kP; ++u) {
+            load_level<T, kFirst, kP>(ring[u], a, ak, an, pap, S.p_k, pr, S.r_k, sm, sp, sjm, sjp, src_k, okm, okp,
+                                      okjm, okjp, u < nz ? u : nz - 1);
+        }
         T cprev = T(0), eprev = T(0);
-        // Forward elimination (Source/LineSolve_K.H:29-42).
-#pragma unroll 4
-        for (int k = 0; k < nz; ++k) {
-            const T* ak0 = a + k * ak;
-            const T axm = ldg(ak0 + 1 * an), axp = ldg(ak0 + 2 * an), aym = ldg(ak0 + 3 * an),
-                    ayp = ldg(ak0 + 4 * an), azm = ldg(ak0 + 5 * an), azp = ldg(ak0 + 6 * an);
-            const T dgk = ldg(pap + k * S.p_k), rr = ldg(pr + k * S.r_k);
-            T vm = T(0), vp = T(0), vjm = T(0), vjp = T(0);
-            if (!kFirst) {
-                const T tm = sm[k * src_k], tp = sp[k * src_k], tjm = sjm[k * src_k], tjp = sjp[k * src_k];
-                vm = okm ? tm : T(0);
-                vp = okp ? tp : T(0);
-                vjm = okjm ? tjm : T(0);
-                vjp = okjp ? tjp : T(0);
-            }
-            const T rk = rr + axm * vm + axp * vp + aym * vjm + ayp * vjp;
-            const T lower = -azm, upper = -azp;
+        auto eliminate = [&](const LevelOps<T>& o, int k) {
+            const T rk = o.rr + o.axm * o.vm + o.axp * o.vp + o.aym * o.vjm + o.ayp * o.vjp;
+            const T lower = -o.azm, upper = -o.azp;
             T ck, ek;
             if (k == 0) {
-                const T m = dgk;
+                const T m = o.dg;
                 ck = upper / m;
                 ek = rk / m;
             } else {
-                const T m = dgk - lower * cprev;
+                const T m = o.dg - lower * cprev;
                 ck = upper / m;
                 ek = (rk - lower * eprev) / m;
             }
-            cl[k * kWave + lane] = ck;
-            out[k * dst_k] = ek;
+            if (kLds) { cls[k * kWave + lane] = ck; } else { gst(cgl + k * kWave + lane, ck); }
+            gst(out + k * dst_k, ek);
             cprev = ck;
             eprev = ek;
+        };
+        const int nfull = (nz / kP) * kP;
+        for (int k0 = 0; k0 < nfull; k0 += kP) {
+#pragma unroll
+            for (int u = 0; u < kP; ++u) {
+                const int k = k0 + u;
+                const LevelOps<T> o = ring[u];
+                const int kn = k + kP < nz ? k + kP : nz - 1;
+                load_level<T, kFirst, kP>(ring[u], a, ak, an, pap, S.p_k, pr, S.r_k, sm, sp, sjm, sjp, src_k, okm, okp,
+                                          okjm, okjp, kn);
+                eliminate(o, k);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kP; ++u) {
+            if (nfull + u < nz) { eliminate(ring[u], nfull + u); }
         }
         // Back substitution (Source/LineSolve_K.H:43-44); e' is re-read from
-        // the output column, c' from LDS.
+        // the output column kB levels ahead (clamped the same way), c' from
+        // LDS. Level m of the ring walk is k = nz - 2 - m.
+        constexpr int kB = 8;
         T o = eprev;
         if (kLast) {
-            const T old = kFirst ? T(0) : src[i + int64_t(j) * src_j + (nz - 1) * src_k];
+            const T old = kFirst ? T(0) : gld(sc + (nz - 1) * src_k);
             const unsigned long long b = dev::abs_bits(double(o - old));
             chg = b > chg ? b : chg;
         }
-#pragma unroll 4
-        for (int k = nz - 2; k >= 0; --k) {
-            const T ek = out[k * dst_k];
-            const T ck = cl[k * kWave + lane];
+        const int nb = nz - 1;   // levels to substitute
+        T eb[kB], ob[kB];
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            const int k = nz - 2 - u >= 0 ? nz - 2 - u : 0;
+            eb[u] = gld(out + k * dst_k);
+            ob[u] = (kLast && !kFirst) ? gld(sc + k * src_k) : T(0);
+        }
+        auto substitute = [&](T ek, T oldk, int k) {
+            const T ck = kLds ? cls[k * kWave + lane] : gld(cgl + k * kWave + lane);
             o = ek - ck * o;
-            out[k * dst_k] = o;
+            gst(out + k * dst_k, o);
             if (kLast) {
-                const T old = kFirst ? T(0) : src[i + int64_t(j) * src_j + k * src_k];
-                const unsigned long long b = dev::abs_bits(double(o - old));
+                const unsigned long long b = dev::abs_bits(double(o - oldk));
                 chg = b > chg ? b : chg;
             }
+        };
+        const int nbfull = nb > 0 ? (nb / kB) * kB : 0;
+        for (int m0 = 0; m0 < nbfull; m0 += kB) {
+#pragma unroll
+            for (int u = 0; u < kB; ++u) {
+                const int k = nz - 2 - (m0 + u);
+                const T ek = eb[u], oldk = ob[u];
+                const int kn = k - kB >= 0 ? k - kB : 0;
+                eb[u] = gld(out + kn * dst_k);
+                ob[u] = (kLast && !kFirst) ? gld(sc + kn * src_k) : T(0);
+                substitute(ek, oldk, k);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            if (nbfull + u < nb) { substitute(eb[u], ob[u], nz - 2 - (nbfull + u)); }
         }
     }
     if (kLast) {
@@ -352,7 +444,11 @@ struct JacobiPlan final : JacobiPlanBase {
             nzmax = std::max(nzmax, t.nz);
         }
         const size_t lds = size_t(kWave) * size_t(nzmax) * sizeof(T);
-        const bool use_lds = lds <= device_info().lds_per_block;
+        // Schedule: c' in LDS (default) or in global memory (GUST_JACOBI_CPRIME
+        // = global: no LDS limit on occupancy, c' round-trips through L2).
+        const char* cp = std::getenv("GUST_JACOBI_CPRIME");
+        const bool force_global = cp && std::strcmp(cp, "global") == 0;
+        const bool use_lds = !force_global && lds <= device_info().lds_per_block;
         const size_t tab_bytes = tab.size() * sizeof(JacobiSys<T>);
         const size_t cbytes = use_lds ? 0 : size_t(tiles) * lds;
         if (own) {
@@ -376,12 +472,18 @@ struct JacobiPlan final : JacobiPlanBase {
     template <bool F, bool L>
     void launch(int dst_scratch, hipStream_t st)
     {
-        const size_t lds = cglob ? 0 : size_t(kWave) * size_t(nzmax) * sizeof(T);
-        if (lds > 65536) {
-            GUST_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&zjacobi_sweep_kernel<T, F, L>),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+        if (cglob) {
+            zjacobi_sweep_kernel<T, F, L, false><<<tiles, kWave, 0, st>>>(dtab, nsys, tiles, dst_scratch, chg, cglob,
+                                                                       nzmax);
+        } else {
+            const size_t lds = size_t(kWave) * size_t(nzmax) * sizeof(T);
+            if (lds > 65536) {
+                GUST_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&zjacobi_sweep_kernel<T, F, L, true>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+            }
+            zjacobi_sweep_kernel<T, F, L, true><<<tiles, kWave, lds, st>>>(dtab, nsys, tiles, dst_scratch, chg,
+                                                                           cglob, nzmax);
         }
-        zjacobi_sweep_kernel<T, F, L><<<tiles, kWave, lds, st>>>(dtab, nsys, tiles, dst_scratch, chg, cglob, nzmax);
         GUST_HIP(hipGetLastError());
     }
 

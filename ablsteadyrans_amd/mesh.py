@@ -1,0 +1,402 @@
+"""Host-side construction of a tile's terrain-following mesh (built once per tile).
+
+Restates the reference's mesh pipeline operation by operation (float64 numpy,
+no contraction, so the node heights and metrics equal the CPU build's):
+
+* z-levels: Source/ZLevels.cpp:66-126 (fixed ratio, or the geometric stretch
+  of extern_erf/ERF_GridUtils.H:211-294 when only dz0 is given);
+* ground heights: Source/TerrainSurface.cpp:302-417 (flat, Witch of Agnesi,
+  ERF grid file with erf_bilinear 220-300) plus the analytic grounds of the
+  benchmark configurations (Gaussian hills, a seeded fractal surface);
+* basic terrain following (BTF): extern_erf/ERF_TerrainMetrics.cpp:94-242 and
+  252-380, with Source/Mesh.cpp:112-150 (OverrideSync, FillBoundary) and the
+  ghost-column fill of Source/Mesh.cpp:37-98;
+* metrics: make_J / make_areas / make_zcc, extern_erf/ERF_TerrainMetrics.cpp:679-807.
+
+Arrays are (nz, ny, nx) numpy float64 with an explicit ``begin`` (i, j, k)
+origin, i fastest -- the Array4 layout. A tile is one box spanning its
+domain. The mesh is an input to the device kernels; nothing here runs per
+iteration.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .fab import Geometry
+
+BOGUS = 1.0e30   # ERF's bogus_large_value (overwritten before any use)
+
+
+def geometry(n_cell, prob_lo, prob_hi, periodic=(0, 0)) -> Geometry:
+    """amrex::Geometry of Source/State.cpp:43-48: domain (0 .. n-1), cell size
+    (prob_hi - prob_lo) / n."""
+    dx = tuple((float(prob_hi[d]) - float(prob_lo[d])) / float(n_cell[d]) for d in range(3))
+    return Geometry((0, 0, 0), tuple(int(n) - 1 for n in n_cell), (int(periodic[0]), int(periodic[1]), 0), dx)
+
+
+# ---------------------------------------------------------------------------
+# z-levels
+
+def geometric_stretch(dz0: float, z_top: float, nz: int, tol: float):
+    """extern_erf/ERF_GridUtils.H:211-260: Newton for the factor r with
+    dz0 (r^nz - 1)/(r - 1) = z_top. Returns (dz0, ratio, converged)."""
+    Nz = float(nz)
+    if nz < 1 or not (z_top > 0.0) or not (dz0 > 0.0):
+        return 0.0, 1.0, False
+    if nz == 1 or dz0 >= z_top / Nz:
+        return z_top / Nz, 1.0, True
+    S = 1.03
+    F = dz0 * ((S ** Nz - 1.0) / (S - 1.0)) - z_top
+    it = 0
+    while abs(F) > tol and it < 50:
+        dF = dz0 * (Nz * S ** (Nz - 1.0) * (S - 1.0) - S ** Nz + 1.0) / (S - 1.0) ** 2.0
+        S -= F / dF
+        smin = 1.0 + tol
+        S = S if S > smin else smin
+        F = dz0 * ((S ** Nz - 1.0) / (S - 1.0)) - z_top
+        it += 1
+    return dz0, S, abs(F) <= tol
+
+
+def make_zlevels(nz: int, prob_hi_z: float, dz0: float = 0.0, stretch_ratio: float = 0.0,
+                 zlevels=None) -> np.ndarray:
+    """Source/ZLevels.cpp:66-126: explicit levels, uniform (dz0 <= 0),
+    geometric stretch to prob_hi z (ratio 0) or a fixed ratio from dz0."""
+    if zlevels is not None:
+        z = np.asarray(zlevels, dtype=np.float64)
+        if z.size != nz + 1:
+            raise ValueError(f"gust: zlevels has {z.size} values; n_cell z + 1 = {nz + 1} are needed")
+        if z[0] != 0:
+            raise ValueError("gust: zlevels: the first value must be 0")
+        if not np.all(np.isfinite(z)) or not np.all(np.diff(z) > 0):
+            raise ValueError("gust: zlevels: values must be finite and increase")
+        return z.copy()
+    z = np.zeros(nz + 1)
+    if not dz0 > 0:
+        dz = prob_hi_z / nz
+        for k in range(nz + 1):
+            z[k] = k * dz
+        return z
+    if stretch_ratio == 0:
+        d0, r, ok = geometric_stretch(dz0, prob_hi_z, nz, 1.0e-8)
+        if not ok:
+            raise ValueError(f"gust: mesh.dz0 = {dz0}: no geometric stretch factor reaches prob_hi z "
+                             "with n_cell z layers")
+        dz = d0
+        for k in range(1, nz + 1):
+            z[k] = z[k - 1] + dz
+            dz *= r
+        z[nz] = prob_hi_z
+        return z
+    dz = dz0
+    for k in range(1, nz + 1):
+        z[k] = z[k - 1] + dz
+        dz *= stretch_ratio
+    return z
+
+
+# ---------------------------------------------------------------------------
+# Ground surfaces: functions of the node index arrays (ii, jj) already mapped
+# to the node each ghost stands for, returning heights.
+
+def node_index(i, lo, hi, periodic):
+    """Source/TerrainSurface.cpp:55-60."""
+    i = np.asarray(i)
+    if not periodic:
+        return np.minimum(np.maximum(i, lo), hi)
+    n = hi - lo
+    return lo + (((i - lo) % n) + n) % n
+
+
+@dataclass
+class ErfGrid:
+    """An ERF terrain grid file (Source/TerrainSurface.cpp:187-218): nx, ny,
+    x[nx], y[ny], z[nx*ny] contiguous in y."""
+
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+
+    @staticmethod
+    def read(path: str) -> "ErfGrid":
+        v = np.loadtxt(path, dtype=np.float64).ravel()
+        nx, ny = int(v[0]), int(v[1])
+        if nx < 1 or ny < 1 or v.size != 2 + nx + ny + nx * ny:
+            raise ValueError(f"gust: terrain.file = {path}: malformed ERF grid")
+        return ErfGrid(v[2:2 + nx].copy(), v[2 + nx:2 + nx + ny].copy(), v[2 + nx + ny:].copy())
+
+    def bilinear(self, x: float, y: float) -> float:
+        """erf_bilinear, Source/TerrainSurface.cpp:220-300 (same branches and order)."""
+        xt, yt, zt = self.x, self.y, self.z
+        nx, ny = xt.size, yt.size
+        jj = np.nonzero(yt > y)[0]
+        jt = (jj[0] - 1) if jj.size else ny - 1
+        jt = max(jt, 0)
+        ii = np.nonzero(xt > x)[0]
+        it = (ii[0] - 1) if ii.size else nx - 1
+        it = max(it, 0)
+        ip1 = min(it + 1, nx - 1)
+        jp1 = min(jt + 1, ny - 1)
+        x1, x2, y1, y2 = float(xt[it]), float(xt[ip1]), float(yt[jt]), float(yt[jp1])
+        z11, z21 = float(zt[it * ny + jt]), float(zt[ip1 * ny + jt])
+        z12, z22 = float(zt[it * ny + jp1]), float(zt[ip1 * ny + jp1])
+        ix = (ip1 != it) and (x2 != x1)
+        iy = (jp1 != jt) and (y2 != y1)
+        if not ix and not iy:
+            return z11
+        if ix and not iy:
+            return ((x2 - x) * z11 + (x - x1) * z21) / (x2 - x1)
+        if not ix and iy:
+            return ((y2 - y) * z11 + (y - y1) * z12) / (y2 - y1)
+        w11 = (x2 - x) * (y2 - y)
+        w21 = (x - x1) * (y2 - y)
+        w12 = (x2 - x) * (y - y1)
+        w22 = (x - x1) * (y - y1)
+        den = (x2 - x1) * (y2 - y1)
+        return (w11 * z11 + w12 * z12 + w21 * z21 + w22 * z22) / den
+
+
+@dataclass
+class Terrain:
+    """A ground description: ``kind`` in flat | agnesi | erf_grid | hills | fractal."""
+
+    kind: str = "flat"
+    # agnesi (Source/TerrainSurface.cpp:329-386)
+    hill_height: float = 0.0
+    hill_half_width: float = 1.0
+    hill_shape: str = "hill"          # hill | ridge_x | ridge_y
+    # erf_grid
+    grid: ErfGrid | None = None
+    # hills: list of (xc, yc, H, L): h = sum H exp(-((x-xc)^2 + (y-yc)^2) / L^2)
+    hills: list = field(default_factory=list)
+    # fractal: a precomputed (ny+1, nx+1) node height array
+    heights: np.ndarray | None = None
+
+
+def ground_heights(t: Terrain, geom: Geometry, prob_lo, prob_hi) -> np.ndarray:
+    """Ground node heights on the nodal slab grown by one ghost node,
+    shape (ny + 3, nx + 3), index (j + 1, i + 1)."""
+    nx, ny, _ = geom.n
+    dx, dy = geom.dx[0], geom.dx[1]
+    i = np.arange(-1, nx + 2)
+    j = np.arange(-1, ny + 2)
+    perx, pery = bool(geom.periodic[0]), bool(geom.periodic[1])
+    out = np.zeros((ny + 3, nx + 3))
+    if t.kind == "flat":
+        return out
+    if t.kind == "agnesi":
+        hm, L = float(t.hill_height), float(t.hill_half_width)
+        xcen = 0.5 * (prob_lo[0] + prob_hi[0])
+        ycen = 0.5 * (prob_lo[1] + prob_hi[1])
+
+        def per_idx(a, n, per):
+            if per:
+                return np.where(a < 0, a + n, np.where(a > n, a - n, a))
+            return np.minimum(np.maximum(a, 0), n)
+
+        ii = per_idx(i, nx, perx)
+        jj = per_idx(j, ny, pery)
+        X = prob_lo[0] + ii * dx - xcen
+        Y = prob_lo[1] + jj * dy - ycen
+        if t.hill_shape == "ridge_x":
+            xl = X / L
+            return np.broadcast_to(hm / (1 + xl * xl), out.shape).copy()
+        if t.hill_shape == "ridge_y":
+            yl = Y / L
+            return np.broadcast_to((hm / (1.0 + yl * yl))[:, None], out.shape).copy()
+        XX, YY = np.meshgrid(X, Y)
+        r = np.sqrt(XX * XX + YY * YY)
+        rl = r / L
+        return hm / (1.0 + rl * rl)
+    ii = node_index(i, 0, nx, perx)
+    jj = node_index(j, 0, ny, pery)
+    if t.kind == "erf_grid":
+        for b, jv in enumerate(jj):
+            y = float(prob_lo[1]) + int(jv) * float(dy)
+            for a, iv in enumerate(ii):
+                x = float(prob_lo[0]) + int(iv) * float(dx)
+                out[b, a] = t.grid.bilinear(x + 1e-3, y + 1e-3)
+        return out
+    if t.kind == "hills":
+        X = prob_lo[0] + ii * dx
+        Y = prob_lo[1] + jj * dy
+        XX, YY = np.meshgrid(X, Y)
+        h = np.zeros_like(XX)
+        for (xc, yc, H, L) in t.hills:
+            h = h + H * np.exp(-((XX - xc) ** 2 + (YY - yc) ** 2) / (L * L))
+        return h
+    if t.kind == "fractal":
+        hv = np.asarray(t.heights, dtype=np.float64)
+        if hv.shape != (ny + 1, nx + 1):
+            raise ValueError("fractal heights must be (ny + 1, nx + 1) nodes")
+        return hv[np.ix_(jj, ii)]
+    raise ValueError(f"unknown terrain kind {t.kind}")
+
+
+def fractal_heights(nx: int, ny: int, seed: int, rms: float, slope: float = -2.0, clip=(0.0, 400.0)):
+    """A seeded isotropic Gaussian random field on the (ny+1, nx+1) nodes with
+    power spectrum |k|^slope (amplitude |k|^(slope/2)), zero mean, the given
+    rms, clipped (the fractal-terrain configuration)."""
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal((ny + 1, nx + 1))
+    W = np.fft.fft2(w)
+    ky = np.fft.fftfreq(ny + 1)[:, None]
+    kx = np.fft.fftfreq(nx + 1)[None, :]
+    k = np.sqrt(kx * kx + ky * ky)
+    amp = np.zeros_like(k)
+    amp[k > 0] = k[k > 0] ** (slope / 2.0)
+    h = np.real(np.fft.ifft2(W * amp))
+    h -= h.mean()
+    h *= rms / np.sqrt(np.mean(h * h))
+    return np.clip(h, clip[0], clip[1])
+
+
+# ---------------------------------------------------------------------------
+# The mesh.
+
+@dataclass
+class TileMesh:
+    geom: Geometry
+    prob_lo: tuple
+    prob_hi: tuple
+    zlevels: np.ndarray
+    z_nd: np.ndarray        # (nz+3, ny+3, nx+3), begin (-1, -1, -1)
+    detJ: np.ndarray        # (nz+2, ny+2, nx+2), begin (-1, -1, -1)
+    ax: np.ndarray          # (nz+2, ny+2, nx+3), begin (-1, -1, -1)
+    ay: np.ndarray          # (nz+2, ny+3, nx+2)
+    az: np.ndarray          # (nz+3, ny+2, nx+2)
+    z_cc: np.ndarray        # (nz+2, ny+2, nx+2)
+
+    BEGIN = (-1, -1, -1)
+
+    def valid(self, name: str) -> np.ndarray:
+        """The valid points of a metric (no ghosts), (nz', ny', nx')."""
+        a = getattr(self, name)
+        if name == "z_nd":
+            return a[1:-1, 1:-1, 1:-1]
+        return a[1:-1, 1:-1, 1:-1]
+
+
+def build_mesh(geom: Geometry, prob_lo, prob_hi, zlevels: np.ndarray, terrain: Terrain) -> TileMesh:
+    """BTF mesh of one tile (Source/Mesh.cpp:107-140 with terrain.smoothing = 0)."""
+    nx, ny, nz = geom.n
+    if zlevels.size != nz + 1:
+        raise ValueError("zlevels must have nz + 1 values")
+    perx, pery = bool(geom.periodic[0]), bool(geom.periodic[1])
+    dzeta = geom.dx[2]
+    # Z[k+1, j+1, i+1] = z_nd(i, j, k), i in [-1, nx+1], k in [-1, nz+1]
+    Z = np.zeros((nz + 3, ny + 3, nx + 3))
+    Z[1] = ground_heights(terrain, geom, prob_lo, prob_hi)
+    # setDomainBndry(bogus): nodes outside the domain along non-periodic axes
+    Z[0] = BOGUS
+    Z[nz + 2] = BOGUS
+    if not perx:
+        Z[2:nz + 2, :, 0] = BOGUS
+        Z[2:nz + 2, :, nx + 2] = BOGUS
+    if not pery:
+        Z[2:nz + 2, 0, :] = BOGUS
+        Z[2:nz + 2, ny + 2, :] = BOGUS
+    # BTF (extern_erf/ERF_TerrainMetrics.cpp:330-356): the clamped column's
+    # ground for every column of the grown box, k = 0 .. nz.
+    ic = np.minimum(np.maximum(np.arange(-1, nx + 2), 0), nx) + 1
+    jc = np.minimum(np.maximum(np.arange(-1, ny + 2), 0), ny) + 1
+    zs = Z[1][np.ix_(jc, ic)].copy()
+    z_top = float(zlevels[nz])
+    z_lev_sfc = float(zlevels[0])
+    Z[1] = zs
+    for k in range(1, nz + 1):
+        z = float(zlevels[k])
+        Z[k + 1] = ((zs - z_lev_sfc) * z_top + (z_top - zs) * z) / (z_top - z_lev_sfc)
+    _fill_boundary_nodal(Z, nx, ny, perx, pery)
+    Z[0] = 2.0 * Z[1] - Z[2]
+    _fill_boundary_nodal(Z, nx, ny, perx, pery)
+    Z[nz + 2] = Z[nz + 1] + 1 * (Z[nz + 1] - Z[nz])
+    # Source/Mesh.cpp:129-133: OverrideSync, FillBoundary, ghost columns.
+    if perx:
+        Z[:, :, nx + 1] = Z[:, :, 1]
+    if pery:
+        Z[:, ny + 1, :] = Z[:, 1, :]
+    _fill_boundary_nodal(Z, nx, ny, perx, pery)
+    _fill_ghost_columns(Z, nx, ny, nz, perx, pery)
+    # metrics (extern_erf/ERF_TerrainMetrics.cpp:679-807)
+    dzInv = 1.0 / dzeta
+    detJ = np.zeros((nz + 2, ny + 2, nx + 2))
+    # cells i in [-1, nx], k in [0, nz]: Z index offsets +1
+    a = Z[2:nz + 3, 0:ny + 2, 0:nx + 2]
+    b = Z[2:nz + 3, 0:ny + 2, 1:nx + 3]
+    c = Z[2:nz + 3, 1:ny + 3, 0:nx + 2]
+    d = Z[2:nz + 3, 1:ny + 3, 1:nx + 3]
+    e = Z[1:nz + 2, 0:ny + 2, 0:nx + 2]
+    f = Z[1:nz + 2, 0:ny + 2, 1:nx + 3]
+    g = Z[1:nz + 2, 1:ny + 3, 0:nx + 2]
+    h = Z[1:nz + 2, 1:ny + 3, 1:nx + 3]
+    detJ[1:] = 0.25 * dzInv * (a + b + c + d - e - f - g - h)
+    _fill_boundary_cell(detJ, nx, ny, perx, pery, nodal=(0, 0))
+    ax = np.zeros((nz + 2, ny + 2, nx + 3))
+    ax[1:] = 0.5 * dzInv * (Z[2:nz + 3, 0:ny + 2, :] + Z[2:nz + 3, 1:ny + 3, :]
+                            - Z[1:nz + 2, 0:ny + 2, :] - Z[1:nz + 2, 1:ny + 3, :])
+    ay = np.zeros((nz + 2, ny + 3, nx + 2))
+    ay[1:] = 0.5 * dzInv * (Z[2:nz + 3, :, 0:nx + 2] + Z[2:nz + 3, :, 1:nx + 3]
+                            - Z[1:nz + 2, :, 0:nx + 2] - Z[1:nz + 2, :, 1:nx + 3])
+    _fill_boundary_cell(ax, nx, ny, perx, pery, nodal=(1, 0))
+    _fill_boundary_cell(ay, nx, ny, perx, pery, nodal=(0, 1))
+    az = np.ones((nz + 3, ny + 2, nx + 2))
+    zcc = np.zeros((nz + 2, ny + 2, nx + 2))
+    zcc[:] = 0.125 * (Z[0:nz + 2, 0:ny + 2, 0:nx + 2] + Z[0:nz + 2, 0:ny + 2, 1:nx + 3]
+                      + Z[0:nz + 2, 1:ny + 3, 0:nx + 2] + Z[0:nz + 2, 1:ny + 3, 1:nx + 3]
+                      + Z[1:nz + 3, 0:ny + 2, 0:nx + 2] + Z[1:nz + 3, 0:ny + 2, 1:nx + 3]
+                      + Z[1:nz + 3, 1:ny + 3, 0:nx + 2] + Z[1:nz + 3, 1:ny + 3, 1:nx + 3])
+    _fill_boundary_cell(zcc, nx, ny, perx, pery, nodal=(0, 0))
+    mesh = TileMesh(geom, tuple(prob_lo), tuple(prob_hi), zlevels.copy(), Z, detJ, ax, ay, az, zcc)
+    ground = Z[1, 1:ny + 2, 1:nx + 2]
+    if not float(ground.max()) < z_top:
+        raise ValueError(f"gust: the terrain reaches {float(ground.max())} m, at or above the top level {z_top} m")
+    th = np.diff(Z[1:nz + 2, 1:ny + 2, 1:nx + 2], axis=0)
+    if not np.all(th > 0):
+        raise ValueError("gust: the terrain-following mesh has inverted or empty cells")
+    return mesh
+
+
+def _fill_boundary_nodal(Z, nx, ny, perx, pery):
+    """FillBoundary of a nodal array with one ghost node (one box): a ghost
+    node along a periodic axis takes its image (node -1 <- n-1, n+1 <- 1)."""
+    if perx:
+        Z[:, :, 0] = Z[:, :, nx]
+        Z[:, :, nx + 2] = Z[:, :, 2]
+    if pery:
+        Z[:, 0, :] = Z[:, ny, :]
+        Z[:, ny + 2, :] = Z[:, 2, :]
+
+
+def _fill_boundary_cell(a, nx, ny, perx, pery, nodal=(0, 0)):
+    """FillBoundary of a one-ghost array at a cell or face location (one box)."""
+    if perx:
+        if nodal[0]:
+            a[..., 0] = a[..., nx]
+            a[..., nx + 2] = a[..., 2]
+        else:
+            a[..., 0] = a[..., nx]
+            a[..., nx + 1] = a[..., 1]
+    if pery:
+        if nodal[1]:
+            a[..., 0, :] = a[..., ny, :]
+            a[..., ny + 2, :] = a[..., 2, :]
+        else:
+            a[..., 0, :] = a[..., ny, :]
+            a[..., ny + 1, :] = a[..., 1, :]
+
+
+def _fill_ghost_columns(Z, nx, ny, nz, perx, pery):
+    """Source/Mesh.cpp:37-98: a node outside a non-periodic face takes its
+    row's edge node (k = 0..nz); every column's rows below the ground and
+    above the top are recomputed from it."""
+    if not perx:
+        Z[1:nz + 2, :, 0] = Z[1:nz + 2, :, 1]
+        Z[1:nz + 2, :, nx + 2] = Z[1:nz + 2, :, nx + 1]
+    if not pery:
+        Z[1:nz + 2, 0, :] = Z[1:nz + 2, 1, :]
+        Z[1:nz + 2, ny + 2, :] = Z[1:nz + 2, ny + 1, :]
+    Z[0] = 2.0 * Z[1] - Z[2]
+    Z[nz + 2] = Z[nz + 1] + 1 * (Z[nz + 1] - Z[nz])

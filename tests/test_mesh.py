@@ -1,0 +1,85 @@
+"""The host mesh builder (ablsteadyrans_amd/mesh.py) against the reference's
+fixtures and closed forms.
+
+* Askervein 60 x 60 x 18 BTF mesh: sum, min, max and sum of squares of z_cc
+  and detJ over the valid cells equal Tests/Discretization/Terrain_Mesh/Gold/
+  askervein_mesh.json to 1e-12 relative (the tolerance of
+  Tests/Supporting_Files/check_mesh.py).
+* Flat ground: J = dz_k / dzeta, areas = J's layer ratio, z_cc the layer
+  midpoints (closed forms).
+* The geometric stretch closes on z_top (extern_erf/ERF_GridUtils.H:273-294).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from ablsteadyrans_amd import mesh as M
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def askervein_grid():
+    d = np.load(os.path.join(GOLD, "askervein_grid.npz"))
+    return M.ErfGrid(d["x"], d["y"], d["z"])
+
+
+def askervein_mesh(n_xy):
+    geom = M.geometry((n_xy, n_xy, 18), (-3000.0, -3000.0, 0.0), (3000.0, 3000.0, 440.0))
+    zl = M.make_zlevels(18, 440.0, dz0=20.0, stretch_ratio=1.08)
+    return M.build_mesh(geom, (-3000.0, -3000.0, 0.0), (3000.0, 3000.0, 440.0), zl,
+                        M.Terrain(kind="erf_grid", grid=askervein_grid()))
+
+
+def test_askervein_mesh_statistics_match_the_gold():
+    m = askervein_mesh(60)
+    g = json.load(open(os.path.join(GOLD, "askervein_mesh.json")))["stats"]
+    for name, arr in (("z", m.z_cc[1:-1, 1:-1, 1:-1]), ("detJ", m.detJ[1:-1, 1:-1, 1:-1])):
+        vals = arr.ravel()
+        got = {"sum": float(np.sum(vals)), "min": float(vals.min()), "max": float(vals.max()),
+               "sum_sq": float(np.sum(vals * vals))}
+        for key in ("sum", "min", "max", "sum_sq"):
+            assert abs(got[key] - g[name][key]) <= 1e-12 * abs(g[name][key]), (name, key, got[key], g[name][key])
+
+
+def test_flat_mesh_closed_forms():
+    geom = M.geometry((6, 5, 7), (0.0, 0.0, 0.0), (600.0, 500.0, 350.0), periodic=(1, 0))
+    zl = M.make_zlevels(7, 350.0, dz0=10.0, stretch_ratio=1.2)
+    m = M.build_mesh(geom, (0.0, 0.0, 0.0), (600.0, 500.0, 350.0), zl, M.Terrain())
+    dz = np.diff(zl)
+    J = m.detJ[1:-1, 1:-1, 1:-1]
+    assert np.allclose(J, (dz / 50.0)[:, None, None], rtol=1e-14, atol=0)
+    assert np.allclose(m.ax[1:-1, 1:-1, 1:-1], (dz / 50.0)[:, None, None], rtol=1e-14, atol=0)
+    assert np.all(m.az == 1.0)
+    zc = 0.5 * (zl[1:] + zl[:-1])
+    assert np.allclose(m.z_cc[1:-1, 1:-1, 1:-1], zc[:, None, None], rtol=1e-14, atol=0)
+    # ghost rows: linear extrapolation below the ground and above the top
+    assert np.allclose(m.z_nd[0], 2 * zl[0] - zl[1])
+    assert np.allclose(m.z_nd[-1], zl[-1] + (zl[-1] - zl[-2]))
+
+
+def test_geometric_stretch_closes_on_the_top():
+    zl = M.make_zlevels(96, 2000.0, dz0=2.0, stretch_ratio=0.0)
+    assert zl[-1] == 2000.0 and zl[1] == 2.0
+    r = (zl[2] - zl[1]) / (zl[1] - zl[0])
+    assert abs(r - 1.0392) < 1e-3          # the ratio quoted for the fractal tile
+    assert np.all(np.diff(zl) > 0)
+
+
+def test_btf_hill_is_terrain_following_and_periodic_ghosts_are_images():
+    geom = M.geometry((16, 12, 8), (0.0, 0.0, 0.0), (1600.0, 1200.0, 800.0), periodic=(1, 1))
+    zl = M.make_zlevels(8, 800.0)
+    t = M.Terrain(kind="hills", hills=[(800.0, 600.0, 100.0, 250.0)])
+    m = M.build_mesh(geom, (0.0, 0.0, 0.0), (1600.0, 1200.0, 800.0), zl, t)
+    Z = m.z_nd
+    # top stays flat, ground follows the hill, periodic ghost nodes are images
+    assert np.allclose(Z[9, 1:-1, 1:-1], 800.0, atol=1e-9)
+    assert Z[1, 7, 9] > 90.0
+    assert np.array_equal(Z[:, :, 0], Z[:, :, 16]) and np.array_equal(Z[:, :, 18], Z[:, :, 2])
+    assert np.array_equal(Z[:, 0, :], Z[:, 12, :])
+    # sum over the domain of J * dzeta = the column depth (the metric's closure)
+    J = m.detJ[1:-1, 1:-1, 1:-1]
+    depth = (Z[9, 1:-1, 1:-1] - Z[1, 1:-1, 1:-1])
+    cd = 0.25 * (depth[:-1, :-1] + depth[1:, :-1] + depth[:-1, 1:] + depth[1:, 1:])
+    assert np.allclose(J.sum(axis=0) * 100.0, cd, rtol=1e-12)
