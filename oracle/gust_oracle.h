@@ -75,6 +75,20 @@ int oracle_zline_lusgs(int loc, const oa4* A, const oa4* ap, const oa4* rhs, int
  * (line_sweep_launches, Source/LineSweeps.H:106-111) for the last call. */
 uint64_t oracle_last_lusgs_launches(void);
 
+/* Terrain Poisson operator (oracle/poisson.c). bc: 6 codes in
+ * amrex::Orientation order xlo, ylo, zlo, xhi, yhi, zhi; 0 periodic,
+ * 1 neumann, 2 dirichlet (Source/Poisson.H:20-34). phi needs one ghost cell,
+ * zp (nodal heights) one ghost node. */
+const char* oracle_poisson_error(void);
+int oracle_poisson_zero_flux_mask(const int* bc);
+int oracle_poisson_fill_ghosts(oa4* phi, const ogeom* g, const int* bc);
+int oracle_poisson_apply(oa4* y, oa4* phi, const oa4* ax, const oa4* ay, const oa4* az, const oa4* J,
+                         const oa4* zp, const oa4* dcross, const oa4* az_mesh, const ogeom* g, const int* bc);
+int oracle_poisson_fluxes(oa4* fx, oa4* fy, oa4* fz, oa4* phi, const oa4* zp, const oa4* bx, const oa4* by,
+                          const oa4* bz, const oa4* dcross, const ogeom* g, const int* bc);
+void oracle_face_divergence(oa4* div, const oa4* fx, const oa4* fy, const oa4* fz, const oa4* ax, const oa4* ay,
+                            const oa4* az, const oa4* J, const ogeom* g);
+
 #ifdef __cplusplus
 }
 #endif

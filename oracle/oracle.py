@@ -55,6 +55,17 @@ def lib():
                                           P(C.c_double)]
         L.oracle_zline_lusgs.argtypes = [C.c_int, P(OA4), P(OA4), P(OA4), C.c_int, P(OGeom), P(OA4), C.c_int,
                                          C.c_int, C.c_int, C.c_int, P(C.c_double)]
+        L.oracle_poisson_error.restype = C.c_char_p
+        I6 = C.c_int * 6
+        L.oracle_poisson_zero_flux_mask.argtypes = [I6]
+        L.oracle_poisson_fill_ghosts.argtypes = [P(OA4), P(OGeom), I6]
+        L.oracle_poisson_apply.argtypes = [P(OA4), P(OA4), P(OA4), P(OA4), P(OA4), P(OA4), P(OA4), P(OA4), P(OA4),
+                                           P(OGeom), I6]
+        L.oracle_poisson_fluxes.argtypes = [P(OA4), P(OA4), P(OA4), P(OA4), P(OA4), P(OA4), P(OA4), P(OA4), P(OA4),
+                                            P(OGeom), I6]
+        L.oracle_face_divergence.argtypes = [P(OA4), P(OA4), P(OA4), P(OA4), P(OA4), P(OA4), P(OA4), P(OA4),
+                                             P(OGeom)]
+        L.oracle_face_divergence.restype = None
         _lib = L
     return _lib
 
@@ -111,3 +122,66 @@ def zline_lusgs(loc, A, A_begin, ap, rhs, v_begin, sweeps, geom, delta, d_begin,
 
 def last_lusgs_launches() -> int:
     return int(lib().oracle_last_lusgs_launches())
+
+
+# ---------------------------------------------------------------------------
+# Terrain Poisson operator. Fields are (nz', ny', nx') numpy arrays with a
+# begin origin; helpers below take (array, begin) pairs.
+
+PERIODIC, NEUMANN, DIRICHLET = 0, 1, 2
+
+
+def _bc(bc):
+    return (C.c_int * 6)(*[int(b) for b in bc])
+
+
+def _v(a, begin):
+    if a is None:
+        return None
+    a = a if a.ndim == 4 else a[None]
+    return C.byref(a4(a, begin))
+
+
+def _pcheck(rc):
+    if rc != 0:
+        raise OracleError(lib().oracle_poisson_error().decode())
+
+
+def poisson_fill_ghosts(phi, phi_begin, geom, bc):
+    """Source/Poisson.cpp:84-125 in place (phi with one ghost cell)."""
+    _pcheck(lib().oracle_poisson_fill_ghosts(_v(phi, phi_begin), geom, _bc(bc)))
+
+
+def poisson_apply(phi, phi_begin, areas, areas_begin, J, J_begin, zp, zp_begin, geom, bc, dcross=None,
+                  az_mesh=None):
+    """A phi on the valid cells (returns (nz, ny, nx)); phi's ghosts are filled.
+    areas = (ax, ay, az) premultiplied, each with the same begin."""
+    n = (geom.hi[2] - geom.lo[2] + 1, geom.hi[1] - geom.lo[1] + 1, geom.hi[0] - geom.lo[0] + 1)
+    y = np.zeros((1,) + n)
+    ax, ay, az = areas
+    _pcheck(lib().oracle_poisson_apply(_v(y, tuple(geom.lo)), _v(phi, phi_begin), _v(ax, areas_begin),
+                                       _v(ay, areas_begin), _v(az, areas_begin), _v(J, J_begin), _v(zp, zp_begin),
+                                       _v(dcross, areas_begin), _v(az_mesh, areas_begin), geom, _bc(bc)))
+    return y[0]
+
+
+def poisson_fluxes(phi, phi_begin, zp, zp_begin, geom, bc, coeffs=(None, None, None), coeff_begin=(0, 0, 0),
+                   dcross=None):
+    """Fluxes on the valid faces (x: (nz, ny, nx+1), ...), begin = geom.lo."""
+    nx, ny, nz = (geom.hi[0] - geom.lo[0] + 1, geom.hi[1] - geom.lo[1] + 1, geom.hi[2] - geom.lo[2] + 1)
+    fx, fy, fz = np.zeros((1, nz, ny, nx + 1)), np.zeros((1, nz, ny + 1, nx)), np.zeros((1, nz + 1, ny, nx))
+    lo = tuple(geom.lo)
+    bx, by, bz = coeffs
+    _pcheck(lib().oracle_poisson_fluxes(_v(fx, lo), _v(fy, lo), _v(fz, lo), _v(phi, phi_begin), _v(zp, zp_begin),
+                                        _v(bx, coeff_begin), _v(by, coeff_begin), _v(bz, coeff_begin),
+                                        _v(dcross, coeff_begin), geom, _bc(bc)))
+    return fx[0], fy[0], fz[0]
+
+
+def face_divergence(F, F_begin, areas, areas_begin, J, J_begin, geom):
+    n = (geom.hi[2] - geom.lo[2] + 1, geom.hi[1] - geom.lo[1] + 1, geom.hi[0] - geom.lo[0] + 1)
+    div = np.zeros((1,) + n)
+    lib().oracle_face_divergence(_v(div, tuple(geom.lo)), _v(F[0], F_begin), _v(F[1], F_begin), _v(F[2], F_begin),
+                                 _v(areas[0], areas_begin), _v(areas[1], areas_begin), _v(areas[2], areas_begin),
+                                 _v(J, J_begin), geom)
+    return div[0]
