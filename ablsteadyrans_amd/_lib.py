@@ -62,6 +62,46 @@ class LineSystemC(C.Structure):
     ]
 
 
+class PoissonOptionsC(C.Structure):
+    """``gust_poisson_options``."""
+
+    _fields_ = [
+        ("reltol", C.c_double),
+        ("abstol", C.c_double),
+        ("max_iter", C.c_int32),
+        ("restart", C.c_int32),
+        ("verbose", C.c_int32),
+        ("precision", C.c_int32),
+        ("pre_sweeps", C.c_int32),
+        ("post_sweeps", C.c_int32),
+        ("coarse_sweeps", C.c_int32),
+    ]
+
+
+class FaceCoeffsC(C.Structure):
+    """``gust_face_coeffs``."""
+
+    _fields_ = [
+        ("x", C.POINTER(Array4)),
+        ("y", C.POINTER(Array4)),
+        ("z", C.POINTER(Array4)),
+        ("z_cross", C.POINTER(Array4)),
+        ("precond_ratio", C.c_double),
+    ]
+
+
+class PoissonResultC(C.Structure):
+    """``gust_poisson_result``."""
+
+    _fields_ = [
+        ("iterations", C.c_int32),
+        ("initial_residual", C.c_double),
+        ("final_residual", C.c_double),
+        ("singular", C.c_int32),
+        ("removed_mean", C.c_double),
+    ]
+
+
 def _load() -> C.CDLL:
     if not os.path.exists(LIB_PATH):
         raise ImportError(
@@ -93,6 +133,22 @@ _sig = {
     "gust_jacobi_plan_sweep": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]),
     "gust_jacobi_plan_info": (C.c_int, [C.c_void_p, _P(C.c_int64), _P(C.c_int32)]),
     "gust_jacobi_plan_destroy": (None, [C.c_void_p]),
+    "gust_poisson_default_options": (C.c_int, [_P(PoissonOptionsC)]),
+    "gust_poisson_create": (C.c_int, [_P(GeometryC), _P(Array4), _P(Array4), _P(Array4), _P(Array4), _P(Array4),
+                                      _P(PoissonOptionsC), _P(C.c_void_p), C.c_void_p]),
+    "gust_poisson_destroy": (None, [C.c_void_p]),
+    "gust_poisson_levels": (C.c_int, [C.c_void_p, _P(C.c_int32)]),
+    "gust_poisson_solve": (C.c_int, [C.c_void_p, _P(Array4), _P(Array4), _P(FaceCoeffsC), _P(C.c_int32),
+                                     _P(Array4), _P(Array4), _P(Array4), _P(PoissonResultC), C.c_void_p]),
+    "gust_poisson_apply": (C.c_int, [C.c_void_p, _P(Array4), _P(Array4), _P(FaceCoeffsC), _P(C.c_int32),
+                                     C.c_void_p]),
+    "gust_poisson_fill_ghosts": (C.c_int, [C.c_void_p, _P(Array4), _P(C.c_int32), C.c_void_p]),
+    "gust_poisson_fluxes": (C.c_int, [C.c_void_p, _P(Array4), _P(FaceCoeffsC), _P(C.c_int32), _P(Array4),
+                                      _P(Array4), _P(Array4), C.c_void_p]),
+    "gust_poisson_vcycle": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _P(FaceCoeffsC), _P(C.c_int32), C.c_int,
+                                      C.c_void_p]),
+    "gust_face_divergence": (C.c_int, [_P(GeometryC), _P(Array4), _P(Array4), _P(Array4), _P(Array4), _P(Array4),
+                                       _P(Array4), _P(Array4), _P(Array4), C.c_void_p]),
 }
 for _name, (_res, _args) in _sig.items():
     _f = getattr(lib, _name)

@@ -1,0 +1,1448 @@
+// poisson.hip -- the terrain-following pressure-correction solve for gfx950.
+//
+// Reference behaviour (GUST):
+//   Source/Poisson.H:119-203, Source/Poisson.cpp:395-731   PoissonSolver
+//   extern_erf/ERF_TerrainPoisson_3D_K.H                  the operator
+//   extern_erf/ERF_TerrainPoisson.cpp:74-264               apply, BCs, fluxes
+//   Source/PoissonCross_K.H                                split z coefficient
+//   Source/Poisson.cpp:733-764                             face_divergence
+//
+// Design (MI355X):
+//  * Fields are compact (nx*ny*nz, i fastest); the boundary conditions are
+//    applied when a stencil reads one cell outside the domain (periodic
+//    image, even or odd mirror), so no ghost-fill pass runs between the
+//    smoother's half sweeps. The values read are exactly the ghost values of
+//    fill_bc_ghosts, so FP64 operator applies equal the CPU build's bits.
+//  * Solver: restarted GMRES in FP64 (deterministic two-stage reductions,
+//    classical Gram-Schmidt with one re-orthogonalisation), right-
+//    preconditioned by one multigrid V-cycle. The multigrid coarsens in x and
+//    y only (columns are never split), re-discretises the same terrain
+//    operator on injected node heights with area-averaged coefficients, and
+//    smooths with red-black ("zebra") z-line relaxation: each column's exact
+//    tridiagonal part, taken from the operator itself by evaluating the
+//    stencil on values linear in the column (terrain_stencil.hpp), is solved
+//    by Thomas. The V-cycle runs in FP64 or, for the mixed-precision path,
+//    in FP32 while GMRES's Arnoldi vectors, the operator and the residuals
+//    stay FP64.
+//  * The AMReX MLMG/GMRES the reference calls are third party; only the
+//    discrete operator and the converged answer are the reference's, so the
+//    iteration counts are this solver's own.
+#include "common.hpp"
+#include "terrain_stencil.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <vector>
+
+namespace gust_amd {
+
+namespace {
+
+using stencil::Lin3;
+using stencil::OpCtx;
+
+constexpr int kThreads = 256;
+constexpr int kRedBlocks = 1024;
+
+// ---------------------------------------------------------------------------
+// Accessors.
+
+template <typename R>
+struct NodeAcc {   // node heights, one ghost node: index (i+1, j+1, k+1)
+    const R* p;
+    int64_t sj, sk;
+    __device__ __forceinline__ R operator()(int i, int j, int k) const
+    {
+        return p[(i + 1) + (j + 1) * sj + int64_t(k + 1) * sk];
+    }
+};
+
+template <typename R>
+struct GridAcc {   // compact array of valid points
+    const R* p;
+    int64_t sj, sk;
+    __device__ __forceinline__ R operator()(int i, int j, int k) const { return p[i + j * sj + int64_t(k) * sk]; }
+};
+
+// A cell field read with the boundary conditions one layer outside the
+// domain: periodic images, even (Neumann) or odd (Dirichlet) mirrors, x
+// then y then z (Source/Poisson.cpp:84-125).
+template <typename R>
+struct MirrorAcc {
+    const R* p;
+    int nx, ny, nz;
+    int64_t sj, sk;
+    int perx, pery;
+    R sxl, sxh, syl, syh, szl, szh;
+    __device__ __forceinline__ void resolve(int& i, int& j, int& k, R& s) const
+    {
+        s = R(1);
+        if (i < 0) {
+            if (perx) { i += nx; } else { i = 0; s = s * sxl; }
+        } else if (i >= nx) {
+            if (perx) { i -= nx; } else { i = nx - 1; s = s * sxh; }
+        }
+        if (j < 0) {
+            if (pery) { j += ny; } else { j = 0; s = s * syl; }
+        } else if (j >= ny) {
+            if (pery) { j -= ny; } else { j = ny - 1; s = s * syh; }
+        }
+        if (k < 0) { k = 0; s = s * szl; }
+        else if (k >= nz) { k = nz - 1; s = s * szh; }
+    }
+    __device__ __forceinline__ R operator()(int i, int j, int k) const
+    {
+        if (i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz) { return p[i + j * sj + int64_t(k) * sk]; }
+        R s;
+        resolve(i, j, k, s);
+        return s * p[i + j * sj + int64_t(k) * sk];
+    }
+};
+
+// The own column (ci, cj) seeded at levels ck-1, ck, ck+1 (mirrors resolved
+// as MirrorAcc does), every other point zero.
+template <typename R>
+struct SeedAcc {
+    MirrorAcc<R> m;
+    int ci, cj, ck;
+    __device__ __forceinline__ Lin3<R> operator()(int i, int j, int k) const
+    {
+        R s;
+        m.resolve(i, j, k, s);
+        Lin3<R> v;
+        if (i == ci && j == cj) {
+            const int d = k - ck;
+            if (d >= -1 && d <= 1) { v.c[d + 1] = s; }
+        }
+        return v;
+    }
+};
+
+template <typename R>
+using Ctx = OpCtx<R, NodeAcc<R>, GridAcc<R>>;
+
+// ---------------------------------------------------------------------------
+// Kernels.
+
+__device__ __forceinline__ void cell_of(int64_t t, int nx, int ny, int& i, int& j, int& k)
+{
+    i = int(t % nx);
+    const int64_t r = t / nx;
+    j = int(r % ny);
+    k = int(r / ny);
+}
+
+// y = A x (mode 0) or y = b - A x (mode 1), on every cell (color < 0) or on
+// the cells with (i + j) % 2 == color.
+template <typename R, int kMode>
+__global__ __launch_bounds__(kThreads) void apply_kernel(Ctx<R> c, MirrorAcc<R> x, const R* __restrict__ b,
+                                                          R* __restrict__ y, int color)
+{
+    const int nxh = color < 0 ? c.nx : (c.nx + 1) / 2;
+    const int64_t n = int64_t(nxh) * c.ny * c.nz;
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        int i, j, k;
+        cell_of(t, nxh, c.ny, i, j, k);
+        if (color >= 0) {
+            i = 2 * i + ((j + color) & 1);
+            if (i >= c.nx) { continue; }
+        }
+        const R v = stencil::apply_cell<R, R>(x, c, i, j, k);
+        const int64_t q = i + j * int64_t(c.nx) + int64_t(k) * c.nx * c.ny;
+        y[q] = kMode == 0 ? v : b[q] - v;
+    }
+}
+
+// The column part of the operator: (lower, diagonal, upper) per cell.
+template <typename R>
+__global__ __launch_bounds__(kThreads) void tridiag_kernel(Ctx<R> c, MirrorAcc<R> m, R* __restrict__ lo,
+                                                           R* __restrict__ dg, R* __restrict__ up)
+{
+    const int64_t n = int64_t(c.nx) * c.ny * c.nz;
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        int i, j, k;
+        cell_of(t, c.nx, c.ny, i, j, k);
+        SeedAcc<R> s{m, i, j, k};
+        const Lin3<R> v = stencil::apply_cell<Lin3<R>, R>(s, c, i, j, k);
+        lo[t] = v.c[0];
+        dg[t] = v.c[1];
+        up[t] = v.c[2];
+    }
+}
+
+// One colour of a zebra sweep, after apply_kernel<1> put b - A x on that
+// colour's cells into r: solve the column tridiagonal T d = r and x += d.
+// r is overwritten with the Thomas e', cs holds c'.
+template <typename R>
+__global__ __launch_bounds__(kThreads) void line_solve_kernel(int nx, int ny, int nz, const R* __restrict__ lo,
+                                                              const R* __restrict__ dg, const R* __restrict__ up,
+                                                              R* __restrict__ r, R* __restrict__ cs,
+                                                              R* __restrict__ x, int color)
+{
+    const int nxh = (nx + 1) / 2;
+    const int64_t ncol = int64_t(nxh) * ny;
+    const int64_t sk = int64_t(nx) * ny;
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < ncol; t += int64_t(gridDim.x) * blockDim.x) {
+        const int j = int(t / nxh);
+        const int i = 2 * int(t % nxh) + ((j + color) & 1);
+        if (i >= nx) { continue; }
+        const int64_t q0 = i + int64_t(j) * nx;
+        R cp = R(0), ep = R(0);
+        for (int k = 0; k < nz; ++k) {
+            const int64_t q = q0 + k * sk;
+            const R a = k > 0 ? lo[q] : R(0);
+            const R m = dg[q] - a * cp;
+            const R cc = up[q] / m;
+            const R ee = (r[q] - a * ep) / m;
+            cs[q] = cc;
+            r[q] = ee;
+            cp = cc;
+            ep = ee;
+        }
+        R d = ep;
+        x[q0 + (nz - 1) * sk] += d;
+        for (int k = nz - 2; k >= 0; --k) {
+            const int64_t q = q0 + k * sk;
+            d = r[q] - cs[q] * d;
+            x[q] += d;
+        }
+    }
+}
+
+// b_c = sum J_f r_f / sum J_f over the fine cells of each coarse cell.
+template <typename R>
+__global__ __launch_bounds__(kThreads) void restrict_kernel(int nxc, int nyc, int nz, int rx, int ry,
+                                                            const R* __restrict__ rf, const R* __restrict__ Jf,
+                                                            R* __restrict__ bc)
+{
+    const int nxf = nxc * rx, nyf = nyc * ry;
+    const int64_t n = int64_t(nxc) * nyc * nz;
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        int I, Jc, k;
+        cell_of(t, nxc, nyc, I, Jc, k);
+        R num = R(0), den = R(0);
+        for (int b = 0; b < ry; ++b)
+            for (int a = 0; a < rx; ++a) {
+                const int64_t q = (I * rx + a) + int64_t(Jc * ry + b) * nxf + int64_t(k) * nxf * nyf;
+                num += Jf[q] * rf[q];
+                den += Jf[q];
+            }
+        bc[t] = num / den;
+    }
+}
+
+// x_f += bilinear (in the coarsened directions) interpolation of x_c.
+template <typename R>
+__global__ __launch_bounds__(kThreads) void prolong_kernel(int nxf, int nyf, int nz, int rx, int ry, MirrorAcc<R> xc,
+                                                           R* __restrict__ xf)
+{
+    const int64_t n = int64_t(nxf) * nyf * nz;
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        int i, j, k;
+        cell_of(t, nxf, nyf, i, j, k);
+        const int I = i / rx, Jc = j / ry;
+        const int di = rx == 2 ? ((i & 1) ? 1 : -1) : 0;
+        const int dj = ry == 2 ? ((j & 1) ? 1 : -1) : 0;
+        const R wx = rx == 2 ? R(0.75) : R(1), wy = ry == 2 ? R(0.75) : R(1);
+        R v = wx * wy * xc(I, Jc, k);
+        if (di) { v += (R(1) - wx) * wy * xc(I + di, Jc, k); }
+        if (dj) { v += wx * (R(1) - wy) * xc(I, Jc + dj, k); }
+        if (di && dj) { v += (R(1) - wx) * (R(1) - wy) * xc(I + di, Jc + dj, k); }
+        xf[t] += v;
+    }
+}
+
+template <typename A, typename B>
+__global__ void convert_kernel(const A* __restrict__ a, B* __restrict__ b, int64_t n)
+{
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        b[t] = B(a[t]);
+    }
+}
+
+template <typename R>
+__global__ void fill_kernel(R* __restrict__ a, R v, int64_t n)
+{
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        a[t] = v;
+    }
+}
+
+// a = b * c (elementwise), the premultiplied areas.
+template <typename R>
+__global__ void mul_kernel(const R* __restrict__ b, const R* __restrict__ c, R* __restrict__ a, int64_t n)
+{
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        a[t] = c ? b[t] * c[t] : b[t];
+    }
+}
+
+// Coarse face values: the mean of the fine faces a coarse face covers.
+template <typename R>
+__global__ void coarsen_faces_kernel(int dir, int nxc, int nyc, int nz, int rx, int ry, const R* __restrict__ f,
+                                     R* __restrict__ c)
+{
+    // face array extents (coarse): x-faces (nxc+1, nyc, nz), y (nxc, nyc+1, nz), z (nxc, nyc, nz+1)
+    const int ex = nxc + (dir == 0), ey = nyc + (dir == 1), ez = nz + (dir == 2);
+    const int fx = nxc * rx + (dir == 0), fy = nyc * ry + (dir == 1);
+    const int64_t n = int64_t(ex) * ey * ez;
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        int I, Jc, k;
+        cell_of(t, ex, ey, I, Jc, k);
+        const int na = dir == 0 ? 1 : rx, nb = dir == 1 ? 1 : ry;
+        R s = R(0);
+        for (int b = 0; b < nb; ++b)
+            for (int a = 0; a < na; ++a) {
+                const int i = dir == 0 ? I * rx : I * rx + a;
+                const int j = dir == 1 ? Jc * ry : Jc * ry + b;
+                s += f[i + int64_t(j) * fx + int64_t(k) * fx * fy];
+            }
+        c[t] = s / R(na * nb);
+    }
+}
+
+// Deterministic reductions: fixed grid, per-block partials, then one block
+// sums the partials in a fixed tree.
+__device__ __forceinline__ double block_sum(double v)
+{
+    __shared__ double sh[kThreads / 64];
+    for (int off = 32; off >= 1; off >>= 1) { v += __shfl_xor(v, off, 64); }
+    if ((threadIdx.x & 63) == 0) { sh[threadIdx.x >> 6] = v; }
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < kThreads / 64; ++w) { s += sh[w]; }
+    }
+    return s;
+}
+
+// partial[y * gridDim.x + x] = sum over block x of a_y[p] * b[p] (a_y = a +
+// y * lda); weight (may be null) multiplies each term.
+__global__ __launch_bounds__(kThreads) void dot_partial_kernel(const double* __restrict__ a, int64_t lda,
+                                                               const double* __restrict__ b, int64_t n,
+                                                               double* __restrict__ partial)
+{
+    const double* ay = a + blockIdx.y * lda;
+    double v = 0;
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        v += ay[t] * b[t];
+    }
+    const double s = block_sum(v);
+    if (threadIdx.x == 0) { partial[blockIdx.y * gridDim.x + blockIdx.x] = s; }
+}
+
+__global__ __launch_bounds__(kThreads) void sum_partial_kernel(const double* __restrict__ partial, int nblk,
+                                                               double* __restrict__ out)
+{
+    double v = 0;
+    for (int t = threadIdx.x; t < nblk; t += blockDim.x) { v += partial[blockIdx.x * nblk + t]; }
+    const double s = block_sum(v);
+    if (threadIdx.x == 0) { out[blockIdx.x] = s; }
+}
+
+// w -= sum_i h[i] V_i (nv vectors, stride ldv).
+__global__ __launch_bounds__(kThreads) void multi_axpy_kernel(double* __restrict__ w, const double* __restrict__ V,
+                                                              int64_t ldv, const double* __restrict__ h, int nv,
+                                                              double sign, int64_t n)
+{
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        double s = 0;
+        for (int i = 0; i < nv; ++i) { s += h[i] * V[i * ldv + t]; }
+        w[t] += sign * s;
+    }
+}
+
+__global__ void scale_copy_kernel(const double* __restrict__ a, double s, double* __restrict__ b, int64_t n)
+{
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        b[t] = a[t] * s;
+    }
+}
+
+__global__ void add_scalar_kernel(double* __restrict__ a, const double* __restrict__ s, double sign, int64_t n)
+{
+    const double v = sign * s[0];
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        a[t] += v;
+    }
+}
+
+template <typename R>
+__global__ void add_scalar_kernel_r(R* __restrict__ a, const double* __restrict__ num, const double* __restrict__ den,
+                                    int64_t n)
+{
+    const R v = R(num[0] / den[0]);
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        a[t] -= v;
+    }
+}
+
+// Copy between a compact field and a view's valid region (dir 0: view ->
+// compact, 1: compact -> view).
+template <typename R>
+__global__ void view_copy_kernel(R* __restrict__ view, int64_t vj, int64_t vk, R* __restrict__ compact, int nx,
+                                 int ny, int nz, int dir)
+{
+    const int64_t n = int64_t(nx) * ny * nz;
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        int i, j, k;
+        cell_of(t, nx, ny, i, j, k);
+        R* v = view + i + j * vj + int64_t(k) * vk;
+        if (dir == 0) { compact[t] = *v; } else { *v = compact[t]; }
+    }
+}
+
+// Fill the one-layer ghost shell of a view from its valid region with the
+// boundary conditions (Source/Poisson.cpp:84-125).
+template <typename R>
+__global__ void fill_view_ghosts_kernel(R* __restrict__ view, int64_t vj, int64_t vk, MirrorAcc<R> m)
+{
+    const int gx = m.nx + 2, gy = m.ny + 2, gz = m.nz + 2;
+    const int64_t n = int64_t(gx) * gy * gz;
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        int i, j, k;
+        cell_of(t, gx, gy, i, j, k);
+        i -= 1;
+        j -= 1;
+        k -= 1;
+        if (i >= 0 && i < m.nx && j >= 0 && j < m.ny && k >= 0 && k < m.nz) { continue; }
+        MirrorAcc<R> vm = m;
+        vm.p = view;
+        vm.sj = vj;
+        vm.sk = vk;
+        view[i + j * vj + int64_t(k) * vk] = vm(i, j, k);
+    }
+}
+
+// Fluxes F = -gradient * beta on the valid faces, zero on masked faces,
+// minus the z_cross part (extern_erf/ERF_TerrainPoisson.cpp:212-264,
+// Source/Poisson.cpp:192-227, 256-273).
+template <typename R>
+__global__ void flux_kernel(int dir, Ctx<R> c, MirrorAcc<R> x, const R* __restrict__ beta, int64_t bj, int64_t bk,
+                            R* __restrict__ f, int64_t fj, int64_t fk)
+{
+    const int ex = c.nx + (dir == 0), ey = c.ny + (dir == 1), ez = c.nz + (dir == 2);
+    const int64_t n = int64_t(ex) * ey * ez;
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        int i, j, k;
+        cell_of(t, ex, ey, i, j, k);
+        R v;
+        if (dir == 0) {
+            const bool zero = ((c.zf & stencil::kXlo) && i == 0) || ((c.zf & stencil::kXhi) && i == c.nx);
+            v = zero ? R(0) : -stencil::grad_x<R, R>(x, c.zp, i, j, k, c.dxinv);
+        } else if (dir == 1) {
+            const bool zero = ((c.zf & stencil::kYlo) && j == 0) || ((c.zf & stencil::kYhi) && j == c.ny);
+            v = zero ? R(0) : -stencil::grad_y<R, R>(x, c.zp, i, j, k, c.dyinv);
+        } else {
+            if ((c.zf & stencil::kZhi) && k == c.nz) {
+                v = R(0);
+            } else if (c.dir_ground && k == 0) {
+                v = -stencil::grad_z_dir<R, R>(x, c.zp, i, j, k, c.dxinv, c.dyinv);
+            } else {
+                v = -stencil::grad_z<R, R>(x, c.zp, i, j, k, c.dxinv, c.dyinv);
+                if (k == 0) { v = -R(0); }
+            }
+        }
+        if (beta) { v *= beta[i + j * bj + int64_t(k) * bk]; }
+        if (dir == 2 && c.dcross.p && !(k == 0 || (k == c.nz && !c.top_open))) {
+            v -= c.dcross(i, j, k) * stencil::z_face_cross<R, R>(x, c.zp, i, j, k, c.dxinv, c.dyinv);
+        }
+        f[i + j * fj + int64_t(k) * fk] = v;
+    }
+}
+
+// face_divergence (Source/Poisson.cpp:733-764) on views.
+__global__ void face_div_kernel(int nx, int ny, int nz, double dxinv, double dyinv, double dzinv, const double* ax,
+                                int64_t axj, int64_t axk, const double* ay, int64_t ayj, int64_t ayk, const double* az,
+                                int64_t azj, int64_t azk, const double* fx, int64_t fxj, int64_t fxk, const double* fy,
+                                int64_t fyj, int64_t fyk, const double* fz, int64_t fzj, int64_t fzk, const double* J,
+                                int64_t Jj, int64_t Jk, double* d, int64_t dj, int64_t dk)
+{
+    const int64_t n = int64_t(nx) * ny * nz;
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        int i, j, k;
+        cell_of(t, nx, ny, i, j, k);
+        auto A = [&](const double* p, int64_t sj, int64_t sk, int a, int b, int c) { return p[a + b * sj + c * sk]; };
+        d[i + j * dj + k * dk] =
+            ((A(ax, axj, axk, i + 1, j, k) * A(fx, fxj, fxk, i + 1, j, k) - A(ax, axj, axk, i, j, k) * A(fx, fxj, fxk, i, j, k)) * dxinv
+             + (A(ay, ayj, ayk, i, j + 1, k) * A(fy, fyj, fyk, i, j + 1, k) - A(ay, ayj, ayk, i, j, k) * A(fy, fyj, fyk, i, j, k)) * dyinv
+             + (A(az, azj, azk, i, j, k + 1) * A(fz, fzj, fzk, i, j, k + 1) - A(az, azj, azk, i, j, k) * A(fz, fzj, fzk, i, j, k)) * dzinv)
+            / A(J, Jj, Jk, i, j, k);
+    }
+}
+
+// min, max and NaN flag of a view's valid region (coefficient checks).
+__global__ void minmax_kernel(const double* __restrict__ p, int64_t sj, int64_t sk, int nx, int ny, int nz,
+                              unsigned long long* __restrict__ flags)
+{
+    const int64_t n = int64_t(nx) * ny * nz;
+    bool bad = false;
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        int i, j, k;
+        cell_of(t, nx, ny, i, j, k);
+        const double v = p[i + j * sj + int64_t(k) * sk];
+        if (!(v > 0) || !(v < std::numeric_limits<double>::infinity())) { bad = true; }
+    }
+    if (bad) { atomicOr(flags, 1ull); }
+}
+
+inline int blocks_for(int64_t n, int cap = 8192)
+{
+    return int(std::max<int64_t>(1, std::min<int64_t>((n + kThreads - 1) / kThreads, cap)));
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { if (p) { (void)hipFree(p); } }
+    void ensure(size_t b)
+    {
+        if (bytes >= b) { return; }
+        if (p) { GUST_HIP(hipFree(p)); p = nullptr; }
+        GUST_HIP(hipMalloc(&p, std::max<size_t>(b, 8)));
+        bytes = std::max<size_t>(b, 8);
+    }
+};
+
+template <typename T>
+T* ptr(const DevBuf& b)
+{
+    return static_cast<T*>(b.p);
+}
+
+enum { PF_PERIODIC = 0, PF_NEUMANN = 1, PF_DIRICHLET = 2 };
+
+struct BcInfo {
+    int face[6];
+    int zf = 0, dir_ground = 0, top_open = 0, singular = 1;
+};
+
+BcInfo bc_info(const int* bc)
+{
+    BcInfo b;
+    for (int f = 0; f < 6; ++f) { b.face[f] = bc[f]; }
+    if (bc[0] == PF_NEUMANN) { b.zf |= stencil::kXlo; }
+    if (bc[3] == PF_NEUMANN) { b.zf |= stencil::kXhi; }
+    if (bc[1] == PF_NEUMANN) { b.zf |= stencil::kYlo; }
+    if (bc[4] == PF_NEUMANN) { b.zf |= stencil::kYhi; }
+    if (bc[5] == PF_NEUMANN) { b.zf |= stencil::kZhi; }
+    b.dir_ground = bc[2] == PF_DIRICHLET;
+    b.top_open = bc[5] != PF_NEUMANN;
+    for (int f = 0; f < 6; ++f) { if (bc[f] == PF_DIRICHLET) { b.singular = 0; } }
+    return b;
+}
+
+template <typename R>
+MirrorAcc<R> mirror(const R* p, int nx, int ny, int nz, const int* per, const BcInfo& b)
+{
+    auto sg = [&](int f) { return b.face[f] == PF_DIRICHLET ? R(-1) : R(1); };
+    MirrorAcc<R> m;
+    m.p = p;
+    m.nx = nx; m.ny = ny; m.nz = nz;
+    m.sj = nx; m.sk = int64_t(nx) * ny;
+    m.perx = per[0]; m.pery = per[1];
+    m.sxl = sg(0); m.sxh = sg(3); m.syl = sg(1); m.syh = sg(4);
+    m.szl = b.dir_ground ? R(-1) : R(1);   // the ground is even unless Dirichlet
+    m.szh = sg(5);
+    return m;
+}
+
+// One multigrid level (or the FP64 operator level).
+template <typename R>
+struct Level {
+    int nx = 0, ny = 0, nz = 0, rx = 1, ry = 1;   // rx/ry: this level's coarsening factor from the finer one
+    double dx[3] = {1, 1, 1};
+    DevBuf zp, J, ax, ay, az, dcross, azm, lo, dg, up, x, b, r, cs;
+    bool has_cross = false;
+    int64_t n() const { return int64_t(nx) * ny * nz; }
+    Ctx<R> ctx(const BcInfo& bc) const
+    {
+        Ctx<R> c;
+        c.zp = NodeAcc<R>{ptr<R>(zp), nx + 3, int64_t(nx + 3) * (ny + 3)};
+        c.ax = GridAcc<R>{ptr<R>(ax), nx + 1, int64_t(nx + 1) * ny};
+        c.ay = GridAcc<R>{ptr<R>(ay), nx, int64_t(nx) * (ny + 1)};
+        c.az = GridAcc<R>{ptr<R>(az), nx, int64_t(nx) * ny};
+        c.J = GridAcc<R>{ptr<R>(J), nx, int64_t(nx) * ny};
+        c.dcross = GridAcc<R>{has_cross ? ptr<R>(dcross) : nullptr, nx, int64_t(nx) * ny};
+        c.az_mesh = GridAcc<R>{has_cross ? ptr<R>(azm) : nullptr, nx, int64_t(nx) * ny};
+        c.dxinv = R(1.0 / dx[0]);
+        c.dyinv = R(1.0 / dx[1]);
+        c.dzinv = R(1.0 / dx[2]);
+        c.nx = nx; c.ny = ny; c.nz = nz;
+        c.zf = bc.zf;
+        c.dir_ground = bc.dir_ground;
+        c.top_open = bc.top_open;
+        return c;
+    }
+};
+
+}   // namespace
+
+// The solver object behind gust_poisson.
+struct PoissonImpl {
+    gust_geometry geom{};
+    gust_poisson_options opt{};
+    int nx = 0, ny = 0, nz = 0;
+    // The FP64 operator level (mesh copy, current premultiplied areas).
+    Level<double> L0;
+    // Mesh areas (unmultiplied) on level 0, for a beta and face_divergence.
+    DevBuf ax_mesh, ay_mesh, az_mesh;
+    // Host copies of the node heights (for the coarse meshes).
+    std::vector<double> zp_host;
+    std::vector<std::unique_ptr<Level<double>>> mg64;
+    std::vector<std::unique_ptr<Level<float>>> mg32;
+    DevBuf krylov_v, krylov_z, w, partial, dots, hcoef, scal;
+    std::vector<double> hbuf;
+    int per[2] = {0, 0};
+
+    int64_t n() const { return int64_t(nx) * ny * nz; }
+};
+
+namespace {
+
+// Setup-time upload; synchronous, so h may be a temporary.
+template <typename T>
+void upload(DevBuf& d, const std::vector<T>& h, hipStream_t st)
+{
+    d.ensure(h.size() * sizeof(T));
+    GUST_HIP(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    GUST_HIP(hipStreamSynchronize(st));
+}
+
+// Read a view's region [lo, hi] (inclusive, within the view) to a compact
+// host vector (setup only).
+std::vector<double> view_to_host(const gust_array4& v, const int lo[3], const int hi[3], hipStream_t st)
+{
+    const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
+    std::vector<double> out(static_cast<size_t>(ex) * ey * ez);
+    const int64_t span = (int64_t(v.end[0] - v.begin[0])) + int64_t(v.end[1] - v.begin[1] - 1) * v.jstride +
+                         int64_t(v.end[2] - v.begin[2] - 1) * v.kstride;
+    std::vector<double> whole(static_cast<size_t>(span));
+    GUST_HIP(hipMemcpyAsync(whole.data(), v.p, size_t(span) * 8, hipMemcpyDeviceToHost, st));
+    GUST_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < ez; ++k)
+        for (int j = 0; j < ey; ++j)
+            for (int i = 0; i < ex; ++i) {
+                out[i + size_t(j) * ex + size_t(k) * ex * ey] =
+                    whole[size_t(offset_of(v, lo[0] + i, lo[1] + j, lo[2] + k))];
+            }
+    return out;
+}
+
+template <typename R>
+void build_coarse_meshes(PoissonImpl& P, std::vector<std::unique_ptr<Level<R>>>& mg, hipStream_t st)
+{
+    mg.clear();
+    // Level 0 of the hierarchy: the fine mesh in precision R.
+    int nx = P.nx, ny = P.ny, nz = P.nz;
+    std::vector<double> zp = P.zp_host;   // (nx+3)(ny+3)(nz+3)
+    double dx = P.geom.dx[0], dy = P.geom.dx[1];
+    const double dz = P.geom.dx[2];
+    int rx = 1, ry = 1;
+    for (int lev = 0;; ++lev) {
+        auto L = std::make_unique<Level<R>>();
+        L->nx = nx; L->ny = ny; L->nz = nz;
+        L->rx = rx; L->ry = ry;
+        L->dx[0] = dx; L->dx[1] = dy; L->dx[2] = dz;
+        std::vector<R> zr(zp.begin(), zp.end());
+        upload(L->zp, zr, st);
+        // J from the node heights (extern_erf/ERF_TerrainMetrics.cpp:679-715)
+        const double dzInv = 1.0 / dz;
+        const int sj = nx + 3;
+        const int64_t sk = int64_t(nx + 3) * (ny + 3);
+        auto Z = [&](int i, int j, int k) { return zp[size_t((i + 1) + (j + 1) * sj + (k + 1) * sk)]; };
+        std::vector<R> J(size_t(nx) * ny * nz);
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    J[i + size_t(j) * nx + size_t(k) * nx * ny] =
+                        R(0.25 * dzInv * (Z(i, j, k + 1) + Z(i + 1, j, k + 1) + Z(i, j + 1, k + 1) + Z(i + 1, j + 1, k + 1)
+                                          - Z(i, j, k) - Z(i + 1, j, k) - Z(i, j + 1, k) - Z(i + 1, j + 1, k)));
+                }
+        upload(L->J, J, st);
+        const int64_t n = L->n();
+        L->ax.ensure(size_t(nx + 1) * ny * nz * sizeof(R));
+        L->ay.ensure(size_t(nx) * (ny + 1) * nz * sizeof(R));
+        L->az.ensure(size_t(nx) * ny * (nz + 1) * sizeof(R));
+        for (DevBuf* b : {&L->lo, &L->dg, &L->up, &L->x, &L->b, &L->r, &L->cs}) { b->ensure(size_t(n) * sizeof(R)); }
+        mg.push_back(std::move(L));
+        // Coarsen x and y by two where the size is even and at least 4.
+        const int cx = (nx % 2 == 0 && nx >= 4) ? 2 : 1;
+        const int cy = (ny % 2 == 0 && ny >= 4) ? 2 : 1;
+        if ((cx == 1 && cy == 1) || lev >= 12 || int64_t(nx) * ny <= 16) { break; }
+        const int ncx = nx / cx, ncy = ny / cy;
+        std::vector<double> zc(size_t(ncx + 3) * (ncy + 3) * (nz + 3));
+        const int csj = ncx + 3;
+        const int64_t csk = int64_t(ncx + 3) * (ncy + 3);
+        auto C = [&](int i, int j, int k) -> double& { return zc[size_t((i + 1) + (j + 1) * csj + (k + 1) * csk)]; };
+        for (int k = 0; k <= nz; ++k)
+            for (int j = -1; j <= ncy + 1; ++j)
+                for (int i = -1; i <= ncx + 1; ++i) {
+                    int ii = i, jj = j;
+                    if (ii < 0) { ii = P.per[0] ? ii + ncx : 0; }
+                    if (ii > ncx) { ii = P.per[0] ? ii - ncx : ncx; }
+                    if (jj < 0) { jj = P.per[1] ? jj + ncy : 0; }
+                    if (jj > ncy) { jj = P.per[1] ? jj - ncy : ncy; }
+                    C(i, j, k) = Z(ii * cx, jj * cy, k);
+                }
+        for (int j = -1; j <= ncy + 1; ++j)
+            for (int i = -1; i <= ncx + 1; ++i) {
+                C(i, j, -1) = 2.0 * C(i, j, 0) - C(i, j, 1);
+                C(i, j, nz + 1) = C(i, j, nz) + 1 * (C(i, j, nz) - C(i, j, nz - 1));
+            }
+        zp.swap(zc);
+        nx = ncx; ny = ncy;
+        dx *= cx; dy *= cy;
+        rx = cx; ry = cy;
+    }
+}
+
+template <typename R>
+void launch_apply(const Level<R>& L, const BcInfo& bc, const int* per, const R* x, const R* b, R* y, int color,
+                  hipStream_t st)
+{
+    const Ctx<R> c = L.ctx(bc);
+    const MirrorAcc<R> m = mirror<R>(x, L.nx, L.ny, L.nz, per, bc);
+    const int nxh = color < 0 ? L.nx : (L.nx + 1) / 2;
+    const int64_t n = int64_t(nxh) * L.ny * L.nz;
+    if (b) { apply_kernel<R, 1><<<blocks_for(n), kThreads, 0, st>>>(c, m, b, y, color); }
+    else { apply_kernel<R, 0><<<blocks_for(n), kThreads, 0, st>>>(c, m, nullptr, y, color); }
+    GUST_HIP(hipGetLastError());
+}
+
+template <typename R>
+void zebra(Level<R>& L, const BcInfo& bc, const int* per, hipStream_t st)
+{
+    for (int color = 0; color < 2; ++color) {
+        launch_apply<R>(L, bc, per, ptr<R>(L.x), ptr<R>(L.b), ptr<R>(L.r), color, st);
+        const int64_t ncol = int64_t((L.nx + 1) / 2) * L.ny;
+        line_solve_kernel<R><<<blocks_for(ncol), kThreads, 0, st>>>(L.nx, L.ny, L.nz, ptr<R>(L.lo), ptr<R>(L.dg),
+                                                                   ptr<R>(L.up), ptr<R>(L.r), ptr<R>(L.cs),
+                                                                   ptr<R>(L.x), color);
+        GUST_HIP(hipGetLastError());
+    }
+}
+
+// J-weighted mean removal of a level's x (the coarsest solve of a singular
+// problem): x -= sum J x / sum J.
+template <typename R>
+void remove_mean(PoissonImpl& P, Level<R>& L, hipStream_t st);
+
+template <>
+void remove_mean<double>(PoissonImpl& P, Level<double>& L, hipStream_t st)
+{
+    const int64_t n = L.n();
+    double* d = ptr<double>(P.dots);
+    dot_partial_kernel<<<dim3(kRedBlocks, 1), kThreads, 0, st>>>(ptr<double>(L.J), 0, ptr<double>(L.x), n,
+                                                                 ptr<double>(P.partial));
+    sum_partial_kernel<<<1, kThreads, 0, st>>>(ptr<double>(P.partial), kRedBlocks, d);
+    // sum J: dot(J, ones), with L.r (free at the coarsest level) as the ones
+    fill_kernel<double><<<blocks_for(n), kThreads, 0, st>>>(ptr<double>(L.r), 1.0, n);
+    dot_partial_kernel<<<dim3(kRedBlocks, 1), kThreads, 0, st>>>(ptr<double>(L.J), 0, ptr<double>(L.r), n,
+                                                                 ptr<double>(P.partial));
+    sum_partial_kernel<<<1, kThreads, 0, st>>>(ptr<double>(P.partial), kRedBlocks, d + 1);
+    add_scalar_kernel_r<double><<<blocks_for(n), kThreads, 0, st>>>(ptr<double>(L.x), d, d + 1, n);
+    GUST_HIP(hipGetLastError());
+}
+
+template <>
+void remove_mean<float>(PoissonImpl& P, Level<float>& L, hipStream_t st)
+{
+    // promote to double for the reduction
+    const int64_t n = L.n();
+    double* tmp = ptr<double>(P.w);   // n doubles of scratch (w is >= level 0 size)
+    double* tj = tmp + n;
+    convert_kernel<float, double><<<blocks_for(n), kThreads, 0, st>>>(ptr<float>(L.x), tmp, n);
+    convert_kernel<float, double><<<blocks_for(n), kThreads, 0, st>>>(ptr<float>(L.J), tj, n);
+    double* d = ptr<double>(P.dots);
+    dot_partial_kernel<<<dim3(kRedBlocks, 1), kThreads, 0, st>>>(tj, 0, tmp, n, ptr<double>(P.partial));
+    sum_partial_kernel<<<1, kThreads, 0, st>>>(ptr<double>(P.partial), kRedBlocks, d);
+    fill_kernel<double><<<blocks_for(n), kThreads, 0, st>>>(tmp, 1.0, n);
+    dot_partial_kernel<<<dim3(kRedBlocks, 1), kThreads, 0, st>>>(tj, 0, tmp, n, ptr<double>(P.partial));
+    sum_partial_kernel<<<1, kThreads, 0, st>>>(ptr<double>(P.partial), kRedBlocks, d + 1);
+    add_scalar_kernel_r<float><<<blocks_for(n), kThreads, 0, st>>>(ptr<float>(L.x), d, d + 1, n);
+    GUST_HIP(hipGetLastError());
+}
+
+template <typename R>
+void vcycle(PoissonImpl& P, std::vector<std::unique_ptr<Level<R>>>& mg, size_t l, const BcInfo& bc,
+            hipStream_t st)
+{
+    Level<R>& L = *mg[l];
+    const int64_t n = L.n();
+    fill_kernel<R><<<blocks_for(n), kThreads, 0, st>>>(ptr<R>(L.x), R(0), n);
+    if (l + 1 == mg.size()) {
+        for (int s = 0; s < P.opt.coarse_sweeps; ++s) { zebra<R>(L, bc, P.per, st); }
+        if (bc.singular) { remove_mean<R>(P, L, st); }
+        return;
+    }
+    for (int s = 0; s < P.opt.pre_sweeps; ++s) { zebra<R>(L, bc, P.per, st); }
+    launch_apply<R>(L, bc, P.per, ptr<R>(L.x), ptr<R>(L.b), ptr<R>(L.r), -1, st);
+    Level<R>& C = *mg[l + 1];
+    restrict_kernel<R><<<blocks_for(C.n()), kThreads, 0, st>>>(C.nx, C.ny, C.nz, C.rx, C.ry, ptr<R>(L.r),
+                                                               ptr<R>(L.J), ptr<R>(C.b));
+    GUST_HIP(hipGetLastError());
+    vcycle<R>(P, mg, l + 1, bc, st);
+    const MirrorAcc<R> mc = mirror<R>(ptr<R>(C.x), C.nx, C.ny, C.nz, P.per, bc);
+    prolong_kernel<R><<<blocks_for(n), kThreads, 0, st>>>(L.nx, L.ny, L.nz, C.rx, C.ry, mc, ptr<R>(L.x));
+    GUST_HIP(hipGetLastError());
+    for (int s = 0; s < P.opt.post_sweeps; ++s) { zebra<R>(L, bc, P.per, st); }
+}
+
+// Premultiplied areas and column tridiagonals on every level of a
+// hierarchy, from level 0's FP64 areas.
+template <typename R>
+void setup_hierarchy(PoissonImpl& P, std::vector<std::unique_ptr<Level<R>>>& mg, const BcInfo& bc, hipStream_t st)
+{
+    for (size_t l = 0; l < mg.size(); ++l) {
+        Level<R>& L = *mg[l];
+        if (l == 0) {
+            const int64_t nxf = int64_t(L.nx + 1) * L.ny * L.nz, nyf = int64_t(L.nx) * (L.ny + 1) * L.nz,
+                          nzf = int64_t(L.nx) * L.ny * (L.nz + 1);
+            convert_kernel<double, R><<<blocks_for(nxf), kThreads, 0, st>>>(ptr<double>(P.L0.ax), ptr<R>(L.ax), nxf);
+            convert_kernel<double, R><<<blocks_for(nyf), kThreads, 0, st>>>(ptr<double>(P.L0.ay), ptr<R>(L.ay), nyf);
+            convert_kernel<double, R><<<blocks_for(nzf), kThreads, 0, st>>>(ptr<double>(P.L0.az), ptr<R>(L.az), nzf);
+            L.has_cross = P.L0.has_cross;
+            if (L.has_cross) {
+                L.dcross.ensure(size_t(nzf) * sizeof(R));
+                L.azm.ensure(size_t(nzf) * sizeof(R));
+                convert_kernel<double, R><<<blocks_for(nzf), kThreads, 0, st>>>(ptr<double>(P.L0.dcross),
+                                                                                ptr<R>(L.dcross), nzf);
+                convert_kernel<double, R><<<blocks_for(nzf), kThreads, 0, st>>>(ptr<double>(P.L0.azm),
+                                                                                ptr<R>(L.azm), nzf);
+            }
+        } else {
+            Level<R>& F = *mg[l - 1];
+            for (int dir = 0; dir < 3; ++dir) {
+                const int64_t ne = int64_t(L.nx + (dir == 0)) * (L.ny + (dir == 1)) * (L.nz + (dir == 2));
+                const R* f = dir == 0 ? ptr<R>(F.ax) : dir == 1 ? ptr<R>(F.ay) : ptr<R>(F.az);
+                R* c = dir == 0 ? ptr<R>(L.ax) : dir == 1 ? ptr<R>(L.ay) : ptr<R>(L.az);
+                coarsen_faces_kernel<R><<<blocks_for(ne), kThreads, 0, st>>>(dir, L.nx, L.ny, L.nz, L.rx, L.ry, f, c);
+            }
+            L.has_cross = F.has_cross;
+            if (L.has_cross) {
+                const int64_t nzf = int64_t(L.nx) * L.ny * (L.nz + 1);
+                L.dcross.ensure(size_t(nzf) * sizeof(R));
+                L.azm.ensure(size_t(nzf) * sizeof(R));
+                coarsen_faces_kernel<R><<<blocks_for(nzf), kThreads, 0, st>>>(2, L.nx, L.ny, L.nz, L.rx, L.ry,
+                                                                              ptr<R>(F.dcross), ptr<R>(L.dcross));
+                coarsen_faces_kernel<R><<<blocks_for(nzf), kThreads, 0, st>>>(2, L.nx, L.ny, L.nz, L.rx, L.ry,
+                                                                              ptr<R>(F.azm), ptr<R>(L.azm));
+            }
+        }
+        GUST_HIP(hipGetLastError());
+        const Ctx<R> c = L.ctx(bc);
+        const MirrorAcc<R> m = mirror<R>(nullptr, L.nx, L.ny, L.nz, P.per, bc);
+        tridiag_kernel<R><<<blocks_for(L.n()), kThreads, 0, st>>>(c, m, ptr<R>(L.lo), ptr<R>(L.dg), ptr<R>(L.up));
+        GUST_HIP(hipGetLastError());
+    }
+}
+
+// z = M^{-1} v (one V-cycle) in the configured precision.
+void precondition(PoissonImpl& P, const double* v, double* z, const BcInfo& bc, hipStream_t st)
+{
+    const int64_t n = P.n();
+    if (P.opt.precision == 1) {
+        Level<float>& L = *P.mg32[0];
+        convert_kernel<double, float><<<blocks_for(n), kThreads, 0, st>>>(v, ptr<float>(L.b), n);
+        vcycle<float>(P, P.mg32, 0, bc, st);
+        convert_kernel<float, double><<<blocks_for(n), kThreads, 0, st>>>(ptr<float>(L.x), z, n);
+    } else {
+        Level<double>& L = *P.mg64[0];
+        GUST_HIP(hipMemcpyAsync(L.b.p, v, size_t(n) * 8, hipMemcpyDeviceToDevice, st));
+        vcycle<double>(P, P.mg64, 0, bc, st);
+        GUST_HIP(hipMemcpyAsync(z, L.x.p, size_t(n) * 8, hipMemcpyDeviceToDevice, st));
+    }
+    GUST_HIP(hipGetLastError());
+}
+
+// dots[i] = <V_i, w> for i < nv; copied to host.
+void multi_dot(PoissonImpl& P, const double* V, int64_t ldv, int nv, const double* w, double* host, hipStream_t st)
+{
+    const int64_t n = P.n();
+    dot_partial_kernel<<<dim3(kRedBlocks, nv), kThreads, 0, st>>>(V, ldv, w, n, ptr<double>(P.partial));
+    sum_partial_kernel<<<nv, kThreads, 0, st>>>(ptr<double>(P.partial), kRedBlocks, ptr<double>(P.dots));
+    GUST_HIP(hipGetLastError());
+    auto* h = static_cast<double*>(Workspace::get_host_pinned(1, size_t(nv) * 8));
+    GUST_HIP(hipMemcpyAsync(h, P.dots.p, size_t(nv) * 8, hipMemcpyDeviceToHost, st));
+    GUST_HIP(hipStreamSynchronize(st));
+    std::memcpy(host, h, size_t(nv) * 8);
+}
+
+double norm2(PoissonImpl& P, const double* v, hipStream_t st)
+{
+    double d;
+    multi_dot(P, v, 0, 1, v, &d, st);
+    return std::sqrt(d);
+}
+
+void set_coef(PoissonImpl& P, const double* h, int nv, hipStream_t st)
+{
+    auto* hp = static_cast<double*>(Workspace::get_host_pinned(2, size_t(nv) * 8));
+    GUST_HIP(hipStreamSynchronize(st));   // the pinned buffer may still feed a previous copy
+    std::memcpy(hp, h, size_t(nv) * 8);
+    GUST_HIP(hipMemcpyAsync(P.hcoef.p, hp, size_t(nv) * 8, hipMemcpyHostToDevice, st));
+}
+
+// J-weighted mean of a level-0 compact field: sum J v / sum J.
+double weighted_mean(PoissonImpl& P, const double* v, hipStream_t st)
+{
+    const int64_t n = P.n();
+    double num, den;
+    multi_dot(P, ptr<double>(P.L0.J), 0, 1, v, &num, st);
+    fill_kernel<double><<<blocks_for(n), kThreads, 0, st>>>(ptr<double>(P.w), 1.0, n);
+    multi_dot(P, ptr<double>(P.L0.J), 0, 1, ptr<double>(P.w), &den, st);
+    return num / den;
+}
+
+void check_poisson_options(const gust_poisson_options& o)
+{
+    std::string err;
+    const double eps = std::numeric_limits<double>::epsilon();
+    if (!std::isfinite(o.reltol) || !(o.reltol >= 10 * eps) || !(o.reltol < 1)) {
+        char b[160];
+        std::snprintf(b, sizeof b, "\n  solver.poisson_reltol must be >= %g (10 machine epsilon) and < 1", 10 * eps);
+        err += b;
+    }
+    if (!std::isfinite(o.abstol) || !(o.abstol >= 0)) { err += "\n  solver.poisson_abstol must be finite and >= 0"; }
+    if (o.max_iter < 1) { err += "\n  solver.poisson_max_iter must be >= 1"; }
+    if (o.restart < 1) { err += "\n  solver.poisson_restart must be >= 1"; }
+    if (o.precision != 0 && o.precision != 1) { err += "\n  precision must be 0 (fp64) or 1 (fp32 V-cycle)"; }
+    if (o.pre_sweeps < 0 || o.post_sweeps < 0 || o.coarse_sweeps < 1) {
+        err += "\n  the V-cycle needs pre/post sweeps >= 0 and coarse sweeps >= 1";
+    }
+    if (!err.empty()) { GUST_FAIL("gust: invalid inputs:%s", err.c_str()); }
+}
+
+void check_bc(const PoissonImpl& P, const int* bc)
+{
+    if (!bc) { GUST_FAIL("gust: PoissonSolver: the boundary conditions are null"); }
+    std::string err;
+    const char* nm[3] = {"x", "y", "z"};
+    for (int d = 0; d < 3; ++d) {
+        for (int s = 0; s < 2; ++s) {
+            const int f = bc[d + 3 * s];
+            if (f < 0 || f > 2) { GUST_FAIL("gust: PoissonSolver: unknown boundary code %d", f); }
+            const bool per = d < 2 ? P.per[d] != 0 : false;
+            if (per != (f == PF_PERIODIC)) {
+                err += std::string("\n  Poisson BC ") + nm[d] + (s ? "hi" : "lo") +
+                       ": periodic exactly when the domain is periodic in " + nm[d];
+            }
+        }
+    }
+    if (!err.empty()) { GUST_FAIL("gust: invalid inputs:%s", err.c_str()); }
+}
+
+// Check and bring in the face coefficients; form a beta on level 0 and the
+// z_cross correction (Source/Poisson.cpp:421-515).
+void prepare(PoissonImpl& P, const gust_face_coeffs* fc, const BcInfo& bc, hipStream_t st)
+{
+    const gust_array4* b[4] = {nullptr, nullptr, nullptr, nullptr};
+    if (fc) { b[0] = fc->x; b[1] = fc->y; b[2] = fc->z; b[3] = fc->z_cross; }
+    const char* bname[4] = {"FaceCoeffs::x", "FaceCoeffs::y", "FaceCoeffs::z", "FaceCoeffs::z_cross"};
+    auto* flag = static_cast<unsigned long long*>(Workspace::get(st, 10, 16));
+    for (int d = 0; d < 4; ++d) {
+        if (!b[d]) { continue; }
+        const int dir = d < 3 ? d : 2;
+        IBox fb;
+        for (int q = 0; q < 3; ++q) {
+            fb.lo[q] = P.geom.lo[q];
+            fb.hi[q] = P.geom.hi[q] + (q == dir ? 1 : 0);
+        }
+        if (!covers(b[d], fb, 1)) { GUST_FAIL("gust: %s must be a face MultiFab on the mesh's box array", bname[d]); }
+        fill_kernel<unsigned long long><<<1, 1, 0, st>>>(flag, 0ull, 1);
+        const double* p = static_cast<const double*>(b[d]->p) + offset_of(*b[d], fb.lo[0], fb.lo[1], fb.lo[2]);
+        const int ex = fb.len(0), ey = fb.len(1), ez = fb.len(2);
+        minmax_kernel<<<blocks_for(int64_t(ex) * ey * ez), kThreads, 0, st>>>(p, b[d]->jstride, b[d]->kstride, ex, ey,
+                                                                             ez, flag);
+        unsigned long long h = 0;
+        GUST_HIP(hipMemcpyAsync(&h, flag, 8, hipMemcpyDeviceToHost, st));
+        GUST_HIP(hipStreamSynchronize(st));
+        if (h) { GUST_FAIL("gust: %s must be finite and > 0 everywhere", bname[d]); }
+    }
+    if (b[3] && bc.dir_ground) { GUST_FAIL("gust: FaceCoeffs::z_cross with a Dirichlet ground is not supported"); }
+    if (fc && (!std::isfinite(fc->precond_ratio) || !(fc->precond_ratio > 0))) {
+        GUST_FAIL("gust: FaceCoeffs::precond_ratio must be finite and > 0");
+    }
+    // a beta, as compact valid-face arrays.
+    DevBuf* mesh[3] = {&P.ax_mesh, &P.ay_mesh, &P.az_mesh};
+    DevBuf* out[3] = {&P.L0.ax, &P.L0.ay, &P.L0.az};
+    for (int d = 0; d < 3; ++d) {
+        const int ex = P.nx + (d == 0), ey = P.ny + (d == 1), ez = P.nz + (d == 2);
+        const int64_t ne = int64_t(ex) * ey * ez;
+        if (!b[d]) {
+            GUST_HIP(hipMemcpyAsync(out[d]->p, mesh[d]->p, size_t(ne) * 8, hipMemcpyDeviceToDevice, st));
+            continue;
+        }
+        double* tmp = ptr<double>(P.w);
+        view_copy_kernel<double><<<blocks_for(ne), kThreads, 0, st>>>(
+            static_cast<double*>(b[d]->p) + offset_of(*b[d], P.geom.lo[0], P.geom.lo[1], P.geom.lo[2]),
+            b[d]->jstride, b[d]->kstride, tmp, ex, ey, ez, 0);
+        mul_kernel<double><<<blocks_for(ne), kThreads, 0, st>>>(ptr<double>(*mesh[d]), tmp, ptr<double>(*out[d]), ne);
+        GUST_HIP(hipGetLastError());
+    }
+    P.L0.has_cross = b[3] != nullptr;
+    if (P.L0.has_cross) {
+        const int64_t ne = int64_t(P.nx) * P.ny * (P.nz + 1);
+        P.L0.dcross.ensure(size_t(ne) * 8);
+        P.L0.azm.ensure(size_t(ne) * 8);
+        double* d = ptr<double>(P.L0.dcross);
+        double* tmp = ptr<double>(P.w);
+        const gust_array4* bzc = b[3];
+        if (b[2]) {
+            view_copy_kernel<double><<<blocks_for(ne), kThreads, 0, st>>>(
+                static_cast<double*>(b[2]->p) + offset_of(*b[2], P.geom.lo[0], P.geom.lo[1], P.geom.lo[2]),
+                b[2]->jstride, b[2]->kstride, d, P.nx, P.ny, P.nz + 1, 0);
+        } else {
+            fill_kernel<double><<<blocks_for(ne), kThreads, 0, st>>>(d, 1.0, ne);
+        }
+        view_copy_kernel<double><<<blocks_for(ne), kThreads, 0, st>>>(
+            static_cast<double*>(bzc->p) + offset_of(*bzc, P.geom.lo[0], P.geom.lo[1], P.geom.lo[2]), bzc->jstride,
+            bzc->kstride, tmp, P.nx, P.ny, P.nz + 1, 0);
+        // d = beta_z - z_cross
+        multi_axpy_kernel<<<blocks_for(ne), kThreads, 0, st>>>(d, tmp, 0, ptr<double>(P.scal), 1, -1.0, ne);
+        GUST_HIP(hipMemcpyAsync(P.L0.azm.p, P.az_mesh.p, size_t(ne) * 8, hipMemcpyDeviceToDevice, st));
+        GUST_HIP(hipGetLastError());
+    }
+}
+
+void phi_in_out(PoissonImpl& P, gust_array4* phi, double* compact, int dir, hipStream_t st)
+{
+    const int64_t n = P.n();
+    view_copy_kernel<double><<<blocks_for(n), kThreads, 0, st>>>(
+        static_cast<double*>(phi->p) + offset_of(*phi, P.geom.lo[0], P.geom.lo[1], P.geom.lo[2]), phi->jstride,
+        phi->kstride, compact, P.nx, P.ny, P.nz, dir);
+    GUST_HIP(hipGetLastError());
+}
+
+void fill_phi_ghosts(PoissonImpl& P, gust_array4* phi, const BcInfo& bc, hipStream_t st)
+{
+    const MirrorAcc<double> m = mirror<double>(nullptr, P.nx, P.ny, P.nz, P.per, bc);
+    const int64_t n = int64_t(P.nx + 2) * (P.ny + 2) * (P.nz + 2);
+    fill_view_ghosts_kernel<double><<<blocks_for(n), kThreads, 0, st>>>(
+        static_cast<double*>(phi->p) + offset_of(*phi, P.geom.lo[0], P.geom.lo[1], P.geom.lo[2]), phi->jstride,
+        phi->kstride, m);
+    GUST_HIP(hipGetLastError());
+}
+
+void check_phi(const PoissonImpl& P, const gust_array4* phi, const char* what)
+{
+    IBox V;
+    for (int q = 0; q < 3; ++q) { V.lo[q] = P.geom.lo[q]; V.hi[q] = P.geom.hi[q]; }
+    if (!covers(phi, grow(V, 1, 1, 1), 1)) { GUST_FAIL("gust: PoissonSolver: %s needs a ghost cell", what); }
+}
+
+void check_cells(const PoissonImpl& P, const gust_array4* a, const char* what)
+{
+    IBox V;
+    for (int q = 0; q < 3; ++q) { V.lo[q] = P.geom.lo[q]; V.hi[q] = P.geom.hi[q]; }
+    if (!covers(a, V, 1)) {
+        GUST_FAIL("gust: PoissonSolver: %s must be on the mesh's BoxArray and DistributionMapping", what);
+    }
+}
+
+void ensure_work(PoissonImpl& P, int m)
+{
+    const int64_t n = P.n();
+    P.w.ensure(size_t(n + 1) * 8 * 2 + 64);
+    P.krylov_v.ensure(size_t(n) * 8 * size_t(m + 1));
+    P.krylov_z.ensure(size_t(n) * 8 * size_t(m));
+    P.partial.ensure(size_t(kRedBlocks) * 8 * size_t(m + 2));
+    P.dots.ensure(size_t(m + 2) * 8 + 64);
+    P.hcoef.ensure(size_t(m + 2) * 8 + 64);
+}
+
+void apply_fp64(PoissonImpl& P, const BcInfo& bc, const double* x, double* y, hipStream_t st)
+{
+    launch_apply<double>(P.L0, bc, P.per, x, nullptr, y, -1, st);
+}
+
+}   // namespace
+
+}   // namespace gust_amd
+
+using namespace gust_amd;
+
+struct gust_poisson {
+    PoissonImpl impl;
+};
+
+extern "C" {
+
+int gust_poisson_default_options(gust_poisson_options* o)
+{
+    return guarded([&] {
+        if (!o) { GUST_FAIL("gust: poisson options are null"); }
+        o->reltol = 1e-10;
+        o->abstol = 0;
+        o->max_iter = 200;
+        o->restart = 50;
+        o->verbose = 0;
+        o->precision = 0;
+        o->pre_sweeps = 2;
+        o->post_sweeps = 2;
+        o->coarse_sweeps = 16;
+    });
+}
+
+int gust_poisson_create(const gust_geometry* geom, const gust_array4* z_nd, const gust_array4* detJ,
+                        const gust_array4* ax, const gust_array4* ay, const gust_array4* az,
+                        const gust_poisson_options* opt, gust_poisson** out, void* stream)
+{
+    return guarded([&] {
+        if (!geom || !z_nd || !detJ || !ax || !ay || !az || !opt || !out) {
+            GUST_FAIL("gust: PoissonSolver: null argument");
+        }
+        *out = nullptr;
+        hipStream_t st = (hipStream_t)stream;
+        check_poisson_options(*opt);
+        if (geom->periodic[2]) { GUST_FAIL("gust: PoissonSolver: the domain cannot be periodic in z"); }
+        auto h = std::make_unique<gust_poisson>();
+        PoissonImpl& P = h->impl;
+        P.geom = *geom;
+        P.opt = *opt;
+        P.nx = geom->hi[0] - geom->lo[0] + 1;
+        P.ny = geom->hi[1] - geom->lo[1] + 1;
+        P.nz = geom->hi[2] - geom->lo[2] + 1;
+        if (P.nx < 1 || P.ny < 1 || P.nz < 2) { GUST_FAIL("gust: PoissonSolver: the domain needs nz >= 2"); }
+        P.per[0] = geom->periodic[0] != 0;
+        P.per[1] = geom->periodic[1] != 0;
+        // node heights with one ghost node
+        IBox nodes;
+        for (int q = 0; q < 3; ++q) { nodes.lo[q] = geom->lo[q] - 1; nodes.hi[q] = geom->hi[q] + 2; }
+        if (!covers(z_nd, nodes, 1)) { GUST_FAIL("gust: PoissonSolver: z_phys_nd needs one ghost node"); }
+        IBox cells;
+        for (int q = 0; q < 3; ++q) { cells.lo[q] = geom->lo[q]; cells.hi[q] = geom->hi[q]; }
+        if (!covers(detJ, cells, 1)) { GUST_FAIL("gust: PoissonSolver: detJ must cover the domain"); }
+        P.zp_host = view_to_host(*z_nd, nodes.lo, nodes.hi, st);
+        auto faces = [&](const gust_array4* a, int dir, DevBuf& dst) {
+            IBox fb = cells;
+            fb.hi[dir] += 1;
+            if (!covers(a, fb, 1)) { GUST_FAIL("gust: PoissonSolver: the mesh areas must cover the faces"); }
+            upload(dst, view_to_host(*a, fb.lo, fb.hi, st), st);
+        };
+        faces(ax, 0, P.ax_mesh);
+        faces(ay, 1, P.ay_mesh);
+        faces(az, 2, P.az_mesh);
+        const int64_t n = P.n();
+        // level 0 FP64: the mesh as given (J from the caller, zp from the caller)
+        Level<double>& L = P.L0;
+        L.nx = P.nx; L.ny = P.ny; L.nz = P.nz;
+        for (int q = 0; q < 3; ++q) { L.dx[q] = geom->dx[q]; }
+        upload(L.zp, P.zp_host, st);
+        upload(L.J, view_to_host(*detJ, cells.lo, cells.hi, st), st);
+        L.ax.ensure(size_t(P.nx + 1) * P.ny * P.nz * 8);
+        L.ay.ensure(size_t(P.nx) * (P.ny + 1) * P.nz * 8);
+        L.az.ensure(size_t(P.nx) * P.ny * (P.nz + 1) * 8);
+        for (DevBuf* b : {&L.x, &L.b, &L.r}) { b->ensure(size_t(n) * 8); }
+        if (opt->precision == 1) { build_coarse_meshes<float>(P, P.mg32, st); }
+        else { build_coarse_meshes<double>(P, P.mg64, st); }
+        P.scal.ensure(64);
+        fill_kernel<double><<<1, 1, 0, st>>>(ptr<double>(P.scal), 1.0, 1);
+        ensure_work(P, std::max(1, opt->restart));
+        GUST_HIP(hipStreamSynchronize(st));
+        *out = h.release();
+    });
+}
+
+void gust_poisson_destroy(gust_poisson* p) { delete p; }
+
+int gust_poisson_levels(const gust_poisson* p, int32_t* levels)
+{
+    return guarded([&] {
+        if (!p || !levels) { GUST_FAIL("gust: poisson handle is null"); }
+        *levels = int32_t(p->impl.opt.precision == 1 ? p->impl.mg32.size() : p->impl.mg64.size());
+    });
+}
+
+int gust_poisson_fill_ghosts(gust_poisson* p, gust_array4* phi, const int32_t* bc, void* stream)
+{
+    return guarded([&] {
+        if (!p || !phi) { GUST_FAIL("gust: PoissonSolver::fill_ghosts: null argument"); }
+        PoissonImpl& P = p->impl;
+        check_phi(P, phi, "phi");
+        check_bc(P, bc);
+        fill_phi_ghosts(P, phi, bc_info(bc), (hipStream_t)stream);
+    });
+}
+
+int gust_poisson_apply(gust_poisson* p, gust_array4* Aphi, gust_array4* phi, const gust_face_coeffs* coeffs,
+                       const int32_t* bc, void* stream)
+{
+    return guarded([&] {
+        if (!p || !phi || !Aphi) { GUST_FAIL("gust: PoissonSolver::apply: null argument"); }
+        hipStream_t st = (hipStream_t)stream;
+        PoissonImpl& P = p->impl;
+        check_phi(P, phi, "phi");
+        check_cells(P, Aphi, "Aphi");
+        check_bc(P, bc);
+        const BcInfo b = bc_info(bc);
+        prepare(P, coeffs, b, st);
+        phi_in_out(P, phi, ptr<double>(P.L0.x), 0, st);
+        apply_fp64(P, b, ptr<double>(P.L0.x), ptr<double>(P.L0.r), st);
+        view_copy_kernel<double><<<blocks_for(P.n()), kThreads, 0, st>>>(
+            static_cast<double*>(Aphi->p) + offset_of(*Aphi, P.geom.lo[0], P.geom.lo[1], P.geom.lo[2]),
+            Aphi->jstride, Aphi->kstride, ptr<double>(P.L0.r), P.nx, P.ny, P.nz, 1);
+        fill_phi_ghosts(P, phi, b, st);
+        GUST_HIP(hipGetLastError());
+    });
+}
+
+int gust_poisson_fluxes(gust_poisson* p, gust_array4* phi, const gust_face_coeffs* coeffs, const int32_t* bc,
+                        gust_array4* fx, gust_array4* fy, gust_array4* fz, void* stream)
+{
+    return guarded([&] {
+        if (!p || !phi || !fx || !fy || !fz) { GUST_FAIL("gust: PoissonSolver fluxes: null argument"); }
+        hipStream_t st = (hipStream_t)stream;
+        PoissonImpl& P = p->impl;
+        check_phi(P, phi, "phi");
+        check_bc(P, bc);
+        const BcInfo b = bc_info(bc);
+        prepare(P, coeffs, b, st);
+        phi_in_out(P, phi, ptr<double>(P.L0.x), 0, st);
+        const Ctx<double> c = P.L0.ctx(b);
+        const MirrorAcc<double> m = mirror<double>(ptr<double>(P.L0.x), P.nx, P.ny, P.nz, P.per, b);
+        gust_array4* F[3] = {fx, fy, fz};
+        const gust_array4* beta[3] = {coeffs ? coeffs->x : nullptr, coeffs ? coeffs->y : nullptr,
+                                      coeffs ? coeffs->z : nullptr};
+        for (int d = 0; d < 3; ++d) {
+            IBox fb;
+            for (int q = 0; q < 3; ++q) { fb.lo[q] = P.geom.lo[q]; fb.hi[q] = P.geom.hi[q] + (q == d); }
+            if (!covers(F[d], fb, 1)) {
+                GUST_FAIL("gust: PoissonSolver::solve: fluxes must be face MultiFabs on the mesh's box array, "
+                          "without ghost cells");
+            }
+            const double* bp = beta[d] ? static_cast<const double*>(beta[d]->p) +
+                                             offset_of(*beta[d], fb.lo[0], fb.lo[1], fb.lo[2])
+                                       : nullptr;
+            const int64_t ne = int64_t(fb.len(0)) * fb.len(1) * fb.len(2);
+            flux_kernel<double><<<blocks_for(ne), kThreads, 0, st>>>(
+                d, c, m, bp, beta[d] ? beta[d]->jstride : 0, beta[d] ? beta[d]->kstride : 0,
+                static_cast<double*>(F[d]->p) + offset_of(*F[d], fb.lo[0], fb.lo[1], fb.lo[2]), F[d]->jstride,
+                F[d]->kstride);
+            GUST_HIP(hipGetLastError());
+        }
+        fill_phi_ghosts(P, phi, b, st);
+    });
+}
+
+int gust_poisson_solve(gust_poisson* p, gust_array4* rhs, gust_array4* phi, const gust_face_coeffs* coeffs,
+                       const int32_t* bc, gust_array4* fx, gust_array4* fy, gust_array4* fz,
+                       gust_poisson_result* result, void* stream)
+{
+    return guarded([&] {
+        if (!p || !rhs || !phi) { GUST_FAIL("gust: PoissonSolver::solve: null argument"); }
+        hipStream_t st = (hipStream_t)stream;
+        PoissonImpl& P = p->impl;
+        check_phi(P, phi, "phi");
+        check_cells(P, rhs, "rhs");
+        check_bc(P, bc);
+        const BcInfo b = bc_info(bc);
+        prepare(P, coeffs, b, st);
+        const int64_t n = P.n();
+        const int m = std::max(1, P.opt.restart);
+        ensure_work(P, m);
+        gust_poisson_result res{};
+        double* bvec = ptr<double>(P.L0.b);
+        double* x = ptr<double>(P.L0.x);
+        double* r = ptr<double>(P.L0.r);
+        phi_in_out(P, rhs, bvec, 0, st);
+        res.singular = b.singular;
+        if (b.singular) {
+            // J is the left null vector: only the mean-free part of rhs is in
+            // the range of A (Source/Poisson.cpp:559-567); rhs is modified.
+            res.removed_mean = weighted_mean(P, bvec, st);
+            auto* hp = static_cast<double*>(Workspace::get_host_pinned(3, 8));
+            hp[0] = res.removed_mean;
+            GUST_HIP(hipMemcpyAsync(P.dots.p, hp, 8, hipMemcpyHostToDevice, st));
+            add_scalar_kernel<<<blocks_for(n), kThreads, 0, st>>>(bvec, ptr<double>(P.dots), -1.0, n);
+            phi_in_out(P, rhs, bvec, 1, st);
+            GUST_HIP(hipStreamSynchronize(st));
+        }
+        if (P.opt.precision == 1) { setup_hierarchy<float>(P, P.mg32, b, st); }
+        else { setup_hierarchy<double>(P, P.mg64, b, st); }
+        // GMRES(m), right preconditioned, zero initial guess.
+        fill_kernel<double><<<blocks_for(n), kThreads, 0, st>>>(x, 0.0, n);
+        double* V = ptr<double>(P.krylov_v);
+        double* Z = ptr<double>(P.krylov_z);
+        double* w = ptr<double>(P.w);
+        const double r0 = norm2(P, bvec, st);
+        res.initial_residual = r0;
+        const double target = std::max(P.opt.reltol * r0, P.opt.abstol);
+        int iters = 0;
+        bool converged = r0 <= target;
+        std::vector<double> H(size_t(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), hv(m + 2), hv2(m + 2);
+        GUST_HIP(hipMemcpyAsync(r, bvec, size_t(n) * 8, hipMemcpyDeviceToDevice, st));
+        double beta = r0;
+        while (!converged && iters < P.opt.max_iter) {
+            scale_copy_kernel<<<blocks_for(n), kThreads, 0, st>>>(r, 1.0 / beta, V, n);
+            std::fill(g.begin(), g.end(), 0.0);
+            g[0] = beta;
+            int j = 0;
+            for (; j < m && iters < P.opt.max_iter; ++j) {
+                double* vj = V + int64_t(j) * n;
+                double* zj = Z + int64_t(j) * n;
+                precondition(P, vj, zj, b, st);
+                apply_fp64(P, b, zj, w, st);
+                // classical Gram-Schmidt, twice
+                multi_dot(P, V, n, j + 1, w, hv.data(), st);
+                set_coef(P, hv.data(), j + 1, st);
+                multi_axpy_kernel<<<blocks_for(n), kThreads, 0, st>>>(w, V, n, ptr<double>(P.hcoef), j + 1, -1.0, n);
+                multi_dot(P, V, n, j + 1, w, hv2.data(), st);
+                set_coef(P, hv2.data(), j + 1, st);
+                multi_axpy_kernel<<<blocks_for(n), kThreads, 0, st>>>(w, V, n, ptr<double>(P.hcoef), j + 1, -1.0, n);
+                for (int i = 0; i <= j; ++i) { H[size_t(i) + size_t(j) * (m + 1)] = hv[i] + hv2[i]; }
+                const double hn = norm2(P, w, st);
+                H[size_t(j + 1) + size_t(j) * (m + 1)] = hn;
+                if (hn > 0) { scale_copy_kernel<<<blocks_for(n), kThreads, 0, st>>>(w, 1.0 / hn, V + int64_t(j + 1) * n, n); }
+                // Givens rotations
+                for (int i = 0; i < j; ++i) {
+                    double& a = H[size_t(i) + size_t(j) * (m + 1)];
+                    double& c2 = H[size_t(i + 1) + size_t(j) * (m + 1)];
+                    const double t = cs[i] * a + sn[i] * c2;
+                    c2 = -sn[i] * a + cs[i] * c2;
+                    a = t;
+                }
+                double& a = H[size_t(j) + size_t(j) * (m + 1)];
+                double& c2 = H[size_t(j + 1) + size_t(j) * (m + 1)];
+                const double den = std::hypot(a, c2);
+                cs[j] = den > 0 ? a / den : 1.0;
+                sn[j] = den > 0 ? c2 / den : 0.0;
+                a = den;
+                c2 = 0.0;
+                g[j + 1] = -sn[j] * g[j];
+                g[j] = cs[j] * g[j];
+                ++iters;
+                if (P.opt.verbose > 1) { std::fprintf(stderr, "gust: GMRES %d: |r| ~ %.6e\n", iters, std::fabs(g[j + 1])); }
+                if (std::fabs(g[j + 1]) <= target || hn == 0) {
+                    ++j;
+                    break;
+                }
+            }
+            // y = H^{-1} g, x += Z y
+            for (int i = j - 1; i >= 0; --i) {
+                double s = g[i];
+                for (int q = i + 1; q < j; ++q) { s -= H[size_t(i) + size_t(q) * (m + 1)] * y[q]; }
+                y[i] = s / H[size_t(i) + size_t(i) * (m + 1)];
+            }
+            set_coef(P, y.data(), j, st);
+            multi_axpy_kernel<<<blocks_for(n), kThreads, 0, st>>>(x, Z, n, ptr<double>(P.hcoef), j, 1.0, n);
+            // true residual
+            launch_apply<double>(P.L0, b, P.per, x, bvec, r, -1, st);
+            beta = norm2(P, r, st);
+            converged = beta <= target;
+        }
+        res.iterations = iters;
+        if (b.singular) {
+            const double mean = weighted_mean(P, x, st);
+            auto* hp = static_cast<double*>(Workspace::get_host_pinned(3, 8));
+            GUST_HIP(hipStreamSynchronize(st));
+            hp[0] = mean;
+            GUST_HIP(hipMemcpyAsync(P.dots.p, hp, 8, hipMemcpyHostToDevice, st));
+            add_scalar_kernel<<<blocks_for(n), kThreads, 0, st>>>(x, ptr<double>(P.dots), -1.0, n);
+        }
+        launch_apply<double>(P.L0, b, P.per, x, bvec, r, -1, st);
+        res.final_residual = norm2(P, r, st);
+        phi_in_out(P, phi, x, 1, st);
+        fill_phi_ghosts(P, phi, b, st);
+        if (result) { *result = res; }
+        if (!converged) {
+            GUST_HIP(hipStreamSynchronize(st));
+            GUST_FAIL("gust: the Poisson solve did not converge in solver.poisson_max_iter = %d iterations (residual "
+                      "%g of %g; solver.poisson_reltol = %g, solver.poisson_abstol = %g)",
+                      P.opt.max_iter, res.final_residual, res.initial_residual, P.opt.reltol, P.opt.abstol);
+        }
+        if (fx || fy || fz) {
+            if (!(fx && fy && fz)) { GUST_FAIL("gust: PoissonSolver::solve: give all three flux arrays or none"); }
+            const Ctx<double> c = P.L0.ctx(b);
+            const MirrorAcc<double> mm = mirror<double>(x, P.nx, P.ny, P.nz, P.per, b);
+            gust_array4* F[3] = {fx, fy, fz};
+            const gust_array4* beta3[3] = {coeffs ? coeffs->x : nullptr, coeffs ? coeffs->y : nullptr,
+                                           coeffs ? coeffs->z : nullptr};
+            for (int d = 0; d < 3; ++d) {
+                IBox fb;
+                for (int q = 0; q < 3; ++q) { fb.lo[q] = P.geom.lo[q]; fb.hi[q] = P.geom.hi[q] + (q == d); }
+                if (!covers(F[d], fb, 1)) {
+                    GUST_FAIL("gust: PoissonSolver::solve: fluxes must be face MultiFabs on the mesh's box array, "
+                              "without ghost cells");
+                }
+                const double* bp = beta3[d] ? static_cast<const double*>(beta3[d]->p) +
+                                                  offset_of(*beta3[d], fb.lo[0], fb.lo[1], fb.lo[2])
+                                            : nullptr;
+                const int64_t ne = int64_t(fb.len(0)) * fb.len(1) * fb.len(2);
+                flux_kernel<double><<<blocks_for(ne), kThreads, 0, st>>>(
+                    d, c, mm, bp, beta3[d] ? beta3[d]->jstride : 0, beta3[d] ? beta3[d]->kstride : 0,
+                    static_cast<double*>(F[d]->p) + offset_of(*F[d], fb.lo[0], fb.lo[1], fb.lo[2]), F[d]->jstride,
+                    F[d]->kstride);
+                GUST_HIP(hipGetLastError());
+            }
+        }
+        GUST_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int gust_poisson_vcycle(gust_poisson* p, const double* b, double* x, const gust_face_coeffs* coeffs,
+                        const int32_t* bc, int setup, void* stream)
+{
+    return guarded([&] {
+        if (!p || !b || !x) { GUST_FAIL("gust: poisson_vcycle: null argument"); }
+        hipStream_t st = (hipStream_t)stream;
+        PoissonImpl& P = p->impl;
+        check_bc(P, bc);
+        const BcInfo bi = bc_info(bc);
+        if (setup) {
+            prepare(P, coeffs, bi, st);
+            if (P.opt.precision == 1) { setup_hierarchy<float>(P, P.mg32, bi, st); }
+            else { setup_hierarchy<double>(P, P.mg64, bi, st); }
+        }
+        precondition(P, b, x, bi, st);
+    });
+}
+
+int gust_face_divergence(const gust_geometry* geom, const gust_array4* ax, const gust_array4* ay,
+                         const gust_array4* az, const gust_array4* detJ, const gust_array4* fx,
+                         const gust_array4* fy, const gust_array4* fz, gust_array4* div, void* stream)
+{
+    return guarded([&] {
+        if (!geom || !ax || !ay || !az || !detJ || !fx || !fy || !fz || !div) {
+            GUST_FAIL("gust: face_divergence: null argument");
+        }
+        IBox cells;
+        for (int q = 0; q < 3; ++q) { cells.lo[q] = geom->lo[q]; cells.hi[q] = geom->hi[q]; }
+        const gust_array4* F[3] = {fx, fy, fz};
+        const gust_array4* A[3] = {ax, ay, az};
+        for (int d = 0; d < 3; ++d) {
+            IBox fb = cells;
+            fb.hi[d] += 1;
+            if (!covers(F[d], fb, 1) || !covers(A[d], fb, 1)) {
+                GUST_FAIL("gust: face_divergence: F must be face MultiFabs on the mesh's box array");
+            }
+        }
+        if (!covers(div, cells, 1) || !covers(detJ, cells, 1)) {
+            GUST_FAIL("gust: face_divergence: div must be on the mesh's box array");
+        }
+        const int nx = cells.len(0), ny = cells.len(1), nz = cells.len(2);
+        auto P0 = [&](const gust_array4* a) {
+            return static_cast<const double*>(a->p) + offset_of(*a, cells.lo[0], cells.lo[1], cells.lo[2]);
+        };
+        face_div_kernel<<<blocks_for(int64_t(nx) * ny * nz), kThreads, 0, (hipStream_t)stream>>>(
+            nx, ny, nz, 1.0 / geom->dx[0], 1.0 / geom->dx[1], 1.0 / geom->dx[2], P0(ax), ax->jstride, ax->kstride,
+            P0(ay), ay->jstride, ay->kstride, P0(az), az->jstride, az->kstride, P0(fx), fx->jstride, fx->kstride,
+            P0(fy), fy->jstride, fy->kstride, P0(fz), fz->jstride, fz->kstride, P0(detJ), detJ->jstride,
+            detJ->kstride,
+            static_cast<double*>(div->p) + offset_of(*div, cells.lo[0], cells.lo[1], cells.lo[2]), div->jstride,
+            div->kstride);
+        GUST_HIP(hipGetLastError());
+    });
+}
+
+}

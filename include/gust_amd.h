@@ -154,6 +154,105 @@ int gust_line_sweeps(const gust_line_options* opt, int loc, const gust_array4* A
  * 0 if valid, else nonzero with the message naming the steady.<key>. */
 int gust_check_line_options(const gust_line_options* opt);
 
+/* ------------------------------------------------------------------------
+ * The terrain Poisson solve (pressure correction, initializer projection).
+ *
+ * Boundary conditions: 6 codes in amrex::Orientation order xlo, ylo, zlo,
+ * xhi, yhi, zhi (Source/Poisson.H:20-34, PoissonFace/PoissonBC). */
+enum { GUST_PF_PERIODIC = 0, GUST_PF_NEUMANN = 1, GUST_PF_DIRICHLET = 2 };
+
+/* Source/Poisson.H:40-69 (PoissonOptions), with the multigrid's own knobs.
+ * precision: 0 = FP64 V-cycle, 1 = FP32 V-cycle under FP64 GMRES. */
+typedef struct gust_poisson_options {
+    double reltol;         /* solver.poisson_reltol, default 1e-10 */
+    double abstol;         /* solver.poisson_abstol, default 0 */
+    int32_t max_iter;      /* solver.poisson_max_iter, default 200 */
+    int32_t restart;       /* solver.poisson_restart, default 50 */
+    int32_t verbose;
+    int32_t precision;
+    int32_t pre_sweeps, post_sweeps, coarse_sweeps;   /* V(2,2), 16 on the coarsest level */
+} gust_poisson_options;
+
+/* Source/Poisson.H:96-103 (FaceCoeffs): optional face coefficients (null =
+ * 1), each a face field on the domain's faces; precond_ratio is accepted for
+ * the interface and checked, the multigrid does not need it. */
+typedef struct gust_face_coeffs {
+    const gust_array4* x;
+    const gust_array4* y;
+    const gust_array4* z;
+    const gust_array4* z_cross;
+    double precond_ratio;
+} gust_face_coeffs;
+
+/* Source/Poisson.H:105-117 (PoissonResult). */
+typedef struct gust_poisson_result {
+    int32_t iterations;
+    double initial_residual, final_residual;
+    int32_t singular;
+    double removed_mean;
+} gust_poisson_result;
+
+typedef struct gust_poisson gust_poisson;
+
+/* The documented defaults (Source/Poisson.H:45-65) and V(2,2). */
+int gust_poisson_default_options(gust_poisson_options* opt);
+
+/* Replaces: PoissonSolver(const Mesh&, const Geometry&, const MultiFab&
+ *   z_phys_nd, const PoissonOptions&) -- Source/Poisson.H:138-139,
+ *   Source/Poisson.cpp:395-403. z_nd: node heights with one ghost node;
+ *   detJ, ax, ay, az: the mesh metrics on the cells / faces. The solver keeps
+ *   its own copy of the mesh and builds the multigrid levels once. */
+int gust_poisson_create(const gust_geometry* geom, const gust_array4* z_nd, const gust_array4* detJ,
+                        const gust_array4* ax, const gust_array4* ay, const gust_array4* az,
+                        const gust_poisson_options* opt, gust_poisson** solver, void* stream);
+void gust_poisson_destroy(gust_poisson* solver);
+/* Multigrid levels built (diagnostic). */
+int gust_poisson_levels(const gust_poisson* solver, int32_t* levels);
+
+/* Replaces: PoissonResult PoissonSolver::solve(MultiFab& rhs, MultiFab& phi,
+ *   const FaceCoeffs&, const PoissonBC&, Array<MultiFab,3>* fluxes)
+ *   -- Source/Poisson.H:151-152, Source/Poisson.cpp:540-623.
+ * rhs: cells (modified when the problem is singular: its J-weighted mean is
+ * removed); phi: cells with one ghost cell, zero initial guess, returned with
+ * its ghosts filled for bc; fx, fy, fz (all or none): the coefficient-
+ * weighted fluxes F = -g on the faces. Fails, naming
+ * solver.poisson_max_iter, when the solve does not converge. Synchronises. */
+int gust_poisson_solve(gust_poisson* solver, gust_array4* rhs, gust_array4* phi, const gust_face_coeffs* coeffs,
+                       const int32_t* bc, gust_array4* fx, gust_array4* fy, gust_array4* fz,
+                       gust_poisson_result* result, void* stream);
+
+/* Replaces: PoissonSolver::apply(MultiFab& Aphi, MultiFab& phi, const
+ *   FaceCoeffs&, const PoissonBC&) -- Source/Poisson.H:156, Source/Poisson.cpp:517-529
+ *   (ERF's terrpoisson_adotx with the zero-flux mask, the Dirichlet ground,
+ *   the z_cross correction). phi's ghosts are filled. */
+int gust_poisson_apply(gust_poisson* solver, gust_array4* Aphi, gust_array4* phi, const gust_face_coeffs* coeffs,
+                       const int32_t* bc, void* stream);
+
+/* Replaces: PoissonSolver::fill_ghosts -- Source/Poisson.H:161, Source/Poisson.cpp:531-538. */
+int gust_poisson_fill_ghosts(gust_poisson* solver, gust_array4* phi, const int32_t* bc, void* stream);
+
+/* The fluxes of a given phi, as solve returns them.
+ * Replaces: TerrainPoisson::getFluxes + the coefficient multiply and
+ *   add_cross_flux of solve -- extern_erf/ERF_TerrainPoisson.cpp:212-264,
+ *   Source/Poisson.cpp:609-619. */
+int gust_poisson_fluxes(gust_poisson* solver, gust_array4* phi, const gust_face_coeffs* coeffs, const int32_t* bc,
+                        gust_array4* fx, gust_array4* fy, gust_array4* fz, void* stream);
+
+/* One multigrid V-cycle x = M^{-1} b on compact device vectors (nx*ny*nz
+ * doubles, i fastest); setup = 1 first takes the coefficients. The
+ * preconditioner of gust_poisson_solve, exposed for timing and tests.
+ * Replaces: the MLMG cycle inside GMRESMLMG (AMReX, third party),
+ *   called from Source/Poisson.cpp:681-687. */
+int gust_poisson_vcycle(gust_poisson* solver, const double* b, double* x, const gust_face_coeffs* coeffs,
+                        const int32_t* bc, int setup, void* stream);
+
+/* Replaces: void face_divergence(const Mesh&, const Geometry&, const
+ *   Array<const MultiFab*,3>& F, MultiFab& div) -- Source/Poisson.H:209-210,
+ *   Source/Poisson.cpp:733-764. */
+int gust_face_divergence(const gust_geometry* geom, const gust_array4* ax, const gust_array4* ay,
+                         const gust_array4* az, const gust_array4* detJ, const gust_array4* fx,
+                         const gust_array4* fy, const gust_array4* fz, gust_array4* div, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

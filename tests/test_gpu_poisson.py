@@ -1,0 +1,251 @@
+"""GPU parity of the terrain Poisson layer against the oracle (through the C-ABI).
+
+* The operator apply, the BC ghost fill, the fluxes and face_divergence are
+  bit-identical to the oracle in FP64 (array_equal).
+* The solve (GMRES + multigrid, FP64 or FP32 V-cycle) converges to the
+  oracle operator's direct solution within 1e-8 relative (max norm), and on
+  the reference's Askervein projection to the gold potential within 1e-8
+  with the divergence reduced by the requested tolerance
+  (Tests/Discretization/Poisson_Askervein, check_poisson.py's criteria).
+* The V-cycle alone is a convergent stationary iteration.
+"""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from ablsteadyrans_amd import mesh as M
+from oracle import oracle as orc
+from tests import poisson_helpers as H
+
+pytestmark = pytest.mark.gpu
+N, D, P = H.N, H.D, H.P
+
+
+def _hill(n=(12, 10, 8), periodic=(0, 0), height=60.0):
+    lo, hi = (0.0, 0.0, 0.0), (n[0] * 25.0, n[1] * 25.0, 400.0)
+    geom = M.geometry(n, lo, hi, periodic)
+    zl = M.make_zlevels(n[2], 400.0, dz0=15.0, stretch_ratio=1.1)
+    t = M.Terrain(kind="hills", hills=[(0.45 * hi[0], 0.55 * hi[1], height, 70.0)])
+    return M.build_mesh(geom, lo, hi, zl, t)
+
+
+def _dev(a, begin):
+    import torch
+    import ablsteadyrans_amd as G
+    a = np.ascontiguousarray(a)
+    if a.ndim == 3:
+        a = a[None]
+    return G.Fab(torch.from_numpy(a.copy()).to("cuda"), begin)
+
+
+def _solver(m, **kw):
+    from ablsteadyrans_amd import poisson as PS
+    return PS.PoissonSolver(PS.DeviceMesh(m), PS.PoissonOptions(**kw))
+
+
+def _coeffs(m, rng, cross):
+    nx, ny, nz = m.geom.n
+    bx = 0.5 + rng.random(m.ax.shape)
+    by = 0.5 + rng.random(m.ay.shape)
+    bz = 0.5 + rng.random(m.az.shape)
+    dc = None
+    zc = None
+    if cross:
+        zc = 0.3 + rng.random(m.az.shape)
+        dc = bz - zc
+    return bx, by, bz, zc, dc
+
+
+def _face_coeffs(bx, by, bz, zc):
+    from ablsteadyrans_amd import poisson as PS
+    # the valid faces (drop the one ghost layer)
+    return PS.FaceCoeffs(_dev(bx[1:-1, 1:-1, 1:-1], (0, 0, 0)), _dev(by[1:-1, 1:-1, 1:-1], (0, 0, 0)),
+                         _dev(bz[1:-1, 1:-1, 1:-1], (0, 0, 0)),
+                         None if zc is None else _dev(zc[1:-1, 1:-1, 1:-1], (0, 0, 0)))
+
+
+CASES = [((0, 0), (N, D), (N, D)), ((1, 0), (N, N), (N, N)), ((0, 1), (D, N), (D, D)), ((1, 1), (N, N), (N, D)),
+         ((0, 0), (D, D), (N, N))]
+
+
+@pytest.mark.parametrize("periodic,bcx,bcz", CASES)
+def test_apply_fluxes_and_ghosts_are_bit_identical(cuda, periodic, bcx, bcz):
+    import torch
+    m = _hill(periodic=periodic)
+    bc = H.bc_for(m, x=bcx, z=bcz)
+    S = _solver(m)
+    rng = np.random.default_rng(11)
+    nx, ny, nz = m.geom.n
+    phi = rng.standard_normal((nz, ny, nx))
+    for with_coeffs in (False, True):
+        cross = with_coeffs and bc[2] != D
+        if with_coeffs:
+            bx, by, bz, zc, dc = _coeffs(m, rng, cross)
+            fc = _face_coeffs(bx, by, bz, zc)
+            ref = H.apply(m, phi, bc, (bx, by, bz), dc)
+        else:
+            fc = None
+            ref = H.apply(m, phi, bc)
+        ph = _dev(np.pad(phi, 1), (-1, -1, -1))
+        Aphi = _dev(np.zeros((nz, ny, nx)), (0, 0, 0))
+        S.apply(Aphi, ph, fc, bc)
+        got = Aphi.numpy()[0]
+        assert np.array_equal(got, ref), np.max(np.abs(got - ref))
+        # phi's ghosts hold the boundary conditions, as the oracle fills them
+        g = np.pad(phi, 1)[None].copy()
+        orc.poisson_fill_ghosts(g, H.B, H.ogeom(m), bc)
+        assert np.array_equal(ph.numpy(), g)
+        # fluxes
+        F = [_dev(np.zeros((nz, ny, nx + 1)), (0, 0, 0)), _dev(np.zeros((nz, ny + 1, nx)), (0, 0, 0)),
+             _dev(np.zeros((nz + 1, ny, nx)), (0, 0, 0))]
+        S.fluxes(ph, fc, bc, F)
+        cf = (bx, by, bz) if with_coeffs else (None, None, None)
+        ref_f = orc.poisson_fluxes(g, H.B, m.z_nd, H.B, H.ogeom(m), bc, coeffs=cf, coeff_begin=H.B,
+                                   dcross=dc if with_coeffs else None)
+        for d in range(3):
+            assert np.array_equal(F[d].numpy()[0], ref_f[d]), d
+    torch.cuda.synchronize()
+
+
+def test_face_divergence_is_bit_identical(cuda):
+    from ablsteadyrans_amd import poisson as PS
+    m = _hill(periodic=(1, 0))
+    nx, ny, nz = m.geom.n
+    rng = np.random.default_rng(4)
+    F = [rng.standard_normal((nz, ny, nx + 1)), rng.standard_normal((nz, ny + 1, nx)),
+         rng.standard_normal((nz + 1, ny, nx))]
+    ref = H.face_div(m, F)
+    dm = PS.DeviceMesh(m)
+    div = _dev(np.zeros((nz, ny, nx)), (0, 0, 0))
+    PS.face_divergence(dm, [_dev(f, (0, 0, 0)) for f in F], div)
+    assert np.array_equal(div.numpy()[0], ref)
+
+
+def _direct(m, bc, rhs, coeffs=None, dcross=None):
+    A = H.sparse_operator(m, bc, coeffs, dcross).tolil()
+    b = rhs.ravel().copy()
+    singular = D not in bc
+    if singular:
+        J = m.detJ[1:-1, 1:-1, 1:-1].ravel()
+        b = b - np.dot(J, b) / J.sum()
+        A[0, :] = 0
+        A[0, 0] = 1.0
+        b[0] = 0.0
+    x = spla.spsolve(A.tocsc(), b)
+    if singular:
+        J = m.detJ[1:-1, 1:-1, 1:-1].ravel()
+        x = x - np.dot(J, x) / J.sum()
+    return x.reshape(rhs.shape)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("periodic,bcx,bcz", CASES)
+def test_solve_converges_to_the_direct_solution(cuda, precision, periodic, bcx, bcz):
+    m = _hill(n=(16, 12, 10), periodic=periodic)
+    bc = H.bc_for(m, x=bcx, z=bcz)
+    nx, ny, nz = m.geom.n
+    rng = np.random.default_rng(5)
+    rhs = rng.standard_normal((nz, ny, nx))
+    ref = _direct(m, bc, rhs)
+    S = _solver(m, reltol=1e-12, precision=precision)
+    r = _dev(rhs, (0, 0, 0))
+    ph = _dev(np.zeros((nz + 2, ny + 2, nx + 2)), (-1, -1, -1))
+    res = S.solve(r, ph, None, bc)
+    got = ph.numpy()[0, 1:-1, 1:-1, 1:-1]
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    assert err <= 1e-8, (err, res)
+    assert res.singular == (D not in bc)
+    assert res.final_residual <= 1e-11 * res.initial_residual
+    assert 1 <= res.iterations <= 60, res
+
+
+def test_solve_with_coefficients_and_split_cross_term(cuda):
+    m = _hill(n=(16, 12, 10), periodic=(0, 0))
+    bc = H.bc_for(m, x=(N, D), z=(N, D))
+    rng = np.random.default_rng(8)
+    bx, by, bz, zc, dc = _coeffs(m, rng, True)
+    nx, ny, nz = m.geom.n
+    rhs = rng.standard_normal((nz, ny, nx))
+    ref = _direct(m, bc, rhs, (bx, by, bz), dc)
+    for precision in (0, 1):
+        S = _solver(m, reltol=1e-12, precision=precision)
+        r = _dev(rhs, (0, 0, 0))
+        ph = _dev(np.zeros((nz + 2, ny + 2, nx + 2)), (-1, -1, -1))
+        F = [_dev(np.zeros((nz, ny, nx + 1)), (0, 0, 0)), _dev(np.zeros((nz, ny + 1, nx)), (0, 0, 0)),
+             _dev(np.zeros((nz + 1, ny, nx)), (0, 0, 0))]
+        res = S.solve(r, ph, _face_coeffs(bx, by, bz, zc), bc, fluxes=F)
+        got = ph.numpy()[0, 1:-1, 1:-1, 1:-1]
+        assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) <= 1e-8, res
+        # A phi = -div(F): the fluxes carry the correction
+        d = H.face_div(m, [f.numpy()[0] for f in F])
+        assert np.max(np.abs(d + rhs)) <= 1e-9 * np.max(np.abs(rhs))
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_askervein_projection_matches_the_gold(cuda, precision):
+    from ablsteadyrans_amd import poisson as PS
+    m, F, bc = H.askervein_problem()
+    nx, ny, nz = m.geom.n
+    dm = PS.DeviceMesh(m)
+    Fd = [_dev(f, (0, 0, 0)) for f in F]
+    rhs = _dev(np.zeros((nz, ny, nx)), (0, 0, 0))
+    PS.face_divergence(dm, Fd, rhs)
+    gold = np.load(f"{H.GOLD}/poisson_askervein.npz")
+    div0 = float(np.sqrt(np.sum(rhs.numpy() ** 2)))
+    assert abs(div0 - float(gold["div_before"])) <= 1e-9 * float(gold["div_before"])
+    S = PS.PoissonSolver(dm, PS.PoissonOptions(reltol=1e-10, precision=precision))
+    ph = _dev(np.zeros((nz + 2, ny + 2, nx + 2)), (-1, -1, -1))
+    Fl = [_dev(np.zeros(f.shape), (0, 0, 0)) for f in F]
+    res = S.solve(rhs, ph, None, bc, fluxes=Fl)
+    phi = ph.numpy()[0, 1:-1, 1:-1, 1:-1]
+    err = np.max(np.abs(phi - gold["phi"])) / np.max(np.abs(gold["phi"]))
+    assert err <= 1e-8, (err, res)
+    U = [F[d] + Fl[d].numpy()[0] for d in range(3)]
+    div1 = H.face_div(m, U)
+    assert np.sqrt(np.sum(div1 ** 2)) <= 10 * 1e-10 * div0
+    print(f"askervein precision {precision}: {res.iterations} GMRES iterations, {S.levels} levels")
+
+
+def test_vcycle_is_a_convergent_iteration(cuda):
+    import torch
+    m = _hill(n=(32, 32, 24), periodic=(0, 0), height=80.0)
+    bc = H.bc_for(m, x=(N, D), z=(N, D))
+    for precision in (0, 1):
+        S = _solver(m, precision=precision)
+        nx, ny, nz = m.geom.n
+        rng = np.random.default_rng(2)
+        b = torch.from_numpy(rng.standard_normal(nx * ny * nz)).to("cuda")
+        x = torch.zeros_like(b)
+        dx = torch.zeros_like(b)
+        norms = []
+        for it in range(6):
+            ph = _dev(np.pad(x.cpu().numpy().reshape(nz, ny, nx), 1), (-1, -1, -1))
+            Ax = _dev(np.zeros((nz, ny, nx)), (0, 0, 0))
+            S.apply(Ax, ph, None, bc)
+            r = b - Ax.data.reshape(-1)
+            norms.append(float(torch.linalg.norm(r)))
+            S.vcycle(r.contiguous(), dx, None, bc, setup=(it == 0))
+            x = x + dx
+        factors = [norms[i + 1] / norms[i] for i in range(len(norms) - 1)]
+        assert max(factors[1:]) < 0.5, (precision, factors)
+
+
+def test_refusals(cuda):
+    import ablsteadyrans_amd as G
+    from ablsteadyrans_amd import poisson as PS
+    m = _hill(periodic=(1, 0))
+    dm = PS.DeviceMesh(m)
+    with pytest.raises(G.GustError, match="solver.poisson_reltol"):
+        PS.PoissonSolver(dm, PS.PoissonOptions(reltol=0.0))
+    S = PS.PoissonSolver(dm)
+    nx, ny, nz = m.geom.n
+    ph = _dev(np.zeros((nz + 2, ny + 2, nx + 2)), (-1, -1, -1))
+    r = _dev(np.ones((nz, ny, nx)), (0, 0, 0))
+    with pytest.raises(G.GustError, match="periodic exactly when the domain is periodic in x"):
+        S.solve(r, ph, None, (N, N, N, N, N, D))
+    bad = np.full(m.ax.shape, -1.0)
+    with pytest.raises(G.GustError, match="FaceCoeffs::x must be finite and > 0"):
+        S.solve(r, ph, PS.FaceCoeffs(x=_dev(bad[1:-1, 1:-1, 1:-1], (0, 0, 0))), (P, N, N, P, N, D))
+    S2 = PS.PoissonSolver(dm, PS.PoissonOptions(max_iter=1, reltol=1e-14))
+    with pytest.raises(G.GustError, match="solver.poisson_max_iter"):
+        S2.solve(r, ph, None, (P, N, N, P, N, D))
