@@ -103,6 +103,14 @@ class PoissonResultC(C.Structure):
 
 
 def _load() -> C.CDLL:
+    # One HIP runtime per process: torch ships its own libamdhip64 and the
+    # device tensors, streams and events handed to the library come from it.
+    # Loading torch first makes the library bind to that runtime instead of
+    # bringing a second one in (a second runtime sees no usable device).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "

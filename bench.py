@@ -136,6 +136,49 @@ def _cpu_baseline(target_s=12.0):
             "ms_per_sweep": round(per_sweep * 1e3, 3)}
 
 
+def _fractal_tile():
+    """The fractal-terrain tile: 256 x 256 x 96 cells at dx = 25 m, a seeded
+    k^-2 random surface (rms 80 m, clipped to 0..400 m), first layer 2 m with
+    the ratio solved so the top is 2000 m, BTF."""
+    from ablsteadyrans_amd import mesh as M
+    lo, hi = (0.0, 0.0, 0.0), (NX * 25.0, NY * 25.0, 2000.0)
+    geom = M.geometry((NX, NY, NZ), lo, hi)
+    zl = M.make_zlevels(NZ, 2000.0, dz0=2.0, stretch_ratio=0.0)
+    h = M.fractal_heights(NX, NY, seed=20264203, rms=80.0, slope=-2.0, clip=(0.0, 400.0))
+    return M.build_mesh(geom, lo, hi, zl, M.Terrain(kind="fractal", heights=h))
+
+
+def _time_vcycle(tile, precision, reps):
+    """One V(2,2) cycle of the pressure-correction multigrid on the tile, with
+    SIMPLEC-like face coefficients; returns (ms per cycle, levels)."""
+    import torch
+    from ablsteadyrans_amd import poisson as PS
+    dm = PS.DeviceMesh(tile)
+    S = PS.PoissonSolver(dm, PS.PoissonOptions(precision=precision))
+    n = NX * NY * NZ
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(7)
+    b = torch.rand(n, generator=gen, dtype=torch.float64, device="cuda") - 0.5
+    x = torch.zeros_like(b)
+    bc = (1, 1, 1, 2, 1, 2)   # neumann sides and ground, open x-hi and top
+    S.vcycle(b, x, None, bc, setup=True)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        S.vcycle(b, x, None, bc, setup=False)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps, S.levels
+
+
+def _vcycle_bytes(s):
+    """Algorithmic bytes of one V(2,2) with x/y semi-coarsening (levels
+    Nc/4^l): (4/3) Nc [(nu1 + nu2) 8 s + 8 s + 3.5 s]."""
+    return 4.0 / 3.0 * NX * NY * NZ * ((2 + 2) * 8 * s + 8 * s + 3.5 * s)
+
+
 def _traffic_from_profile():
     p = os.path.join(ROOT, "profiles", "zjacobi_sweep_c3_fp64_pmc.json")
     if os.path.exists(p):
@@ -153,6 +196,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--profile-only", action="store_true", help="sweeps only (for rocprofv3 runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only a plain 1 GiB device copy, repeated (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (a tile is solved on one device); "
@@ -163,6 +208,15 @@ def main():
     import ablsteadyrans_amd as G
 
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(max(1, args.steps)):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8,
+                          "launches": max(1, args.steps)}))
+        return
     sys64 = _systems(torch.float64, 20264203)
     plan64 = G.JacobiPlan(sys64)
     unknowns = plan64.unknowns
@@ -172,10 +226,17 @@ def main():
         sys32 = _systems(torch.float32, 20264203)
         plan32 = G.JacobiPlan(sys32)
         _time_sweeps(plan32, args.steps, args.warmup)
-        print(json.dumps({"profile_only": True, "ms_per_sweep_fp64": ms_ev}))
+        del plan64, plan32, sys64, sys32
+        tile = _fractal_tile()
+        v64, _ = _time_vcycle(tile, 0, 3)
+        v32, _ = _time_vcycle(tile, 1, 3)
+        print(json.dumps({"profile_only": True, "ms_per_sweep_fp64": ms_ev, "vcycle_ms": [v64, v32]}))
         return
     run64 = _time_run(plan64, 2, 20)
     del plan64
+    tile = _fractal_tile()
+    vc64, levels = _time_vcycle(tile, 0, 10)
+    vc32, _ = _time_vcycle(tile, 1, 10)
     sys32 = _systems(torch.float32, 20264203)
     plan32 = G.JacobiPlan(sys32)
     ms32, _ = _time_sweeps(plan32, args.steps, args.warmup)
@@ -209,6 +270,9 @@ def main():
         "extra": {"fp32_sweep_gbs": round(gbs32, 2), "fp32_ms_per_sweep": round(ms32, 5),
                   "fp32_frac": round(gbs32 / HBM_PEAK_GBS, 4),
                   "fp64_line_solve_2_sweeps_ms": round(run64, 4), "fp32_line_solve_2_sweeps_ms": round(run32, 4),
+                  "vcycle_levels": levels,
+                  "fp64_vcycle_ms": round(vc64, 4), "fp64_vcycle_gbs": round(_vcycle_bytes(8) / (vc64 * 1e-3) / 1e9, 1),
+                  "fp32_vcycle_ms": round(vc32, 4), "fp32_vcycle_gbs": round(_vcycle_bytes(4) / (vc32 * 1e-3) / 1e9, 1),
                   "note": "steady solves/s and p99 need the full SIMPLEC iteration (not yet on this path)"},
     }
     print(json.dumps(line))

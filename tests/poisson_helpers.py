@@ -53,35 +53,46 @@ def apply(m, phi_valid, bc, coeffs=None, dcross=None):
 
 
 def sparse_operator(m, bc, coeffs=None, dcross=None):
+    """The oracle operator as a sparse matrix, by probing with coloured unit
+    vectors: colour i mod 3 along an axis (the stencil reaches one cell), or
+    the index itself along a periodic axis whose length is not a multiple of
+    3 (where i mod 3 would alias across the seam)."""
     nx, ny, nz = m.geom.n
+    if nx < 3 or ny < 3:
+        raise ValueError("probing needs nx, ny >= 3")
     n = nx * ny * nz
+    per = m.geom.periodic
+    cx = nx if (per[0] and nx % 3) else 3
+    cy = ny if (per[1] and ny % 3) else 3
     rows, cols, vals = [], [], []
     K, Jj, I = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
-    for c in range(27):
-        ci, cj, ck = c % 3, (c // 3) % 3, c // 9
-        probe = ((I % 3 == ci) & (Jj % 3 == cj) & (K % 3 == ck)).astype(np.float64)
-        y = apply(m, probe, bc, coeffs, dcross)
-        # each row p sees at most one probed q within +-1: find it
-        for dk in (-1, 0, 1):
-            for dj in (-1, 0, 1):
-                for di in (-1, 0, 1):
-                    qi, qj, qk = I + di, Jj + dj, K + dk
-                    if m.geom.periodic[0]:
-                        qi = qi % nx
-                    if m.geom.periodic[1]:
-                        qj = qj % ny
-                    ok = (qi >= 0) & (qi < nx) & (qj >= 0) & (qj < ny) & (qk >= 0) & (qk < nz)
-                    sel = ok & ((qi % 3) == ci) & ((qj % 3) == cj) & ((qk % 3) == ck)
-                    if nx < 3 or ny < 3:
-                        raise ValueError("probing needs nx, ny >= 3")
-                    p = (K * ny + Jj) * nx + I
-                    q = (qk * ny + qj) * nx + qi
-                    rows.append(p[sel])
-                    cols.append(q[sel])
-                    vals.append(y[sel])
-    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
-    A.sum_duplicates()
-    return A
+    p = (K * ny + Jj) * nx + I
+    for ck in range(3):
+        for cj in range(cy):
+            for ci in range(cx):
+                probe = ((I % cx == ci) & (Jj % cy == cj) & (K % 3 == ck)).astype(np.float64)
+                y = apply(m, probe, bc, coeffs, dcross)
+                for dk in (-1, 0, 1):
+                    for dj in (-1, 0, 1):
+                        for di in (-1, 0, 1):
+                            qi, qj, qk = I + di, Jj + dj, K + dk
+                            if per[0]:
+                                qi = qi % nx
+                            if per[1]:
+                                qj = qj % ny
+                            ok = (qi >= 0) & (qi < nx) & (qj >= 0) & (qj < ny) & (qk >= 0) & (qk < nz)
+                            sel = ok & ((qi % cx) == ci) & ((qj % cy) == cj) & ((qk % 3) == ck)
+                            q = (qk * ny + qj) * nx + qi
+                            rows.append(p[sel])
+                            cols.append(q[sel])
+                            vals.append(y[sel])
+    # a (p, q) pair can be listed twice when two offsets wrap to one cell
+    r = np.concatenate(rows)
+    c = np.concatenate(cols)
+    v = np.concatenate(vals)
+    key = r.astype(np.int64) * n + c
+    _, first = np.unique(key, return_index=True)
+    return sp.csr_matrix((v[first], (r[first], c[first])), shape=(n, n))
 
 
 def askervein_problem(n_xy=30):
