@@ -157,6 +157,13 @@ _sig = {
                                       C.c_void_p]),
     "gust_face_divergence": (C.c_int, [_P(GeometryC), _P(Array4), _P(Array4), _P(Array4), _P(Array4), _P(Array4),
                                        _P(Array4), _P(Array4), _P(Array4), C.c_void_p]),
+    "gust_omega_from_w": (C.c_int, [_P(GeometryC), _P(Array4), _P(Array4), _P(Array4), _P(Array4), _P(Array4),
+                                    C.c_int32, C.c_void_p]),
+    "gust_w_from_omega": (C.c_int, [_P(GeometryC), _P(Array4), _P(Array4), _P(Array4), _P(Array4), _P(Array4),
+                                    C.c_void_p]),
+    "gust_transport_fluxes": (C.c_int, [_P(GeometryC)] + [_P(Array4)] * 9 + [C.c_void_p]),
+    "gust_velocity_divergence": (C.c_int, [_P(GeometryC)] + [_P(Array4)] * 9 + [C.c_void_p]),
+    "gust_pressure_gradient": (C.c_int, [_P(GeometryC)] + [_P(Array4)] * 8 + [C.c_void_p]),
 }
 for _name, (_res, _args) in _sig.items():
     _f = getattr(lib, _name)

@@ -33,6 +33,31 @@ const char* loc_words(int loc)
     }
 }
 
+void check_field(const char* op, const char* what, const gust_array4* v, const gust_geometry& g, int loc,
+                 int ngrow)
+{
+    const IBox vb = loc_box(g, loc);
+    if (!covers(v, vb, 1)) {
+        GUST_FAIL("gust: %s: %s is not on the mesh's BoxArray and DistributionMapping at the %s", op, what,
+                  loc_words(loc));
+    }
+    if (!covers(v, grow(vb, ngrow, ngrow, ngrow), 1)) {
+        GUST_FAIL("gust: %s: %s needs at least %d ghost cells", op, what, ngrow);
+    }
+}
+
+void check_nodal_heights(const char* op, const gust_array4* z, const gust_geometry& g)
+{
+    IBox nodes;
+    for (int d = 0; d < 3; ++d) {
+        nodes.lo[d] = g.lo[d] - 1;
+        nodes.hi[d] = g.hi[d] + 2;
+    }
+    if (!covers(z, nodes, 1)) {
+        GUST_FAIL("gust: %s: z_nd must be the mesh's node heights (nodal, same layout, one ghost node)", op);
+    }
+}
+
 namespace {
 
 struct Buf {

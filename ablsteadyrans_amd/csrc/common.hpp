@@ -114,6 +114,31 @@ inline int64_t offset_of(const gust_array4& v, int i, int j, int k, int n = 0)
            int64_t(n) * v.nstride;
 }
 
+// A view as kernels read it: component 0 at (i, j, k).
+template <typename T>
+struct DView {
+    T* p;
+    int64_t js, ks;
+    int b0, b1, b2;
+    __host__ __device__ __forceinline__ T& operator()(int i, int j, int k) const
+    {
+        return p[int64_t(i - b0) + int64_t(j - b1) * js + int64_t(k - b2) * ks];
+    }
+};
+
+template <typename T>
+DView<T> dview(const gust_array4* v)
+{
+    return DView<T>{static_cast<T*>(v->p), v->jstride, v->kstride, v->begin[0], v->begin[1], v->begin[2]};
+}
+
+// check_operator_field (Source/CVFluxes.cpp:95-110) on a view: the field at
+// loc must cover the domain's box at loc grown by ngrow.
+void check_field(const char* op, const char* what, const gust_array4* v, const gust_geometry& g, int loc,
+                 int ngrow);
+// check_nodal_heights (Source/CVFluxes.cpp:112-119).
+void check_nodal_heights(const char* op, const gust_array4* z, const gust_geometry& g);
+
 // ---------------------------------------------------------------------------
 // Workspace: grow-only device buffers keyed by (device, stream, slot). A
 // buffer is reused by later calls on the same stream (stream order keeps
@@ -163,6 +188,30 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
         v = o > v ? o : v;
     }
     return v;
+}
+
+// Global-memory accesses through address-space-1 pointers: where a pointer
+// comes out of a table in memory the compiler cannot infer its space and
+// would emit flat instructions (which also count against the LDS counter and
+// serialise the waits).
+#define GUST_GLOBAL __attribute__((address_space(1)))
+
+template <typename T>
+__device__ __forceinline__ T ldg(const T* p)
+{
+    return __builtin_nontemporal_load((const GUST_GLOBAL T*)(p));
+}
+
+template <typename T>
+__device__ __forceinline__ T gld(const T* p)
+{
+    return *((const GUST_GLOBAL T*)(p));
+}
+
+template <typename T>
+__device__ __forceinline__ void gst(T* p, T v)
+{
+    *((GUST_GLOBAL T*)(p)) = v;
 }
 
 }   // namespace gust_amd::dev

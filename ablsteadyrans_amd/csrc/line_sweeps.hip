@@ -57,29 +57,9 @@ struct JacobiSys {
     int pad;
 };
 
-// Global-memory accesses through address-space-1 pointers: the pointers
-// come out of a table in memory, so the compiler cannot infer their space
-// and would emit flat instructions (which also count against the LDS
-// counter and serialise the waits).
-#define GUST_GLOBAL __attribute__((address_space(1)))
-
-template <typename T>
-__device__ __forceinline__ T ldg(const T* p)
-{
-    return __builtin_nontemporal_load((const GUST_GLOBAL T*)(p));
-}
-
-template <typename T>
-__device__ __forceinline__ T gld(const T* p)
-{
-    return *((const GUST_GLOBAL T*)(p));
-}
-
-template <typename T>
-__device__ __forceinline__ void gst(T* p, T v)
-{
-    *((GUST_GLOBAL T*)(p)) = v;
-}
+using dev::gld;
+using dev::gst;
+using dev::ldg;
 
 // Which system holds column tile t (tile0 ascending).
 template <typename S>

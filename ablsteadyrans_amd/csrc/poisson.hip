@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -102,27 +103,123 @@ struct MirrorAcc {
     }
 };
 
-// The own column (ci, cj) seeded at levels ck-1, ck, ck+1 (mirrors resolved
-// as MirrorAcc does), every other point zero.
+// The 15 points of the operator's footprint around (ci, cj, ck) held in
+// registers: columns own, x-, x+, y-, y+ at levels ck-1, ck, ck+1. The
+// stencil reads it with offsets known at compile time once inlined, so
+// every read is a register.
 template <typename R>
-struct SeedAcc {
-    MirrorAcc<R> m;
+struct Patch {
+    R v[5][3];
     int ci, cj, ck;
-    __device__ __forceinline__ Lin3<R> operator()(int i, int j, int k) const
+    __device__ __forceinline__ R operator()(int i, int j, int k) const
     {
-        R s;
-        m.resolve(i, j, k, s);
-        Lin3<R> v;
-        if (i == ci && j == cj) {
-            const int d = k - ck;
-            if (d >= -1 && d <= 1) { v.c[d + 1] = s; }
-        }
-        return v;
+        const int di = i - ci, dj = j - cj;
+        const int col = di == -1 ? 1 : di == 1 ? 2 : dj == -1 ? 3 : dj == 1 ? 4 : 0;
+        return v[col][k - ck + 1];
     }
 };
 
+__device__ __forceinline__ void patch_point(int col, int d, int i, int j, int k, int& a, int& b, int& c)
+{
+    a = i + (col == 1 ? -1 : col == 2 ? 1 : 0);
+    b = j + (col == 3 ? -1 : col == 4 ? 1 : 0);
+    c = k + d;
+}
+
+template <typename R, class X>
+__device__ __forceinline__ Patch<R> load_patch(const X& x, int i, int j, int k)
+{
+    Patch<R> p;
+    p.ci = i;
+    p.cj = j;
+    p.ck = k;
+#pragma unroll
+    for (int col = 0; col < 5; ++col)
+#pragma unroll
+        for (int d = -1; d <= 1; ++d) {
+            int a, b, c;
+            patch_point(col, d, i, j, k, a, b, c);
+            p.v[col][d + 1] = x(a, b, c);
+        }
+    return p;
+}
+
 template <typename R>
-using Ctx = OpCtx<R, NodeAcc<R>, GridAcc<R>>;
+using Ctx = OpCtx<R, NodeAcc<R>, GridAcc<R>, stencil::ArrayMetric<R>>;
+
+// Split ("red-black") layout of a multigrid level: the cells of colour
+// c = (i + j) % 2 are stored apart, each colour as (H, ny, nz) with
+// H = ceil(nx / 2), so a half sweep of one colour reads and writes
+// contiguous memory. Cell (i, j, k) is slot c * plane + i / 2 + H * (j + ny * k);
+// when nx is odd, one slot per row of one colour is padding.
+struct Split {
+    int nx, ny, nz, H;
+    int64_t plane;   // H * ny * nz
+    __host__ __device__ __forceinline__ int64_t at(int i, int j, int k) const
+    {
+        return int64_t((i + j) & 1) * plane + (i >> 1) + int64_t(H) * (j + int64_t(ny) * k);
+    }
+    // The cell of slot t (t < 2 * plane); false for a padding slot.
+    __host__ __device__ __forceinline__ bool cell(int64_t t, int& i, int& j, int& k) const
+    {
+        const int c = t >= plane ? 1 : 0;
+        const int64_t r = t - c * plane;
+        const int ih = int(r % H);
+        const int64_t q = r / H;
+        j = int(q % ny);
+        k = int(q / ny);
+        i = 2 * ih + ((j + c) & 1);
+        return i < nx;
+    }
+};
+
+// A split-layout field read with the boundary conditions (as MirrorAcc).
+template <typename R>
+struct SplitMirror {
+    MirrorAcc<R> m;   // resolve only
+    Split s;
+    const R* p;
+    __device__ __forceinline__ R operator()(int i, int j, int k) const
+    {
+        R sg;
+        m.resolve(i, j, k, sg);
+        return sg * p[s.at(i, j, k)];
+    }
+};
+
+// The horizontal neighbour columns (x-, x+, y-, y+) of column (i, j) as the
+// operator couples them: periodic images wrapped; a column outside a
+// non-periodic side is absent (its mirror folds into the own column); a
+// column met twice (a periodic direction of one or two cells) is kept once,
+// at its first appearance, the own column counting first.
+__device__ __forceinline__ void nb_columns(int i, int j, int nx, int ny, int perx, int pery, int (&ni)[4],
+                                           int (&nj)[4], bool (&ok)[4])
+{
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        int a = i + (p == 0 ? -1 : p == 1 ? 1 : 0);
+        int b = j + (p == 2 ? -1 : p == 3 ? 1 : 0);
+        bool v = true;
+        if (a < 0) { if (perx) { a += nx; } else { v = false; a = i; } }
+        else if (a >= nx) { if (perx) { a -= nx; } else { v = false; a = i; } }
+        if (b < 0) { if (pery) { b += ny; } else { v = false; b = j; } }
+        else if (b >= ny) { if (pery) { b -= ny; } else { v = false; b = j; } }
+        if (a == i && b == j) { v = false; }
+#pragma unroll
+        for (int q = 0; q < p; ++q) {
+            if (ok[q] && ni[q] == a && nj[q] == b) { v = false; }
+        }
+        ni[p] = a;
+        nj[p] = b;
+        ok[p] = v;
+    }
+}
+
+// A level's explicit stencil, one bundle of kBundle reals per split slot:
+// the own column's lower entry, 1 / Thomas pivot, (k-1, k, k+1) of the x-,
+// x+, y-, y+ neighbour columns, the own diagonal and upper entries. The
+// Thomas c' has a plane of its own (read by the back substitution alone).
+enum : int { kBLo = 0, kBMinv = 1, kBNb = 2, kBDg = 14, kBUp = 15, kBundle = 16 };
 
 // ---------------------------------------------------------------------------
 // Kernels.
@@ -135,100 +232,496 @@ __device__ __forceinline__ void cell_of(int64_t t, int nx, int ny, int& i, int& 
     k = int(r / ny);
 }
 
-// y = A x (mode 0) or y = b - A x (mode 1), on every cell (color < 0) or on
-// the cells with (i + j) % 2 == color.
+// y = A x (mode 0) or y = b - A x (mode 1) on the operator level.
 template <typename R, int kMode>
 __global__ __launch_bounds__(kThreads) void apply_kernel(Ctx<R> c, MirrorAcc<R> x, const R* __restrict__ b,
-                                                          R* __restrict__ y, int color)
-{
-    const int nxh = color < 0 ? c.nx : (c.nx + 1) / 2;
-    const int64_t n = int64_t(nxh) * c.ny * c.nz;
-    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
-        int i, j, k;
-        cell_of(t, nxh, c.ny, i, j, k);
-        if (color >= 0) {
-            i = 2 * i + ((j + color) & 1);
-            if (i >= c.nx) { continue; }
-        }
-        const R v = stencil::apply_cell<R, R>(x, c, i, j, k);
-        const int64_t q = i + j * int64_t(c.nx) + int64_t(k) * c.nx * c.ny;
-        y[q] = kMode == 0 ? v : b[q] - v;
-    }
-}
-
-// The column part of the operator: (lower, diagonal, upper) per cell.
-template <typename R>
-__global__ __launch_bounds__(kThreads) void tridiag_kernel(Ctx<R> c, MirrorAcc<R> m, R* __restrict__ lo,
-                                                           R* __restrict__ dg, R* __restrict__ up)
+                                                          R* __restrict__ y)
 {
     const int64_t n = int64_t(c.nx) * c.ny * c.nz;
     for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
         int i, j, k;
         cell_of(t, c.nx, c.ny, i, j, k);
-        SeedAcc<R> s{m, i, j, k};
-        const Lin3<R> v = stencil::apply_cell<Lin3<R>, R>(s, c, i, j, k);
-        lo[t] = v.c[0];
-        dg[t] = v.c[1];
-        up[t] = v.c[2];
+        const Patch<R> px = load_patch<R>(x, i, j, k);
+        const R v = stencil::apply_cell<R, R>(px, c, i, j, k);
+        y[t] = kMode == 0 ? v : b[t] - v;
     }
 }
 
-// One colour of a zebra sweep, after apply_kernel<1> put b - A x on that
-// colour's cells into r: solve the column tridiagonal T d = r and x += d.
-// r is overwritten with the Thomas e', cs holds c'.
+// The mesh factors of a level (stencil::ArrayMetric) from its node heights.
 template <typename R>
-__global__ __launch_bounds__(kThreads) void line_solve_kernel(int nx, int ny, int nz, const R* __restrict__ lo,
-                                                              const R* __restrict__ dg, const R* __restrict__ up,
-                                                              R* __restrict__ r, R* __restrict__ cs,
-                                                              R* __restrict__ x, int color)
+__global__ void metric_kernel(stencil::ZMetric<R, NodeAcc<R>> zm, int nx, int ny, int nz, R* __restrict__ ry,
+                              R* __restrict__ rx, R* __restrict__ zf)
 {
-    const int nxh = (nx + 1) / 2;
-    const int64_t ncol = int64_t(nxh) * ny;
-    const int64_t sk = int64_t(nx) * ny;
-    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < ncol; t += int64_t(gridDim.x) * blockDim.x) {
-        const int j = int(t / nxh);
-        const int i = 2 * int(t % nxh) + ((j + color) & 1);
-        if (i >= nx) { continue; }
-        const int64_t q0 = i + int64_t(j) * nx;
-        R cp = R(0), ep = R(0);
-        for (int k = 0; k < nz; ++k) {
-            const int64_t q = q0 + k * sk;
-            const R a = k > 0 ? lo[q] : R(0);
-            const R m = dg[q] - a * cp;
-            const R cc = up[q] / m;
-            const R ee = (r[q] - a * ep) / m;
-            cs[q] = cc;
-            r[q] = ee;
-            cp = cc;
-            ep = ee;
-        }
-        R d = ep;
-        x[q0 + (nz - 1) * sk] += d;
-        for (int k = nz - 2; k >= 0; --k) {
-            const int64_t q = q0 + k * sk;
-            d = r[q] - cs[q] * d;
-            x[q] += d;
-        }
-    }
-}
-
-// b_c = sum J_f r_f / sum J_f over the fine cells of each coarse cell.
-template <typename R>
-__global__ __launch_bounds__(kThreads) void restrict_kernel(int nxc, int nyc, int nz, int rx, int ry,
-                                                            const R* __restrict__ rf, const R* __restrict__ Jf,
-                                                            R* __restrict__ bc)
-{
-    const int nxf = nxc * rx, nyf = nyc * ry;
-    const int64_t n = int64_t(nxc) * nyc * nz;
+    const int64_t n_ry = int64_t(nx + 1) * ny * (nz + 1), n_rx = int64_t(nx) * (ny + 1) * (nz + 1),
+                  n_zf = int64_t(nx) * ny * (nz + 1);
+    const int64_t n = n_ry + n_rx + n_zf;
     for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        int i, j, k;
+        if (t < n_ry) {
+            cell_of(t, nx + 1, ny, i, j, k);
+            ry[t] = zm.ry(i, j, k);
+        } else if (t < n_ry + n_rx) {
+            const int64_t u = t - n_ry;
+            cell_of(u, nx, ny + 1, i, j, k);
+            rx[u] = zm.rx(i, j, k);
+        } else {
+            const int64_t u = t - n_ry - n_rx;
+            cell_of(u, nx, ny, i, j, k);
+            zf[u] = zm.hzi(i, j, k);
+            zf[u + n_zf] = zm.hxz(i, j, k);
+            zf[u + 2 * n_zf] = zm.hez(i, j, k);
+        }
+    }
+}
+
+// The explicit 15-point stencil of a level: each coefficient is the
+// operator applied to a probe that is 1 at one point (column, level) and 0
+// elsewhere, with the boundary mirrors resolved exactly as the operator
+// reads them (a mirrored point carries its sign to the point it images).
+// Writes the bundle's lo, dg, up and neighbour entries.
+template <typename R>
+__global__ __launch_bounds__(kThreads) void coef_seed_kernel(Ctx<R> c, MirrorAcc<R> m, Split s, int perx, int pery,
+                                                             R* __restrict__ bundle)
+{
+    const int64_t n = 2 * s.plane;
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        int i, j, k;
+        if (!s.cell(t, i, j, k)) { continue; }
+        int ni[4], nj[4];
+        bool ok[4];
+        nb_columns(i, j, c.nx, c.ny, perx, pery, ni, nj, ok);
+        // the resolved footprint
+        int ri[5][3], rj[5][3], rk[5][3];
+        R sg[5][3];
+#pragma unroll
+        for (int col = 0; col < 5; ++col)
+#pragma unroll
+            for (int d = -1; d <= 1; ++d) {
+                int a, b, cc;
+                patch_point(col, d, i, j, k, a, b, cc);
+                R sv;
+                m.resolve(a, b, cc, sv);
+                ri[col][d + 1] = a;
+                rj[col][d + 1] = b;
+                rk[col][d + 1] = cc;
+                sg[col][d + 1] = sv;
+            }
+#pragma unroll 1
+        for (int tp = 0; tp < 15; ++tp) {
+            const int p = tp / 3, d = tp % 3 - 1;
+            const int ti = p == 0 ? i : ni[p - 1], tj = p == 0 ? j : nj[p - 1];
+            R v = R(0);
+            if (p == 0 || ok[p - 1]) {
+                Patch<R> pt;
+                pt.ci = i;
+                pt.cj = j;
+                pt.ck = k;
+#pragma unroll
+                for (int col = 0; col < 5; ++col)
+#pragma unroll
+                    for (int e = 0; e < 3; ++e) {
+                        const bool hit = ri[col][e] == ti && rj[col][e] == tj && rk[col][e] == k + d;
+                        pt.v[col][e] = hit ? sg[col][e] : R(0);
+                    }
+                v = stencil::apply_cell<R, R>(pt, c, i, j, k);
+            }
+            const int e = p == 0 ? (d == -1 ? kBLo : d == 0 ? kBDg : kBUp) : kBNb + 3 * (p - 1) + d + 1;
+            bundle[t * kBundle + e] = v;
+        }
+    }
+}
+
+// Thomas factors of every column: 1 / pivot (into the bundle) and c' (its
+// own plane), the part of the elimination that does not depend on the
+// right-hand side.
+template <typename R>
+__global__ void coef_factor_kernel(Split s, R* __restrict__ bundle, R* __restrict__ cpl)
+{
+    const int64_t rowc = int64_t(s.H) * s.ny, ncol = 2 * rowc;
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < ncol; t += int64_t(gridDim.x) * blockDim.x) {
+        const int c = t >= rowc ? 1 : 0;
+        const int64_t r = t - c * rowc;
+        const int ih = int(r % s.H), j = int(r / s.H);
+        if (2 * ih + ((j + c) & 1) >= s.nx) { continue; }
+        const int64_t q0 = c * s.plane + r;
+        R cprev = R(0);
+        for (int k = 0; k < s.nz; ++k) {
+            const int64_t q = q0 + k * rowc;
+            R* bq = bundle + q * kBundle;
+            const R lo = bq[kBLo], dg = bq[kBDg], up = bq[kBUp];
+            const R minv = R(1) / (k == 0 ? dg : dg - lo * cprev);
+            const R cp = up * minv;
+            bq[kBMinv] = minv;
+            cpl[q] = cp;
+            cprev = cp;
+        }
+    }
+}
+
+// A cell's bundle in 16-byte loads.
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void load_bundle(float (&c)[kBundle], const float* p)
+{
+    const v4f* v = reinterpret_cast<const v4f*>(p);
+#pragma unroll
+    for (int m = 0; m < kBundle / 4; ++m) {
+        const v4f w = dev::gld(v + m);
+        c[4 * m] = w.x;
+        c[4 * m + 1] = w.y;
+        c[4 * m + 2] = w.z;
+        c[4 * m + 3] = w.w;
+    }
+}
+
+__device__ __forceinline__ void load_bundle(double (&c)[kBundle], const double* p)
+{
+    const v2d* v = reinterpret_cast<const v2d*>(p);
+#pragma unroll
+    for (int m = 0; m < kBundle / 2; ++m) {
+        const v2d w = dev::gld(v + m);
+        c[2 * m] = w.x;
+        c[2 * m + 1] = w.y;
+    }
+}
+
+// The neighbour-column terms of a cell: sum over the four columns of the
+// bundle's (k-1, k, k+1) entries times x there.
+template <typename R>
+__device__ __forceinline__ R nb_terms(const R (&c)[kBundle], const R (&xm)[4], const R (&x0)[4], const R (&xp)[4])
+{
+    R v = R(0);
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        v = v + (c[kBNb + 3 * p] * xm[p] + c[kBNb + 3 * p + 1] * x0[p] + c[kBNb + 3 * p + 2] * xp[p]);
+    }
+    return v;
+}
+
+// The operands of one level of a column's forward elimination.
+template <typename R>
+struct ZLevel {
+    R c[kBundle];
+    R bb;
+    R xn[4];   // the neighbour columns at the next level up
+};
+
+template <typename R, bool kNb>
+__device__ __forceinline__ void zload(ZLevel<R>& o, const R* __restrict__ bundle, int64_t q,
+                                      const R* __restrict__ rhs, const R* x, const int64_t (&nb)[4],
+                                      const bool (&ok)[4], int64_t xk)
+{
+    using dev::gld;
+    load_bundle(o.c, bundle + q * kBundle);
+    o.bb = gld(rhs + q);
+    if (kNb) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const R v = gld(x + nb[p] + xk);
+            o.xn[p] = ok[p] ? v : R(0);
+        }
+    }
+}
+
+// One colour of a zebra sweep as one pass per column: the right-hand side
+// b - (neighbour columns) x, the forward elimination with the stored
+// factors, the back substitution into x. The neighbours of a column all
+// have the other colour, so the columns of one launch are independent.
+//   kZero:  x is zero on entry (the first half sweep): no neighbour terms.
+//   kFromR: the right-hand side was formed beforehand (rhs = r), for levels
+//           where a periodic seam joins two columns of one colour.
+//   kLds:   e' stays in LDS (64 x nz reals); else in eg (split layout).
+//   kP:     levels of operands prefetched ahead (a register ring indexed at
+//           compile time); kP x (loads per level) stays under the 63
+//           vector memory operations a wave can have outstanding.
+// One wave per 64 columns of the colour; each lane walks its column.
+template <typename R, bool kZero, bool kFromR, bool kLds, int kP>
+__global__ __launch_bounds__(64) void zebra_kernel(Split s, int color, int perx, int pery,
+                                                   const R* __restrict__ bundle, const R* __restrict__ cpl,
+                                                   const R* __restrict__ rhs, R* __restrict__ x, R* __restrict__ eg)
+{
+    using dev::gld;
+    using dev::gst;
+    extern __shared__ unsigned char lds_raw[];
+    R* el = reinterpret_cast<R*>(lds_raw);
+    constexpr bool kNb = !kZero && !kFromR;
+    const int lane = threadIdx.x;
+    const int64_t rowc = int64_t(s.H) * s.ny;
+    const int64_t t = int64_t(dev::xcd_remap(blockIdx.x, gridDim.x)) * 64 + lane;
+    if (t >= rowc) { return; }
+    const int ih = int(t % s.H), j = int(t / s.H);
+    const int i = 2 * ih + ((j + color) & 1);
+    if (i >= s.nx) { return; }
+    const int nz = s.nz;
+    const int64_t q0 = color * s.plane + t;
+    int64_t nb[4] = {0, 0, 0, 0};
+    bool ok[4] = {false, false, false, false};
+    R xm1[4] = {R(0), R(0), R(0), R(0)}, x0[4] = {R(0), R(0), R(0), R(0)};
+    if (kNb) {
+        int ni[4], nj[4];
+        nb_columns(i, j, s.nx, s.ny, perx, pery, ni, nj, ok);
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            nb[p] = s.at(ni[p], nj[p], 0);
+            const R v = gld(x + nb[p]);
+            x0[p] = ok[p] ? v : R(0);
+        }
+    }
+    ZLevel<R> ring[kP];
+#pragma unroll
+    for (int u = 0; u < kP; ++u) {
+        const int k = u < nz ? u : nz - 1;
+        const int kx = k + 1 < nz ? k + 1 : nz - 1;
+        zload<R, kNb>(ring[u], bundle, q0 + k * rowc, rhs, x, nb, ok, kx * rowc);
+    }
+    R eprev = R(0);
+    auto eliminate = [&](const ZLevel<R>& o, int k) {
+        R r = o.bb;
+        if (kNb) {
+            r = r - nb_terms(o.c, xm1, x0, o.xn);
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                xm1[p] = x0[p];
+                x0[p] = o.xn[p];
+            }
+        }
+        const R e = (r - o.c[kBLo] * eprev) * o.c[kBMinv];
+        if (kLds) { el[k * 64 + lane] = e; } else { gst(eg + q0 + k * rowc, e); }
+        eprev = e;
+    };
+    const int nfull = (nz / kP) * kP;
+    for (int k0 = 0; k0 < nfull; k0 += kP) {
+#pragma unroll
+        for (int u = 0; u < kP; ++u) {
+            const int k = k0 + u;
+            const ZLevel<R> o = ring[u];
+            const int kn = k + kP < nz ? k + kP : nz - 1;
+            const int kx = kn + 1 < nz ? kn + 1 : nz - 1;
+            zload<R, kNb>(ring[u], bundle, q0 + kn * rowc, rhs, x, nb, ok, kx * rowc);
+            eliminate(o, k);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kP; ++u) {
+        if (nfull + u < nz) { eliminate(ring[u], nfull + u); }
+    }
+    // Back substitution: x_k = e'_k - c'_k x_{k+1}.
+    constexpr int kB = 8;
+    R xo = eprev;
+    gst(x + q0 + (nz - 1) * rowc, xo);
+    const int nbk = nz - 1;
+    R cb[kB], eb[kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+        const int k = nz - 2 - u >= 0 ? nz - 2 - u : 0;
+        cb[u] = gld(cpl + q0 + k * rowc);
+        eb[u] = kLds ? R(0) : gld(eg + q0 + k * rowc);
+    }
+    auto substitute = [&](R ck, R ek, int k) {
+        const R e = kLds ? el[k * 64 + lane] : ek;
+        xo = e - ck * xo;
+        gst(x + q0 + k * rowc, xo);
+    };
+    const int nbfull = nbk > 0 ? (nbk / kB) * kB : 0;
+    for (int m0 = 0; m0 < nbfull; m0 += kB) {
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            const int k = nz - 2 - (m0 + u);
+            const R ck = cb[u], ek = eb[u];
+            const int kn = k - kB >= 0 ? k - kB : 0;
+            cb[u] = gld(cpl + q0 + kn * rowc);
+            eb[u] = kLds ? R(0) : gld(eg + q0 + kn * rowc);
+            substitute(ck, ek, k);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+        if (nbfull + u < nbk) { substitute(cb[u], eb[u], nz - 2 - (nbfull + u)); }
+    }
+}
+
+// The same half sweep with one wave per column, for levels with too few
+// columns to fill the chip one lane per column. Lane L holds levels
+// L*M .. L*M+M-1; both Thomas recurrences, e'_k = (r_k - lo_k e'_{k-1}) / m_k
+// and x_k = e'_k - c'_k x_{k+1}, are affine in the carried value, so each
+// lane composes its levels' maps, a shuffle scan across the wave composes
+// the lanes', and each lane then replays its levels from its carry-in.
+template <typename R, bool kZero, bool kFromR, int M>
+__global__ __launch_bounds__(256) void zebra_scan_kernel(Split s, int color, int perx, int pery,
+                                                         const R* __restrict__ bundle, const R* __restrict__ cpl,
+                                                         const R* __restrict__ rhs, R* __restrict__ x)
+{
+    using dev::gld;
+    using dev::gst;
+    constexpr bool kNb = !kZero && !kFromR;
+    const int lane = threadIdx.x & 63;
+    const int64_t rowc = int64_t(s.H) * s.ny;
+    const int64_t t = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    if (t >= rowc) { return; }
+    const int ih = int(t % s.H), j = int(t / s.H);
+    const int i = 2 * ih + ((j + color) & 1);
+    if (i >= s.nx) { return; }   // uniform over the wave
+    const int nz = s.nz;
+    const int64_t q0 = color * s.plane + t;
+    int64_t nb[4] = {0, 0, 0, 0};
+    bool ok[4] = {false, false, false, false};
+    if (kNb) {
+        int ni[4], nj[4];
+        nb_columns(i, j, s.nx, s.ny, perx, pery, ni, nj, ok);
+#pragma unroll
+        for (int p = 0; p < 4; ++p) { nb[p] = s.at(ni[p], nj[p], 0); }
+    }
+    // Forward: e_k = bet_k + alp_k e_{k-1} on my levels.
+    R alp[M], bet[M], g[M];
+    R A = R(1), B = R(0);
+#pragma unroll
+    for (int u = 0; u < M; ++u) {
+        const int k = lane * M + u;
+        alp[u] = R(1);
+        bet[u] = R(0);
+        g[u] = R(1);
+        if (k < nz) {
+            const int64_t q = q0 + k * rowc;
+            R c[kBundle];
+            load_bundle(c, bundle + q * kBundle);
+            R r = gld(rhs + q);
+            if (kNb) {
+                const int km = k > 0 ? k - 1 : 0, kp = k + 1 < nz ? k + 1 : nz - 1;
+                R xm[4], x0[4], xp[4];
+#pragma unroll
+                for (int p = 0; p < 4; ++p) {
+                    const R a = gld(x + nb[p] + km * rowc), b0 = gld(x + nb[p] + k * rowc),
+                            cc = gld(x + nb[p] + kp * rowc);
+                    xm[p] = ok[p] ? a : R(0);
+                    x0[p] = ok[p] ? b0 : R(0);
+                    xp[p] = ok[p] ? cc : R(0);
+                }
+                r = r - nb_terms(c, xm, x0, xp);
+            }
+            alp[u] = -c[kBLo] * c[kBMinv];
+            bet[u] = r * c[kBMinv];
+            g[u] = -gld(cpl + q);
+        }
+        B = bet[u] + alp[u] * B;
+        A = alp[u] * A;
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const R Ao = __shfl_up(A, off, 64), Bo = __shfl_up(B, off, 64);
+        if (lane >= off) {
+            B = B + A * Bo;
+            A = A * Ao;
+        }
+    }
+    R e = __shfl_up(B, 1, 64);
+    if (lane == 0) { e = R(0); }
+    R ev[M];
+#pragma unroll
+    for (int u = 0; u < M; ++u) {
+        e = bet[u] + alp[u] * e;
+        ev[u] = e;
+    }
+    // Backward: x_k = e_k + g_k x_{k+1} on my levels, top down.
+    R C = R(1), D = R(0);
+#pragma unroll
+    for (int u = M - 1; u >= 0; --u) {
+        const int k = lane * M + u;
+        const R eu = k < nz ? ev[u] : R(0);
+        D = eu + g[u] * D;
+        C = g[u] * C;
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const R Co = __shfl_down(C, off, 64), Do = __shfl_down(D, off, 64);
+        if (lane + off < 64) {
+            D = D + C * Do;
+            C = C * Co;
+        }
+    }
+    R xv = __shfl_down(D, 1, 64);
+    if (lane == 63) { xv = R(0); }
+#pragma unroll
+    for (int u = M - 1; u >= 0; --u) {
+        const int k = lane * M + u;
+        if (k < nz) {
+            xv = ev[u] + g[u] * xv;
+            gst(x + q0 + k * rowc, xv);
+        }
+    }
+}
+
+// The neighbour-column part of one colour's right-hand side,
+// r = b - (neighbour columns) x, for the levels zebra_kernel<kFromR> serves.
+template <typename R>
+__global__ __launch_bounds__(kThreads) void offcol_kernel(Split s, int color, int perx, int pery,
+                                                          const R* __restrict__ bundle, const R* __restrict__ b,
+                                                          const R* __restrict__ x, R* __restrict__ r)
+{
+    const int64_t rowc = int64_t(s.H) * s.ny;
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < s.plane; t += int64_t(gridDim.x) * blockDim.x) {
+        const int ih = int(t % s.H);
+        const int64_t rr = t / s.H;
+        const int j = int(rr % s.ny), k = int(rr / s.ny);
+        const int i = 2 * ih + ((j + color) & 1);
+        if (i >= s.nx) { continue; }
+        int ni[4], nj[4];
+        bool ok[4];
+        nb_columns(i, j, s.nx, s.ny, perx, pery, ni, nj, ok);
+        const int64_t q = color * s.plane + t;
+        const int km = k > 0 ? k - 1 : 0, kp = k + 1 < s.nz ? k + 1 : s.nz - 1;
+        R c[kBundle];
+        load_bundle(c, bundle + q * kBundle);
+        R xm[4], x0[4], xp[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const int64_t c0 = s.at(ni[p], nj[p], 0);
+            xm[p] = ok[p] ? x[c0 + km * rowc] : R(0);
+            x0[p] = ok[p] ? x[c0 + k * rowc] : R(0);
+            xp[p] = ok[p] ? x[c0 + kp * rowc] : R(0);
+        }
+        r[q] = b[q] - nb_terms(c, xm, x0, xp);
+    }
+}
+
+// b_c = sum J_f r_f / sum J_f over the fine cells of each coarse cell, with
+// the fine residual r_f = b_f - A_f x_f formed on the fly.
+template <typename R>
+__global__ __launch_bounds__(kThreads) void resid_restrict_kernel(Split f, int perx, int pery,
+                                                                  const R* __restrict__ bundle,
+                                                                  const R* __restrict__ b, const R* __restrict__ x,
+                                                                  const R* __restrict__ Js, Split sc, int rx, int ry,
+                                                                  R* __restrict__ bc)
+{
+    const int64_t rowc = int64_t(f.H) * f.ny;
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < 2 * sc.plane;
+         t += int64_t(gridDim.x) * blockDim.x) {
         int I, Jc, k;
-        cell_of(t, nxc, nyc, I, Jc, k);
+        if (!sc.cell(t, I, Jc, k)) { continue; }
+        const int km = k > 0 ? k - 1 : 0, kp = k + 1 < f.nz ? k + 1 : f.nz - 1;
         R num = R(0), den = R(0);
-        for (int b = 0; b < ry; ++b)
+        for (int bb = 0; bb < ry; ++bb)
             for (int a = 0; a < rx; ++a) {
-                const int64_t q = (I * rx + a) + int64_t(Jc * ry + b) * nxf + int64_t(k) * nxf * nyf;
-                num += Jf[q] * rf[q];
-                den += Jf[q];
+                const int i = I * rx + a, j = Jc * ry + bb;
+                const int64_t q = f.at(i, j, k);
+                const int64_t c0 = f.at(i, j, 0);
+                R c[kBundle];
+                load_bundle(c, bundle + q * kBundle);
+                R v = b[q] - (c[kBLo] * x[c0 + km * rowc] + c[kBDg] * x[c0 + k * rowc] + c[kBUp] * x[c0 + kp * rowc]);
+                int ni[4], nj[4];
+                bool ok[4];
+                nb_columns(i, j, f.nx, f.ny, perx, pery, ni, nj, ok);
+                R xm[4], x0[4], xp[4];
+#pragma unroll
+                for (int p = 0; p < 4; ++p) {
+                    const int64_t n0 = f.at(ni[p], nj[p], 0);
+                    xm[p] = ok[p] ? x[n0 + km * rowc] : R(0);
+                    x0[p] = ok[p] ? x[n0 + k * rowc] : R(0);
+                    xp[p] = ok[p] ? x[n0 + kp * rowc] : R(0);
+                }
+                v = v - nb_terms(c, xm, x0, xp);
+                num += Js[q] * v;
+                den += Js[q];
             }
         bc[t] = num / den;
     }
@@ -236,13 +729,13 @@ __global__ __launch_bounds__(kThreads) void restrict_kernel(int nxc, int nyc, in
 
 // x_f += bilinear (in the coarsened directions) interpolation of x_c.
 template <typename R>
-__global__ __launch_bounds__(kThreads) void prolong_kernel(int nxf, int nyf, int nz, int rx, int ry, MirrorAcc<R> xc,
+__global__ __launch_bounds__(kThreads) void prolong_kernel(Split f, int rx, int ry, SplitMirror<R> xc,
                                                            R* __restrict__ xf)
 {
-    const int64_t n = int64_t(nxf) * nyf * nz;
-    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < 2 * f.plane;
+         t += int64_t(gridDim.x) * blockDim.x) {
         int i, j, k;
-        cell_of(t, nxf, nyf, i, j, k);
+        if (!f.cell(t, i, j, k)) { continue; }
         const int I = i / rx, Jc = j / ry;
         const int di = rx == 2 ? ((i & 1) ? 1 : -1) : 0;
         const int dj = ry == 2 ? ((j & 1) ? 1 : -1) : 0;
@@ -252,6 +745,29 @@ __global__ __launch_bounds__(kThreads) void prolong_kernel(int nxf, int nyf, int
         if (dj) { v += wx * (R(1) - wy) * xc(I, Jc + dj, k); }
         if (di && dj) { v += (R(1) - wx) * (R(1) - wy) * xc(I + di, Jc + dj, k); }
         xf[t] += v;
+    }
+}
+
+// Natural (i fastest) <-> split layout, with a precision change.
+template <typename A, typename B>
+__global__ void to_split_kernel(const A* __restrict__ nat, B* __restrict__ sp, Split s)
+{
+    const int64_t n = int64_t(s.nx) * s.ny * s.nz;
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        int i, j, k;
+        cell_of(t, s.nx, s.ny, i, j, k);
+        sp[s.at(i, j, k)] = B(nat[t]);
+    }
+}
+
+template <typename A, typename B>
+__global__ void from_split_kernel(const A* __restrict__ sp, B* __restrict__ nat, Split s)
+{
+    const int64_t n = int64_t(s.nx) * s.ny * s.nz;
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        int i, j, k;
+        cell_of(t, s.nx, s.ny, i, j, k);
+        nat[t] = B(sp[s.at(i, j, k)]);
     }
 }
 
@@ -432,23 +948,23 @@ __global__ void flux_kernel(int dir, Ctx<R> c, MirrorAcc<R> x, const R* __restri
         R v;
         if (dir == 0) {
             const bool zero = ((c.zf & stencil::kXlo) && i == 0) || ((c.zf & stencil::kXhi) && i == c.nx);
-            v = zero ? R(0) : -stencil::grad_x<R, R>(x, c.zp, i, j, k, c.dxinv);
+            v = zero ? R(0) : -stencil::grad_x<R, R>(x, c.mt, i, j, k, c.dxinv);
         } else if (dir == 1) {
             const bool zero = ((c.zf & stencil::kYlo) && j == 0) || ((c.zf & stencil::kYhi) && j == c.ny);
-            v = zero ? R(0) : -stencil::grad_y<R, R>(x, c.zp, i, j, k, c.dyinv);
+            v = zero ? R(0) : -stencil::grad_y<R, R>(x, c.mt, i, j, k, c.dyinv);
         } else {
             if ((c.zf & stencil::kZhi) && k == c.nz) {
                 v = R(0);
             } else if (c.dir_ground && k == 0) {
                 v = -stencil::grad_z_dir<R, R>(x, c.zp, i, j, k, c.dxinv, c.dyinv);
             } else {
-                v = -stencil::grad_z<R, R>(x, c.zp, i, j, k, c.dxinv, c.dyinv);
+                v = -stencil::grad_z<R, R>(x, c.mt, i, j, k, c.dxinv, c.dyinv);
                 if (k == 0) { v = -R(0); }
             }
         }
         if (beta) { v *= beta[i + j * bj + int64_t(k) * bk]; }
         if (dir == 2 && c.dcross.p && !(k == 0 || (k == c.nz && !c.top_open))) {
-            v -= c.dcross(i, j, k) * stencil::z_face_cross<R, R>(x, c.zp, i, j, k, c.dxinv, c.dyinv);
+            v -= c.dcross(i, j, k) * stencil::z_face_cross<R, R>(x, c.mt, i, j, k, c.dxinv, c.dyinv);
         }
         f[i + j * fj + int64_t(k) * fk] = v;
     }
@@ -556,18 +1072,31 @@ MirrorAcc<R> mirror(const R* p, int nx, int ny, int nz, const int* per, const Bc
     return m;
 }
 
-// One multigrid level (or the FP64 operator level).
+// One multigrid level (or the FP64 operator level). The mesh, the areas and
+// the mesh factors are in natural layout (i fastest); a multigrid level's
+// stencil planes and x, b, r are in the split layout.
 template <typename R>
 struct Level {
     int nx = 0, ny = 0, nz = 0, rx = 1, ry = 1;   // rx/ry: this level's coarsening factor from the finer one
     double dx[3] = {1, 1, 1};
-    DevBuf zp, J, ax, ay, az, dcross, azm, lo, dg, up, x, b, r, cs;
+    DevBuf zp, J, ax, ay, az, dcross, azm, mry, mrx, mzf;
+    DevBuf coef, Js, x, b, r;
     bool has_cross = false;
     int64_t n() const { return int64_t(nx) * ny * nz; }
+    Split split() const
+    {
+        Split s;
+        s.nx = nx; s.ny = ny; s.nz = nz;
+        s.H = (nx + 1) / 2;
+        s.plane = int64_t(s.H) * ny * nz;
+        return s;
+    }
+    int64_t slots() const { return 2 * split().plane; }
     Ctx<R> ctx(const BcInfo& bc) const
     {
         Ctx<R> c;
         c.zp = NodeAcc<R>{ptr<R>(zp), nx + 3, int64_t(nx + 3) * (ny + 3)};
+        c.mt = stencil::ArrayMetric<R>{ptr<R>(mry), ptr<R>(mrx), ptr<R>(mzf), nx, ny, nz};
         c.ax = GridAcc<R>{ptr<R>(ax), nx + 1, int64_t(nx + 1) * ny};
         c.ay = GridAcc<R>{ptr<R>(ay), nx, int64_t(nx) * (ny + 1)};
         c.az = GridAcc<R>{ptr<R>(az), nx, int64_t(nx) * ny};
@@ -584,6 +1113,25 @@ struct Level {
         return c;
     }
 };
+
+// The mesh factors of a level from its node heights (setup).
+template <typename R>
+void fill_metric(Level<R>& L, hipStream_t st)
+{
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    L.mry.ensure(size_t(nx + 1) * ny * (nz + 1) * sizeof(R));
+    L.mrx.ensure(size_t(nx) * (ny + 1) * (nz + 1) * sizeof(R));
+    L.mzf.ensure(size_t(3) * nx * ny * (nz + 1) * sizeof(R));
+    stencil::ZMetric<R, NodeAcc<R>> zm;
+    zm.zp = NodeAcc<R>{ptr<R>(L.zp), nx + 3, int64_t(nx + 3) * (ny + 3)};
+    zm.dxinv = R(1.0 / L.dx[0]);
+    zm.dyinv = R(1.0 / L.dx[1]);
+    const int64_t tot =
+        int64_t(nx + 1) * ny * (nz + 1) + int64_t(nx) * (ny + 1) * (nz + 1) + int64_t(nx) * ny * (nz + 1);
+    metric_kernel<R><<<blocks_for(tot), kThreads, 0, st>>>(zm, nx, ny, nz, ptr<R>(L.mry), ptr<R>(L.mrx),
+                                                           ptr<R>(L.mzf));
+    GUST_HIP(hipGetLastError());
+}
 
 }   // namespace
 
@@ -655,6 +1203,7 @@ void build_coarse_meshes(PoissonImpl& P, std::vector<std::unique_ptr<Level<R>>>&
         L->dx[0] = dx; L->dx[1] = dy; L->dx[2] = dz;
         std::vector<R> zr(zp.begin(), zp.end());
         upload(L->zp, zr, st);
+        fill_metric<R>(*L, st);
         // J from the node heights (extern_erf/ERF_TerrainMetrics.cpp:679-715)
         const double dzInv = 1.0 / dz;
         const int sj = nx + 3;
@@ -669,11 +1218,22 @@ void build_coarse_meshes(PoissonImpl& P, std::vector<std::unique_ptr<Level<R>>>&
                                           - Z(i, j, k) - Z(i + 1, j, k) - Z(i, j + 1, k) - Z(i + 1, j + 1, k)));
                 }
         upload(L->J, J, st);
-        const int64_t n = L->n();
+        const Split sp = L->split();
+        const int64_t ns = L->slots();
+        std::vector<R> Js(size_t(ns), R(0));   // padding slots hold J = 0
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) { Js[size_t(sp.at(i, j, k))] = J[i + size_t(j) * nx + size_t(k) * nx * ny]; }
+        upload(L->Js, Js, st);
         L->ax.ensure(size_t(nx + 1) * ny * nz * sizeof(R));
         L->ay.ensure(size_t(nx) * (ny + 1) * nz * sizeof(R));
         L->az.ensure(size_t(nx) * ny * (nz + 1) * sizeof(R));
-        for (DevBuf* b : {&L->lo, &L->dg, &L->up, &L->x, &L->b, &L->r, &L->cs}) { b->ensure(size_t(n) * sizeof(R)); }
+        L->coef.ensure(size_t(kBundle + 1) * size_t(ns) * sizeof(R));   // bundles, then the c' plane
+        for (DevBuf* b : {&L->x, &L->b, &L->r}) {
+            b->ensure(size_t(ns) * sizeof(R));
+            GUST_HIP(hipMemsetAsync(b->p, 0, size_t(ns) * sizeof(R), st));   // padding slots stay 0
+        }
+        GUST_HIP(hipMemsetAsync(L->coef.p, 0, size_t(kBundle + 1) * size_t(ns) * sizeof(R), st));
         mg.push_back(std::move(L));
         // Coarsen x and y by two where the size is even and at least 4.
         const int cx = (nx % 2 == 0 && nx >= 4) ? 2 : 1;
@@ -707,47 +1267,117 @@ void build_coarse_meshes(PoissonImpl& P, std::vector<std::unique_ptr<Level<R>>>&
 }
 
 template <typename R>
-void launch_apply(const Level<R>& L, const BcInfo& bc, const int* per, const R* x, const R* b, R* y, int color,
-                  hipStream_t st)
+void launch_apply(const Level<R>& L, const BcInfo& bc, const int* per, const R* x, const R* b, R* y, hipStream_t st)
 {
     const Ctx<R> c = L.ctx(bc);
     const MirrorAcc<R> m = mirror<R>(x, L.nx, L.ny, L.nz, per, bc);
-    const int nxh = color < 0 ? L.nx : (L.nx + 1) / 2;
-    const int64_t n = int64_t(nxh) * L.ny * L.nz;
-    if (b) { apply_kernel<R, 1><<<blocks_for(n), kThreads, 0, st>>>(c, m, b, y, color); }
-    else { apply_kernel<R, 0><<<blocks_for(n), kThreads, 0, st>>>(c, m, nullptr, y, color); }
+    const int64_t n = L.n();
+    if (b) { apply_kernel<R, 1><<<blocks_for(n), kThreads, 0, st>>>(c, m, b, y); }
+    else { apply_kernel<R, 0><<<blocks_for(n), kThreads, 0, st>>>(c, m, nullptr, y); }
     GUST_HIP(hipGetLastError());
 }
 
+// A periodic direction of odd length (more than one cell) puts two columns
+// of one colour side by side across the seam; such a level forms each
+// colour's right-hand side in a pass of its own before the line solves.
 template <typename R>
-void zebra(Level<R>& L, const BcInfo& bc, const int* per, hipStream_t st)
+bool seam_joins_colour(const Level<R>& L, const int* per)
 {
-    for (int color = 0; color < 2; ++color) {
-        launch_apply<R>(L, bc, per, ptr<R>(L.x), ptr<R>(L.b), ptr<R>(L.r), color, st);
-        const int64_t ncol = int64_t((L.nx + 1) / 2) * L.ny;
-        line_solve_kernel<R><<<blocks_for(ncol), kThreads, 0, st>>>(L.nx, L.ny, L.nz, ptr<R>(L.lo), ptr<R>(L.dg),
-                                                                   ptr<R>(L.up), ptr<R>(L.r), ptr<R>(L.cs),
-                                                                   ptr<R>(L.x), color);
+    return (per[0] && L.nx > 1 && (L.nx & 1)) || (per[1] && L.ny > 1 && (L.ny & 1));
+}
+
+// e' of one wave's 64 columns in LDS when it fits this budget.
+constexpr size_t kZebraLds = 64 * 1024;
+
+// Levels with fewer columns per colour than this run the half sweeps one
+// wave per column (zebra_scan_kernel); larger ones one lane per column.
+int64_t scan_cols()
+{
+    static const int64_t v = [] {
+        const char* e = std::getenv("GUST_ZEBRA_SCAN_COLS");
+        return e ? std::max<int64_t>(0, std::atoll(e)) : int64_t(16384);
+    }();
+    return v;
+}
+
+template <typename R>
+const R* bundle_of(const Level<R>& L) { return ptr<R>(L.coef); }
+template <typename R>
+const R* cp_of(const Level<R>& L) { return ptr<R>(L.coef) + size_t(kBundle) * size_t(L.slots()); }
+
+template <typename R, bool kZero, bool kFromR, int M>
+void launch_scan(const Level<R>& L, const int* per, int color, const R* rhs, hipStream_t st)
+{
+    const Split s = L.split();
+    const int64_t rowc = int64_t(s.H) * s.ny;
+    zebra_scan_kernel<R, kZero, kFromR, M><<<int((rowc + 3) / 4), 256, 0, st>>>(s, color, per[0], per[1],
+                                                                                bundle_of(L), cp_of(L), rhs,
+                                                                                ptr<R>(L.x));
+}
+
+template <typename R, bool kZero, bool kFromR>
+void launch_zebra(const Level<R>& L, const int* per, int color, const R* rhs, hipStream_t st)
+{
+    const Split s = L.split();
+    const int64_t rowc = int64_t(s.H) * s.ny;
+    if (rowc < scan_cols() && L.nz <= 256) {
+        const int M = (L.nz + 63) / 64;
+        if (M == 1) { launch_scan<R, kZero, kFromR, 1>(L, per, color, rhs, st); }
+        else if (M == 2) { launch_scan<R, kZero, kFromR, 2>(L, per, color, rhs, st); }
+        else if (M == 3) { launch_scan<R, kZero, kFromR, 3>(L, per, color, rhs, st); }
+        else { launch_scan<R, kZero, kFromR, 4>(L, per, color, rhs, st); }
         GUST_HIP(hipGetLastError());
+        return;
+    }
+    const int blocks = int((rowc + 63) / 64);
+    const size_t lds = size_t(64) * size_t(L.nz) * sizeof(R);
+    constexpr int kP = sizeof(R) == 8 ? 2 : 6;
+    if (lds <= kZebraLds) {
+        zebra_kernel<R, kZero, kFromR, true, kP><<<blocks, 64, lds, st>>>(s, color, per[0], per[1], bundle_of(L),
+                                                                          cp_of(L), rhs, ptr<R>(L.x), nullptr);
+    } else {
+        zebra_kernel<R, kZero, kFromR, false, kP><<<blocks, 64, 0, st>>>(s, color, per[0], per[1], bundle_of(L),
+                                                                         cp_of(L), rhs, ptr<R>(L.x), ptr<R>(L.r));
+    }
+    GUST_HIP(hipGetLastError());
+}
+
+// One zebra sweep (colour 0, then colour 1); first: x is zero on entry.
+template <typename R>
+void zebra(Level<R>& L, const int* per, bool first, hipStream_t st)
+{
+    if (!seam_joins_colour(L, per)) {
+        if (first) { launch_zebra<R, true, false>(L, per, 0, ptr<R>(L.b), st); }
+        else { launch_zebra<R, false, false>(L, per, 0, ptr<R>(L.b), st); }
+        launch_zebra<R, false, false>(L, per, 1, ptr<R>(L.b), st);
+        return;
+    }
+    const Split s = L.split();
+    if (first) { fill_kernel<R><<<blocks_for(L.slots()), kThreads, 0, st>>>(ptr<R>(L.x), R(0), L.slots()); }
+    for (int color = 0; color < 2; ++color) {
+        offcol_kernel<R><<<blocks_for(s.plane), kThreads, 0, st>>>(s, color, per[0], per[1], bundle_of(L),
+                                                                   ptr<R>(L.b), ptr<R>(L.x), ptr<R>(L.r));
+        GUST_HIP(hipGetLastError());
+        launch_zebra<R, false, true>(L, per, color, ptr<R>(L.r), st);
     }
 }
 
 // J-weighted mean removal of a level's x (the coarsest solve of a singular
-// problem): x -= sum J x / sum J.
+// problem): x -= sum J x / sum J, over the split slots (padding has J = 0).
 template <typename R>
 void remove_mean(PoissonImpl& P, Level<R>& L, hipStream_t st);
 
 template <>
 void remove_mean<double>(PoissonImpl& P, Level<double>& L, hipStream_t st)
 {
-    const int64_t n = L.n();
+    const int64_t n = L.slots();
     double* d = ptr<double>(P.dots);
-    dot_partial_kernel<<<dim3(kRedBlocks, 1), kThreads, 0, st>>>(ptr<double>(L.J), 0, ptr<double>(L.x), n,
+    dot_partial_kernel<<<dim3(kRedBlocks, 1), kThreads, 0, st>>>(ptr<double>(L.Js), 0, ptr<double>(L.x), n,
                                                                  ptr<double>(P.partial));
     sum_partial_kernel<<<1, kThreads, 0, st>>>(ptr<double>(P.partial), kRedBlocks, d);
     // sum J: dot(J, ones), with L.r (free at the coarsest level) as the ones
     fill_kernel<double><<<blocks_for(n), kThreads, 0, st>>>(ptr<double>(L.r), 1.0, n);
-    dot_partial_kernel<<<dim3(kRedBlocks, 1), kThreads, 0, st>>>(ptr<double>(L.J), 0, ptr<double>(L.r), n,
+    dot_partial_kernel<<<dim3(kRedBlocks, 1), kThreads, 0, st>>>(ptr<double>(L.Js), 0, ptr<double>(L.r), n,
                                                                  ptr<double>(P.partial));
     sum_partial_kernel<<<1, kThreads, 0, st>>>(ptr<double>(P.partial), kRedBlocks, d + 1);
     add_scalar_kernel_r<double><<<blocks_for(n), kThreads, 0, st>>>(ptr<double>(L.x), d, d + 1, n);
@@ -758,11 +1388,11 @@ template <>
 void remove_mean<float>(PoissonImpl& P, Level<float>& L, hipStream_t st)
 {
     // promote to double for the reduction
-    const int64_t n = L.n();
-    double* tmp = ptr<double>(P.w);   // n doubles of scratch (w is >= level 0 size)
+    const int64_t n = L.slots();
+    double* tmp = ptr<double>(P.w);   // 2 n doubles of scratch (w holds 2 level-0 vectors)
     double* tj = tmp + n;
     convert_kernel<float, double><<<blocks_for(n), kThreads, 0, st>>>(ptr<float>(L.x), tmp, n);
-    convert_kernel<float, double><<<blocks_for(n), kThreads, 0, st>>>(ptr<float>(L.J), tj, n);
+    convert_kernel<float, double><<<blocks_for(n), kThreads, 0, st>>>(ptr<float>(L.Js), tj, n);
     double* d = ptr<double>(P.dots);
     dot_partial_kernel<<<dim3(kRedBlocks, 1), kThreads, 0, st>>>(tj, 0, tmp, n, ptr<double>(P.partial));
     sum_partial_kernel<<<1, kThreads, 0, st>>>(ptr<double>(P.partial), kRedBlocks, d);
@@ -778,28 +1408,33 @@ void vcycle(PoissonImpl& P, std::vector<std::unique_ptr<Level<R>>>& mg, size_t l
             hipStream_t st)
 {
     Level<R>& L = *mg[l];
-    const int64_t n = L.n();
-    fill_kernel<R><<<blocks_for(n), kThreads, 0, st>>>(ptr<R>(L.x), R(0), n);
     if (l + 1 == mg.size()) {
-        for (int s = 0; s < P.opt.coarse_sweeps; ++s) { zebra<R>(L, bc, P.per, st); }
+        for (int s = 0; s < P.opt.coarse_sweeps; ++s) { zebra<R>(L, P.per, s == 0, st); }
         if (bc.singular) { remove_mean<R>(P, L, st); }
         return;
     }
-    for (int s = 0; s < P.opt.pre_sweeps; ++s) { zebra<R>(L, bc, P.per, st); }
-    launch_apply<R>(L, bc, P.per, ptr<R>(L.x), ptr<R>(L.b), ptr<R>(L.r), -1, st);
+    if (P.opt.pre_sweeps == 0) {
+        fill_kernel<R><<<blocks_for(L.slots()), kThreads, 0, st>>>(ptr<R>(L.x), R(0), L.slots());
+    }
+    for (int s = 0; s < P.opt.pre_sweeps; ++s) { zebra<R>(L, P.per, s == 0, st); }
     Level<R>& C = *mg[l + 1];
-    restrict_kernel<R><<<blocks_for(C.n()), kThreads, 0, st>>>(C.nx, C.ny, C.nz, C.rx, C.ry, ptr<R>(L.r),
-                                                               ptr<R>(L.J), ptr<R>(C.b));
+    const Split sf = L.split(), sc = C.split();
+    resid_restrict_kernel<R><<<blocks_for(C.slots()), kThreads, 0, st>>>(sf, P.per[0], P.per[1], bundle_of(L),
+                                                                         ptr<R>(L.b), ptr<R>(L.x), ptr<R>(L.Js), sc,
+                                                                         C.rx, C.ry, ptr<R>(C.b));
     GUST_HIP(hipGetLastError());
     vcycle<R>(P, mg, l + 1, bc, st);
-    const MirrorAcc<R> mc = mirror<R>(ptr<R>(C.x), C.nx, C.ny, C.nz, P.per, bc);
-    prolong_kernel<R><<<blocks_for(n), kThreads, 0, st>>>(L.nx, L.ny, L.nz, C.rx, C.ry, mc, ptr<R>(L.x));
+    SplitMirror<R> mc;
+    mc.m = mirror<R>(nullptr, C.nx, C.ny, C.nz, P.per, bc);
+    mc.s = sc;
+    mc.p = ptr<R>(C.x);
+    prolong_kernel<R><<<blocks_for(L.slots()), kThreads, 0, st>>>(sf, C.rx, C.ry, mc, ptr<R>(L.x));
     GUST_HIP(hipGetLastError());
-    for (int s = 0; s < P.opt.post_sweeps; ++s) { zebra<R>(L, bc, P.per, st); }
+    for (int s = 0; s < P.opt.post_sweeps; ++s) { zebra<R>(L, P.per, false, st); }
 }
 
-// Premultiplied areas and column tridiagonals on every level of a
-// hierarchy, from level 0's FP64 areas.
+// Premultiplied areas and the explicit stencil (with its column Thomas
+// factors) on every level of a hierarchy, from level 0's FP64 areas.
 template <typename R>
 void setup_hierarchy(PoissonImpl& P, std::vector<std::unique_ptr<Level<R>>>& mg, const BcInfo& bc, hipStream_t st)
 {
@@ -842,7 +1477,11 @@ void setup_hierarchy(PoissonImpl& P, std::vector<std::unique_ptr<Level<R>>>& mg,
         GUST_HIP(hipGetLastError());
         const Ctx<R> c = L.ctx(bc);
         const MirrorAcc<R> m = mirror<R>(nullptr, L.nx, L.ny, L.nz, P.per, bc);
-        tridiag_kernel<R><<<blocks_for(L.n()), kThreads, 0, st>>>(c, m, ptr<R>(L.lo), ptr<R>(L.dg), ptr<R>(L.up));
+        const Split sp = L.split();
+        coef_seed_kernel<R><<<blocks_for(L.slots()), kThreads, 0, st>>>(c, m, sp, P.per[0], P.per[1], ptr<R>(L.coef));
+        GUST_HIP(hipGetLastError());
+        coef_factor_kernel<R><<<blocks_for(2 * int64_t(sp.H) * sp.ny), kThreads, 0, st>>>(
+            sp, ptr<R>(L.coef), ptr<R>(L.coef) + size_t(kBundle) * size_t(L.slots()));
         GUST_HIP(hipGetLastError());
     }
 }
@@ -853,14 +1492,14 @@ void precondition(PoissonImpl& P, const double* v, double* z, const BcInfo& bc, 
     const int64_t n = P.n();
     if (P.opt.precision == 1) {
         Level<float>& L = *P.mg32[0];
-        convert_kernel<double, float><<<blocks_for(n), kThreads, 0, st>>>(v, ptr<float>(L.b), n);
+        to_split_kernel<double, float><<<blocks_for(n), kThreads, 0, st>>>(v, ptr<float>(L.b), L.split());
         vcycle<float>(P, P.mg32, 0, bc, st);
-        convert_kernel<float, double><<<blocks_for(n), kThreads, 0, st>>>(ptr<float>(L.x), z, n);
+        from_split_kernel<float, double><<<blocks_for(n), kThreads, 0, st>>>(ptr<float>(L.x), z, L.split());
     } else {
         Level<double>& L = *P.mg64[0];
-        GUST_HIP(hipMemcpyAsync(L.b.p, v, size_t(n) * 8, hipMemcpyDeviceToDevice, st));
+        to_split_kernel<double, double><<<blocks_for(n), kThreads, 0, st>>>(v, ptr<double>(L.b), L.split());
         vcycle<double>(P, P.mg64, 0, bc, st);
-        GUST_HIP(hipMemcpyAsync(z, L.x.p, size_t(n) * 8, hipMemcpyDeviceToDevice, st));
+        from_split_kernel<double, double><<<blocks_for(n), kThreads, 0, st>>>(ptr<double>(L.x), z, L.split());
     }
     GUST_HIP(hipGetLastError());
 }
@@ -1053,7 +1692,10 @@ void check_cells(const PoissonImpl& P, const gust_array4* a, const char* what)
 void ensure_work(PoissonImpl& P, int m)
 {
     const int64_t n = P.n();
-    P.w.ensure(size_t(n + 1) * 8 * 2 + 64);
+    // two level-0 vectors, or two split-layout vectors of any level
+    // (remove_mean<float>): (nx + 1) ny nz covers a split level's slots
+    const int64_t slots = int64_t(P.nx + 1) * P.ny * P.nz;
+    P.w.ensure(size_t(std::max(n, slots) + 1) * 8 * 2 + 64);
     P.krylov_v.ensure(size_t(n) * 8 * size_t(m + 1));
     P.krylov_z.ensure(size_t(n) * 8 * size_t(m));
     P.partial.ensure(size_t(kRedBlocks) * 8 * size_t(m + 2));
@@ -1063,7 +1705,7 @@ void ensure_work(PoissonImpl& P, int m)
 
 void apply_fp64(PoissonImpl& P, const BcInfo& bc, const double* x, double* y, hipStream_t st)
 {
-    launch_apply<double>(P.L0, bc, P.per, x, nullptr, y, -1, st);
+    launch_apply<double>(P.L0, bc, P.per, x, nullptr, y, st);
 }
 
 }   // namespace
@@ -1139,6 +1781,7 @@ int gust_poisson_create(const gust_geometry* geom, const gust_array4* z_nd, cons
         L.nx = P.nx; L.ny = P.ny; L.nz = P.nz;
         for (int q = 0; q < 3; ++q) { L.dx[q] = geom->dx[q]; }
         upload(L.zp, P.zp_host, st);
+        fill_metric<double>(L, st);
         upload(L.J, view_to_host(*detJ, cells.lo, cells.hi, st), st);
         L.ax.ensure(size_t(P.nx + 1) * P.ny * P.nz * 8);
         L.ay.ensure(size_t(P.nx) * (P.ny + 1) * P.nz * 8);
@@ -1337,7 +1980,7 @@ int gust_poisson_solve(gust_poisson* p, gust_array4* rhs, gust_array4* phi, cons
             set_coef(P, y.data(), j, st);
             multi_axpy_kernel<<<blocks_for(n), kThreads, 0, st>>>(x, Z, n, ptr<double>(P.hcoef), j, 1.0, n);
             // true residual
-            launch_apply<double>(P.L0, b, P.per, x, bvec, r, -1, st);
+            launch_apply<double>(P.L0, b, P.per, x, bvec, r, st);
             beta = norm2(P, r, st);
             converged = beta <= target;
         }
@@ -1350,7 +1993,7 @@ int gust_poisson_solve(gust_poisson* p, gust_array4* rhs, gust_array4* phi, cons
             GUST_HIP(hipMemcpyAsync(P.dots.p, hp, 8, hipMemcpyHostToDevice, st));
             add_scalar_kernel<<<blocks_for(n), kThreads, 0, st>>>(x, ptr<double>(P.dots), -1.0, n);
         }
-        launch_apply<double>(P.L0, b, P.per, x, bvec, r, -1, st);
+        launch_apply<double>(P.L0, b, P.per, x, bvec, r, st);
         res.final_residual = norm2(P, r, st);
         phi_in_out(P, phi, x, 1, st);
         fill_phi_ghosts(P, phi, b, st);

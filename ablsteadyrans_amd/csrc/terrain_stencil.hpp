@@ -82,80 +82,120 @@ __device__ __forceinline__ V zero_of() { return V(0); }
 // Face-mask bits (extern_erf/ERF_TerrainPoisson_3D_K.H:281-285).
 enum : int { kXlo = 1, kXhi = 2, kYlo = 4, kYhi = 8, kZlo = 16, kZhi = 32 };
 
+// The mesh-only factors of the stencil, from the node heights zp (one ghost
+// node) on the fly (ZMetric) or from arrays filled once per mesh
+// (ArrayMetric). Every division of the operator is one of these, so the
+// array form removes them from the per-apply work while the stored values
+// are the same quotients, bit for bit.
+//   ry(i, j, k): h_xi / h_zeta on the y-edge at node x i, cell y j, node z k
+//   rx(i, j, k): h_eta / h_zeta on the x-edge at cell x i, node y j, node z k
+//   hzi(i, j, k): 8 / (column depth over k-1..k+1) on the z-face (i, j, k)
+//   hxz, hez(i, j, k): the face slopes h_xi, h_eta on that z-face
+template <typename R, class Z>
+struct ZMetric {
+    Z zp;
+    R dxinv, dyinv;
+    __device__ __forceinline__ R ry(int i, int j, int k) const
+    {
+        const R fourth = R(0.25);
+        const R h_xi = fourth * (zp(i + 1, j, k) - zp(i - 1, j, k) + zp(i + 1, j + 1, k) - zp(i - 1, j + 1, k)) * dxinv;
+        const R h_zeta = fourth * (zp(i, j, k + 1) - zp(i, j, k - 1) + zp(i, j + 1, k + 1) - zp(i, j + 1, k - 1));
+        return h_xi / h_zeta;
+    }
+    __device__ __forceinline__ R rx(int i, int j, int k) const
+    {
+        const R fourth = R(0.25);
+        const R h_eta = fourth * (zp(i, j + 1, k) - zp(i, j - 1, k) + zp(i + 1, j + 1, k) - zp(i + 1, j - 1, k)) * dyinv;
+        const R h_zeta = fourth * (zp(i, j, k + 1) - zp(i, j, k - 1) + zp(i + 1, j, k + 1) - zp(i + 1, j, k - 1));
+        return h_eta / h_zeta;
+    }
+    __device__ __forceinline__ R hzi(int i, int j, int k) const
+    {
+        return R(8.0) / ((zp(i, j, k + 1) + zp(i + 1, j, k + 1) + zp(i, j + 1, k + 1) + zp(i + 1, j + 1, k + 1))
+                         - (zp(i, j, k - 1) + zp(i + 1, j, k - 1) + zp(i, j + 1, k - 1) + zp(i + 1, j + 1, k - 1)));
+    }
+    __device__ __forceinline__ R hxz(int i, int j, int k) const
+    {
+        return R(0.5) * (zp(i + 1, j + 1, k) + zp(i + 1, j, k) - zp(i, j + 1, k) - zp(i, j, k)) * dxinv;
+    }
+    __device__ __forceinline__ R hez(int i, int j, int k) const
+    {
+        return R(0.5) * (zp(i + 1, j + 1, k) + zp(i, j + 1, k) - zp(i + 1, j, k) - zp(i, j, k)) * dyinv;
+    }
+};
+
+template <typename R>
+struct ArrayMetric {
+    const R* ryp;   // (nx+1) x ny x (nz+1)
+    const R* rxp;   // nx x (ny+1) x (nz+1)
+    const R* zfp;   // 3 x nx x ny x (nz+1): hzi, hxz, hez
+    int nx, ny, nz;
+    __device__ __forceinline__ R ry(int i, int j, int k) const
+    {
+        return ryp[i + int64_t(j) * (nx + 1) + int64_t(k) * (nx + 1) * ny];
+    }
+    __device__ __forceinline__ R rx(int i, int j, int k) const
+    {
+        return rxp[i + int64_t(j) * nx + int64_t(k) * nx * (ny + 1)];
+    }
+    __device__ __forceinline__ R zf(int c, int i, int j, int k) const
+    {
+        return zfp[i + int64_t(j) * nx + int64_t(k) * nx * ny + int64_t(c) * nx * ny * (nz + 1)];
+    }
+    __device__ __forceinline__ R hzi(int i, int j, int k) const { return zf(0, i, j, k); }
+    __device__ __forceinline__ R hxz(int i, int j, int k) const { return zf(1, i, j, k); }
+    __device__ __forceinline__ R hez(int i, int j, int k) const { return zf(2, i, j, k); }
+};
+
 // The low x-face gradient of cell (i, j, k) (ERF's px_lo; flux_x = -this).
-template <typename V, typename R, class X, class Z>
-__device__ __forceinline__ V grad_x(const X& x, const Z& zp, int i, int j, int k, R dxinv)
+template <typename V, typename R, class X, class Mt>
+__device__ __forceinline__ V grad_x(const X& x, const Mt& mt, int i, int j, int k, R dxinv)
 {
-    const R half = R(0.5), fourth = R(0.25);
-    R h_xi, h_zeta;
+    const R half = R(0.5);
     V px_lo = (x(i, j, k) - x(i - 1, j, k)) * dxinv;
     V pz_lo_md_hi = half * (x(i, j, k + 1) + x(i - 1, j, k + 1) - x(i, j, k) - x(i - 1, j, k));
-    h_xi = fourth * (zp(i + 1, j, k + 1) - zp(i - 1, j, k + 1) + zp(i + 1, j + 1, k + 1) - zp(i - 1, j + 1, k + 1))
-           * dxinv;
-    h_zeta = fourth * (zp(i, j, k + 2) - zp(i, j, k) + zp(i, j + 1, k + 2) - zp(i, j + 1, k));
-    pz_lo_md_hi *= h_xi / h_zeta;
+    pz_lo_md_hi *= mt.ry(i, j, k + 1);
     V pz_lo_md_lo = half * (x(i, j, k) + x(i - 1, j, k) - x(i, j, k - 1) - x(i - 1, j, k - 1));
-    h_xi = fourth * (zp(i + 1, j, k) - zp(i - 1, j, k) + zp(i + 1, j + 1, k) - zp(i - 1, j + 1, k)) * dxinv;
-    h_zeta = fourth * (zp(i, j, k + 1) - zp(i, j, k - 1) + zp(i, j + 1, k + 1) - zp(i, j + 1, k - 1));
-    pz_lo_md_lo *= h_xi / h_zeta;
+    pz_lo_md_lo *= mt.ry(i, j, k);
     px_lo -= half * (pz_lo_md_hi + pz_lo_md_lo);
     return px_lo;
 }
 
-template <typename V, typename R, class X, class Z>
-__device__ __forceinline__ V grad_y(const X& x, const Z& zp, int i, int j, int k, R dyinv)
+template <typename V, typename R, class X, class Mt>
+__device__ __forceinline__ V grad_y(const X& x, const Mt& mt, int i, int j, int k, R dyinv)
 {
-    const R half = R(0.5), fourth = R(0.25);
-    R h_eta, h_zeta;
+    const R half = R(0.5);
     V py_lo = (x(i, j, k) - x(i, j - 1, k)) * dyinv;
     V pz_md_lo_hi = half * (x(i, j, k + 1) + x(i, j - 1, k + 1) - x(i, j, k) - x(i, j - 1, k));
-    h_eta = fourth * (zp(i, j + 1, k + 1) - zp(i, j - 1, k + 1) + zp(i + 1, j + 1, k + 1) - zp(i + 1, j - 1, k + 1))
-            * dyinv;
-    h_zeta = fourth * (zp(i, j, k + 2) - zp(i, j, k) + zp(i + 1, j, k + 2) - zp(i + 1, j, k));
-    pz_md_lo_hi *= h_eta / h_zeta;
+    pz_md_lo_hi *= mt.rx(i, j, k + 1);
     V pz_md_lo_lo = half * (x(i, j, k) + x(i, j - 1, k) - x(i, j, k - 1) - x(i, j - 1, k - 1));
-    h_eta = fourth * (zp(i, j + 1, k) - zp(i, j - 1, k) + zp(i + 1, j + 1, k) - zp(i + 1, j - 1, k)) * dyinv;
-    h_zeta = fourth * (zp(i, j, k + 1) - zp(i, j, k - 1) + zp(i + 1, j, k + 1) - zp(i + 1, j, k - 1));
-    pz_md_lo_lo *= h_eta / h_zeta;
+    pz_md_lo_lo *= mt.rx(i, j, k);
     py_lo -= half * (pz_md_lo_hi + pz_md_lo_lo);
     return py_lo;
 }
 
 // The low z-face gradient of cell (i, j, k) (ERF's pz_lo before the ground
 // zeroing).
-template <typename V, typename R, class X, class Z>
-__device__ __forceinline__ V grad_z(const X& x, const Z& zp, int i, int j, int k, R dxinv, R dyinv)
+template <typename V, typename R, class X, class Mt>
+__device__ __forceinline__ V grad_z(const X& x, const Mt& mt, int i, int j, int k, R dxinv, R dyinv)
 {
-    const R half = R(0.5), fourth = R(0.25);
-    R h_xi, h_eta, h_zeta;
+    const R half = R(0.5);
     V pz_lo = (x(i, j, k) - x(i, j, k - 1));
-    const R hzeta_inv_on_zlo = R(8.0) / ((zp(i, j, k + 1) + zp(i + 1, j, k + 1) + zp(i, j + 1, k + 1)
-                                          + zp(i + 1, j + 1, k + 1))
-                                         - (zp(i, j, k - 1) + zp(i + 1, j, k - 1) + zp(i, j + 1, k - 1)
-                                            + zp(i + 1, j + 1, k - 1)));
-    pz_lo *= hzeta_inv_on_zlo;
+    pz_lo *= mt.hzi(i, j, k);
     const V px_hi_md_lo = half * (x(i + 1, j, k) - x(i, j, k) + x(i + 1, j, k - 1) - x(i, j, k - 1)) * dxinv;
     const V px_lo_md_lo = half * (x(i, j, k) - x(i - 1, j, k) + x(i, j, k - 1) - x(i - 1, j, k - 1)) * dxinv;
     const V py_md_hi_lo = half * (x(i, j + 1, k) - x(i, j, k) + x(i, j + 1, k - 1) - x(i, j, k - 1)) * dyinv;
     const V py_md_lo_lo = half * (x(i, j, k) - x(i, j - 1, k) + x(i, j, k - 1) - x(i, j - 1, k - 1)) * dyinv;
     V pz_hi_md_lo = half * (x(i + 1, j, k) + x(i, j, k) - x(i + 1, j, k - 1) - x(i, j, k - 1));
-    h_xi = fourth * (zp(i + 2, j, k) - zp(i, j, k) + zp(i + 2, j + 1, k) - zp(i, j + 1, k)) * dxinv;
-    h_zeta = fourth * (zp(i + 1, j, k + 1) - zp(i + 1, j, k - 1) + zp(i + 1, j + 1, k + 1) - zp(i + 1, j + 1, k - 1));
-    pz_hi_md_lo *= h_xi / h_zeta;
+    pz_hi_md_lo *= mt.ry(i + 1, j, k);
     V pz_lo_md_lo = half * (x(i, j, k) + x(i - 1, j, k) - x(i, j, k - 1) - x(i - 1, j, k - 1));
-    h_xi = fourth * (zp(i + 1, j, k) - zp(i - 1, j, k) + zp(i + 1, j + 1, k) - zp(i - 1, j + 1, k)) * dxinv;
-    h_zeta = fourth * (zp(i, j, k + 1) - zp(i, j, k - 1) + zp(i, j + 1, k + 1) - zp(i, j + 1, k - 1));
-    pz_lo_md_lo *= h_xi / h_zeta;
+    pz_lo_md_lo *= mt.ry(i, j, k);
     V pz_md_hi_lo = half * (x(i, j + 1, k) + x(i, j, k) - x(i, j + 1, k - 1) - x(i, j, k - 1));
-    h_eta = fourth * (zp(i, j + 2, k) - zp(i, j, k) + zp(i + 1, j + 2, k) - zp(i + 1, j, k)) * dyinv;
-    h_zeta = fourth * (zp(i, j + 1, k + 1) - zp(i, j + 1, k - 1) + zp(i + 1, j + 1, k + 1) - zp(i + 1, j + 1, k - 1));
-    pz_md_hi_lo *= h_eta / h_zeta;
+    pz_md_hi_lo *= mt.rx(i, j + 1, k);
     V pz_md_lo_lo = half * (x(i, j, k) + x(i, j - 1, k) - x(i, j, k - 1) - x(i, j - 1, k - 1));
-    h_eta = fourth * (zp(i, j + 1, k) - zp(i, j - 1, k) + zp(i + 1, j + 1, k) - zp(i + 1, j - 1, k)) * dyinv;
-    h_zeta = fourth * (zp(i, j, k + 1) - zp(i, j, k - 1) + zp(i + 1, j, k + 1) - zp(i + 1, j, k - 1));
-    pz_md_lo_lo *= h_eta / h_zeta;
-    const R h_xi_on_zlo = half * (zp(i + 1, j + 1, k) + zp(i + 1, j, k) - zp(i, j + 1, k) - zp(i, j, k)) * dxinv;
-    const R h_eta_on_zlo = half * (zp(i + 1, j + 1, k) + zp(i, j + 1, k) - zp(i + 1, j, k) - zp(i, j, k)) * dyinv;
+    pz_md_lo_lo *= mt.rx(i, j, k);
+    const R h_xi_on_zlo = mt.hxz(i, j, k);
+    const R h_eta_on_zlo = mt.hez(i, j, k);
     pz_lo -= half * h_xi_on_zlo * ((px_hi_md_lo + px_lo_md_lo) - (pz_hi_md_lo + pz_lo_md_lo));
     pz_lo -= half * h_eta_on_zlo * ((py_md_hi_lo + py_md_lo_lo) - (pz_md_hi_lo + pz_md_lo_lo));
     return pz_lo;
@@ -199,40 +239,33 @@ __device__ __forceinline__ V grad_z_dir(const X& x, const Z& zp, int i, int j, i
 }
 
 // The terrain cross part of the z-face gradient (Source/PoissonCross_K.H:16-58).
-template <typename V, typename R, class X, class Z>
-__device__ __forceinline__ V z_face_cross(const X& x, const Z& zp, int i, int j, int k, R dxinv, R dyinv)
+template <typename V, typename R, class X, class Mt>
+__device__ __forceinline__ V z_face_cross(const X& x, const Mt& mt, int i, int j, int k, R dxinv, R dyinv)
 {
-    const R half = R(0.5), quarter = R(0.25);
+    const R half = R(0.5);
     const V px_hi_md_lo = half * (x(i + 1, j, k) - x(i, j, k) + x(i + 1, j, k - 1) - x(i, j, k - 1)) * dxinv;
     const V px_lo_md_lo = half * (x(i, j, k) - x(i - 1, j, k) + x(i, j, k - 1) - x(i - 1, j, k - 1)) * dxinv;
     const V py_md_hi_lo = half * (x(i, j + 1, k) - x(i, j, k) + x(i, j + 1, k - 1) - x(i, j, k - 1)) * dyinv;
     const V py_md_lo_lo = half * (x(i, j, k) - x(i, j - 1, k) + x(i, j, k - 1) - x(i, j - 1, k - 1)) * dyinv;
     V pz_hi_md_lo = half * (x(i + 1, j, k) + x(i, j, k) - x(i + 1, j, k - 1) - x(i, j, k - 1));
-    R h_xi = quarter * (zp(i + 2, j, k) - zp(i, j, k) + zp(i + 2, j + 1, k) - zp(i, j + 1, k)) * dxinv;
-    R h_zeta = quarter * (zp(i + 1, j, k + 1) - zp(i + 1, j, k - 1) + zp(i + 1, j + 1, k + 1) - zp(i + 1, j + 1, k - 1));
-    pz_hi_md_lo *= h_xi / h_zeta;
+    pz_hi_md_lo *= mt.ry(i + 1, j, k);
     V pz_lo_md_lo = half * (x(i, j, k) + x(i - 1, j, k) - x(i, j, k - 1) - x(i - 1, j, k - 1));
-    h_xi = quarter * (zp(i + 1, j, k) - zp(i - 1, j, k) + zp(i + 1, j + 1, k) - zp(i - 1, j + 1, k)) * dxinv;
-    h_zeta = quarter * (zp(i, j, k + 1) - zp(i, j, k - 1) + zp(i, j + 1, k + 1) - zp(i, j + 1, k - 1));
-    pz_lo_md_lo *= h_xi / h_zeta;
+    pz_lo_md_lo *= mt.ry(i, j, k);
     V pz_md_hi_lo = half * (x(i, j + 1, k) + x(i, j, k) - x(i, j + 1, k - 1) - x(i, j, k - 1));
-    R h_eta = quarter * (zp(i, j + 2, k) - zp(i, j, k) + zp(i + 1, j + 2, k) - zp(i + 1, j, k)) * dyinv;
-    h_zeta = quarter * (zp(i, j + 1, k + 1) - zp(i, j + 1, k - 1) + zp(i + 1, j + 1, k + 1) - zp(i + 1, j + 1, k - 1));
-    pz_md_hi_lo *= h_eta / h_zeta;
+    pz_md_hi_lo *= mt.rx(i, j + 1, k);
     V pz_md_lo_lo = half * (x(i, j, k) + x(i, j - 1, k) - x(i, j, k - 1) - x(i, j - 1, k - 1));
-    h_eta = quarter * (zp(i, j + 1, k) - zp(i, j - 1, k) + zp(i + 1, j + 1, k) - zp(i + 1, j - 1, k)) * dyinv;
-    h_zeta = quarter * (zp(i, j, k + 1) - zp(i, j, k - 1) + zp(i + 1, j, k + 1) - zp(i + 1, j, k - 1));
-    pz_md_lo_lo *= h_eta / h_zeta;
-    const R h_xi_on_zlo = half * (zp(i + 1, j + 1, k) + zp(i + 1, j, k) - zp(i, j + 1, k) - zp(i, j, k)) * dxinv;
-    const R h_eta_on_zlo = half * (zp(i + 1, j + 1, k) + zp(i, j + 1, k) - zp(i + 1, j, k) - zp(i, j, k)) * dyinv;
+    pz_md_lo_lo *= mt.rx(i, j, k);
+    const R h_xi_on_zlo = mt.hxz(i, j, k);
+    const R h_eta_on_zlo = mt.hez(i, j, k);
     return half * h_xi_on_zlo * ((px_hi_md_lo + px_lo_md_lo) - (pz_hi_md_lo + pz_lo_md_lo))
          + half * h_eta_on_zlo * ((py_md_hi_lo + py_md_lo_lo) - (pz_md_hi_lo + pz_md_lo_lo));
 }
 
 // Everything the operator at one cell needs besides the field.
-template <typename R, class Z, class F>
+template <typename R, class Z, class F, class Mt>
 struct OpCtx {
-    Z zp;                 // node heights (one ghost node)
+    Z zp;                 // node heights (one ghost node; the one-sided ground face reads them)
+    Mt mt;                // the mesh-only factors
     F ax, ay, az, J;      // premultiplied face areas (valid faces), cell J
     F dcross, az_mesh;    // z_cross correction (beta_z - z_cross) and mesh az; dcross.p null: none
     R dxinv, dyinv, dzinv;
@@ -245,15 +278,15 @@ struct OpCtx {
 // (A x)(i, j, k): extern_erf/ERF_TerrainPoisson_3D_K.H:318-530 plus the
 // Dirichlet ground face (Source/Poisson.cpp:162-190) and the z_cross
 // correction (Source/Poisson.cpp:231-254).
-template <typename V, typename R, class X, class Z, class F>
-__device__ __forceinline__ V apply_cell(const X& x, const OpCtx<R, Z, F>& c, int i, int j, int k)
+template <typename V, typename R, class X, class Z, class F, class Mt>
+__device__ __forceinline__ V apply_cell(const X& x, const OpCtx<R, Z, F, Mt>& c, int i, int j, int k)
 {
-    V px_hi = grad_x<V, R>(x, c.zp, i + 1, j, k, c.dxinv);
-    V px_lo = grad_x<V, R>(x, c.zp, i, j, k, c.dxinv);
-    V py_hi = grad_y<V, R>(x, c.zp, i, j + 1, k, c.dyinv);
-    V py_lo = grad_y<V, R>(x, c.zp, i, j, k, c.dyinv);
-    V pz_hi = grad_z<V, R>(x, c.zp, i, j, k + 1, c.dxinv, c.dyinv);
-    V pz_lo = grad_z<V, R>(x, c.zp, i, j, k, c.dxinv, c.dyinv);
+    V px_hi = grad_x<V, R>(x, c.mt, i + 1, j, k, c.dxinv);
+    V px_lo = grad_x<V, R>(x, c.mt, i, j, k, c.dxinv);
+    V py_hi = grad_y<V, R>(x, c.mt, i, j + 1, k, c.dyinv);
+    V py_lo = grad_y<V, R>(x, c.mt, i, j, k, c.dyinv);
+    V pz_hi = grad_z<V, R>(x, c.mt, i, j, k + 1, c.dxinv, c.dyinv);
+    V pz_lo = grad_z<V, R>(x, c.mt, i, j, k, c.dxinv, c.dyinv);
     const V zero = zero_of<V>();
     if (k == 0) { pz_lo = zero; }
     if ((c.zf & kXlo) && i == 0) { px_lo = zero; }
@@ -274,10 +307,10 @@ __device__ __forceinline__ V apply_cell(const X& x, const OpCtx<R, Z, F>& c, int
         V fhi = zero, flo = zero;
         if (!(k + 1 == c.nz && !c.top_open)) {
             fhi = c.az_mesh(i, j, k + 1) * c.dcross(i, j, k + 1)
-                  * z_face_cross<V, R>(x, c.zp, i, j, k + 1, c.dxinv, c.dyinv);
+                  * z_face_cross<V, R>(x, c.mt, i, j, k + 1, c.dxinv, c.dyinv);
         }
         if (!(k == 0)) {
-            flo = c.az_mesh(i, j, k) * c.dcross(i, j, k) * z_face_cross<V, R>(x, c.zp, i, j, k, c.dxinv, c.dyinv);
+            flo = c.az_mesh(i, j, k) * c.dcross(i, j, k) * z_face_cross<V, R>(x, c.mt, i, j, k, c.dxinv, c.dyinv);
         }
         y = y + (fhi - flo) * c.dzinv / Jc;
     }

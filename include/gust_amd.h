@@ -253,6 +253,52 @@ int gust_face_divergence(const gust_geometry* geom, const gust_array4* ax, const
                          const gust_array4* az, const gust_array4* detJ, const gust_array4* fx,
                          const gust_array4* fy, const gust_array4* fz, gust_array4* div, void* stream);
 
+/* ---- Mass flux on the terrain-following C-grid (Source/MassFlux.H) ------
+ * u on x-faces, v on y-faces, w and Omega on z-faces, p on cells; z_nd the
+ * node heights with one ghost node; the mesh's ax, ay, az, detJ. Ghost
+ * points that an operator reads are the caller's to fill, as in the
+ * reference. */
+
+/* Omega = w - z_x u_f - z_y v_f on the valid z-faces grown by ngrow (0|1);
+ * u, v need ngrow + 1 ghosts, w and omega ngrow.
+ * Replaces: void omega_from_w(const MultiFab& u, v, w, const MultiFab& z_nd,
+ *   const Mesh&, const Geometry&, MultiFab& omega, int ngrow) --
+ *   Source/MassFlux.H:49-53, Source/MassFlux.cpp:51-74. */
+int gust_omega_from_w(const gust_geometry* geom, const gust_array4* u, const gust_array4* v, const gust_array4* w,
+                      const gust_array4* z_nd, gust_array4* omega, int32_t ngrow, void* stream);
+
+/* w = Omega + z_x u_f + z_y v_f on the valid z-faces (u, v: one ghost).
+ * Replaces: void w_from_omega(...) -- Source/MassFlux.H:55-58,
+ *   Source/MassFlux.cpp:76-99. */
+int gust_w_from_omega(const gust_geometry* geom, const gust_array4* u, const gust_array4* v,
+                      const gust_array4* omega, const gust_array4* z_nd, gust_array4* w, void* stream);
+
+/* F_x = a_x u, F_y = a_y v, F_z = Omega on the valid faces and one ghost
+ * face (u, v: two ghosts, w: one; F: one).
+ * Replaces: void transport_fluxes(const MultiFab& u, v, w, z_nd, const Mesh&,
+ *   const Geometry&, Array<MultiFab,3>& F) -- Source/MassFlux.H:60-66,
+ *   Source/MassFlux.cpp:101-127. */
+int gust_transport_fluxes(const gust_geometry* geom, const gust_array4* u, const gust_array4* v,
+                          const gust_array4* w, const gust_array4* z_nd, const gust_array4* ax,
+                          const gust_array4* ay, gust_array4* Fx, gust_array4* Fy, gust_array4* Fz, void* stream);
+
+/* div = face_divergence(u, v, Omega) on the valid cells (u, v: one ghost).
+ * Replaces: void velocity_divergence(const MultiFab& u, v, w, z_nd,
+ *   const Mesh&, const Geometry&, MultiFab& div) -- Source/MassFlux.H:68-72,
+ *   Source/MassFlux.cpp:129-136. */
+int gust_velocity_divergence(const gust_geometry* geom, const gust_array4* u, const gust_array4* v,
+                             const gust_array4* w, const gust_array4* z_nd, const gust_array4* ax,
+                             const gust_array4* ay, const gust_array4* az, const gust_array4* detJ,
+                             gust_array4* div, void* stream);
+
+/* (G p)_x, (G p)_y, (G p)_z on the valid faces (p: one ghost; detJ: one).
+ * Replaces: void pressure_gradient(const MultiFab& p, const MultiFab& z_nd,
+ *   const Mesh&, const Geometry&, const Array<MultiFab*,3>& Gp) --
+ *   Source/MassFlux.H:74-76, Source/MassFlux.cpp:138-179. */
+int gust_pressure_gradient(const gust_geometry* geom, const gust_array4* p, const gust_array4* z_nd,
+                           const gust_array4* detJ, const gust_array4* ax, const gust_array4* ay, gust_array4* gx,
+                           gust_array4* gy, gust_array4* gz, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -89,6 +89,20 @@ int oracle_poisson_fluxes(oa4* fx, oa4* fy, oa4* fz, oa4* phi, const oa4* zp, co
 void oracle_face_divergence(oa4* div, const oa4* fx, const oa4* fy, const oa4* fz, const oa4* ax, const oa4* ay,
                             const oa4* az, const oa4* J, const ogeom* g);
 
+/* Mass-flux operators of the terrain-following C-grid (oracle/massflux.c,
+ * Source/MassFlux.cpp). z is z_phys_nd; views are read at the ghost points
+ * the reference's operators read (the caller fills them). */
+void oracle_omega_from_w(oa4* omega, const oa4* u, const oa4* v, const oa4* w, const oa4* z, const ogeom* g,
+                         int ngrow);
+void oracle_w_from_omega(oa4* w, const oa4* u, const oa4* v, const oa4* omega, const oa4* z, const ogeom* g);
+void oracle_transport_fluxes(oa4* Fx, oa4* Fy, oa4* Fz, const oa4* u, const oa4* v, const oa4* w, const oa4* z,
+                             const oa4* ax, const oa4* ay, const ogeom* g);
+/* omega_scratch: valid z-faces, receives Omega. */
+void oracle_velocity_divergence(oa4* div, const oa4* u, const oa4* v, const oa4* w, const oa4* z, const oa4* ax,
+                                const oa4* ay, const oa4* az, const oa4* J, const ogeom* g, oa4* omega_scratch);
+void oracle_pressure_gradient(oa4* gx, oa4* gy, oa4* gz, const oa4* p, const oa4* z, const oa4* J, const oa4* ax,
+                              const oa4* ay, const ogeom* g);
+
 #ifdef __cplusplus
 }
 #endif

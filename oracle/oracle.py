@@ -66,6 +66,15 @@ def lib():
         L.oracle_face_divergence.argtypes = [P(OA4), P(OA4), P(OA4), P(OA4), P(OA4), P(OA4), P(OA4), P(OA4),
                                              P(OGeom)]
         L.oracle_face_divergence.restype = None
+        V = P(OA4)
+        L.oracle_omega_from_w.argtypes = [V, V, V, V, V, P(OGeom), C.c_int]
+        L.oracle_w_from_omega.argtypes = [V, V, V, V, V, P(OGeom)]
+        L.oracle_transport_fluxes.argtypes = [V, V, V, V, V, V, V, V, V, P(OGeom)]
+        L.oracle_velocity_divergence.argtypes = [V, V, V, V, V, V, V, V, V, P(OGeom), V]
+        L.oracle_pressure_gradient.argtypes = [V, V, V, V, V, V, V, V, P(OGeom)]
+        for f in ("oracle_omega_from_w", "oracle_w_from_omega", "oracle_transport_fluxes",
+                  "oracle_velocity_divergence", "oracle_pressure_gradient"):
+            getattr(L, f).restype = None
         _lib = L
     return _lib
 
@@ -185,3 +194,64 @@ def face_divergence(F, F_begin, areas, areas_begin, J, J_begin, geom):
                                  _v(areas[0], areas_begin), _v(areas[1], areas_begin), _v(areas[2], areas_begin),
                                  _v(J, J_begin), geom)
     return div[0]
+
+
+# --- mass flux (oracle/massflux.c) -------------------------------------------------------------------------
+
+def _shape_at(geom, loc, grow=(0, 0, 0)):
+    n = [geom.hi[d] - geom.lo[d] + 1 + ((loc >> d) & 1) + 2 * grow[d] for d in range(3)]
+    return (1, n[2], n[1], n[0])
+
+
+def _begin_at(geom, grow=(0, 0, 0)):
+    return tuple(geom.lo[d] - grow[d] for d in range(3))
+
+
+def omega_from_w(u, v, w, vel_begin, z, z_begin, geom, ngrow=0):
+    """Source/MassFlux.cpp:51-74: Omega on the valid z-faces grown by ngrow;
+    returns (array, begin)."""
+    g3 = (ngrow, ngrow, ngrow)
+    om = np.zeros(_shape_at(geom, 4, g3))
+    b = _begin_at(geom, g3)
+    lib().oracle_omega_from_w(_v(om, b), _v(u, vel_begin), _v(v, vel_begin), _v(w, vel_begin), _v(z, z_begin),
+                              geom, ngrow)
+    return om[0], b
+
+
+def w_from_omega(u, v, omega, vel_begin, omega_begin, z, z_begin, geom):
+    """Source/MassFlux.cpp:76-99: w on the valid z-faces (nz+1, ny, nx)."""
+    w = np.zeros(_shape_at(geom, 4))
+    lib().oracle_w_from_omega(_v(w, tuple(geom.lo)), _v(u, vel_begin), _v(v, vel_begin), _v(omega, omega_begin),
+                              _v(z, z_begin), geom)
+    return w[0]
+
+
+def transport_fluxes(u, v, w, vel_begin, z, z_begin, ax, ay, area_begin, geom):
+    """Source/MassFlux.cpp:101-127: (Fx, Fy, Fz) on the valid faces and one
+    ghost face, each with begin = lo - 1."""
+    g1 = (1, 1, 1)
+    F = [np.zeros(_shape_at(geom, 1 << d, g1)) for d in range(3)]
+    b = _begin_at(geom, g1)
+    lib().oracle_transport_fluxes(_v(F[0], b), _v(F[1], b), _v(F[2], b), _v(u, vel_begin), _v(v, vel_begin),
+                                  _v(w, vel_begin), _v(z, z_begin), _v(ax, area_begin), _v(ay, area_begin), geom)
+    return F[0][0], F[1][0], F[2][0], b
+
+
+def velocity_divergence(u, v, w, vel_begin, z, z_begin, ax, ay, az, area_begin, J, J_begin, geom):
+    """Source/MassFlux.cpp:129-136: D(U) on the valid cells (nz, ny, nx)."""
+    div = np.zeros(_shape_at(geom, 0))
+    om = np.zeros(_shape_at(geom, 4))
+    lo = tuple(geom.lo)
+    lib().oracle_velocity_divergence(_v(div, lo), _v(u, vel_begin), _v(v, vel_begin), _v(w, vel_begin),
+                                     _v(z, z_begin), _v(ax, area_begin), _v(ay, area_begin), _v(az, area_begin),
+                                     _v(J, J_begin), geom, _v(om, lo))
+    return div[0]
+
+
+def pressure_gradient(p, p_begin, z, z_begin, J, J_begin, ax, ay, area_begin, geom):
+    """Source/MassFlux.cpp:138-179: (Gx, Gy, Gz) on the valid faces."""
+    G = [np.zeros(_shape_at(geom, 1 << d)) for d in range(3)]
+    lo = tuple(geom.lo)
+    lib().oracle_pressure_gradient(_v(G[0], lo), _v(G[1], lo), _v(G[2], lo), _v(p, p_begin), _v(z, z_begin),
+                                   _v(J, J_begin), _v(ax, area_begin), _v(ay, area_begin), geom)
+    return G[0][0], G[1][0], G[2][0]

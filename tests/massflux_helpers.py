@@ -1,0 +1,159 @@
+"""Shared helpers of the mass-flux tests: the hill mesh of
+Tests/Unit_Tests/test_MassFlux.cpp:26-36 (16x12x8 << m cells over 1000 x 800 m,
+periodic sides, a 120 m hill), point positions as the mean of a location's
+nodes (Tests/Unit_Tests/TransportTestHelpers.H:80-94), a smooth manufactured
+flow with its exact divergence and pressure gradient, and the reference test's
+random fields and control-volume weighted dot product (test_MassFlux.cpp:137-187)."""
+from __future__ import annotations
+
+import numpy as np
+
+from ablsteadyrans_amd import mesh as M
+from oracle import oracle as orc
+
+LX, LY, HILL, TOP = 1000.0, 800.0, 120.0, 600.0
+B1 = (-1, -1, -1)
+
+
+def hill_mesh(m: int, smooth_ghosts: bool = True, flat: bool = False):
+    n = (16 << m, 12 << m, 8 << m)
+    lo, hi = (0.0, 0.0, 0.0), (LX, LY, TOP)
+    geom = M.geometry(n, lo, hi, (1, 1))
+    zl = M.make_zlevels(n[2], TOP, dz0=TOP / n[2] * 0.5, stretch_ratio=0.0)
+    t = M.Terrain(kind="flat") if flat else M.Terrain(kind="hills", hills=[(500.0, 400.0, HILL, 180.0)])
+    mesh = M.build_mesh(geom, lo, hi, zl, t)
+    if smooth_ghosts:
+        # test_MassFlux.cpp:35 / TransportTestHelpers.H:59-72: cubic ghost nodes
+        Z = mesh.z_nd
+        nz = n[2]
+        Z[0] = 3 * Z[1] - 3 * Z[2] + Z[3]
+        Z[nz + 2] = 3 * Z[nz + 1] - 3 * Z[nz] + Z[nz - 1]
+    return mesh
+
+
+def ogeom(mesh):
+    g = mesh.geom
+    return orc.geom_c(g.lo, g.hi, g.periodic, g.dx)
+
+
+def positions(mesh, loc: int, grow: int = 1):
+    """(X, Y, Z) of every point of a field at loc (bit d set: nodal along d),
+    valid points and `grow` ghost layers; shape (nz', ny', nx')."""
+    nx, ny, nz = mesh.geom.n
+    dx, dy = mesh.geom.dx[0], mesh.geom.dx[1]
+    plo = mesh.prob_lo
+    nd = [(loc >> d) & 1 for d in range(3)]
+    ii = np.arange(-grow, nx + nd[0] + grow)
+    jj = np.arange(-grow, ny + nd[1] + grow)
+    kk = np.arange(-grow, nz + nd[2] + grow)
+    x = plo[0] + (ii + (0.0 if nd[0] else 0.5)) * dx
+    y = plo[1] + (jj + (0.0 if nd[1] else 0.5)) * dy
+    Zn = mesh.z_nd   # [k+1, j+1, i+1]
+    z = np.zeros((kk.size, jj.size, ii.size))
+    cnt = 0
+    for c in range(1 if nd[2] else 2):
+        for b in range(1 if nd[1] else 2):
+            for a in range(1 if nd[0] else 2):
+                z += Zn[np.ix_(kk + 1 + c, jj + 1 + b, ii + 1 + a)]
+                cnt += 1
+    z /= cnt
+    X = np.broadcast_to(x[None, None, :], z.shape).copy()
+    Y = np.broadcast_to(y[None, :, None], z.shape).copy()
+    return X, Y, z
+
+
+class Flow:
+    """u = 1 + 0.3 s c c, v = 0.2 c s c, w = 0.1 c c s and p = 50 c c c (s, c:
+    sine and cosine of a x, b y, c z), periodic over the tile in x and y."""
+
+    def __init__(self):
+        self.a, self.b, self.c = 2 * np.pi / LX, 2 * np.pi / LY, np.pi / TOP
+
+    def _t(self, X, Y, Z):
+        return (np.sin(self.a * X), np.cos(self.a * X), np.sin(self.b * Y), np.cos(self.b * Y), np.sin(self.c * Z),
+                np.cos(self.c * Z))
+
+    def vel(self, d, X, Y, Z):
+        sx, cx, sy, cy, sz, cz = self._t(X, Y, Z)
+        if d == 0:
+            return 1.0 + 0.3 * sx * cy * cz
+        if d == 1:
+            return 0.2 * cx * sy * cz
+        return 0.1 * cx * cy * sz
+
+    def div(self, X, Y, Z):
+        sx, cx, sy, cy, sz, cz = self._t(X, Y, Z)
+        return (0.3 * self.a + 0.2 * self.b + 0.1 * self.c) * cx * cy * cz
+
+    def p(self, X, Y, Z):
+        sx, cx, sy, cy, sz, cz = self._t(X, Y, Z)
+        return 50.0 * cx * cy * cz
+
+    def grad(self, d, X, Y, Z):
+        sx, cx, sy, cy, sz, cz = self._t(X, Y, Z)
+        if d == 0:
+            return -50.0 * self.a * sx * cy * cz
+        if d == 1:
+            return -50.0 * self.b * cx * sy * cz
+        return -50.0 * self.c * cx * cy * sz
+
+
+def velocity(mesh, flow, grow=1):
+    """u, v, w with `grow` ghost layers, begin (-grow,)*3, shaped (1, ...)."""
+    return [flow.vel(d, *positions(mesh, 1 << d, grow))[None].copy() for d in range(3)]
+
+
+def pressure(mesh, flow, grow=1):
+    return flow.p(*positions(mesh, 0, grow))[None].copy()
+
+
+def periodic_fill(a, grow, nodal):
+    """FillBoundary along x and y of a (1, nz', ny', nx') array with `grow`
+    ghosts: ghosts take the periodic images (a nodal axis: the image of
+    valid face i is i + n with the hi face the copy of the lo face)."""
+    _, _, ey, ex = a.shape
+    for axis, (e, nd) in ((3, (ex, nodal[0])), (2, (ey, nodal[1]))):
+        n = e - 2 * grow - nd
+        idx = np.arange(e) - grow
+        src = np.mod(idx, n) + grow
+        a[...] = np.take(a, src, axis=axis)
+    return a
+
+
+def random_field(mesh, loc, seed, margin, grow=1):
+    """Random values in [-1, 1) on the valid points, zero within `margin`
+    points of the ground and the top, zero z ghosts, periodic x/y images
+    (test_MassFlux.cpp:137-160)."""
+    nx, ny, nz = mesh.geom.n
+    nd = [(loc >> d) & 1 for d in range(3)]
+    rng = np.random.default_rng(seed)
+    v = rng.uniform(-1, 1, size=(nz + nd[2], ny + nd[1], nx + nd[0]))
+    kk = np.arange(nz + nd[2])
+    top = (nz - kk) if nd[2] else (nz - 1 - kk)
+    v[(kk < margin) | (top < margin)] = 0.0
+    a = np.zeros((1, nz + nd[2] + 2 * grow, ny + nd[1] + 2 * grow, nx + nd[0] + 2 * grow))
+    a[0, grow:grow + v.shape[0], grow:grow + v.shape[1], grow:grow + v.shape[2]] = v
+    if nd[0]:   # the hi face is the lo face's copy (OverrideSync)
+        a[0, :, :, grow + nx] = a[0, :, :, grow]
+    if nd[1]:
+        a[0, :, grow + ny, :] = a[0, :, grow, :]
+    return periodic_fill(a, grow, nd)
+
+
+def weighted_dot(mesh, a_valid, b_valid, loc):
+    """sum over owned points of V a b, V = J dx dy dzeta (a face: the mean of
+    its two cells); periodic duplicate faces counted once."""
+    nx, ny, nz = mesh.geom.n
+    dx, dy, dz = mesh.geom.dx
+    J = mesh.detJ   # [k+1, j+1, i+1]
+    nd = [(loc >> d) & 1 for d in range(3)]
+    ii, jj, kk = np.arange(nx), np.arange(ny), np.arange(nz + nd[2])
+    Jc = J[np.ix_(kk + 1, jj + 1, ii + 1)]
+    for d in range(3):
+        if nd[d]:
+            off = [1, 1, 1]
+            off[d] = 0
+            Jm = J[np.ix_(kk + off[2], jj + off[1], ii + off[0])]
+            Jc = 0.5 * (Jm + Jc)
+    own = (slice(0, nz + nd[2]), slice(0, ny), slice(0, nx))
+    return float(np.sum(Jc * dx * dy * dz * a_valid[own] * b_valid[own]))

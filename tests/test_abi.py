@@ -89,3 +89,44 @@ def test_array4_view_matches_amrex_indexing():
     for (i, j, k, n) in ((1, -2, -1, 0), (4, 2, 3, 2), (6, 4, 5, 1)):
         off = (i - a.begin[0]) + (j - a.begin[1]) * a.jstride + (k - a.begin[2]) * a.kstride + n * a.nstride
         assert base[off] == f.data[n, k - a.begin[2], j - a.begin[1], i - a.begin[0]]
+
+
+class _HostMesh:
+    """The fields of a DeviceMesh as host FABs (refusal tests only)."""
+
+    def __init__(self, nx=6, ny=5, nz=4):
+        import ablsteadyrans_amd as G
+        self.geom = G.Geometry.of_size(nx, ny, nz)
+        b = (-1, -1, -1)
+        f = lambda shape: G.Fab(np.zeros((1,) + shape), b)
+        self.z_nd = f((nz + 3, ny + 3, nx + 3))
+        self.detJ = f((nz + 2, ny + 2, nx + 2))
+        self.ax = f((nz + 2, ny + 2, nx + 3))
+        self.ay = f((nz + 2, ny + 3, nx + 2))
+        self.az = f((nz + 3, ny + 2, nx + 2))
+
+
+def test_mass_flux_refusals_before_any_device_call():
+    import ablsteadyrans_amd as G
+    from ablsteadyrans_amd import massflux as MF
+    m = _HostMesh()
+    g = m.geom
+    u1, v1, w1 = (_host_fab(g, loc, 1, 1) for loc in (G.XFACE, G.YFACE, G.ZFACE))
+    u0 = _host_fab(g, G.XFACE, 1, 0)
+    om = _host_fab(g, G.ZFACE, 1, 0)
+    with pytest.raises(G.GustError, match="gust: omega_from_w: ngrow must be 0 or 1"):
+        MF.omega_from_w(m, u1, v1, w1, om, ngrow=2, stream=0)
+    with pytest.raises(G.GustError, match="gust: omega_from_w: u needs at least 1 ghost cells"):
+        MF.omega_from_w(m, u0, v1, w1, om, stream=0)
+    with pytest.raises(G.GustError, match="gust: omega_from_w: omega needs at least 1 ghost cells"):
+        MF.omega_from_w(m, _host_fab(g, G.XFACE, 1, 2), _host_fab(g, G.YFACE, 1, 2), w1, om, ngrow=1, stream=0)
+    with pytest.raises(G.GustError, match="gust: transport_fluxes: u needs at least 2 ghost cells"):
+        MF.transport_fluxes(m, u1, v1, w1, (u1, v1, w1), stream=0)
+    with pytest.raises(G.GustError, match="gust: velocity_divergence: div is not on the mesh's BoxArray"):
+        MF.velocity_divergence(m, u1, v1, w1, _host_fab(G.Geometry.of_size(3, 3, 3), G.CELL, 1, 0), stream=0)
+    p0 = _host_fab(g, G.CELL, 1, 0)
+    with pytest.raises(G.GustError, match="gust: pressure_gradient: p needs at least 1 ghost cells"):
+        MF.pressure_gradient(m, p0, (u0, _host_fab(g, G.YFACE, 1, 0), om), stream=0)
+    m.z_nd = G.Fab(np.zeros((1, 6, 7, 8)), (0, 0, 0))
+    with pytest.raises(G.GustError, match="z_nd must be the mesh's node heights"):
+        MF.w_from_omega(m, u1, v1, om, om, stream=0)
