@@ -102,6 +102,20 @@ class PoissonResultC(C.Structure):
     ]
 
 
+class PressureCorrectionResultC(C.Structure):
+    """``gust_pressure_correction_result``."""
+
+    _fields_ = [
+        ("poisson", PoissonResultC),
+        ("divergence_before", C.c_double),
+        ("divergence_after", C.c_double),
+        ("max_dp", C.c_double),
+        ("max_du", C.c_double),
+        ("max_dv", C.c_double),
+        ("max_dw", C.c_double),
+    ]
+
+
 def _load() -> C.CDLL:
     # One HIP runtime per process: torch ships its own libamdhip64 and the
     # device tensors, streams and events handed to the library come from it.
@@ -164,6 +178,12 @@ _sig = {
     "gust_transport_fluxes": (C.c_int, [_P(GeometryC)] + [_P(Array4)] * 9 + [C.c_void_p]),
     "gust_velocity_divergence": (C.c_int, [_P(GeometryC)] + [_P(Array4)] * 9 + [C.c_void_p]),
     "gust_pressure_gradient": (C.c_int, [_P(GeometryC)] + [_P(Array4)] * 8 + [C.c_void_p]),
+    "gust_pressure_correction": (C.c_int, [C.c_void_p, _P(GeometryC)] + [_P(Array4)] * 12 +
+                                 [_P(C.c_int32), C.c_double, _P(Array4), C.c_int32, _P(PressureCorrectionResultC),
+                                  C.c_void_p]),
+    "gust_local_dtau": (C.c_int, [_P(GeometryC)] + [_P(Array4)] * 6 + [C.c_double, _P(Array4), C.c_void_p]),
+    "gust_ser_cfl": (C.c_int, [C.c_double] * 6 + [_P(C.c_double)]),
+    "gust_face_dtau": (C.c_int, [_P(GeometryC)] + [_P(Array4)] * 4 + [C.c_void_p]),
 }
 for _name, (_res, _args) in _sig.items():
     _f = getattr(lib, _name)

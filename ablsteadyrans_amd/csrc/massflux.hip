@@ -17,89 +17,13 @@
 // CPU build's bits. velocity_divergence is one kernel: the two z-face Omegas
 // of a cell are formed in registers from the same expression
 // omega_from_w evaluates, then combined as face_divergence combines them.
-#include "common.hpp"
+#include "cgrid.hpp"
 
 namespace gust_amd {
 
 namespace {
 
-constexpr int kThreads = 256;
-
-inline int blocks_for(int64_t n)
-{
-    return int(std::max<int64_t>(1, std::min<int64_t>((n + kThreads - 1) / kThreads, 16384)));
-}
-
-struct Box3 {
-    int lo0, lo1, lo2, n0, n1, n2;
-    __host__ __device__ int64_t npts() const { return int64_t(n0) * n1 * n2; }
-    __device__ __forceinline__ void at(int64_t t, int& i, int& j, int& k) const
-    {
-        i = lo0 + int(t % n0);
-        const int64_t r = t / n0;
-        j = lo1 + int(r % n1);
-        k = lo2 + int(r / n1);
-    }
-};
-
-Box3 box3(const IBox& b)
-{
-    return Box3{b.lo[0], b.lo[1], b.lo[2], b.len(0), b.len(1), b.len(2)};
-}
-
-using V = DView<double>;
-using CV = DView<const double>;
-
-__device__ __forceinline__ double slope_x(const CV& z, int i, int j, int k, double dxinv)
-{
-    double s = 0.0;
-    s += z(i + 1, j, k) - z(i, j, k);
-    s += z(i + 1, j + 1, k) - z(i, j + 1, k);
-    return (1.0 * (s * 0.5)) * dxinv;
-}
-
-__device__ __forceinline__ double slope_y(const CV& z, int i, int j, int k, double dyinv)
-{
-    double s = 0.0;
-    s += z(i, j + 1, k) - z(i, j, k);
-    s += z(i + 1, j + 1, k) - z(i + 1, j, k);
-    return (1.0 * (s * 0.5)) * dyinv;
-}
-
-__device__ __forceinline__ double u_at_zface(const CV& u, int i, int j, int k)
-{
-    double s = 0.0;
-    s += u(i, j, k - 1);
-    s += u(i + 1, j, k - 1);
-    s += u(i, j, k);
-    s += u(i + 1, j, k);
-    return s * 0.25;
-}
-
-__device__ __forceinline__ double v_at_zface(const CV& v, int i, int j, int k)
-{
-    double s = 0.0;
-    s += v(i, j, k - 1);
-    s += v(i, j + 1, k - 1);
-    s += v(i, j, k);
-    s += v(i, j + 1, k);
-    return s * 0.25;
-}
-
-__device__ __forceinline__ double omega_at(const CV& u, const CV& v, const CV& w, const CV& z, int i, int j, int k,
-                                           double dxinv, double dyinv)
-{
-    return w(i, j, k) - slope_x(z, i, j, k, dxinv) * u_at_zface(u, i, j, k) -
-           slope_y(z, i, j, k, dyinv) * v_at_zface(v, i, j, k);
-}
-
-__device__ __forceinline__ double J_at_face(const CV& J, int i, int j, int k, int d)
-{
-    double s = 0.0;
-    s += J(i - (d == 0), j - (d == 1), k - (d == 2));
-    s += J(i, j, k);
-    return s * 0.5;
-}
+using namespace cgrid;
 
 template <int D>
 __device__ __forceinline__ double slope_dp(const CV& z, const CV& p, int i, int j, int k, int klo, double dxinv,
@@ -110,14 +34,10 @@ __device__ __forceinline__ double slope_dp(const CV& z, const CV& p, int i, int 
     return s * dp * dzinv;
 }
 
-#define GRID_LOOP(b)                                                                                             \
-    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < (b).npts();                            \
-         t += int64_t(gridDim.x) * blockDim.x)
-
 __global__ __launch_bounds__(kThreads) void omega_kernel(Box3 b, CV u, CV v, CV w, CV z, V om, double dxinv,
                                                           double dyinv)
 {
-    GRID_LOOP(b)
+    GUST_GRID_LOOP(b)
     {
         int i, j, k;
         b.at(t, i, j, k);
@@ -128,7 +48,7 @@ __global__ __launch_bounds__(kThreads) void omega_kernel(Box3 b, CV u, CV v, CV 
 __global__ __launch_bounds__(kThreads) void w_kernel(Box3 b, CV u, CV v, CV om, CV z, V w, double dxinv,
                                                       double dyinv)
 {
-    GRID_LOOP(b)
+    GUST_GRID_LOOP(b)
     {
         int i, j, k;
         b.at(t, i, j, k);
@@ -139,7 +59,7 @@ __global__ __launch_bounds__(kThreads) void w_kernel(Box3 b, CV u, CV v, CV om, 
 
 __global__ __launch_bounds__(kThreads) void area_flux_kernel(Box3 b, CV a, CV q, V f)
 {
-    GRID_LOOP(b)
+    GUST_GRID_LOOP(b)
     {
         int i, j, k;
         b.at(t, i, j, k);
@@ -151,7 +71,7 @@ __global__ __launch_bounds__(kThreads) void divergence_kernel(Box3 b, CV u, CV v
                                                                CV J, V div, double dxinv, double dyinv,
                                                                double dzinv)
 {
-    GRID_LOOP(b)
+    GUST_GRID_LOOP(b)
     {
         int i, j, k;
         b.at(t, i, j, k);
@@ -168,7 +88,7 @@ template <int D>
 __global__ __launch_bounds__(kThreads) void grad_h_kernel(Box3 b, CV p, CV z, CV J, CV a, V g, int klo, double dxinv,
                                                            double dyinv, double dzinv)
 {
-    GRID_LOOP(b)
+    GUST_GRID_LOOP(b)
     {
         int i, j, k;
         b.at(t, i, j, k);
@@ -185,26 +105,12 @@ __global__ __launch_bounds__(kThreads) void grad_h_kernel(Box3 b, CV p, CV z, CV
 
 __global__ __launch_bounds__(kThreads) void grad_z_kernel(Box3 b, CV p, CV J, V g, double dzinv)
 {
-    GRID_LOOP(b)
+    GUST_GRID_LOOP(b)
     {
         int i, j, k;
         b.at(t, i, j, k);
         g(i, j, k) = (p(i, j, k) - p(i, j, k - 1)) * dzinv / J_at_face(J, i, j, k, 2);
     }
-}
-
-void check_geom(const char* op, const gust_geometry* g)
-{
-    if (!g) { GUST_FAIL("gust: %s: null geometry", op); }
-    for (int d = 0; d < 3; ++d) {
-        if (g->hi[d] < g->lo[d] || !(g->dx[d] > 0)) { GUST_FAIL("gust: %s: invalid geometry", op); }
-    }
-}
-
-void check_mesh_field(const char* op, const char* what, const gust_array4* v, const gust_geometry& g, int loc,
-                      int ngrow)
-{
-    check_field(op, what, v, g, loc, ngrow);
 }
 
 }   // namespace

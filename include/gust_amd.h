@@ -299,6 +299,59 @@ int gust_pressure_gradient(const gust_geometry* geom, const gust_array4* p, cons
                            const gust_array4* detJ, const gust_array4* ax, const gust_array4* ay, gust_array4* gx,
                            gust_array4* gy, gust_array4* gz, void* stream);
 
+/* ---- SIMPLEC steps (Source/Simplec.H) ---------------------------------- */
+
+/* Source/Simplec.H:116-123 (PressureCorrectionResult). */
+typedef struct gust_pressure_correction_result {
+    gust_poisson_result poisson;
+    double divergence_before;   /* max |div u*| */
+    double divergence_after;    /* max |div u| after the correction */
+    double max_dp;              /* max |alpha_p p'| */
+    double max_du, max_dv, max_dw;
+} gust_pressure_correction_result;
+
+/* The pressure correction on (u, v, w, p) in place: beta = 1/d on faces with
+ * d > 0 (keep = 1; d = 0: beta = 1, keep = 0; d must be finite and >= 0),
+ * rhs = div u* (- source), the Poisson solve for phi with fluxes
+ * F = -beta grad phi, p += alpha_p phi, u, v, Omega += keep F, w from Omega,
+ * seams and periodic ghosts of u, v, w refreshed, div after. u, v, w and p
+ * need one ghost (filled). solver: a gust_poisson on the same mesh (its
+ * tolerance is the correction's). source may be null. Aborts (returns
+ * nonzero) on a net source in the singular case unless project_net_source.
+ * Replaces: PressureCorrectionResult pressure_correction(State& q, const
+ *   PredictorFields& d, PoissonSolver&, const PoissonBC&, Real alpha_p, const
+ *   Mesh&, const Geometry&, SimplecWorkspace&, const MultiFab* source, bool
+ *   project_net_source) -- Source/Simplec.H:139-142, Source/Simplec.cpp:134-220. */
+int gust_pressure_correction(gust_poisson* solver, const gust_geometry* geom, gust_array4* u, gust_array4* v,
+                             gust_array4* w, gust_array4* p, const gust_array4* z_nd, const gust_array4* ax,
+                             const gust_array4* ay, const gust_array4* az, const gust_array4* detJ,
+                             const gust_array4* d_u, const gust_array4* d_v, const gust_array4* d_w,
+                             const int32_t* bc, double alpha_p, const gust_array4* source,
+                             int32_t project_net_source, gust_pressure_correction_result* result, void* stream);
+
+/* ---- Pseudo-time (Source/PseudoTime.H, Source/Predictor.H) ------------- */
+
+/* dtau = cfl / max(|u_c|/dx + |v_c|/dy + |Omega_c|/(J dzeta) + 2 nu (1/dx^2 +
+ * 1/dy^2), 1e-12) on the valid cells; u, v need one ghost; nu may be null.
+ * Replaces: void local_dtau(const State& q, const MultiFab* nu, const Mesh&,
+ *   const Geometry&, Real cfl, MultiFab& dtau) -- Source/PseudoTime.H:35-36,
+ *   Source/PseudoTime.cpp:12-51. */
+int gust_local_dtau(const gust_geometry* geom, const gust_array4* u, const gust_array4* v, const gust_array4* w,
+                    const gust_array4* z_nd, const gust_array4* detJ, const gust_array4* nu, double cfl,
+                    gust_array4* dtau, void* stream);
+
+/* min(cfl_max, growth cfl, cfl0 r^-p) with the reference's guards (host).
+ * Replaces: Real ser_cfl(Real cfl, Real r, Real cfl0, Real cfl_max, Real p,
+ *   Real growth) -- Source/PseudoTime.H:47-48, Source/PseudoTime.cpp:53-61. */
+int gust_ser_cfl(double cfl, double r, double cfl0, double cfl_max, double p, double growth, double* out);
+
+/* Face means of a cell dtau (one-sided on non-periodic boundary faces);
+ * refuses a dtau_cc that is not finite and > 0 everywhere (synchronises).
+ * Replaces: void face_dtau(const MultiFab& dtau_cc, const Mesh&, const
+ *   Geometry&, Array<MultiFab,3>& dtau_faces) -- Source/Predictor.cpp:30-77. */
+int gust_face_dtau(const gust_geometry* geom, const gust_array4* dtau_cc, gust_array4* dtau_x, gust_array4* dtau_y,
+                   gust_array4* dtau_z, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -103,6 +103,14 @@ void oracle_velocity_divergence(oa4* div, const oa4* u, const oa4* v, const oa4*
 void oracle_pressure_gradient(oa4* gx, oa4* gy, oa4* gz, const oa4* p, const oa4* z, const oa4* J, const oa4* ax,
                               const oa4* ay, const ogeom* g);
 
+/* Pseudo-time (oracle/pseudotime.c). local_dtau returns -1 for a cfl that is
+ * not finite and > 0; face_dtau returns -1 unless dtau_cc is finite and > 0
+ * everywhere. nu may be null. */
+int oracle_local_dtau(oa4* dtau, const oa4* u, const oa4* v, const oa4* w, const oa4* z, const oa4* J, const oa4* nu,
+                      double cfl, const ogeom* g, oa4* omega_scratch);
+double oracle_ser_cfl(double cfl, double r, double cfl0, double cfl_max, double p, double growth);
+int oracle_face_dtau(oa4* fx, oa4* fy, oa4* fz, const oa4* d, const ogeom* g);
+
 #ifdef __cplusplus
 }
 #endif

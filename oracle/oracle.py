@@ -72,6 +72,10 @@ def lib():
         L.oracle_transport_fluxes.argtypes = [V, V, V, V, V, V, V, V, V, P(OGeom)]
         L.oracle_velocity_divergence.argtypes = [V, V, V, V, V, V, V, V, V, P(OGeom), V]
         L.oracle_pressure_gradient.argtypes = [V, V, V, V, V, V, V, V, P(OGeom)]
+        L.oracle_local_dtau.argtypes = [V, V, V, V, V, V, V, C.c_double, P(OGeom), V]
+        L.oracle_ser_cfl.argtypes = [C.c_double] * 6
+        L.oracle_ser_cfl.restype = C.c_double
+        L.oracle_face_dtau.argtypes = [V, V, V, V, P(OGeom)]
         for f in ("oracle_omega_from_w", "oracle_w_from_omega", "oracle_transport_fluxes",
                   "oracle_velocity_divergence", "oracle_pressure_gradient"):
             getattr(L, f).restype = None
@@ -255,3 +259,36 @@ def pressure_gradient(p, p_begin, z, z_begin, J, J_begin, ax, ay, area_begin, ge
     lib().oracle_pressure_gradient(_v(G[0], lo), _v(G[1], lo), _v(G[2], lo), _v(p, p_begin), _v(z, z_begin),
                                    _v(J, J_begin), _v(ax, area_begin), _v(ay, area_begin), geom)
     return G[0][0], G[1][0], G[2][0]
+
+
+# --- pseudo-time (oracle/pseudotime.c) ---------------------------------------------------------------------
+
+K_MIN_RATE = 1.0e-12
+
+
+def local_dtau(u, v, w, vel_begin, z, z_begin, J, J_begin, geom, cfl, nu=None, nu_begin=None):
+    """Source/PseudoTime.cpp:12-51: dtau on the valid cells (nz, ny, nx)."""
+    lo = tuple(geom.lo)
+    d = np.zeros(_shape_at(geom, 0))
+    om = np.zeros(_shape_at(geom, 4))
+    rc = lib().oracle_local_dtau(_v(d, lo), _v(u, vel_begin), _v(v, vel_begin), _v(w, vel_begin), _v(z, z_begin),
+                                 _v(J, J_begin), _v(nu, nu_begin if nu_begin is not None else lo), float(cfl), geom,
+                                 _v(om, lo))
+    if rc != 0:
+        raise OracleError("local_dtau: cfl must be finite and > 0")
+    return d[0]
+
+
+def ser_cfl(cfl, r, cfl0, cfl_max, p, growth):
+    """Source/PseudoTime.cpp:53-61."""
+    return lib().oracle_ser_cfl(cfl, r, cfl0, cfl_max, p, growth)
+
+
+def face_dtau(dtau_cc, geom):
+    """Source/Predictor.cpp:30-77: (dx, dy, dz) face means of valid dtau (nz, ny, nx)."""
+    lo = tuple(geom.lo)
+    d = np.ascontiguousarray(dtau_cc, dtype=np.float64)[None]
+    F = [np.zeros(_shape_at(geom, 1 << q)) for q in range(3)]
+    if lib().oracle_face_dtau(_v(F[0], lo), _v(F[1], lo), _v(F[2], lo), _v(d, lo), geom) != 0:
+        raise OracleError("face_dtau: dtau_cc must be finite and > 0 everywhere")
+    return F[0][0], F[1][0], F[2][0]

@@ -1,0 +1,139 @@
+"""The pressure correction of Source/Simplec.cpp:134-220 composed from the
+oracle's operators (TEST INFRASTRUCTURE): face_coefficients (87-112), the
+divergence, the Poisson solve done exactly (a sparse direct solve of the
+oracle operator with beta = 1/d; the singular case pinned at one cell and the
+J-weighted means removed as Source/Poisson.cpp:559-567 and 603-611 do), the
+fluxes, and the updates of p, u, v, Omega and w."""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse.linalg as spla
+
+from oracle import oracle as orc
+from tests import poisson_helpers as PH
+
+B = (-1, -1, -1)
+
+
+def beta_keep(d):
+    on = d > 0
+    return np.where(on, 1.0 / np.where(on, d, 1.0), 1.0), on.astype(np.float64)
+
+
+def _embed(mesh_arr, valid):
+    out = np.ones_like(mesh_arr)
+    out[1:-1, 1:-1, 1:-1] = valid
+    return out
+
+
+def periodic_images(a, per, nodal):
+    """FillBoundary of a one-ghost (1, nz', ny', nx') array: every ghost point
+    whose periodic image (x and/or y) is a valid point takes its value."""
+    a = a.copy()
+    _, ez, ey, ex = a.shape
+    nvx, nvy = ex - 2, ey - 2              # valid points along x, y (faces included)
+    nx, ny = nvx - nodal[0], nvy - nodal[1]   # the period
+    K, Jj, I = np.meshgrid(np.arange(ez) - 1, np.arange(ey) - 1, np.arange(ex) - 1, indexing="ij")
+    inside = (I >= 0) & (I < nvx) & (Jj >= 0) & (Jj < nvy) & (K >= 0) & (K < ez - 2)
+    A, Cc = I.copy(), Jj.copy()
+    if per[0]:
+        A = np.where(A < 0, A + nx, np.where(A >= nvx, A - nx, A))
+    if per[1]:
+        Cc = np.where(Cc < 0, Cc + ny, np.where(Cc >= nvy, Cc - ny, Cc))
+    ok = (~inside) & (A >= 0) & (A < nvx) & (Cc >= 0) & (Cc < nvy) & (K >= 0) & (K < ez - 2) & ((A != I) | (Cc != Jj))
+    src = a[0, K[ok] + 1, Cc[ok] + 1, A[ok] + 1]
+    a[0, K[ok] + 1, Jj[ok] + 1, I[ok] + 1] = src
+    return a
+
+
+def pressure_correction(mesh, u, v, w, p, d, bc, alpha_p, source=None):
+    """u, v, w, p: one-ghost (1, ...) arrays (begin -1); d = (d_u, d_v, d_w)
+    valid-face arrays. Returns (u, v, w, p, phi, F, div_before, div_after)."""
+    g = PH.ogeom(mesh)
+    per = mesh.geom.periodic
+    betas, keeps = zip(*(beta_keep(x) for x in d))
+    div0 = orc.velocity_divergence(u, v, w, B, mesh.z_nd, B, mesh.ax, mesh.ay, mesh.az, B, mesh.detJ, B, g)
+    rhs = div0 - (source if source is not None else 0.0)
+    coeffs = (_embed(mesh.ax, betas[0]), _embed(mesh.ay, betas[1]), _embed(mesh.az, betas[2]))
+    A = PH.sparse_operator(mesh, bc, coeffs=coeffs).tocsr()
+    J = mesh.detJ[1:-1, 1:-1, 1:-1]
+    singular = all(b != orc.DIRICHLET for b in bc)
+    r = rhs.ravel().copy()
+    if singular:
+        r -= np.sum(J.ravel() * r) / np.sum(J)
+        A = A.tolil()
+        A[0, :] = 0
+        A[0, 0] = 1.0
+        r[0] = 0.0
+        A = A.tocsr()
+    phi = spla.spsolve(A.tocsc(), r).reshape(rhs.shape)
+    if singular:
+        phi -= np.sum(J * phi) / np.sum(J)
+    F = orc.poisson_fluxes(np.pad(phi, 1)[None].copy(), B, mesh.z_nd, B, g, bc, coeffs=coeffs, coeff_begin=B)
+    p = p.copy()
+    p[0, 1:-1, 1:-1, 1:-1] += alpha_p * phi
+    om, ob = orc.omega_from_w(u, v, w, B, mesh.z_nd, B, g, 0)
+    u, v = u.copy(), v.copy()
+    u[0, 1:-1, 1:-1, 1:-1] += F[0] * keeps[0]
+    v[0, 1:-1, 1:-1, 1:-1] += F[1] * keeps[1]
+    om = om + F[2] * keeps[2]
+    nx, ny, nz = mesh.geom.n
+    if per[0]:   # OverrideSync: the high seam copy takes the low one
+        u[0, 1:-1, 1:-1, nx + 1] = u[0, 1:-1, 1:-1, 1]
+    if per[1]:
+        v[0, 1:-1, ny + 1, 1:-1] = v[0, 1:-1, 1, 1:-1]
+    u = periodic_images(u, per, (1, 0))
+    v = periodic_images(v, per, (0, 1))
+    wn = orc.w_from_omega(u, v, om[None].copy(), B, ob, mesh.z_nd, B, g)
+    w = w.copy()
+    w[0, 1:-1, 1:-1, 1:-1] = wn
+    w = periodic_images(w, per, (0, 0))
+    div1 = orc.velocity_divergence(u, v, w, B, mesh.z_nd, B, mesh.ax, mesh.ay, mesh.az, B, mesh.detJ, B, g)
+    return u, v, w, p, phi, F, div0, div1
+
+
+def random_state(mesh, seed, open_top=False):
+    """A predicted state as test_Simplec.cpp:176-189 draws it: u* in [5, 10),
+    v* in [-2, 2), w* in [-0.5, 0.5), p in [-5, 5), d in [20, 60) on the faces
+    with d_w = 0 on the ground and top rows (their Dirichlet rows; an open
+    top, the pressure's Dirichlet face, keeps its d_w and w); periodic
+    images, even z ghosts. w on the Dirichlet rows is the no-flux w
+    (Omega = 0; 0 on a flat mesh) so no net source enters."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = mesh.geom.n
+    per = mesh.geom.periodic
+
+    def field(loc, lo, hi):
+        nd = [(loc >> q) & 1 for q in range(3)]
+        a = np.zeros((1, nz + nd[2] + 2, ny + nd[1] + 2, nx + nd[0] + 2))
+        a[0, 1:-1, 1:-1, 1:-1] = rng.uniform(lo, hi, size=(nz + nd[2], ny + nd[1], nx + nd[0]))
+        if nd[0] and per[0]:
+            a[0, 1:-1, 1:-1, nx + 1] = a[0, 1:-1, 1:-1, 1]
+        if nd[1] and per[1]:
+            a[0, 1:-1, ny + 1, 1:-1] = a[0, 1:-1, 1, 1:-1]
+        a[0, 0] = a[0, 1]
+        a[0, -1] = a[0, -2]
+        return periodic_images(a, per, nd[:2])
+
+    u = field(1, 5, 10)
+    v = field(2, -2, 2)
+    w = field(4, -0.5, 0.5)
+    p = field(0, -5, 5)
+    d = [rng.uniform(20, 60, size=(nz + (q == 2), ny + (q == 1), nx + (q == 0))) for q in range(3)]
+    d[2][0] = 0.0
+    if not open_top:
+        d[2][-1] = 0.0
+    if per[0]:   # a periodic seam face is one unknown: its two copies agree
+        d[0][:, :, nx] = d[0][:, :, 0]
+    if per[1]:
+        d[1][:, ny, :] = d[1][:, 0, :]
+    g = PH.ogeom(mesh)
+    om0 = np.zeros((1, nz + 1, ny, nx))
+    wnf = orc.w_from_omega(u, v, om0, B, tuple(mesh.geom.lo), mesh.z_nd, B, g)
+    w[0, 1, 1:-1, 1:-1] = wnf[0]
+    if not open_top:
+        w[0, nz + 1, 1:-1, 1:-1] = wnf[nz]
+    w[0, 0] = w[0, 1]
+    w[0, -1] = w[0, -2]
+    w = periodic_images(w, per, (0, 0))
+    return u, v, w, p, d
