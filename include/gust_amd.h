@@ -329,6 +329,27 @@ int gust_pressure_correction(gust_poisson* solver, const gust_geometry* geom, gu
                              const int32_t* bc, double alpha_p, const gust_array4* source,
                              int32_t project_net_source, gust_pressure_correction_result* result, void* stream);
 
+/* ---- The implicit operator A1 (Source/ImplicitOperator.H) -------------- */
+
+/* A1's seven coefficients (GUST_COEF_* components of A at loc, valid
+ * points): first-order bounded upwind convection from the transport fluxes
+ * F (one ghost) plus the principal diffusion with the face mean of the cell
+ * diffusivity g (one ghost); face_mean 0 arithmetic, 1 logarithmic (z-faces).
+ * Replaces: void implicit_coefficients(Loc, const Array<const MultiFab*,3>& F,
+ *   const MultiFab& g, const MultiFab& z_nd, const Mesh&, const Geometry&,
+ *   MultiFab& A, FaceMean) -- Source/ImplicitOperator.H:59-65,
+ *   Source/ImplicitOperator.cpp:95-121. */
+int gust_implicit_coefficients(int32_t loc, const gust_geometry* geom, const gust_array4* Fx, const gust_array4* Fy,
+                               const gust_array4* Fz, const gust_array4* g, const gust_array4* z_nd,
+                               const gust_array4* detJ, int32_t face_mean, gust_array4* A, void* stream);
+
+/* out = A1 q on the valid points at loc (q: one ghost).
+ * Replaces: void apply_implicit(Loc, const MultiFab& A, const MultiFab& q,
+ *   const Mesh&, MultiFab& out) -- Source/ImplicitOperator.H:69-70,
+ *   Source/ImplicitOperator.cpp:123-135. */
+int gust_apply_implicit(int32_t loc, const gust_geometry* geom, const gust_array4* A, const gust_array4* q,
+                        gust_array4* out, void* stream);
+
 /* ---- Pseudo-time (Source/PseudoTime.H, Source/Predictor.H) ------------- */
 
 /* dtau = cfl / max(|u_c|/dx + |v_c|/dy + |Omega_c|/(J dzeta) + 2 nu (1/dx^2 +

@@ -111,6 +111,14 @@ int oracle_local_dtau(oa4* dtau, const oa4* u, const oa4* v, const oa4* w, const
 double oracle_ser_cfl(double cfl, double r, double cfl0, double cfl_max, double p, double growth);
 int oracle_face_dtau(oa4* fx, oa4* fy, oa4* fz, const oa4* d, const ogeom* g);
 
+/* The implicit operator A1 (oracle/implicit.c, Source/ImplicitOperator.cpp).
+ * A: 7 components at loc on the valid points; F: the transport fluxes with
+ * one ghost; g: cell diffusivity with one ghost; logz: the logarithmic
+ * z-face mean of g. Returns -1 for an unknown loc. */
+int oracle_implicit_coefficients(int loc, const oa4* fx, const oa4* fy, const oa4* fz, const oa4* g, const oa4* z,
+                                 const oa4* J, const ogeom* geo, int logz, oa4* A);
+int oracle_apply_implicit(int loc, const oa4* A, const oa4* q, const ogeom* geo, oa4* out);
+
 #ifdef __cplusplus
 }
 #endif

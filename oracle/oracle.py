@@ -76,6 +76,8 @@ def lib():
         L.oracle_ser_cfl.argtypes = [C.c_double] * 6
         L.oracle_ser_cfl.restype = C.c_double
         L.oracle_face_dtau.argtypes = [V, V, V, V, P(OGeom)]
+        L.oracle_implicit_coefficients.argtypes = [C.c_int, V, V, V, V, V, V, P(OGeom), C.c_int, V]
+        L.oracle_apply_implicit.argtypes = [C.c_int, V, V, P(OGeom), V]
         for f in ("oracle_omega_from_w", "oracle_w_from_omega", "oracle_transport_fluxes",
                   "oracle_velocity_divergence", "oracle_pressure_gradient"):
             getattr(L, f).restype = None
@@ -292,3 +294,23 @@ def face_dtau(dtau_cc, geom):
     if lib().oracle_face_dtau(_v(F[0], lo), _v(F[1], lo), _v(F[2], lo), _v(d, lo), geom) != 0:
         raise OracleError("face_dtau: dtau_cc must be finite and > 0 everywhere")
     return F[0][0], F[1][0], F[2][0]
+
+
+# --- implicit operator A1 (oracle/implicit.c) --------------------------------------------------------------
+
+def implicit_coefficients(loc, F, F_begin, g, g_begin, z, z_begin, J, J_begin, geom, logz=False):
+    """Source/ImplicitOperator.cpp:95-121: A (7, nz', ny', nx') at loc (valid points, begin lo)."""
+    A = np.zeros((7,) + _shape_at(geom, loc)[1:])
+    rc = lib().oracle_implicit_coefficients(loc, _v(F[0], F_begin), _v(F[1], F_begin), _v(F[2], F_begin),
+                                            _v(g, g_begin), _v(z, z_begin), _v(J, J_begin), geom, int(logz),
+                                            _v(A, tuple(geom.lo)))
+    if rc != 0:
+        raise OracleError("implicit_coefficients: unknown location")
+    return A
+
+
+def apply_implicit(loc, A, q, q_begin, geom):
+    """Source/ImplicitOperator.cpp:123-135: A1 q on the valid points at loc."""
+    out = np.zeros(_shape_at(geom, loc))
+    lib().oracle_apply_implicit(loc, _v(A, tuple(geom.lo)), _v(q, q_begin), geom, _v(out, tuple(geom.lo)))
+    return out[0]
