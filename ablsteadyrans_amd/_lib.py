@@ -184,6 +184,18 @@ _sig = {
     "gust_implicit_coefficients": (C.c_int, [C.c_int32, _P(GeometryC)] + [_P(Array4)] * 6 +
                                    [C.c_int32, _P(Array4), C.c_void_p]),
     "gust_apply_implicit": (C.c_int, [C.c_int32, _P(GeometryC)] + [_P(Array4)] * 3 + [C.c_void_p]),
+    "gust_convective_fluxes": (C.c_int, [C.c_int32, _P(GeometryC)] + [_P(Array4)] * 4 +
+                               [C.c_int32, C.c_double] + [_P(Array4)] * 3 + [C.c_void_p]),
+    "gust_flux_divergence": (C.c_int, [C.c_int32, _P(GeometryC)] + [_P(Array4)] * 5 + [C.c_void_p]),
+    "gust_convection": (C.c_int, [C.c_int32, _P(GeometryC)] + [_P(Array4)] * 5 +
+                        [C.c_int32, C.c_double, C.c_int32, _P(Array4), C.c_void_p]),
+    "gust_scalar_diffusive_fluxes": (C.c_int, [_P(GeometryC)] + [_P(Array4)] * 3 + [C.c_int32] + [_P(Array4)] * 3 +
+                                     [C.c_void_p]),
+    "gust_scalar_diffusion": (C.c_int, [_P(GeometryC)] + [_P(Array4)] * 4 + [C.c_int32, _P(Array4), C.c_void_p]),
+    "gust_momentum_diffusive_fluxes": (C.c_int, [C.c_int32, _P(GeometryC)] + [_P(Array4)] * 5 + [C.c_int32] +
+                                       [_P(Array4)] * 3 + [C.c_void_p]),
+    "gust_momentum_diffusion": (C.c_int, [_P(GeometryC)] + [_P(Array4)] * 6 + [C.c_int32] + [_P(Array4)] * 3 +
+                                [C.c_void_p]),
     "gust_local_dtau": (C.c_int, [_P(GeometryC)] + [_P(Array4)] * 6 + [C.c_double, _P(Array4), C.c_void_p]),
     "gust_ser_cfl": (C.c_int, [C.c_double] * 6 + [_P(C.c_double)]),
     "gust_face_dtau": (C.c_int, [_P(GeometryC)] + [_P(Array4)] * 4 + [C.c_void_p]),

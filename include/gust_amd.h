@@ -350,6 +350,68 @@ int gust_implicit_coefficients(int32_t loc, const gust_geometry* geom, const gus
 int gust_apply_implicit(int32_t loc, const gust_geometry* geom, const gust_array4* A, const gust_array4* q,
                         gust_array4* out, void* stream);
 
+/* ---- Convection and diffusion over control volumes ----------------------
+ * (Source/Convection.H, Source/Diffusion.H, Source/CVFluxes.H). A field at
+ * loc owns the control volume centred on its points; G[d] are the fluxes on
+ * that volume's d-faces (location loc ^ (1 << d)) covering cv_flux_box
+ * (d-centres [lo-1, hi+1] where loc is nodal along d, else d-nodes
+ * [lo, hi+1]). Limiter: 0 van Leer, 1 van Albada, 2 upwind
+ * (Source/Limiter_K.H:19). face_mean: 0 arithmetic, 1 logarithmic. */
+enum { GUST_LIMITER_VAN_LEER = 0, GUST_LIMITER_VAN_ALBADA = 1, GUST_LIMITER_UPWIND = 2 };
+
+/* Convective fluxes F q_face (MUSCL, upwind side; q: two ghosts; F: one).
+ * Replaces: void convective_fluxes(Loc, const MultiFab& q, const
+ *   Array<const MultiFab*,3>& F, const ConvectionOptions&, const Mesh&,
+ *   Array<MultiFab,3>& G) -- Source/Convection.H:81-84, Convection.cpp:117-132. */
+int gust_convective_fluxes(int32_t loc, const gust_geometry* geom, const gust_array4* q, const gust_array4* Fx,
+                           const gust_array4* Fy, const gust_array4* Fz, int32_t limiter, double smooth_eps,
+                           gust_array4* Gx, gust_array4* Gy, gust_array4* Gz, void* stream);
+
+/* out = (1/J) [d_xi G_x + d_eta G_y + d_zeta G_z] on the valid points.
+ * Replaces: void flux_divergence(Loc, const Array<MultiFab,3>& G, const Mesh&,
+ *   const Geometry&, MultiFab& out) -- Source/CVFluxes.H:46-53, CVFluxes.cpp:69-80. */
+int gust_flux_divergence(int32_t loc, const gust_geometry* geom, const gust_array4* Gx, const gust_array4* Gy,
+                         const gust_array4* Gz, const gust_array4* detJ, gust_array4* out, void* stream);
+
+/* C(q) (bounded: C(q) - q C(1)) in one fused pass.
+ * Replaces: void convection(Loc, const MultiFab& q, const Array<const
+ *   MultiFab*,3>& F, const ConvectionOptions&, const Mesh&, const Geometry&,
+ *   MultiFab& out) -- Source/Convection.H:86-90, Convection.cpp:134-141. */
+int gust_convection(int32_t loc, const gust_geometry* geom, const gust_array4* q, const gust_array4* Fx,
+                    const gust_array4* Fy, const gust_array4* Fz, const gust_array4* detJ, int32_t limiter,
+                    double smooth_eps, int32_t bounded, gust_array4* out, void* stream);
+
+/* gamma (grad q . A) on the cell faces (q, gamma: one ghost).
+ * Replaces: void scalar_diffusive_fluxes(...) -- Source/Diffusion.H:40-42,
+ *   Diffusion.cpp:63-77. */
+int gust_scalar_diffusive_fluxes(const gust_geometry* geom, const gust_array4* q, const gust_array4* gamma,
+                                 const gust_array4* z_nd, int32_t face_mean, gust_array4* Gx, gust_array4* Gy,
+                                 gust_array4* Gz, void* stream);
+
+/* D(q) on the cells, fused.
+ * Replaces: void scalar_diffusion(...) -- Source/Diffusion.H:45-46,
+ *   Diffusion.cpp:79-85. */
+int gust_scalar_diffusion(const gust_geometry* geom, const gust_array4* q, const gust_array4* gamma,
+                          const gust_array4* z_nd, const gust_array4* detJ, int32_t face_mean, gust_array4* out,
+                          void* stream);
+
+/* tau_c . A of c-momentum on the control-volume faces of component c (u, v,
+ * w: two ghosts; nu: one).
+ * Replaces: void momentum_diffusive_fluxes(...) -- Source/Diffusion.H:51-54,
+ *   Diffusion.cpp:87-109. */
+int gust_momentum_diffusive_fluxes(int32_t c, const gust_geometry* geom, const gust_array4* u, const gust_array4* v,
+                                   const gust_array4* w, const gust_array4* nu, const gust_array4* z_nd,
+                                   int32_t face_mean, gust_array4* Gx, gust_array4* Gy, gust_array4* Gz,
+                                   void* stream);
+
+/* D_c(u) on the c-faces for c = 0, 1, 2, fused.
+ * Replaces: void momentum_diffusion(...) -- Source/Diffusion.H:57-60,
+ *   Diffusion.cpp:111-119. */
+int gust_momentum_diffusion(const gust_geometry* geom, const gust_array4* u, const gust_array4* v,
+                            const gust_array4* w, const gust_array4* nu, const gust_array4* z_nd,
+                            const gust_array4* detJ, int32_t face_mean, gust_array4* out_u, gust_array4* out_v,
+                            gust_array4* out_w, void* stream);
+
 /* ---- Pseudo-time (Source/PseudoTime.H, Source/Predictor.H) ------------- */
 
 /* dtau = cfl / max(|u_c|/dx + |v_c|/dy + |Omega_c|/(J dzeta) + 2 nu (1/dx^2 +

@@ -119,6 +119,23 @@ int oracle_implicit_coefficients(int loc, const oa4* fx, const oa4* fy, const oa
                                  const oa4* J, const ogeom* geo, int logz, oa4* A);
 int oracle_apply_implicit(int loc, const oa4* A, const oa4* q, const ogeom* geo, oa4* out);
 
+/* Convection and diffusion over control volumes (oracle/transport.c,
+ * Source/Convection.cpp, Source/Diffusion.cpp, Source/CVFluxes.cpp).
+ * limiter: 0 van Leer, 1 van Albada, 2 upwind (Source/Limiter_K.H:19).
+ * G[d]: at loc ^ (1 << d), covering cv_flux_box(loc, valid, d). q needs two
+ * ghosts for convection, one for diffusion; F and the diffusivities one.
+ * convective_fluxes returns -1 for a bad loc/limiter, -2 for a bad eps. */
+int oracle_convective_fluxes(int loc, const oa4* q, const oa4* fx, const oa4* fy, const oa4* fz, int limiter,
+                             double eps, const ogeom* g, oa4* G0, oa4* G1, oa4* G2);
+int oracle_flux_divergence(int loc, const oa4* G0, const oa4* G1, const oa4* G2, const oa4* J, const ogeom* g,
+                           oa4* out);
+int oracle_subtract_transport_divergence(int loc, const oa4* q, const oa4* fx, const oa4* fy, const oa4* fz,
+                                         const oa4* J, const ogeom* g, oa4* out);
+int oracle_scalar_diffusive_fluxes(const oa4* q, const oa4* gamma, const oa4* z, const ogeom* g, int logz, oa4* G0,
+                                   oa4* G1, oa4* G2);
+int oracle_momentum_diffusive_fluxes(int c, const oa4* u, const oa4* v, const oa4* w, const oa4* nu, const oa4* z,
+                                     const ogeom* g, int logz, oa4* G0, oa4* G1, oa4* G2);
+
 #ifdef __cplusplus
 }
 #endif

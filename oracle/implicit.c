@@ -26,8 +26,10 @@ static double log_mean(double a, double b)
     return (1.0 - s) * direct + s * series;
 }
 
-/* face_gamma<S> (Source/Diffusion_K.H:49-64). */
-static double face_gamma(const oa4* g, int ts, int i, int j, int k, int logz)
+/* face_gamma<S> (Source/Diffusion_K.H:49-64); shared with transport.c. */
+double oracle_face_gamma(const oa4* g, int ts, int i, int j, int k, int logz);
+#define face_gamma oracle_face_gamma
+double oracle_face_gamma(const oa4* g, int ts, int i, int j, int k, int logz)
 {
     if (!logz || !os_nodal(ts, 2)) { return os_interp(g, OS_CC, ts, i, j, k); }
     const int lx = os_lo_off(OS_CC, ts, 0), ly = os_lo_off(OS_CC, ts, 1);

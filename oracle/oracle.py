@@ -78,6 +78,11 @@ def lib():
         L.oracle_face_dtau.argtypes = [V, V, V, V, P(OGeom)]
         L.oracle_implicit_coefficients.argtypes = [C.c_int, V, V, V, V, V, V, P(OGeom), C.c_int, V]
         L.oracle_apply_implicit.argtypes = [C.c_int, V, V, P(OGeom), V]
+        L.oracle_convective_fluxes.argtypes = [C.c_int, V, V, V, V, C.c_int, C.c_double, P(OGeom), V, V, V]
+        L.oracle_flux_divergence.argtypes = [C.c_int, V, V, V, V, P(OGeom), V]
+        L.oracle_subtract_transport_divergence.argtypes = [C.c_int, V, V, V, V, V, P(OGeom), V]
+        L.oracle_scalar_diffusive_fluxes.argtypes = [V, V, V, P(OGeom), C.c_int, V, V, V]
+        L.oracle_momentum_diffusive_fluxes.argtypes = [C.c_int, V, V, V, V, V, P(OGeom), C.c_int, V, V, V]
         for f in ("oracle_omega_from_w", "oracle_w_from_omega", "oracle_transport_fluxes",
                   "oracle_velocity_divergence", "oracle_pressure_gradient"):
             getattr(L, f).restype = None
@@ -314,3 +319,92 @@ def apply_implicit(loc, A, q, q_begin, geom):
     out = np.zeros(_shape_at(geom, loc))
     lib().oracle_apply_implicit(loc, _v(A, tuple(geom.lo)), _v(q, q_begin), geom, _v(out, tuple(geom.lo)))
     return out[0]
+
+
+# --- convection and diffusion (oracle/transport.c) ---------------------------------------------------------
+
+VAN_LEER, VAN_ALBADA, UPWIND = 0, 1, 2
+
+
+def cv_flux_box(loc, geom, d):
+    """(lo, hi) of cv_flux_box(loc, valid, d) (Source/CVFluxes.cpp:54-67)."""
+    lo, hi = [], []
+    for a in range(3):
+        nod = (loc >> a) & 1
+        l, h = geom.lo[a], geom.hi[a] + nod
+        if a == d:
+            l, h = (geom.lo[a] - 1, geom.hi[a] + 1) if nod else (geom.lo[a], geom.hi[a] + 1)
+        lo.append(l)
+        hi.append(h)
+    return tuple(lo), tuple(hi)
+
+
+def _cv_arrays(loc, geom):
+    G, begins = [], []
+    for d in range(3):
+        lo, hi = cv_flux_box(loc, geom, d)
+        G.append(np.zeros((1, hi[2] - lo[2] + 1, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1)))
+        begins.append(lo)
+    return G, begins
+
+
+def convective_fluxes(loc, q, q_begin, F, F_begin, geom, limiter=VAN_LEER, eps=1e-5):
+    """Source/Convection.cpp:117-132: ([G0, G1, G2], [begins])."""
+    G, b = _cv_arrays(loc, geom)
+    rc = lib().oracle_convective_fluxes(loc, _v(q, q_begin), _v(F[0], F_begin), _v(F[1], F_begin),
+                                        _v(F[2], F_begin), limiter, float(eps), geom, _v(G[0], b[0]), _v(G[1], b[1]),
+                                        _v(G[2], b[2]))
+    if rc != 0:
+        raise OracleError("convective_fluxes: bad location, limiter or eps")
+    return G, b
+
+
+def flux_divergence(loc, G, begins, J, J_begin, geom):
+    """Source/CVFluxes.cpp:69-80: out on the valid points of loc."""
+    out = np.zeros(_shape_at(geom, loc))
+    lib().oracle_flux_divergence(loc, _v(G[0], begins[0]), _v(G[1], begins[1]), _v(G[2], begins[2]), _v(J, J_begin),
+                                 geom, _v(out, tuple(geom.lo)))
+    return out[0]
+
+
+def convection(loc, q, q_begin, F, F_begin, J, J_begin, geom, limiter=VAN_LEER, eps=1e-5, bounded=False):
+    """Source/Convection.cpp:134-141."""
+    G, b = convective_fluxes(loc, q, q_begin, F, F_begin, geom, limiter, eps)
+    out = flux_divergence(loc, G, b, J, J_begin, geom)[None].copy()
+    if bounded:
+        lib().oracle_subtract_transport_divergence(loc, _v(q, q_begin), _v(F[0], F_begin), _v(F[1], F_begin),
+                                                   _v(F[2], F_begin), _v(J, J_begin), geom,
+                                                   _v(out, tuple(geom.lo)))
+    return out[0]
+
+
+def scalar_diffusive_fluxes(q, q_begin, gamma, g_begin, z, z_begin, geom, logz=False):
+    G, b = _cv_arrays(0, geom)
+    lib().oracle_scalar_diffusive_fluxes(_v(q, q_begin), _v(gamma, g_begin), _v(z, z_begin), geom, int(logz),
+                                         _v(G[0], b[0]), _v(G[1], b[1]), _v(G[2], b[2]))
+    return G, b
+
+
+def scalar_diffusion(q, q_begin, gamma, g_begin, z, z_begin, J, J_begin, geom, logz=False):
+    """Source/Diffusion.cpp:79-85."""
+    G, b = scalar_diffusive_fluxes(q, q_begin, gamma, g_begin, z, z_begin, geom, logz)
+    return flux_divergence(0, G, b, J, J_begin, geom)
+
+
+def momentum_diffusive_fluxes(c, vel, vel_begin, nu, nu_begin, z, z_begin, geom, logz=False):
+    G, b = _cv_arrays(1 << c, geom)
+    rc = lib().oracle_momentum_diffusive_fluxes(c, _v(vel[0], vel_begin), _v(vel[1], vel_begin),
+                                                _v(vel[2], vel_begin), _v(nu, nu_begin), _v(z, z_begin), geom,
+                                                int(logz), _v(G[0], b[0]), _v(G[1], b[1]), _v(G[2], b[2]))
+    if rc != 0:
+        raise OracleError("momentum_diffusive_fluxes: bad component")
+    return G, b
+
+
+def momentum_diffusion(vel, vel_begin, nu, nu_begin, z, z_begin, J, J_begin, geom, logz=False):
+    """Source/Diffusion.cpp:111-119: [D_u, D_v, D_w] on the valid faces."""
+    out = []
+    for c in range(3):
+        G, b = momentum_diffusive_fluxes(c, vel, vel_begin, nu, nu_begin, z, z_begin, geom, logz)
+        out.append(flux_divergence(1 << c, G, b, J, J_begin, geom))
+    return out

@@ -36,6 +36,24 @@ def ogeom(mesh):
     return orc.geom_c(g.lo, g.hi, g.periodic, g.dx)
 
 
+def ext_nodes(mesh, g: int = 2):
+    """Node heights with g ghost layers: periodic images along x and y (the
+    hill meshes are periodic), cubic extrapolation of the columns below the
+    lowest and above the highest ghost node."""
+    nx, ny, nz = mesh.geom.n
+    Z = mesh.z_nd   # [k+1, j+1, i+1], one ghost
+    ii = np.mod(np.arange(-g, nx + 1 + g), nx)
+    jj = np.mod(np.arange(-g, ny + 1 + g), ny)
+    out = np.zeros((nz + 3 + 2 * (g - 1), ny + 1 + 2 * g, nx + 1 + 2 * g))
+    out[g - 1:g + nz + 2] = Z[np.ix_(np.arange(nz + 3), jj + 1, ii + 1)]
+    for e in range(g - 1):   # below: index g-2-e from the three above it
+        kq = g - 2 - e
+        out[kq] = 3 * out[kq + 1] - 3 * out[kq + 2] + out[kq + 3]
+        kt = g + nz + 2 + e
+        out[kt] = 3 * out[kt - 1] - 3 * out[kt - 2] + out[kt - 3]
+    return out
+
+
 def positions(mesh, loc: int, grow: int = 1):
     """(X, Y, Z) of every point of a field at loc (bit d set: nodal along d),
     valid points and `grow` ghost layers; shape (nz', ny', nx')."""
@@ -48,13 +66,14 @@ def positions(mesh, loc: int, grow: int = 1):
     kk = np.arange(-grow, nz + nd[2] + grow)
     x = plo[0] + (ii + (0.0 if nd[0] else 0.5)) * dx
     y = plo[1] + (jj + (0.0 if nd[1] else 0.5)) * dy
-    Zn = mesh.z_nd   # [k+1, j+1, i+1]
+    g = max(grow, 1)
+    Zn = mesh.z_nd if g == 1 else ext_nodes(mesh, g)   # [k+g, j+g, i+g]
     z = np.zeros((kk.size, jj.size, ii.size))
     cnt = 0
     for c in range(1 if nd[2] else 2):
         for b in range(1 if nd[1] else 2):
             for a in range(1 if nd[0] else 2):
-                z += Zn[np.ix_(kk + 1 + c, jj + 1 + b, ii + 1 + a)]
+                z += Zn[np.ix_(kk + g + c, jj + g + b, ii + g + a)]
                 cnt += 1
     z /= cnt
     X = np.broadcast_to(x[None, None, :], z.shape).copy()
