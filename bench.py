@@ -173,10 +173,38 @@ def _time_vcycle(tile, precision, reps):
     return e0.elapsed_time(e1) / reps, S.levels
 
 
-def _vcycle_bytes(s):
-    """Algorithmic bytes of one V(2,2) with x/y semi-coarsening (levels
-    Nc/4^l): (4/3) Nc [(nu1 + nu2) 8 s + 8 s + 3.5 s]."""
-    return 4.0 / 3.0 * NX * NY * NZ * ((2 + 2) * 8 * s + 8 * s + 3.5 * s)
+def _mg_levels(nx, ny, nz):
+    """Cells per multigrid level: x/y halved while even and >= 4, until
+    nx*ny <= 16 (or 12 levels)."""
+    cells = []
+    for lev in range(64):
+        cells.append(nx * ny * nz)
+        cx = 2 if (nx % 2 == 0 and nx >= 4) else 1
+        cy = 2 if (ny % 2 == 0 and ny >= 4) else 1
+        if (cx == 1 and cy == 1) or lev >= 12 or nx * ny <= 16:
+            break
+        nx //= cx
+        ny //= cy
+    return cells
+
+
+def _vcycle_bytes(s, pre=2, post=2, coarse=16):
+    """Algorithmic bytes of one preconditioner call (V(pre,post) with the
+    explicit 15-point level stencils), reals of s bytes per cell:
+      zebra sweep       18: 15 stencil coefficients (own column as lower,
+                            1/pivot, c'; 12 neighbour), b, x of the other
+                            colour (read once), x written;
+      residual+restrict 18.25: 15 coefficients, b, x, J, 1/4 coarse b;
+      prolongation       2.25: fine x read and written, 1/4 coarse x;
+    coarse_sweeps sweeps on the coarsest level, and the FP64 vector's
+    conversion into and out of the level-0 layout (8 + s each way)."""
+    cells = _mg_levels(NX, NY, NZ)
+    b = 0.0
+    for n in cells[:-1]:
+        b += n * s * ((pre + post) * 18 + 18.25 + 2.25)
+    b += cells[-1] * s * coarse * 18
+    b += cells[0] * (16 + 2 * s)
+    return b
 
 
 def _traffic_from_profile():
