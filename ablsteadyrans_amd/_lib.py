@@ -116,6 +116,36 @@ class PressureCorrectionResultC(C.Structure):
     ]
 
 
+class FacesC(C.Structure):
+    """``gust_faces``: one view per velocity component."""
+
+    _fields_ = [("c", C.POINTER(Array4) * 3)]
+
+
+class PredictorOptionsC(C.Structure):
+    """``gust_predictor_options``."""
+
+    _fields_ = [
+        ("sweeps", C.c_int32),
+        ("rotation_relax", C.c_double),
+        ("w_vnn_cap", C.c_double),
+        ("w_vnn_slope_margin", C.c_double),
+        ("coriolis_f", C.c_double),
+        ("gravity", C.c_double),
+        ("theta0", C.c_double),
+        ("sweep", LineOptionsC),
+    ]
+
+
+class MomentumPredictorResultC(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("max_du", "max_dv", "max_dw", "last_sweep_du", "last_sweep_dv",
+                                          "last_sweep_dw")]
+
+
+class ScalarSweepResultC(C.Structure):
+    _fields_ = [("max_dq", C.c_double), ("last_sweep", C.c_double), ("clipped", C.c_int64)]
+
+
 def _load() -> C.CDLL:
     # One HIP runtime per process: torch ships its own libamdhip64 and the
     # device tensors, streams and events handed to the library come from it.
@@ -196,6 +226,11 @@ _sig = {
                                        [_P(Array4)] * 3 + [C.c_void_p]),
     "gust_momentum_diffusion": (C.c_int, [_P(GeometryC)] + [_P(Array4)] * 6 + [C.c_int32] + [_P(Array4)] * 3 +
                                 [C.c_void_p]),
+    "gust_momentum_predictor": (C.c_int, [_P(GeometryC), _P(FacesC), _P(Array4), _P(Array4), _P(FacesC), _P(Array4),
+                                          _P(FacesC), _P(Array4), _P(FacesC), _P(FacesC), _P(PredictorOptionsC),
+                                          _P(FacesC), _P(FacesC), _P(MomentumPredictorResultC), C.c_void_p]),
+    "gust_scalar_sweep": (C.c_int, [_P(GeometryC), C.c_int32, _P(Array4), _P(Array4), _P(Array4), _P(Array4), C.c_int32,
+                                    C.c_double, C.c_int32, _P(LineOptionsC), _P(ScalarSweepResultC), C.c_void_p]),
     "gust_local_dtau": (C.c_int, [_P(GeometryC)] + [_P(Array4)] * 6 + [C.c_double, _P(Array4), C.c_void_p]),
     "gust_ser_cfl": (C.c_int, [C.c_double] * 6 + [_P(C.c_double)]),
     "gust_face_dtau": (C.c_int, [_P(GeometryC)] + [_P(Array4)] * 4 + [C.c_void_p]),

@@ -18,12 +18,16 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <vector>
 
 namespace gust_amd {
 
 namespace {
 
 using namespace cgrid;
+
+__device__ __forceinline__ double sc_max(double a, double b) { return (a < b) ? b : a; }   // std::max
+__device__ __forceinline__ double sc_min(double a, double b) { return (b < a) ? b : a; }   // std::min
 
 // ---------------------------------------------------------------------------
 // Box kernels.
@@ -144,6 +148,176 @@ __global__ __launch_bounds__(kThreads) void fill_periodic_kernel(Box3 grown, V f
             f(i, j, k) = f(a, c, k);
         }
     }
+}
+
+// a_P = keep/dtau + A_P, d = keep (a_P - sum a_nb) (Source/Predictor.cpp:186-201).
+__global__ __launch_bounds__(kThreads) void ap_d_kernel(Box3 b, CV A, int64_t an, CV dt, CV mask, int has_mask, V ap,
+                                                         V d)
+{
+    GUST_GRID_LOOP(b)
+    {
+        int i, j, k;
+        b.at(t, i, j, k);
+        const double* a = &A(i, j, k);
+        const double keep = has_mask ? 1.0 - mask(i, j, k) : 1.0;
+        const double p = keep / dt(i, j, k) + a[0];
+        ap(i, j, k) = p;
+        const double nb = a[1 * an] + a[2 * an] + a[3 * an] + a[4 * an] + a[5 * an] + a[6 * an];
+        d(i, j, k) = keep * (p - nb);
+    }
+}
+
+// The w rows' VNN cap, plain or slope-aware (Source/Predictor.cpp:202-252).
+__global__ __launch_bounds__(kThreads) void w_cap_kernel(Box3 b, CV A, int64_t an, CV dt, CV zp, CV mask, int has_mask,
+                                                          V ap, V d, double inv_cap, double s, double dxi0,
+                                                          double dxi1, int slope_aware)
+{
+    GUST_GRID_LOOP(b)
+    {
+        int i, j, k;
+        b.at(t, i, j, k);
+        const double* a = &A(i, j, k);
+        const double keep = has_mask ? 1.0 - mask(i, j, k) : 1.0;
+        double cap;
+        if (slope_aware) {
+            double term = inv_cap * (a[5 * an] + a[6 * an]);
+            const double hx = 0.5 * (zp(i + 1, j + 1, k) + zp(i + 1, j, k) - zp(i, j + 1, k) - zp(i, j, k)) * dxi0;
+            const double hy = 0.5 * (zp(i + 1, j + 1, k) + zp(i, j + 1, k) - zp(i + 1, j, k) - zp(i, j, k)) * dxi1;
+            const double slope2 = hx * hx + hy * hy;
+            if (slope2 > 0) {
+                const double root = sqrt(1.0 + slope2) + 1.0;
+                const double rc = slope2 / (root * root);
+                const double lim = sc_max(0.0, 1.0 / (s * rc) - 1.0) / dt(i, j, k);
+                term = sc_min(term, lim);
+            }
+            cap = keep * term;
+        } else {
+            cap = keep * inv_cap * (a[5 * an] + a[6 * an]);
+        }
+        ap(i, j, k) += cap;
+        d(i, j, k) += cap;
+    }
+}
+
+// rhs = -R (MultiFab::Copy then mult(-1)).
+__global__ __launch_bounds__(kThreads) void negate_kernel(Box3 b, CV R, V rhs)
+{
+    GUST_GRID_LOOP(b)
+    {
+        int i, j, k;
+        b.at(t, i, j, k);
+        rhs(i, j, k) = R(i, j, k) * -1.0;
+    }
+}
+
+// Per-level sums of a face field over the faces counted once (one thread
+// per level, i fastest then j: a fixed order, so the sums are reproducible).
+__global__ __launch_bounds__(kThreads) void level_sum_kernel(CV f, int ilo, int ihi, int jlo, int jhi, int klo, int nz,
+                                                              double* out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nz) { return; }
+    double s = 0.0;
+    for (int j = jlo; j <= jhi; ++j)
+        for (int i = ilo; i <= ihi; ++i) { s += f(i, j, klo + k); }
+    out[k] = s;
+}
+
+// Point-implicit rotation (Source/Predictor.cpp:260-342).
+template <bool kIsU>
+__global__ __launch_bounds__(kThreads) void coriolis_kernel(Box3 b, CV A, int64_t an, CV dt, CV ro, CV mask,
+                                                             int has_mask, V ap, V d, V r, const double* mean, int klo,
+                                                             double fs, double relax, int periodic, int lo_edge,
+                                                             int hi_edge)
+{
+    GUST_GRID_LOOP(b)
+    {
+        int i, j, k;
+        b.at(t, i, j, k);
+        const double keep = has_mask ? 1.0 - mask(i, j, k) : 1.0;
+        if (keep == 0) { continue; }
+        const int n = kIsU ? i : j;
+        if (!periodic && (n == lo_edge || n == hi_edge)) { continue; }
+        const double* a = &A(i, j, k);
+        const double nb = a[1 * an] + a[2 * an] + a[3 * an] + a[4 * an] + a[5 * an] + a[6 * an];
+        const double net = sc_max(a[0] - nb, 0.0);
+        const double tau_c = 1.0 / sc_max(fabs(fs) / relax, net);
+        const double tau_own = 1.0 / (1.0 / dt(i, j, k) + net);
+        const double stiff = fs * fs * sc_max(tau_c, tau_own);
+        ap(i, j, k) += stiff;
+        d(i, j, k) += stiff;
+        double s = 0.0;   // interp<YF, XF> (u rows) or interp<XF, YF> (v rows) of the other residual
+        if (kIsU) {
+            s += ro(i - 1, j, k);
+            s += ro(i, j, k);
+            s += ro(i - 1, j + 1, k);
+            s += ro(i, j + 1, k);
+        } else {
+            s += ro(i, j - 1, k);
+            s += ro(i + 1, j - 1, k);
+            s += ro(i, j, k);
+            s += ro(i + 1, j, k);
+        }
+        const double avg = s * 0.25;
+        const double m = mean[k - klo];
+        r(i, j, k) += (kIsU ? -fs : fs) * (tau_c * m + tau_own * (avg - m));
+    }
+}
+
+// Point-implicit buoyancy on the w rows (Source/Predictor.cpp:343-395).
+__global__ __launch_bounds__(kThreads) void buoyancy_kernel(Box3 b, CV dt, CV th, CV rt, CV At, CV z, CV mask,
+                                                             int has_mask, V ap, V d, V r, double gt, int klo, int khi)
+{
+    GUST_GRID_LOOP(b)
+    {
+        int i, j, k;
+        b.at(t, i, j, k);
+        const double keep = has_mask ? 1.0 - mask(i, j, k) : 1.0;
+        if (keep == 0) { continue; }
+        // The ground and top faces are boundary rows (Dirichlet under every
+        // boundary condition of the reference): no term, and no read of
+        // theta's rows outside the domain.
+        if (k == klo || k == khi) { continue; }
+        auto zc = [&](int kk) {
+            return 0.125 * (z(i, j, kk) + z(i + 1, j, kk) + z(i, j + 1, kk) + z(i + 1, j + 1, kk) + z(i, j, kk + 1) +
+                            z(i + 1, j, kk + 1) + z(i, j + 1, kk + 1) + z(i + 1, j + 1, kk + 1));
+        };
+        const double gamma = (th(i, j, k) - th(i, j, k - 1)) / (zc(k) - zc(k - 1));
+        const double ap_theta = 0.5 * (At(i, j, k - 1) + At(i, j, k));
+        const double tau = 1.0 / (1.0 / dt(i, j, k) + ap_theta);
+        const double stiff = gt * sc_max(gamma, 0.0) * tau;
+        ap(i, j, k) += stiff;
+        d(i, j, k) += stiff;
+        r(i, j, k) -= gt * tau * 0.5 * (rt(i, j, k - 1) + rt(i, j, k));
+    }
+}
+
+// ap = 1/dtau + A_P (Source/Simplec.cpp:247-257).
+__global__ __launch_bounds__(kThreads) void scalar_ap_kernel(Box3 b, CV A, CV dt, V ap)
+{
+    GUST_GRID_LOOP(b)
+    {
+        int i, j, k;
+        b.at(t, i, j, k);
+        ap(i, j, k) = 1.0 / dt(i, j, k) + A(i, j, k);
+    }
+}
+
+// The k floor: raise values below it, count them (Source/Simplec.cpp:262-277).
+__global__ __launch_bounds__(kThreads) void clip_kernel(Box3 b, V f, double floor, unsigned long long* count)
+{
+    unsigned long long c = 0;
+    GUST_GRID_LOOP(b)
+    {
+        int i, j, k;
+        b.at(t, i, j, k);
+        if (f(i, j, k) < floor) {
+            f(i, j, k) = floor;
+            ++c;
+        }
+    }
+    for (int off = 32; off >= 1; off >>= 1) { c += __shfl_xor(c, off, 64); }
+    if ((threadIdx.x & 63) == 0 && c) { atomicAdd(count, c); }
 }
 
 // ---------------------------------------------------------------------------
@@ -368,6 +542,189 @@ int gust_pressure_correction(gust_poisson* solver, const gust_geometry* geom, gu
         fill_boundary(st, g, GUST_LOC_ZFACE, w);
         check_rc(gust_velocity_divergence(geom, u, v, w, z_nd, ax, ay, az, detJ, &div.v, stream));
         res.divergence_after = red.max_abs(cells, &div.v);
+        if (result) { *result = res; }
+    });
+}
+
+int gust_momentum_predictor(const gust_geometry* geom, gust_faces* vel, const gust_array4* theta,
+                            const gust_array4* z_nd, const gust_faces* R, const gust_array4* R_theta,
+                            const gust_faces* A, const gust_array4* A_theta, const gust_faces* dtau,
+                            const gust_faces* mask, const gust_predictor_options* opt, gust_faces* ap, gust_faces* d,
+                            gust_momentum_predictor_result* result, void* stream)
+{
+    return guarded([&] {
+        const char* op = "momentum_predictor";
+        hipStream_t st = (hipStream_t)stream;
+        check_geom(op, geom);
+        if (!vel || !R || !A || !dtau || !opt || !ap || !d) { GUST_FAIL("gust: momentum_predictor: null argument"); }
+        const double relax = opt->rotation_relax, cap = opt->w_vnn_cap, margin = opt->w_vnn_slope_margin;
+        if (std::isnan(relax) || !(relax > 0 && relax <= 1)) {
+            GUST_FAIL("gust: momentum_predictor: rotation_relax must be > 0 and <= 1 (simplec.rotation_relax)");
+        }
+        if (std::isnan(cap) || !(cap == 0 || (cap > 0 && std::isfinite(cap) && std::isfinite(1 / cap)))) {
+            GUST_FAIL("gust: momentum_predictor: w_vnn_cap must be 0 (off), or finite and > 0 with a finite "
+                      "reciprocal (simplec.w_vnn_cap)");
+        }
+        if (std::isnan(margin) || !(margin == 0 || (margin >= 1 && std::isfinite(margin)))) {
+            GUST_FAIL("gust: momentum_predictor: w_vnn_slope_margin must be 0 (off), or finite and >= 1 "
+                      "(simplec.w_vnn_cap_slope_margin)");
+        }
+        if (opt->sweeps < 1) { GUST_FAIL("gust: momentum_predictor: sweeps must be >= 1 (got %d)", opt->sweeps); }
+        const gust_geometry& g = *geom;
+        const char* dname[3] = {"dtau_x", "dtau_y", "dtau_z"};
+        Reducer red(st);
+        for (int c = 0; c < 3; ++c) {
+            check_field(op, dname[c], dtau->c[c], g, 1 << c, 0);
+            const IBox fb = loc_box(g, 1 << c);
+            // finite and > 0: the non-negative flag and a zero minimum both fail
+            if (red.flag_bad_nonneg(fb, dtau->c[c]) || !(red.min_pos(fb, dtau->c[c]) > 0)) {
+                GUST_FAIL("gust: momentum_predictor: dtau_faces must be finite and > 0 everywhere");
+            }
+        }
+        const bool rotation = opt->coriolis_f != 0;
+        const bool buoyancy = opt->gravity > 0;
+        if ((rotation || buoyancy) && !mask) {
+            GUST_FAIL("gust: momentum_predictor: the point-implicit buoyancy and rotation need the boundary "
+                      "conditions (the Dirichlet rows they skip); give in.bc, or gravity 0 and Coriolis off");
+        }
+        check_rc(gust_check_line_options(&opt->sweep));
+        gust_momentum_predictor_result out{};
+        double* dmax[3] = {&out.max_du, &out.max_dv, &out.max_dw};
+        double* last[3] = {&out.last_sweep_du, &out.last_sweep_dv, &out.last_sweep_dw};
+        const double dxi0 = 1.0 / g.dx[0], dxi1 = 1.0 / g.dx[1];
+        for (int c = 0; c < 3; ++c) {
+            const int loc = 1 << c;
+            const IBox fb = loc_box(g, loc);
+            const Box3 b = box3(fb);
+            check_field(op, "q", vel->c[c], g, loc, 0);
+            check_field(op, "R", R->c[c], g, loc, 0);
+            check_field(op, "A", A->c[c], g, loc, 0);
+            if (A->c[c]->ncomp != GUST_N_COEFS) { GUST_FAIL("gust: momentum_predictor: A needs 7 components"); }
+            check_field(op, "ap", ap->c[c], g, loc, 0);
+            check_field(op, "d", d->c[c], g, loc, 0);
+            const gust_array4* m = mask ? mask->c[c] : nullptr;
+            if (m) { check_field(op, "mask", m, g, loc, 0); }
+            const CV mk = dview<const double>(m ? m : A->c[c]);
+            const int64_t an = A->c[c]->nstride;
+            ap_d_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<const double>(A->c[c]), an,
+                                                                   dview<const double>(dtau->c[c]), mk, m ? 1 : 0,
+                                                                   dview<double>(ap->c[c]), dview<double>(d->c[c]));
+            if (c == 2 && cap > 0) {
+                check_nodal_heights(op, z_nd, g);
+                w_cap_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(
+                    b, dview<const double>(A->c[c]), an, dview<const double>(dtau->c[c]), dview<const double>(z_nd),
+                    mk, m ? 1 : 0, dview<double>(ap->c[c]), dview<double>(d->c[c]), 1.0 / cap, margin, dxi0, dxi1,
+                    margin > 0 ? 1 : 0);
+            }
+            Scratch rhs = scratch(st, 44, fb, 0), delta = scratch(st, 45, fb, 1);
+            negate_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<const double>(R->c[c]),
+                                                                     dview<double>(&rhs.v));
+            GUST_HIP(hipGetLastError());
+            if (c < 2 && rotation) {
+                // the other residual with its periodic images (FillBoundary)
+                const int co = 1 - c, lo_other = 1 << co;
+                const IBox ob = loc_box(g, lo_other);
+                Scratch og = scratch(st, 46, ob, 1);
+                box_op(st, grow(ob, 1, 1, 1), &og.v, nullptr, 0.0, 3);
+                box_add(st, ob, &og.v, R->c[co]);
+                fill_boundary(st, g, lo_other, &og.v);
+                // per-level mean over every face once
+                const bool per_o = g.periodic[co] != 0;
+                IBox sb = ob;
+                if (per_o) { sb.hi[co] -= 1; }
+                const int nz = g.hi[2] - g.lo[2] + 1;
+                auto* sums = static_cast<double*>(Workspace::get(st, 47, size_t(nz) * sizeof(double)));
+                level_sum_kernel<<<(nz + 63) / 64, 64, 0, st>>>(dview<const double>(&og.v), sb.lo[0], sb.hi[0],
+                                                                sb.lo[1], sb.hi[1], g.lo[2], nz, sums);
+                std::vector<double> h(static_cast<size_t>(nz));
+                GUST_HIP(hipMemcpyAsync(h.data(), sums, size_t(nz) * sizeof(double), hipMemcpyDeviceToHost, st));
+                GUST_HIP(hipStreamSynchronize(st));
+                const double npts_level = double((g.hi[co] - g.lo[co] + 1) + (per_o ? 0 : 1)) *
+                                          double(g.hi[c] - g.lo[c] + 1);
+                for (double& x : h) { x /= npts_level; }
+                auto* dmean = static_cast<double*>(Workspace::get(st, 48, size_t(nz) * sizeof(double)));
+                GUST_HIP(hipMemcpyAsync(dmean, h.data(), size_t(nz) * sizeof(double), hipMemcpyHostToDevice, st));
+                const int lo_edge = g.lo[c], hi_edge = g.hi[c] + 1;
+                const int periodic = g.periodic[c] ? 1 : 0;
+                if (c == 0) {
+                    coriolis_kernel<true><<<blocks_for(b.npts()), kThreads, 0, st>>>(
+                        b, dview<const double>(A->c[c]), an, dview<const double>(dtau->c[c]),
+                        dview<const double>(&og.v), mk, m ? 1 : 0, dview<double>(ap->c[c]), dview<double>(d->c[c]),
+                        dview<double>(&rhs.v), dmean, g.lo[2], opt->coriolis_f, relax, periodic, lo_edge, hi_edge);
+                } else {
+                    coriolis_kernel<false><<<blocks_for(b.npts()), kThreads, 0, st>>>(
+                        b, dview<const double>(A->c[c]), an, dview<const double>(dtau->c[c]),
+                        dview<const double>(&og.v), mk, m ? 1 : 0, dview<double>(ap->c[c]), dview<double>(d->c[c]),
+                        dview<double>(&rhs.v), dmean, g.lo[2], opt->coriolis_f, relax, periodic, lo_edge, hi_edge);
+                }
+                GUST_HIP(hipGetLastError());
+                GUST_HIP(hipStreamSynchronize(st));   // h is read by the copy above
+            }
+            if (c == 2 && buoyancy) {
+                check_field(op, "R_theta", R_theta, g, GUST_LOC_CELL, 0);
+                check_field(op, "theta", theta, g, GUST_LOC_CELL, 1);
+                if (!A_theta) { GUST_FAIL("gust: momentum_predictor: the buoyancy needs theta's implicit rows"); }
+                check_field(op, "A_theta", A_theta, g, GUST_LOC_CELL, 0);
+                check_nodal_heights(op, z_nd, g);
+                const double gt = opt->gravity / opt->theta0;
+                buoyancy_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(
+                    b, dview<const double>(dtau->c[c]), dview<const double>(theta), dview<const double>(R_theta),
+                    dview<const double>(A_theta), dview<const double>(z_nd), mk, m ? 1 : 0, dview<double>(ap->c[c]),
+                    dview<double>(d->c[c]), dview<double>(&rhs.v), gt, g.lo[2], g.hi[2] + 1);
+                GUST_HIP(hipGetLastError());
+            }
+            check_rc(gust_line_sweeps(&opt->sweep, loc, A->c[c], ap->c[c], &rhs.v, opt->sweeps, geom, &delta.v,
+                                      last[c], stream));
+            *dmax[c] = red.max_abs(fb, &delta.v);
+            box_add(st, fb, vel->c[c], &delta.v);
+        }
+        if (result) { *result = out; }
+    });
+}
+
+int gust_scalar_sweep(const gust_geometry* geom, int32_t eq, gust_array4* field, const gust_array4* R_eq,
+                      const gust_array4* A, const gust_array4* dtau, int32_t sweeps, double floor, int32_t clip,
+                      const gust_line_options* sweep, gust_scalar_sweep_result* result, void* stream)
+{
+    return guarded([&] {
+        const char* op = "scalar_sweep";
+        hipStream_t st = (hipStream_t)stream;
+        check_geom(op, geom);
+        if (eq != GUST_EQ_THETA && eq != GUST_EQ_TKE) { GUST_FAIL("gust: scalar_sweep: eq must be theta or tke"); }
+        const gust_geometry& g = *geom;
+        check_field(op, "field", field, g, GUST_LOC_CELL, 1);
+        check_field(op, "R", R_eq, g, GUST_LOC_CELL, 0);
+        check_field(op, "dtau", dtau, g, GUST_LOC_CELL, 0);
+        check_field(op, "A", A, g, GUST_LOC_CELL, 0);
+        if (A->ncomp != GUST_N_COEFS) { GUST_FAIL("gust: scalar_sweep: A needs 7 components"); }
+        if (std::isnan(floor) || !(floor >= 0)) { GUST_FAIL("gust: scalar_sweep: the floor must be >= 0"); }
+        if (!sweep) { GUST_FAIL("gust: scalar_sweep: null line options"); }
+        Reducer red(st);
+        const IBox cells = loc_box(g, GUST_LOC_CELL);
+        if (red.flag_bad_nonneg(cells, dtau) || !(red.min_pos(cells, dtau) > 0)) {
+            GUST_FAIL("gust: scalar_sweep: dtau must be finite and > 0 everywhere");
+        }
+        const Box3 b = box3(cells);
+        Scratch ap = scratch(st, 51, cells, 0), rhs = scratch(st, 52, cells, 0), delta = scratch(st, 53, cells, 1);
+        negate_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<const double>(R_eq), dview<double>(&rhs.v));
+        scalar_ap_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<const double>(A), dview<const double>(dtau),
+                                                                    dview<double>(&ap.v));
+        GUST_HIP(hipGetLastError());
+        gust_scalar_sweep_result res{};
+        check_rc(gust_line_sweeps(sweep, GUST_LOC_CELL, A, &ap.v, &rhs.v, sweeps, geom, &delta.v, &res.last_sweep,
+                                  stream));
+        res.max_dq = red.max_abs(cells, &delta.v);
+        box_add(st, cells, field, &delta.v);
+        if (eq == GUST_EQ_TKE && clip) {
+            auto* cnt = static_cast<unsigned long long*>(Workspace::get(st, 54, 64));
+            auto* hc = static_cast<unsigned long long*>(Workspace::get_host_pinned(54, 64));
+            GUST_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), st));
+            clip_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<double>(field), floor, cnt);
+            GUST_HIP(hipGetLastError());
+            GUST_HIP(hipMemcpyAsync(hc, cnt, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+            GUST_HIP(hipStreamSynchronize(st));
+            res.clipped = static_cast<int64_t>(*hc);
+        }
         if (result) { *result = res; }
     });
 }

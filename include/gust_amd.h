@@ -412,6 +412,75 @@ int gust_momentum_diffusion(const gust_geometry* geom, const gust_array4* u, con
                             const gust_array4* detJ, int32_t face_mean, gust_array4* out_u, gust_array4* out_v,
                             gust_array4* out_w, void* stream);
 
+/* One field per velocity component: x-faces, y-faces, z-faces. */
+typedef struct gust_faces {
+    gust_array4* c[3];
+} gust_faces;
+
+/* The momentum predictor's settings (Source/Predictor.H:242-252,
+ * Source/Simplec.H:26-40): line sweeps, simplec.rotation_relax,
+ * simplec.w_vnn_cap and _slope_margin, the Coriolis parameter f (0: off;
+ * coriolis_parameter(in.coriolis)), gravity (0: off) and theta0. */
+typedef struct gust_predictor_options {
+    int32_t sweeps;
+    double rotation_relax;
+    double w_vnn_cap, w_vnn_slope_margin;
+    double coriolis_f;
+    double gravity, theta0;
+    gust_line_options sweep;
+} gust_predictor_options;
+
+/* Source/Predictor.H:214-220 (MomentumPredictorResult). */
+typedef struct gust_momentum_predictor_result {
+    double max_du, max_dv, max_dw;
+    double last_sweep_du, last_sweep_dv, last_sweep_dw;
+} gust_momentum_predictor_result;
+
+/* The momentum predictor: for u, v, w in turn, a_P = keep/dtau + A_P,
+ * d = keep (a_P - sum a_nb) (keep = 1 - the Dirichlet-row mask), the w
+ * rows' VNN cap (plain or slope-aware), rhs = -R, the point-implicit rotation
+ * (per-level mean of the other residual at the inertial step, the
+ * fluctuation at the row's own step) and buoyancy, the line sweeps, and
+ * q += delta. A holds each component's implicit rows (implicit_rows: A1 with
+ * the boundary conditions' folds; 7 components); mask holds the Dirichlet
+ * rows (null: none; required with rotation or buoyancy). ap and d receive the
+ * SIMPLEC fields the pressure correction uses.
+ * Replaces: MomentumPredictorResult momentum_predictor(State& q, const
+ *   ResidualFields& R, const Array<const MultiFab*,3>& F, const
+ *   ResidualInputs&, const Array<MultiFab,3>& dtau_faces, const Mesh&, const
+ *   Geometry&, int sweeps, PredictorFields& ap, const SurfaceState*, const
+ *   ClosureFields*, Real rotation_relax, const LineSweepOptions&, Real
+ *   w_vnn_cap, Real w_vnn_slope_margin) -- Source/Predictor.H:242-252,
+ *   Source/Predictor.cpp:129-405. */
+int gust_momentum_predictor(const gust_geometry* geom, gust_faces* vel, const gust_array4* theta,
+                            const gust_array4* z_nd, const gust_faces* R, const gust_array4* R_theta,
+                            const gust_faces* A, const gust_array4* A_theta, const gust_faces* dtau,
+                            const gust_faces* mask, const gust_predictor_options* opt, gust_faces* ap, gust_faces* d,
+                            gust_momentum_predictor_result* result, void* stream);
+
+/* Equations of the scalar sweep (Source/Residual.H, Equation). */
+enum { GUST_EQ_THETA = 0, GUST_EQ_TKE = 1 };
+
+/* Source/Simplec.H:144-149 (ScalarSweepResult). */
+typedef struct gust_scalar_sweep_result {
+    double max_dq;
+    double last_sweep;
+    int64_t clipped;
+} gust_scalar_sweep_result;
+
+/* One z-line step of theta or k in delta form: (1/dtau + A1) dq = -R, the
+ * sweeps, field += dq; k clipped at floor when clip (counted). A: the
+ * equation's implicit rows (7 components).
+ * Replaces: ScalarSweepResult scalar_sweep(Equation, MultiFab& field, const
+ *   MultiFab& R_eq, const Array<const MultiFab*,3>& F, const MultiFab& z_nd,
+ *   const ResidualInputs&, const MultiFab& dtau, const Mesh&, const Geometry&,
+ *   int sweeps, Real floor, const SurfaceState*, const ClosureFields*,
+ *   SimplecWorkspace&, const LineSweepOptions&, bool clip) --
+ *   Source/Simplec.H:158-163, Source/Simplec.cpp:222-281. */
+int gust_scalar_sweep(const gust_geometry* geom, int32_t eq, gust_array4* field, const gust_array4* R_eq,
+                      const gust_array4* A, const gust_array4* dtau, int32_t sweeps, double floor, int32_t clip,
+                      const gust_line_options* sweep, gust_scalar_sweep_result* result, void* stream);
+
 /* ---- Pseudo-time (Source/PseudoTime.H, Source/Predictor.H) ------------- */
 
 /* dtau = cfl / max(|u_c|/dx + |v_c|/dy + |Omega_c|/(J dzeta) + 2 nu (1/dx^2 +

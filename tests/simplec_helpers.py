@@ -137,3 +137,126 @@ def random_state(mesh, seed, open_top=False):
     w[0, -1] = w[0, -2]
     w = periodic_images(w, per, (0, 0))
     return u, v, w, p, d
+
+
+# --- momentum predictor and scalar sweep (Source/Predictor.cpp:129-405, Source/Simplec.cpp:222-281) ---------
+
+def _zline(loc, A, ap, rhs, sweeps, geom):
+    """The oracle's z-line Jacobi on valid arrays: returns (delta valid, change)."""
+    sh = rhs.shape
+    d = np.zeros((1, sh[0] + 2, sh[1] + 2, sh[2] + 2))
+    lo = tuple(geom.lo)
+    ch = orc.zline_jacobi(loc, A, lo, ap[None].copy(), rhs[None].copy(), lo, sweeps, geom, d,
+                          (lo[0] - 1, lo[1] - 1, lo[2] - 1))
+    return d[0, 1:-1, 1:-1, 1:-1].copy(), ch
+
+
+def predictor_fields(A, dt, mask=None):
+    """a_P = keep/dtau + A_P and d = keep (a_P - sum a_nb) (Predictor.cpp:186-201)."""
+    keep = 1.0 - mask if mask is not None else 1.0
+    ap = keep / dt + A[0]
+    nb = A[1] + A[2] + A[3] + A[4] + A[5] + A[6]
+    return ap, keep * (ap - nb)
+
+
+def momentum_predictor(mesh, vel, R, A, dtau, sweeps, mask=None, w_vnn_cap=0.0, margin=0.0, coriolis_f=0.0,
+                       relax=0.3, gravity=0.0, theta0=300.0, theta=None, R_theta=None, A_theta=None):
+    """vel: one-ghost (1, ...) arrays; R, A, dtau, mask: valid arrays per component.
+    Returns (new vel, ap, d, max |delta| per component, last sweep change per component)."""
+    g = PH.ogeom(mesh)
+    nx, ny, nz = mesh.geom.n
+    per = mesh.geom.periodic
+    dxi = (1.0 / mesh.geom.dx[0], 1.0 / mesh.geom.dx[1])
+    vel = [v.copy() for v in vel]
+    aps, ds, dmax, last = [], [], [], []
+    for c in range(3):
+        loc = 1 << c
+        m = mask[c] if mask is not None else None
+        keep = 1.0 - m if m is not None else np.ones_like(dtau[c])
+        ap, d = predictor_fields(A[c], dtau[c], m)
+        if c == 2 and w_vnn_cap > 0:
+            term = (A[c][5] + A[c][6]) / w_vnn_cap
+            if margin > 0:
+                Z = mesh.z_nd   # [k+1, j+1, i+1]
+                z = lambda di, dj: Z[1:nz + 2, 1 + dj:ny + 1 + dj, 1 + di:nx + 1 + di]
+                hx = 0.5 * (z(1, 1) + z(1, 0) - z(0, 1) - z(0, 0)) * dxi[0]
+                hy = 0.5 * (z(1, 1) + z(0, 1) - z(1, 0) - z(0, 0)) * dxi[1]
+                s2 = hx * hx + hy * hy
+                root = np.sqrt(1.0 + s2) + 1.0
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    lim = np.maximum(0.0, 1.0 / (margin * s2 / (root * root)) - 1.0) / dtau[c]
+                term = np.where(s2 > 0, np.minimum(term, lim), term)
+            ap = ap + keep * term
+            d = d + keep * term
+        rhs = -R[c]
+        if c < 2 and coriolis_f != 0:
+            co = 1 - c
+            other = R[co]
+            # per-level mean over every face once
+            sl = [slice(None)] * 3
+            if per[co]:
+                sl[2 - co] = slice(0, other.shape[2 - co] - 1)
+            mean = other[tuple(sl)].sum(axis=(1, 2)) / (
+                ((nx if co == 0 else ny) + (0 if per[co] else 1)) * (nx if c == 0 else ny))
+            og = np.zeros((other.shape[0] + 2, other.shape[1] + 2, other.shape[2] + 2))
+            og[1:-1, 1:-1, 1:-1] = other
+            ogp = periodic_images(og[None].copy(), per, (co == 0, co == 1))[0][1:-1]   # drop the z ghosts
+            nb = A[c][1] + A[c][2] + A[c][3] + A[c][4] + A[c][5] + A[c][6]
+            net = np.maximum(A[c][0] - nb, 0.0)
+            tau_c = 1.0 / np.maximum(abs(coriolis_f) / relax, net)
+            tau_own = 1.0 / (1.0 / dtau[c] + net)
+            stiff = coriolis_f ** 2 * np.maximum(tau_c, tau_own)
+            K, Jj, I = np.meshgrid(*(np.arange(s) for s in ap.shape), indexing="ij")
+            if c == 0:   # R.v at the x-face (i, j): y-faces (i-1, j), (i, j), (i-1, j+1), (i, j+1)
+                avg = 0.25 * (ogp[K, Jj + 1, I] + ogp[K, Jj + 1, I + 1] + ogp[K, Jj + 2, I] + ogp[K, Jj + 2, I + 1])
+                n, nn = I, nx
+            else:        # R.u at the y-face (i, j): x-faces (i, j-1), (i+1, j-1), (i, j), (i+1, j)
+                avg = 0.25 * (ogp[K, Jj, I + 1] + ogp[K, Jj, I + 2] + ogp[K, Jj + 1, I + 1] + ogp[K, Jj + 1, I + 2])
+                n, nn = Jj, ny
+            on = keep != 0
+            if not per[c]:
+                on = on & (n != 0) & (n != nn)
+            mk = mean[K]
+            ap = np.where(on, ap + stiff, ap)
+            d = np.where(on, d + stiff, d)
+            sg = -coriolis_f if c == 0 else coriolis_f
+            rhs = np.where(on, rhs + sg * (tau_c * mk + tau_own * (avg - mk)), rhs)
+        if c == 2 and gravity > 0:
+            gt = gravity / theta0
+            Z = mesh.z_nd
+            zc = 0.125 * (Z[0:nz + 2, 1:ny + 1, 1:nx + 1] + Z[0:nz + 2, 1:ny + 1, 2:nx + 2] + Z[0:nz + 2, 2:ny + 2, 1:nx + 1]
+                          + Z[0:nz + 2, 2:ny + 2, 2:nx + 2] + Z[1:nz + 3, 1:ny + 1, 1:nx + 1] + Z[1:nz + 3, 1:ny + 1, 2:nx + 2]
+                          + Z[1:nz + 3, 2:ny + 2, 1:nx + 1] + Z[1:nz + 3, 2:ny + 2, 2:nx + 2])   # cells k = -1 .. nz
+            th = theta[0, :, 1:-1, 1:-1]   # k = -1 .. nz
+            kk = np.arange(1, nz)          # interior faces
+            gamma = (th[kk + 1] - th[kk]) / (zc[kk + 1] - zc[kk])
+            ap_t = 0.5 * (A_theta[0][kk - 1] + A_theta[0][kk])
+            tau = 1.0 / (1.0 / dtau[c][kk] + ap_t)
+            stiff = gt * np.maximum(gamma, 0.0) * tau
+            on = keep[kk] != 0
+            ap[kk] = np.where(on, ap[kk] + stiff, ap[kk])
+            d[kk] = np.where(on, d[kk] + stiff, d[kk])
+            rhs[kk] = np.where(on, rhs[kk] - gt * tau * 0.5 * (R_theta[kk - 1] + R_theta[kk]), rhs[kk])
+        delta, ch = _zline(loc, A[c], ap, rhs, sweeps, g)
+        vel[c][0, 1:-1, 1:-1, 1:-1] += delta
+        aps.append(ap)
+        ds.append(d)
+        dmax.append(np.max(np.abs(delta)))
+        last.append(ch)
+    return vel, aps, ds, dmax, last
+
+
+def scalar_sweep(mesh, field, R, A, dtau, sweeps, floor=0.0, clip=False):
+    """field: one-ghost (1, ...) array; returns (field, max |dq|, last change, clipped)."""
+    g = PH.ogeom(mesh)
+    ap = 1.0 / dtau + A[0]
+    delta, ch = _zline(0, A, ap, -R, sweeps, g)
+    f = field.copy()
+    f[0, 1:-1, 1:-1, 1:-1] += delta
+    clipped = 0
+    if clip:
+        v = f[0, 1:-1, 1:-1, 1:-1]
+        low = v < floor
+        clipped = int(low.sum())
+        v[low] = floor
+    return f, np.max(np.abs(delta)), ch, clipped
