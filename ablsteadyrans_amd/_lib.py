@@ -146,6 +146,34 @@ class ScalarSweepResultC(C.Structure):
     _fields_ = [("max_dq", C.c_double), ("last_sweep", C.c_double), ("clipped", C.c_int64)]
 
 
+class JvpScalesC(C.Structure):
+    """``gust_jvp_scales`` (Source/Residual.H:72-83)."""
+
+    _fields_ = [("velocity", C.c_double), ("theta", C.c_double), ("tke", C.c_double), ("pressure", C.c_double),
+                ("step", C.c_double)]
+
+
+class ResidualsC(C.Structure):
+    """``gust_residuals``: u, v, w, p, theta, k residual views."""
+
+    _fields_ = [("R", C.POINTER(Array4) * 6)]
+
+
+class SteadyParamsC(C.Structure):
+    """``gust_steady_params`` (Source/SteadyCriterion.H:13-33)."""
+
+    _fields_ = [("residual_tol", C.c_double), ("residual_drop_orders", C.c_double),
+                ("residual_ref_iterations", C.c_int32), ("monitor_rel_tol", C.c_double),
+                ("monitor_window", C.c_int32)]
+
+
+class SteadyResultC(C.Structure):
+    """``gust_steady_result`` (Source/SteadyCriterion.H:57-64)."""
+
+    _fields_ = [("status", C.c_int32), ("iteration", C.c_int32), ("residuals_met", C.c_int32),
+                ("monitors_met", C.c_int32), ("nreasons", C.c_int32), ("reasons_len", C.c_int64)]
+
+
 def _load() -> C.CDLL:
     # One HIP runtime per process: torch ships its own libamdhip64 and the
     # device tensors, streams and events handed to the library come from it.
@@ -234,6 +262,31 @@ _sig = {
     "gust_local_dtau": (C.c_int, [_P(GeometryC)] + [_P(Array4)] * 6 + [C.c_double, _P(Array4), C.c_void_p]),
     "gust_ser_cfl": (C.c_int, [C.c_double] * 6 + [_P(C.c_double)]),
     "gust_face_dtau": (C.c_int, [_P(GeometryC)] + [_P(Array4)] * 4 + [C.c_void_p]),
+    "gust_deterministic_sum": (C.c_int, [_P(GeometryC), _P(Array4), C.c_int32, _P(C.c_double), C.c_void_p]),
+    "gust_global_max": (C.c_int, [_P(GeometryC), C.c_int32, _P(Array4), C.c_int32, _P(C.c_double), C.c_void_p]),
+    "gust_scaled_sums": (C.c_int, [_P(GeometryC), C.c_int32, _P(Array4), _P(Array4), _P(Array4), C.c_double,
+                                   _P(C.c_double), C.c_void_p]),
+    "gust_residual_scales": (C.c_int, [_P(JvpScalesC), C.c_double, _P(C.c_double)]),
+    "gust_residual_scale_height": (C.c_int, [_P(C.c_double), C.c_int32, C.c_double, C.c_double, _P(C.c_double)]),
+    "gust_stage_norms": (C.c_int, [_P(GeometryC), _P(ResidualsC)] + [_P(Array4)] * 5 +
+                         [_P(JvpScalesC), C.c_double, _P(C.c_double), _P(C.c_double), _P(C.c_int32), C.c_void_p]),
+    "gust_confidence": (C.c_int, [_P(GeometryC), _P(ResidualsC), _P(Array4), _P(JvpScalesC), C.c_double,
+                                  _P(C.c_double), _P(C.c_double), _P(C.c_double), C.c_void_p]),
+    "gust_steady_default_params": (C.c_int, [_P(SteadyParamsC)]),
+    "gust_steady_validate": (C.c_int, [_P(SteadyParamsC), _P(C.c_int32), C.c_char_p, C.c_int64]),
+    "gust_steady_create": (C.c_int, [_P(SteadyParamsC), C.c_int32, _P(C.c_char_p), C.c_int32, _P(C.c_char_p),
+                                     _P(C.c_double), _P(C.c_void_p)]),
+    "gust_steady_destroy": (C.c_int, [C.c_void_p]),
+    "gust_steady_add": (C.c_int, [C.c_void_p, _P(C.c_double), C.c_int32, _P(C.c_double), C.c_int32]),
+    "gust_steady_verdict": (C.c_int, [C.c_void_p, _P(SteadyResultC), C.c_char_p, C.c_int64]),
+    "gust_steady_iterations": (C.c_int, [C.c_void_p, _P(C.c_int32)]),
+    "gust_steady_residual_thresholds": (C.c_int, [C.c_void_p, _P(C.c_double), C.c_int32]),
+    "gust_steady_seed_reference": (C.c_int, [C.c_void_p, _P(C.c_double), C.c_int32]),
+    "gust_steady_history_save": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, _P(C.c_int64)]),
+    "gust_steady_history_restore": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
+    "gust_steady_relative_residual": (C.c_int, [C.c_void_p, _P(C.c_double), C.c_int32, C.c_int32, C.c_double,
+                                                _P(C.c_double)]),
+    "gust_steady_next_cfl": (C.c_int, [C.c_void_p, C.c_int32] + [C.c_double] * 6 + [_P(C.c_double)]),
 }
 for _name, (_res, _args) in _sig.items():
     _f = getattr(lib, _name)

@@ -1,6 +1,7 @@
 """Build the native parts in-tree.
 
-* ``ablsteadyrans_amd/libgust_amd.so``: the HIP library (every ``csrc/*.hip``)
+* ``ablsteadyrans_amd/libgust_amd.so``: the HIP library (every ``csrc/*.hip``,
+  plus the host-only ``csrc/*.cpp``)
   for gfx950, C-ABI declared in ``include/gust_amd.h``.
 * ``oracle/liboracle.so``: the FP64 C restatement of the reference (test
   infrastructure; built here so the checker exists on the GPU host too).
@@ -47,6 +48,15 @@ def build_hip(verbose: bool = False) -> str:
         o = os.path.join(PKG, "_build", os.path.basename(s) + ".o")
         cmd = [HIPCC] + [f for f in HIP_FLAGS if f != "-shared"] + [
             "-c", "-I", os.path.join(ROOT, "include"), "-I", CSRC, s, "-o", o]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        objs.append(o)
+    # Host-only C++ (the steady criterion): the host compiler, same rounding.
+    for s in sorted(glob.glob(os.path.join(CSRC, "*.cpp"))):
+        o = os.path.join(PKG, "_build", os.path.basename(s) + ".o")
+        cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-c",
+               "-I", os.path.join(ROOT, "include"), "-I", CSRC, s, "-o", o]
         if verbose:
             print(" ".join(cmd))
         _run(cmd)

@@ -504,6 +504,175 @@ int gust_ser_cfl(double cfl, double r, double cfl0, double cfl_max, double p, do
 int gust_face_dtau(const gust_geometry* geom, const gust_array4* dtau_cc, gust_array4* dtau_x, gust_array4* dtau_y,
                    gust_array4* dtau_z, void* stream);
 
+/* ---- Reductions with rank- and order-independent bits ------------------ */
+
+/* Sum of component comp of a cell-centred field over the tile's valid cells:
+ * one device thread per (i, j) column adds its values in k order; the host
+ * adds the column sums in storage order (i fastest, then j). A MultiFab of
+ * several boxes calls this per box and adds the box sums in box-index order
+ * after the all-reduce of one slot per box (the shim's part). Synchronises.
+ * Replaces: Real deterministic_sum(const MultiFab&, int comp) --
+ *   Source/Reductions.H:19, Source/Reductions.cpp:11-53 (one box's slot). */
+int gust_deterministic_sum(const gust_geometry* geom, const gust_array4* a, int32_t comp, double* sum,
+                           void* stream);
+
+/* Maximum of component comp over the valid points at loc: NaN if any valid
+ * value is NaN, zero reported as +0. Synchronises.
+ * Replaces: Real global_max(const MultiFab&, int comp) --
+ *   Source/Reductions.H:24, Source/Reductions.cpp:55-66. */
+int gust_global_max(const gust_geometry* geom, int32_t loc, const gust_array4* a, int32_t comp, double* max,
+                    void* stream);
+
+/* Over the owned points at loc (on a periodic axis along which loc is
+ * nodal, the hi face is the lo face's copy and is skipped): sums[0] = sum
+ * |(q - offset) / s|, sums[1] = sum (v / s)^2, sums[2] = the point count;
+ * each column in k order on the device, the columns in storage order on
+ * the host. offset may be null. Synchronises.
+ * Replaces: template <bool OFFSET> void scaled_sums(const MultiFab& q, const
+ *   MultiFab& v, const MultiFab* offset, Real s, const Periodicity&,
+ *   Vector<double>& box_sums) -- Source/Residual.cpp:101-150 (one box). */
+int gust_scaled_sums(const gust_geometry* geom, int32_t loc, const gust_array4* q, const gust_array4* v,
+                     const gust_array4* offset, double s, double sums[3], void* stream);
+
+/* Typical field magnitudes, Source/Residual.H:72-83 (JvpScales; defaults
+ * 10 m/s, 10 K, 0.1 m^2/s^2, 100 m^2/s^2, sqrt(DBL_EPSILON)). */
+typedef struct gust_jvp_scales {
+    double velocity, theta, tke, pressure, step;
+} gust_jvp_scales;
+
+/* scale[n] for u, v, w, p, theta, k: U^2/H three times, U/H, U theta_s/H,
+ * U k_s/H, U = velocity.
+ * Replaces: residual_scales(const JvpScales&, Real H, Real scale[6]) --
+ *   Source/Simplec.cpp:374-382. */
+int gust_residual_scales(const gust_jvp_scales* s, double height, double scale[6]);
+
+/* The depth of the z-levels, or the box height when the two agree to
+ * 1000 eps of the box height.
+ * Replaces: Real residual_scale_height(const Mesh&, const Geometry&) --
+ *   Source/Mesh.cpp:332-340. */
+int gust_residual_scale_height(const double* zlevels, int32_t nlevels, double prob_lo_z, double prob_hi_z,
+                               double* height);
+
+/* The six residual fields without ghosts: u, v, w (x-, y-, z-faces), p,
+ * theta, k (cells); Source/Residual.H:61-68 (ResidualFields). */
+typedef struct gust_residuals {
+    const gust_array4* R[6];
+} gust_residuals;
+
+/* res[n] = sqrt(sum R_n^2) / sqrt(npts_n) / scale[n] over the valid points
+ * at the field's location (both copies of a periodic face, as numPts);
+ * mon[0] = max |u|, |v|, |w| over their valid faces; mon[1] = the mean of k
+ * over the cells; mon[2] = the mean of u* over the ground slab (nx x ny
+ * cells at the domain's lowest k) when ustar is not null. *nmon = 2 or 3.
+ * The sums use a fixed reduction tree: the same tile gives the same bits on
+ * every call. Synchronises.
+ * Replaces: void stage_norms(const ResidualFields&, const State&, const
+ *   SurfaceState*, const JvpScales&, Real height, const Geometry&,
+ *   vector<Real>& res, vector<Real>& mon) -- Source/Simplec.cpp:385-409. */
+int gust_stage_norms(const gust_geometry* geom, const gust_residuals* R, const gust_array4* u, const gust_array4* v,
+                     const gust_array4* w, const gust_array4* tke, const gust_array4* ustar,
+                     const gust_jvp_scales* s, double height, double res[6], double mon[3], int32_t* nmon,
+                     void* stream);
+
+/* The share of cells whose residuals (u, v, w at both of the cell's faces)
+ * are all at most threshold[n] scale[n], and the largest height above the
+ * ground (node-mean centre minus node-mean ground of the column) of the
+ * other cells, 0 if none. z_nd needs the cells' nodes. Synchronises.
+ * Replaces: bool SimplecStage::confidence(const vector<Real>& threshold,
+ *   Real& share, Real& height) -- Source/Simplec.cpp:411-445, with
+ *   height_above_ground -- Source/Closure.cpp:261-293. */
+int gust_confidence(const gust_geometry* geom, const gust_residuals* R, const gust_array4* z_nd,
+                    const gust_jvp_scales* s, double height, const double threshold[6], double* share, double* top,
+                    void* stream);
+
+/* ---- Steady criterion (host only) --------------------------------------- */
+
+/* Source/SteadyCriterion.H:13-33 (SteadyParams). */
+typedef struct gust_steady_params {
+    double residual_tol;
+    double residual_drop_orders;
+    int32_t residual_ref_iterations;
+    double monitor_rel_tol;
+    int32_t monitor_window;
+} gust_steady_params;
+
+/* Source/SteadyCriterion.H:50-55 (SteadyStatus). */
+enum { GUST_STEADY_NOT_STEADY = 0, GUST_STEADY_STEADY = 1, GUST_STEADY_DIVERGED = 2 };
+
+/* Source/SteadyCriterion.H:57-64 (SteadyVerdict); the reasons are returned
+ * as text, one line each. */
+typedef struct gust_steady_result {
+    int32_t status;
+    int32_t iteration;
+    int32_t residuals_met;
+    int32_t monitors_met;
+    int32_t nreasons;
+    int64_t reasons_len;   /* bytes of the full text, without the NUL */
+} gust_steady_result;
+
+typedef struct gust_steady gust_steady;
+
+/* Replaces: SteadyParams{} -- Source/SteadyCriterion.H:16-32. */
+int gust_steady_default_params(gust_steady_params* p);
+
+/* One message per violated range, one line each, into buf (NUL-terminated,
+ * truncated to cap); *count = the number of violations.
+ * Replaces: vector<string> validate(const SteadyParams&) --
+ *   Source/SteadyCriterion.cpp:23-48. */
+int gust_steady_validate(const gust_steady_params* p, int32_t* count, char* buf, int64_t cap);
+
+/* Fails (the reference aborts) if p fails validation, a monitor scale is not
+ * finite and > 0, or there is no residual.
+ * Replaces: SteadyCriterion(const SteadyParams&, vector<string>
+ *   residual_names, vector<MonitorSpec>) -- Source/SteadyCriterion.cpp:50-68. */
+int gust_steady_create(const gust_steady_params* p, int32_t nres, const char* const* residual_names, int32_t nmon,
+                       const char* const* monitor_names, const double* monitor_scales, gust_steady** out);
+int gust_steady_destroy(gust_steady* c);
+
+/* Replaces: void SteadyCriterion::add(const vector<Real>& residuals, const
+ *   vector<Real>& monitors) -- Source/SteadyCriterion.cpp:96-131. */
+int gust_steady_add(gust_steady* c, const double* residuals, int32_t nres, const double* monitors, int32_t nmon);
+
+/* reasons: the reasons' text, one per line (may be null with cap 0).
+ * Replaces: SteadyVerdict SteadyCriterion::verdict() const --
+ *   Source/SteadyCriterion.cpp:133-227. */
+int gust_steady_verdict(const gust_steady* c, gust_steady_result* v, char* reasons, int64_t cap);
+
+/* Replaces: int SteadyCriterion::iterations() const -- Source/SteadyCriterion.H:105. */
+int gust_steady_iterations(const gust_steady* c, int32_t* n);
+
+/* out[q] = min(residual_tol, 10^-drop max(reference_q, residual_tol)).
+ * Replaces: vector<Real> SteadyCriterion::residual_thresholds() const --
+ *   Source/SteadyCriterion.cpp:229-238. */
+int gust_steady_residual_thresholds(const gust_steady* c, double* out, int32_t n);
+
+/* Replaces: void SteadyCriterion::seed_reference(const vector<Real>&) --
+ *   Source/SteadyCriterion.cpp:85-94. */
+int gust_steady_seed_reference(gust_steady* c, const double* ref, int32_t n);
+
+/* The recorded history (Source/SteadyCriterion.H:69-78, SteadyHistory) as a
+ * byte image: int32 count, nonfinite, len + bytes of the nonfinite text,
+ * int32 nres, res_ref[nres], res_last[nres] (double), int32 nmon, then per
+ * monitor int32 n and n doubles, oldest first; little-endian. *size = the
+ * image's size; nothing is written if cap is smaller.
+ * Replaces: const SteadyHistory& SteadyCriterion::history() const and
+ *   void restore(const SteadyHistory&) -- Source/SteadyCriterion.cpp:70-83. */
+int gust_steady_history_save(const gust_steady* c, void* buf, int64_t cap, int64_t* size);
+int gust_steady_history_restore(gust_steady* c, const void* buf, int64_t size);
+
+/* r_n after the last add: the previous value when the history holds a NaN or
+ * infinity, 1 during the reference hold (iterations <= hold), else max_q
+ * |res_q| / max(reference_q, residual_tol).
+ * Replaces: the r_n block of SteadyVerdict Driver::step(State&) --
+ *   Source/Driver.cpp:421-437. */
+int gust_steady_relative_residual(const gust_steady* c, const double* residuals, int32_t nres, int32_t hold,
+                                  double previous, double* r);
+
+/* cfl0 during the hold, else ser_cfl(cfl, r, cfl0, cfl_max, p, growth).
+ * Replaces: Real Driver::next_cfl(Real r) const -- Source/Driver.cpp:325-334. */
+int gust_steady_next_cfl(const gust_steady* c, int32_t hold, double cfl, double r, double cfl0, double cfl_max,
+                         double ser_exponent, double max_growth, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -136,6 +136,73 @@ int oracle_scalar_diffusive_fluxes(const oa4* q, const oa4* gamma, const oa4* z,
 int oracle_momentum_diffusive_fluxes(int c, const oa4* u, const oa4* v, const oa4* w, const oa4* nu, const oa4* z,
                                      const ogeom* g, int logz, oa4* G0, oa4* G1, oa4* G2);
 
+/* Reductions of the control loop (oracle/reductions.c, Source/Reductions.cpp,
+ * Source/Residual.cpp:101-150, Source/Simplec.cpp:374-445). lo/hi: the
+ * inclusive box of the points reduced. stage_norms returns the number of
+ * monitors (2, or 3 with ustar: the ground slab at k = lo[2]). */
+double oracle_box_sum(const oa4* a, const int lo[3], const int hi[3]);
+double oracle_global_max(const oa4* a, const int lo[3], const int hi[3]);
+void oracle_scaled_sums(const oa4* q, const oa4* v, const oa4* off, double s, int loc, const ogeom* g,
+                        double out[3]);
+void oracle_residual_scales(double U, double theta_s, double k_s, double H, double scale[6]);
+int oracle_stage_norms(const oa4* const R[6], const oa4* u, const oa4* v, const oa4* w, const oa4* tke,
+                       const oa4* ustar, double U, double theta_s, double k_s, double H, const ogeom* g,
+                       double res[6], double mon[3]);
+void oracle_confidence(const oa4* const R[6], const oa4* z, double U, double theta_s, double k_s, double H,
+                       const double threshold[6], const ogeom* g, double* share, double* top);
+
+/* The residual's physics (oracle/physics.c). Sides periodic in x and y.
+ * Face types as Source/BoundaryConditions.H:76 (FaceType). */
+enum { OFACE_PERIODIC = 0, OFACE_INFLOW, OFACE_OUTFLOW, OFACE_SLIP_WALL, OFACE_NO_SLIP_WALL, OFACE_SURFACE_LAYER,
+       OFACE_GEOSTROPHIC };
+/* Components of the surface-state slab (Source/SurfaceLayer.H:163-174). */
+enum { OSS_USTAR = 0, OSS_SPEED, OSS_D0, OSS_TAU0, OSS_TAU1, OSS_TAU2, OSS_DRAG, OSS_KWALL, OSS_AREA, OSS_INVL,
+       OSS_N };
+
+typedef struct {   /* Source/Closure.H:78-105 (ClosureOptions) */
+    double Cmu0, Cb, Rt_crit, Rt_min, max_geom_lscale, tke_floor, sigma_k, nu_molecular, alpha_molecular, kappa;
+} oclosure;
+
+typedef struct {   /* Source/SurfaceLayer.H:100-146 (neutral, first centre) */
+    double z0, Cmu0, kappa;
+    int wall_k_dirichlet, floor_one;
+} osurface;
+
+typedef struct {   /* Source/Forcing.H:40-96 */
+    double gravity, theta0;
+    int coriolis, three_d, geostrophic;
+    double latitude, rotation_period, ug, vg;
+    double pgrad[3];
+} oforcing;
+
+typedef struct {   /* the z faces of Source/BoundaryConditions.H:82-100 */
+    int zlo, zhi;
+    double zhi_theta_grad;
+    double wall_velocity[2][3];   /* zlo, zhi */
+    double ug, vg;                /* a geostrophic top's wind */
+} obc;
+
+void oracle_height_above_ground(const oa4* z, const ogeom* g, oa4* dist);
+/* Returns -1 if the wall distance plus z0 is not > 0 somewhere. */
+int oracle_closure_evaluate(const oa4* u, const oa4* v, const oa4* w, const oa4* theta, const oa4* tke, const oa4* z,
+                            const oa4* dist, double z0, const oclosure* opt, double gravity, double theta0,
+                            const ogeom* g, oa4* nu, oa4* gth, oa4* gk, oa4* Ps, oa4* Pb, oa4* eps, oa4* dSdk);
+/* ss: OSS_N components over the ground columns and one lateral ghost column. */
+int oracle_surface_state(const oa4* u, const oa4* v, const oa4* z, const osurface* o, const ogeom* g, oa4* ss);
+/* q: u, v, w, p, theta, k with two ghosts; -2 unless x and y are periodic. */
+int oracle_fill_ghosts(oa4* const q[6], const oa4* z, const obc* bc, const osurface* so, const ogeom* g, oa4* ss);
+double oracle_coriolis_parameter(int enabled, double latitude, double rotation_period);
+void oracle_fill_theta_ref(int n, const double* zs, const double* ts, const oa4* z, oa4* theta_ref);
+void oracle_forcing(const oa4* u, const oa4* v, const oa4* w, const oa4* theta, const oa4* theta_ref,
+                    const oforcing* f, const ogeom* g, oa4* Ru, oa4* Rv, oa4* Rw);
+void oracle_set_ground_fluxes(const oa4* ss, const ogeom* g, oa4* Gu, oa4* Gv, oa4* Gt, oa4* Gk);
+void oracle_apply_wall_k(const oa4* ss, const oa4* tke, int dirichlet, const ogeom* g, oa4* Rk);
+void oracle_apply_rows(const oa4* u, const oa4* v, const oa4* w, const oa4* z, const obc* bc, const ogeom* g,
+                       oa4* Rw);
+/* eq 0..4 = u, v, w, theta, k; dSdk may be null. */
+void oracle_implicit_folds(int eq, const oa4* ss, const oa4* dSdk, const oa4* J, const obc* bc, const osurface* so,
+                           const ogeom* g, oa4* A);
+
 #ifdef __cplusplus
 }
 #endif

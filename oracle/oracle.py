@@ -39,6 +39,27 @@ class OGeom(C.Structure):
     ]
 
 
+class OClosure(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("Cmu0", "Cb", "Rt_crit", "Rt_min", "max_geom_lscale", "tke_floor",
+                                          "sigma_k", "nu_molecular", "alpha_molecular", "kappa")]
+
+
+class OSurface(C.Structure):
+    _fields_ = [("z0", C.c_double), ("Cmu0", C.c_double), ("kappa", C.c_double), ("wall_k_dirichlet", C.c_int),
+                ("floor_one", C.c_int)]
+
+
+class OForcing(C.Structure):
+    _fields_ = [("gravity", C.c_double), ("theta0", C.c_double), ("coriolis", C.c_int), ("three_d", C.c_int),
+                ("geostrophic", C.c_int), ("latitude", C.c_double), ("rotation_period", C.c_double),
+                ("ug", C.c_double), ("vg", C.c_double), ("pgrad", C.c_double * 3)]
+
+
+class OBc(C.Structure):
+    _fields_ = [("zlo", C.c_int), ("zhi", C.c_int), ("zhi_theta_grad", C.c_double),
+                ("wall_velocity", (C.c_double * 3) * 2), ("ug", C.c_double), ("vg", C.c_double)]
+
+
 _lib = None
 
 
@@ -83,6 +104,39 @@ def lib():
         L.oracle_subtract_transport_divergence.argtypes = [C.c_int, V, V, V, V, V, P(OGeom), V]
         L.oracle_scalar_diffusive_fluxes.argtypes = [V, V, V, P(OGeom), C.c_int, V, V, V]
         L.oracle_momentum_diffusive_fluxes.argtypes = [C.c_int, V, V, V, V, V, P(OGeom), C.c_int, V, V, V]
+        I3 = C.c_int * 3
+        D = C.c_double
+        L.oracle_box_sum.argtypes = [V, I3, I3]
+        L.oracle_box_sum.restype = D
+        L.oracle_global_max.argtypes = [V, I3, I3]
+        L.oracle_global_max.restype = D
+        L.oracle_scaled_sums.argtypes = [V, V, V, D, C.c_int, P(OGeom), P(D)]
+        L.oracle_scaled_sums.restype = None
+        L.oracle_residual_scales.argtypes = [D, D, D, D, P(D)]
+        L.oracle_residual_scales.restype = None
+        L.oracle_stage_norms.argtypes = [P(OA4) * 6, V, V, V, V, V, D, D, D, D, P(OGeom), P(D), P(D)]
+        L.oracle_confidence.argtypes = [P(OA4) * 6, V, D, D, D, D, P(D), P(OGeom), P(D), P(D)]
+        L.oracle_confidence.restype = None
+        L.oracle_height_above_ground.argtypes = [V, P(OGeom), V]
+        L.oracle_height_above_ground.restype = None
+        L.oracle_closure_evaluate.argtypes = [V, V, V, V, V, V, V, D, P(OClosure), D, D, P(OGeom), V, V, V, V, V, V,
+                                              V]
+        L.oracle_surface_state.argtypes = [V, V, V, P(OSurface), P(OGeom), V]
+        L.oracle_fill_ghosts.argtypes = [P(OA4) * 6, V, P(OBc), P(OSurface), P(OGeom), V]
+        L.oracle_coriolis_parameter.argtypes = [C.c_int, D, D]
+        L.oracle_coriolis_parameter.restype = D
+        L.oracle_fill_theta_ref.argtypes = [C.c_int, P(D), P(D), V, V]
+        L.oracle_fill_theta_ref.restype = None
+        L.oracle_forcing.argtypes = [V, V, V, V, V, P(OForcing), P(OGeom), V, V, V]
+        L.oracle_forcing.restype = None
+        L.oracle_set_ground_fluxes.argtypes = [V, P(OGeom), V, V, V, V]
+        L.oracle_set_ground_fluxes.restype = None
+        L.oracle_apply_wall_k.argtypes = [V, V, C.c_int, P(OGeom), V]
+        L.oracle_apply_wall_k.restype = None
+        L.oracle_apply_rows.argtypes = [V, V, V, V, P(OBc), P(OGeom), V]
+        L.oracle_apply_rows.restype = None
+        L.oracle_implicit_folds.argtypes = [C.c_int, V, V, V, P(OBc), P(OSurface), P(OGeom), V]
+        L.oracle_implicit_folds.restype = None
         for f in ("oracle_omega_from_w", "oracle_w_from_omega", "oracle_transport_fluxes",
                   "oracle_velocity_divergence", "oracle_pressure_gradient"):
             getattr(L, f).restype = None
@@ -408,3 +462,54 @@ def momentum_diffusion(vel, vel_begin, nu, nu_begin, z, z_begin, J, J_begin, geo
         G, b = momentum_diffusive_fluxes(c, vel, vel_begin, nu, nu_begin, z, z_begin, geom, logz)
         out.append(flux_divergence(1 << c, G, b, J, J_begin, geom))
     return out
+
+
+# --- reductions (oracle/reductions.c) ----------------------------------------------------------------------
+
+def box_sum(a, begin, lo, hi):
+    """Source/Reductions.cpp:11-53 over one box: column sums in k order, added in storage order."""
+    return lib().oracle_box_sum(_v(a, begin), (C.c_int * 3)(*lo), (C.c_int * 3)(*hi))
+
+
+def global_max(a, begin, lo, hi):
+    """Source/Reductions.cpp:55-66."""
+    return lib().oracle_global_max(_v(a, begin), (C.c_int * 3)(*lo), (C.c_int * 3)(*hi))
+
+
+def scaled_sums(q, v, begin, s, loc, geom, offset=None):
+    """Source/Residual.cpp:101-150 on one box: (sum |q~|, sum v~^2, count)."""
+    out = (C.c_double * 3)()
+    lib().oracle_scaled_sums(_v(q, begin), _v(v, begin), _v(offset, begin), float(s), int(loc), geom, out)
+    return out[0], out[1], out[2]
+
+
+def residual_scales(U, theta_s, k_s, H):
+    """Source/Simplec.cpp:374-382."""
+    out = (C.c_double * 6)()
+    lib().oracle_residual_scales(U, theta_s, k_s, H, out)
+    return list(out)
+
+
+def _rviews(R, begin):
+    keep = [a4(r if r.ndim == 4 else r[None], begin) for r in R]
+    arr = (C.POINTER(OA4) * 6)(*[C.pointer(k) for k in keep])
+    return arr, keep
+
+
+def stage_norms(R, R_begin, u, v, w, vel_begin, tke, tke_begin, U, theta_s, k_s, H, geom, ustar=None,
+                ustar_begin=None):
+    """Source/Simplec.cpp:385-409: (res[6], monitors)."""
+    arr, _keep = _rviews(R, R_begin)
+    res, mon = (C.c_double * 6)(), (C.c_double * 3)()
+    n = lib().oracle_stage_norms(arr, _v(u, vel_begin), _v(v, vel_begin), _v(w, vel_begin), _v(tke, tke_begin),
+                                 _v(ustar, ustar_begin), U, theta_s, k_s, H, geom, res, mon)
+    return list(res), list(mon[:n])
+
+
+def confidence(R, R_begin, z, z_begin, U, theta_s, k_s, H, thresholds, geom):
+    """Source/Simplec.cpp:411-445: (share, top height)."""
+    arr, _keep = _rviews(R, R_begin)
+    share, top = C.c_double(0), C.c_double(0)
+    lib().oracle_confidence(arr, _v(z, z_begin), U, theta_s, k_s, H, (C.c_double * 6)(*thresholds), geom,
+                            C.byref(share), C.byref(top))
+    return share.value, top.value
