@@ -174,6 +174,90 @@ class SteadyResultC(C.Structure):
                 ("monitors_met", C.c_int32), ("nreasons", C.c_int32), ("reasons_len", C.c_int64)]
 
 
+class ClosureOptionsC(C.Structure):
+    """``gust_closure_options`` (Source/Closure.H:78-105)."""
+
+    _fields_ = [(n, C.c_double) for n in ("Cmu0", "Cb", "Rt_crit", "Rt_min", "max_geom_lscale", "tke_floor",
+                                          "sigma_k", "nu_molecular", "alpha_molecular", "kappa")]
+
+
+class SurfaceOptionsC(C.Structure):
+    """``gust_surface_options`` (Source/SurfaceLayer.H:100-146)."""
+
+    _fields_ = [("z0", C.c_double), ("Cmu0", C.c_double), ("kappa", C.c_double), ("wall_k", C.c_int32),
+                ("denominator_floor", C.c_int32)]
+
+
+class ForcingC(C.Structure):
+    """``gust_forcing`` (Source/Forcing.H:40-96)."""
+
+    _fields_ = [("gravity", C.c_double), ("theta0", C.c_double), ("coriolis", C.c_int32), ("coriolis_3d", C.c_int32),
+                ("geostrophic", C.c_int32), ("latitude", C.c_double), ("rotation_period", C.c_double),
+                ("ug", C.c_double), ("vg", C.c_double), ("pressure_gradient", C.c_double * 3)]
+
+
+class PhysicsC(C.Structure):
+    """``gust_physics`` (Source/Residual.H:90-128)."""
+
+    _fields_ = [("use_closure", C.c_int32), ("closure", ClosureOptionsC), ("nu", C.POINTER(Array4)),
+                ("gamma_theta", C.POINTER(Array4)), ("gamma_tke", C.POINTER(Array4)), ("surface", SurfaceOptionsC),
+                ("forcing", ForcingC), ("use_bc", C.c_int32), ("face", C.c_int32 * 6),
+                ("zhi_theta_grad", C.c_double), ("wall_velocity", (C.c_double * 3) * 6),
+                ("n_theta_ref", C.c_int32), ("theta_ref_z", C.c_double * 64), ("theta_ref", C.c_double * 64),
+                ("momentum_limiter", C.c_int32), ("theta_limiter", C.c_int32), ("momentum_eps", C.c_double),
+                ("theta_eps", C.c_double), ("face_mean", C.c_int32)]
+
+
+class StateC(C.Structure):
+    """``gust_state``: u, v, w, p, theta, k views."""
+
+    _fields_ = [("f", C.POINTER(Array4) * 6)]
+
+
+class MeshC(C.Structure):
+    """``gust_mesh``."""
+
+    _fields_ = [("z_nd", C.POINTER(Array4)), ("detJ", C.POINTER(Array4)), ("ax", C.POINTER(Array4)),
+                ("ay", C.POINTER(Array4)), ("az", C.POINTER(Array4))]
+
+
+class ClosureFieldsC(C.Structure):
+    """``gust_closure_fields``."""
+
+    _fields_ = [(n, C.POINTER(Array4)) for n in ("nu", "gamma_theta", "gamma_tke", "Ps", "Pb", "eps", "dSdk")]
+
+
+class SimplecOptionsC(C.Structure):
+    """``gust_simplec_options`` (Source/Simplec.H:71-83 and the stage's settings)."""
+
+    _fields_ = [("pressure_reltol", C.c_double), ("alpha_p", C.c_double), ("rotation_relax", C.c_double),
+                ("w_vnn_cap", C.c_double), ("w_vnn_cap_slope_margin", C.c_double), ("sweeps", C.c_int32),
+                ("sweep", LineOptionsC), ("tke_floor", C.c_double), ("scales", JvpScalesC), ("height", C.c_double),
+                ("poisson", PoissonOptionsC)]
+
+
+class SimplecInfoC(C.Structure):
+    """``gust_simplec_info`` (Source/Simplec.H:178-184)."""
+
+    _fields_ = [("predictor", MomentumPredictorResultC), ("pressure", PressureCorrectionResultC),
+                ("theta", ScalarSweepResultC), ("tke", ScalarSweepResultC)]
+
+
+class DriverOptionsC(C.Structure):
+    """``gust_driver_options`` (Source/Driver.H:58-66)."""
+
+    _fields_ = [("max_iterations", C.c_int32), ("cfl0", C.c_double), ("cfl_max", C.c_double),
+                ("ser_exponent", C.c_double), ("ser_max_growth", C.c_double), ("criterion", SteadyParamsC)]
+
+
+class DriverResultC(C.Structure):
+    """``gust_driver_result``."""
+
+    _fields_ = [("iterations", C.c_int32), ("status", C.c_int32), ("cfl", C.c_double),
+                ("relative_residual", C.c_double), ("res", C.c_double * 6), ("mon", C.c_double * 3),
+                ("nmon", C.c_int32)]
+
+
 def _load() -> C.CDLL:
     # One HIP runtime per process: torch ships its own libamdhip64 and the
     # device tensors, streams and events handed to the library come from it.
@@ -272,6 +356,25 @@ _sig = {
                          [_P(JvpScalesC), C.c_double, _P(C.c_double), _P(C.c_double), _P(C.c_int32), C.c_void_p]),
     "gust_confidence": (C.c_int, [_P(GeometryC), _P(ResidualsC), _P(Array4), _P(JvpScalesC), C.c_double,
                                   _P(C.c_double), _P(C.c_double), _P(C.c_double), C.c_void_p]),
+    "gust_height_above_ground": (C.c_int, [_P(GeometryC), _P(Array4), _P(Array4), C.c_void_p]),
+    "gust_closure_evaluate": (C.c_int, [_P(GeometryC), _P(StateC), _P(Array4), _P(Array4), C.c_double,
+                                        _P(ClosureOptionsC), C.c_double, C.c_double, _P(ClosureFieldsC), C.c_void_p]),
+    "gust_surface_state": (C.c_int, [_P(GeometryC), _P(Array4), _P(Array4), _P(Array4), _P(SurfaceOptionsC),
+                                     _P(Array4), C.c_void_p]),
+    "gust_fill_ghosts": (C.c_int, [_P(GeometryC), _P(StateC), _P(MeshC), _P(PhysicsC), _P(Array4), C.c_void_p]),
+    "gust_residual": (C.c_int, [_P(GeometryC), _P(StateC), _P(MeshC), _P(PhysicsC), _P(ResidualsC),
+                                _P(ClosureFieldsC), _P(Array4), C.c_void_p]),
+    "gust_implicit_rows": (C.c_int, [C.c_int32, _P(GeometryC), _P(FacesC), _P(MeshC), _P(PhysicsC),
+                                     _P(ClosureFieldsC), _P(Array4), _P(Array4), C.c_void_p]),
+    "gust_simplec_default_options": (C.c_int, [_P(SimplecOptionsC)]),
+    "gust_simplec_create": (C.c_int, [_P(GeometryC), _P(MeshC), _P(PhysicsC), _P(SimplecOptionsC), _P(C.c_void_p),
+                                      C.c_void_p]),
+    "gust_simplec_destroy": (C.c_int, [C.c_void_p]),
+    "gust_simplec_init": (C.c_int, [C.c_void_p, _P(StateC), C.c_void_p]),
+    "gust_simplec_iterate": (C.c_int, [C.c_void_p, _P(StateC), _P(Array4), _P(SimplecInfoC), C.c_void_p]),
+    "gust_simplec_norms": (C.c_int, [C.c_void_p, _P(C.c_double), _P(C.c_double), _P(C.c_int32)]),
+    "gust_simplec_nu": (C.c_int, [C.c_void_p, _P(Array4)]),
+    "gust_steady_solve": (C.c_int, [C.c_void_p, _P(StateC), _P(DriverOptionsC), _P(DriverResultC), C.c_void_p]),
     "gust_steady_default_params": (C.c_int, [_P(SteadyParamsC)]),
     "gust_steady_validate": (C.c_int, [_P(SteadyParamsC), _P(C.c_int32), C.c_char_p, C.c_int64]),
     "gust_steady_create": (C.c_int, [_P(SteadyParamsC), C.c_int32, _P(C.c_char_p), C.c_int32, _P(C.c_char_p),

@@ -1,0 +1,429 @@
+// simplec_stage.hip -- one SIMPLEC iteration of the steady RANS equations on
+// a tile and the steady driver loop around it (Source/Simplec.cpp:293-506,
+// Source/Driver.cpp:325-492 with one stage). The stage owns its fields for
+// its lifetime (allocated at creation, as the reference's workspace is kept
+// after first use) and composes the library's entry points in the
+// reference's order:
+//
+//   1. ghosts and the wall state at q;          2. R(q), the closure's fields, the norms;
+//   3. the momentum predictor with the implicit rows at q's transport;
+//   4. ghosts, the pressure correction;         5. theta's convection at the corrected
+//      transport, the theta and k sweeps with the rows at the corrected transport.
+#include "cgrid.hpp"
+
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+namespace gust_amd {
+
+namespace {
+
+using cgrid::blocks_for;
+using cgrid::Box3;
+using cgrid::box3;
+using cgrid::check_geom;
+using cgrid::CV;
+using cgrid::kThreads;
+using cgrid::V;
+
+// c1 -= c0; R += c1 (Source/Simplec.cpp:497-498).
+__global__ __launch_bounds__(kThreads) void conv_update_kernel(Box3 b, V c1, CV c0, V r)
+{
+    GUST_GRID_LOOP(b)
+    {
+        int i, j, k;
+        b.at(t, i, j, k);
+        const double d = c1(i, j, k) - c0(i, j, k);
+        c1(i, j, k) = d;
+        r(i, j, k) += d;
+    }
+}
+
+// A device field owned by the stage.
+struct Field {
+    double* p = nullptr;
+    gust_array4 v{};
+    Field() = default;
+    Field(const Field&) = delete;
+    Field& operator=(const Field&) = delete;
+    ~Field()
+    {
+        if (p) { (void)hipFree(p); }
+    }
+    void alloc(const IBox& b, int ncomp)
+    {
+        const int64_t n = b.npts();
+        GUST_HIP(hipMalloc(&p, size_t(n) * ncomp * sizeof(double)));
+        GUST_HIP(hipMemset(p, 0, size_t(n) * ncomp * sizeof(double)));
+        v.p = p;
+        v.jstride = b.len(0);
+        v.kstride = int64_t(b.len(0)) * b.len(1);
+        v.nstride = n;
+        for (int d = 0; d < 3; ++d) {
+            v.begin[d] = b.lo[d];
+            v.end[d] = b.hi[d] + 1;
+        }
+        v.ncomp = ncomp;
+    }
+};
+
+void check_rc(int rc)
+{
+    if (rc != 0) { throw Fail{rc}; }
+}
+
+const int kLocs[6] = {GUST_LOC_XFACE, GUST_LOC_YFACE, GUST_LOC_ZFACE, GUST_LOC_CELL, GUST_LOC_CELL, GUST_LOC_CELL};
+
+bool dirichlet_normal(int t)
+{
+    return t == GUST_FACE_INFLOW || t == GUST_FACE_SLIP_WALL || t == GUST_FACE_NO_SLIP_WALL ||
+           t == GUST_FACE_SURFACE_LAYER || t == GUST_FACE_GEOSTROPHIC;
+}
+
+}   // namespace
+
+struct SimplecStage {
+    gust_geometry g{};
+    gust_array4 z{}, J{}, ax{}, ay{}, az{};
+    gust_mesh mesh{};
+    gust_physics phys{};
+    gust_simplec_options opt{};
+    gust_poisson* solver = nullptr;
+    int32_t pbc[6]{};
+    bool wall = false;
+    Field R[6], cf[7], ss, F[3], dtf[3], A[3], At, Ak, mask[3], ap[3], d[3], conv0, conv1, dtau;
+    gust_closure_fields cfv{};
+    double res[6]{}, mon[3]{};
+    int32_t nmon = 0;
+
+    ~SimplecStage()
+    {
+        if (solver) { gust_poisson_destroy(solver); }
+    }
+
+    gust_residuals residuals() const
+    {
+        gust_residuals r;
+        for (int n = 0; n < 6; ++n) { r.R[n] = &R[n].v; }
+        return r;
+    }
+};
+
+namespace {
+
+SimplecStage* as(gust_simplec* s) { return reinterpret_cast<SimplecStage*>(s); }
+const SimplecStage* as(const gust_simplec* s) { return reinterpret_cast<const SimplecStage*>(s); }
+
+void check_options(const char* op, const gust_simplec_options& o)
+{
+    if (std::isnan(o.pressure_reltol) || !(o.pressure_reltol > 0 && o.pressure_reltol < 1)) {
+        GUST_FAIL("gust: %s: simplec.pressure_reltol must be in (0, 1)", op);
+    }
+    if (std::isnan(o.alpha_p) || !(o.alpha_p > 0 && o.alpha_p <= 1)) {
+        GUST_FAIL("gust: %s: simplec.alpha_p must be in (0, 1]", op);
+    }
+    if (std::isnan(o.rotation_relax) || !(o.rotation_relax > 0 && o.rotation_relax <= 1)) {
+        GUST_FAIL("gust: %s: simplec.rotation_relax must be in (0, 1]", op);
+    }
+    if (o.sweeps < 1) { GUST_FAIL("gust: %s: sweeps must be >= 1 (steady.zline_sweeps)", op); }
+    if (std::isnan(o.tke_floor) || !(o.tke_floor >= 0)) {
+        GUST_FAIL("gust: %s: tke_floor must be >= 0 (turb.tke_floor)", op);
+    }
+    for (const double sc : {o.scales.velocity, o.scales.theta, o.scales.tke}) {
+        if (std::isnan(sc) || !(sc > 0 && std::isfinite(sc))) {
+            GUST_FAIL("gust: %s: the residual scales (jvp_scales) must be finite and > 0", op);
+        }
+    }
+    if (!(o.height > 0 && std::isfinite(o.height))) { GUST_FAIL("gust: %s: the scale height must be > 0", op); }
+}
+
+double coriolis_f(const gust_forcing& f)
+{
+    if (!f.coriolis) { return 0; }
+    const double pi = 3.14159265358979323846;
+    return 4 * pi * std::sin(f.latitude * pi / 180) / f.rotation_period;
+}
+
+// One iteration (Source/Simplec.cpp:452-506).
+void iterate(SimplecStage& s, gust_state* q, const gust_array4* dtau, gust_simplec_info* info, void* stream)
+{
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const gust_geometry* geom = &s.g;
+    gust_array4* ssv = s.wall ? &s.ss.v : nullptr;
+    gust_simplec_info local{};
+    gust_simplec_info& inf = info ? *info : local;
+    // 1-2. ghosts, the residual, the norms
+    check_rc(gust_fill_ghosts(geom, q, &s.mesh, &s.phys, ssv, stream));
+    gust_residuals R = s.residuals();
+    check_rc(gust_residual(geom, q, &s.mesh, &s.phys, &R, s.phys.use_closure ? &s.cfv : nullptr, ssv, stream));
+    check_rc(gust_stage_norms(geom, &R, q->f[0], q->f[1], q->f[2], q->f[5], ssv, &s.opt.scales, s.opt.height, s.res,
+                              s.mon, &s.nmon, stream));
+    // 3. the momentum predictor
+    check_rc(gust_transport_fluxes(geom, q->f[0], q->f[1], q->f[2], &s.z, &s.ax, &s.ay, &s.F[0].v, &s.F[1].v,
+                                   &s.F[2].v, stream));
+    check_rc(gust_face_dtau(geom, dtau, &s.dtf[0].v, &s.dtf[1].v, &s.dtf[2].v, stream));
+    gust_faces Ff{{&s.F[0].v, &s.F[1].v, &s.F[2].v}};
+    const gust_closure_fields* cf = s.phys.use_closure ? &s.cfv : nullptr;
+    const int eqs[3] = {GUST_EQ_U, GUST_EQ_V, GUST_EQ_W};
+    for (int c = 0; c < 3; ++c) {
+        check_rc(gust_implicit_rows(eqs[c], geom, &Ff, &s.mesh, &s.phys, cf, ssv, &s.A[c].v, stream));
+    }
+    const bool buoy = s.phys.forcing.gravity > 0;
+    if (buoy) { check_rc(gust_implicit_rows(GUST_EQ_THETA, geom, &Ff, &s.mesh, &s.phys, cf, ssv, &s.At.v, stream)); }
+    gust_faces vel{{q->f[0], q->f[1], q->f[2]}};
+    gust_faces Rf{{&s.R[0].v, &s.R[1].v, &s.R[2].v}};
+    gust_faces Af{{&s.A[0].v, &s.A[1].v, &s.A[2].v}};
+    gust_faces Df{{&s.dtf[0].v, &s.dtf[1].v, &s.dtf[2].v}};
+    gust_faces Mf{{&s.mask[0].v, &s.mask[1].v, &s.mask[2].v}};
+    gust_faces Pf{{&s.ap[0].v, &s.ap[1].v, &s.ap[2].v}};
+    gust_faces Qf{{&s.d[0].v, &s.d[1].v, &s.d[2].v}};
+    gust_predictor_options po{};
+    po.sweeps = s.opt.sweeps;
+    po.rotation_relax = s.opt.rotation_relax;
+    po.w_vnn_cap = s.opt.w_vnn_cap;
+    po.w_vnn_slope_margin = s.opt.w_vnn_cap_slope_margin;
+    po.coriolis_f = coriolis_f(s.phys.forcing);
+    po.gravity = s.phys.forcing.gravity;
+    po.theta0 = s.phys.forcing.theta0;
+    po.sweep = s.opt.sweep;
+    check_rc(gust_momentum_predictor(geom, &vel, q->f[4], &s.z, &Rf, &s.R[4].v, &Af, buoy ? &s.At.v : nullptr, &Df,
+                                     &Mf, &po, &Pf, &Qf, &inf.predictor, stream));
+    // 4. the pressure correction on u* (its ghosts refilled; the wall state
+    // of the iteration stays that of q)
+    check_rc(gust_fill_ghosts(geom, q, &s.mesh, &s.phys, nullptr, stream));
+    check_rc(gust_pressure_correction(s.solver, geom, q->f[0], q->f[1], q->f[2], q->f[3], &s.z, &s.ax, &s.ay, &s.az,
+                                      &s.J, &s.d[0].v, &s.d[1].v, &s.d[2].v, s.pbc, s.opt.alpha_p, nullptr, 0,
+                                      &inf.pressure, stream));
+    // 5. theta's convection at the corrected transport, the scalar sweeps
+    check_rc(gust_convection(GUST_LOC_CELL, geom, q->f[4], &s.F[0].v, &s.F[1].v, &s.F[2].v, &s.J,
+                             s.phys.theta_limiter, s.phys.theta_eps, 0, &s.conv0.v, stream));
+    check_rc(gust_fill_ghosts(geom, q, &s.mesh, &s.phys, nullptr, stream));
+    check_rc(gust_transport_fluxes(geom, q->f[0], q->f[1], q->f[2], &s.z, &s.ax, &s.ay, &s.F[0].v, &s.F[1].v,
+                                   &s.F[2].v, stream));
+    check_rc(gust_convection(GUST_LOC_CELL, geom, q->f[4], &s.F[0].v, &s.F[1].v, &s.F[2].v, &s.J,
+                             s.phys.theta_limiter, s.phys.theta_eps, 0, &s.conv1.v, stream));
+    {
+        const Box3 b = box3(loc_box(s.g, GUST_LOC_CELL));
+        conv_update_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<double>(&s.conv1.v),
+                                                                      dview<const double>(&s.conv0.v),
+                                                                      dview<double>(&s.R[4].v));
+        GUST_HIP(hipGetLastError());
+    }
+    check_rc(gust_implicit_rows(GUST_EQ_THETA, geom, &Ff, &s.mesh, &s.phys, cf, ssv, &s.At.v, stream));
+    check_rc(gust_scalar_sweep(geom, GUST_EQ_THETA, q->f[4], &s.R[4].v, &s.At.v, dtau, s.opt.sweeps, 0.0, 0,
+                               &s.opt.sweep, &inf.theta, stream));
+    check_rc(gust_implicit_rows(GUST_EQ_TKE, geom, &Ff, &s.mesh, &s.phys, cf, ssv, &s.Ak.v, stream));
+    check_rc(gust_scalar_sweep(geom, GUST_EQ_TKE, q->f[5], &s.R[5].v, &s.Ak.v, dtau, s.opt.sweeps, s.opt.tke_floor, 1,
+                               &s.opt.sweep, &inf.tke, stream));
+}
+
+}   // namespace
+
+}   // namespace gust_amd
+
+using namespace gust_amd;
+
+extern "C" {
+
+int gust_simplec_default_options(gust_simplec_options* o)
+{
+    return guarded([&] {
+        if (!o) { GUST_FAIL("gust: SimplecOptions: null output"); }
+        std::memset(o, 0, sizeof *o);
+        o->pressure_reltol = 1.0e-3;
+        o->alpha_p = 1;
+        o->rotation_relax = 0.3;
+        o->w_vnn_cap = 0;
+        o->w_vnn_cap_slope_margin = 4;
+        o->sweeps = 2;
+        o->sweep = gust_line_options{GUST_SWEEP_ZJACOBI, GUST_GHOSTS_LAGGED, 0, GUST_ORDER_SEQUENTIAL, 0};
+        o->tke_floor = 0;
+        o->scales = gust_jvp_scales{10, 10, 0.1, 100, std::sqrt(DBL_EPSILON)};
+        o->height = 1;
+        check_rc(gust_poisson_default_options(&o->poisson));
+    });
+}
+
+int gust_simplec_create(const gust_geometry* geom, const gust_mesh* mesh, const gust_physics* phys,
+                        const gust_simplec_options* opt, gust_simplec** out, void* stream)
+{
+    return guarded([&] {
+        const char* op = "SimplecStage";
+        check_geom(op, geom);
+        if (!mesh || !phys || !opt || !out) { GUST_FAIL("gust: %s: null argument", op); }
+        *out = nullptr;
+        check_options(op, *opt);
+        if (!phys->use_bc) { GUST_FAIL("gust: %s: the residual inputs need the boundary conditions (bc)", op); }
+        if (!phys->use_closure && !phys->nu) {
+            GUST_FAIL("gust: %s: without a closure the residual inputs need the diffusivities", op);
+        }
+        check_nodal_heights(op, mesh->z_nd, *geom);
+        check_field(op, "detJ", mesh->detJ, *geom, GUST_LOC_CELL, 1);
+        check_field(op, "ax", mesh->ax, *geom, GUST_LOC_XFACE, 1);
+        check_field(op, "ay", mesh->ay, *geom, GUST_LOC_YFACE, 1);
+        check_field(op, "az", mesh->az, *geom, GUST_LOC_ZFACE, 1);
+        auto s = std::make_unique<SimplecStage>();
+        s->g = *geom;
+        s->z = *mesh->z_nd;
+        s->J = *mesh->detJ;
+        s->ax = *mesh->ax;
+        s->ay = *mesh->ay;
+        s->az = *mesh->az;
+        s->mesh = gust_mesh{&s->z, &s->J, &s->ax, &s->ay, &s->az};
+        s->phys = *phys;
+        s->opt = *opt;
+        s->wall = phys->face[2] == GUST_FACE_SURFACE_LAYER;
+        const gust_geometry& g = *geom;
+        const IBox cells = loc_box(g, GUST_LOC_CELL);
+        for (int n = 0; n < 6; ++n) { s->R[n].alloc(loc_box(g, kLocs[n]), 1); }
+        for (int n = 0; n < 3; ++n) { s->cf[n].alloc(grow(cells, 1, 1, 1), 1); }
+        for (int n = 3; n < 7; ++n) { s->cf[n].alloc(cells, 1); }
+        s->cfv = gust_closure_fields{&s->cf[0].v, &s->cf[1].v, &s->cf[2].v, &s->cf[3].v,
+                                     &s->cf[4].v, &s->cf[5].v, &s->cf[6].v};
+        IBox slab = grow(cells, 1, 1, 0);
+        slab.hi[2] = slab.lo[2];
+        s->ss.alloc(slab, GUST_SS_N);
+        for (int c = 0; c < 3; ++c) {
+            const IBox fb = loc_box(g, 1 << c);
+            s->F[c].alloc(grow(fb, 1, 1, 1), 1);
+            s->dtf[c].alloc(fb, 1);
+            s->A[c].alloc(fb, GUST_N_COEFS);
+            s->mask[c].alloc(fb, 1);
+            s->ap[c].alloc(fb, 1);
+            s->d[c].alloc(fb, 1);
+        }
+        s->At.alloc(cells, GUST_N_COEFS);
+        s->Ak.alloc(cells, GUST_N_COEFS);
+        s->conv0.alloc(cells, 1);
+        s->conv1.alloc(cells, 1);
+        // The Dirichlet rows: w's face points on the ground and the top
+        // (BoundaryConditions::dirichlet_rows, Source/BoundaryConditions.cpp:852-876).
+        {
+            const IBox wb = loc_box(g, GUST_LOC_ZFACE);
+            const int64_t plane = int64_t(wb.len(0)) * wb.len(1);
+            std::vector<double> ones(size_t(plane), 1.0);
+            double* base = s->mask[2].p;
+            if (dirichlet_normal(phys->face[2])) {
+                GUST_HIP(hipMemcpy(base, ones.data(), size_t(plane) * sizeof(double), hipMemcpyHostToDevice));
+            }
+            if (dirichlet_normal(phys->face[5])) {
+                GUST_HIP(hipMemcpy(base + plane * (wb.len(2) - 1), ones.data(), size_t(plane) * sizeof(double),
+                                   hipMemcpyHostToDevice));
+            }
+        }
+        // The Poisson controls with the correction's tolerance; the types of
+        // the faces as ERF's get_fft_bc (Source/BoundaryConditions.cpp:300-320).
+        gust_poisson_options po = opt->poisson;
+        po.reltol = opt->pressure_reltol;
+        check_rc(gust_poisson_create(geom, mesh->z_nd, mesh->detJ, mesh->ax, mesh->ay, mesh->az, &po, &s->solver,
+                                     stream));
+        for (int f = 0; f < 6; ++f) {
+            const int d = f % 3;
+            if (d < 2 && geom->periodic[d]) { s->pbc[f] = GUST_PF_PERIODIC; }
+            else if (phys->face[f] == GUST_FACE_OUTFLOW) { s->pbc[f] = GUST_PF_DIRICHLET; }
+            else { s->pbc[f] = GUST_PF_NEUMANN; }
+        }
+        *out = reinterpret_cast<gust_simplec*>(s.release());
+    });
+}
+
+int gust_simplec_destroy(gust_simplec* s)
+{
+    delete as(s);
+    return 0;
+}
+
+int gust_simplec_init(gust_simplec* s, gust_state* q, void* stream)
+{
+    return guarded([&] {
+        if (!s) { GUST_FAIL("gust: SimplecStage::init: null stage"); }
+        SimplecStage& st = *as(s);
+        check_rc(gust_fill_ghosts(&st.g, q, &st.mesh, &st.phys, st.wall ? &st.ss.v : nullptr, stream));
+    });
+}
+
+int gust_simplec_iterate(gust_simplec* s, gust_state* q, const gust_array4* dtau, gust_simplec_info* info,
+                         void* stream)
+{
+    return guarded([&] {
+        if (!s) { GUST_FAIL("gust: SimplecStage::iterate: null stage"); }
+        SimplecStage& st = *as(s);
+        check_field("SimplecStage::iterate", "dtau", dtau, st.g, GUST_LOC_CELL, 0);
+        iterate(st, q, dtau, info, stream);
+    });
+}
+
+int gust_simplec_norms(const gust_simplec* s, double res[6], double mon[3], int32_t* nmon)
+{
+    return guarded([&] {
+        if (!s || !res || !mon || !nmon) { GUST_FAIL("gust: SimplecStage::residual_norms: null argument"); }
+        const SimplecStage& st = *as(s);
+        for (int n = 0; n < 6; ++n) { res[n] = st.res[n]; }
+        for (int n = 0; n < 3; ++n) { mon[n] = st.mon[n]; }
+        *nmon = st.nmon;
+    });
+}
+
+int gust_simplec_nu(const gust_simplec* s, gust_array4* nu)
+{
+    return guarded([&] {
+        if (!s || !nu) { GUST_FAIL("gust: SimplecStage::nu: null argument"); }
+        const SimplecStage& st = *as(s);
+        *nu = st.phys.use_closure ? st.cf[0].v : *st.phys.nu;
+    });
+}
+
+int gust_steady_solve(gust_simplec* s, gust_state* q, const gust_driver_options* opt, gust_driver_result* result,
+                      void* stream)
+{
+    return guarded([&] {
+        const char* op = "Driver::run";
+        if (!s || !q || !opt || !result) { GUST_FAIL("gust: %s: null argument", op); }
+        SimplecStage& st = *as(s);
+        if (opt->max_iterations < 1) { GUST_FAIL("gust: %s: steady.max_iterations must be >= 1", op); }
+        if (!(opt->cfl0 > 0 && std::isfinite(opt->cfl0)) || !(opt->cfl_max >= opt->cfl0 && std::isfinite(opt->cfl_max))) {
+            GUST_FAIL("gust: %s: steady.cfl0 must be > 0 and steady.cfl_max >= steady.cfl0", op);
+        }
+        const char* names[6] = {"u", "v", "w", "p", "theta", "tke"};
+        const char* mnames[3] = {"max_speed", "mean_tke", "mean_ustar"};
+        const double mscales[3] = {st.opt.scales.velocity, st.opt.scales.tke, 0.1};
+        const int nmon = st.wall ? 3 : 2;
+        gust_steady* crit = nullptr;
+        check_rc(gust_steady_create(&opt->criterion, 6, names, nmon, mnames, mscales, &crit));
+        struct Holder {
+            gust_steady* c;
+            ~Holder() { gust_steady_destroy(c); }
+        } hold_c{crit};
+        Field dt;
+        dt.alloc(loc_box(st.g, GUST_LOC_CELL), 1);
+        gust_array4 nu = st.phys.use_closure ? st.cf[0].v : *st.phys.nu;   // zero before the first residual
+        const int hold = opt->criterion.residual_ref_iterations;
+        double cfl = opt->cfl0, r = 0;
+        gust_steady_result v{};
+        int it = 0;
+        check_rc(gust_simplec_init(s, q, stream));
+        while (it < opt->max_iterations) {
+            check_rc(gust_local_dtau(&st.g, q->f[0], q->f[1], q->f[2], &st.z, &st.J, &nu, cfl, &dt.v, stream));
+            iterate(st, q, &dt.v, nullptr, stream);
+            ++it;
+            check_rc(gust_steady_add(crit, st.res, 6, st.mon, nmon));
+            check_rc(gust_steady_relative_residual(crit, st.res, 6, hold, r, &r));
+            check_rc(gust_steady_verdict(crit, &v, nullptr, 0));
+            if (v.status != GUST_STEADY_NOT_STEADY) { break; }
+            check_rc(gust_steady_next_cfl(crit, hold, cfl, r, opt->cfl0, opt->cfl_max, opt->ser_exponent,
+                                          opt->ser_max_growth, &cfl));
+        }
+        result->iterations = it;
+        result->status = v.status;
+        result->cfl = cfl;
+        result->relative_residual = r;
+        for (int n = 0; n < 6; ++n) { result->res[n] = st.res[n]; }
+        for (int n = 0; n < 3; ++n) { result->mon[n] = st.mon[n]; }
+        result->nmon = nmon;
+    });
+}
+
+}   // extern "C"

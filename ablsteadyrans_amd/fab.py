@@ -50,7 +50,7 @@ class Geometry:
         for d in range(3):
             g.lo[d] = int(self.lo[d])
             g.hi[d] = int(self.hi[d])
-            g.periodic[d] = int(self.periodic[d])
+            g.periodic[d] = int(self.periodic[d]) if d < len(self.periodic) else 0
             g.dx[d] = float(self.dx[d])
         return g
 

@@ -585,6 +585,225 @@ int gust_confidence(const gust_geometry* geom, const gust_residuals* R, const gu
                     const gust_jvp_scales* s, double height, const double threshold[6], double* share, double* top,
                     void* stream);
 
+/* ---- The residual and its physics (periodic sides) ---------------------- */
+
+/* Face types, Source/BoundaryConditions.H:76 (FaceType); indexed like
+ * amrex::Orientation: xlo, ylo, zlo, xhi, yhi, zhi. */
+enum { GUST_FACE_PERIODIC = 0, GUST_FACE_INFLOW, GUST_FACE_OUTFLOW, GUST_FACE_SLIP_WALL, GUST_FACE_NO_SLIP_WALL,
+       GUST_FACE_SURFACE_LAYER, GUST_FACE_GEOSTROPHIC };
+
+/* The k-L closure's options, Source/Closure.H:78-105 (ClosureOptions:
+ * Cmu0 0.5562, Cb 0.35, Rt_crit -1, Rt_min -3, max_geom_lscale 30 m,
+ * tke_floor 0 = machine epsilon, sigma_k 1, nu_molecular 0, alpha_molecular
+ * 0, kappa 0.41). nut_relax 1, no stratification gate, no realizability cap. */
+typedef struct gust_closure_options {
+    double Cmu0, Cb, Rt_crit, Rt_min, max_geom_lscale, tke_floor, sigma_k, nu_molecular, alpha_molecular, kappa;
+} gust_closure_options;
+
+/* The wall model, Source/SurfaceLayer.H:100-146 (SurfaceLayerOptions):
+ * neutral stability, the reference point at the first cell centre.
+ * wall_k: 0 dirichlet, 1 source; denominator_floor: 0 log_law, 1 one. */
+typedef struct gust_surface_options {
+    double z0, Cmu0, kappa;
+    int32_t wall_k, denominator_floor;
+} gust_surface_options;
+
+/* Source/Forcing.H:40-96 (BuoyancyOptions, CoriolisOptions, the kinematic
+ * pressure-gradient force). */
+typedef struct gust_forcing {
+    double gravity, theta0;
+    int32_t coriolis, coriolis_3d, geostrophic;
+    double latitude, rotation_period, ug, vg;
+    double pressure_gradient[3];
+} gust_forcing;
+
+/* The residual's inputs, Source/Residual.H:90-128 (ResidualInputs) on this
+ * scope: the closure (use_closure) or given cell diffusivities with one
+ * ghost (nu, gamma_theta, gamma_tke); the wall model on a surface_layer
+ * ground; the forcing; the boundary faces (use_bc; the x and y faces must be
+ * periodic); theta_ref(z) as up to 64 (height, theta) points; the convection
+ * limiters (GUST_LIMITER_*) and van Albada eps of momentum and theta; the
+ * diffusivities' face mean (0 arithmetic, 1 logarithmic). */
+typedef struct gust_physics {
+    int32_t use_closure;
+    gust_closure_options closure;
+    const gust_array4* nu;
+    const gust_array4* gamma_theta;
+    const gust_array4* gamma_tke;
+    gust_surface_options surface;
+    gust_forcing forcing;
+    int32_t use_bc;
+    int32_t face[6];
+    double zhi_theta_grad;
+    double wall_velocity[6][3];
+    int32_t n_theta_ref;
+    double theta_ref_z[64];
+    double theta_ref[64];
+    int32_t momentum_limiter, theta_limiter;
+    double momentum_eps, theta_eps;
+    int32_t face_mean;
+} gust_physics;
+
+/* The six state fields u, v, w (faces), p, theta, k (cells), each with two
+ * ghost layers (Source/State.H:13). */
+typedef struct gust_state {
+    gust_array4* f[6];
+} gust_state;
+
+/* The mesh's nodal heights and metrics, each with one ghost layer. */
+typedef struct gust_mesh {
+    const gust_array4* z_nd;
+    const gust_array4* detJ;
+    const gust_array4* ax;
+    const gust_array4* ay;
+    const gust_array4* az;
+} gust_mesh;
+
+/* What the closure gives: nu, gamma_theta, gamma_tke with one ghost cell;
+ * Ps, Pb, eps, dSdk on the valid cells (Source/Closure.H:112-127). */
+typedef struct gust_closure_fields {
+    gust_array4* nu;
+    gust_array4* gamma_theta;
+    gust_array4* gamma_tke;
+    gust_array4* Ps;
+    gust_array4* Pb;
+    gust_array4* eps;
+    gust_array4* dSdk;
+} gust_closure_fields;
+
+/* Components of the surface-state slab (Source/SurfaceLayer.H:163-174):
+ * one cell thick at the ground, one lateral ghost column. */
+enum { GUST_SS_USTAR = 0, GUST_SS_SPEED, GUST_SS_D0, GUST_SS_TAU0, GUST_SS_TAU1, GUST_SS_TAU2, GUST_SS_DRAG,
+       GUST_SS_KWALL, GUST_SS_AREA, GUST_SS_INVL, GUST_SS_N };
+
+/* Cells and one ghost cell: the node-mean centre height minus the
+ * node-mean ground of the column; the vertical ghosts copy the first and
+ * last cells.
+ * Replaces: void height_above_ground(const MultiFab& z_nd, MultiFab& dist)
+ *   -- Source/Closure.H:141, Source/Closure.cpp:261-293. */
+int gust_height_above_ground(const gust_geometry* geom, const gust_array4* z_nd, gust_array4* dist, void* stream);
+
+/* The closure's fields at the state (ghosts filled) with the wall distance
+ * dist (one ghost) and the ground roughness z0.
+ * Replaces: void Closure::evaluate(const State&, const MultiFab& dist,
+ *   const Roughness&, const BuoyancyOptions&, const Mesh&, const Geometry&,
+ *   ClosureFields&) const -- Source/Closure.H:148-149,
+ *   Source/Closure.cpp:366-466. */
+int gust_closure_evaluate(const gust_geometry* geom, const gust_state* q, const gust_array4* z_nd,
+                          const gust_array4* dist, double z0, const gust_closure_options* opt, double gravity,
+                          double theta0, gust_closure_fields* out, void* stream);
+
+/* The surface state of every ground column and one lateral ghost column
+ * into ss (GUST_SS_N components); fails if a u* is not positive and finite
+ * (synchronises).
+ * Replaces: void compute_surface_state(const State&, const
+ *   SurfaceLayerOptions&, const BuoyancyOptions&, const Mesh&, const
+ *   Geometry&, SurfaceState&) -- Source/SurfaceLayer.H:178-179,
+ *   Source/SurfaceLayer.cpp:355-428. */
+int gust_surface_state(const gust_geometry* geom, const gust_array4* u, const gust_array4* v,
+                       const gust_array4* z_nd, const gust_surface_options* opt, gust_array4* ss, void* stream);
+
+/* Every ghost of the state: the periodic images along x and y, the z faces'
+ * rules (the wall law's gradient on a surface_layer ground, w = Omega-free
+ * on the ground and the top). ss (may be null) receives the surface state.
+ * Replaces: void BoundaryConditions::fill_ghosts(State&, const
+ *   SurfaceLayerOptions*, const BuoyancyOptions&, bool homogeneous,
+ *   SurfaceState*) const -- Source/BoundaryConditions.H:163-164,
+ *   Source/BoundaryConditions.cpp:597-666 (not homogeneous). */
+int gust_fill_ghosts(const gust_geometry* geom, gust_state* q, const gust_mesh* mesh, const gust_physics* phys,
+                     gust_array4* ss, void* stream);
+
+/* R(q) with the state's ghosts filled: R_c = C - D + G p - S (momentum),
+ * R_p = D(U), R_theta = C - D, R_k = bounded upwind C - D - S_k, the wall
+ * model's ground fluxes and wall-k rows, the Dirichlet rows of w, and the
+ * periodic seam sync of R_u, R_v. cf and ss (may be null) receive the
+ * closure's fields and the surface state.
+ * Replaces: void residual(const State&, const ResidualInputs&, const Mesh&,
+ *   const Geometry&, ResidualFields&, ClosureFields*) --
+ *   Source/Residual.H:133-134, Source/Residual.cpp:159-256. */
+int gust_residual(const gust_geometry* geom, const gust_state* q, const gust_mesh* mesh, const gust_physics* phys,
+                  gust_residuals* R, gust_closure_fields* cf, gust_array4* ss, void* stream);
+
+/* The implicit rows of equation eq (GUST_EQ_U..GUST_EQ_TKE) at its
+ * location: A1 from the transport fluxes F (one ghost) and the equation's
+ * diffusivity, the Patankar diagonal dSdk for k, the wall model's rows and
+ * the z faces' folds (7 components, valid points). cf: the closure's
+ * fields (null: the given diffusivities); ss: the surface state (needed
+ * with a surface_layer ground).
+ * Replaces: void implicit_rows(const Array<const MultiFab*,3>& F, const
+ *   MultiFab& z_nd, const ResidualInputs&, Equation, const Mesh&, const
+ *   Geometry&, MultiFab& A, const SurfaceState*, const ClosureFields*) --
+ *   Source/Residual.H:157-159, Source/Residual.cpp:272-304. */
+enum { GUST_EQ_U = 10, GUST_EQ_V = 11, GUST_EQ_W = 12 };
+int gust_implicit_rows(int32_t eq, const gust_geometry* geom, const gust_faces* F, const gust_mesh* mesh,
+                       const gust_physics* phys, const gust_closure_fields* cf, const gust_array4* ss,
+                       gust_array4* A, void* stream);
+
+/* ---- The SIMPLEC stage and the steady driver ---------------------------- */
+
+/* Source/Simplec.H:71-83 (SimplecOptions: pressure_reltol 1e-3, alpha_p 1,
+ * rotation_relax 0.3, w_vnn_cap 0, w_vnn_cap_slope_margin 4) and the stage's
+ * other settings: steady.zline_sweeps and the sweep options, turb.tke_floor
+ * (the clip of k), the residual scales (JvpScales), the residual scale
+ * height (residual_scale_height), the Poisson controls (their reltol is
+ * replaced by pressure_reltol; precision 1 runs the V-cycles in FP32). */
+typedef struct gust_simplec_options {
+    double pressure_reltol, alpha_p, rotation_relax, w_vnn_cap, w_vnn_cap_slope_margin;
+    int32_t sweeps;
+    gust_line_options sweep;
+    double tke_floor;
+    gust_jvp_scales scales;
+    double height;
+    gust_poisson_options poisson;
+} gust_simplec_options;
+
+/* Source/Simplec.H:178-184 (SimplecIterationInfo). */
+typedef struct gust_simplec_info {
+    gust_momentum_predictor_result predictor;
+    gust_pressure_correction_result pressure;
+    gust_scalar_sweep_result theta, tke;
+} gust_simplec_info;
+
+typedef struct gust_simplec gust_simplec;
+
+/* Replaces: SimplecOptions{} -- Source/Simplec.H:71-83. */
+int gust_simplec_default_options(gust_simplec_options* opt);
+
+/* The stage on a tile: its fields (R, the closure's fields, the surface
+ * state, the transport fluxes, the implicit rows, ap and d, the Dirichlet
+ * masks) and its Poisson solver, allocated here and kept. mesh's views and
+ * phys's pointees are referenced and must outlive the stage.
+ * Replaces: SimplecStage(const SimplecOptions&, int sweeps, const
+ *   ResidualInputs&, Closure*, const PoissonOptions&, const Mesh&, const
+ *   Geometry&, const MultiFab& z_nd, Real tke_floor, const BoxArray&, const
+ *   DistributionMapping&, const LineSweepOptions&) -- Source/Simplec.H:197-200,
+ *   Source/Simplec.cpp:293-336. */
+int gust_simplec_create(const gust_geometry* geom, const gust_mesh* mesh, const gust_physics* phys,
+                        const gust_simplec_options* opt, gust_simplec** out, void* stream);
+int gust_simplec_destroy(gust_simplec* s);
+
+/* The boundary conditions' ghosts at q.
+ * Replaces: void SimplecStage::init(State&) -- Source/Simplec.cpp:338-359. */
+int gust_simplec_init(gust_simplec* s, gust_state* q, void* stream);
+
+/* One SIMPLEC iteration with the cell pseudo-time step dtau: ghosts and
+ * surface state, R(q) and the norms, the momentum predictor, the ghosts,
+ * the pressure correction, theta's convection at the corrected transport,
+ * the theta and k sweeps. info may be null. Synchronises.
+ * Replaces: void SimplecStage::iterate(State&, const MultiFab& dtau) --
+ *   Source/Simplec.H:204, Source/Simplec.cpp:452-506. */
+int gust_simplec_iterate(gust_simplec* s, gust_state* q, const gust_array4* dtau, gust_simplec_info* info,
+                         void* stream);
+
+/* The scaled residual norms and monitors of the last iteration's R(q).
+ * Replaces: residual_norms() and monitors() -- Source/Simplec.H:205-206. */
+int gust_simplec_norms(const gust_simplec* s, double res[6], double mon[3], int32_t* nmon);
+
+/* A view of the momentum diffusivity the driver's dtau reads (the closure's
+ * nu at the last residual, zero before the first; or the given nu).
+ * Replaces: const MultiFab& SimplecStage::nu() const -- Source/Simplec.H:214. */
+int gust_simplec_nu(const gust_simplec* s, gust_array4* nu);
+
 /* ---- Steady criterion (host only) --------------------------------------- */
 
 /* Source/SteadyCriterion.H:13-33 (SteadyParams). */
@@ -672,6 +891,34 @@ int gust_steady_relative_residual(const gust_steady* c, const double* residuals,
  * Replaces: Real Driver::next_cfl(Real r) const -- Source/Driver.cpp:325-334. */
 int gust_steady_next_cfl(const gust_steady* c, int32_t hold, double cfl, double r, double cfl0, double cfl_max,
                          double ser_exponent, double max_growth, double* out);
+
+/* ---- The steady driver ---------------------------------------------------- */
+
+/* Source/Driver.H:58-66 (SteadyOptions: max_iterations 10000, cfl0 1, cfl_max 10,
+ * ser_exponent 1, ser_max_growth 2) and the criterion. */
+typedef struct gust_driver_options {
+    int32_t max_iterations;
+    double cfl0, cfl_max, ser_exponent, ser_max_growth;
+    gust_steady_params criterion;
+} gust_driver_options;
+
+typedef struct gust_driver_result {
+    int32_t iterations;
+    int32_t status;             /* GUST_STEADY_* */
+    double cfl;                 /* the CFL of the next iteration */
+    double relative_residual;   /* r_n of the last iteration */
+    double res[6];
+    double mon[3];
+    int32_t nmon;
+} gust_driver_result;
+
+/* The steady loop on one stage from q: local dtau at the current CFL, the
+ * iteration, the criterion, r_n and the SER CFL, until Steady, Diverged or
+ * max_iterations. Synchronises every iteration.
+ * Replaces: DriverResult Driver::run(State&) with one SIMPLEC stage --
+ *   Source/Driver.cpp:403-492 (step, verdict, run). */
+int gust_steady_solve(gust_simplec* s, gust_state* q, const gust_driver_options* opt, gust_driver_result* result,
+                      void* stream);
 
 #ifdef __cplusplus
 }

@@ -99,4 +99,8 @@ def random_state(tile, seed, U=8.0, amp=0.5):
     _, _, zc = positions(tile, 0)
     q["theta"][0] += 300.0 + 0.003 * zc
     q["tke"] = np.abs(q["tke"]) + 0.05
+    # a periodic seam face is one unknown: the hi copy is the lo one (OverrideSync)
+    nx, ny = g.hi[0] - g.lo[0] + 1, g.hi[1] - g.lo[1] + 1
+    q["u"][0, :, :, 2 + nx] = q["u"][0, :, :, 2]
+    q["v"][0, :, 2 + ny, :] = q["v"][0, :, 2, :]
     return q

@@ -1,0 +1,151 @@
+"""The residual's physics on the GPU against the oracle: heights bit for
+bit; the closure, the wall model, the ghost fills, the residual of all six
+equations and the implicit rows of all five transported fields on a hill
+tile with a random state, within 1e-12 of each field's scale (the device's
+log and pow differ from the host's in the last bit; everything else is the
+same operations in the same order)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from oracle import physics as P
+from tests import physics_helpers as PHY
+
+B1, B2 = (-1, -1, -1), (-2, -2, -2)
+
+
+def _fab(a, begin):
+    import torch
+    from ablsteadyrans_amd.fab import Fab
+    return Fab(torch.from_numpy(np.ascontiguousarray(a)).cuda(), begin)
+
+
+def _inputs(phys_o: P.Physics, with_closure=True):
+    from ablsteadyrans_amd import physics as GP
+    f = phys_o.forcing
+    gi = GP.ResidualInputs(
+        closure=GP.ClosureOptions() if with_closure else None,
+        surface=GP.SurfaceLayerOptions(z0=phys_o.surface.z0, wall_k_dirichlet=phys_o.surface.wall_k_dirichlet),
+        forcing=GP.ForcingOptions(gravity=f.gravity, theta0=f.theta0, coriolis=f.coriolis, coriolis_3d=f.three_d,
+                                  geostrophic=f.geostrophic, latitude=f.latitude, rotation_period=f.rotation_period,
+                                  ug=f.ug, vg=f.vg, pressure_gradient=f.pressure_gradient),
+        faces=(0, 0, phys_o.bc.zlo, 0, 0, phys_o.bc.zhi) if phys_o.bc is not None else None,
+        zhi_theta_grad=phys_o.bc.zhi_theta_grad if phys_o.bc is not None else 0.0,
+        theta_ref_z=phys_o.theta_ref_z, theta_ref=phys_o.theta_ref)
+    return gi
+
+
+def _setup(seed=3, wall=True, zhi=P.SLIP_WALL):
+    tile = PHY.hill_mesh(n=(16, 12, 10), height=90.0, half=200.0)
+    m = PHY.omesh(tile)
+    q = PHY.random_state(tile, seed, U=6.0, amp=0.3)
+    phys = P.Physics(bc=P.Boundary(zlo=P.SURFACE_LAYER if wall else P.SLIP_WALL, zhi=zhi, zhi_theta_grad=0.003),
+                     surface=P.SurfaceOptions(z0=0.1),
+                     forcing=P.Forcing(coriolis=True, latitude=45.0, geostrophic=True, ug=6.0, vg=1.0,
+                                       pressure_gradient=(1e-4, 0.0, 0.0)),
+                     theta_ref_z=(0.0, 300.0, 1000.0), theta_ref=(300.0, 300.0, 302.0))
+    return tile, m, q, phys
+
+
+def _dstate(tile, q):
+    from ablsteadyrans_amd import physics as GP
+    s = GP.DeviceState(tile.geom)
+    for n in GP.FIELDS:
+        s[n].data.copy_(__import__("torch").from_numpy(q[n]))
+    return s
+
+
+def _close(a, b, scale_rel=1e-12, what=""):
+    scale = max(np.abs(b).max(), 1e-300)
+    err = np.abs(a - b).max()
+    assert err <= scale_rel * scale, f"{what}: max err {err:.3e} vs scale {scale:.3e}"
+
+
+@pytest.mark.gpu
+def test_heights_are_bitwise(cuda):
+    from ablsteadyrans_amd import physics as GP
+    from ablsteadyrans_amd.fab import Fab
+    from ablsteadyrans_amd.poisson import DeviceMesh
+    tile, m, _, _ = _setup()
+    ref = P.height_above_ground(m)
+    d = Fab.at_loc(tile.geom, 0, 1, 1)
+    GP.height_above_ground(DeviceMesh(tile), d)
+    assert np.array_equal(d.numpy(), ref)
+
+
+@pytest.mark.gpu
+def test_closure_and_surface_state_match(cuda):
+    from ablsteadyrans_amd import physics as GP
+    from ablsteadyrans_amd.fab import Fab
+    from ablsteadyrans_amd.poisson import DeviceMesh
+    tile, m, q, phys = _setup()
+    dm = DeviceMesh(tile)
+    dist = P.height_above_ground(m)
+    ref = P.closure_evaluate(q, m, P.ClosureOptions(), dist, 0.1, phys.forcing)
+    out = GP.ClosureFields(tile.geom)
+    GP.closure_evaluate(dm, _dstate(tile, q), _fab(dist, B1), 0.1, GP.ClosureOptions(), out)
+    for n in ("nu", "gamma_theta", "gamma_tke", "Ps", "Pb", "eps", "dSdk"):
+        _close(out[n].numpy(), ref[n], 1e-13, n)
+    ss_ref = np.zeros((P.SS_N, 1, 14, 18))
+    orc.lib().oracle_surface_state(orc._v(q["u"], B2), orc._v(q["v"], B2), orc._v(m.z, B1), phys.surface.c(), m.geom,
+                                   orc._v(ss_ref, (-1, -1, 0)))
+    ss = GP.surface_slab(tile.geom)
+    GP.surface_state(dm, _dstate(tile, q), GP.SurfaceLayerOptions(z0=0.1), ss)
+    for c in range(P.SS_N):
+        _close(ss.numpy()[c], ss_ref[c], 1e-14, f"ss[{c}]")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("zhi", [P.SLIP_WALL, P.GEOSTROPHIC, P.NO_SLIP_WALL])
+def test_fill_ghosts_match(cuda, zhi):
+    from ablsteadyrans_amd import physics as GP
+    from ablsteadyrans_amd.poisson import DeviceMesh
+    tile, m, q, phys = _setup(zhi=zhi)
+    ds = _dstate(tile, q)
+    P.fill_ghosts(q, m, phys)
+    GP.fill_ghosts(DeviceMesh(tile), ds, _inputs(phys))
+    for n in ("u", "v", "w", "p", "theta", "tke"):
+        _close(ds[n].numpy(), q[n], 1e-13, n)
+    for n in ("w", "p", "theta", "tke"):
+        assert np.array_equal(ds[n].numpy(), q[n]), n
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("with_closure,wall", [(True, True), (True, False), (False, True)])
+def test_residual_and_implicit_rows_match(cuda, with_closure, wall):
+    import torch
+    from ablsteadyrans_amd import physics as GP
+    from ablsteadyrans_amd.fab import Fab
+    from ablsteadyrans_amd.poisson import DeviceMesh
+    tile, m, q, phys = _setup(seed=7, wall=wall)
+    if not with_closure:
+        rng = np.random.default_rng(1)
+        phys.closure = None
+        for name in ("nu", "gamma_theta", "gamma_tke"):
+            setattr(phys, name, rng.uniform(0.5, 5.0, size=(1, 12, 14, 18)))
+    P.fill_ghosts(q, m, phys)
+    R_ref, cf_ref, ss_ref = P.residual(q, m, phys)
+    gi = _inputs(phys, with_closure)
+    if not with_closure:
+        gi.nu, gi.gamma_theta, gi.gamma_tke = (_fab(getattr(phys, n), B1) for n in ("nu", "gamma_theta", "gamma_tke"))
+    dm = DeviceMesh(tile)
+    ds = _dstate(tile, q)
+    R = {n: Fab.at_loc(tile.geom, loc, 1, 0) for n, loc in zip(GP.FIELDS, GP.LOCS)}
+    cf = GP.ClosureFields(tile.geom) if with_closure else None
+    ss = GP.surface_slab(tile.geom)
+    GP.residual(dm, ds, gi, R, cf, ss)
+    torch.cuda.synchronize()
+    for n in GP.FIELDS:
+        _close(R[n].numpy(), R_ref[n], 1e-11, n)
+    # the implicit rows of the five transported fields
+    Fx, Fy, Fz, Fb = orc.transport_fluxes(q["u"], q["v"], q["w"], B2, m.z, B1, m.ax, m.ay, B1, m.geom)
+    F = [Fx[None], Fy[None], Fz[None]]
+    Fd = [_fab(f, Fb) for f in F]
+    for eq_o, eq_g, loc in ((P.EQ_U, GP.EQ_U, 1), (P.EQ_V, GP.EQ_V, 2), (P.EQ_W, GP.EQ_W, 4),
+                            (P.EQ_THETA, GP.EQ_THETA, 0), (P.EQ_TKE, GP.EQ_TKE, 0)):
+        A_ref = P.implicit_rows(eq_o, F, Fb, m, phys, cf_ref, ss_ref)
+        A = Fab.at_loc(tile.geom, loc, 7, 0)
+        GP.implicit_rows(eq_g, dm, Fd, gi, A, cf, ss if wall else None)
+        _close(A.numpy(), A_ref, 1e-12, f"A[{eq_o}]")
