@@ -84,31 +84,64 @@ def test_iterations_match_the_oracle_composite(cuda):
             assert np.abs(a - b).max() <= 1e-8 * scale, (it, n, np.abs(a - b).max(), scale)
 
 
+def _reference_column():
+    """Tests/Canonical_RANS/Neutral_Column/inputs_steady_column: 4 x 4 x 64
+    over 2560 x 2560 x 1000 m, periodic sides, surface layer z0 = 0.1 m, slip
+    top, uniform theta, the force F = 1.225e-4 m/s^2 (u* = 0.35 m/s), the
+    closure uncapped (max_geom_lscale 1e12) with the logarithmic face mean;
+    the first guess the log law with u* = 0.25 m/s and its equilibrium k."""
+    import torch
+    from ablsteadyrans_amd import mesh as M
+    from ablsteadyrans_amd import physics as GP
+    geom = M.geometry((4, 4, 64), (0.0, 0.0, 0.0), (2560.0, 2560.0, 1000.0), (1, 1))
+    zl = M.make_zlevels(64, 1000.0)
+    tile = M.build_mesh(geom, (0.0, 0.0, 0.0), (2560.0, 2560.0, 1000.0), zl, M.Terrain(kind="flat"))
+    inputs = GP.ResidualInputs(closure=GP.ClosureOptions(max_geom_lscale=1.0e12),
+                               surface=GP.SurfaceLayerOptions(z0=0.1),
+                               forcing=GP.ForcingOptions(pressure_gradient=(1.225e-4, 0.0, 0.0)),
+                               log_face_mean=True)
+    q = GP.DeviceState(tile.geom)
+    _, _, zu = PHY.positions(tile, 1)
+    u = 0.25 / 0.41 * np.log((np.maximum(zu, 0.0) + 0.1) / 0.1)
+    q["u"].data.copy_(torch.from_numpy(u[None].copy()))
+    q["theta"].data.fill_(300.0)
+    q["tke"].data.fill_(0.25 * 0.25 / (0.5562 * 0.5562))
+    return tile, inputs, q
+
+
 @pytest.mark.gpu
-def test_neutral_column_reaches_steady(cuda):
-    """A flat neutral column under geostrophic forcing and rotation with the
-    k-L closure and the wall model: the driver reaches the criterion (two
-    orders below the early peak, monitors settled), the wall speed-up is the
-    log law's (u* from the first centre), and a second solve gives the same
-    bits."""
+def test_neutral_column_matches_the_reference_deck(cuda):
+    """The reference's steady column (Tests/Canonical_RANS/Neutral_Column,
+    README: steady at 6 orders and residual_tol 1e-8, mean u* = sqrt(F H) to
+    1e-4, the first cell's speed the wall law's to 1e-6, the wall k =
+    u*^2/Cmu0^2 to 1e-6, every column identical to 1e-10; the reference
+    measured 2231 iterations), run by the steady driver on the GPU; a second
+    run gives the same bits."""
     from ablsteadyrans_amd import physics as GP
     from ablsteadyrans_amd.poisson import DeviceMesh
-    tile, m, q, phys = _neutral_column()
-    dm = DeviceMesh(tile)
-    gi = _inputs(phys)
-    opt = GP.SimplecOptions(sweeps=4, tke_floor=1e-6, height=600.0)
-    dopt = GP.DriverOptions(max_iterations=3000, cfl0=1.0, cfl_max=2.0, residual_drop_orders=2.0,
-                            residual_ref_iterations=20, monitor_window=50)
-    results = []
+    dopt = GP.DriverOptions(max_iterations=20000, cfl0=1.0, cfl_max=50.0, residual_tol=1.0e-8,
+                            residual_drop_orders=6.0)
+    runs = []
     for _ in range(2):
-        stage = GP.SimplecStage(dm, gi, opt)
-        ds = _state(tile, q)
-        r = stage.solve(ds, dopt)
-        results.append((r, {n: ds[n].numpy().copy() for n in GP.FIELDS}))
-    r, f = results[0]
+        tile, inputs, q = _reference_column()
+        dm = DeviceMesh(tile)
+        stage = GP.SimplecStage(dm, inputs, GP.SimplecOptions(sweeps=2, height=1000.0))
+        r = stage.solve(q, dopt)
+        runs.append((r, {n: q[n].numpy().copy() for n in GP.FIELDS}))
+        print(f"steady column: status {r.status}, {r.iterations} iterations (reference 2231), "
+              f"residuals {list(r.res)}, monitors {list(r.mon)[:r.nmon]}", flush=True)
+    r, f = runs[0]
     assert r.status == 1, (r.status, r.iterations, list(r.res))
-    assert r.iterations < 3000
-    assert all(x < 5e-5 for x in r.res)
+    ustar = r.mon[2]
+    assert abs(ustar - np.sqrt(1.225e-4 * 1000.0)) <= 1e-4 * 0.35, ustar
+    u = f["u"][0, 2:-2, 2:-2, 2:-2]
+    k = f["tke"][0, 2:-2, 2:-2, 2:-2]
+    for a in (u, k):
+        assert np.abs(a - a[:, :1, :1]).max() <= 1e-10 * np.abs(a).max()
+    d0 = 1000.0 / 64 / 2
+    uc = 0.5 * (u[0, 0, 0] + u[0, 0, 1])
+    assert abs(uc - ustar / 0.41 * np.log((d0 + 0.1) / 0.1)) <= 1e-6 * uc
+    assert abs(k[0, 0, 0] - ustar * ustar / 0.5562 ** 2) <= 1e-6 * k[0, 0, 0]
     for n in GP.FIELDS:
-        assert np.array_equal(f[n], results[1][1][n]), n
-    assert results[1][0].iterations == r.iterations
+        assert np.array_equal(f[n], runs[1][1][n]), n
+    assert runs[1][0].iterations == r.iterations
