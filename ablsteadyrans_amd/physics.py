@@ -317,7 +317,7 @@ class SimplecStage:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
+        if h is not None and h.value and _lib is not None and _lib.lib is not None:
             _lib.lib.gust_simplec_destroy(h)
             self._h = None
 

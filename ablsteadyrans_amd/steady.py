@@ -88,7 +88,7 @@ class SteadyCriterion:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
+        if h is not None and h.value and _lib is not None and _lib.lib is not None:
             _lib.lib.gust_steady_destroy(h)
             self._h = None
 
