@@ -227,6 +227,32 @@ class ClosureFieldsC(C.Structure):
     _fields_ = [(n, C.POINTER(Array4)) for n in ("nu", "gamma_theta", "gamma_tke", "Ps", "Pb", "eps", "dSdk")]
 
 
+class InflowProfileC(C.Structure):
+    """``gust_inflow_profile`` (Source/InflowProfile.H:42-66, the table in memory)."""
+
+    _fields_ = [("kind", C.c_int32), ("ustar", C.c_double), ("z0", C.c_double), ("speed_ref", C.c_double),
+                ("z_ref", C.c_double), ("alpha", C.c_double), ("direction", C.c_double), ("Cmu0", C.c_double),
+                ("kappa", C.c_double), ("unbounded", C.c_int32), ("l_max", C.c_double), ("n", C.c_int32),
+                ("z", C.POINTER(C.c_double)), ("u", C.POINTER(C.c_double)), ("v", C.POINTER(C.c_double)),
+                ("theta", C.POINTER(C.c_double)), ("k", C.POINTER(C.c_double)), ("n_theta_ref", C.c_int32),
+                ("theta_ref_z", C.c_double * 64), ("theta_ref", C.c_double * 64)]
+
+
+class InitOptionsC(C.Structure):
+    """``gust_init_options`` (Source/Initializer.H:63-98)."""
+
+    _fields_ = [("first_guess", C.c_int32), ("projection", C.c_int32), ("z_agl_min", C.c_double),
+                ("alpha_h", C.c_double), ("alpha_v", C.c_double), ("k_decay_height", C.c_double),
+                ("obrien", C.c_int32), ("column_theta", C.c_int32)]
+
+
+class InitResultC(C.Structure):
+    """``gust_init_result`` (Source/Initializer.H:110-120)."""
+
+    _fields_ = [("poisson", PoissonResultC), ("divergence_before", C.c_double), ("divergence_after", C.c_double),
+                ("obrien_top_residual", C.c_double)]
+
+
 class SimplecOptionsC(C.Structure):
     """``gust_simplec_options`` (Source/Simplec.H:71-83 and the stage's settings)."""
 
@@ -390,6 +416,19 @@ _sig = {
     "gust_steady_relative_residual": (C.c_int, [C.c_void_p, _P(C.c_double), C.c_int32, C.c_int32, C.c_double,
                                                 _P(C.c_double)]),
     "gust_steady_next_cfl": (C.c_int, [C.c_void_p, C.c_int32] + [C.c_double] * 6 + [_P(C.c_double)]),
+    "gust_inflow_default_profile": (C.c_int, [_P(InflowProfileC)]),
+    "gust_check_inflow_profile": (C.c_int, [_P(InflowProfileC)]),
+    "gust_inflow_eval": (C.c_int, [_P(InflowProfileC), C.c_int32] + [C.c_void_p] * 6 + [C.c_void_p]),
+    "gust_init_default_options": (C.c_int, [_P(InitOptionsC)]),
+    "gust_check_init_options": (C.c_int, [_P(InitOptionsC)]),
+    "gust_initializer_poisson_bc": (C.c_int, [_P(GeometryC), _P(C.c_int32), _P(C.c_int32)]),
+    "gust_obrien_omega": (C.c_int, [_P(GeometryC), _P(MeshC), _P(Array4), _P(Array4), _P(Array4), _P(C.c_double),
+                                    C.c_void_p]),
+    "gust_first_guess_state": (C.c_int, [_P(GeometryC), _P(MeshC), _P(InflowProfileC), _P(InitOptionsC), _P(StateC),
+                                         _P(C.c_double), C.c_void_p]),
+    "gust_initialize_state": (C.c_int, [_P(GeometryC), _P(MeshC), _P(InflowProfileC), _P(InitOptionsC),
+                                        _P(C.c_int32), C.c_void_p, _P(StateC), _P(Array4), _P(InitResultC),
+                                        C.c_void_p]),
 }
 for _name, (_res, _args) in _sig.items():
     _f = getattr(lib, _name)

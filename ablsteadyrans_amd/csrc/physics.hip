@@ -17,13 +17,14 @@
 //                                           rules, fills, rows, folds (z faces)
 //   Source/Forcing.cpp:77-170, 285-299      theta_ref, buoyancy, Coriolis,
 //                                           pressure-gradient force
-//   Source/Heights_K.H:20-68                NodeZ, height_above_ground<T>, point_z<T>
+//   Source/Heights_K.H:20-68                (heights.hpp) NodeZ, point_z<T>
 //
 // Every point value is formed with the reference's operations in its order
 // (contraction off), so the fields equal the FP64 CPU restatement bit for
 // bit except where the device's log and pow enter (the wall law's u*, the
 // closure's k^1.5), which differ from the host's in the last bit.
 #include "cgrid.hpp"
+#include "heights.hpp"
 #include "stencil.hpp"
 
 #include <cfloat>
@@ -50,40 +51,9 @@ constexpr int kSlotDist = 70, kSlotNu = 71, kSlotGth = 72, kSlotGk = 73, kSlotPs
 
 // ---- heights -------------------------------------------------------------
 
-// Node heights with the lateral indices clamped to the stored nodes and the
-// levels extrapolated linearly beyond them (Heights_K.H:20-33).
-struct NodeZ {
-    CV z;
-    int lo0, lo1, lo2, hi0, hi1, hi2;
-    __device__ __forceinline__ double operator()(int i, int j, int k) const
-    {
-        i = i < lo0 ? lo0 : (i > hi0 ? hi0 : i);
-        j = j < lo1 ? lo1 : (j > hi1 ? hi1 : j);
-        if (k < lo2) { return z(i, j, lo2) + (lo2 - k) * (z(i, j, lo2) - z(i, j, lo2 + 1)); }
-        if (k > hi2) { return z(i, j, hi2) + (k - hi2) * (z(i, j, hi2) - z(i, j, hi2 - 1)); }
-        return z(i, j, k);
-    }
-};
-
-NodeZ node_z(const gust_array4* z)
-{
-    return NodeZ{dview<const double>(z), z->begin[0], z->begin[1], z->begin[2], z->end[0] - 1, z->end[1] - 1,
-                 z->end[2] - 1};
-}
-
-template <int T>
-__device__ __forceinline__ double point_z(const NodeZ& z, int i, int j, int k)
-{
-    constexpr int nx = sc::nodal(T, 0) ? 1 : 2, ny = sc::nodal(T, 1) ? 1 : 2, nz = sc::nodal(T, 2) ? 1 : 2;
-    double zp = 0;
-#pragma unroll
-    for (int a = 0; a < nx; ++a)
-#pragma unroll
-        for (int b = 0; b < ny; ++b)
-#pragma unroll
-            for (int c = 0; c < nz; ++c) { zp += z(i + a, j + b, k + c); }
-    return zp / (nx * ny * nz);
-}
+using heights::node_z;
+using heights::NodeZ;
+using heights::point_z;
 
 __global__ __launch_bounds__(kThreads) void height_kernel(Box3 b, CV z, V dist, int klo)
 {
@@ -441,37 +411,8 @@ __global__ __launch_bounds__(kThreads) void periodic_kernel(Box3 fab, V a, int v
 
 // ---- forcing ------------------------------------------------------------------
 
-struct Profile {
-    int n;
-    double z[64], t[64];
-};
-
-__global__ __launch_bounds__(kThreads) void theta_ref_kernel(Box3 b, CV z, Profile p, V tr)
-{
-    GUST_GRID_LOOP(b)
-    {
-        int i, j, k;
-        b.at(t, i, j, k);
-        const double zc = sc::interp<sc::NODE, sc::CC>(z, i, j, k);
-        double v = p.t[0];
-        if (zc >= p.z[p.n - 1]) {
-            v = p.t[p.n - 1];
-        } else {
-            for (int m = 1; m < p.n; ++m) {
-                if (zc < p.z[m]) {
-                    if (zc > p.z[m - 1]) {
-                        const double wgt = (zc - p.z[m - 1]) / (p.z[m] - p.z[m - 1]);
-                        v = p.t[m - 1] + wgt * (p.t[m] - p.t[m - 1]);
-                    } else {
-                        v = p.t[m - 1];
-                    }
-                    break;
-                }
-            }
-        }
-        tr(i, j, k) = v;
-    }
-}
+using heights::Sounding;
+using heights::theta_ref_kernel;
 
 struct Rot {
     int on, three_d;
@@ -1207,12 +1148,7 @@ int gust_residual(const gust_geometry* geom, const gust_state* q, const gust_mes
         // forcing
         {
             gust_array4 tr = scratch_view(st, kSlotTref, grow(cells, 1, 1, 1));
-            Profile prof;
-            prof.n = p.n_theta_ref;
-            for (int m = 0; m < 64; ++m) {
-                prof.z[m] = m < p.n_theta_ref ? p.theta_ref_z[m] : 0.0;
-                prof.t[m] = m < p.n_theta_ref ? p.theta_ref[m] : 0.0;
-            }
+            const Sounding prof = heights::sounding(p.n_theta_ref, p.theta_ref_z, p.theta_ref);
             const Box3 tb = box3(grow(cells, 1, 1, 1));
             theta_ref_kernel<<<blocks_for(tb.npts()), kThreads, 0, st>>>(tb, dview<const double>(mesh->z_nd), prof,
                                                                          dview<double>(&tr));

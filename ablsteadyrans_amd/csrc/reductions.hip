@@ -10,6 +10,7 @@
 // fixed two-level tree instead (block partials, then one block per field):
 // the same tile gives the same bits on every call and any stream.
 #include "cgrid.hpp"
+#include "reduce.hpp"
 
 #include <cmath>
 #include <cstring>
@@ -320,6 +321,18 @@ void check_residuals(const char* op, const gust_geometry& g, const gust_residual
 }
 
 }   // namespace
+
+double sum_squares(hipStream_t st, const gust_array4* a, const IBox& b)
+{
+    if (b.npts() * 2 >= int64_t(std::numeric_limits<unsigned>::max())) {
+        GUST_FAIL("gust: sum_squares: tile too large");
+    }
+    RSet set{};
+    set.f[0] = rfield(a, b, kSumSq);
+    double out = 0;
+    multi_reduce(st, set, 1, &out);
+    return out;
+}
 
 }   // namespace gust_amd
 

@@ -920,6 +920,125 @@ typedef struct gust_driver_result {
 int gust_steady_solve(gust_simplec* s, gust_state* q, const gust_driver_options* opt, gust_driver_result* result,
                       void* stream);
 
+/* ---- The mass-consistent initializer --------------------------------------
+ * Source/Initializer.H, Source/InflowProfile.H: the first guess from an
+ * inflow profile by height above the ground, then one anisotropic
+ * projection by the terrain Poisson solver. */
+
+enum { GUST_INFLOW_LOG_LAW = 0, GUST_INFLOW_POWER_LAW = 1, GUST_INFLOW_TABLE = 2 };
+
+/* Source/InflowProfile.H:42-66 (InflowProfileOptions; defaults from
+ * gust_inflow_default_profile) with the file kind's table given in memory
+ * (the caller reads "z u v theta [k]"; or a converged column's table,
+ * InflowProfile.H:81-89): n rows, z strictly increasing from 0, theta > 0,
+ * k >= 0 or k = NULL (k from the table's shear); host arrays, copied by each
+ * call. unbounded: bc.inflow_tke_length = unbounded (else the closure's cap
+ * l_max = turb.max_geom_lscale). The sounding theta_ref(z) (bc.theta_profile_z,
+ * bc.theta_profile): 1 to 64 points, heights increasing. */
+typedef struct gust_inflow_profile {
+    int32_t kind;
+    double ustar, z0;
+    double speed_ref, z_ref, alpha;
+    double direction;     /* degrees from the x axis */
+    double Cmu0, kappa;
+    int32_t unbounded;
+    double l_max;
+    int32_t n;
+    const double* z;
+    const double* u;
+    const double* v;
+    const double* theta;
+    const double* k;
+    int32_t n_theta_ref;
+    double theta_ref_z[64];
+    double theta_ref[64];
+} gust_inflow_profile;
+
+/* The reference's defaults: log law, u* 0.4, z0 0.1, speed 10 at 10 m with
+ * alpha 0.14, direction 0, Cmu0 0.5562, kappa 0.41, the closure's cap at 30 m,
+ * theta_ref 300 K. */
+int gust_inflow_default_profile(gust_inflow_profile* p);
+
+/* 0 if valid, else nonzero with the first problem, naming its bc.* key.
+ * Replaces: check_inflow_profile_options -- Source/InflowProfile.cpp:14-47,
+ *   and the table's and the sounding's checks of the InflowProfile
+ *   constructors (InflowProfile.cpp:49-145, Source/Forcing.cpp:30-49). */
+int gust_check_inflow_profile(const gust_inflow_profile* p);
+
+/* The profile at n heights above the ground (device arrays; any output may
+ * be NULL): the wind, theta, the equilibrium k and nu_t.
+ * Replaces: InflowProfile::View::eval -- Source/InflowProfile.H:149-200,
+ *   with the view's constants of Source/InflowProfile.cpp:178-205. */
+int gust_inflow_eval(const gust_inflow_profile* p, int32_t n, const double* z, double* u, double* v, double* theta,
+                     double* k, double* nut, void* stream);
+
+/* Source/Initializer.H:63-98 (InitOptions), the profile first guess:
+ * first_guess 0 none | 1 profile (init.first_guess); projection (1);
+ * z_agl_min (0), alpha_h, alpha_v (1), k_decay_height (0), obrien (0);
+ * column_theta: theta from the table by height above the ground instead of
+ * the sounding by absolute height (a stratified column's table). */
+typedef struct gust_init_options {
+    int32_t first_guess;
+    int32_t projection;
+    double z_agl_min;
+    double alpha_h, alpha_v;
+    double k_decay_height;
+    int32_t obrien;
+    int32_t column_theta;
+} gust_init_options;
+
+int gust_init_default_options(gust_init_options* o);
+
+/* Replaces: check_init_options (the profile's keys) -- Source/Initializer.cpp:109-179. */
+int gust_check_init_options(const gust_init_options* o);
+
+/* Source/Initializer.H:110-120 (InitResult). */
+typedef struct gust_init_result {
+    gust_poisson_result poisson;
+    double divergence_before, divergence_after;   /* 2-norms of velocity_divergence (1/s) */
+    double obrien_top_residual;                   /* max |Omega_top| removed (m/s) */
+} gust_init_result;
+
+/* The correction's boundary types for the face types faces[6] (GUST_FACE_*,
+ * orientation order): Dirichlet on inflow and outflow faces, Neumann
+ * elsewhere, periodic where the geometry is.
+ * Replaces: PoissonBC initializer_poisson_bc(const BoundaryInputs&, const
+ *   Geometry&) -- Source/Initializer.H:108, Source/Initializer.cpp:181-195. */
+int gust_initializer_poisson_bc(const gust_geometry* geom, const int32_t* faces, int32_t* pbc);
+
+/* Omega on the valid z-faces from the column continuity of (u, v) from the
+ * ground up, the top residual E removed with (h / depth)^2; *top_residual =
+ * max |E| before the removal. Synchronises.
+ * Replaces: Real obrien_omega(const MultiFab& u, v, z_nd, const Mesh&,
+ *   const Geometry&, MultiFab& omega) -- Source/Initializer.H:156-157,
+ *   Source/Initializer.cpp:197-267. */
+int gust_obrien_omega(const gust_geometry* geom, const gust_mesh* mesh, const gust_array4* u, const gust_array4* v,
+                      gust_array4* omega, double* top_residual, void* stream);
+
+/* Steps 1-3 of the initializer on q (two ghosts): u, v from the profile at
+ * every point, w = 0 but for the ground faces' kinematic value, k the
+ * profile's with the decay above k_decay_height, theta the sounding (one
+ * ghost cell) or the table's, p' = 0, then O'Brien when asked;
+ * *obrien_residual its top residual. Synchronises.
+ * Replaces: Real first_guess_state(const InitOptions&, const BoundaryInputs&,
+ *   const Mesh&, const Geometry&, State&, ...) -- Source/Initializer.H:133-135,
+ *   Source/Initializer.cpp:269-335. */
+int gust_first_guess_state(const gust_geometry* geom, const gust_mesh* mesh, const gust_inflow_profile* prof,
+                           const gust_init_options* opt, gust_state* q, double* obrien_residual, void* stream);
+
+/* The first guess, then (projection = 1) the correction div(Sigma grad
+ * lambda) = div(u0) by solver (created on this mesh by the caller, its
+ * options the solver.poisson_* controls), (u, v, Omega) += F, the periodic
+ * images of u and v, w from Omega. lambda: cells with one ghost cell, or
+ * NULL. The state's other ghost cells are not filled. Synchronises.
+ * Replaces: InitResult initialize_state(const InitOptions&, const
+ *   BoundaryInputs&, const PoissonOptions&, const Mesh&, const Geometry&,
+ *   State&, MultiFab* lambda, ...) -- Source/Initializer.H:146-149,
+ *   Source/Initializer.cpp:344-418. */
+int gust_initialize_state(const gust_geometry* geom, const gust_mesh* mesh, const gust_inflow_profile* prof,
+                          const gust_init_options* opt, const int32_t* faces, gust_poisson* solver, gust_state* q,
+                          gust_array4* lambda, gust_init_result* result, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

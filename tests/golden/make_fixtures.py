@@ -8,6 +8,11 @@ tests/golden/ (data only: inputs and expected outputs, no source text).
                        phi of poisson_askervein_plt00000 (30 x 30 x 18,
                        i fastest) and the gold JSON numbers (div_before,
                        div_after, iterations, reltol) of both back ends.
+  init_askervein.npz   Tests/Initialization/Mass_Consistent_Askervein/Gold/:
+                       the cell fields of init_askervein_plt00000 (u, v, w
+                       as the plotfile's face means, p, theta, tke, z,
+                       detJ; (nz, ny, nx), i fastest) and the gold JSON
+                       numbers.
 
 Run from the repository root with the reference tree at /root/reference.
 """
@@ -69,6 +74,13 @@ def main():
     np.savez_compressed(os.path.join(OUT, "poisson_askervein.npz"), phi=phi[0],
                         div_before=ga["div_before"], div_after_fft=ga["div_after"], iterations_fft=ga["iterations"],
                         div_after_mlmg=gb["div_after"], iterations_mlmg=gb["iterations"], reltol=ga["reltol"])
+    base = os.path.join(REF, "Tests/Initialization/Mass_Consistent_Askervein/Gold")
+    names, lo, hi, st = read_plotfile_cells(os.path.join(base, "init_askervein_plt00000"))
+    assert names == ["u", "v", "w", "p", "theta", "tke", "z", "detJ"], names
+    gi = json.load(open(os.path.join(base, "init_askervein.json")))
+    np.savez_compressed(os.path.join(OUT, "init_askervein.npz"), **{n: st[q] for q, n in enumerate(names)},
+                        **{k: gi[k] for k in ("div_before", "div_after", "iterations", "reltol", "speed_max", "wmax",
+                                              "theta_min", "theta_max", "p_absmax", "k_min")})
     print("wrote fixtures to", OUT)
 
 
