@@ -19,6 +19,13 @@ inline int blocks_for(int64_t n)
     return int(std::max<int64_t>(1, std::min<int64_t>((n + kThreads - 1) / kThreads, 16384)));
 }
 
+// The grid of a reduction kernel: grid-stride over the box, one atomic per
+// block (2 blocks per CU).
+inline int reduce_blocks(int64_t n)
+{
+    return int(std::max<int64_t>(1, std::min<int64_t>((n + kThreads - 1) / kThreads, 512)));
+}
+
 struct Box3 {
     int lo0, lo1, lo2, n0, n1, n2;
     __host__ __device__ int64_t npts() const { return int64_t(n0) * n1 * n2; }

@@ -156,6 +156,82 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
     return v;
 }
 
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
+{
+    for (int off = 32; off >= 1; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off, 64);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
+{
+    for (int off = 32; off >= 1; off >>= 1) { v += __shfl_xor(v, off, 64); }
+    return v;
+}
+
+// Block-wide max / min / sum of v over a block of kBlock threads (a multiple
+// of 64); every thread of the block must call it (it holds two barriers).
+// The result is valid in thread 0. One atomic per block instead of one per wave keeps a reduction
+// over millions of points from serialising on a single address.
+template <int kBlock>
+__device__ __forceinline__ unsigned long long block_max_u64(unsigned long long v)
+{
+    __shared__ unsigned long long s[kBlock / 64];
+    v = wave_max_u64(v);
+    if ((threadIdx.x & 63) == 0) { s[threadIdx.x >> 6] = v; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / 64; ++w) { v = s[w] > v ? s[w] : v; }
+    }
+    __syncthreads();   // s is reused by the next call
+    return v;
+}
+
+template <int kBlock>
+__device__ __forceinline__ unsigned long long block_min_u64(unsigned long long v)
+{
+    __shared__ unsigned long long s[kBlock / 64];
+    v = wave_min_u64(v);
+    if ((threadIdx.x & 63) == 0) { s[threadIdx.x >> 6] = v; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / 64; ++w) { v = s[w] < v ? s[w] : v; }
+    }
+    __syncthreads();   // s is reused by the next call
+    return v;
+}
+
+template <int kBlock>
+__device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v)
+{
+    __shared__ unsigned long long s[kBlock / 64];
+    v = wave_sum_u64(v);
+    if ((threadIdx.x & 63) == 0) { s[threadIdx.x >> 6] = v; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / 64; ++w) { v += s[w]; }
+    }
+    __syncthreads();   // s is reused by the next call
+    return v;
+}
+
+// Block-wide sum of doubles in a fixed order (a shuffle tree per wave, then
+// the waves in index order): the same bits on every call. Valid in thread 0.
+template <int kBlock>
+__device__ __forceinline__ double block_sum_f64(double v)
+{
+    __shared__ double s[kBlock / 64];
+    for (int off = 32; off >= 1; off >>= 1) { v += __shfl_down(v, off, 64); }
+    if ((threadIdx.x & 63) == 0) { s[threadIdx.x >> 6] = v; }
+    __syncthreads();
+    double r = s[0];
+    for (int w = 1; w < kBlock / 64; ++w) { r += s[w]; }
+    __syncthreads();   // s is reused by the next call
+    return r;
+}
+
 // Global-memory accesses through address-space-1 pointers: where a pointer
 // comes out of a table in memory the compiler cannot infer its space and
 // would emit flat instructions (which also count against the LDS counter and

@@ -208,6 +208,7 @@ __global__ __launch_bounds__(kThreads) void obrien_kernel(int lo0, int lo1, int 
                                                           unsigned long long* worst)
 {
     const int ncol = nx * ny;
+    unsigned long long wmax = 0;
     for (int t = blockIdx.x * kThreads + threadIdx.x; t < ncol; t += gridDim.x * kThreads) {
         const int i = lo0 + t % nx, j = lo1 + t / nx;
         om(i, j, klo) = 0.0;
@@ -216,7 +217,7 @@ __global__ __launch_bounds__(kThreads) void obrien_kernel(int lo0, int lo1, int 
         }
         const double E = om(i, j, klo + nz);
         const double aE = fabs(E);
-        if (aE == aE) { atomicMax(worst, dev::abs_bits(aE)); }
+        if (aE == aE) { wmax = dev::abs_bits(aE) > wmax ? dev::abs_bits(aE) : wmax; }
         auto zf = [&](int k) { return 0.25 * (zn(i, j, k) + zn(i + 1, j, k) + zn(i, j + 1, k) + zn(i + 1, j + 1, k)); };
         const double zg = zf(klo), depth = zf(klo + nz) - zg;
         for (int k = klo + 1; k <= klo + nz; ++k) {
@@ -225,6 +226,8 @@ __global__ __launch_bounds__(kThreads) void obrien_kernel(int lo0, int lo1, int 
         }
         om(i, j, klo + nz) = 0.0;
     }
+    wmax = dev::block_max_u64<kThreads>(wmax);
+    if (threadIdx.x == 0 && wmax) { atomicMax(worst, wmax); }
 }
 
 // ---- host side -------------------------------------------------------------------

@@ -106,9 +106,9 @@ __global__ __launch_bounds__(kThreads) void max_kernel(Box3 b, CV a, unsigned lo
             m = key > m ? key : m;
         }
     }
-    m = dev::wave_max_u64(m);
-    nan = dev::wave_max_u64(nan);
-    if ((threadIdx.x & 63) == 0) {
+    m = dev::block_max_u64<kThreads>(m);
+    nan = dev::block_max_u64<kThreads>(nan);
+    if (threadIdx.x == 0) {
         if (m) { atomicMax(out, m); }
         if (nan) { atomicOr(out + 1, 1ull); }
     }
@@ -224,9 +224,9 @@ __global__ __launch_bounds__(kThreads) void confidence_kernel(Box3 b, CV ru, CV 
         const unsigned long long key = order_key(val);
         top = key > top ? key : top;
     }
-    for (int off = 32; off >= 1; off >>= 1) { bad += __shfl_xor(bad, off, 64); }
-    top = dev::wave_max_u64(top);
-    if ((threadIdx.x & 63) == 0) {
+    bad = dev::block_sum_u64<kThreads>(bad);
+    top = dev::block_max_u64<kThreads>(top);
+    if (threadIdx.x == 0) {
         if (bad) { atomicAdd(out, bad); }
         atomicMax(out + 1, top);
     }
@@ -376,7 +376,7 @@ int gust_global_max(const gust_geometry* geom, int32_t loc, const gust_array4* a
         auto* d = static_cast<unsigned long long*>(Workspace::get(st, kSlotOut, 2 * sizeof(unsigned long long)));
         GUST_HIP(hipMemsetAsync(d, 0, 2 * sizeof(unsigned long long), st));
         const Box3 b = box3(vb);
-        max_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(b, comp_view(a, comp), d);
+        max_kernel<<<cgrid::reduce_blocks(b.npts()), kThreads, 0, st>>>(b, comp_view(a, comp), d);
         const unsigned long long* h = fetch<unsigned long long>(st, d, 2, kSlotOut);
         if (h[1]) {
             *max = std::numeric_limits<double>::quiet_NaN();
@@ -521,7 +521,7 @@ int gust_confidence(const gust_geometry* geom, const gust_residuals* R, const gu
         GUST_HIP(hipMemsetAsync(d, 0, 2 * sizeof(unsigned long long), st));
         const IBox cells = loc_box(g, GUST_LOC_CELL);
         const Box3 b = box3(cells);
-        confidence_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(
+        confidence_kernel<<<cgrid::reduce_blocks(b.npts()), kThreads, 0, st>>>(
             b, dview<const double>(R->R[0]), dview<const double>(R->R[1]), dview<const double>(R->R[2]),
             dview<const double>(R->R[3]), dview<const double>(R->R[4]), dview<const double>(R->R[5]),
             dview<const double>(z_nd), cells.lo[2], th, d);

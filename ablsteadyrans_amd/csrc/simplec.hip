@@ -34,15 +34,16 @@ __device__ __forceinline__ double sc_min(double a, double b) { return (b < a) ? 
 
 __global__ __launch_bounds__(kThreads) void nonneg_flag_kernel(Box3 b, CV d, unsigned int* flag)
 {
-    bool bad = false;
+    unsigned long long bad = 0;
     GUST_GRID_LOOP(b)
     {
         int i, j, k;
         b.at(t, i, j, k);
         const double x = d(i, j, k);
-        if (!(x >= 0) || !(x < INFINITY)) { bad = true; }
+        if (!(x >= 0) || !(x < INFINITY)) { bad = 1; }
     }
-    if (bad) { atomicOr(flag, 1u); }
+    bad = dev::block_max_u64<kThreads>(bad);
+    if (threadIdx.x == 0 && bad) { atomicOr(flag, 1u); }
 }
 
 __global__ __launch_bounds__(kThreads) void beta_keep_kernel(Box3 b, CV d, V beta, V keep)
@@ -70,8 +71,8 @@ __global__ __launch_bounds__(kThreads) void max_abs_kernel(Box3 b, CV a, CV m, i
         const unsigned long long bits = dev::abs_bits(x);
         best = bits > best ? bits : best;
     }
-    best = dev::wave_max_u64(best);
-    if ((threadIdx.x & 63) == 0 && best) { atomicMax(out, best); }
+    best = dev::block_max_u64<kThreads>(best);
+    if (threadIdx.x == 0 && best) { atomicMax(out, best); }
 }
 
 // min a over the box (a > 0), as ordered bits into *out (atomic min).
@@ -85,11 +86,8 @@ __global__ __launch_bounds__(kThreads) void min_pos_kernel(Box3 b, CV a, unsigne
         const unsigned long long bits = dev::abs_bits(a(i, j, k));
         best = bits < best ? bits : best;
     }
-    for (int off = 32; off >= 1; off >>= 1) {
-        const unsigned long long o = __shfl_xor(best, off, 64);
-        best = o < best ? o : best;
-    }
-    if ((threadIdx.x & 63) == 0) { atomicMin(out, best); }
+    best = dev::block_min_u64<kThreads>(best);
+    if (threadIdx.x == 0) { atomicMin(out, best); }
 }
 
 // y op= x (op 0: y += a x, 1: y *= x, 2: y -= x, 3: y = 0).
@@ -210,17 +208,38 @@ __global__ __launch_bounds__(kThreads) void negate_kernel(Box3 b, CV R, V rhs)
     }
 }
 
-// Per-level sums of a face field over the faces counted once (one thread
-// per level, i fastest then j: a fixed order, so the sums are reproducible).
-__global__ __launch_bounds__(kThreads) void level_sum_kernel(CV f, int ilo, int ihi, int jlo, int jhi, int klo, int nz,
-                                                              double* out)
+// Per-level sums of a face field over the faces counted once: block
+// (chunk, level) adds its contiguous share of the level (i fastest, then j)
+// and reduces it with a fixed tree; the level's partials are then added in
+// chunk order, so the sums are the same on every call.
+constexpr int kLevelChunks = 16;
+
+__global__ __launch_bounds__(kThreads) void level_partial_kernel(CV f, int ilo, int ni, int jlo, int nj, int klo,
+                                                                  double* part)
+{
+    const int k = blockIdx.y;
+    const int64_t n = int64_t(ni) * nj;
+    const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t b0 = int64_t(blockIdx.x) * per;
+    const int64_t b1 = b0 + per < n ? b0 + per : n;
+    double s = 0.0;
+    for (int64_t t = b0 + threadIdx.x; t < b1; t += kThreads) {
+        const int i = ilo + int(t % ni), j = jlo + int(t / ni);
+        s += f(i, j, klo + k);
+    }
+    s = dev::block_sum_f64<kThreads>(s);
+    if (threadIdx.x == 0) { part[int64_t(k) * gridDim.x + blockIdx.x] = s; }
+}
+
+// The level means: the partials in chunk order over the level's face count.
+__global__ __launch_bounds__(kThreads) void level_mean_kernel(const double* part, int nchunks, int nz, double npts,
+                                                               double* mean)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nz) { return; }
     double s = 0.0;
-    for (int j = jlo; j <= jhi; ++j)
-        for (int i = ilo; i <= ihi; ++i) { s += f(i, j, klo + k); }
-    out[k] = s;
+    for (int c = 0; c < nchunks; ++c) { s += part[int64_t(k) * nchunks + c]; }
+    mean[k] = s / npts;
 }
 
 // Point-implicit rotation (Source/Predictor.cpp:260-342).
@@ -316,8 +335,8 @@ __global__ __launch_bounds__(kThreads) void clip_kernel(Box3 b, V f, double floo
             ++c;
         }
     }
-    for (int off = 32; off >= 1; off >>= 1) { c += __shfl_xor(c, off, 64); }
-    if ((threadIdx.x & 63) == 0 && c) { atomicAdd(count, c); }
+    c = dev::block_sum_u64<kThreads>(c);
+    if (threadIdx.x == 0 && c) { atomicAdd(count, c); }
 }
 
 // ---------------------------------------------------------------------------
@@ -358,7 +377,7 @@ public:
     {
         const Box3 b = box3(box);
         GUST_HIP(hipMemsetAsync(dev_, 0, sizeof(unsigned long long), st_));
-        max_abs_kernel<<<blocks_for(b.npts()), kThreads, 0, st_>>>(b, dview<const double>(a),
+        max_abs_kernel<<<cgrid::reduce_blocks(b.npts()), kThreads, 0, st_>>>(b, dview<const double>(a),
                                                                    dview<const double>(mask ? mask : a),
                                                                    mask ? 1 : 0, dev_);
         return fetch();
@@ -367,14 +386,14 @@ public:
     {
         const Box3 b = box3(box);
         GUST_HIP(hipMemsetAsync(dev_, 0xff, sizeof(unsigned long long), st_));
-        min_pos_kernel<<<blocks_for(b.npts()), kThreads, 0, st_>>>(b, dview<const double>(a), dev_);
+        min_pos_kernel<<<cgrid::reduce_blocks(b.npts()), kThreads, 0, st_>>>(b, dview<const double>(a), dev_);
         return fetch();
     }
     bool flag_bad_nonneg(const IBox& box, const gust_array4* a)
     {
         const Box3 b = box3(box);
         GUST_HIP(hipMemsetAsync(dev_, 0, sizeof(unsigned long long), st_));
-        nonneg_flag_kernel<<<blocks_for(b.npts()), kThreads, 0, st_>>>(b, dview<const double>(a),
+        nonneg_flag_kernel<<<cgrid::reduce_blocks(b.npts()), kThreads, 0, st_>>>(b, dview<const double>(a),
                                                                        reinterpret_cast<unsigned int*>(dev_));
         GUST_HIP(hipGetLastError());
         GUST_HIP(hipMemcpyAsync(host_, dev_, sizeof(unsigned long long), hipMemcpyDeviceToHost, st_));
@@ -633,17 +652,16 @@ int gust_momentum_predictor(const gust_geometry* geom, gust_faces* vel, const gu
                 IBox sb = ob;
                 if (per_o) { sb.hi[co] -= 1; }
                 const int nz = g.hi[2] - g.lo[2] + 1;
-                auto* sums = static_cast<double*>(Workspace::get(st, 47, size_t(nz) * sizeof(double)));
-                level_sum_kernel<<<(nz + 63) / 64, 64, 0, st>>>(dview<const double>(&og.v), sb.lo[0], sb.hi[0],
-                                                                sb.lo[1], sb.hi[1], g.lo[2], nz, sums);
-                std::vector<double> h(static_cast<size_t>(nz));
-                GUST_HIP(hipMemcpyAsync(h.data(), sums, size_t(nz) * sizeof(double), hipMemcpyDeviceToHost, st));
-                GUST_HIP(hipStreamSynchronize(st));
+                auto* part = static_cast<double*>(Workspace::get(st, 47, size_t(nz) * kLevelChunks * sizeof(double)));
+                level_partial_kernel<<<dim3(kLevelChunks, nz), kThreads, 0, st>>>(
+                    dview<const double>(&og.v), sb.lo[0], sb.hi[0] - sb.lo[0] + 1, sb.lo[1], sb.hi[1] - sb.lo[1] + 1,
+                    g.lo[2], part);
                 const double npts_level = double((g.hi[co] - g.lo[co] + 1) + (per_o ? 0 : 1)) *
                                           double(g.hi[c] - g.lo[c] + 1);
-                for (double& x : h) { x /= npts_level; }
                 auto* dmean = static_cast<double*>(Workspace::get(st, 48, size_t(nz) * sizeof(double)));
-                GUST_HIP(hipMemcpyAsync(dmean, h.data(), size_t(nz) * sizeof(double), hipMemcpyHostToDevice, st));
+                level_mean_kernel<<<(nz + kThreads - 1) / kThreads, kThreads, 0, st>>>(part, kLevelChunks, nz,
+                                                                                      npts_level, dmean);
+                GUST_HIP(hipGetLastError());
                 const int lo_edge = g.lo[c], hi_edge = g.hi[c] + 1;
                 const int periodic = g.periodic[c] ? 1 : 0;
                 if (c == 0) {
@@ -658,7 +676,6 @@ int gust_momentum_predictor(const gust_geometry* geom, gust_faces* vel, const gu
                         dview<double>(&rhs.v), dmean, g.lo[2], opt->coriolis_f, relax, periodic, lo_edge, hi_edge);
                 }
                 GUST_HIP(hipGetLastError());
-                GUST_HIP(hipStreamSynchronize(st));   // h is read by the copy above
             }
             if (c == 2 && buoyancy) {
                 check_field(op, "R_theta", R_theta, g, GUST_LOC_CELL, 0);
@@ -719,7 +736,7 @@ int gust_scalar_sweep(const gust_geometry* geom, int32_t eq, gust_array4* field,
             auto* cnt = static_cast<unsigned long long*>(Workspace::get(st, 54, 64));
             auto* hc = static_cast<unsigned long long*>(Workspace::get_host_pinned(54, 64));
             GUST_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), st));
-            clip_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<double>(field), floor, cnt);
+            clip_kernel<<<cgrid::reduce_blocks(b.npts()), kThreads, 0, st>>>(b, dview<double>(field), floor, cnt);
             GUST_HIP(hipGetLastError());
             GUST_HIP(hipMemcpyAsync(hc, cnt, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
             GUST_HIP(hipStreamSynchronize(st));

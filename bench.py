@@ -1,22 +1,26 @@
 #!/usr/bin/env python3
 """Benchmark of the SIMPLEC hot path on one MI355X.
 
-Workload (the kernel-level configuration of the path): the fractal-terrain
-tile of 256 x 256 x 96 cells, 65,536 columns, the five sweep variables of an
-iteration -- u on x-faces, v on y-faces, w on z-faces, theta and k on cells --
-each a seeded random M-matrix line system (coupling recipe of
-Tests/Unit_Tests/test_LineSweeps.cpp:76-115). One step = one z-line Jacobi
-sweep over all five systems in one launch (a steady-state sweep: the input
-iterate is the previous sweep's), FP64. Inputs are resident in HBM before the
-timed region.
+Headline (value): steady solves per second at a 6-order residual drop, FP64,
+on the reference's Neutral_Column deck (Tests/Canonical_RANS/Neutral_Column/
+inputs_steady_column: 4 x 4 x 64 cells over 2560 x 2560 x 1000 m, periodic
+sides, surface layer z0 = 0.1 m, slip top, the driving force F = 1.225e-4
+m/s^2, the uncapped closure with the logarithmic face mean, two z-line
+sweeps, CFL ramp to 50, steady at residual_tol 1e-8 and a 6-order drop; the
+reference measured 2231 iterations). One step = one whole steady solve from
+the deck's first guess (the log law with u* = 0.25 m/s and its equilibrium
+k), the driver loop on the device (gust_steady_solve); the state is resident
+in HBM before the timed region. The mixed form (FP32 multigrid V-cycles
+under the FP64 GMRES of every pressure correction) is timed the same way.
 
-value = algorithmic HBM bytes of the step / step time, with 10 reals per
-unknown (6 off-diagonal coefficients, a_P, rhs, the input and the output
-iterate). The roofline object carries the same figure for the sweep kernel
-timed with HIP events on its own stream; traffic comes from the committed
-rocprofv3 counter summary when present. cpu_baseline: the oracle (FP64 C
-restatement of the reference's zline_jacobi, one host thread) on a bounded
-sample of the same workload.
+Also measured (extra): one SIMPLEC iteration on the Gaussian-hill tile
+(200 x 200 x 64, dz1 = 1 m, FP64 and mixed); the z-line Jacobi sweep of the
+five variables on the fractal-terrain tile (256 x 256 x 96, 65,536 columns;
+the roofline object: algorithmic bytes per launch / launch time from HIP
+events on the launch stream, 10 reals per unknown); the V(2,2) cycle.
+cpu_baseline: the oracle's SIMPLEC iteration composite (FP64 C pieces and an
+exact sparse pressure solve, one host thread) on the same column deck for a
+bounded number of iterations, scaled by the deck's 2231 iterations.
 """
 from __future__ import annotations
 
@@ -207,6 +211,148 @@ def _vcycle_bytes(s, pre=2, post=2, coarse=16):
     return b
 
 
+COLUMN_ITERATIONS_REF = 2231   # Tests/Canonical_RANS/Neutral_Column/README.md:26
+
+
+def _column_deck():
+    """The Neutral_Column deck: (tile, residual inputs, simplec options,
+    driver options, first-guess profile)."""
+    from ablsteadyrans_amd import initializer as GI
+    from ablsteadyrans_amd import mesh as M
+    from ablsteadyrans_amd import physics as GP
+    geom = M.geometry((4, 4, 64), (0.0, 0.0, 0.0), (2560.0, 2560.0, 1000.0), (1, 1))
+    tile = M.build_mesh(geom, (0.0, 0.0, 0.0), (2560.0, 2560.0, 1000.0), M.make_zlevels(64, 1000.0),
+                        M.Terrain(kind="flat"))
+    inputs = GP.ResidualInputs(closure=GP.ClosureOptions(max_geom_lscale=1.0e12),
+                               surface=GP.SurfaceLayerOptions(z0=0.1),
+                               forcing=GP.ForcingOptions(pressure_gradient=(1.225e-4, 0.0, 0.0)),
+                               log_face_mean=True)
+    dopt = GP.DriverOptions(max_iterations=20000, cfl0=1.0, cfl_max=50.0, residual_tol=1.0e-8,
+                            residual_drop_orders=6.0)
+    prof = GI.InflowProfile(kind=GI.LOG_LAW, ustar=0.25, z0=0.1, unbounded=True)
+    return tile, inputs, dopt, prof
+
+
+def _time_solves(precision, steps, warmup):
+    """Whole steady solves of the column deck from its first guess: returns
+    (seconds per solve, iterations, status, u*)."""
+    import torch
+    from ablsteadyrans_amd import initializer as GI
+    from ablsteadyrans_amd import physics as GP
+    from ablsteadyrans_amd.poisson import DeviceMesh
+    tile, inputs, dopt, prof = _column_deck()
+    dm = DeviceMesh(tile)
+    stage = GP.SimplecStage(dm, inputs, GP.SimplecOptions(sweeps=2, height=1000.0, poisson_precision=precision))
+    q = GP.DeviceState(tile.geom)
+    opt = GI.InitOptions(first_guess=1, projection=False)
+    times, last = [], None
+    for rep in range(warmup + steps):
+        GI.first_guess_state(dm, prof, opt, q)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = stage.solve(q, dopt)
+        torch.cuda.synchronize()
+        if rep >= warmup:
+            times.append(time.perf_counter() - t0)
+        last = r
+    return sum(times) / len(times), int(last.iterations), int(last.status), float(last.mon[2]), times
+
+
+def _hill_tile():
+    """The Gaussian hill h = 100 exp(-(r/250 m)^2) m centred in 4 x 4 km at
+    dx = 20 m, 64 levels from dz1 = 1 m growing by 1.08, periodic sides."""
+    from ablsteadyrans_amd import mesh as M
+    zl = M.make_zlevels(64, 1.0, dz0=1.0, stretch_ratio=1.08)
+    top = float(zl[-1])
+    geom = M.geometry((200, 200, 64), (0.0, 0.0, 0.0), (4000.0, 4000.0, top), (1, 1))
+    t = M.Terrain(kind="hills", hills=[(2000.0, 2000.0, 100.0, 250.0)])
+    return M.build_mesh(geom, (0.0, 0.0, 0.0), (4000.0, 4000.0, top), zl, t), top
+
+
+def _time_hill_iterations(precision, iters, warmup=3):
+    """ms per SIMPLEC iteration on the hill tile (neutral ABL with rotation,
+    the log-law first guess, local dtau at CFL 2), HIP events on the stream."""
+    import torch
+    from ablsteadyrans_amd import initializer as GI
+    from ablsteadyrans_amd import physics as GP
+    from ablsteadyrans_amd.fab import Fab
+    from ablsteadyrans_amd.poisson import DeviceMesh
+    tile, top = _hill_tile()
+    dm = DeviceMesh(tile)
+    inputs = GP.ResidualInputs(closure=GP.ClosureOptions(tke_floor=1e-6), surface=GP.SurfaceLayerOptions(z0=0.03),
+                               forcing=GP.ForcingOptions(coriolis=True, coriolis_3d=False, latitude=90.0,
+                                                         rotation_period=125663.7061435917, geostrophic=True,
+                                                         ug=10.0))
+    stage = GP.SimplecStage(dm, inputs, GP.SimplecOptions(sweeps=2, tke_floor=1e-6, height=top,
+                                                          poisson_precision=precision))
+    q = GP.DeviceState(tile.geom)
+    GI.first_guess_state(dm, GI.InflowProfile(kind=GI.LOG_LAW, ustar=0.4, z0=0.03), GI.InitOptions(first_guess=1), q)
+    dtau = Fab.at_loc(tile.geom, 0, 1, 0)
+    stage.init(q)
+    import ctypes as C
+    from ablsteadyrans_amd import _lib
+    g = tile.geom.c()
+
+    def step():
+        nu = stage.nu_view()
+        _lib.check(_lib.lib.gust_local_dtau(C.byref(g), C.byref(q["u"].array4()), C.byref(q["v"].array4()),
+                                            C.byref(q["w"].array4()), C.byref(dm.z_nd.array4()),
+                                            C.byref(dm.detJ.array4()), C.byref(nu), 2.0, C.byref(dtau.array4()),
+                                            C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        stage.iterate(q, dtau)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(iters):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / iters
+
+
+def _cpu_column_baseline(target_s=15.0):
+    """The oracle's SIMPLEC iteration composite on the column deck (FP64 C
+    pieces, an exact sparse pressure solve; one host thread), iterated from
+    the deck's first guess at local dtau with CFL 1 for about target_s;
+    reported as steady solves/s with the deck's 2231 iterations."""
+    import numpy as np
+    from oracle import oracle as orc
+    from oracle import physics as P
+    from tests import physics_helpers as PHY
+    from tests import stage_helpers as STH
+    tile, _, _, _ = _column_deck()
+    m = PHY.omesh(tile)
+    g = m.geom
+    q = PHY.zeros_state(tile.geom)
+    _, _, zu = PHY.positions(tile, 1)
+    q["u"][0] = 0.25 / 0.41 * np.log((np.maximum(zu, 0.0) + 0.1) / 0.1)
+    q["theta"][:] = 300.0
+    q["tke"][:] = 0.25 * 0.25 / (0.5562 * 0.5562)
+    phys = P.Physics(closure=P.ClosureOptions(max_geom_lscale=1.0e12),
+                     bc=P.Boundary(zlo=P.SURFACE_LAYER, zhi=P.SLIP_WALL),
+                     surface=P.SurfaceOptions(z0=0.1), forcing=P.Forcing(pressure_gradient=(1.225e-4, 0.0, 0.0)),
+                     log_face_mean=True)
+    iters = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < target_s:
+        q1 = {n: a[:, 1:-1, 1:-1, 1:-1].copy() for n, a in q.items()}
+        dtau = orc.local_dtau(q1["u"], q1["v"], q1["w"], (-1, -1, -1), m.z, (-1, -1, -1), m.J, (-1, -1, -1), g, 1.0)
+        STH.simplec_iteration(tile, m, q, phys, dtau, sweeps=2)
+        iters += 1
+    dt = time.perf_counter() - t0
+    per_it = dt / iters
+    return {"value": round(1.0 / (per_it * COLUMN_ITERATIONS_REF), 5), "unit": "steady solves/s", "cores": 1,
+            "kind": "port",
+            "sample": f"the oracle's SIMPLEC iteration composite on the Neutral_Column deck (4x4x64), {iters} "
+                      f"iterations in {dt:.1f} s = {per_it * 1e3:.1f} ms per iteration, 1 thread, scaled by the "
+                      f"deck's {COLUMN_ITERATIONS_REF} iterations to steady",
+            "ms_per_iteration": round(per_it * 1e3, 3)}
+
+
 def _traffic_from_profile():
     p = os.path.join(ROOT, "profiles", "zjacobi_sweep_c3_fp64_pmc.json")
     if os.path.exists(p):
@@ -220,9 +366,9 @@ def _traffic_from_profile():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--steps", type=int, default=3, help="timed steady solves per precision")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed steady solves per precision")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--profile-only", action="store_true", help="sweeps only (for rocprofv3 runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only a plain 1 GiB device copy, repeated (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -236,72 +382,102 @@ def main():
     import ablsteadyrans_amd as G
 
     torch.cuda.set_device(0)
+    sweep_steps, sweep_warmup = 200, 20
     if args.copy_calibration:
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
         dst = torch.empty_like(src)
-        for _ in range(max(1, args.steps)):
+        for _ in range(sweep_steps):
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8,
-                          "launches": max(1, args.steps)}))
+                          "launches": sweep_steps}))
         return
-    sys64 = _systems(torch.float64, 20264203)
-    plan64 = G.JacobiPlan(sys64)
-    unknowns = plan64.unknowns
-    bytes64 = REALS_PER_UNKNOWN * 8 * unknowns
-    ms_ev, ms_wall = _time_sweeps(plan64, args.steps, args.warmup)
     if args.profile_only:
+        sys64 = _systems(torch.float64, 20264203)
+        plan64 = G.JacobiPlan(sys64)
+        ms_ev, _ = _time_sweeps(plan64, sweep_steps, sweep_warmup)
         sys32 = _systems(torch.float32, 20264203)
         plan32 = G.JacobiPlan(sys32)
-        _time_sweeps(plan32, args.steps, args.warmup)
+        _time_sweeps(plan32, sweep_steps, sweep_warmup)
         del plan64, plan32, sys64, sys32
         tile = _fractal_tile()
         v64, _ = _time_vcycle(tile, 0, 3)
         v32, _ = _time_vcycle(tile, 1, 3)
         print(json.dumps({"profile_only": True, "ms_per_sweep_fp64": ms_ev, "vcycle_ms": [v64, v32]}))
         return
+
+    # 1. the headline: whole steady solves of the column deck, FP64 and mixed
+    print("bench: steady solves of the Neutral_Column deck (FP64)", file=sys.stderr, flush=True)
+    s64, it64, st64, us64, t64 = _time_solves(0, args.steps, args.warmup)
+    print("bench: steady solves of the Neutral_Column deck (mixed)", file=sys.stderr, flush=True)
+    s32, it32, st32, us32, t32 = _time_solves(1, args.steps, args.warmup)
+    # 2. one SIMPLEC iteration on the hill tile
+    print("bench: SIMPLEC iterations on the 200x200x64 hill tile", file=sys.stderr, flush=True)
+    hill64 = _time_hill_iterations(0, 10)
+    hill32 = _time_hill_iterations(1, 10)
+    # 3. the z-line sweep (the roofline kernel) and the V-cycle on the fractal tile
+    print("bench: z-line sweeps and V-cycles on the 256x256x96 fractal tile", file=sys.stderr, flush=True)
+    sys64 = _systems(torch.float64, 20264203)
+    plan64 = G.JacobiPlan(sys64)
+    unknowns = plan64.unknowns
+    bytes64 = REALS_PER_UNKNOWN * 8 * unknowns
+    ms_ev, _ = _time_sweeps(plan64, sweep_steps, sweep_warmup)
     run64 = _time_run(plan64, 2, 20)
-    del plan64
+    del plan64, sys64
     tile = _fractal_tile()
     vc64, levels = _time_vcycle(tile, 0, 10)
     vc32, _ = _time_vcycle(tile, 1, 10)
     sys32 = _systems(torch.float32, 20264203)
     plan32 = G.JacobiPlan(sys32)
-    ms32, _ = _time_sweeps(plan32, args.steps, args.warmup)
+    ms32, _ = _time_sweeps(plan32, sweep_steps, sweep_warmup)
     run32 = _time_run(plan32, 2, 20)
     bytes32 = REALS_PER_UNKNOWN * 4 * unknowns
     gbs64 = bytes64 / (ms_ev * 1e-3) / 1e9
     gbs32 = bytes32 / (ms32 * 1e-3) / 1e9
-    cpu = None if args.no_cpu else _cpu_baseline()
+    cpu = cpu_sweep = None
+    if not args.no_cpu:
+        print("bench: CPU baselines (oracle, one thread)", file=sys.stderr, flush=True)
+        cpu = _cpu_column_baseline()
+        cpu_sweep = _cpu_baseline()
+    value = 1.0 / s64
     line = {
         "metric": "steady solves/s at 6-order residual drop, mixed vs FP64; sweep HBM GB/s; p99",
-        "value": round(gbs64, 2),
-        "unit": "GB/s",
+        "value": round(value, 4),
+        "unit": "steady solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_wall, 5),
+        "ms_per_step": round(s64 * 1e3, 2),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded random M-matrix line systems, resident in HBM)",
-        "config": {"workload": "z-line Jacobi sweep, 5 variables (u, v, w, theta, k) on 256x256x96 "
-                               "(65,536 columns x 96 levels), one batched launch per sweep",
-                   "unknowns_per_sweep": unknowns, "reals_per_unknown": REALS_PER_UNKNOWN,
-                   "sweep_kind": "steady-state sweep (input iterate nonzero)"},
+        "data": "synthetic: the reference's Neutral_Column deck (its first guess), resident in HBM",
+        "config": {"workload": "steady solve to a 6-order residual drop (residual_tol 1e-8) of the reference's "
+                               "Neutral_Column deck: 4x4x64 cells, 2560x2560x1000 m, periodic sides, surface "
+                               "layer z0 0.1 m, F = 1.225e-4 m/s^2, 2 z-line sweeps, CFL 1 -> 50",
+                   "iterations": it64, "iterations_reference": COLUMN_ITERATIONS_REF, "status": st64,
+                   "ustar": round(us64, 8), "seconds_per_solve": [round(t, 3) for t in t64]},
         "roofline": {"bound": "hbm", "achieved": round(gbs64, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": _traffic_from_profile(),
-                     "kernel": "zjacobi_sweep_kernel<double,false,false>",
+                     "kernel": "zjacobi_sweep_kernel<double,false,false> (the metric's sweep HBM GB/s: 5 "
+                               "variables on 256x256x96, one launch)",
                      "algorithmic_bytes_per_launch": bytes64, "avg_launch_ms": round(ms_ev, 5)},
         "cpu_baseline": cpu,
-        "extra": {"fp32_sweep_gbs": round(gbs32, 2), "fp32_ms_per_sweep": round(ms32, 5),
-                  "fp32_frac": round(gbs32 / HBM_PEAK_GBS, 4),
+        "extra": {"mixed_steady_solves_per_s": round(1.0 / s32, 4), "mixed_iterations": it32, "mixed_status": st32,
+                  "mixed_ustar": round(us32, 8), "mixed_vs_fp64": round(s64 / s32, 3),
+                  "ms_per_iteration_column_fp64": round(s64 * 1e3 / max(it64, 1), 3),
+                  "hill_200x200x64_ms_per_iteration_fp64": round(hill64, 3),
+                  "hill_200x200x64_ms_per_iteration_mixed": round(hill32, 3),
+                  "sweep_fp64_gbs": round(gbs64, 2), "sweep_fp64_ms": round(ms_ev, 5),
+                  "sweep_fp32_gbs": round(gbs32, 2), "sweep_fp32_ms": round(ms32, 5),
+                  "sweep_fp32_frac": round(gbs32 / HBM_PEAK_GBS, 4),
                   "fp64_line_solve_2_sweeps_ms": round(run64, 4), "fp32_line_solve_2_sweeps_ms": round(run32, 4),
                   "vcycle_levels": levels,
                   "fp64_vcycle_ms": round(vc64, 4), "fp64_vcycle_gbs": round(_vcycle_bytes(8) / (vc64 * 1e-3) / 1e9, 1),
                   "fp32_vcycle_ms": round(vc32, 4), "fp32_vcycle_gbs": round(_vcycle_bytes(4) / (vc32 * 1e-3) / 1e9, 1),
-                  "note": "steady solves/s and p99 need the full SIMPLEC iteration (not yet on this path)"},
+                  "cpu_sweep_baseline": cpu_sweep,
+                  "note": "p99 needs the serving loop (not built)"},
     }
     print(json.dumps(line))
 
