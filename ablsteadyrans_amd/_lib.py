@@ -196,6 +196,17 @@ class ForcingC(C.Structure):
                 ("ug", C.c_double), ("vg", C.c_double), ("pressure_gradient", C.c_double * 3)]
 
 
+class InflowProfileC(C.Structure):
+    """``gust_inflow_profile`` (Source/InflowProfile.H:42-66, the table in memory)."""
+
+    _fields_ = [("kind", C.c_int32), ("ustar", C.c_double), ("z0", C.c_double), ("speed_ref", C.c_double),
+                ("z_ref", C.c_double), ("alpha", C.c_double), ("direction", C.c_double), ("Cmu0", C.c_double),
+                ("kappa", C.c_double), ("unbounded", C.c_int32), ("l_max", C.c_double), ("n", C.c_int32),
+                ("z", C.POINTER(C.c_double)), ("u", C.POINTER(C.c_double)), ("v", C.POINTER(C.c_double)),
+                ("theta", C.POINTER(C.c_double)), ("k", C.POINTER(C.c_double)), ("n_theta_ref", C.c_int32),
+                ("theta_ref_z", C.c_double * 64), ("theta_ref", C.c_double * 64)]
+
+
 class PhysicsC(C.Structure):
     """``gust_physics`` (Source/Residual.H:90-128)."""
 
@@ -205,7 +216,7 @@ class PhysicsC(C.Structure):
                 ("zhi_theta_grad", C.c_double), ("wall_velocity", (C.c_double * 3) * 6),
                 ("n_theta_ref", C.c_int32), ("theta_ref_z", C.c_double * 64), ("theta_ref", C.c_double * 64),
                 ("momentum_limiter", C.c_int32), ("theta_limiter", C.c_int32), ("momentum_eps", C.c_double),
-                ("theta_eps", C.c_double), ("face_mean", C.c_int32)]
+                ("theta_eps", C.c_double), ("face_mean", C.c_int32), ("inflow", InflowProfileC)]
 
 
 class StateC(C.Structure):
@@ -225,17 +236,6 @@ class ClosureFieldsC(C.Structure):
     """``gust_closure_fields``."""
 
     _fields_ = [(n, C.POINTER(Array4)) for n in ("nu", "gamma_theta", "gamma_tke", "Ps", "Pb", "eps", "dSdk")]
-
-
-class InflowProfileC(C.Structure):
-    """``gust_inflow_profile`` (Source/InflowProfile.H:42-66, the table in memory)."""
-
-    _fields_ = [("kind", C.c_int32), ("ustar", C.c_double), ("z0", C.c_double), ("speed_ref", C.c_double),
-                ("z_ref", C.c_double), ("alpha", C.c_double), ("direction", C.c_double), ("Cmu0", C.c_double),
-                ("kappa", C.c_double), ("unbounded", C.c_int32), ("l_max", C.c_double), ("n", C.c_int32),
-                ("z", C.POINTER(C.c_double)), ("u", C.POINTER(C.c_double)), ("v", C.POINTER(C.c_double)),
-                ("theta", C.POINTER(C.c_double)), ("k", C.POINTER(C.c_double)), ("n_theta_ref", C.c_int32),
-                ("theta_ref_z", C.c_double * 64), ("theta_ref", C.c_double * 64)]
 
 
 class InitOptionsC(C.Structure):

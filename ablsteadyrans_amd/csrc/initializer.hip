@@ -21,6 +21,7 @@
 // host's in the last bit.
 #include "cgrid.hpp"
 #include "heights.hpp"
+#include "profile.hpp"
 #include "reduce.hpp"
 #include "stencil.hpp"
 
@@ -45,78 +46,12 @@ using cgrid::V;
 using heights::NodeZ;
 
 // Workspace slots of this file.
-constexpr int kSlotTable = 110, kSlotRhs = 111, kSlotLam = 112, kSlotF = 113 /* 3 slots */, kSlotOm = 116,
+constexpr int kSlotRhs = 111, kSlotLam = 112, kSlotF = 113 /* 3 slots */, kSlotOm = 116,
               kSlotCoef = 117 /* 4 slots */, kSlotDh = 121, kSlotZero = 122, kSlotWorst = 123, kSlotWorstHost = 124;
 
-// ---- the inflow profile -------------------------------------------------------
-
-struct ProfileView {
-    int kind;   // 0 log_law, 1 power_law, 2 table
-    double ustar, z0, speed_ref, z_ref, alpha, cosd, sind, Cmu0, kappa, inv_lmax;
-    const double *tz, *tu, *tv, *tt, *tk;
-    int n, has_k;
-    const double *rz, *rt;
-    int rn;
-};
-
-__device__ __forceinline__ double dmaxd(double a, double b) { return (a > b) ? a : b; }
-
-// Linear in a table of n increasing abscissae, constant beyond the ends
-// (InflowProfile.H:137-147).
-__device__ double table_interp(const double* x, const double* y, int n, double xq)
-{
-    if (n == 1 || xq <= x[0]) { return y[0]; }
-    if (xq >= x[n - 1]) { return y[n - 1]; }
-    int m = 1;
-    while (m < n - 1 && !(xq < x[m])) { ++m; }
-    const double w = (xq - x[m - 1]) / (x[m] - x[m - 1]);
-    return y[m - 1] + w * (y[m] - y[m - 1]);
-}
-
-// The profile at height z above the ground (InflowProfile.H:149-200): the
-// wind, theta, the equilibrium k of the k-L closure and nu_t = l u*_l.
-__device__ void profile_eval(const ProfileView& p, double z, double& u, double& v, double& theta, double& k,
-                             double& nut)
-{
-    z = dmaxd(z, 0.0);
-    // l / l_raw = 1 / (1 + l_raw / l_max), exactly 1 when unbounded
-    const double c = 1.0 / (1.0 + p.kappa * (z + p.z0) * p.inv_lmax);
-    double ustar_l;
-    if (p.kind == 2) {
-        u = table_interp(p.tz, p.tu, p.n, z);
-        v = table_interp(p.tz, p.tv, p.n, z);
-        theta = table_interp(p.tz, p.tt, p.n, z);
-        int m = 1;
-        while (m < p.n - 1 && !(z < p.tz[m])) { ++m; }
-        const double s0 = sqrt(p.tu[m - 1] * p.tu[m - 1] + p.tv[m - 1] * p.tv[m - 1]);
-        const double s1 = sqrt(p.tu[m] * p.tu[m] + p.tv[m] * p.tv[m]);
-        const double dsdz = (p.n > 1) ? (s1 - s0) / (p.tz[m] - p.tz[m - 1]) : 0.0;
-        ustar_l = p.kappa * (z + p.z0) * dmaxd(dsdz, 0.0) * c;
-        if (p.has_k) {
-            k = table_interp(p.tz, p.tk, p.n, z);
-            ustar_l = p.Cmu0 * sqrt(dmaxd(k, 0.0));
-        } else {
-            k = ustar_l * ustar_l / (p.Cmu0 * p.Cmu0);
-        }
-        nut = p.kappa * ustar_l * (z + p.z0) * c;
-        return;
-    }
-    double S;
-    if (p.kind == 0) {
-        S = p.ustar / p.kappa * log((z + p.z0) / p.z0);
-        ustar_l = p.ustar * c;
-    } else {
-        S = p.speed_ref * pow(z / p.z_ref, p.alpha);
-        // the slope at z0 below z0: max(z, z0) written in arithmetic
-        const double dsdz = p.alpha * S / (0.5 * (z + p.z0 + fabs(z - p.z0)));
-        ustar_l = p.kappa * (z + p.z0) * dsdz * c;
-    }
-    u = S * p.cosd;
-    v = S * p.sind;
-    theta = table_interp(p.rz, p.rt, p.rn, z);
-    k = ustar_l * ustar_l / (p.Cmu0 * p.Cmu0);
-    nut = p.kappa * ustar_l * (z + p.z0) * c;
-}
+using profile::dmaxd;
+using profile::profile_eval;
+using profile::ProfileView;
 
 __global__ __launch_bounds__(kThreads) void eval_kernel(ProfileView p, int n, const double* z, double* u, double* v,
                                                         double* th, double* k, double* nut)
@@ -295,77 +230,6 @@ void periodic_fill(hipStream_t st, const gust_geometry& g, gust_array4* a, int l
     GUST_HIP(hipGetLastError());
 }
 
-bool finite_pos(double x) { return !std::isnan(x) && x > 0 && std::isfinite(x); }
-
-// The problems with a profile, each naming its key (InflowProfile.cpp:14-47,
-// the table's rules 60-118 and the sounding's, Forcing.cpp:30-49).
-std::vector<std::string> profile_problems(const gust_inflow_profile& p)
-{
-    std::vector<std::string> err;
-    auto num = [](double x) { return std::to_string(x); };
-    if (!finite_pos(p.z0)) { err.push_back("bc.inflow_z0 = " + num(p.z0) + ": must be finite and > 0"); }
-    if (p.kind == GUST_INFLOW_LOG_LAW) {
-        if (!finite_pos(p.ustar)) { err.push_back("bc.inflow_ustar = " + num(p.ustar) + ": must be finite and > 0"); }
-    } else if (p.kind == GUST_INFLOW_POWER_LAW) {
-        if (std::isnan(p.speed_ref) || !(p.speed_ref >= 0 && std::isfinite(p.speed_ref))) {
-            err.push_back("bc.inflow_speed = " + num(p.speed_ref) + ": must be finite and >= 0");
-        }
-        if (!finite_pos(p.z_ref)) { err.push_back("bc.inflow_zref = " + num(p.z_ref) + ": must be finite and > 0"); }
-        if (!finite_pos(p.alpha)) { err.push_back("bc.inflow_alpha = " + num(p.alpha) + ": must be finite and > 0"); }
-    } else if (p.kind == GUST_INFLOW_TABLE) {
-        if (p.n < 2 || !p.z || !p.u || !p.v || !p.theta) {
-            err.push_back("bc.inflow_file: the table needs at least two rows of z u v theta [k]");
-        } else {
-            for (int r = 0; r < p.n; ++r) {
-                const double vals[5] = {p.z[r], p.u[r], p.v[r], p.theta[r], p.k ? p.k[r] : 0.0};
-                bool fin = true;
-                for (double x : vals) { fin = fin && std::isfinite(x); }
-                const std::string row = std::to_string(r + 1);
-                if (!fin) {
-                    err.push_back("bc.inflow_file: row " + row + " holds a value that is not finite");
-                } else if (r == 0 && p.z[0] != 0) {
-                    err.push_back("bc.inflow_file: the first height must be 0 (the ground)");
-                } else if (r > 0 && !(p.z[r] > p.z[r - 1])) {
-                    err.push_back("bc.inflow_file: heights must increase strictly (row " + row + ")");
-                } else if (!(p.theta[r] > 0)) {
-                    err.push_back("bc.inflow_file: theta must be > 0 (row " + row + ")");
-                } else if (p.k && p.k[r] < 0) {
-                    err.push_back("bc.inflow_file: k must be >= 0 (row " + row + ")");
-                }
-                if (!err.empty()) { break; }
-            }
-        }
-    } else {
-        err.push_back("bc.inflow_profile: must be log_law, power_law or a table");
-    }
-    if (!std::isfinite(p.direction)) {
-        err.push_back("bc.inflow_direction = " + num(p.direction) + ": must be finite (degrees)");
-    }
-    if (!finite_pos(p.Cmu0)) { err.push_back("bc.inflow_Cmu0 = " + num(p.Cmu0) + ": must be finite and > 0"); }
-    if (!finite_pos(p.kappa)) { err.push_back("bc.inflow_kappa = " + num(p.kappa) + ": must be finite and > 0"); }
-    if (!p.unbounded && !finite_pos(p.l_max)) {
-        err.push_back("turb.max_geom_lscale = " + num(p.l_max) +
-                      ": must be finite and > 0 (the length cap of bc.inflow_tke_length = closure)");
-    }
-    if (p.n_theta_ref < 1 || p.n_theta_ref > 64) {
-        err.push_back("reference profile: z and theta need the same number of points, at least one (at most 64)");
-    } else {
-        for (int m = 0; m < p.n_theta_ref; ++m) {
-            const double z = p.theta_ref_z[m], t = p.theta_ref[m];
-            if (!std::isfinite(z) || std::isnan(t) || !(t > 0 && std::isfinite(t))) {
-                err.push_back("reference profile point " + std::to_string(m) +
-                              ": z must be finite and theta finite and > 0");
-                break;
-            }
-            if (m > 0 && !(z > p.theta_ref_z[m - 1])) {
-                err.push_back("reference profile: heights must increase strictly (point " + std::to_string(m) + ")");
-                break;
-            }
-        }
-    }
-    return err;
-}
-
 std::vector<std::string> init_problems(const gust_init_options& o)
 {
     std::vector<std::string> err;
@@ -387,54 +251,6 @@ std::vector<std::string> init_problems(const gust_init_options& o)
 void fail_first(const char* op, const std::vector<std::string>& err)
 {
     if (!err.empty()) { GUST_FAIL("gust: %s: %s", op, err.front().c_str()); }
-}
-
-// The profile's view with its tables copied to the device (the stream is
-// synchronised, so the host arrays may go at once).
-ProfileView make_view(hipStream_t st, const gust_inflow_profile& p)
-{
-    ProfileView v{};
-    v.kind = p.kind;
-    v.ustar = p.ustar;
-    v.z0 = p.z0;
-    v.speed_ref = p.speed_ref;
-    v.z_ref = p.z_ref;
-    v.alpha = p.alpha;
-    const double a = p.direction * 3.14159265358979323846 / 180;
-    v.cosd = std::cos(a);
-    v.sind = std::sin(a);
-    v.Cmu0 = p.Cmu0;
-    v.kappa = p.kappa;
-    v.inv_lmax = p.unbounded ? 0.0 : 1.0 / p.l_max;
-    const int n = p.kind == GUST_INFLOW_TABLE ? p.n : 0;
-    const int rn = p.n_theta_ref;
-    const size_t total = size_t(5) * n + size_t(2) * rn;
-    std::vector<double> h(total, 0.0);
-    for (int r = 0; r < n; ++r) {
-        h[r] = p.z[r];
-        h[n + r] = p.u[r];
-        h[2 * n + r] = p.v[r];
-        h[3 * n + r] = p.theta[r];
-        h[4 * n + r] = p.k ? p.k[r] : 0.0;
-    }
-    for (int m = 0; m < rn; ++m) {
-        h[5 * n + m] = p.theta_ref_z[m];
-        h[5 * n + rn + m] = p.theta_ref[m];
-    }
-    double* d = static_cast<double*>(Workspace::get(st, kSlotTable, total * sizeof(double)));
-    GUST_HIP(hipMemcpyAsync(d, h.data(), total * sizeof(double), hipMemcpyHostToDevice, st));
-    GUST_HIP(hipStreamSynchronize(st));
-    v.tz = d;
-    v.tu = d + n;
-    v.tv = d + 2 * n;
-    v.tt = d + 3 * n;
-    v.tk = d + 4 * n;
-    v.n = n;
-    v.has_k = (n > 0 && p.k) ? 1 : 0;
-    v.rz = d + 5 * n;
-    v.rt = d + 5 * n + rn;
-    v.rn = rn;
-    return v;
 }
 
 template <int T>
@@ -499,7 +315,7 @@ double first_guess_impl(hipStream_t st, const gust_geometry& g, const gust_mesh&
                         const gust_init_options& opt, gust_state& q)
 {
     if (opt.first_guess == 0) { return 0.0; }
-    const ProfileView pv = make_view(st, prof);
+    const ProfileView pv = profile::device_view(st, prof);
     const gust_array4* z = mesh.z_nd;
     // 1. u, v from the profile at every point; w = 0 but on the ground faces,
     // which carry no flux (Omega_0 = 0 there).
@@ -545,7 +361,7 @@ void check_common(const char* op, const gust_geometry* geom, const gust_mesh* me
     check_geom(op, geom);
     if (!prof || !opt) { GUST_FAIL("gust: %s: null profile or options", op); }
     fail_first(op, init_problems(*opt));
-    fail_first(op, profile_problems(*prof));
+    fail_first(op, profile::problems(*prof));
     check_mesh(op, *geom, mesh);
     check_state(op, *geom, q);
 }
@@ -584,7 +400,7 @@ int gust_check_inflow_profile(const gust_inflow_profile* p)
 {
     return guarded([&] {
         if (!p) { GUST_FAIL("gust: InflowProfile: null profile"); }
-        fail_first("InflowProfile", profile_problems(*p));
+        fail_first("InflowProfile", profile::problems(*p));
     });
 }
 
@@ -594,11 +410,11 @@ int gust_inflow_eval(const gust_inflow_profile* p, int32_t n, const double* z, d
     return guarded([&] {
         const char* op = "InflowProfile::eval";
         if (!p) { GUST_FAIL("gust: %s: null profile", op); }
-        fail_first(op, profile_problems(*p));
+        fail_first(op, profile::problems(*p));
         if (n < 0 || (n > 0 && !z)) { GUST_FAIL("gust: %s: n >= 0 heights needed", op); }
         if (n == 0) { return; }
         auto st = static_cast<hipStream_t>(stream);
-        const ProfileView pv = make_view(st, *p);
+        const ProfileView pv = profile::device_view(st, *p);
         eval_kernel<<<blocks_for(n), kThreads, 0, st>>>(pv, n, z, u, v, theta, k, nut);
         GUST_HIP(hipGetLastError());
     });

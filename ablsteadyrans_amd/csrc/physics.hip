@@ -25,6 +25,7 @@
 // closure's k^1.5), which differ from the host's in the last bit.
 #include "cgrid.hpp"
 #include "heights.hpp"
+#include "profile.hpp"
 #include "stencil.hpp"
 
 #include <cfloat>
@@ -271,17 +272,16 @@ __global__ __launch_bounds__(kThreads) void surface_kernel(Box3 cols, CV u, CV v
 enum : int { R_EVEN = 0, R_ODD, R_VALUE, R_PROFILE, R_GRADIENT };
 enum : int { F_U = 0, F_V, F_W, F_P, F_THETA, F_TKE };
 
-int rule_for(int f, int t, int lo)
+int rule_for(int f, int t, int d, int lo)
 {
-    // d = 2 only (the sides are periodic).
-    const bool normal_vel = (f == F_W);
+    const bool normal_vel = (f == F_U && d == 0) || (f == F_V && d == 1) || (f == F_W && d == 2);
     const bool vel = f == F_U || f == F_V || f == F_W;
     switch (t) {
     case GUST_FACE_INFLOW: return f == F_P ? R_EVEN : R_PROFILE;
     case GUST_FACE_OUTFLOW: return f == F_P ? R_ODD : R_EVEN;
     case GUST_FACE_SLIP_WALL:
         if (normal_vel) { return R_VALUE; }
-        return (f == F_THETA && !lo) ? R_GRADIENT : R_EVEN;
+        return (f == F_THETA && d == 2 && !lo) ? R_GRADIENT : R_EVEN;
     case GUST_FACE_NO_SLIP_WALL: return vel ? R_VALUE : R_EVEN;
     case GUST_FACE_SURFACE_LAYER:
         if (normal_vel) { return R_VALUE; }
@@ -328,11 +328,46 @@ __device__ __forceinline__ double ground_gradient(const SSV& ss, const SurfOpt& 
 }
 
 __device__ __forceinline__ int wrap(int i, int lo, int n) { return lo + ((i - lo) % n + n) % n; }
+__device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
+
+// The surface-state column the ground gradient of column (i, j) comes from
+// (BoundaryConditions.cpp:617-647): computed on the ground columns and one
+// ghost column; FillBoundary gives each ghost column whose periodic image is
+// a ground column that image; a column still unset takes the one clamped
+// into the domain along the non-periodic directions.
+struct GgMap {
+    int lo0, hi0, lo1, hi1, per0, per1, nx, ny;
+};
+
+__device__ __forceinline__ void gg_column(const GgMap& m, int i, int j, int& si, int& sj)
+{
+    for (int pass = 0; pass < 2; ++pass) {
+        if (i >= m.lo0 && i <= m.hi0 && j >= m.lo1 && j <= m.hi1) {
+            si = i;
+            sj = j;
+            return;
+        }
+        const int ii = (m.per0 && (i < m.lo0 || i > m.hi0)) ? wrap(i, m.lo0, m.nx) : i;
+        const int jj = (m.per1 && (j < m.lo1 || j > m.hi1)) ? wrap(j, m.lo1, m.ny) : j;
+        if (ii >= m.lo0 && ii <= m.hi0 && jj >= m.lo1 && jj <= m.hi1) {
+            si = ii;
+            sj = jj;
+            return;
+        }
+        const bool computed = i >= m.lo0 - 1 && i <= m.hi0 + 1 && j >= m.lo1 - 1 && j <= m.hi1 + 1;
+        if (computed || pass == 1) {
+            si = i;
+            sj = j;
+            return;
+        }
+        i = m.per0 ? i : clampi(i, m.lo0, m.hi0);
+        j = m.per1 ? j : clampi(j, m.lo1, m.hi1);
+    }
+}
 
 template <int T>
 __global__ __launch_bounds__(kThreads) void fill_z_kernel(Box3 cols, V a, NodeZ z, int edge_lo, int edge_hi,
-                                                         FaceFill lo, FaceFill hi, SSV ss, SurfOpt so, int glo0,
-                                                         int glo1, int nx, int ny)
+                                                         FaceFill lo, FaceFill hi, SSV ss, SurfOpt so, GgMap gm)
 {
     GUST_GRID_LOOP(cols)
     {
@@ -346,8 +381,11 @@ __global__ __launch_bounds__(kThreads) void fill_z_kernel(Box3 cols, V a, NodeZ 
                 double G = f.G;
                 if (f.wall) {
                     const int ib = sc::nodal(T, 0) ? i - 1 : i, jb = sc::nodal(T, 1) ? j - 1 : j;
-                    const double Ga = ground_gradient(ss, so, f.comp, wrap(i, glo0, nx), wrap(j, glo1, ny));
-                    const double Gb = ground_gradient(ss, so, f.comp, wrap(ib, glo0, nx), wrap(jb, glo1, ny));
+                    int ai, aj, bi, bj;
+                    gg_column(gm, i, j, ai, aj);
+                    gg_column(gm, ib, jb, bi, bj);
+                    const double Ga = ground_gradient(ss, so, f.comp, ai, aj);
+                    const double Gb = ground_gradient(ss, so, f.comp, bi, bj);
                     G = 0.5 * (Gb + Ga);
                 }
                 const int c1 = is_lo ? edge + 1 : edge - 1;
@@ -392,19 +430,69 @@ __global__ __launch_bounds__(kThreads) void no_flux_w_kernel(Box3 cols, V w, CV 
     }
 }
 
-// Every cell of the fab outside the lateral valid range takes its periodic
-// image (the images of ghost layers along z included).
-__global__ __launch_bounds__(kThreads) void periodic_kernel(Box3 fab, V a, int vlo0, int vhi0, int vlo1, int vhi1,
-                                                           int nx, int ny)
+// fill_face along x or y (BoundaryConditions.cpp:418-470): the even, odd,
+// value and profile rules on both faces of direction d, one thread per point
+// of the ghost slab's plane (the plane's own index along d is unused).
+struct SideFill {
+    int rule;
+    double V;
+};
+
+template <int T>
+__global__ __launch_bounds__(kThreads) void fill_side_kernel(Box3 plane, V a, int d, int edge_lo, int edge_hi,
+                                                            SideFill flo, SideFill fhi, NodeZ z,
+                                                            profile::ProfileView prof, int f, int klo)
+{
+    GUST_GRID_LOOP(plane)
+    {
+        int i, j, k;
+        plane.at(t, i, j, k);
+        const bool nodal_d = ((T >> d) & 1) != 0;
+        for (int side = 0; side < 2; ++side) {
+            const SideFill& s = side == 0 ? flo : fhi;
+            const bool is_lo = side == 0;
+            const int edge = is_lo ? edge_lo : edge_hi;
+            const int mirror = nodal_d ? 2 * edge : (is_lo ? 2 * edge - 1 : 2 * edge + 1);
+            const double sgn = s.rule == R_ODD ? -1.0 : 1.0;
+            const int first = (nodal_d && (s.rule == R_VALUE || s.rule == R_PROFILE)) ? 0 : 1;
+            for (int layer = first; layer <= 2; ++layer) {
+                const int pd = is_lo ? edge - layer : edge + layer;
+                const int pi = d == 0 ? pd : i, pj = d == 1 ? pd : j;
+                if (s.rule == R_PROFILE) {
+                    double u, v, th, kk, nut;
+                    profile::profile_eval(prof, heights::height_above_ground<T>(z, pi, pj, k, klo), u, v, th, kk,
+                                          nut);
+                    a(pi, pj, k) = f == F_U ? u : f == F_V ? v : f == F_THETA ? th : f == F_TKE ? kk : 0.0;
+                    continue;
+                }
+                if (layer == 0) {
+                    a(pi, pj, k) = s.V;
+                    continue;
+                }
+                const int m = mirror - pd;
+                const int mi = d == 0 ? m : i, mj = d == 1 ? m : j;
+                a(pi, pj, k) = s.rule == R_VALUE ? 2 * s.V - a(mi, mj, k) : sgn * a(mi, mj, k);
+            }
+        }
+    }
+}
+
+// FillBoundary(periodicity) on one tile: a ghost point whose periodic image
+// is a valid point takes its value.
+__global__ __launch_bounds__(kThreads) void fill_boundary_kernel(Box3 fab, V a, int vlo0, int vhi0, int vlo1,
+                                                                int vhi1, int vlo2, int vhi2, int per0, int per1,
+                                                                int nx, int ny)
 {
     GUST_GRID_LOOP(fab)
     {
         int i, j, k;
         fab.at(t, i, j, k);
-        const bool in_i = i >= vlo0 && i <= vhi0, in_j = j >= vlo1 && j <= vhi1;
-        if (in_i && in_j) { continue; }
-        const int si = in_i ? i : wrap(i, vlo0, nx);
-        const int sj = in_j ? j : wrap(j, vlo1, ny);
+        const bool in_i = i >= vlo0 && i <= vhi0, in_j = j >= vlo1 && j <= vhi1, in_k = k >= vlo2 && k <= vhi2;
+        if ((in_i && in_j && in_k) || !in_k) { continue; }
+        int si = i, sj = j;
+        if (per0 && !in_i) { si = i < vlo0 ? i + nx : i - nx; }
+        if (per1 && !in_j) { sj = j < vlo1 ? j + ny : j - ny; }
+        if (si < vlo0 || si > vhi0 || sj < vlo1 || sj > vhi1) { continue; }
         a(i, j, k) = a(si, sj, k);
     }
 }
@@ -520,6 +608,33 @@ __global__ __launch_bounds__(kThreads) void w_rows_kernel(Box3 cols, V r, CV w, 
     }
 }
 
+// R = q - V on the x (d = 0, R_u) or y (d = 1, R_v) faces with a Dirichlet
+// normal velocity: V the inflow profile's at the point's height, else 0
+// (BoundaryConditions.cpp:800-834).
+template <int T>
+__global__ __launch_bounds__(kThreads) void side_rows_kernel(Box3 plane, V r, CV q, int d, int face_lo, int face_hi,
+                                                            int do_lo, int do_hi, int prof_lo, int prof_hi, NodeZ z,
+                                                            profile::ProfileView prof, int klo)
+{
+    GUST_GRID_LOOP(plane)
+    {
+        int i, j, k;
+        plane.at(t, i, j, k);
+        for (int side = 0; side < 2; ++side) {
+            if (side == 0 ? !do_lo : !do_hi) { continue; }
+            const int face = side == 0 ? face_lo : face_hi;
+            const int pi = d == 0 ? face : i, pj = d == 1 ? face : j;
+            double Vd = 0.0;
+            if (side == 0 ? prof_lo : prof_hi) {
+                double u, v, th, kk, nut;
+                profile::profile_eval(prof, heights::height_above_ground<T>(z, pi, pj, k, klo), u, v, th, kk, nut);
+                Vd = d == 0 ? u : v;
+            }
+            r(pi, pj, k) = q(pi, pj, k) - Vd;
+        }
+    }
+}
+
 // The hi face of a periodic nodal axis takes the lo face's value.
 __global__ __launch_bounds__(kThreads) void seam_copy_kernel(Box3 plane, V a, int axis, int n)
 {
@@ -575,46 +690,61 @@ struct Fold {
     double fold;
 };
 
+// The wall model's rows (SurfaceLayer.cpp:494-567) on the ground plane.
 template <int T>
 __global__ __launch_bounds__(kThreads) void wall_fold_kernel(Box3 cols, AV A, SSV ss, CV J, int k0, int eq,
-                                                            int wall, int wall_k_dirichlet, double dzinv, int edge_lo,
-                                                            int edge_hi, Fold flo, Fold fhi)
+                                                            int wall_k_dirichlet, double dzinv)
 {
     GUST_GRID_LOOP(cols)
     {
         int i, j, kk;
         cols.at(t, i, j, kk);
-        if (wall) {
-            A(i, j, k0, GUST_COEF_P) -= A(i, j, k0, GUST_COEF_ZM);
-            A(i, j, k0, GUST_COEF_ZM) = 0;
-            if (eq == 0) {
-                A(i, j, k0, GUST_COEF_P) += 0.5 * (ss(GUST_SS_DRAG, i - 1, j) + ss(GUST_SS_DRAG, i, j)) * dzinv /
-                                            sc::interp<sc::CC, T>(J, i, j, k0);
-            } else if (eq == 1) {
-                A(i, j, k0, GUST_COEF_P) += 0.5 * (ss(GUST_SS_DRAG, i, j - 1) + ss(GUST_SS_DRAG, i, j)) * dzinv /
-                                            sc::interp<sc::CC, T>(J, i, j, k0);
-            } else if (eq == 4) {
-                const double a = sc::dmax(ss(GUST_SS_USTAR, i, j), kWallKUstarMin) / ss(GUST_SS_D0, i, j);
-                if (wall_k_dirichlet) {
-                    for (int n = 0; n < GUST_N_COEFS; ++n) { A(i, j, k0, n) = 0; }
-                    A(i, j, k0, GUST_COEF_P) = a;
-                } else {
-                    A(i, j, k0, GUST_COEF_P) += a;
-                }
+        A(i, j, k0, GUST_COEF_P) -= A(i, j, k0, GUST_COEF_ZM);
+        A(i, j, k0, GUST_COEF_ZM) = 0;
+        if (eq == 0) {
+            A(i, j, k0, GUST_COEF_P) += 0.5 * (ss(GUST_SS_DRAG, i - 1, j) + ss(GUST_SS_DRAG, i, j)) * dzinv /
+                                        sc::interp<sc::CC, T>(J, i, j, k0);
+        } else if (eq == 1) {
+            A(i, j, k0, GUST_COEF_P) += 0.5 * (ss(GUST_SS_DRAG, i, j - 1) + ss(GUST_SS_DRAG, i, j)) * dzinv /
+                                        sc::interp<sc::CC, T>(J, i, j, k0);
+        } else if (eq == 4) {
+            const double a = sc::dmax(ss(GUST_SS_USTAR, i, j), kWallKUstarMin) / ss(GUST_SS_D0, i, j);
+            if (wall_k_dirichlet) {
+                for (int n = 0; n < GUST_N_COEFS; ++n) { A(i, j, k0, n) = 0; }
+                A(i, j, k0, GUST_COEF_P) = a;
+            } else {
+                A(i, j, k0, GUST_COEF_P) += a;
             }
         }
+    }
+}
+
+// apply_to_implicit along one direction (BoundaryConditions.cpp:885-940):
+// the identity row at a Dirichlet face point, else the first row's ghost
+// coupling folded into the diagonal or onto the opposite neighbour; one
+// thread per point of the face plane (its index along d unused).
+__global__ __launch_bounds__(kThreads) void bc_fold_kernel(Box3 plane, AV A, int d, int edge_lo, int edge_hi,
+                                                          Fold flo, Fold fhi)
+{
+    GUST_GRID_LOOP(plane)
+    {
+        int i, j, k;
+        plane.at(t, i, j, k);
+        const int cm = d == 0 ? GUST_COEF_XM : d == 1 ? GUST_COEF_YM : GUST_COEF_ZM;
+        const int cp = d == 0 ? GUST_COEF_XP : d == 1 ? GUST_COEF_YP : GUST_COEF_ZP;
         for (int side = 0; side < 2; ++side) {
             const Fold& f = side == 0 ? flo : fhi;
             const int edge = side == 0 ? edge_lo : edge_hi;
-            const int ghost = side == 0 ? GUST_COEF_ZM : GUST_COEF_ZP, other = side == 0 ? GUST_COEF_ZP : GUST_COEF_ZM;
+            const int pi = d == 0 ? edge : i, pj = d == 1 ? edge : j, pk = d == 2 ? edge : k;
+            const int ghost = side == 0 ? cm : cp, other = side == 0 ? cp : cm;
             if (f.dirichlet_pt) {
-                for (int n = 0; n < GUST_N_COEFS; ++n) { A(i, j, edge, n) = 0; }
-                A(i, j, edge, GUST_COEF_P) = 1;
+                for (int n = 0; n < GUST_N_COEFS; ++n) { A(pi, pj, pk, n) = 0; }
+                A(pi, pj, pk, GUST_COEF_P) = 1;
             } else {
-                const double ag = A(i, j, edge, ghost);
-                A(i, j, edge, ghost) = 0;
-                if (f.to_other) { A(i, j, edge, other) += ag; }
-                else { A(i, j, edge, GUST_COEF_P) += f.fold * ag; }
+                const double ag = A(pi, pj, pk, ghost);
+                A(pi, pj, pk, ghost) = 0;
+                if (f.to_other) { A(pi, pj, pk, other) += ag; }
+                else { A(pi, pj, pk, GUST_COEF_P) += f.fold * ag; }
             }
         }
     }
@@ -739,12 +869,20 @@ void check_physics(const char* op, const gust_geometry& g, const gust_physics* p
         check_field(op, "gamma_tke", p->gamma_tke, g, GUST_LOC_CELL, 1);
     }
     if (p->use_bc) {
-        if (!g.periodic[0] || !g.periodic[1]) {
-            GUST_FAIL("gust: %s: this build's boundary conditions need periodic x and y sides", op);
-        }
-        for (const int f : {0, 1, 3, 4}) {
-            if (p->face[f] != GUST_FACE_PERIODIC) {
-                GUST_FAIL("gust: %s: bc faces xlo, ylo, xhi, yhi must be periodic (the geometry is)", op);
+        const char* keys[6] = {"xlo", "ylo", "zlo", "xhi", "yhi", "zhi"};
+        for (const int o : {0, 1, 3, 4}) {
+            const int t = p->face[o];
+            const bool per = g.periodic[o % 3] != 0;
+            if (per != (t == GUST_FACE_PERIODIC)) {
+                GUST_FAIL("gust: %s: bc.%s must be periodic exactly where the geometry is", op, keys[o]);
+            }
+            if (!per && !(t == GUST_FACE_INFLOW || t == GUST_FACE_OUTFLOW || t == GUST_FACE_SLIP_WALL ||
+                          t == GUST_FACE_NO_SLIP_WALL)) {
+                GUST_FAIL("gust: %s: bc.%s must be inflow, outflow, slip_wall or no_slip_wall", op, keys[o]);
+            }
+            if (t == GUST_FACE_INFLOW) {
+                const std::vector<std::string> err = profile::problems(p->inflow);
+                if (!err.empty()) { GUST_FAIL("gust: %s: bc.%s = inflow: %s", op, keys[o], err.front().c_str()); }
             }
         }
         const int zlo = p->face[2], zhi = p->face[5];
@@ -859,6 +997,24 @@ void check_ss(const char* op, const gust_geometry& g, const gust_array4* ss)
     }
 }
 
+IBox view_box(const gust_array4* a)
+{
+    IBox b;
+    for (int d = 0; d < 3; ++d) {
+        b.lo[d] = a->begin[d];
+        b.hi[d] = a->end[d] - 1;
+    }
+    return b;
+}
+
+GgMap gg_map(const gust_geometry& g)
+{
+    return GgMap{g.lo[0], g.hi[0], g.lo[1], g.hi[1], g.periodic[0] ? 1 : 0, g.periodic[1] ? 1 : 0,
+                 g.hi[0] - g.lo[0] + 1, g.hi[1] - g.lo[1] + 1};
+}
+
+// The fills along z over every lateral column of the field's fab (the ghost
+// slab of the z faces spans the lateral ghosts, BoundaryConditions.cpp:83-104).
 template <int T>
 void launch_fill_z(hipStream_t st, const gust_geometry& g, gust_array4* a, int f, const gust_physics& p,
                    const gust_array4* z, const gust_array4* ss)
@@ -868,7 +1024,7 @@ void launch_fill_z(hipStream_t st, const gust_geometry& g, gust_array4* a, int f
         const bool lo = side == 0;
         const int t = lo ? p.face[2] : p.face[5];
         FaceFill& x = ff[side];
-        x.rule = rule_for(f, t, lo ? 1 : 0);
+        x.rule = rule_for(f, t, 2, lo ? 1 : 0);
         x.V = 0;
         if (t == GUST_FACE_GEOSTROPHIC) { x.V = f == F_U ? p.forcing.ug : f == F_V ? p.forcing.vg : 0.0; }
         if (t == GUST_FACE_NO_SLIP_WALL && f <= F_W) { x.V = p.wall_velocity[lo ? 2 : 5][f]; }
@@ -876,56 +1032,110 @@ void launch_fill_z(hipStream_t st, const gust_geometry& g, gust_array4* a, int f
         x.wall = (t == GUST_FACE_SURFACE_LAYER) ? 1 : 0;
         x.comp = f == F_U ? 0 : f == F_V ? 1 : 2;
     }
-    IBox cols = loc_box(g, T);
-    cols.hi[2] = cols.lo[2];
+    IBox cols = view_box(a);
+    cols.lo[2] = cols.hi[2] = g.lo[2];
     const Box3 b = box3(cols);
-    const int nx = g.hi[0] - g.lo[0] + 1, ny = g.hi[1] - g.lo[1] + 1;
     const int edge_lo = g.lo[2], edge_hi = sc::nodal(T, 2) ? g.hi[2] + 1 : g.hi[2];
     SSV sv = ss ? ss_view(ss) : SSV{nullptr, 0, 0, 0, 0};
     fill_z_kernel<T><<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<double>(a), node_z(z), edge_lo, edge_hi, ff[0],
-                                                               ff[1], sv, surf_opt(p.surface), g.lo[0], g.lo[1], nx, ny);
+                                                               ff[1], sv, surf_opt(p.surface), gg_map(g));
     GUST_HIP(hipGetLastError());
 }
 
-void launch_periodic(hipStream_t st, const gust_geometry& g, gust_array4* a, int loc)
+// The fills along x (d = 0) or y (d = 1) over the ghost slab's plane: the
+// whole fab along x for d = 1, the domain (or the fab where periodic) along
+// the later directions.
+template <int T>
+void launch_fill_side(hipStream_t st, const gust_geometry& g, gust_array4* a, int f, int d, const gust_physics& p,
+                      const gust_array4* z, const profile::ProfileView& prof)
 {
-    const IBox vb = loc_box(g, loc);
-    const IBox fab = grow(vb, 2, 2, 2);
-    const Box3 b = box3(fab);
-    periodic_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<double>(a), vb.lo[0], vb.hi[0], vb.lo[1],
-                                                               vb.hi[1], g.hi[0] - g.lo[0] + 1, g.hi[1] - g.lo[1] + 1);
+    SideFill sf[2];
+    for (int side = 0; side < 2; ++side) {
+        const bool lo = side == 0;
+        const int o = lo ? d : d + 3;
+        const int t = p.face[o];
+        sf[side].rule = rule_for(f, t, d, lo ? 1 : 0);
+        sf[side].V = 0;
+        if (t == GUST_FACE_NO_SLIP_WALL && f <= F_W) { sf[side].V = p.wall_velocity[o][f]; }
+    }
+    const IBox fab = view_box(a);
+    IBox plane;
+    for (int e = 0; e < 3; ++e) {
+        if (e == d) {
+            plane.lo[e] = plane.hi[e] = g.lo[e];
+        } else if (e < d || (e < 2 && g.periodic[e])) {
+            plane.lo[e] = fab.lo[e];
+            plane.hi[e] = fab.hi[e];
+        } else {
+            plane.lo[e] = g.lo[e];
+            plane.hi[e] = g.hi[e] + (sc::nodal(T, e) ? 1 : 0);
+        }
+    }
+    const int edge_lo = g.lo[d], edge_hi = sc::nodal(T, d) ? g.hi[d] + 1 : g.hi[d];
+    const Box3 b = box3(plane);
+    fill_side_kernel<T><<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<double>(a), d, edge_lo, edge_hi, sf[0],
+                                                                  sf[1], node_z(z), prof, f, g.lo[2]);
     GUST_HIP(hipGetLastError());
 }
 
-// fill_ghosts on validated arguments; ss receives the surface state.
+void launch_fill_boundary(hipStream_t st, const gust_geometry& g, gust_array4* a, int loc)
+{
+    if (!g.periodic[0] && !g.periodic[1]) { return; }
+    const IBox vb = loc_box(g, loc);
+    const Box3 b = box3(view_box(a));
+    fill_boundary_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(
+        b, dview<double>(a), vb.lo[0], vb.hi[0], vb.lo[1], vb.hi[1], vb.lo[2], vb.hi[2], g.periodic[0] ? 1 : 0,
+        g.periodic[1] ? 1 : 0, g.hi[0] - g.lo[0] + 1, g.hi[1] - g.lo[1] + 1);
+    GUST_HIP(hipGetLastError());
+}
+
+// fill_no_flux_w's columns: those whose u, v points and nodes exist
+// (BoundaryConditions.cpp:766-778).
+IBox no_flux_columns(const gust_array4* w, const gust_array4* u, const gust_array4* v, const gust_array4* z, int klo)
+{
+    IBox c;
+    c.lo[0] = std::max({w->begin[0], u->begin[0], v->begin[0], z->begin[0]});
+    c.hi[0] = std::min({w->end[0] - 1, u->end[0] - 2, v->end[0] - 1, z->end[0] - 2});
+    c.lo[1] = std::max({w->begin[1], u->begin[1], v->begin[1], z->begin[1]});
+    c.hi[1] = std::min({w->end[1] - 1, u->end[1] - 1, v->end[1] - 2, z->end[1] - 2});
+    c.lo[2] = c.hi[2] = klo;
+    return c;
+}
+
+// fill_ghosts on validated arguments (BoundaryConditions.cpp:597-666); ss
+// receives the surface state.
 void fill_ghosts_impl(hipStream_t st, const gust_geometry& g, const gust_state& q, const gust_array4* z,
                       const gust_physics& p, gust_array4* ss)
 {
     const int locs[6] = {GUST_LOC_XFACE, GUST_LOC_YFACE, GUST_LOC_ZFACE, GUST_LOC_CELL, GUST_LOC_CELL, GUST_LOC_CELL};
-    for (int n = 0; n < 6; ++n) { launch_periodic(st, g, q.f[n], locs[n]); }
+    for (int n = 0; n < 6; ++n) { launch_fill_boundary(st, g, q.f[n], locs[n]); }
     const bool wall = wall_on(p);
+    bool inflow = false;
+    for (int o = 0; o < 6; ++o) { inflow = inflow || p.face[o] == GUST_FACE_INFLOW; }
+    const profile::ProfileView prof = inflow ? profile::device_view(st, p.inflow) : profile::ProfileView{};
+    for (int d = 0; d < 2; ++d) {
+        if (g.periodic[d]) { continue; }
+        launch_fill_side<sc::XF>(st, g, q.f[0], F_U, d, p, z, prof);
+        launch_fill_side<sc::YF>(st, g, q.f[1], F_V, d, p, z, prof);
+        launch_fill_side<sc::ZF>(st, g, q.f[2], F_W, d, p, z, prof);
+        for (int n = 3; n < 6; ++n) { launch_fill_side<sc::CC>(st, g, q.f[n], n, d, p, z, prof); }
+    }
     if (wall) { launch_surface(st, g, q.f[0], q.f[1], z, p.surface, ss); }
     const gust_array4* sv = wall ? ss : nullptr;
     launch_fill_z<sc::XF>(st, g, q.f[0], F_U, p, z, sv);
-    launch_periodic(st, g, q.f[0], GUST_LOC_XFACE);
     launch_fill_z<sc::YF>(st, g, q.f[1], F_V, p, z, sv);
-    launch_periodic(st, g, q.f[1], GUST_LOC_YFACE);
     launch_fill_z<sc::ZF>(st, g, q.f[2], F_W, p, z, sv);
     {
-        IBox cols = loc_box(g, GUST_LOC_CELL);
-        cols.hi[2] = cols.lo[2];
-        const Box3 b = box3(cols);
+        const Box3 b = box3(no_flux_columns(q.f[2], q.f[0], q.f[1], z, g.lo[2]));
         no_flux_w_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(
             b, dview<double>(q.f[2]), dview<const double>(q.f[0]), dview<const double>(q.f[1]), dview<const double>(z),
             g.lo[2], g.hi[2] + 1, dirichlet_normal(p.face[2]) ? 1 : 0, dirichlet_normal(p.face[5]) ? 1 : 0,
             1.0 / g.dx[0], 1.0 / g.dx[1]);
         GUST_HIP(hipGetLastError());
+        launch_fill_boundary(st, g, q.f[2], GUST_LOC_ZFACE);
     }
-    launch_periodic(st, g, q.f[2], GUST_LOC_ZFACE);
-    for (int n = 3; n < 6; ++n) {
-        launch_fill_z<sc::CC>(st, g, q.f[n], n, p, z, sv);
-        launch_periodic(st, g, q.f[n], GUST_LOC_CELL);
-    }
+    for (int n = 3; n < 6; ++n) { launch_fill_z<sc::CC>(st, g, q.f[n], n, p, z, sv); }
+    for (int n = 0; n < 6; ++n) { launch_fill_boundary(st, g, q.f[n], locs[n]); }
 }
 
 Rot rotation(const gust_forcing& f)
@@ -1198,6 +1408,27 @@ int gust_residual(const gust_geometry* geom, const gust_state* q, const gust_mes
                                                                       p.surface.wall_k == 0 ? 1 : 0);
         }
         if (p.use_bc) {
+            bool inflow = false;
+            for (int o = 0; o < 6; ++o) { inflow = inflow || p.face[o] == GUST_FACE_INFLOW; }
+            const profile::ProfileView prof = inflow ? profile::device_view(st, p.inflow) : profile::ProfileView{};
+            for (int d = 0; d < 2; ++d) {
+                if (g.periodic[d]) { continue; }
+                const int tl = p.face[d], th = p.face[d + 3];
+                const int loc = 1 << d;
+                IBox plane = loc_box(g, loc);
+                plane.hi[d] = plane.lo[d];
+                const Box3 b = box3(plane);
+                auto launch = [&](auto tc) {
+                    constexpr int T = decltype(tc)::value;
+                    side_rows_kernel<T><<<blocks_for(b.npts()), kThreads, 0, st>>>(
+                        b, dview<double>(Rw[d]), dview<const double>(q->f[d]), d, g.lo[d], g.hi[d] + 1,
+                        dirichlet_normal(tl) ? 1 : 0, dirichlet_normal(th) ? 1 : 0, tl == GUST_FACE_INFLOW ? 1 : 0,
+                        th == GUST_FACE_INFLOW ? 1 : 0, node_z(mesh->z_nd), prof, g.lo[2]);
+                };
+                if (d == 0) { launch(std::integral_constant<int, sc::XF>{}); }
+                else { launch(std::integral_constant<int, sc::YF>{}); }
+                GUST_HIP(hipGetLastError());
+            }
             w_rows_kernel<<<blocks_for(bg.npts()), kThreads, 0, st>>>(
                 bg, dview<double>(Rw[2]), dview<const double>(q->f[2]), dview<const double>(q->f[0]),
                 dview<const double>(q->f[1]), dview<const double>(mesh->z_nd), g.lo[2], g.hi[2] + 1,
@@ -1259,37 +1490,47 @@ int gust_implicit_rows(int32_t eq, const gust_geometry* geom, const gust_faces* 
         }
         if (!p.use_bc && !wall) { return; }
         const int f = e == 0 ? F_U : e == 1 ? F_V : e == 2 ? F_W : e == 3 ? F_THETA : F_TKE;
-        Fold fl[2];
-        const bool nodal_z = (loc & 4) != 0;
-        for (int side = 0; side < 2; ++side) {
-            const bool lo = side == 0;
-            const int t = p.use_bc ? (lo ? p.face[2] : p.face[5]) : GUST_FACE_PERIODIC;
-            const int rule = rule_for(f, t, lo ? 1 : 0);
-            Fold& x = fl[side];
-            x.dirichlet_pt = nodal_z && (rule == R_VALUE || rule == R_PROFILE);
-            x.to_other = 0;
-            x.fold = 0;
-            if (nodal_z) { x.to_other = 1; }
-            else if (rule == R_EVEN) { x.fold = -1; }
-            else if (rule == R_ODD || rule == R_VALUE) { x.fold = 1; }
-            else if (rule == R_GRADIENT) { x.to_other = 1; }
+        const IBox vb = loc_box(g, loc);
+        if (wall) {
+            IBox cols = vb;
+            cols.hi[2] = cols.lo[2];
+            const Box3 b = box3(cols);
+            auto launch = [&](auto tc) {
+                constexpr int T = decltype(tc)::value;
+                wall_fold_kernel<T><<<blocks_for(b.npts()), kThreads, 0, st>>>(
+                    b, a_view(A), ss_view(ss), dview<const double>(mesh->detJ), g.lo[2], e,
+                    p.surface.wall_k == 0 ? 1 : 0, 1.0 / g.dx[2]);
+            };
+            if (loc == 1) { launch(std::integral_constant<int, 1>{}); }
+            else if (loc == 2) { launch(std::integral_constant<int, 2>{}); }
+            else if (loc == 4) { launch(std::integral_constant<int, 4>{}); }
+            else { launch(std::integral_constant<int, 0>{}); }
+            GUST_HIP(hipGetLastError());
         }
-        IBox cols = loc_box(g, loc);
-        cols.hi[2] = cols.lo[2];
-        const Box3 b = box3(cols);
-        const SSV sv = wall ? ss_view(ss) : SSV{nullptr, 0, 0, 0, 0};
-        const int edge_hi = nodal_z ? g.hi[2] + 1 : g.hi[2];
-        auto launch = [&](auto tc) {
-            constexpr int T = decltype(tc)::value;
-            wall_fold_kernel<T><<<blocks_for(b.npts()), kThreads, 0, st>>>(
-                b, a_view(A), sv, dview<const double>(mesh->detJ), g.lo[2], e, wall ? 1 : 0,
-                p.surface.wall_k == 0 ? 1 : 0, 1.0 / g.dx[2], g.lo[2], edge_hi, fl[0], fl[1]);
-        };
-        if (loc == 1) { launch(std::integral_constant<int, 1>{}); }
-        else if (loc == 2) { launch(std::integral_constant<int, 2>{}); }
-        else if (loc == 4) { launch(std::integral_constant<int, 4>{}); }
-        else { launch(std::integral_constant<int, 0>{}); }
-        GUST_HIP(hipGetLastError());
+        if (!p.use_bc) { return; }
+        for (int d = 0; d < 3; ++d) {
+            if (d < 2 && g.periodic[d]) { continue; }
+            const bool nodal_d = ((loc >> d) & 1) != 0;
+            Fold fl[2];
+            for (int side = 0; side < 2; ++side) {
+                const bool lo = side == 0;
+                const int rule = rule_for(f, p.face[lo ? d : d + 3], d, lo ? 1 : 0);
+                Fold& x = fl[side];
+                x.dirichlet_pt = nodal_d && (rule == R_VALUE || rule == R_PROFILE);
+                x.to_other = 0;
+                x.fold = 0;
+                if (nodal_d) { x.to_other = 1; }
+                else if (rule == R_EVEN) { x.fold = -1; }
+                else if (rule == R_ODD || rule == R_VALUE) { x.fold = 1; }
+                else if (rule == R_GRADIENT) { x.to_other = 1; }
+            }
+            IBox plane = vb;
+            plane.hi[d] = plane.lo[d];
+            const Box3 b = box3(plane);
+            bc_fold_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(b, a_view(A), d, vb.lo[d], vb.hi[d], fl[0],
+                                                                     fl[1]);
+            GUST_HIP(hipGetLastError());
+        }
     });
 }
 

@@ -299,19 +299,30 @@ int gust_simplec_create(const gust_geometry* geom, const gust_mesh* mesh, const 
         s->Ak.alloc(cells, GUST_N_COEFS);
         s->conv0.alloc(cells, 1);
         s->conv1.alloc(cells, 1);
-        // The Dirichlet rows: w's face points on the ground and the top
-        // (BoundaryConditions::dirichlet_rows, Source/BoundaryConditions.cpp:852-876).
-        {
-            const IBox wb = loc_box(g, GUST_LOC_ZFACE);
-            const int64_t plane = int64_t(wb.len(0)) * wb.len(1);
-            std::vector<double> ones(size_t(plane), 1.0);
-            double* base = s->mask[2].p;
-            if (dirichlet_normal(phys->face[2])) {
-                GUST_HIP(hipMemcpy(base, ones.data(), size_t(plane) * sizeof(double), hipMemcpyHostToDevice));
+        // The Dirichlet rows (BoundaryConditions::dirichlet_rows,
+        // Source/BoundaryConditions.cpp:852-876): each velocity's face points
+        // on a face of its own direction with a Dirichlet normal value
+        // (inflow, the walls, the ground, the top).
+        for (int c = 0; c < 3; ++c) {
+            if (c < 2 && g.periodic[c]) { continue; }
+            const IBox fb = loc_box(g, 1 << c);
+            std::vector<double> m(size_t(fb.npts()), 0.0);
+            bool any = false;
+            for (int side = 0; side < 2; ++side) {
+                if (!dirichlet_normal(phys->face[side == 0 ? c : c + 3])) { continue; }
+                any = true;
+                const int edge = side == 0 ? fb.lo[c] : fb.hi[c];
+                for (int k = fb.lo[2]; k <= fb.hi[2]; ++k)
+                    for (int j = fb.lo[1]; j <= fb.hi[1]; ++j)
+                        for (int i = fb.lo[0]; i <= fb.hi[0]; ++i) {
+                            const int pc = c == 0 ? i : c == 1 ? j : k;
+                            if (pc != edge) { continue; }
+                            m[size_t(i - fb.lo[0]) + size_t(fb.len(0)) * (size_t(j - fb.lo[1]) +
+                                                                          size_t(fb.len(1)) * size_t(k - fb.lo[2]))] = 1.0;
+                        }
             }
-            if (dirichlet_normal(phys->face[5])) {
-                GUST_HIP(hipMemcpy(base + plane * (wb.len(2) - 1), ones.data(), size_t(plane) * sizeof(double),
-                                   hipMemcpyHostToDevice));
+            if (any) {
+                GUST_HIP(hipMemcpy(s->mask[c].p, m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice));
             }
         }
         // The Poisson controls with the correction's tolerance; the types of

@@ -82,8 +82,9 @@ class ForcingOptions:
 class ResidualInputs:
     """Source/Residual.H:90-128 on this build's scope. ``closure`` None takes
     the given cell diffusivities (Fabs with one ghost); ``faces`` in
-    amrex::Orientation order (xlo, ylo, zlo, xhi, yhi, zhi), None: no
-    boundary conditions."""
+    amrex::Orientation order (xlo, ylo, zlo, xhi, yhi, zhi), periodic exactly
+    where the geometry is, None: no boundary conditions; ``inflow`` the
+    inflow profile (initializer.InflowProfile) an inflow face takes."""
     closure: ClosureOptions | None = field(default_factory=ClosureOptions)
     nu: Fab | None = None
     gamma_theta: Fab | None = None
@@ -100,6 +101,7 @@ class ResidualInputs:
     theta_limiter: int = VAN_LEER
     theta_eps: float = 1e-5
     log_face_mean: bool = False
+    inflow: object = None   # initializer.InflowProfile: the profile an inflow face takes
 
     def c(self):
         p = _lib.PhysicsC()
@@ -131,6 +133,11 @@ class ResidualInputs:
         p.momentum_limiter, p.theta_limiter = int(self.momentum_limiter), int(self.theta_limiter)
         p.momentum_eps, p.theta_eps = float(self.momentum_eps), float(self.theta_eps)
         p.face_mean = 1 if self.log_face_mean else 0
+        _lib.check(_lib.lib.gust_inflow_default_profile(C.byref(p.inflow)))
+        if self.inflow is not None:
+            ic, keep = self.inflow.c()
+            p.inflow = ic
+            self._keep.append(keep)
         return p
 
 

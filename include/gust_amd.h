@@ -617,13 +617,44 @@ typedef struct gust_forcing {
     double pressure_gradient[3];
 } gust_forcing;
 
+enum { GUST_INFLOW_LOG_LAW = 0, GUST_INFLOW_POWER_LAW = 1, GUST_INFLOW_TABLE = 2 };
+
+/* Source/InflowProfile.H:42-66 (InflowProfileOptions; defaults from
+ * gust_inflow_default_profile) with the file kind's table given in memory
+ * (the caller reads "z u v theta [k]"; or a converged column's table,
+ * InflowProfile.H:81-89): n rows, z strictly increasing from 0, theta > 0,
+ * k >= 0 or k = NULL (k from the table's shear); host arrays, copied by each
+ * call. unbounded: bc.inflow_tke_length = unbounded (else the closure's cap
+ * l_max = turb.max_geom_lscale). The sounding theta_ref(z) (bc.theta_profile_z,
+ * bc.theta_profile): 1 to 64 points, heights increasing. */
+typedef struct gust_inflow_profile {
+    int32_t kind;
+    double ustar, z0;
+    double speed_ref, z_ref, alpha;
+    double direction;     /* degrees from the x axis */
+    double Cmu0, kappa;
+    int32_t unbounded;
+    double l_max;
+    int32_t n;
+    const double* z;
+    const double* u;
+    const double* v;
+    const double* theta;
+    const double* k;
+    int32_t n_theta_ref;
+    double theta_ref_z[64];
+    double theta_ref[64];
+} gust_inflow_profile;
+
 /* The residual's inputs, Source/Residual.H:90-128 (ResidualInputs) on this
  * scope: the closure (use_closure) or given cell diffusivities with one
  * ghost (nu, gamma_theta, gamma_tke); the wall model on a surface_layer
- * ground; the forcing; the boundary faces (use_bc; the x and y faces must be
- * periodic); theta_ref(z) as up to 64 (height, theta) points; the convection
- * limiters (GUST_LIMITER_*) and van Albada eps of momentum and theta; the
- * diffusivities' face mean (0 arithmetic, 1 logarithmic). */
+ * ground; the forcing; the boundary faces (use_bc; GUST_FACE_* in
+ * orientation order, periodic exactly where the geometry is); theta_ref(z)
+ * as up to 64 (height, theta) points; the convection limiters
+ * (GUST_LIMITER_*) and van Albada eps of momentum and theta; the
+ * diffusivities' face mean (0 arithmetic, 1 logarithmic); the inflow profile
+ * an inflow face takes (Source/BoundaryConditions.H:82-90, bc.inflow_*). */
 typedef struct gust_physics {
     int32_t use_closure;
     gust_closure_options closure;
@@ -642,6 +673,7 @@ typedef struct gust_physics {
     int32_t momentum_limiter, theta_limiter;
     double momentum_eps, theta_eps;
     int32_t face_mean;
+    gust_inflow_profile inflow;
 } gust_physics;
 
 /* The six state fields u, v, w (faces), p, theta, k (cells), each with two
@@ -924,35 +956,6 @@ int gust_steady_solve(gust_simplec* s, gust_state* q, const gust_driver_options*
  * Source/Initializer.H, Source/InflowProfile.H: the first guess from an
  * inflow profile by height above the ground, then one anisotropic
  * projection by the terrain Poisson solver. */
-
-enum { GUST_INFLOW_LOG_LAW = 0, GUST_INFLOW_POWER_LAW = 1, GUST_INFLOW_TABLE = 2 };
-
-/* Source/InflowProfile.H:42-66 (InflowProfileOptions; defaults from
- * gust_inflow_default_profile) with the file kind's table given in memory
- * (the caller reads "z u v theta [k]"; or a converged column's table,
- * InflowProfile.H:81-89): n rows, z strictly increasing from 0, theta > 0,
- * k >= 0 or k = NULL (k from the table's shear); host arrays, copied by each
- * call. unbounded: bc.inflow_tke_length = unbounded (else the closure's cap
- * l_max = turb.max_geom_lscale). The sounding theta_ref(z) (bc.theta_profile_z,
- * bc.theta_profile): 1 to 64 points, heights increasing. */
-typedef struct gust_inflow_profile {
-    int32_t kind;
-    double ustar, z0;
-    double speed_ref, z_ref, alpha;
-    double direction;     /* degrees from the x axis */
-    double Cmu0, kappa;
-    int32_t unbounded;
-    double l_max;
-    int32_t n;
-    const double* z;
-    const double* u;
-    const double* v;
-    const double* theta;
-    const double* k;
-    int32_t n_theta_ref;
-    double theta_ref_z[64];
-    double theta_ref[64];
-} gust_inflow_profile;
 
 /* The reference's defaults: log law, u* 0.4, z0 0.1, speed 10 at 10 m with
  * alpha 0.14, direction 0, Cmu0 0.5562, kappa 0.41, the closure's cap at 30 m,

@@ -175,12 +175,25 @@ typedef struct {   /* Source/Forcing.H:40-96 */
     double pgrad[3];
 } oforcing;
 
-typedef struct {   /* the z faces of Source/BoundaryConditions.H:82-100 */
-    int zlo, zhi;
+typedef struct {   /* Source/InflowProfile.H:95-133 (View) */
+    int kind;                     /* 0 log_law, 1 power_law, 2 table */
+    double ustar, z0, speed_ref, z_ref, alpha, cosd, sind, Cmu0, kappa, inv_lmax;
+    const double *tz, *tu, *tv, *tt, *tk;
+    int n, has_k;
+    const double *rz, *rt;        /* the sounding (analytic kinds' theta) */
+    int rn;
+} oprofile;
+
+typedef struct {   /* Source/BoundaryConditions.H:82-100 */
+    int face[6];                  /* OFACE_*, orientation order xlo, ylo, zlo, xhi, yhi, zhi */
     double zhi_theta_grad;
-    double wall_velocity[2][3];   /* zlo, zhi */
+    double wall_velocity[6][3];
     double ug, vg;                /* a geostrophic top's wind */
+    const oprofile* profile;      /* the inflow profile (an inflow face needs it) */
 } obc;
+
+/* Source/InflowProfile.H:149-200: (u, v, theta, k, nu_t) at height z above the ground. */
+void oracle_profile_eval(const oprofile* p, double z, double out[5]);
 
 void oracle_height_above_ground(const oa4* z, const ogeom* g, oa4* dist);
 /* Returns -1 if the wall distance plus z0 is not > 0 somewhere. */
@@ -189,7 +202,7 @@ int oracle_closure_evaluate(const oa4* u, const oa4* v, const oa4* w, const oa4*
                             const ogeom* g, oa4* nu, oa4* gth, oa4* gk, oa4* Ps, oa4* Pb, oa4* eps, oa4* dSdk);
 /* ss: OSS_N components over the ground columns and one lateral ghost column. */
 int oracle_surface_state(const oa4* u, const oa4* v, const oa4* z, const osurface* o, const ogeom* g, oa4* ss);
-/* q: u, v, w, p, theta, k with two ghosts; -2 unless x and y are periodic. */
+/* q: u, v, w, p, theta, k with two ghosts; -3 if an inflow face has no profile. */
 int oracle_fill_ghosts(oa4* const q[6], const oa4* z, const obc* bc, const osurface* so, const ogeom* g, oa4* ss);
 double oracle_coriolis_parameter(int enabled, double latitude, double rotation_period);
 void oracle_fill_theta_ref(int n, const double* zs, const double* ts, const oa4* z, oa4* theta_ref);
@@ -198,7 +211,9 @@ void oracle_forcing(const oa4* u, const oa4* v, const oa4* w, const oa4* theta, 
 void oracle_set_ground_fluxes(const oa4* ss, const ogeom* g, oa4* Gu, oa4* Gv, oa4* Gt, oa4* Gk);
 void oracle_apply_wall_k(const oa4* ss, const oa4* tke, int dirichlet, const ogeom* g, oa4* Rk);
 void oracle_apply_rows(const oa4* u, const oa4* v, const oa4* w, const oa4* z, const obc* bc, const ogeom* g,
-                       oa4* Rw);
+                       oa4* Ru, oa4* Rv, oa4* Rw);
+/* Source/BoundaryConditions.cpp:849-876: 1 at the Dirichlet face points of eq at its location, else 0. */
+void oracle_dirichlet_rows(int eq, const obc* bc, const ogeom* g, oa4* mask);
 /* eq 0..4 = u, v, w, theta, k; dSdk may be null. */
 void oracle_implicit_folds(int eq, const oa4* ss, const oa4* dSdk, const oa4* J, const obc* bc, const osurface* so,
                            const ogeom* g, oa4* A);

@@ -69,6 +69,50 @@ class InitOptions:
     column_theta: bool = False
 
 
+def c_profile(p: InflowProfile):
+    """The C oracle's view of the profile (Source/InflowProfile.cpp:178-205):
+    (OProfile, the arrays it points into)."""
+    import ctypes as C
+    o = orc.OProfile()
+    o.kind = int(p.kind)
+    o.ustar, o.z0, o.speed_ref, o.z_ref, o.alpha = p.ustar, p.z0, p.speed_ref, p.z_ref, p.alpha
+    a = p.direction * 3.14159265358979323846 / 180
+    o.cosd, o.sind = math.cos(a), math.sin(a)
+    o.Cmu0, o.kappa = p.Cmu0, p.kappa
+    o.inv_lmax = 0.0 if p.unbounded else 1.0 / p.l_max
+    keep = []
+
+    def arr(x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        keep.append(x)
+        return x.ctypes.data_as(C.POINTER(C.c_double))
+
+    if p.kind == TABLE:
+        tz, tu, tv, tt, tk = p.table
+        o.tz, o.tu, o.tv, o.tt = arr(tz), arr(tu), arr(tv), arr(tt)
+        o.n = len(tz)
+        o.has_k = 1 if tk is not None else 0
+        o.tk = arr(tk if tk is not None else np.zeros(len(tz)))
+    o.rz, o.rt = arr(p.theta_ref_z), arr(p.theta_ref)
+    o.rn = len(p.theta_ref_z)
+    return o, keep
+
+
+def profile_eval_c(p: InflowProfile, z):
+    """The C oracle's evaluation at the heights z: (u, v, theta, k, nu_t)."""
+    import ctypes as C
+    o, keep = c_profile(p)
+    z = np.asarray(z, dtype=np.float64)
+    out = np.zeros((5,) + z.shape)
+    buf = (C.c_double * 5)()
+    for idx in np.ndindex(z.shape):
+        orc.lib().oracle_profile_eval(C.byref(o), float(z[idx]), buf)
+        for n in range(5):
+            out[(n,) + idx] = buf[n]
+    del keep
+    return tuple(out)
+
+
 def table_interp(x, y, xq):
     """Source/InflowProfile.H:137-147: linear in the table, constant beyond
     its ends."""

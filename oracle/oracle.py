@@ -55,9 +55,18 @@ class OForcing(C.Structure):
                 ("ug", C.c_double), ("vg", C.c_double), ("pgrad", C.c_double * 3)]
 
 
+class OProfile(C.Structure):
+    _fields_ = [("kind", C.c_int), ("ustar", C.c_double), ("z0", C.c_double), ("speed_ref", C.c_double),
+                ("z_ref", C.c_double), ("alpha", C.c_double), ("cosd", C.c_double), ("sind", C.c_double),
+                ("Cmu0", C.c_double), ("kappa", C.c_double), ("inv_lmax", C.c_double),
+                ("tz", C.POINTER(C.c_double)), ("tu", C.POINTER(C.c_double)), ("tv", C.POINTER(C.c_double)),
+                ("tt", C.POINTER(C.c_double)), ("tk", C.POINTER(C.c_double)), ("n", C.c_int), ("has_k", C.c_int),
+                ("rz", C.POINTER(C.c_double)), ("rt", C.POINTER(C.c_double)), ("rn", C.c_int)]
+
+
 class OBc(C.Structure):
-    _fields_ = [("zlo", C.c_int), ("zhi", C.c_int), ("zhi_theta_grad", C.c_double),
-                ("wall_velocity", (C.c_double * 3) * 2), ("ug", C.c_double), ("vg", C.c_double)]
+    _fields_ = [("face", C.c_int * 6), ("zhi_theta_grad", C.c_double), ("wall_velocity", (C.c_double * 3) * 6),
+                ("ug", C.c_double), ("vg", C.c_double), ("profile", C.POINTER(OProfile))]
 
 
 _lib = None
@@ -133,8 +142,12 @@ def lib():
         L.oracle_set_ground_fluxes.restype = None
         L.oracle_apply_wall_k.argtypes = [V, V, C.c_int, P(OGeom), V]
         L.oracle_apply_wall_k.restype = None
-        L.oracle_apply_rows.argtypes = [V, V, V, V, P(OBc), P(OGeom), V]
+        L.oracle_apply_rows.argtypes = [V, V, V, V, P(OBc), P(OGeom), V, V, V]
         L.oracle_apply_rows.restype = None
+        L.oracle_dirichlet_rows.argtypes = [C.c_int, P(OBc), P(OGeom), V]
+        L.oracle_dirichlet_rows.restype = None
+        L.oracle_profile_eval.argtypes = [P(OProfile), D, P(D)]
+        L.oracle_profile_eval.restype = None
         L.oracle_implicit_folds.argtypes = [C.c_int, V, V, V, P(OBc), P(OSurface), P(OGeom), V]
         L.oracle_implicit_folds.restype = None
         for f in ("oracle_omega_from_w", "oracle_w_from_omega", "oracle_transport_fluxes",

@@ -1,6 +1,5 @@
 /*
- * physics.c -- oracle of the residual's physics on a tile with periodic
- * sides: heights above the ground, the k-L closure, the neutral surface
+ * physics.c -- oracle of the residual's physics on a tile: heights above the ground, the k-L closure, the neutral surface
  * layer, the ghost fills of the ground and the top, the momentum forcing,
  * and the wall and boundary rows of the residual and the implicit side.
  *
@@ -21,12 +20,14 @@
  *   Source/SurfaceLayer.cpp:538-567     apply_wall_k_to_implicit
  *   Source/StabilityFunctions.H:37-47   most_phi (Businger-Dyer)
  *   Source/BoundaryConditions.cpp:38-65 rule_for
- *   Source/BoundaryConditions.cpp:418-534, 540-576, 597-666, 756-797
- *                                       fill_face, fill_gradient_z,
+ *   Source/BoundaryConditions.cpp:83-104, 418-534, 540-576, 597-666, 756-797
+ *                                       ghost_slab, fill_face, fill_gradient_z,
  *                                       ground_gradients, fill_ghosts,
- *                                       fill_no_flux_w (z faces; sides periodic)
- *   Source/BoundaryConditions.cpp:800-850, 878-940
- *                                       apply_rows, apply_to_implicit (z faces)
+ *                                       fill_no_flux_w (every face type)
+ *   Source/BoundaryConditions.cpp:800-940
+ *                                       apply_rows, dirichlet_rows,
+ *                                       apply_to_implicit
+ *   Source/InflowProfile.H:137-200      table_interp, View::eval
  *   Source/Heights_K.H:20-68            NodeZ, height_above_ground<T>, point_z<T>
  *   Source/Forcing.cpp:77-170, 285-299  coriolis_parameter, fill_theta_ref,
  *                                       subtract_buoyancy, subtract_coriolis,
@@ -318,7 +319,62 @@ int oracle_surface_state(const oa4* u, const oa4* v, const oa4* z, const osurfac
 
 #define SS(ss, n, i, j) ((ss)->p[((i) - (ss)->begin[0]) + ((j) - (ss)->begin[1]) * (ss)->jstride + (n) * (ss)->nstride])
 
-/* ---- ghost fills of the ground and the top (periodic sides) --------- */
+/* ---- the inflow profile -------------------------------------------------- */
+
+static double table_interp(const double* x, const double* y, int n, double xq)
+{
+    if (n == 1 || xq <= x[0]) { return y[0]; }
+    if (xq >= x[n - 1]) { return y[n - 1]; }
+    int m = 1;
+    while (m < n - 1 && !(xq < x[m])) { ++m; }
+    const double w = (xq - x[m - 1]) / (x[m] - x[m - 1]);
+    return y[m - 1] + w * (y[m] - y[m - 1]);
+}
+
+void oracle_profile_eval(const oprofile* p, double z, double out[5])
+{
+    z = os_max(z, 0.0);
+    const double c = 1.0 / (1.0 + p->kappa * (z + p->z0) * p->inv_lmax);
+    double ustar_l, u, v, th, k;
+    if (p->kind == 2) {
+        u = table_interp(p->tz, p->tu, p->n, z);
+        v = table_interp(p->tz, p->tv, p->n, z);
+        th = table_interp(p->tz, p->tt, p->n, z);
+        int m = 1;
+        while (m < p->n - 1 && !(z < p->tz[m])) { ++m; }
+        const double s0 = sqrt(p->tu[m - 1] * p->tu[m - 1] + p->tv[m - 1] * p->tv[m - 1]);
+        const double s1 = sqrt(p->tu[m] * p->tu[m] + p->tv[m] * p->tv[m]);
+        const double dsdz = (p->n > 1) ? (s1 - s0) / (p->tz[m] - p->tz[m - 1]) : 0.0;
+        ustar_l = p->kappa * (z + p->z0) * os_max(dsdz, 0.0) * c;
+        if (p->has_k) {
+            k = table_interp(p->tz, p->tk, p->n, z);
+            ustar_l = p->Cmu0 * sqrt(os_max(k, 0.0));
+        } else {
+            k = ustar_l * ustar_l / (p->Cmu0 * p->Cmu0);
+        }
+    } else {
+        double S;
+        if (p->kind == 0) {
+            S = p->ustar / p->kappa * log((z + p->z0) / p->z0);
+            ustar_l = p->ustar * c;
+        } else {
+            S = p->speed_ref * pow(z / p->z_ref, p->alpha);
+            const double dsdz = p->alpha * S / (0.5 * (z + p->z0 + fabs(z - p->z0)));
+            ustar_l = p->kappa * (z + p->z0) * dsdz * c;
+        }
+        u = S * p->cosd;
+        v = S * p->sind;
+        th = table_interp(p->rz, p->rt, p->rn, z);
+        k = ustar_l * ustar_l / (p->Cmu0 * p->Cmu0);
+    }
+    out[0] = u;
+    out[1] = v;
+    out[2] = th;
+    out[3] = k;
+    out[4] = p->kappa * ustar_l * (z + p->z0) * c;
+}
+
+/* ---- ghost fills ------------------------------------------------------- */
 
 enum { R_EVEN, R_ODD, R_VALUE, R_PROFILE, R_GRADIENT };
 enum { F_U, F_V, F_W, F_P, F_THETA, F_TKE };
@@ -345,6 +401,21 @@ static int rule_for(int f, int t, int d, int lo)
     }
 }
 
+static int dirichlet_normal(int t)
+{
+    return t == OFACE_INFLOW || t == OFACE_SLIP_WALL || t == OFACE_NO_SLIP_WALL || t == OFACE_SURFACE_LAYER ||
+           t == OFACE_GEOSTROPHIC;
+}
+
+static int face_of(int d, int lo) { return lo ? d : d + 3; }
+
+static double profile_value(const oprofile* p, int f, double h)
+{
+    double o[5];
+    oracle_profile_eval(p, h, o);
+    return f == F_U ? o[0] : f == F_V ? o[1] : f == F_THETA ? o[2] : f == F_TKE ? o[3] : 0.0;
+}
+
 /* The wall law's gradient per column (BoundaryConditions.cpp:540-576). */
 static void ground_gradient(const oa4* ss, const osurface* o, int i, int j, double G[3])
 {
@@ -365,99 +436,199 @@ static void ground_gradient(const oa4* ss, const osurface* o, int i, int j, doub
 }
 
 static int wrap(int i, int lo, int n) { return lo + ((i - lo) % n + n) % n; }
+static int os_imax(int a, int b) { return a > b ? a : b; }
+static int os_imin(int a, int b) { return a < b ? a : b; }
+static int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
 
-/* Periodic images along x and y for every cell of a, its z ghosts included
- * (FillBoundary, then the ghost-copy exchange). */
-static void periodic_fill(oa4* a, int t, const ogeom* g)
+/* The surface-state column the ground gradient of column (i, j) comes from
+ * (BoundaryConditions.cpp:617-647): the gradient is computed on the ground
+ * columns and one ghost column (ground_gradients); FillBoundary then gives
+ * every ghost column whose periodic image is a ground column that image; a
+ * column still unset takes the one clamped into the domain along the
+ * non-periodic directions (itself then a ground column, an image, or a
+ * computed ghost column). */
+static void gg_column(const ogeom* g, int i, int j, int* si, int* sj)
 {
     const int nx = g->hi[0] - g->lo[0] + 1, ny = g->hi[1] - g->lo[1] + 1;
-    const int vx1 = g->hi[0] + os_nodal(t, 0), vy1 = g->hi[1] + os_nodal(t, 1);
+    for (int pass = 0; pass < 2; ++pass) {
+        const int valid = i >= g->lo[0] && i <= g->hi[0] && j >= g->lo[1] && j <= g->hi[1];
+        if (valid) { *si = i; *sj = j; return; }
+        const int ii = (g->periodic[0] && (i < g->lo[0] || i > g->hi[0])) ? wrap(i, g->lo[0], nx) : i;
+        const int jj = (g->periodic[1] && (j < g->lo[1] || j > g->hi[1])) ? wrap(j, g->lo[1], ny) : j;
+        if (ii >= g->lo[0] && ii <= g->hi[0] && jj >= g->lo[1] && jj <= g->hi[1]) { *si = ii; *sj = jj; return; }
+        const int computed = i >= g->lo[0] - 1 && i <= g->hi[0] + 1 && j >= g->lo[1] - 1 && j <= g->hi[1] + 1;
+        if (computed || pass == 1) { *si = i; *sj = j; return; }
+        i = g->periodic[0] ? i : clampi(i, g->lo[0], g->hi[0]);
+        j = g->periodic[1] ? j : clampi(j, g->lo[1], g->hi[1]);
+    }
+}
+
+/* FillBoundary(periodicity) on one box: every ghost point whose periodic
+ * image is a valid point takes its value. */
+static void fill_boundary(oa4* a, int t, const ogeom* g)
+{
+    if (!g->periodic[0] && !g->periodic[1]) { return; }
+    const int nx = g->hi[0] - g->lo[0] + 1, ny = g->hi[1] - g->lo[1] + 1;
+    const int vh[3] = {g->hi[0] + os_nodal(t, 0), g->hi[1] + os_nodal(t, 1), g->hi[2] + os_nodal(t, 2)};
     for (int k = a->begin[2]; k < a->end[2]; ++k)
         for (int j = a->begin[1]; j < a->end[1]; ++j)
             for (int i = a->begin[0]; i < a->end[0]; ++i) {
-                if (i >= g->lo[0] && i <= vx1 && j >= g->lo[1] && j <= vy1) { continue; }
-                const int si = (i >= g->lo[0] && i <= vx1) ? i : wrap(i, g->lo[0], nx);
-                const int sj = (j >= g->lo[1] && j <= vy1) ? j : wrap(j, g->lo[1], ny);
+                const int in_i = i >= g->lo[0] && i <= vh[0], in_j = j >= g->lo[1] && j <= vh[1];
+                const int in_k = k >= g->lo[2] && k <= vh[2];
+                if (in_i && in_j && in_k) { continue; }
+                int si = i, sj = j;
+                if (g->periodic[0] && !in_i) { si = i < g->lo[0] ? i + nx : i - nx; }
+                if (g->periodic[1] && !in_j) { sj = j < g->lo[1] ? j + ny : j - ny; }
+                if (si < g->lo[0] || si > vh[0] || sj < g->lo[1] || sj > vh[1] || !in_k) { continue; }
                 AT(a, i, j, k) = AT(a, si, sj, k);
             }
 }
 
-static void fill_z(oa4* a, int t, int f, const obc* bc, const oa4* z, const oa4* ss, const osurface* so,
-                   const double dxinv[3], const oa4* u, const oa4* v, const ogeom* g)
+/* The ranges of the ghost slab of face (d, lo) along the other directions
+ * (BoundaryConditions.cpp:83-104): the whole fab along directions filled
+ * before (e < d) and periodic ones, the domain along later non-periodic ones. */
+static void slab_ranges(const oa4* a, int t, const ogeom* g, int d, int rlo[3], int rhi[3])
 {
-    const int nodal_z = os_nodal(t, 2);
-    const int xs = g->hi[0] + os_nodal(t, 0), ys = g->hi[1] + os_nodal(t, 1);
-    for (int lo = 1; lo >= 0; --lo) {
-        const int ft = lo ? bc->zlo : bc->zhi;
-        const int rule = rule_for(f, ft, 2, lo);
-        double V = 0;
-        if (ft == OFACE_GEOSTROPHIC) { V = f == F_U ? bc->ug : f == F_V ? bc->vg : 0.0; }
-        if (ft == OFACE_NO_SLIP_WALL && f <= F_W) { V = bc->wall_velocity[lo ? 0 : 1][f]; }
-        const int edge = lo ? g->lo[2] : (nodal_z ? g->hi[2] + 1 : g->hi[2]);
-        for (int j = g->lo[1]; j <= ys; ++j)
-            for (int i = g->lo[0]; i <= xs; ++i) {
-                if (rule == R_GRADIENT) {
-                    double G;
-                    if (ft == OFACE_SURFACE_LAYER) {
-                        const int comp = f == F_U ? 0 : f == F_V ? 1 : 2;
-                        const int ia = i, ja = j;
-                        const int ib = os_nodal(t, 0) ? i - 1 : i, jb = os_nodal(t, 1) ? j - 1 : j;
-                        double Ga[3], Gb[3];
-                        ground_gradient(ss, so, wrap(ia, g->lo[0], g->hi[0] - g->lo[0] + 1),
-                                        wrap(ja, g->lo[1], g->hi[1] - g->lo[1] + 1), Ga);
-                        ground_gradient(ss, so, wrap(ib, g->lo[0], g->hi[0] - g->lo[0] + 1),
-                                        wrap(jb, g->lo[1], g->hi[1] - g->lo[1] + 1), Gb);
-                        G = 0.5 * (Gb[comp] + Ga[comp]);
-                    } else {
-                        G = bc->zhi_theta_grad;
-                    }
-                    const int c1 = lo ? edge + 1 : edge - 1;
-                    const int g1 = lo ? edge - 1 : edge + 1, g2 = lo ? edge - 2 : edge + 2;
-                    const double zc1 = point_z(z, t, i, j, c1), zg1 = point_z(z, t, i, j, g1),
-                                 zg2 = point_z(z, t, i, j, g2);
-                    AT(a, i, j, g1) = AT(a, i, j, c1) + (zg1 - zc1) * G;
-                    AT(a, i, j, g2) = AT(a, i, j, g1) + (zg2 - zg1) * G;
-                    continue;
-                }
-                if (nodal_z && rule == R_VALUE) { AT(a, i, j, edge) = V; }
-                const int mirror = nodal_z ? 2 * edge : (lo ? 2 * edge - 1 : 2 * edge + 1);
-                const int sgn = rule == R_ODD ? -1 : 1;
-                for (int layer = 1; layer <= 2; ++layer) {
-                    const int kg = lo ? edge - layer : edge + layer;
-                    const int m = mirror - kg;
-                    AT(a, i, j, kg) = rule == R_VALUE ? 2 * V - AT(a, i, j, m) : sgn * AT(a, i, j, m);
-                }
-            }
-        if (f == F_W && (ft == OFACE_INFLOW || ft == OFACE_SLIP_WALL || ft == OFACE_NO_SLIP_WALL ||
-                         ft == OFACE_SURFACE_LAYER || ft == OFACE_GEOSTROPHIC)) {
-            /* fill_no_flux_w: the face takes Omega = 0, ghosts mirrored about it */
-            for (int j = g->lo[1]; j <= g->hi[1]; ++j)
-                for (int i = g->lo[0]; i <= g->hi[0]; ++i) {
-                    const double sx = os_deriv(z, OS_NODE, OS_ZF, 0, i, j, edge) * dxinv[0];
-                    const double sy = os_deriv(z, OS_NODE, OS_ZF, 1, i, j, edge) * dxinv[1];
-                    const double Vw = sx * os_interp(u, OS_XF, OS_ZF, i, j, edge) +
-                                      sy * os_interp(v, OS_YF, OS_ZF, i, j, edge);
-                    AT(a, i, j, edge) = Vw;
-                    const int sg = lo ? -1 : 1;
-                    for (int m = 1; m <= 2; ++m) { AT(a, i, j, edge + sg * m) = 2 * Vw - AT(a, i, j, edge - sg * m); }
-                }
+    for (int e = 0; e < 3; ++e) {
+        if (e == d) { rlo[e] = rhi[e] = 0; continue; }
+        if (e < d || (e < 2 && g->periodic[e])) {
+            rlo[e] = a->begin[e];
+            rhi[e] = a->end[e] - 1;
+        } else {
+            rlo[e] = g->lo[e];
+            rhi[e] = g->hi[e] + os_nodal(t, e);
         }
     }
 }
 
+/* fill_face (BoundaryConditions.cpp:418-470) for the even, odd, value and
+ * profile rules. */
+static void fill_face(oa4* a, int t, int f, int d, int lo, int rule, double V, const oprofile* prof, const oa4* z,
+                      const ogeom* g)
+{
+    const int nodal_d = os_nodal(t, d);
+    const int edge = lo ? g->lo[d] : (nodal_d ? g->hi[d] + 1 : g->hi[d]);
+    const int klo = g->lo[2];
+    int rlo[3], rhi[3];
+    slab_ranges(a, t, g, d, rlo, rhi);
+    const int mirror = nodal_d ? 2 * edge : (lo ? 2 * edge - 1 : 2 * edge + 1);
+    const int sgn = rule == R_ODD ? -1 : 1;
+    for (int layer = (nodal_d && (rule == R_VALUE || rule == R_PROFILE)) ? 0 : 1; layer <= 2; ++layer) {
+        const int pd = lo ? edge - layer : edge + layer;
+        int lo3[3] = {rlo[0], rlo[1], rlo[2]}, hi3[3] = {rhi[0], rhi[1], rhi[2]};
+        lo3[d] = hi3[d] = pd;
+        for (int k = lo3[2]; k <= hi3[2]; ++k)
+            for (int j = lo3[1]; j <= hi3[1]; ++j)
+                for (int i = lo3[0]; i <= hi3[0]; ++i) {
+                    if (rule == R_PROFILE) {
+                        AT(a, i, j, k) = profile_value(prof, f, hag(z, t, i, j, k, klo));
+                        continue;
+                    }
+                    if (layer == 0) { AT(a, i, j, k) = V; continue; }
+                    const int p = d == 0 ? i : d == 1 ? j : k;
+                    const int m = mirror - p;
+                    const int mi = d == 0 ? m : i, mj = d == 1 ? m : j, mk = d == 2 ? m : k;
+                    AT(a, i, j, k) = rule == R_VALUE ? 2 * V - AT(a, mi, mj, mk) : sgn * AT(a, mi, mj, mk);
+                }
+    }
+}
+
+/* fill_gradient_z (BoundaryConditions.cpp:472-514): the ground (wall law,
+ * per column) or the top (a constant). */
+static void fill_gradient_z(oa4* a, int t, int f, int lo, int wall, double Gc, const oa4* z, const oa4* ss,
+                            const osurface* so, const ogeom* g)
+{
+    const int edge = lo ? g->lo[2] : g->hi[2];
+    const int c1 = lo ? edge + 1 : edge - 1;
+    const int g1 = lo ? edge - 1 : edge + 1, g2 = lo ? edge - 2 : edge + 2;
+    const int comp = f == F_U ? 0 : f == F_V ? 1 : 2;
+    for (int j = a->begin[1]; j < a->end[1]; ++j)
+        for (int i = a->begin[0]; i < a->end[0]; ++i) {
+            double G = Gc;
+            if (wall) {
+                const int ib = os_nodal(t, 0) ? i - 1 : i, jb = os_nodal(t, 1) ? j - 1 : j;
+                int ai, aj, bi, bj;
+                gg_column(g, i, j, &ai, &aj);
+                gg_column(g, ib, jb, &bi, &bj);
+                double Ga[3], Gb[3];
+                ground_gradient(ss, so, ai, aj, Ga);
+                ground_gradient(ss, so, bi, bj, Gb);
+                G = 0.5 * (Gb[comp] + Ga[comp]);
+            }
+            const double zc1 = point_z(z, t, i, j, c1), zg1 = point_z(z, t, i, j, g1), zg2 = point_z(z, t, i, j, g2);
+            AT(a, i, j, g1) = AT(a, i, j, c1) + (zg1 - zc1) * G;
+            AT(a, i, j, g2) = AT(a, i, j, g1) + (zg2 - zg1) * G;
+        }
+}
+
+static void fill_field(oa4* a, int t, int f, int d, const obc* bc, const oa4* z, const oa4* ss, const osurface* so,
+                       const ogeom* g)
+{
+    for (int lo = 1; lo >= 0; --lo) {
+        const int ft = bc->face[face_of(d, lo)];
+        const int rule = rule_for(f, ft, d, lo);
+        double V = 0;
+        if (ft == OFACE_GEOSTROPHIC) { V = f == F_U ? bc->ug : f == F_V ? bc->vg : 0.0; }
+        if (ft == OFACE_NO_SLIP_WALL && f <= F_W) { V = bc->wall_velocity[face_of(d, lo)][f]; }
+        if (rule == R_GRADIENT) {
+            if (ft == OFACE_SURFACE_LAYER) {
+                fill_gradient_z(a, t, f, lo, 1, 0.0, z, ss, so, g);
+            } else {
+                fill_gradient_z(a, t, f, lo, 0, bc->zhi_theta_grad, z, ss, so, g);
+            }
+        } else {
+            fill_face(a, t, f, d, lo, rule, V, bc->profile, z, g);
+        }
+    }
+}
+
+/* fill_no_flux_w (BoundaryConditions.cpp:756-797): the ground and top faces
+ * of w take Omega = 0, w = z_x u_f + z_y v_f, ghosts mirrored about it, on
+ * the columns whose u, v points and nodes exist. */
+static void fill_no_flux_w(oa4* w, const oa4* u, const oa4* v, const oa4* z, const obc* bc, const ogeom* g)
+{
+    const double dxinv[3] = {1.0 / g->dx[0], 1.0 / g->dx[1], 1.0 / g->dx[2]};
+    int i0 = w->begin[0], i1 = w->end[0] - 1, j0 = w->begin[1], j1 = w->end[1] - 1;
+    i0 = os_imax(i0, os_imax(u->begin[0], os_imax(v->begin[0], z->begin[0])));
+    i1 = os_imin(i1, os_imin(u->end[0] - 2, os_imin(v->end[0] - 1, z->end[0] - 2)));
+    j0 = os_imax(j0, os_imax(u->begin[1], os_imax(v->begin[1], z->begin[1])));
+    j1 = os_imin(j1, os_imin(u->end[1] - 1, os_imin(v->end[1] - 2, z->end[1] - 2)));
+    for (int lo = 1; lo >= 0; --lo) {
+        if (!dirichlet_normal(bc->face[face_of(2, lo)])) { continue; }
+        const int edge = lo ? g->lo[2] : g->hi[2] + 1;
+        for (int j = j0; j <= j1; ++j)
+            for (int i = i0; i <= i1; ++i) {
+                const double sx = os_deriv(z, OS_NODE, OS_ZF, 0, i, j, edge) * dxinv[0];
+                const double sy = os_deriv(z, OS_NODE, OS_ZF, 1, i, j, edge) * dxinv[1];
+                const double Vw = sx * os_interp(u, OS_XF, OS_ZF, i, j, edge) + sy * os_interp(v, OS_YF, OS_ZF, i, j, edge);
+                AT(w, i, j, edge) = Vw;
+                const int sg = lo ? -1 : 1;
+                for (int m = 1; m <= 2; ++m) { AT(w, i, j, edge + sg * m) = 2 * Vw - AT(w, i, j, edge - sg * m); }
+            }
+    }
+    fill_boundary(w, OS_ZF, g);
+}
+
 int oracle_fill_ghosts(oa4* const q[6], const oa4* z, const obc* bc, const osurface* so, const ogeom* g, oa4* ss)
 {
-    if (!g->periodic[0] || !g->periodic[1]) { return -2; }
     const int t[6] = {OS_XF, OS_YF, OS_ZF, OS_CC, OS_CC, OS_CC};
-    for (int n = 0; n < 6; ++n) { periodic_fill(q[n], t[n], g); }
-    if (bc->zlo == OFACE_SURFACE_LAYER) {
-        if (oracle_surface_state(q[0], q[1], z, so, g, ss) != 0) { return -1; }
+    for (int o = 0; o < 6; ++o) {
+        if (bc->face[o] == OFACE_INFLOW && !bc->profile) { return -3; }
     }
-    const double dxinv[3] = {1.0 / g->dx[0], 1.0 / g->dx[1], 1.0 / g->dx[2]};
-    for (int n = 0; n < 6; ++n) {
-        fill_z(q[n], t[n], n, bc, z, ss, so, dxinv, q[0], q[1], g);
-        periodic_fill(q[n], t[n], g);
+    for (int n = 0; n < 6; ++n) { fill_boundary(q[n], t[n], g); }
+    const int wall = bc->face[2] == OFACE_SURFACE_LAYER;
+    for (int d = 0; d < 3; ++d) {
+        if (d == 2 && wall) {
+            if (oracle_surface_state(q[0], q[1], z, so, g, ss) != 0) { return -1; }
+        }
+        if (d < 2 && g->periodic[d]) { continue; }
+        fill_field(q[0], t[0], F_U, d, bc, z, ss, so, g);
+        fill_field(q[1], t[1], F_V, d, bc, z, ss, so, g);
+        fill_field(q[2], t[2], F_W, d, bc, z, ss, so, g);
+        if (d == 2) { fill_no_flux_w(q[2], q[0], q[1], z, bc, g); }
+        for (int n = 3; n < 6; ++n) { fill_field(q[n], t[n], n, d, bc, z, ss, so, g); }
     }
+    for (int n = 0; n < 6; ++n) { fill_boundary(q[n], t[n], g); }
     return 0;
 }
 
@@ -575,23 +746,59 @@ void oracle_apply_wall_k(const oa4* ss, const oa4* tke, int dirichlet, const oge
 }
 
 void oracle_apply_rows(const oa4* u, const oa4* v, const oa4* w, const oa4* z, const obc* bc, const ogeom* g,
-                       oa4* Rw)
+                       oa4* Ru, oa4* Rv, oa4* Rw)
 {
     const double dxinv[3] = {1.0 / g->dx[0], 1.0 / g->dx[1], 1.0 / g->dx[2]};
-    for (int lo = 1; lo >= 0; --lo) {
-        const int ft = lo ? bc->zlo : bc->zhi;
-        if (!(ft == OFACE_INFLOW || ft == OFACE_SLIP_WALL || ft == OFACE_NO_SLIP_WALL || ft == OFACE_SURFACE_LAYER ||
-              ft == OFACE_GEOSTROPHIC)) {
-            continue;
+    const int klo = g->lo[2];
+    const oa4* qf[3] = {u, v, w};
+    oa4* rf[3] = {Ru, Rv, Rw};
+    for (int d = 0; d < 3; ++d) {
+        if (d < 2 && g->periodic[d]) { continue; }
+        const int t = d == 0 ? OS_XF : d == 1 ? OS_YF : OS_ZF;
+        for (int lo = 1; lo >= 0; --lo) {
+            const int ft = bc->face[face_of(d, lo)];
+            if (!dirichlet_normal(ft)) { continue; }
+            const int face = lo ? g->lo[d] : g->hi[d] + 1;
+            int lo3[3] = {g->lo[0], g->lo[1], g->lo[2]};
+            int hi3[3] = {g->hi[0] + os_nodal(t, 0), g->hi[1] + os_nodal(t, 1), g->hi[2] + os_nodal(t, 2)};
+            lo3[d] = hi3[d] = face;
+            for (int k = lo3[2]; k <= hi3[2]; ++k)
+                for (int j = lo3[1]; j <= hi3[1]; ++j)
+                    for (int i = lo3[0]; i <= hi3[0]; ++i) {
+                        double V = 0;
+                        if (d < 2) {
+                            if (ft == OFACE_INFLOW) { V = profile_value(bc->profile, d, hag(z, t, i, j, k, klo)); }
+                        } else {
+                            const double sx = os_deriv(z, OS_NODE, OS_ZF, 0, i, j, face) * dxinv[0];
+                            const double sy = os_deriv(z, OS_NODE, OS_ZF, 1, i, j, face) * dxinv[1];
+                            V = sx * os_interp(u, OS_XF, OS_ZF, i, j, face) + sy * os_interp(v, OS_YF, OS_ZF, i, j, face);
+                        }
+                        AT(rf[d], i, j, k) = AT(qf[d], i, j, k) - V;
+                    }
         }
-        const int face = lo ? g->lo[2] : g->hi[2] + 1;
-        for (int j = g->lo[1]; j <= g->hi[1]; ++j)
-            for (int i = g->lo[0]; i <= g->hi[0]; ++i) {
-                const double sx = os_deriv(z, OS_NODE, OS_ZF, 0, i, j, face) * dxinv[0];
-                const double sy = os_deriv(z, OS_NODE, OS_ZF, 1, i, j, face) * dxinv[1];
-                const double V = sx * os_interp(u, OS_XF, OS_ZF, i, j, face) + sy * os_interp(v, OS_YF, OS_ZF, i, j, face);
-                AT(Rw, i, j, face) = AT(w, i, j, face) - V;
-            }
+    }
+}
+
+void oracle_dirichlet_rows(int eq, const obc* bc, const ogeom* g, oa4* mask)
+{
+    const int t = eq == 0 ? OS_XF : eq == 1 ? OS_YF : eq == 2 ? OS_ZF : OS_CC;
+    const int f = eq == 0 ? F_U : eq == 1 ? F_V : eq == 2 ? F_W : eq == 3 ? F_THETA : F_TKE;
+    const int vh[3] = {g->hi[0] + os_nodal(t, 0), g->hi[1] + os_nodal(t, 1), g->hi[2] + os_nodal(t, 2)};
+    for (int k = g->lo[2]; k <= vh[2]; ++k)
+        for (int j = g->lo[1]; j <= vh[1]; ++j)
+            for (int i = g->lo[0]; i <= vh[0]; ++i) { AT(mask, i, j, k) = 0; }
+    for (int d = 0; d < 3; ++d) {
+        if (d < 2 && g->periodic[d]) { continue; }
+        for (int lo = 1; lo >= 0; --lo) {
+            const int rule = rule_for(f, bc->face[face_of(d, lo)], d, lo);
+            if (!(os_nodal(t, d) && (rule == R_VALUE || rule == R_PROFILE))) { continue; }
+            const int edge = lo ? g->lo[d] : g->hi[d] + 1;
+            int lo3[3] = {g->lo[0], g->lo[1], g->lo[2]}, hi3[3] = {vh[0], vh[1], vh[2]};
+            lo3[d] = hi3[d] = edge;
+            for (int k = lo3[2]; k <= hi3[2]; ++k)
+                for (int j = lo3[1]; j <= hi3[1]; ++j)
+                    for (int i = lo3[0]; i <= hi3[0]; ++i) { AT(mask, i, j, k) = 1; }
+        }
     }
 }
 
@@ -610,7 +817,7 @@ void oracle_implicit_folds(int eq, const oa4* ss, const oa4* dSdk, const oa4* J,
             for (int j = g->lo[1]; j <= g->hi[1]; ++j)
                 for (int i = g->lo[0]; i <= g->hi[0]; ++i) { AC(i, j, k, OC_P) += AT(dSdk, i, j, k); }
     }
-    if (bc->zlo == OFACE_SURFACE_LAYER && eq != 2) {
+    if (bc->face[2] == OFACE_SURFACE_LAYER && eq != 2) {
         const double dzinv = 1.0 / g->dx[2];
         for (int j = g->lo[1]; j <= ys; ++j)
             for (int i = g->lo[0]; i <= xs; ++i) {
@@ -638,32 +845,42 @@ void oracle_implicit_folds(int eq, const oa4* ss, const oa4* dSdk, const oa4* J,
                 }
         }
     }
-    /* apply_to_implicit along z (the sides are periodic) */
-    for (int lo = 1; lo >= 0; --lo) {
-        const int ft = lo ? bc->zlo : bc->zhi;
-        const int rule = rule_for(f, ft, 2, lo);
-        const int nodal_d = os_nodal(t, 2);
-        const int dirichlet_pt = nodal_d && (rule == R_VALUE || rule == R_PROFILE);
-        const int edge = lo ? g->lo[2] : (nodal_d ? g->hi[2] + 1 : g->hi[2]);
-        const int ghost = lo ? OC_ZM : OC_ZP, other = lo ? OC_ZP : OC_ZM;
-        double fold = 0;
-        int to_other = 0;
-        if (nodal_d) { to_other = 1; }
-        else if (rule == R_EVEN) { fold = -1; }
-        else if (rule == R_ODD || rule == R_VALUE) { fold = 1; }
-        else if (rule == R_GRADIENT) { to_other = 1; }
-        for (int j = g->lo[1]; j <= ys; ++j)
-            for (int i = g->lo[0]; i <= xs; ++i) {
-                if (dirichlet_pt) {
-                    for (int n = 0; n < OC_N; ++n) { AC(i, j, edge, n) = 0; }
-                    AC(i, j, edge, OC_P) = 1;
-                } else {
-                    const double ag = AC(i, j, edge, ghost);
-                    AC(i, j, edge, ghost) = 0;
-                    if (to_other) { AC(i, j, edge, other) += ag; }
-                    else { AC(i, j, edge, OC_P) += fold * ag; }
-                }
-            }
+    /* apply_to_implicit (BoundaryConditions.cpp:885-940) on every non-periodic face */
+    const int vh[3] = {g->hi[0] + os_nodal(t, 0), g->hi[1] + os_nodal(t, 1), g->hi[2] + os_nodal(t, 2)};
+    for (int d = 0; d < 3; ++d) {
+        if (d < 2 && g->periodic[d]) { continue; }
+        for (int lo = 1; lo >= 0; --lo) {
+            const int ft = bc->face[face_of(d, lo)];
+            if (ft == OFACE_PERIODIC) { continue; }   /* no boundary condition given (residual without bc) */
+            const int rule = rule_for(f, ft, d, lo);
+            const int nodal_d = os_nodal(t, d);
+            const int dirichlet_pt = nodal_d && (rule == R_VALUE || rule == R_PROFILE);
+            const int edge = lo ? g->lo[d] : (nodal_d ? g->hi[d] + 1 : g->hi[d]);
+            const int cm = d == 0 ? OC_XM : d == 1 ? OC_YM : OC_ZM;
+            const int cp = d == 0 ? OC_XP : d == 1 ? OC_YP : OC_ZP;
+            const int ghost = lo ? cm : cp, other = lo ? cp : cm;
+            double fold = 0;
+            int to_other = 0;
+            if (nodal_d) { to_other = 1; }
+            else if (rule == R_EVEN) { fold = -1; }
+            else if (rule == R_ODD || rule == R_VALUE) { fold = 1; }
+            else if (rule == R_GRADIENT) { to_other = 1; }
+            int lo3[3] = {g->lo[0], g->lo[1], g->lo[2]}, hi3[3] = {vh[0], vh[1], vh[2]};
+            lo3[d] = hi3[d] = edge;
+            for (int k = lo3[2]; k <= hi3[2]; ++k)
+                for (int j = lo3[1]; j <= hi3[1]; ++j)
+                    for (int i = lo3[0]; i <= hi3[0]; ++i) {
+                        if (dirichlet_pt) {
+                            for (int n = 0; n < OC_N; ++n) { AC(i, j, k, n) = 0; }
+                            AC(i, j, k, OC_P) = 1;
+                        } else {
+                            const double ag = AC(i, j, k, ghost);
+                            AC(i, j, k, ghost) = 0;
+                            if (to_other) { AC(i, j, k, other) += ag; }
+                            else { AC(i, j, k, OC_P) += fold * ag; }
+                        }
+                    }
+        }
     }
 #undef AC
 }

@@ -4,7 +4,7 @@ Source/Residual.cpp:272-304, composed from the oracle's C pieces.
 TEST INFRASTRUCTURE ONLY (see ``oracle.py``): used by tests/ and bench.py's
 CPU leg as the checker of the HIP library.
 
-Scope, as the HIP library's: x and y periodic; the ground ``surface_layer``
+Scope, as the HIP library's: periodic, inflow, outflow, slip or no-slip sides; the ground ``surface_layer``
 (neutral, first-centre reference), ``slip_wall`` or ``no_slip_wall``; the top
 ``slip_wall``, ``no_slip_wall`` or ``geostrophic``; the k-L closure (or given
 diffusivities); buoyancy, Coriolis, geostrophic and pressure-gradient
@@ -88,18 +88,41 @@ class Forcing:
 
 @dataclass
 class Boundary:
-    """The z faces of Source/BoundaryConditions.H:82-100 (sides periodic)."""
+    """Source/BoundaryConditions.H:82-100: the face types (the ground and the
+    top, and the sides where the geometry is not periodic), the top's theta
+    gradient, the walls' velocities and the inflow profile an inflow face
+    takes (oracle/initializer.py's InflowProfile)."""
     zlo: int = SURFACE_LAYER
     zhi: int = SLIP_WALL
     zhi_theta_grad: float = 0.0
-    wall_velocity: tuple = ((0.0, 0.0, 0.0), (0.0, 0.0, 0.0))
+    wall_velocity: tuple = ((0.0, 0.0, 0.0), (0.0, 0.0, 0.0))    # zlo, zhi
+    xlo: int = PERIODIC
+    xhi: int = PERIODIC
+    ylo: int = PERIODIC
+    yhi: int = PERIODIC
+    side_wall_velocity: tuple = ((0.0,) * 3,) * 4                # xlo, ylo, xhi, yhi
+    profile: object = None
+
+    def faces(self):
+        return (self.xlo, self.ylo, self.zlo, self.xhi, self.yhi, self.zhi)
 
     def c(self, forcing: Forcing):
-        b = orc.OBc(self.zlo, self.zhi, self.zhi_theta_grad)
-        for s in range(2):
+        b = orc.OBc()
+        for o, t in enumerate(self.faces()):
+            b.face[o] = int(t)
+        b.zhi_theta_grad = self.zhi_theta_grad
+        for o, vel in zip((2, 5), self.wall_velocity):
             for d in range(3):
-                b.wall_velocity[s][d] = self.wall_velocity[s][d]
+                b.wall_velocity[o][d] = vel[d]
+        for o, vel in zip((0, 1, 3, 4), self.side_wall_velocity):
+            for d in range(3):
+                b.wall_velocity[o][d] = vel[d]
         b.ug, b.vg = forcing.ug, forcing.vg
+        if self.profile is not None:
+            from .initializer import c_profile
+            pc, keep = c_profile(self.profile)
+            b.profile = C.pointer(pc)
+            b._keep = (pc, keep)
         return b
 
 
@@ -175,8 +198,8 @@ def fill_ghosts(q, mesh: Mesh, phys: Physics):
     so = (phys.surface or SurfaceOptions()).c()
     bcc = phys.bc.c(phys.forcing)
     rc = orc.lib().oracle_fill_ghosts(arr, _v(mesh.z, B1), bcc, so, g, _v(ss, (-1, -1, g.lo[2])))
-    if rc == -2:
-        raise orc.OracleError("fill_ghosts: the sides must be periodic")
+    if rc == -3:
+        raise orc.OracleError("fill_ghosts: an inflow face needs the inflow profile")
     if rc != 0:
         raise orc.OracleError("compute_surface_state: no positive, finite friction velocity")
     return ss if phys.bc.zlo == SURFACE_LAYER else None
@@ -255,11 +278,23 @@ def residual(q, mesh: Mesh, phys: Physics):
                                       int(phys.surface.wall_k_dirichlet), g, _v(R["tke"], lo))
     if phys.bc is not None:
         orc.lib().oracle_apply_rows(_v(q["u"], B2), _v(q["v"], B2), _v(q["w"], B2), _v(mesh.z, B1),
-                                    phys.bc.c(phys.forcing), g, _v(R["w"], lo))
-    # OverrideSync: the periodic hi faces take the lo faces' values
-    R["u"][0, :, :, -1] = R["u"][0, :, :, 0]
-    R["v"][0, :, -1, :] = R["v"][0, :, 0, :]
+                                    phys.bc.c(phys.forcing), g, _v(R["u"], lo), _v(R["v"], lo), _v(R["w"], lo))
+    # OverrideSync: the hi faces of a periodic direction take the lo faces' values
+    if g.periodic[0]:
+        R["u"][0, :, :, -1] = R["u"][0, :, :, 0]
+    if g.periodic[1]:
+        R["v"][0, :, -1, :] = R["v"][0, :, 0, :]
     return R, cf, ss
+
+
+def dirichlet_rows(eq, mesh: Mesh, phys: Physics):
+    """Source/BoundaryConditions.cpp:849-876: 1 at the Dirichlet face points
+    of eq at its location (valid points, begin lo), else 0."""
+    g = mesh.geom
+    loc = (1, 2, 4, 0, 0)[eq]
+    m = np.zeros(_shape(g, loc, 0))
+    orc.lib().oracle_dirichlet_rows(eq, phys.bc.c(phys.forcing), g, _v(m, tuple(g.lo)))
+    return m
 
 
 def implicit_rows(eq, F, Fb, mesh: Mesh, phys: Physics, cf=None, ss=None):

@@ -33,6 +33,22 @@ def coriolis_f(f: P.Forcing):
     return 4 * math.pi * math.sin(f.latitude * math.pi / 180) / f.rotation_period
 
 
+def poisson_bc(m, phys: P.Physics):
+    """The correction's boundary types (Source/BoundaryConditions.cpp:300-320,
+    ERF's get_fft_bc): periodic where the geometry is, Dirichlet on outflow
+    faces, Neumann elsewhere."""
+    out = []
+    for o, t in enumerate(phys.bc.faces()):
+        d = o % 3
+        if d < 2 and m.geom.periodic[d]:
+            out.append(orc.PERIODIC)
+        elif t == P.OUTFLOW:
+            out.append(orc.DIRICHLET)
+        else:
+            out.append(orc.NEUMANN)
+    return tuple(out)
+
+
 def simplec_iteration(tile, m, q, phys: P.Physics, dtau, sweeps=2, alpha_p=1.0, tke_floor=0.0, relax=0.3,
                       w_vnn_cap=0.0, margin=4.0):
     """q (dict of two-ghost arrays) is updated in place; returns the residual R(q)."""
@@ -46,10 +62,7 @@ def simplec_iteration(tile, m, q, phys: P.Physics, dtau, sweeps=2, alpha_p=1.0, 
     A = [P.implicit_rows(eq, F, Fb, m, phys, cf, ss) for eq in (P.EQ_U, P.EQ_V, P.EQ_W)]
     buoy = phys.forcing.gravity > 0
     At = P.implicit_rows(P.EQ_THETA, F, Fb, m, phys, cf, ss) if buoy else None
-    mask_w = np.zeros_like(dtf[2])
-    mask_w[0] = 1.0
-    mask_w[nz] = 1.0
-    masks = [np.zeros_like(dtf[0]), np.zeros_like(dtf[1]), mask_w]
+    masks = [P.dirichlet_rows(eq, m, phys)[0] for eq in (P.EQ_U, P.EQ_V, P.EQ_W)]
     vel = [one(q[n]) for n in ("u", "v", "w")]
     vel, aps, ds, _, _ = SH.momentum_predictor(tile, vel, [R["u"][0], R["v"][0], R["w"][0]], A, dtf, sweeps,
                                                mask=masks, w_vnn_cap=w_vnn_cap, margin=margin,
@@ -59,7 +72,7 @@ def simplec_iteration(tile, m, q, phys: P.Physics, dtau, sweeps=2, alpha_p=1.0, 
     for n, a in zip(("u", "v", "w"), vel):
         put_valid(q, n, a)
     P.fill_ghosts(q, m, phys)
-    bc = (orc.PERIODIC, orc.PERIODIC, orc.NEUMANN, orc.PERIODIC, orc.PERIODIC, orc.NEUMANN)
+    bc = poisson_bc(m, phys)
     u, v, w, p, *_ = SH.pressure_correction(tile, one(q["u"]), one(q["v"]), one(q["w"]), one(q["p"]), ds, bc, alpha_p)
     for n, a in zip(("u", "v", "w", "p"), (u, v, w, p)):
         put_valid(q, n, a)

@@ -149,3 +149,101 @@ def test_residual_and_implicit_rows_match(cuda, with_closure, wall):
         A = Fab.at_loc(tile.geom, loc, 7, 0)
         GP.implicit_rows(eq_g, dm, Fd, gi, A, cf, ss if wall else None)
         _close(A.numpy(), A_ref, 1e-12, f"A[{eq_o}]")
+
+
+# ---- non-periodic sides: inflow, outflow, slip and no-slip walls ----------------
+
+def _gprofile(p):
+    from ablsteadyrans_amd import initializer as GI
+    return GI.InflowProfile(kind=p.kind, ustar=p.ustar, z0=p.z0, speed_ref=p.speed_ref, z_ref=p.z_ref,
+                            alpha=p.alpha, direction=p.direction, Cmu0=p.Cmu0, kappa=p.kappa, unbounded=p.unbounded,
+                            l_max=p.l_max, table=p.table, theta_ref_z=tuple(p.theta_ref_z),
+                            theta_ref=tuple(p.theta_ref))
+
+
+def _open_inputs(phys_o: P.Physics):
+    gi = _inputs(phys_o)
+    b = phys_o.bc
+    gi.faces = b.faces()
+    wv = [(0.0, 0.0, 0.0)] * 6
+    for o, v in zip((0, 1, 3, 4), b.side_wall_velocity):
+        wv[o] = tuple(v)
+    for o, v in zip((2, 5), b.wall_velocity):
+        wv[o] = tuple(v)
+    gi.wall_velocity = tuple(wv)
+    gi.inflow = _gprofile(b.profile) if b.profile is not None else None
+    return gi
+
+
+OPEN = [((0, 0), (P.INFLOW, P.SLIP_WALL, P.OUTFLOW, P.NO_SLIP_WALL)),
+        ((0, 1), (P.INFLOW, P.PERIODIC, P.OUTFLOW, P.PERIODIC)),
+        ((1, 0), (P.PERIODIC, P.NO_SLIP_WALL, P.PERIODIC, P.SLIP_WALL)),
+        ((0, 0), (P.OUTFLOW, P.INFLOW, P.INFLOW, P.OUTFLOW))]
+
+
+def _open_setup(case, seed=4):
+    from ablsteadyrans_amd import mesh as M
+    from oracle import initializer as OI
+    per, (xlo, ylo, xhi, yhi) = OPEN[case]
+    n = (14, 12, 10)
+    lo, hi = (0.0, 0.0, 0.0), (700.0, 600.0, 500.0)
+    geom = M.geometry(n, lo, hi, per)
+    zl = M.make_zlevels(n[2], 500.0, dz0=10.0, stretch_ratio=1.15)
+    tile = M.build_mesh(geom, lo, hi, zl, M.Terrain(kind="hills", hills=[(350.0, 300.0, 70.0, 130.0)]))
+    m = PHY.omesh(tile)
+    q = PHY.random_state(tile, seed, U=6.0, amp=0.3)
+    prof = OI.InflowProfile(kind=OI.LOG_LAW, ustar=0.45, z0=0.1, direction=15.0, theta_ref_z=(0.0, 300.0, 1000.0),
+                            theta_ref=(300.0, 300.0, 302.0))
+    bc = P.Boundary(zlo=P.SURFACE_LAYER, zhi=P.SLIP_WALL, zhi_theta_grad=0.003, xlo=xlo, ylo=ylo, xhi=xhi, yhi=yhi,
+                    side_wall_velocity=((0.0,) * 3, (1.2, 0.0, 0.0), (0.0,) * 3, (0.0, 0.0, 0.7)), profile=prof)
+    phys = P.Physics(bc=bc, surface=P.SurfaceOptions(z0=0.1),
+                     forcing=P.Forcing(coriolis=True, latitude=45.0, geostrophic=True, ug=6.0, vg=1.0),
+                     theta_ref_z=(0.0, 300.0, 1000.0), theta_ref=(300.0, 300.0, 302.0))
+    return tile, m, q, phys
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", range(len(OPEN)))
+def test_open_side_fill_ghosts_match(cuda, case):
+    from ablsteadyrans_amd import physics as GP
+    from ablsteadyrans_amd.poisson import DeviceMesh
+    tile, m, q, phys = _open_setup(case)
+    ds = _dstate(tile, q)
+    ss_ref = P.fill_ghosts(q, m, phys)
+    ss = GP.surface_slab(tile.geom)
+    GP.fill_ghosts(DeviceMesh(tile), ds, _open_inputs(phys), ss)
+    for n in ("u", "v", "w", "p", "theta", "tke"):
+        _close(ds[n].numpy(), q[n], 1e-13, n)
+    for c in range(P.SS_N):
+        _close(ss.numpy()[c], ss_ref[c], 1e-13, f"ss[{c}]")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", range(len(OPEN)))
+def test_open_side_residual_and_implicit_rows_match(cuda, case):
+    import torch
+    from ablsteadyrans_amd import physics as GP
+    from ablsteadyrans_amd.fab import Fab
+    from ablsteadyrans_amd.poisson import DeviceMesh
+    tile, m, q, phys = _open_setup(case, seed=9)
+    P.fill_ghosts(q, m, phys)
+    R_ref, cf_ref, ss_ref = P.residual(q, m, phys)
+    gi = _open_inputs(phys)
+    dm = DeviceMesh(tile)
+    ds = _dstate(tile, q)
+    R = {n: Fab.at_loc(tile.geom, loc, 1, 0) for n, loc in zip(GP.FIELDS, GP.LOCS)}
+    cf = GP.ClosureFields(tile.geom)
+    ss = GP.surface_slab(tile.geom)
+    GP.residual(dm, ds, gi, R, cf, ss)
+    torch.cuda.synchronize()
+    for n in GP.FIELDS:
+        _close(R[n].numpy(), R_ref[n], 1e-11, n)
+    Fx, Fy, Fz, Fb = orc.transport_fluxes(q["u"], q["v"], q["w"], B2, m.z, B1, m.ax, m.ay, B1, m.geom)
+    F = [Fx[None], Fy[None], Fz[None]]
+    Fd = [_fab(f, Fb) for f in F]
+    for eq_o, eq_g, loc in ((P.EQ_U, GP.EQ_U, 1), (P.EQ_V, GP.EQ_V, 2), (P.EQ_W, GP.EQ_W, 4),
+                            (P.EQ_THETA, GP.EQ_THETA, 0), (P.EQ_TKE, GP.EQ_TKE, 0)):
+        A_ref = P.implicit_rows(eq_o, F, Fb, m, phys, cf_ref, ss_ref)
+        A = Fab.at_loc(tile.geom, loc, 7, 0)
+        GP.implicit_rows(eq_g, dm, Fd, gi, A, cf, ss)
+        _close(A.numpy(), A_ref, 1e-12, f"A[{eq_o}]")

@@ -1,0 +1,190 @@
+"""The oracle's boundary conditions on non-periodic sides (oracle/physics.c
+after Source/BoundaryConditions.cpp), pinned to the rules the reference's
+header states (Source/BoundaryConditions.H:8-57) in closed form:
+
+* inflow: every field the profile at its own height above the ground (the
+  face point of u included), p' zero gradient;
+* outflow: zero gradient (u mirrored about its face point), p' odd;
+* slip wall: the normal velocity 0 on the face and odd about it, the rest
+  even; no-slip wall: every velocity the wall's value;
+* the Dirichlet rows (mask 1 on the face points of a Dirichlet normal
+  velocity), the residual rows R = q - V there, and the implicit folds
+  (identity rows, even/odd/value folds into the diagonal, the normal
+  velocity's mirror onto the opposite neighbour).
+The C profile evaluation is checked against the numpy one (two
+restatements of Source/InflowProfile.H:149-200)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from ablsteadyrans_amd import mesh as M
+from oracle import initializer as OI
+from oracle import oracle as orc
+from oracle import physics as P
+from tests import physics_helpers as PHY
+from tests.linesys import COEF_P, XM, XP
+
+B1, B2 = (-1, -1, -1), (-2, -2, -2)
+
+
+def _open_hill(n=(12, 10, 8), periodic=(0, 0)):
+    lo, hi = (0.0, 0.0, 0.0), (600.0, 500.0, 400.0)
+    geom = M.geometry(n, lo, hi, periodic)
+    zl = M.make_zlevels(n[2], 400.0, dz0=10.0, stretch_ratio=1.15)
+    return M.build_mesh(geom, lo, hi, zl, M.Terrain(kind="hills", hills=[(300.0, 250.0, 50.0, 110.0)]))
+
+
+PROF = OI.InflowProfile(kind=OI.LOG_LAW, ustar=0.45, z0=0.05, direction=12.0, theta_ref_z=(0.0, 1000.0),
+                        theta_ref=(290.0, 300.0))
+
+
+def _phys(periodic=(0, 0), wall_u=1.5):
+    bc = P.Boundary(zlo=P.SURFACE_LAYER, zhi=P.SLIP_WALL, zhi_theta_grad=0.003,
+                    xlo=P.PERIODIC if periodic[0] else P.INFLOW, xhi=P.PERIODIC if periodic[0] else P.OUTFLOW,
+                    ylo=P.PERIODIC if periodic[1] else P.SLIP_WALL, yhi=P.PERIODIC if periodic[1] else P.NO_SLIP_WALL,
+                    side_wall_velocity=((0.0,) * 3, (0.0,) * 3, (0.0,) * 3, (wall_u, 0.0, 0.0)), profile=PROF)
+    return P.Physics(closure=P.ClosureOptions(tke_floor=1e-6), bc=bc, surface=P.SurfaceOptions(z0=0.05),
+                     theta_ref_z=(0.0, 1000.0), theta_ref=(290.0, 300.0))
+
+
+def _filled(periodic=(0, 0), seed=5):
+    tile = _open_hill(periodic=periodic)
+    m = PHY.omesh(tile)
+    q = PHY.random_state(tile, seed, U=6.0, amp=0.1)
+    phys = _phys(periodic)
+    P.fill_ghosts(q, m, phys)
+    return tile, m, q, phys
+
+
+def _h(m, loc, I, J, K):
+    return OI.height_above_ground(loc, m.z, B1, np.asarray(I), np.asarray(J), np.asarray(K), 0)
+
+
+def test_c_profile_matches_numpy():
+    z = np.concatenate([[0.0, 1e-3, 0.04], np.geomspace(0.1, 1200.0, 40)])
+    tz = np.array([0.0, 3.0, 20.0, 90.0, 400.0])
+    for p in (PROF, OI.InflowProfile(kind=OI.POWER_LAW, speed_ref=7.0, alpha=0.2, z0=0.1, unbounded=True),
+              OI.InflowProfile(kind=OI.TABLE, z0=0.1, table=(tz, tz * 0.05 + 2, tz * 0.0 + 0.5, tz * 0 + 300, None)),
+              OI.InflowProfile(kind=OI.TABLE, z0=0.1, table=(tz, tz * 0.05 + 2, tz * 0.0 + 0.5, tz * 0 + 300,
+                                                             np.linspace(0.9, 0.1, 5)))):
+        a = OI.profile_eval_c(p, z)
+        b = OI.inflow_eval(p, z)
+        for x, y in zip(a, b):
+            y = np.broadcast_to(y, z.shape)
+            assert np.allclose(x, y, rtol=1e-15, atol=1e-300)
+
+
+def test_inflow_face_takes_the_profile_at_every_point():
+    tile, m, q, _ = _filled()
+    nx, ny, nz = tile.geom.n
+    # u: the face point i = 0 and the ghosts i = -1, -2, over the valid j, k
+    J, K = np.meshgrid(np.arange(ny), np.arange(nz), indexing="ij")
+    for i in (0, -1, -2):
+        h = _h(m, 1, np.full(J.shape, i), J, K)
+        u = OI.inflow_eval(PROF, h)[0]
+        assert np.allclose(q["u"][0, K + 2, J + 2, i + 2], u, rtol=1e-15, atol=0), i
+    # theta and k in the ghost cells, v at its ghost faces
+    J, K = np.meshgrid(np.arange(ny), np.arange(nz), indexing="ij")
+    for i in (-1, -2):
+        h = _h(m, 0, np.full(J.shape, i), J, K)
+        _, _, th, k, _ = OI.inflow_eval(PROF, h)
+        assert np.allclose(q["theta"][0, K + 2, J + 2, i + 2], th, rtol=1e-15)
+        assert np.allclose(q["tke"][0, K + 2, J + 2, i + 2], k, rtol=1e-15)
+        # (the wall faces j = 0 and ny take the walls' value: y is filled after x)
+        Jv, Kv = np.meshgrid(np.arange(1, ny), np.arange(nz), indexing="ij")
+        hv = _h(m, 2, np.full(Jv.shape, i), Jv, Kv)
+        assert np.allclose(q["v"][0, Kv + 2, Jv + 2, i + 2], OI.inflow_eval(PROF, hv)[1], rtol=1e-15)
+    # p' zero gradient, w the profile's 0
+    assert np.array_equal(q["p"][0, 2:-2, 2:-2, 1], q["p"][0, 2:-2, 2:-2, 2])
+    assert np.array_equal(q["p"][0, 2:-2, 2:-2, 0], q["p"][0, 2:-2, 2:-2, 3])
+    assert np.all(q["w"][0, 3:-3, 2:-2, 0:2] == 0.0)
+
+
+def test_outflow_is_zero_gradient_and_p_odd():
+    tile, m, q, _ = _filled()
+    nx, ny, nz = tile.geom.n
+    u = q["u"][0, 2:-2, 2:-2]
+    # u's face point nx is valid; the ghosts mirror about it
+    assert np.array_equal(u[:, :, nx + 3], u[:, :, nx + 1])
+    assert np.array_equal(u[:, :, nx + 4], u[:, :, nx])
+    p = q["p"][0, 2:-2, 2:-2]
+    assert np.array_equal(p[:, :, nx + 2], -p[:, :, nx + 1])
+    assert np.array_equal(p[:, :, nx + 3], -p[:, :, nx])
+    t = q["theta"][0, 2:-2, 2:-2]
+    assert np.array_equal(t[:, :, nx + 2], t[:, :, nx + 1])
+
+
+def test_slip_and_no_slip_walls():
+    tile, m, q, _ = _filled()
+    nx, ny, nz = tile.geom.n
+    v = q["v"][0, 2:-2]          # (nz, ny+5, nx+4), j offset 2
+    assert np.all(v[:, 2, :] == 0.0)                              # the slip face j = 0
+    assert np.array_equal(v[:, 1, :], -v[:, 3, :])                # odd about it
+    u = q["u"][0, 2:-2]
+    assert np.array_equal(u[:, 1, :], u[:, 2, :])                 # tangential even
+    # no-slip yhi with the wall's u = 1.5: v = 0 on the face, u = 1.5 there
+    assert np.all(v[:, ny + 2, :] == 0.0)
+    assert np.array_equal(v[:, ny + 3, :], -v[:, ny + 1, :])
+    assert np.allclose(u[:, ny + 2, :], 2 * 1.5 - u[:, ny + 1, :], rtol=0, atol=1e-15)
+    t = q["theta"][0, 2:-2]
+    assert np.array_equal(t[:, ny + 2, :], t[:, ny + 1, :])
+
+
+def test_periodic_sides_are_unchanged_by_the_general_fill():
+    """With periodic sides the lateral ghosts are the periodic images."""
+    tile, m, q, _ = _filled(periodic=(1, 1))
+    nx, ny, nz = tile.geom.n
+    t = q["theta"][0, 2:-2]
+    assert np.array_equal(t[:, :, 0:2], t[:, :, nx:nx + 2])
+    assert np.array_equal(t[:, 0:2, :], t[:, ny:ny + 2, :])
+
+
+def test_dirichlet_rows_residual_rows_and_folds():
+    tile, m, q, phys = _filled()
+    nx, ny, nz = tile.geom.n
+    mu = P.dirichlet_rows(P.EQ_U, m, phys)[0]
+    assert np.all(mu[:, :, 0] == 1) and np.all(mu[:, :, nx] == 0) and mu.sum() == ny * nz
+    mv = P.dirichlet_rows(P.EQ_V, m, phys)[0]
+    assert np.all(mv[:, 0, :] == 1) and np.all(mv[:, ny, :] == 1) and mv.sum() == 2 * nx * nz
+    mw = P.dirichlet_rows(P.EQ_W, m, phys)[0]
+    assert np.all(mw[0] == 1) and np.all(mw[nz] == 1)
+    assert P.dirichlet_rows(P.EQ_THETA, m, phys).sum() == 0
+    R, cf, ss = P.residual(q, m, phys)
+    J, K = np.meshgrid(np.arange(ny), np.arange(nz), indexing="ij")
+    h = _h(m, 1, np.zeros_like(J), J, K)
+    assert np.allclose(R["u"][0, K, J, 0], q["u"][0, K + 2, J + 2, 2] - OI.inflow_eval(PROF, h)[0], rtol=0,
+                       atol=1e-14)
+    assert np.array_equal(R["v"][0, :, 0, :], q["v"][0, 2:-2, 2, 2:-2])            # slip: R = v - 0
+    assert np.array_equal(R["v"][0, :, ny, :], q["v"][0, 2:-2, ny + 2, 2:-2])      # no-slip's normal value 0
+    # the implicit rows against the bare operator
+    Fx, Fy, Fz, Fb = orc.transport_fluxes(q["u"], q["v"], q["w"], B2, m.z, B1, m.ax, m.ay, B1, m.geom)
+    F = [Fx[None], Fy[None], Fz[None]]
+    bare = P.Physics(closure=phys.closure, bc=None, surface=None, theta_ref_z=phys.theta_ref_z,
+                     theta_ref=phys.theta_ref)
+    for eq in (P.EQ_U, P.EQ_THETA):
+        A = P.implicit_rows(eq, F, Fb, m, phys, cf, ss)
+        A0 = P.implicit_rows(eq, F, Fb, m, bare, cf, None)
+        if eq == P.EQ_U:
+            # identity rows on the inflow face; the outflow face point's XP moved onto XM
+            assert np.all(A[COEF_P, :, :, 0] == 1) and np.all(A[1:, :, :, 0] == 0)
+            assert np.all(A[XP, 1:, :, nx] == 0)
+            assert np.array_equal(A[XM, 1:, 1:-1, nx], A0[XM, 1:, 1:-1, nx] +
+                                  A0[XP, 1:, 1:-1, nx])
+        else:
+            # theta: the profile ghost on xlo drops out; outflow (even) folds -a into the diagonal
+            assert np.all(A[XM, :, :, 0] == 0) and np.all(A[XP, :, :, nx - 1] == 0)
+            k = slice(1, -1)
+            assert np.array_equal(A[COEF_P, k, 1:-1, 0], A0[COEF_P, k, 1:-1, 0])
+            assert np.allclose(A[COEF_P, k, 1:-1, nx - 1],
+                               A0[COEF_P, k, 1:-1, nx - 1] - A0[XP, k, 1:-1, nx - 1], rtol=1e-15)
+
+
+def test_inflow_without_a_profile_is_refused():
+    tile = _open_hill()
+    m = PHY.omesh(tile)
+    q = PHY.random_state(tile, 1)
+    phys = _phys()
+    phys.bc.profile = None
+    with pytest.raises(orc.OracleError, match="inflow profile"):
+        P.fill_ghosts(q, m, phys)
