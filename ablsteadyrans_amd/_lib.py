@@ -207,6 +207,15 @@ class InflowProfileC(C.Structure):
                 ("theta_ref_z", C.c_double * 64), ("theta_ref", C.c_double * 64)]
 
 
+class DampingC(C.Structure):
+    """``gust_damping`` (Source/BoundaryConditions.H:53-57, 107-121)."""
+
+    _fields_ = [("rayleigh_zdamp", C.c_double), ("rayleigh_dampcoef", C.c_double),
+                ("rayleigh_fields", C.c_int32 * 5), ("rayleigh_above_ground", C.c_int32),
+                ("sponge_width", C.c_double), ("sponge_strength", C.c_double), ("ztop", C.c_double),
+                ("prob_lo", C.c_double * 2), ("prob_hi", C.c_double * 2)]
+
+
 class PhysicsC(C.Structure):
     """``gust_physics`` (Source/Residual.H:90-128)."""
 
@@ -216,7 +225,8 @@ class PhysicsC(C.Structure):
                 ("zhi_theta_grad", C.c_double), ("wall_velocity", (C.c_double * 3) * 6),
                 ("n_theta_ref", C.c_int32), ("theta_ref_z", C.c_double * 64), ("theta_ref", C.c_double * 64),
                 ("momentum_limiter", C.c_int32), ("theta_limiter", C.c_int32), ("momentum_eps", C.c_double),
-                ("theta_eps", C.c_double), ("face_mean", C.c_int32), ("inflow", InflowProfileC)]
+                ("theta_eps", C.c_double), ("face_mean", C.c_int32), ("inflow", InflowProfileC),
+                ("damping", DampingC)]
 
 
 class StateC(C.Structure):

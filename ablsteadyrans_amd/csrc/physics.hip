@@ -635,6 +635,71 @@ __global__ __launch_bounds__(kThreads) void side_rows_kernel(Box3 plane, V r, CV
     }
 }
 
+// ---- the Rayleigh and sponge layers (BoundaryConditions.cpp:942-1101) -------------
+
+struct DampCoef {
+    int rayleigh, sponge;
+    double zd, rc, ztop, sw, inv_w, ss;
+    double plo[2], phi[2];
+    int lateral[4];   // xlo, xhi, ylo, yhi hold an inflow or outflow face
+    __device__ __forceinline__ double rayleigh_part(double z) const
+    {
+        const double pi = 3.14159265358979323846;
+        if (rayleigh && z > zd) {
+            const double s = sin(0.5 * pi * (z - zd) / (ztop - zd));
+            return rc * s * s;
+        }
+        return 0.0;
+    }
+    __device__ __forceinline__ double sponge_part(double x, double y) const
+    {
+        double c = 0.0;
+        if (sponge) {
+            const double dist[4] = {x - plo[0], phi[0] - x, y - plo[1], phi[1] - y};
+            for (int n = 0; n < 4; ++n) {
+                if (lateral[n] && dist[n] < sw) {
+                    const double r = 1 - dist[n] * inv_w;
+                    c += ss * r * r;
+                }
+            }
+        }
+        return c;
+    }
+};
+
+// R += c (q - q_ref) (BoundaryConditions.cpp:1022-1068).
+template <int T>
+__global__ __launch_bounds__(kThreads) void damping_kernel(Box3 b, V r, CV a, DampCoef dc, int f, int with_rayleigh,
+                                                          int above_ground, NodeZ z, profile::ProfileView prof,
+                                                          int klo, double dx0, double dx1)
+{
+    GUST_GRID_LOOP(b)
+    {
+        int i, j, k;
+        b.at(t, i, j, k);
+        const double h = heights::height_above_ground<T>(z, i, j, k, klo);
+        const double zc = point_z<T>(z, i, j, k);
+        const double x = dc.plo[0] + (i + (sc::nodal(T, 0) ? 0.0 : 0.5)) * dx0;
+        const double y = dc.plo[1] + (j + (sc::nodal(T, 1) ? 0.0 : 0.5)) * dx1;
+        const double zr = above_ground ? h : zc;
+        const double cr = with_rayleigh ? dc.rayleigh_part(zr) : 0.0;
+        const double cs = dc.sponge_part(x, y);
+        auto ref_at = [&](double zz) {
+            double u, v, th, kk, nut;
+            profile::profile_eval(prof, zz, u, v, th, kk, nut);
+            return f == F_U ? u : f == F_V ? v : f == F_THETA ? th : f == F_TKE ? kk : 0.0;
+        };
+        if (cr > 0) {
+            const double ref = (f == F_W) ? 0.0 : ref_at(f == F_THETA ? zr : h);
+            r(i, j, k) += cr * (a(i, j, k) - ref);
+        }
+        if (cs > 0) {
+            const double ref = (f == F_W) ? 0.0 : ref_at(h);
+            r(i, j, k) += cs * (a(i, j, k) - ref);
+        }
+    }
+}
+
 // The hi face of a periodic nodal axis takes the lo face's value.
 __global__ __launch_bounds__(kThreads) void seam_copy_kernel(Box3 plane, V a, int axis, int n)
 {
@@ -673,6 +738,25 @@ struct AV {
 AV a_view(const gust_array4* a)
 {
     return AV{static_cast<double*>(a->p), a->jstride, a->kstride, a->nstride, a->begin[0], a->begin[1], a->begin[2]};
+}
+
+// The layers' coefficient on the implicit diagonal (BoundaryConditions.cpp:1070-1099).
+template <int T>
+__global__ __launch_bounds__(kThreads) void damping_diag_kernel(Box3 b, AV A, DampCoef dc, int with_rayleigh,
+                                                               int above_ground, NodeZ z, int klo, double dx0,
+                                                               double dx1)
+{
+    GUST_GRID_LOOP(b)
+    {
+        int i, j, k;
+        b.at(t, i, j, k);
+        const double h = heights::height_above_ground<T>(z, i, j, k, klo);
+        const double zc = point_z<T>(z, i, j, k);
+        const double x = dc.plo[0] + (i + (sc::nodal(T, 0) ? 0.0 : 0.5)) * dx0;
+        const double y = dc.plo[1] + (j + (sc::nodal(T, 1) ? 0.0 : 0.5)) * dx1;
+        A(i, j, k, GUST_COEF_P) += (with_rayleigh ? dc.rayleigh_part(above_ground ? h : zc) : 0.0) +
+                                   dc.sponge_part(x, y);
+    }
 }
 
 __global__ __launch_bounds__(kThreads) void add_dsdk_kernel(Box3 b, AV A, CV ds)
@@ -885,6 +969,27 @@ void check_physics(const char* op, const gust_geometry& g, const gust_physics* p
                 if (!err.empty()) { GUST_FAIL("gust: %s: bc.%s = inflow: %s", op, keys[o], err.front().c_str()); }
             }
         }
+        {
+            const gust_damping& dm = p->damping;
+            if (std::isnan(dm.rayleigh_zdamp) || !(dm.rayleigh_zdamp >= 0 && dm.rayleigh_zdamp < dm.ztop) ||
+                std::isnan(dm.rayleigh_dampcoef) || !(dm.rayleigh_dampcoef >= 0 && std::isfinite(dm.rayleigh_dampcoef))) {
+                if (dm.rayleigh_zdamp != 0 || dm.rayleigh_dampcoef != 0) {
+                    GUST_FAIL("gust: %s: bc.rayleigh_zdamp must be in [0, the top level) and bc.rayleigh_dampcoef "
+                              "finite and >= 0 (1/s)", op);
+                }
+            }
+            if (dm.rayleigh_zdamp > 0 && !(dm.rayleigh_dampcoef > 0)) {
+                GUST_FAIL("gust: %s: bc.rayleigh_zdamp > 0 needs bc.rayleigh_dampcoef > 0", op);
+            }
+            if (std::isnan(dm.sponge_width) || !(dm.sponge_width >= 0) || std::isnan(dm.sponge_strength) ||
+                !(dm.sponge_strength >= 0)) {
+                GUST_FAIL("gust: %s: bc.sponge_width and bc.sponge_strength must be >= 0", op);
+            }
+            if (dm.rayleigh_zdamp > 0 || dm.sponge_width > 0) {
+                const std::vector<std::string> err = profile::problems(p->inflow);
+                if (!err.empty()) { GUST_FAIL("gust: %s: the damping layers' profile: %s", op, err.front().c_str()); }
+            }
+        }
         const int zlo = p->face[2], zhi = p->face[5];
         if (!(zlo == GUST_FACE_SURFACE_LAYER || zlo == GUST_FACE_SLIP_WALL || zlo == GUST_FACE_NO_SLIP_WALL)) {
             GUST_FAIL("gust: %s: bc.zlo must be surface_layer, slip_wall or no_slip_wall", op);
@@ -995,6 +1100,42 @@ void check_ss(const char* op, const gust_geometry& g, const gust_array4* ss)
                   "components",
                   op, int(GUST_SS_N));
     }
+}
+
+bool damping_on(const gust_physics& p) { return p.damping.rayleigh_zdamp > 0 || p.damping.sponge_width > 0; }
+
+DampCoef damp_coef(const gust_physics& p)
+{
+    const gust_damping& d = p.damping;
+    DampCoef c{};
+    c.zd = d.rayleigh_zdamp;
+    c.rc = d.rayleigh_dampcoef;
+    c.ztop = d.ztop;
+    c.sw = d.sponge_width;
+    c.ss = d.sponge_strength;
+    c.rayleigh = c.zd > 0 ? 1 : 0;
+    c.sponge = c.sw > 0 ? 1 : 0;
+    c.inv_w = c.sponge ? 1 / c.sw : 0.0;
+    for (int k = 0; k < 2; ++k) {
+        c.plo[k] = d.prob_lo[k];
+        c.phi[k] = d.prob_hi[k];
+    }
+    for (int dd = 0; dd < 2; ++dd) {
+        for (int side = 0; side < 2; ++side) {
+            const int t = p.face[side == 0 ? dd : dd + 3];
+            c.lateral[2 * dd + side] = (t == GUST_FACE_INFLOW || t == GUST_FACE_OUTFLOW) ? 1 : 0;
+        }
+    }
+    return c;
+}
+
+template <class F>
+void dispatch_loc(int loc, F&& f)
+{
+    if (loc == 1) { f(std::integral_constant<int, 1>{}); }
+    else if (loc == 2) { f(std::integral_constant<int, 2>{}); }
+    else if (loc == 4) { f(std::integral_constant<int, 4>{}); }
+    else { f(std::integral_constant<int, 0>{}); }
 }
 
 IBox view_box(const gust_array4* a)
@@ -1408,9 +1549,26 @@ int gust_residual(const gust_geometry* geom, const gust_state* q, const gust_mes
                                                                       p.surface.wall_k == 0 ? 1 : 0);
         }
         if (p.use_bc) {
-            bool inflow = false;
+            bool inflow = damping_on(p);
             for (int o = 0; o < 6; ++o) { inflow = inflow || p.face[o] == GUST_FACE_INFLOW; }
             const profile::ProfileView prof = inflow ? profile::device_view(st, p.inflow) : profile::ProfileView{};
+            if (damping_on(p)) {
+                const DampCoef dcf = damp_coef(p);
+                const int ff[5] = {F_U, F_V, F_W, F_THETA, F_TKE};
+                const int qn[5] = {0, 1, 2, 4, 5};
+                for (int n = 0; n < 5; ++n) {
+                    const int loc = n < 3 ? (1 << n) : GUST_LOC_CELL;
+                    const Box3 b = box3(loc_box(g, loc));
+                    dispatch_loc(loc, [&](auto tc) {
+                        constexpr int T = decltype(tc)::value;
+                        damping_kernel<T><<<blocks_for(b.npts()), kThreads, 0, st>>>(
+                            b, dview<double>(Rw[qn[n]]), dview<const double>(q->f[qn[n]]), dcf, ff[n],
+                            p.damping.rayleigh_fields[n], p.damping.rayleigh_above_ground, node_z(mesh->z_nd), prof,
+                            g.lo[2], g.dx[0], g.dx[1]);
+                    });
+                    GUST_HIP(hipGetLastError());
+                }
+            }
             for (int d = 0; d < 2; ++d) {
                 if (g.periodic[d]) { continue; }
                 const int tl = p.face[d], th = p.face[d + 3];
@@ -1508,6 +1666,17 @@ int gust_implicit_rows(int32_t eq, const gust_geometry* geom, const gust_faces* 
             GUST_HIP(hipGetLastError());
         }
         if (!p.use_bc) { return; }
+        if (damping_on(p)) {
+            const Box3 b = box3(vb);
+            const DampCoef dcf = damp_coef(p);
+            dispatch_loc(loc, [&](auto tc) {
+                constexpr int T = decltype(tc)::value;
+                damping_diag_kernel<T><<<blocks_for(b.npts()), kThreads, 0, st>>>(
+                    b, a_view(A), dcf, p.damping.rayleigh_fields[e], p.damping.rayleigh_above_ground,
+                    node_z(mesh->z_nd), g.lo[2], g.dx[0], g.dx[1]);
+            });
+            GUST_HIP(hipGetLastError());
+        }
         for (int d = 0; d < 3; ++d) {
             if (d < 2 && g.periodic[d]) { continue; }
             const bool nodal_d = ((loc >> d) & 1) != 0;

@@ -79,6 +79,34 @@ class ForcingOptions:
 
 
 @dataclass
+class Damping:
+    """bc.rayleigh_* and bc.sponge_* (Source/BoundaryConditions.H:107-121):
+    ``fields`` among u, v, w, theta, tke; ``ztop`` the mesh's top level;
+    ``prob_lo``/``prob_hi`` the domain's x and y extent (the sponge's distances)."""
+    rayleigh_zdamp: float = 0.0
+    rayleigh_dampcoef: float = 0.0
+    fields: tuple = ("u", "v", "w", "theta", "tke")
+    above_ground: bool = False
+    sponge_width: float = 0.0
+    sponge_strength: float = 0.0
+    ztop: float = 0.0
+    prob_lo: tuple = (0.0, 0.0)
+    prob_hi: tuple = (0.0, 0.0)
+
+    def c(self):
+        d = _lib.DampingC()
+        d.rayleigh_zdamp, d.rayleigh_dampcoef = float(self.rayleigh_zdamp), float(self.rayleigh_dampcoef)
+        for n, name in enumerate(("u", "v", "w", "theta", "tke")):
+            d.rayleigh_fields[n] = 1 if name in self.fields else 0
+        d.rayleigh_above_ground = 1 if self.above_ground else 0
+        d.sponge_width, d.sponge_strength = float(self.sponge_width), float(self.sponge_strength)
+        d.ztop = float(self.ztop)
+        for k in range(2):
+            d.prob_lo[k], d.prob_hi[k] = float(self.prob_lo[k]), float(self.prob_hi[k])
+        return d
+
+
+@dataclass
 class ResidualInputs:
     """Source/Residual.H:90-128 on this build's scope. ``closure`` None takes
     the given cell diffusivities (Fabs with one ghost); ``faces`` in
@@ -102,6 +130,7 @@ class ResidualInputs:
     theta_eps: float = 1e-5
     log_face_mean: bool = False
     inflow: object = None   # initializer.InflowProfile: the profile an inflow face takes
+    damping: object = None  # Damping: the Rayleigh and sponge layers (off when None)
 
     def c(self):
         p = _lib.PhysicsC()
@@ -138,6 +167,8 @@ class ResidualInputs:
             ic, keep = self.inflow.c()
             p.inflow = ic
             self._keep.append(keep)
+        if self.damping is not None:
+            p.damping = self.damping.c()
         return p
 
 

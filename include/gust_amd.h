@@ -646,6 +646,24 @@ typedef struct gust_inflow_profile {
     double theta_ref[64];
 } gust_inflow_profile;
 
+/* The Rayleigh and sponge layers, Source/BoundaryConditions.H:53-57, 107-121
+ * (bc.rayleigh_*, bc.sponge_*; both off by default): above rayleigh_zdamp (> 0
+ * enables) R += c (q - q_ref), c = rayleigh_dampcoef sin^2(pi/2 (z - zdamp) /
+ * (ztop - zdamp)), for the fields rayleigh_fields (u, v, w, theta, k) with
+ * z absolute or above the ground (rayleigh_above_ground); within
+ * sponge_width (> 0 enables) of an inflow or outflow face c = sponge_strength
+ * (1 - s/width)^2 for every field; q_ref the inflow profile (w: 0); c is
+ * added to the implicit diagonal. ztop: the mesh's top level; prob_lo,
+ * prob_hi: the domain's x and y extent (m). */
+typedef struct gust_damping {
+    double rayleigh_zdamp, rayleigh_dampcoef;
+    int32_t rayleigh_fields[5];
+    int32_t rayleigh_above_ground;
+    double sponge_width, sponge_strength;
+    double ztop;
+    double prob_lo[2], prob_hi[2];
+} gust_damping;
+
 /* The residual's inputs, Source/Residual.H:90-128 (ResidualInputs) on this
  * scope: the closure (use_closure) or given cell diffusivities with one
  * ghost (nu, gamma_theta, gamma_tke); the wall model on a surface_layer
@@ -674,6 +692,7 @@ typedef struct gust_physics {
     double momentum_eps, theta_eps;
     int32_t face_mean;
     gust_inflow_profile inflow;
+    gust_damping damping;
 } gust_physics;
 
 /* The six state fields u, v, w (faces), p, theta, k (cells), each with two

@@ -184,13 +184,27 @@ typedef struct {   /* Source/InflowProfile.H:95-133 (View) */
     int rn;
 } oprofile;
 
+typedef struct {   /* Source/BoundaryConditions.H:107-121 (Rayleigh and sponge layers) */
+    double rayleigh_zdamp, rayleigh_dampcoef;
+    int rayleigh_fields[5];       /* u, v, w, theta, k */
+    int rayleigh_above_ground;
+    double sponge_width, sponge_strength;
+    double ztop;                  /* the mesh's top level */
+    double prob_lo[2], prob_hi[2];
+} odamping;
+
 typedef struct {   /* Source/BoundaryConditions.H:82-100 */
     int face[6];                  /* OFACE_*, orientation order xlo, ylo, zlo, xhi, yhi, zhi */
     double zhi_theta_grad;
     double wall_velocity[6][3];
     double ug, vg;                /* a geostrophic top's wind */
-    const oprofile* profile;      /* the inflow profile (an inflow face needs it) */
+    const oprofile* profile;      /* the inflow profile (an inflow face or the damping needs it) */
+    odamping damping;
 } obc;
+
+/* Source/BoundaryConditions.cpp:1022-1068: R += c (q - q_ref) on the valid
+ * points of u, v, w, theta, k (q with two ghosts, R begin lo). */
+void oracle_add_damping(oa4* const q[6], const oa4* z, const obc* bc, const ogeom* g, oa4* const R[6]);
 
 /* Source/InflowProfile.H:149-200: (u, v, theta, k, nu_t) at height z above the ground. */
 void oracle_profile_eval(const oprofile* p, double z, double out[5]);
@@ -215,8 +229,8 @@ void oracle_apply_rows(const oa4* u, const oa4* v, const oa4* w, const oa4* z, c
 /* Source/BoundaryConditions.cpp:849-876: 1 at the Dirichlet face points of eq at its location, else 0. */
 void oracle_dirichlet_rows(int eq, const obc* bc, const ogeom* g, oa4* mask);
 /* eq 0..4 = u, v, w, theta, k; dSdk may be null. */
-void oracle_implicit_folds(int eq, const oa4* ss, const oa4* dSdk, const oa4* J, const obc* bc, const osurface* so,
-                           const ogeom* g, oa4* A);
+void oracle_implicit_folds(int eq, const oa4* ss, const oa4* dSdk, const oa4* J, const oa4* zn, const obc* bc,
+                           const osurface* so, const ogeom* g, oa4* A);
 
 #ifdef __cplusplus
 }

@@ -64,9 +64,15 @@ class OProfile(C.Structure):
                 ("rz", C.POINTER(C.c_double)), ("rt", C.POINTER(C.c_double)), ("rn", C.c_int)]
 
 
+class ODamping(C.Structure):
+    _fields_ = [("rayleigh_zdamp", C.c_double), ("rayleigh_dampcoef", C.c_double), ("rayleigh_fields", C.c_int * 5),
+                ("rayleigh_above_ground", C.c_int), ("sponge_width", C.c_double), ("sponge_strength", C.c_double),
+                ("ztop", C.c_double), ("prob_lo", C.c_double * 2), ("prob_hi", C.c_double * 2)]
+
+
 class OBc(C.Structure):
     _fields_ = [("face", C.c_int * 6), ("zhi_theta_grad", C.c_double), ("wall_velocity", (C.c_double * 3) * 6),
-                ("ug", C.c_double), ("vg", C.c_double), ("profile", C.POINTER(OProfile))]
+                ("ug", C.c_double), ("vg", C.c_double), ("profile", C.POINTER(OProfile)), ("damping", ODamping)]
 
 
 _lib = None
@@ -148,7 +154,9 @@ def lib():
         L.oracle_dirichlet_rows.restype = None
         L.oracle_profile_eval.argtypes = [P(OProfile), D, P(D)]
         L.oracle_profile_eval.restype = None
-        L.oracle_implicit_folds.argtypes = [C.c_int, V, V, V, P(OBc), P(OSurface), P(OGeom), V]
+        L.oracle_implicit_folds.argtypes = [C.c_int, V, V, V, V, P(OBc), P(OSurface), P(OGeom), V]
+        L.oracle_add_damping.argtypes = [P(OA4) * 6, V, P(OBc), P(OGeom), P(OA4) * 6]
+        L.oracle_add_damping.restype = None
         L.oracle_implicit_folds.restype = None
         for f in ("oracle_omega_from_w", "oracle_w_from_omega", "oracle_transport_fluxes",
                   "oracle_velocity_divergence", "oracle_pressure_gradient"):

@@ -779,6 +779,71 @@ void oracle_apply_rows(const oa4* u, const oa4* v, const oa4* w, const oa4* z, c
     }
 }
 
+/* ---- the Rayleigh and sponge layers ------------------------------------- */
+
+static int damping_on(const obc* bc) { return bc->damping.rayleigh_zdamp > 0 || bc->damping.sponge_width > 0; }
+
+static double rayleigh_part(const odamping* d, double z)
+{
+    const double pi = 3.14159265358979323846;
+    if (d->rayleigh_zdamp > 0 && z > d->rayleigh_zdamp) {
+        const double s = sin(0.5 * pi * (z - d->rayleigh_zdamp) / (d->ztop - d->rayleigh_zdamp));
+        return d->rayleigh_dampcoef * s * s;
+    }
+    return 0;
+}
+
+static double sponge_part(const obc* bc, double x, double y)
+{
+    const odamping* d = &bc->damping;
+    double c = 0;
+    if (d->sponge_width > 0) {
+        const double inv_w = 1 / d->sponge_width;
+        const double dist[4] = {x - d->prob_lo[0], d->prob_hi[0] - x, y - d->prob_lo[1], d->prob_hi[1] - y};
+        const int lat[4] = {bc->face[0], bc->face[3], bc->face[1], bc->face[4]};
+        for (int n = 0; n < 4; ++n) {
+            if ((lat[n] == OFACE_INFLOW || lat[n] == OFACE_OUTFLOW) && dist[n] < d->sponge_width) {
+                const double r = 1 - dist[n] * inv_w;
+                c += d->sponge_strength * r * r;
+            }
+        }
+    }
+    return c;
+}
+
+void oracle_add_damping(oa4* const q[6], const oa4* z, const obc* bc, const ogeom* g, oa4* const R[6])
+{
+    if (!damping_on(bc)) { return; }
+    const int klo = g->lo[2];
+    const int ff[5] = {F_U, F_V, F_W, F_THETA, F_TKE};
+    const int qn[5] = {0, 1, 2, 4, 5};
+    for (int n = 0; n < 5; ++n) {
+        const int f = ff[n];
+        const int t = n < 3 ? (1 << n) : OS_CC;
+        const int with_rayleigh = bc->damping.rayleigh_fields[n];
+        const int vh[3] = {g->hi[0] + os_nodal(t, 0), g->hi[1] + os_nodal(t, 1), g->hi[2] + os_nodal(t, 2)};
+        for (int k = g->lo[2]; k <= vh[2]; ++k)
+            for (int j = g->lo[1]; j <= vh[1]; ++j)
+                for (int i = g->lo[0]; i <= vh[0]; ++i) {
+                    const double h = hag(z, t, i, j, k, klo), zc = point_z(z, t, i, j, k);
+                    const double x = bc->damping.prob_lo[0] + (i + (os_nodal(t, 0) ? 0.0 : 0.5)) * g->dx[0];
+                    const double y = bc->damping.prob_lo[1] + (j + (os_nodal(t, 1) ? 0.0 : 0.5)) * g->dx[1];
+                    const double zr = bc->damping.rayleigh_above_ground ? h : zc;
+                    const double cr = with_rayleigh ? rayleigh_part(&bc->damping, zr) : 0.0;
+                    const double cs = sponge_part(bc, x, y);
+                    const double a = AT(q[qn[n]], i, j, k);
+                    if (cr > 0) {
+                        const double ref = f == F_W ? 0.0 : profile_value(bc->profile, f, f == F_THETA ? zr : h);
+                        AT(R[qn[n]], i, j, k) += cr * (a - ref);
+                    }
+                    if (cs > 0) {
+                        const double ref = f == F_W ? 0.0 : profile_value(bc->profile, f, h);
+                        AT(R[qn[n]], i, j, k) += cs * (a - ref);
+                    }
+                }
+    }
+}
+
 void oracle_dirichlet_rows(int eq, const obc* bc, const ogeom* g, oa4* mask)
 {
     const int t = eq == 0 ? OS_XF : eq == 1 ? OS_YF : eq == 2 ? OS_ZF : OS_CC;
@@ -804,8 +869,8 @@ void oracle_dirichlet_rows(int eq, const obc* bc, const ogeom* g, oa4* mask)
 
 /* The wall model's rows and the z faces' folds of the implicit side A
  * (7 components at the field's location): eq 0..4 = u, v, w, theta, k. */
-void oracle_implicit_folds(int eq, const oa4* ss, const oa4* dSdk, const oa4* J, const obc* bc, const osurface* so,
-                           const ogeom* g, oa4* A)
+void oracle_implicit_folds(int eq, const oa4* ss, const oa4* dSdk, const oa4* J, const oa4* zn, const obc* bc,
+                           const osurface* so, const ogeom* g, oa4* A)
 {
     const int t = eq == 0 ? OS_XF : eq == 1 ? OS_YF : eq == 2 ? OS_ZF : OS_CC;
     const int f = eq == 0 ? F_U : eq == 1 ? F_V : eq == 2 ? F_W : eq == 3 ? F_THETA : F_TKE;
@@ -845,8 +910,22 @@ void oracle_implicit_folds(int eq, const oa4* ss, const oa4* dSdk, const oa4* J,
                 }
         }
     }
-    /* apply_to_implicit (BoundaryConditions.cpp:885-940) on every non-periodic face */
+    /* apply_to_implicit (BoundaryConditions.cpp:885-940): the damping layers'
+     * diagonal first, then every non-periodic face */
     const int vh[3] = {g->hi[0] + os_nodal(t, 0), g->hi[1] + os_nodal(t, 1), g->hi[2] + os_nodal(t, 2)};
+    if (damping_on(bc) && zn) {
+        const int with_rayleigh = bc->damping.rayleigh_fields[eq];
+        for (int k = g->lo[2]; k <= vh[2]; ++k)
+            for (int j = g->lo[1]; j <= vh[1]; ++j)
+                for (int i = g->lo[0]; i <= vh[0]; ++i) {
+                    const double h = hag(zn, t, i, j, k, k0), zc = point_z(zn, t, i, j, k);
+                    const double x = bc->damping.prob_lo[0] + (i + (os_nodal(t, 0) ? 0.0 : 0.5)) * g->dx[0];
+                    const double y = bc->damping.prob_lo[1] + (j + (os_nodal(t, 1) ? 0.0 : 0.5)) * g->dx[1];
+                    AC(i, j, k, OC_P) += (with_rayleigh ? rayleigh_part(&bc->damping,
+                                                                        bc->damping.rayleigh_above_ground ? h : zc)
+                                                        : 0.0) + sponge_part(bc, x, y);
+                }
+    }
     for (int d = 0; d < 3; ++d) {
         if (d < 2 && g->periodic[d]) { continue; }
         for (int lo = 1; lo >= 0; --lo) {

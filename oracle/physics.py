@@ -102,6 +102,7 @@ class Boundary:
     yhi: int = PERIODIC
     side_wall_velocity: tuple = ((0.0,) * 3,) * 4                # xlo, ylo, xhi, yhi
     profile: object = None
+    damping: object = None                                        # Damping (off when None)
 
     def faces(self):
         return (self.xlo, self.ylo, self.zlo, self.xhi, self.yhi, self.zhi)
@@ -118,12 +119,36 @@ class Boundary:
             for d in range(3):
                 b.wall_velocity[o][d] = vel[d]
         b.ug, b.vg = forcing.ug, forcing.vg
+        if self.damping is not None:
+            d = self.damping
+            b.damping.rayleigh_zdamp, b.damping.rayleigh_dampcoef = d.rayleigh_zdamp, d.rayleigh_dampcoef
+            for n, name in enumerate(("u", "v", "w", "theta", "tke")):
+                b.damping.rayleigh_fields[n] = 1 if name in d.fields else 0
+            b.damping.rayleigh_above_ground = 1 if d.above_ground else 0
+            b.damping.sponge_width, b.damping.sponge_strength = d.sponge_width, d.sponge_strength
+            b.damping.ztop = d.ztop
+            for k in range(2):
+                b.damping.prob_lo[k], b.damping.prob_hi[k] = d.prob_lo[k], d.prob_hi[k]
         if self.profile is not None:
             from .initializer import c_profile
             pc, keep = c_profile(self.profile)
             b.profile = C.pointer(pc)
             b._keep = (pc, keep)
         return b
+
+
+@dataclass
+class Damping:
+    """The Rayleigh and sponge layers (Source/BoundaryConditions.H:107-121)."""
+    rayleigh_zdamp: float = 0.0
+    rayleigh_dampcoef: float = 0.0
+    fields: tuple = ("u", "v", "w", "theta", "tke")
+    above_ground: bool = False
+    sponge_width: float = 0.0
+    sponge_strength: float = 0.0
+    ztop: float = 0.0
+    prob_lo: tuple = (0.0, 0.0)
+    prob_hi: tuple = (0.0, 0.0)
 
 
 @dataclass
@@ -277,6 +302,11 @@ def residual(q, mesh: Mesh, phys: Physics):
         orc.lib().oracle_apply_wall_k(_v(ss, (-1, -1, g.lo[2])), _v(q["tke"], B2),
                                       int(phys.surface.wall_k_dirichlet), g, _v(R["tke"], lo))
     if phys.bc is not None:
+        names = ("u", "v", "w", "p", "theta", "tke")
+        qv = (C.POINTER(orc.OA4) * 6)(*[C.pointer(orc.a4(q[n], B2)) for n in names])
+        rviews = [orc.a4(R[n], lo) for n in names]
+        rv = (C.POINTER(orc.OA4) * 6)(*[C.pointer(x) for x in rviews])
+        orc.lib().oracle_add_damping(qv, _v(mesh.z, B1), phys.bc.c(phys.forcing), g, rv)
         orc.lib().oracle_apply_rows(_v(q["u"], B2), _v(q["v"], B2), _v(q["w"], B2), _v(mesh.z, B1),
                                     phys.bc.c(phys.forcing), g, _v(R["u"], lo), _v(R["v"], lo), _v(R["w"], lo))
     # OverrideSync: the hi faces of a periodic direction take the lo faces' values
@@ -317,5 +347,6 @@ def implicit_rows(eq, F, Fb, mesh: Mesh, phys: Physics, cf=None, ss=None):
         sv = None
         if phys.bc is not None and phys.bc.zlo == SURFACE_LAYER:
             raise orc.OracleError("implicit_rows: a surface_layer ground needs the surface options")
-    orc.lib().oracle_implicit_folds(eq, sv, _v(dS, tuple(g.lo)), _v(mesh.J, B1), bcc, so, g, _v(A, tuple(g.lo)))
+    orc.lib().oracle_implicit_folds(eq, sv, _v(dS, tuple(g.lo)), _v(mesh.J, B1), _v(mesh.z, B1), bcc, so, g,
+                                    _v(A, tuple(g.lo)))
     return A

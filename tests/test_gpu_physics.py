@@ -172,6 +172,12 @@ def _open_inputs(phys_o: P.Physics):
         wv[o] = tuple(v)
     gi.wall_velocity = tuple(wv)
     gi.inflow = _gprofile(b.profile) if b.profile is not None else None
+    if b.damping is not None:
+        from ablsteadyrans_amd import physics as GP
+        d = b.damping
+        gi.damping = GP.Damping(rayleigh_zdamp=d.rayleigh_zdamp, rayleigh_dampcoef=d.rayleigh_dampcoef,
+                                fields=d.fields, above_ground=d.above_ground, sponge_width=d.sponge_width,
+                                sponge_strength=d.sponge_strength, ztop=d.ztop, prob_lo=d.prob_lo, prob_hi=d.prob_hi)
     return gi
 
 
@@ -243,6 +249,42 @@ def test_open_side_residual_and_implicit_rows_match(cuda, case):
     Fd = [_fab(f, Fb) for f in F]
     for eq_o, eq_g, loc in ((P.EQ_U, GP.EQ_U, 1), (P.EQ_V, GP.EQ_V, 2), (P.EQ_W, GP.EQ_W, 4),
                             (P.EQ_THETA, GP.EQ_THETA, 0), (P.EQ_TKE, GP.EQ_TKE, 0)):
+        A_ref = P.implicit_rows(eq_o, F, Fb, m, phys, cf_ref, ss_ref)
+        A = Fab.at_loc(tile.geom, loc, 7, 0)
+        GP.implicit_rows(eq_g, dm, Fd, gi, A, cf, ss)
+        _close(A.numpy(), A_ref, 1e-12, f"A[{eq_o}]")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("above_ground", [False, True])
+def test_damping_layers_match(cuda, above_ground):
+    """The Rayleigh layer (w and theta, absolute or above-ground height) and
+    the sponge of the inflow/outflow faces in the residual and the rows."""
+    import torch
+    from ablsteadyrans_amd import physics as GP
+    from ablsteadyrans_amd.fab import Fab
+    from ablsteadyrans_amd.poisson import DeviceMesh
+    tile, m, q, phys = _open_setup(0, seed=13)
+    ztop = float(tile.zlevels[-1])
+    phys.bc.damping = P.Damping(rayleigh_zdamp=0.45 * ztop, rayleigh_dampcoef=0.2, fields=("w", "theta", "u"),
+                                above_ground=above_ground, ztop=ztop, sponge_width=140.0,
+                                sponge_strength=0.05, prob_lo=(0.0, 0.0), prob_hi=(700.0, 600.0))
+    P.fill_ghosts(q, m, phys)
+    R_ref, cf_ref, ss_ref = P.residual(q, m, phys)
+    gi = _open_inputs(phys)
+    dm = DeviceMesh(tile)
+    ds = _dstate(tile, q)
+    R = {n: Fab.at_loc(tile.geom, loc, 1, 0) for n, loc in zip(GP.FIELDS, GP.LOCS)}
+    cf = GP.ClosureFields(tile.geom)
+    ss = GP.surface_slab(tile.geom)
+    GP.residual(dm, ds, gi, R, cf, ss)
+    torch.cuda.synchronize()
+    for n in GP.FIELDS:
+        _close(R[n].numpy(), R_ref[n], 1e-11, n)
+    Fx, Fy, Fz, Fb = orc.transport_fluxes(q["u"], q["v"], q["w"], B2, m.z, B1, m.ax, m.ay, B1, m.geom)
+    F = [Fx[None], Fy[None], Fz[None]]
+    Fd = [_fab(f, Fb) for f in F]
+    for eq_o, eq_g, loc in ((P.EQ_U, GP.EQ_U, 1), (P.EQ_W, GP.EQ_W, 4), (P.EQ_THETA, GP.EQ_THETA, 0)):
         A_ref = P.implicit_rows(eq_o, F, Fb, m, phys, cf_ref, ss_ref)
         A = Fab.at_loc(tile.geom, loc, 7, 0)
         GP.implicit_rows(eq_g, dm, Fd, gi, A, cf, ss)

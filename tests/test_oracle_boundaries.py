@@ -188,3 +188,73 @@ def test_inflow_without_a_profile_is_refused():
     phys.bc.profile = None
     with pytest.raises(orc.OracleError, match="inflow profile"):
         P.fill_ghosts(q, m, phys)
+
+
+def _point_z(m, loc, I, J, K):
+    """The node-mean height of points of type loc (Heights_K.H:55-68)."""
+    nx_ = 1 if loc & 1 else 2
+    ny_ = 1 if loc & 2 else 2
+    nz_ = 1 if loc & 4 else 2
+    zp = np.zeros(np.shape(I))
+    for a in range(nx_):
+        for b in range(ny_):
+            for c in range(nz_):
+                zp = zp + OI.node_z(m.z, B1, I + a, J + b, K + c)
+    return zp / (nx_ * ny_ * nz_)
+
+
+def test_rayleigh_and_sponge_layers():
+    """R gains c (q - q_ref) with c = coef sin^2(pi/2 (z - zd)/(ztop - zd))
+    above zd (for the fields named) plus strength (1 - s/width)^2 within the
+    width of the inflow and outflow faces; q_ref the profile (w: 0, theta's
+    Rayleigh reference the sounding at the absolute height); the implicit
+    diagonal gains c."""
+    import copy
+    tile, m, q, phys = _filled()
+    nx, ny, nz = tile.geom.n
+    ztop = float(tile.zlevels[-1])
+    d = P.Damping(rayleigh_zdamp=180.0, rayleigh_dampcoef=0.2, fields=("w", "theta"), ztop=ztop, sponge_width=130.0,
+                  sponge_strength=0.05, prob_lo=(0.0, 0.0), prob_hi=(600.0, 500.0))
+    phys_d = copy.deepcopy(phys)
+    phys_d.bc.damping = d
+    R0, cf, ss = P.residual(q, m, phys)
+    Rd, _, _ = P.residual(q, m, phys_d)
+
+    def coef(loc, I, J, K, rayleigh):
+        zc = _point_z(m, loc, I, J, K)
+        s = np.sin(0.5 * np.pi * (zc - 180.0) / (ztop - 180.0))
+        cr = np.where(zc > 180.0, 0.2 * s * s, 0.0) if rayleigh else 0.0
+        x = (I + (0.0 if loc & 1 else 0.5)) * tile.geom.dx[0]
+        cs = np.zeros(np.shape(I))
+        for dist in (x, 600.0 - x):   # the inflow and outflow faces
+            r = 1 - dist / 130.0
+            cs = cs + np.where(dist < 130.0, 0.05 * r * r, 0.0)
+        return cr, cs, zc
+
+    K, J, I = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    cr, cs, zc = coef(0, I, J, K, True)
+    h = _h(m, 0, I, J, K)
+    th = q["theta"][0, 2:-2, 2:-2, 2:-2]
+    ref_r = OI.table_interp(PROF.theta_ref_z, PROF.theta_ref, zc)
+    ref_s = OI.inflow_eval(PROF, h)[2]
+    assert np.allclose(Rd["theta"][0] - R0["theta"][0], cr * (th - ref_r) + cs * (th - ref_s), rtol=1e-10, atol=1e-13)
+    k = q["tke"][0, 2:-2, 2:-2, 2:-2]
+    assert np.allclose(Rd["tke"][0] - R0["tke"][0], cs * (k - OI.inflow_eval(PROF, h)[3]), rtol=1e-10, atol=1e-13)
+    Kw, Jw, Iw = np.meshgrid(np.arange(1, nz), np.arange(ny), np.arange(nx), indexing="ij")
+    crw, csw, _ = coef(4, Iw, Jw, Kw, True)
+    w = q["w"][0, 3:-3, 2:-2, 2:-2]
+    assert np.allclose(Rd["w"][0, 1:-1] - R0["w"][0, 1:-1], (crw + csw) * w, rtol=1e-10, atol=1e-13)
+    # the diagonal of theta's rows
+    Fx, Fy, Fz, Fb = orc.transport_fluxes(q["u"], q["v"], q["w"], B2, m.z, B1, m.ax, m.ay, B1, m.geom)
+    F = [Fx[None], Fy[None], Fz[None]]
+    A0 = P.implicit_rows(P.EQ_THETA, F, Fb, m, phys, cf, ss)
+    Ad = P.implicit_rows(P.EQ_THETA, F, Fb, m, phys_d, cf, ss)
+    inner = (slice(None), slice(None), slice(1, -1))
+    assert np.allclose((Ad[COEF_P] - A0[COEF_P])[inner], (cr + cs)[inner], rtol=1e-12, atol=1e-15)
+    # u has no Rayleigh (not named): only the sponge; its inflow face keeps the identity row
+    Au0 = P.implicit_rows(P.EQ_U, F, Fb, m, phys, cf, ss)
+    Aud = P.implicit_rows(P.EQ_U, F, Fb, m, phys_d, cf, ss)
+    assert np.all(Aud[COEF_P, :, :, 0] == 1)
+    Ku, Ju, Iu = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(1, nx), indexing="ij")
+    _, csu, _ = coef(1, Iu, Ju, Ku, False)
+    assert np.allclose((Aud[COEF_P] - Au0[COEF_P])[:, :, 1:nx], csu, rtol=1e-12, atol=1e-15)
