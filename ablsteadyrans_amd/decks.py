@@ -1,0 +1,180 @@
+"""The reference's steady test decks as ready configurations of this
+library: the mesh, the residual inputs, the initializer's profile and
+options, the stage and driver settings of each inputs file, and the
+reference's measured iteration count.
+
+* :func:`neutral_column` -- Tests/Canonical_RANS/Neutral_Column/inputs_steady_column
+* :func:`neutral_abl_flat` -- Tests/Canonical_RANS/Neutral_ABL_Flat/inputs_steady_neutral_flat
+* :func:`neutral_hill_3d` -- Tests/Canonical_RANS/Neutral_Hill_3D/inputs_steady_hill3d
+* :func:`askervein` -- Tests/Real_Terrain/Askervein/inputs_steady_askervein
+  (the terrain grid, extern_erf/askervein.txt, is the caller's)
+
+:func:`prepare` builds the device mesh, the state at the deck's first guess
+(the mass-consistent initializer) and the stage; :meth:`Deck.solve` runs the
+steady driver.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+from . import initializer as GI
+from . import mesh as M
+from . import physics as GP
+from . import poisson as PS
+from . import reductions as RD
+
+
+@dataclass
+class Deck:
+    name: str
+    tile: object
+    inputs: GP.ResidualInputs
+    profile: GI.InflowProfile
+    init: GI.InitOptions
+    stage: GP.SimplecOptions
+    driver: GP.DriverOptions
+    iterations_reference: int
+    seconds_reference: float | None = None
+    ranks_reference: int | None = None
+    init_reltol: float = 1e-10
+    state: object = field(default=None, repr=False)
+    mesh: object = field(default=None, repr=False)
+    simplec: object = field(default=None, repr=False)
+
+    def solve(self):
+        """Driver::run from the prepared state (Source/Driver.cpp:403-492)."""
+        return self.simplec.solve(self.state, self.driver)
+
+
+def _height(tile, prob_lo_z, prob_hi_z):
+    return RD.residual_scale_height(tile.zlevels, prob_lo_z, prob_hi_z)
+
+
+def neutral_column() -> Deck:
+    """4 x 4 x 64 over 2560 x 2560 x 1000 m, periodic sides, surface layer
+    z0 = 0.1 m, slip top, uniform theta, the force F = 1.225e-4 m/s^2
+    (u* = 0.35 m/s), the uncapped closure with the logarithmic face mean, two
+    sweeps, CFL 1 -> 50, steady at residual_tol 1e-8 and a 6-order drop; the
+    first guess the log law with u* = 0.25 m/s and its equilibrium k (no
+    projection: flat). The reference measured 2231 iterations."""
+    geom = M.geometry((4, 4, 64), (0.0, 0.0, 0.0), (2560.0, 2560.0, 1000.0), (1, 1))
+    tile = M.build_mesh(geom, (0.0, 0.0, 0.0), (2560.0, 2560.0, 1000.0), M.make_zlevels(64, 1000.0),
+                        M.Terrain(kind="flat"))
+    inputs = GP.ResidualInputs(closure=GP.ClosureOptions(max_geom_lscale=1.0e12),
+                               surface=GP.SurfaceLayerOptions(z0=0.1),
+                               forcing=GP.ForcingOptions(pressure_gradient=(1.225e-4, 0.0, 0.0)),
+                               log_face_mean=True)
+    return Deck("Neutral_Column", tile, inputs, GI.InflowProfile(kind=GI.LOG_LAW, ustar=0.25, z0=0.1, unbounded=True),
+                GI.InitOptions(first_guess=1, projection=False), GP.SimplecOptions(sweeps=2, height=1000.0),
+                GP.DriverOptions(max_iterations=20000, cfl0=1.0, cfl_max=50.0, residual_tol=1.0e-8,
+                                 residual_drop_orders=6.0), 2231)
+
+
+def neutral_abl_flat() -> Deck:
+    """8 x 8 x 64 over 2560 x 2560 x 1000 m, periodic sides, surface layer
+    z0 = 0.1 m, slip top with 0.03 K/m above the sounding (300 K to 700 m,
+    309 K at 1000 m), the geostrophic wind (10, 0) m/s at latitude 90
+    (horizontal Coriolis terms), the closure's defaults with a k floor of
+    1e-6, four sweeps, CFL cap 2, the residual reference over 100
+    iterations, a 2-order drop; the first guess the log law (u* = 0.35 m/s,
+    z0 = 0.1 m, the wall-law length). The reference measured about 2850
+    iterations (13 s on two ranks)."""
+    geom = M.geometry((8, 8, 64), (0.0, 0.0, 0.0), (2560.0, 2560.0, 1000.0), (1, 1))
+    tile = M.build_mesh(geom, (0.0, 0.0, 0.0), (2560.0, 2560.0, 1000.0), M.make_zlevels(64, 1000.0),
+                        M.Terrain(kind="flat"))
+    sounding = dict(theta_ref_z=(0.0, 700.0, 1000.0), theta_ref=(300.0, 300.0, 309.0))
+    inputs = GP.ResidualInputs(closure=GP.ClosureOptions(tke_floor=1e-6), surface=GP.SurfaceLayerOptions(z0=0.1),
+                               forcing=GP.ForcingOptions(coriolis=True, coriolis_3d=False, latitude=90.0,
+                                                         rotation_period=125663.7061435917, geostrophic=True,
+                                                         ug=10.0, vg=0.0),
+                               zhi_theta_grad=0.03, **sounding)
+    prof = GI.InflowProfile(kind=GI.LOG_LAW, ustar=0.35, z0=0.1, unbounded=True, **sounding)
+    return Deck("Neutral_ABL_Flat", tile, inputs, prof, GI.InitOptions(first_guess=1),
+                GP.SimplecOptions(sweeps=4, tke_floor=1e-6, height=1000.0),
+                GP.DriverOptions(max_iterations=20000, cfl0=1.0, cfl_max=2.0, residual_ref_iterations=100,
+                                 residual_drop_orders=2.0), 2850, 13.0, 2)
+
+
+def neutral_hill_3d() -> Deck:
+    """64 x 64 x 20 over 5120 x 5120 x 800 m, periodic sides, the round
+    Witch-of-Agnesi hill (100 m, half width 500 m), the surface layer (z0 =
+    0.1 m), a slip top with 0.03 K/m over the sounding, Rayleigh damping of w
+    and theta above 550 m (0.2 1/s), the geostrophic wind (10, 0) m/s at
+    latitude 90, the closure's defaults with a k floor of 1e-6, sixteen
+    sweeps, CFL cap 4, rotation_relax 1, the residual reference over 100
+    iterations, a 2-order drop; the first guess the log law (u* = 0.35 m/s,
+    z0 = 0.1 m) made divergence-free. The reference measured 2092 iterations
+    (148 s on four ranks)."""
+    lo, hi = (-2560.0, -2560.0, 0.0), (2560.0, 2560.0, 800.0)
+    geom = M.geometry((64, 64, 20), lo, hi, (1, 1))
+    zl = M.make_zlevels(20, 800.0)
+    tile = M.build_mesh(geom, lo, hi, zl, M.Terrain(kind="agnesi", hill_height=100.0, hill_half_width=500.0))
+    sounding = dict(theta_ref_z=(0.0, 700.0, 1000.0), theta_ref=(300.0, 300.0, 309.0))
+    prof = GI.InflowProfile(kind=GI.LOG_LAW, ustar=0.35, z0=0.1, **sounding)
+    inputs = GP.ResidualInputs(closure=GP.ClosureOptions(tke_floor=1e-6), surface=GP.SurfaceLayerOptions(z0=0.1),
+                               forcing=GP.ForcingOptions(coriolis=True, coriolis_3d=False, latitude=90.0,
+                                                         rotation_period=125663.7061435917, geostrophic=True,
+                                                         ug=10.0, vg=0.0),
+                               zhi_theta_grad=0.03, inflow=prof,
+                               damping=GP.Damping(rayleigh_zdamp=550.0, rayleigh_dampcoef=0.2, fields=("w", "theta"),
+                                                  ztop=float(zl[-1]), prob_lo=lo[:2], prob_hi=hi[:2]),
+                               **sounding)
+    return Deck("Neutral_Hill_3D", tile, inputs, prof, GI.InitOptions(first_guess=1),
+                GP.SimplecOptions(sweeps=16, tke_floor=1e-6, height=_height(tile, 0.0, 800.0), rotation_relax=1.0),
+                GP.DriverOptions(max_iterations=20000, cfl0=1.0, cfl_max=4.0, residual_ref_iterations=100,
+                                 residual_drop_orders=2.0), 2092, 148.0, 4)
+
+
+ASKERVEIN_FACES = (GP.INFLOW, GP.SLIP_WALL, GP.SURFACE_LAYER, GP.OUTFLOW, GP.SLIP_WALL, GP.SLIP_WALL)
+
+
+def askervein_tile(grid: M.ErfGrid):
+    """30 x 30 cells of 200 m, 18 layers from 20 m growing by 1.08 (the
+    computational top 440 m), ERF's Askervein grid, no smoothing."""
+    lo, hi = (-3000.0, -3000.0, 0.0), (3000.0, 3000.0, 440.0)
+    geom = M.geometry((30, 30, 18), lo, hi)
+    zl = M.make_zlevels(18, 440.0, dz0=20.0, stretch_ratio=1.08)
+    return M.build_mesh(geom, lo, hi, zl, M.Terrain(kind="erf_grid", grid=grid))
+
+
+def askervein(grid: M.ErfGrid) -> Deck:
+    """The Askervein mesh, inflow at xlo (the log law, u* = 0.6 m/s, z0 =
+    0.03 m, the closure's length cap), outflow at xhi, slip walls along y,
+    the surface layer (z0 = 0.03 m), a slip top, no gravity, a k floor of
+    1e-6, four sweeps, CFL cap 2, the residual reference over 100
+    iterations, the default 4-order drop; the mass-consistent first guess
+    with alpha_v = 0.5. The reference measured 175 iterations (2 s on two
+    ranks)."""
+    tile = askervein_tile(grid)
+    prof = GI.InflowProfile(kind=GI.LOG_LAW, ustar=0.6, z0=0.03, direction=0.0)
+    inputs = GP.ResidualInputs(closure=GP.ClosureOptions(tke_floor=1e-6), surface=GP.SurfaceLayerOptions(z0=0.03),
+                               forcing=GP.ForcingOptions(gravity=0.0), faces=ASKERVEIN_FACES, inflow=prof)
+    return Deck("Askervein", tile, inputs, prof, GI.InitOptions(first_guess=1, alpha_h=1.0, alpha_v=0.5),
+                GP.SimplecOptions(sweeps=4, tke_floor=1e-6, height=_height(tile, 0.0, 440.0)),
+                GP.DriverOptions(max_iterations=3000, cfl0=1.0, cfl_max=2.0, residual_ref_iterations=100), 175,
+                2.0, 2)
+
+
+def prepare(deck: Deck, precision: int = 0) -> Deck:
+    """The device mesh, the state at the deck's first guess and the stage
+    (precision 1: FP32 multigrid V-cycles in every pressure correction)."""
+    deck.mesh = PS.DeviceMesh(deck.tile)
+    deck.state = GP.DeviceState(deck.tile.geom)
+    faces = deck.inputs.faces
+    if deck.init.projection:
+        solver = PS.PoissonSolver(deck.mesh, PS.PoissonOptions(reltol=deck.init_reltol))
+        GI.initialize_state(deck.mesh, deck.profile, deck.init, faces, solver, deck.state)
+    else:
+        GI.first_guess_state(deck.mesh, deck.profile, deck.init, deck.state)
+    deck.stage.poisson_precision = precision
+    deck.simplec = GP.SimplecStage(deck.mesh, deck.inputs, deck.stage)
+    return deck
+
+
+def reinitialize(deck: Deck) -> None:
+    """The state back at the deck's first guess (the stage and mesh kept)."""
+    if deck.init.projection:
+        solver = PS.PoissonSolver(deck.mesh, PS.PoissonOptions(reltol=deck.init_reltol))
+        GI.initialize_state(deck.mesh, deck.profile, deck.init, deck.inputs.faces, solver, deck.state)
+    else:
+        GI.first_guess_state(deck.mesh, deck.profile, deck.init, deck.state)

@@ -13,7 +13,10 @@ k), the driver loop on the device (gust_steady_solve); the state is resident
 in HBM before the timed region. The mixed form (FP32 multigrid V-cycles
 under the FP64 GMRES of every pressure correction) is timed the same way.
 
-Also measured (extra): one SIMPLEC iteration on the Gaussian-hill tile
+Also measured (extra): the reference's own steady decks with published CPU
+times (Askervein, Neutral_ABL_Flat, Neutral_Hill_3D: iterations to the
+verdict and wall time, beside the reference's); one SIMPLEC iteration on the
+Gaussian-hill tile
 (200 x 200 x 64, dz1 = 1 m, FP64 and mixed); the z-line Jacobi sweep of the
 five variables on the fractal-terrain tile (256 x 256 x 96, 65,536 columns;
 the roofline object: algorithmic bytes per launch / launch time from HIP
@@ -214,48 +217,56 @@ def _vcycle_bytes(s, pre=2, post=2, coarse=16):
 COLUMN_ITERATIONS_REF = 2231   # Tests/Canonical_RANS/Neutral_Column/README.md:26
 
 
-def _column_deck():
-    """The Neutral_Column deck: (tile, residual inputs, simplec options,
-    driver options, first-guess profile)."""
-    from ablsteadyrans_amd import initializer as GI
+def _askervein_grid():
+    """extern_erf/askervein.txt (the reference's terrain grid), as the test
+    fixture holds it."""
+    import numpy as np
     from ablsteadyrans_amd import mesh as M
-    from ablsteadyrans_amd import physics as GP
-    geom = M.geometry((4, 4, 64), (0.0, 0.0, 0.0), (2560.0, 2560.0, 1000.0), (1, 1))
-    tile = M.build_mesh(geom, (0.0, 0.0, 0.0), (2560.0, 2560.0, 1000.0), M.make_zlevels(64, 1000.0),
-                        M.Terrain(kind="flat"))
-    inputs = GP.ResidualInputs(closure=GP.ClosureOptions(max_geom_lscale=1.0e12),
-                               surface=GP.SurfaceLayerOptions(z0=0.1),
-                               forcing=GP.ForcingOptions(pressure_gradient=(1.225e-4, 0.0, 0.0)),
-                               log_face_mean=True)
-    dopt = GP.DriverOptions(max_iterations=20000, cfl0=1.0, cfl_max=50.0, residual_tol=1.0e-8,
-                            residual_drop_orders=6.0)
-    prof = GI.InflowProfile(kind=GI.LOG_LAW, ustar=0.25, z0=0.1, unbounded=True)
-    return tile, inputs, dopt, prof
+    d = np.load(os.path.join(ROOT, "tests", "golden", "askervein_grid.npz"))
+    return M.ErfGrid(d["x"], d["y"], d["z"])
 
 
 def _time_solves(precision, steps, warmup):
     """Whole steady solves of the column deck from its first guess: returns
-    (seconds per solve, iterations, status, u*)."""
+    (seconds per solve, iterations, status, u*, the solves' times)."""
     import torch
-    from ablsteadyrans_amd import initializer as GI
-    from ablsteadyrans_amd import physics as GP
-    from ablsteadyrans_amd.poisson import DeviceMesh
-    tile, inputs, dopt, prof = _column_deck()
-    dm = DeviceMesh(tile)
-    stage = GP.SimplecStage(dm, inputs, GP.SimplecOptions(sweeps=2, height=1000.0, poisson_precision=precision))
-    q = GP.DeviceState(tile.geom)
-    opt = GI.InitOptions(first_guess=1, projection=False)
+    from ablsteadyrans_amd import decks as D
+    deck = D.prepare(D.neutral_column(), precision)
     times, last = [], None
     for rep in range(warmup + steps):
-        GI.first_guess_state(dm, prof, opt, q)
+        D.reinitialize(deck)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        r = stage.solve(q, dopt)
+        r = deck.solve()
         torch.cuda.synchronize()
         if rep >= warmup:
             times.append(time.perf_counter() - t0)
         last = r
     return sum(times) / len(times), int(last.iterations), int(last.status), float(last.mon[2]), times
+
+
+def _time_reference_decks():
+    """The reference's own steady decks with published CPU times (Askervein:
+    175 iterations, 2 s on two ranks; Neutral_ABL_Flat: about 2850
+    iterations, 13 s on two ranks; Neutral_Hill_3D: 2092 iterations, 148 s
+    on four ranks), each from its initializer state."""
+    import torch
+    from ablsteadyrans_amd import decks as D
+    out = []
+    for deck in (D.askervein(_askervein_grid()), D.neutral_abl_flat(), D.neutral_hill_3d()):
+        D.prepare(deck)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = deck.solve()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out.append({"deck": deck.name, "cells": list(deck.tile.geom.n), "status": int(r.status),
+                    "iterations": int(r.iterations), "seconds": round(dt, 3),
+                    "iterations_reference": deck.iterations_reference, "seconds_reference": deck.seconds_reference,
+                    "reference_ranks": deck.ranks_reference,
+                    "speedup_vs_reference": round(deck.seconds_reference / dt, 2),
+                    "ms_per_iteration": round(dt * 1e3 / max(int(r.iterations), 1), 3)})
+    return out
 
 
 def _hill_tile():
@@ -324,7 +335,8 @@ def _cpu_column_baseline(target_s=15.0):
     from oracle import physics as P
     from tests import physics_helpers as PHY
     from tests import stage_helpers as STH
-    tile, _, _, _ = _column_deck()
+    from ablsteadyrans_amd import decks as D
+    tile = D.neutral_column().tile
     m = PHY.omesh(tile)
     g = m.geom
     q = PHY.zeros_state(tile.geom)
@@ -415,7 +427,11 @@ def main():
     print("bench: SIMPLEC iterations on the 200x200x64 hill tile", file=sys.stderr, flush=True)
     hill64 = _time_hill_iterations(0, 10)
     hill32 = _time_hill_iterations(1, 10)
-    # 3. the z-line sweep (the roofline kernel) and the V-cycle on the fractal tile
+    # 3. the reference's own steady decks with published CPU times
+    print("bench: the reference's steady decks (Askervein, Neutral_ABL_Flat, Neutral_Hill_3D)", file=sys.stderr,
+          flush=True)
+    decks = _time_reference_decks()
+    # 4. the z-line sweep (the roofline kernel) and the V-cycle on the fractal tile
     print("bench: z-line sweeps and V-cycles on the 256x256x96 fractal tile", file=sys.stderr, flush=True)
     sys64 = _systems(torch.float64, 20264203)
     plan64 = G.JacobiPlan(sys64)
@@ -476,6 +492,7 @@ def main():
                   "vcycle_levels": levels,
                   "fp64_vcycle_ms": round(vc64, 4), "fp64_vcycle_gbs": round(_vcycle_bytes(8) / (vc64 * 1e-3) / 1e9, 1),
                   "fp32_vcycle_ms": round(vc32, 4), "fp32_vcycle_gbs": round(_vcycle_bytes(4) / (vc32 * 1e-3) / 1e9, 1),
+                  "reference_decks": decks,
                   "cpu_sweep_baseline": cpu_sweep,
                   "note": "p99 needs the serving loop (not built)"},
     }

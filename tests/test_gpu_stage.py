@@ -147,35 +147,6 @@ def test_neutral_column_matches_the_reference_deck(cuda):
     assert runs[1][0].iterations == r.iterations
 
 
-def _reference_abl_flat():
-    """Tests/Canonical_RANS/Neutral_ABL_Flat/inputs_steady_neutral_flat: 8 x 8
-    x 64 over 2560 x 2560 x 1000 m, periodic sides, surface layer z0 = 0.1 m,
-    slip top with 0.03 K/m above the sounding (300 K to 700 m, 309 K at
-    1000 m), the geostrophic wind (10, 0) m/s at latitude 90 with f = 1e-4
-    1/s (horizontal Coriolis terms only), the closure's defaults, k floor
-    1e-6; the first guess the log law (u* = 0.35 m/s, z0 = 0.1 m, the wall-law
-    length) made divergence-free."""
-    from ablsteadyrans_amd import initializer as GI
-    from ablsteadyrans_amd import mesh as M
-    from ablsteadyrans_amd import physics as GP
-    from ablsteadyrans_amd import poisson as PS
-    geom = M.geometry((8, 8, 64), (0.0, 0.0, 0.0), (2560.0, 2560.0, 1000.0), (1, 1))
-    tile = M.build_mesh(geom, (0.0, 0.0, 0.0), (2560.0, 2560.0, 1000.0), M.make_zlevels(64, 1000.0),
-                        M.Terrain(kind="flat"))
-    sounding = dict(theta_ref_z=(0.0, 700.0, 1000.0), theta_ref=(300.0, 300.0, 309.0))
-    inputs = GP.ResidualInputs(closure=GP.ClosureOptions(tke_floor=1e-6), surface=GP.SurfaceLayerOptions(z0=0.1),
-                               forcing=GP.ForcingOptions(coriolis=True, coriolis_3d=False, latitude=90.0,
-                                                         rotation_period=125663.7061435917, geostrophic=True,
-                                                         ug=10.0, vg=0.0),
-                               zhi_theta_grad=0.03, **sounding)
-    dm = PS.DeviceMesh(tile)
-    q = GP.DeviceState(tile.geom)
-    prof = GI.InflowProfile(kind=GI.LOG_LAW, ustar=0.35, z0=0.1, unbounded=True, **sounding)
-    solver = PS.PoissonSolver(dm, PS.PoissonOptions(reltol=1e-10))
-    GI.initialize_state(dm, prof, GI.InitOptions(first_guess=1), inputs.faces, solver, q)
-    return tile, dm, inputs, q
-
-
 @pytest.mark.gpu
 def test_neutral_abl_flat_matches_the_reference_deck(cuda):
     """The reference's rotating, stratified flat ABL (Tests/Canonical_RANS/
@@ -184,10 +155,9 @@ def test_neutral_abl_flat_matches_the_reference_deck(cuda):
     5 % of u*^2/Cmu0^2): four z-line Jacobi sweeps, CFL cap 2, the residual
     reference over the first 100 iterations."""
     from ablsteadyrans_amd import physics as GP
-    tile, dm, inputs, q = _reference_abl_flat()
-    stage = GP.SimplecStage(dm, inputs, GP.SimplecOptions(sweeps=4, tke_floor=1e-6, height=1000.0))
-    dopt = GP.DriverOptions(max_iterations=20000, cfl0=1.0, cfl_max=2.0, residual_ref_iterations=100,
-                            residual_drop_orders=2.0)
+    from ablsteadyrans_amd import decks as D
+    deck = D.prepare(D.neutral_abl_flat())
+    stage, q, dopt = deck.simplec, deck.state, deck.driver
     r = stage.solve(q, dopt)
     ustar = r.mon[2]
     print(f"steady ABL: status {r.status}, {r.iterations} iterations (reference about 2850), u* {ustar:.4f} "
@@ -232,28 +202,11 @@ def test_iterations_with_open_sides_match_the_oracle_composite(cuda):
             assert np.abs(a - b).max() <= 1e-8 * scale, (it, n, np.abs(a - b).max(), scale)
 
 
-def _reference_askervein():
-    """Tests/Real_Terrain/Askervein/inputs_steady_askervein: the Askervein
-    mesh of the initializer deck, inflow at xlo (the log law, u* = 0.6 m/s,
-    z0 = 0.03 m, the closure's length cap), outflow at xhi, slip walls along
-    y, the surface layer (z0 = 0.03 m), a slip top, no gravity, k floor 1e-6;
-    the mass-consistent first guess with alpha_v = 0.5."""
-    from ablsteadyrans_amd import initializer as GI
-    from ablsteadyrans_amd import physics as GP
-    from ablsteadyrans_amd import poisson as PS
-    from ablsteadyrans_amd import reductions as RD
+def _askervein_grid():
+    from ablsteadyrans_amd import mesh as M
     from tests import init_helpers as IH
-    tile = IH.askervein_tile()
-    prof = GI.InflowProfile(kind=GI.LOG_LAW, ustar=0.6, z0=0.03, direction=0.0)
-    inputs = GP.ResidualInputs(closure=GP.ClosureOptions(tke_floor=1e-6), surface=GP.SurfaceLayerOptions(z0=0.03),
-                               forcing=GP.ForcingOptions(gravity=0.0), faces=IH.ASKERVEIN_FACES, inflow=prof)
-    dm = PS.DeviceMesh(tile)
-    q = GP.DeviceState(tile.geom)
-    solver = PS.PoissonSolver(dm, PS.PoissonOptions(reltol=1e-10))
-    GI.initialize_state(dm, prof, GI.InitOptions(first_guess=1, alpha_h=1.0, alpha_v=0.5), IH.ASKERVEIN_FACES, solver,
-                        q)
-    height = RD.residual_scale_height(tile.zlevels, 0.0, 440.0)
-    return tile, dm, inputs, q, height
+    d = np.load(f"{IH.GOLD}/askervein_grid.npz")
+    return M.ErfGrid(d["x"], d["y"], d["z"])
 
 
 @pytest.mark.gpu
@@ -264,9 +217,9 @@ def test_askervein_matches_the_reference_deck(cuda):
     default 4-order drop."""
     import time
     from ablsteadyrans_amd import physics as GP
-    tile, dm, inputs, q, height = _reference_askervein()
-    stage = GP.SimplecStage(dm, inputs, GP.SimplecOptions(sweeps=4, tke_floor=1e-6, height=height))
-    dopt = GP.DriverOptions(max_iterations=3000, cfl0=1.0, cfl_max=2.0, residual_ref_iterations=100)
+    from ablsteadyrans_amd import decks as D
+    deck = D.prepare(D.askervein(_askervein_grid()))
+    stage, q, dopt = deck.simplec, deck.state, deck.driver
     t0 = time.perf_counter()
     r = stage.solve(q, dopt)
     dt = time.perf_counter() - t0
@@ -278,42 +231,6 @@ def test_askervein_matches_the_reference_deck(cuda):
     assert np.all(np.isfinite(u)) and q["tke"].numpy()[0, 2:-2, 2:-2, 2:-2].min() > 0
 
 
-def _reference_hill3d():
-    """Tests/Canonical_RANS/Neutral_Hill_3D/inputs_steady_hill3d: 64 x 64 x 20
-    over 5120 x 5120 x 800 m, periodic sides, the round Witch-of-Agnesi hill
-    (100 m, half width 500 m), the surface layer (z0 = 0.1 m), a slip top with
-    0.03 K/m over the sounding (300 K to 700 m, 309 K at 1000 m), Rayleigh
-    damping of w and theta above 550 m (0.2 1/s), the geostrophic wind
-    (10, 0) m/s at latitude 90 (horizontal Coriolis terms), the closure's
-    defaults with a k floor of 1e-6; the first guess the log law (u* = 0.35
-    m/s, z0 = 0.1 m) made divergence-free."""
-    from ablsteadyrans_amd import initializer as GI
-    from ablsteadyrans_amd import mesh as M
-    from ablsteadyrans_amd import physics as GP
-    from ablsteadyrans_amd import poisson as PS
-    from ablsteadyrans_amd import reductions as RD
-    lo, hi = (-2560.0, -2560.0, 0.0), (2560.0, 2560.0, 800.0)
-    geom = M.geometry((64, 64, 20), lo, hi, (1, 1))
-    zl = M.make_zlevels(20, 800.0)
-    tile = M.build_mesh(geom, lo, hi, zl, M.Terrain(kind="agnesi", hill_height=100.0, hill_half_width=500.0))
-    sounding = dict(theta_ref_z=(0.0, 700.0, 1000.0), theta_ref=(300.0, 300.0, 309.0))
-    prof = GI.InflowProfile(kind=GI.LOG_LAW, ustar=0.35, z0=0.1, **sounding)
-    inputs = GP.ResidualInputs(closure=GP.ClosureOptions(tke_floor=1e-6), surface=GP.SurfaceLayerOptions(z0=0.1),
-                               forcing=GP.ForcingOptions(coriolis=True, coriolis_3d=False, latitude=90.0,
-                                                         rotation_period=125663.7061435917, geostrophic=True,
-                                                         ug=10.0, vg=0.0),
-                               zhi_theta_grad=0.03, inflow=prof,
-                               damping=GP.Damping(rayleigh_zdamp=550.0, rayleigh_dampcoef=0.2, fields=("w", "theta"),
-                                                  ztop=float(zl[-1]), prob_lo=lo[:2], prob_hi=hi[:2]),
-                               **sounding)
-    dm = PS.DeviceMesh(tile)
-    q = GP.DeviceState(tile.geom)
-    solver = PS.PoissonSolver(dm, PS.PoissonOptions(reltol=1e-10))
-    GI.initialize_state(dm, prof, GI.InitOptions(first_guess=1), inputs.faces, solver, q)
-    height = RD.residual_scale_height(tile.zlevels, 0.0, 800.0)
-    return tile, dm, inputs, q, height
-
-
 @pytest.mark.gpu
 def test_neutral_hill_3d_matches_the_reference_deck(cuda):
     """The reference's 3-D hill (Tests/Canonical_RANS/Neutral_Hill_3D, README:
@@ -322,11 +239,9 @@ def test_neutral_hill_3d_matches_the_reference_deck(cuda):
     the residual reference over the first 100 iterations."""
     import time
     from ablsteadyrans_amd import physics as GP
-    tile, dm, inputs, q, height = _reference_hill3d()
-    stage = GP.SimplecStage(dm, inputs, GP.SimplecOptions(sweeps=16, tke_floor=1e-6, height=height,
-                                                          rotation_relax=1.0))
-    dopt = GP.DriverOptions(max_iterations=20000, cfl0=1.0, cfl_max=4.0, residual_ref_iterations=100,
-                            residual_drop_orders=2.0)
+    from ablsteadyrans_amd import decks as D
+    deck = D.prepare(D.neutral_hill_3d())
+    stage, q, dopt = deck.simplec, deck.state, deck.driver
     t0 = time.perf_counter()
     r = stage.solve(q, dopt)
     dt = time.perf_counter() - t0
