@@ -1,0 +1,141 @@
+// deferred.hpp -- device-side checks and reported values of a sequence of
+// operations, read back with one copy and one synchronisation.
+//
+// Each request takes a 64-bit slot in device memory that kernels update
+// atomically (a maximum or a count starts at 0, a minimum at all ones) and a
+// host callback. resolve() copies the slots to pinned host memory, waits for
+// the stream once and runs the callbacks in request order; the first failing
+// check raises its error there. An entry point that finds a batch current on
+// its thread adds its requests to it (a SIMPLEC iteration resolves once);
+// otherwise it opens its own and resolves before it returns, so its results
+// and errors are the same as when every check was read at once.
+#pragma once
+
+#include "common.hpp"
+
+#include <cstring>
+#include <functional>
+#include <utility>
+#include <vector>
+
+namespace gust_amd {
+
+class Deferred {
+public:
+    static constexpr int kZero = 128, kOnes = 128;
+    using Callback = std::function<void(unsigned long long)>;
+
+    explicit Deferred(hipStream_t st) : st_(st) {}
+    Deferred(const Deferred&) = delete;
+    Deferred& operator=(const Deferred&) = delete;
+
+    hipStream_t stream() const { return st_; }
+
+    // A slot that starts at 0 (atomicMax of ordered bits, atomicAdd, atomicOr).
+    unsigned long long* zero_slot(Callback f)
+    {
+        init();
+        if (nz_ >= kZero) { flush(); init(); }
+        const int s = nz_++;
+        cbs_.emplace_back(s, std::move(f));
+        return dev_ + s;
+    }
+    // A slot that starts at all ones (atomicMin).
+    unsigned long long* ones_slot(Callback f)
+    {
+        init();
+        if (no_ >= kOnes) { flush(); init(); }
+        const int s = kZero + no_++;
+        cbs_.emplace_back(s, std::move(f));
+        return dev_ + s;
+    }
+    // A host callback with no slot, run in order at resolve (a check of
+    // values other callbacks have stored).
+    void then(std::function<void()> f)
+    {
+        cbs_.emplace_back(-1, [g = std::move(f)](unsigned long long) { g(); });
+    }
+    bool pending() const { return !cbs_.empty(); }
+
+    // One copy, one synchronisation, the callbacks in order.
+    void resolve()
+    {
+        if (cbs_.empty()) { return; }
+        flush();
+    }
+
+    static Deferred*& current()
+    {
+        static thread_local Deferred* cur = nullptr;
+        return cur;
+    }
+
+    // Makes a batch the thread's current one for its lifetime.
+    class Scope {
+    public:
+        explicit Scope(Deferred& d) : prev_(current()) { current() = &d; }
+        ~Scope() { current() = prev_; }
+        Scope(const Scope&) = delete;
+        Scope& operator=(const Scope&) = delete;
+
+    private:
+        Deferred* prev_;
+    };
+
+private:
+    void init();
+    void flush()
+    {
+        if (!inited_) {
+            std::vector<std::pair<int, Callback>> cbs;
+            cbs.swap(cbs_);
+            for (auto& c : cbs) { c.second(0); }
+            return;
+        }
+        GUST_HIP(hipMemcpyAsync(host_, dev_, sizeof(unsigned long long) * (kZero + kOnes), hipMemcpyDeviceToHost, st_));
+        GUST_HIP(hipStreamSynchronize(st_));
+        std::vector<std::pair<int, Callback>> cbs;
+        cbs.swap(cbs_);
+        std::vector<unsigned long long> vals(host_, host_ + kZero + kOnes);
+        inited_ = false;
+        nz_ = no_ = 0;
+        for (auto& c : cbs) { c.second(c.first >= 0 ? vals[size_t(c.first)] : 0ull); }
+    }
+
+    hipStream_t st_;
+    unsigned long long* dev_ = nullptr;
+    unsigned long long* host_ = nullptr;
+    int nz_ = 0, no_ = 0;
+    bool inited_ = false;
+    std::vector<std::pair<int, Callback>> cbs_;
+};
+
+// The batch an entry point reports into: the thread's current one, or its
+// own (resolved by finish()).
+class DeferredUse {
+public:
+    explicit DeferredUse(hipStream_t st) : own_(st), cur_(Deferred::current() ? Deferred::current() : &own_),
+                                           scope_(*cur_) {}
+    Deferred& operator*() { return *cur_; }
+    Deferred* operator->() { return cur_; }
+    // Resolves the batch if this entry point opened it.
+    void finish()
+    {
+        if (cur_ == &own_) { own_.resolve(); }
+    }
+
+private:
+    Deferred own_;
+    Deferred* cur_;
+    Deferred::Scope scope_;
+};
+
+// Ordered bits of |x| <-> x (non-negative doubles order as their bits).
+inline double bits_to_double(unsigned long long b)
+{
+    double d;
+    std::memcpy(&d, &b, sizeof d);
+    return d;
+}
+
+}   // namespace gust_amd

@@ -25,11 +25,14 @@
 //    library is compiled without floating-point contraction, so FP64 results
 //    equal the CPU build bit for bit.
 #include "common.hpp"
+#include "deferred.hpp"
 
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <map>
 #include <memory>
 
 namespace gust_amd {
@@ -292,6 +295,11 @@ __global__ void zero_ghosts_kernel(T* p, int64_t jstride, int64_t kstride, int n
     }
 }
 
+__global__ void copy_u64_kernel(const unsigned long long* src, unsigned long long* dst, int n)
+{
+    for (int t = threadIdx.x; t < n; t += blockDim.x) { dst[t] = src[t]; }
+}
+
 __global__ void zero_u64_kernel(unsigned long long* p, int n)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -439,14 +447,50 @@ struct JacobiPlan final : JacobiPlanBase {
         } else {
             scratch = static_cast<T*>(Workspace::get(st, 0, scratch_elems * sizeof(T) + 256));
             chg = static_cast<unsigned long long*>(Workspace::get(st, 1, size_t(n) * 8 + 8));
-            dtab = static_cast<JacobiSys<T>*>(Workspace::get(st, 2, tab_bytes));
             if (cbytes) { cglob = static_cast<T*>(Workspace::get(st, 5, cbytes)); }
         }
         for (auto& t : tab) { t.s = scratch + reinterpret_cast<size_t>(t.s); }
-        // The table travels once; the synchronisation keeps the host copy's
-        // lifetime out of the picture.
-        GUST_HIP(hipMemcpyAsync(dtab, tab.data(), tab_bytes, hipMemcpyHostToDevice, st));
-        GUST_HIP(hipStreamSynchronize(st));
+        if (own) {
+            // The table travels once; the synchronisation keeps the host
+            // copy's lifetime out of the picture.
+            GUST_HIP(hipMemcpyAsync(dtab, tab.data(), tab_bytes, hipMemcpyHostToDevice, st));
+            GUST_HIP(hipStreamSynchronize(st));
+        } else {
+            dtab = cached_table(st, tab);
+        }
+    }
+
+    // A device copy of the table, uploaded the first time this table (its
+    // bytes) is seen on the stream: an iteration that sweeps the same
+    // systems every time copies nothing and waits for nothing.
+    static JacobiSys<T>* cached_table(hipStream_t st, const std::vector<JacobiSys<T>>& tab)
+    {
+        struct Entry {
+            std::vector<char> key;
+            JacobiSys<T>* dev;
+        };
+        static std::mutex mu;
+        static std::map<std::pair<int, hipStream_t>, std::vector<Entry>> cache;
+        const size_t bytes = tab.size() * sizeof(JacobiSys<T>);
+        std::vector<char> key(bytes);
+        std::memcpy(key.data(), tab.data(), bytes);
+        int device = 0;
+        GUST_HIP(hipGetDevice(&device));
+        std::lock_guard<std::mutex> lock(mu);
+        std::vector<Entry>& list = cache[{device, st}];
+        for (Entry& e : list) {
+            if (e.key == key) { return e.dev; }
+        }
+        if (list.size() >= 64) {
+            GUST_HIP(hipStreamSynchronize(st));
+            for (Entry& e : list) { (void)hipFree(e.dev); }
+            list.clear();
+        }
+        JacobiSys<T>* dev = nullptr;
+        GUST_HIP(hipMalloc(&dev, bytes));
+        GUST_HIP(hipMemcpy(dev, tab.data(), bytes, hipMemcpyHostToDevice));
+        list.push_back(Entry{std::move(key), dev});
+        return dev;
     }
 
     template <bool F, bool L>
@@ -531,7 +575,15 @@ void zjacobi_run(int nsys, const gust_line_system* sys, int sweeps, double* chan
     JacobiPlan<T> plan;
     plan.init(nsys, sys, false, st);
     plan.run(sweeps, changes_dev, st);
-    if (change_host) { plan.change_host(change_host, 1, st); }
+    if (!change_host) { return; }
+    if (Deferred* dq = Deferred::current()) {
+        // the change travels with the batch's other values
+        unsigned long long* slot = dq->zero_slot([change_host](unsigned long long b) { *change_host = bits_to_double(b); });
+        copy_u64_kernel<<<1, 64, 0, st>>>(plan.chg, slot, 1);
+        GUST_HIP(hipGetLastError());
+        return;
+    }
+    plan.change_host(change_host, 1, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -899,9 +951,7 @@ void zjacobi_one(int loc, const gust_array4* A, const gust_array4* ap, const gus
     q.ap = *ap;
     q.rhs = *rhs;
     q.delta = *delta;
-    double c = 0;
-    zjacobi_run<T>(1, &q, sweeps, nullptr, &c, st);
-    if (change) { *change = c; }
+    zjacobi_run<T>(1, &q, sweeps, nullptr, change, st);
 }
 
 }   // namespace

@@ -24,6 +24,7 @@
 // bit except where the device's log and pow enter (the wall law's u*, the
 // closure's k^1.5), which differ from the host's in the last bit.
 #include "cgrid.hpp"
+#include "deferred.hpp"
 #include "heights.hpp"
 #include "profile.hpp"
 #include "stencil.hpp"
@@ -1077,20 +1078,22 @@ void launch_surface(hipStream_t st, const gust_geometry& g, const gust_array4* u
                     const gust_array4* z, const gust_surface_options& o, gust_array4* ss)
 {
     const Box3 cols = box3(slab_box(g));
-    auto* bad = static_cast<unsigned*>(Workspace::get(st, kSlotBad, sizeof(unsigned)));
-    GUST_HIP(hipMemsetAsync(bad, 0, sizeof(unsigned), st));
+    // The count of bad columns is read with the caller's other checks (the
+    // low word of a slot that starts at zero).
+    DeferredUse dq(st);
+    const double z0 = o.z0;
+    auto* bad = reinterpret_cast<unsigned*>(dq->zero_slot([z0](unsigned long long n) {
+        if (n) {
+            GUST_FAIL("gust: compute_surface_state: no positive, finite friction velocity at %llu columns (surface "
+                      "layer z0 = %g)",
+                      n, z0);
+        }
+    }));
     surface_kernel<<<blocks_for(cols.npts()), kThreads, 0, st>>>(cols, dview<const double>(u), dview<const double>(v),
                                                                  dview<const double>(z), surf_opt(o), 1.0 / g.dx[0],
                                                                  1.0 / g.dx[1], ss_view(ss), g.lo[2], bad);
     GUST_HIP(hipGetLastError());
-    unsigned* h = static_cast<unsigned*>(Workspace::get_host_pinned(kSlotBad, sizeof(unsigned)));
-    GUST_HIP(hipMemcpyAsync(h, bad, sizeof(unsigned), hipMemcpyDeviceToHost, st));
-    GUST_HIP(hipStreamSynchronize(st));
-    if (*h) {
-        GUST_FAIL("gust: compute_surface_state: no positive, finite friction velocity at %u columns (surface layer "
-                  "z0 = %g)",
-                  *h, o.z0);
-    }
+    dq.finish();
 }
 
 void check_ss(const char* op, const gust_geometry& g, const gust_array4* ss)

@@ -11,6 +11,7 @@
 // then forms the three face fields in one launch each, periodic images read
 // by index wrap instead of a ghost fill.
 #include "cgrid.hpp"
+#include "deferred.hpp"
 
 #include <cmath>
 
@@ -142,13 +143,12 @@ int gust_face_dtau(const gust_geometry* geom, const gust_array4* dtau_cc, gust_a
         check_field(op, "dtau_z", dtau_z, *geom, GUST_LOC_ZFACE, 0);
         hipStream_t st = (hipStream_t)stream;
         const Box3 bc = box3(loc_box(*geom, GUST_LOC_CELL));
-        auto* flag = static_cast<unsigned int*>(Workspace::get(st, 20, 64));
-        auto* hflag = static_cast<unsigned int*>(Workspace::get_host_pinned(20, 64));
-        GUST_HIP(hipMemsetAsync(flag, 0, sizeof(unsigned int), st));
+        // the positivity flag is read with the caller's other checks
+        DeferredUse dq(st);
+        auto* flag = reinterpret_cast<unsigned int*>(dq->zero_slot([](unsigned long long f) {
+            if (f) { GUST_FAIL("gust: face_dtau: dtau_cc must be finite and > 0 everywhere"); }
+        }));
         positive_flag_kernel<<<blocks_for(bc.npts()), kThreads, 0, st>>>(bc, dview<const double>(dtau_cc), flag);
-        GUST_HIP(hipMemcpyAsync(hflag, flag, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
-        GUST_HIP(hipStreamSynchronize(st));
-        if (*hflag) { GUST_FAIL("gust: face_dtau: dtau_cc must be finite and > 0 everywhere"); }
         const CV d = dview<const double>(dtau_cc);
         const int nx = geom->hi[0] - geom->lo[0] + 1, ny = geom->hi[1] - geom->lo[1] + 1;
         const int nz = geom->hi[2] - geom->lo[2] + 1;
@@ -162,6 +162,7 @@ int gust_face_dtau(const gust_geometry* geom, const gust_array4* dtau_cc, gust_a
         face_dtau_kernel<2><<<blocks_for(bz.npts()), kThreads, 0, st>>>(bz, d, dview<double>(dtau_z), geom->lo[2],
                                                                         geom->hi[2], nz, 0);
         GUST_HIP(hipGetLastError());
+        dq.finish();
     });
 }
 

@@ -10,9 +10,13 @@
 // fixed two-level tree instead (block partials, then one block per field):
 // the same tile gives the same bits on every call and any stream.
 #include "cgrid.hpp"
+#include "deferred.hpp"
 #include "reduce.hpp"
 
+#include <array>
 #include <cmath>
+#include <functional>
+#include <memory>
 #include <cstring>
 #include <limits>
 
@@ -275,8 +279,8 @@ RField rfield(const gust_array4* a, const IBox& b, int op)
     return f;
 }
 
-// Reduce every field of s (n of them) with the fixed tree; host values out.
-void multi_reduce(hipStream_t st, const RSet& s, int n, double* out)
+// Reduce every field of s (n of them) with the fixed tree into device memory.
+double* multi_reduce_dev(hipStream_t st, const RSet& s, int n)
 {
     int64_t most = 1;
     for (int q = 0; q < n; ++q) {
@@ -287,8 +291,40 @@ void multi_reduce(hipStream_t st, const RSet& s, int n, double* out)
     double* dout = static_cast<double*>(Workspace::get(st, kSlotOut, kMaxFields * sizeof(double)));
     multi_partial_kernel<<<dim3(nb, n), kThreads, 0, st>>>(s, part);
     multi_final_kernel<<<n, kThreads, 0, st>>>(s, part, nb, dout);
-    const double* h = fetch<double>(st, dout, n, kSlotOut);
+    return dout;
+}
+
+// ... and host values out.
+void multi_reduce(hipStream_t st, const RSet& s, int n, double* out)
+{
+    const double* h = fetch<double>(st, multi_reduce_dev(st, s, n), n, kSlotOut);
     for (int q = 0; q < n; ++q) { out[q] = h[q]; }
+}
+
+struct SlotPtrs {
+    unsigned long long* p[kMaxFields];
+};
+
+__global__ void to_slots_kernel(const double* src, SlotPtrs dst, int n)
+{
+    const int t = threadIdx.x;
+    if (t < n) { *dst.p[t] = __double_as_longlong(src[t]); }
+}
+
+// ... read with the caller's batch: done(values) runs when it resolves.
+void multi_reduce_then(hipStream_t st, const RSet& s, int n, std::function<void(const double*)> done)
+{
+    const double* dout = multi_reduce_dev(st, s, n);
+    DeferredUse dq(st);
+    auto vals = std::make_shared<std::array<double, kMaxFields>>();
+    SlotPtrs dst{};
+    for (int q = 0; q < n; ++q) {
+        dst.p[q] = dq->zero_slot([vals, q](unsigned long long b) { (*vals)[size_t(q)] = bits_to_double(b); });
+    }
+    to_slots_kernel<<<1, 64, 0, st>>>(dout, dst, n);
+    GUST_HIP(hipGetLastError());
+    dq->then([vals, f = std::move(done)] { f(vals->data()); });
+    dq.finish();
 }
 
 const char* const kResNames[6] = {"R.u", "R.v", "R.w", "R.p", "R.theta", "R.tke"};
@@ -477,24 +513,23 @@ int gust_stage_norms(const gust_geometry* geom, const gust_residuals* R, const g
         set.f[n++] = rfield(w, loc_box(g, GUST_LOC_ZFACE), kMaxAbs);
         set.f[n++] = rfield(tke, cells, kSum);
         if (ustar) { set.f[n++] = rfield(ustar, slab, kSum); }
-        double out[kMaxFields];
-        multi_reduce(static_cast<hipStream_t>(stream), set, n, out);
-
-        double scale[6];
+        // The norms arrive with the caller's batch (at once when there is none).
+        double scale[6], npts[6];
         scales_of(*s, height, scale);
-        for (int q = 0; q < 6; ++q) {
-            const double n2 = std::sqrt(out[q]);
-            const double npts = double(loc_box(g, kResLoc[q]).npts());
-            res[q] = n2 / std::sqrt(npts) / scale[q];
-        }
-        mon[0] = std::max(std::max(out[6], out[7]), out[8]);
-        mon[1] = out[9] / double(cells.npts());
-        mon[2] = 0.0;
-        *nmon = 2;
-        if (ustar) {
-            mon[2] = out[10] / double(int64_t(cells.len(0)) * cells.len(1));
-            *nmon = 3;
-        }
+        for (int q = 0; q < 6; ++q) { npts[q] = double(loc_box(g, kResLoc[q]).npts()); }
+        const double ncells = double(cells.npts()), ncols = double(int64_t(cells.len(0)) * cells.len(1));
+        const bool wall = ustar != nullptr;
+        std::array<double, 6> sc, np;
+        std::copy(scale, scale + 6, sc.begin());
+        std::copy(npts, npts + 6, np.begin());
+        multi_reduce_then(static_cast<hipStream_t>(stream), set, n,
+                          [=](const double* out) {
+                              for (int q = 0; q < 6; ++q) { res[q] = std::sqrt(out[q]) / std::sqrt(np[q]) / sc[q]; }
+                              mon[0] = std::max(std::max(out[6], out[7]), out[8]);
+                              mon[1] = out[9] / ncells;
+                              mon[2] = wall ? out[10] / ncols : 0.0;
+                              *nmon = wall ? 3 : 2;
+                          });
     });
 }
 

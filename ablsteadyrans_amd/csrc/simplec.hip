@@ -13,10 +13,12 @@
 // are order-independent (atomic max over the bits of |x|), so every result
 // is reproducible; they return to the host where the reference reduces too.
 #include "cgrid.hpp"
+#include "deferred.hpp"
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -366,54 +368,51 @@ Scratch scratch(hipStream_t st, int slot, const IBox& valid, int g)
     return s;
 }
 
+// Reductions into a deferred batch: each registers a slot and the callback
+// that receives its value when the batch resolves.
 class Reducer {
 public:
-    explicit Reducer(hipStream_t st) : st_(st)
-    {
-        dev_ = static_cast<unsigned long long*>(Workspace::get(st, 50, 16 * sizeof(unsigned long long)));
-        host_ = static_cast<unsigned long long*>(Workspace::get_host_pinned(50, 16 * sizeof(unsigned long long)));
-    }
-    double max_abs(const IBox& box, const gust_array4* a, const gust_array4* mask = nullptr)
+    explicit Reducer(Deferred& dq) : dq_(dq), st_(dq.stream()) {}
+    void max_abs(const IBox& box, const gust_array4* a, std::function<void(double)> f,
+                 const gust_array4* mask = nullptr)
     {
         const Box3 b = box3(box);
-        GUST_HIP(hipMemsetAsync(dev_, 0, sizeof(unsigned long long), st_));
+        unsigned long long* slot = dq_.zero_slot([f](unsigned long long v) { f(bits_to_double(v)); });
         max_abs_kernel<<<cgrid::reduce_blocks(b.npts()), kThreads, 0, st_>>>(b, dview<const double>(a),
-                                                                   dview<const double>(mask ? mask : a),
-                                                                   mask ? 1 : 0, dev_);
-        return fetch();
-    }
-    double min_pos(const IBox& box, const gust_array4* a)
-    {
-        const Box3 b = box3(box);
-        GUST_HIP(hipMemsetAsync(dev_, 0xff, sizeof(unsigned long long), st_));
-        min_pos_kernel<<<cgrid::reduce_blocks(b.npts()), kThreads, 0, st_>>>(b, dview<const double>(a), dev_);
-        return fetch();
-    }
-    bool flag_bad_nonneg(const IBox& box, const gust_array4* a)
-    {
-        const Box3 b = box3(box);
-        GUST_HIP(hipMemsetAsync(dev_, 0, sizeof(unsigned long long), st_));
-        nonneg_flag_kernel<<<cgrid::reduce_blocks(b.npts()), kThreads, 0, st_>>>(b, dview<const double>(a),
-                                                                       reinterpret_cast<unsigned int*>(dev_));
+                                                                             dview<const double>(mask ? mask : a),
+                                                                             mask ? 1 : 0, slot);
         GUST_HIP(hipGetLastError());
-        GUST_HIP(hipMemcpyAsync(host_, dev_, sizeof(unsigned long long), hipMemcpyDeviceToHost, st_));
-        GUST_HIP(hipStreamSynchronize(st_));
-        return (host_[0] & 0xffffffffull) != 0;
+    }
+    void min_pos(const IBox& box, const gust_array4* a, std::function<void(double)> f)
+    {
+        const Box3 b = box3(box);
+        unsigned long long* slot = dq_.ones_slot([f](unsigned long long v) { f(bits_to_double(v)); });
+        min_pos_kernel<<<cgrid::reduce_blocks(b.npts()), kThreads, 0, st_>>>(b, dview<const double>(a), slot);
+        GUST_HIP(hipGetLastError());
+    }
+    // Fails with msg when a is negative, NaN or infinite anywhere.
+    void check_nonneg(const IBox& box, const gust_array4* a, std::string msg)
+    {
+        const Box3 b = box3(box);
+        unsigned long long* slot = dq_.zero_slot([msg](unsigned long long v) {
+            if (v & 0xffffffffull) { GUST_FAIL("%s", msg.c_str()); }
+        });
+        nonneg_flag_kernel<<<cgrid::reduce_blocks(b.npts()), kThreads, 0, st_>>>(b, dview<const double>(a),
+                                                                                 reinterpret_cast<unsigned int*>(slot));
+        GUST_HIP(hipGetLastError());
+    }
+    // Fails with msg unless a is finite and > 0 everywhere.
+    void check_positive(const IBox& box, const gust_array4* a, const std::string& msg)
+    {
+        check_nonneg(box, a, msg);
+        min_pos(box, a, [msg](double m) {
+            if (!(m > 0)) { GUST_FAIL("%s", msg.c_str()); }
+        });
     }
 
 private:
-    double fetch()
-    {
-        GUST_HIP(hipGetLastError());
-        GUST_HIP(hipMemcpyAsync(host_, dev_, sizeof(unsigned long long), hipMemcpyDeviceToHost, st_));
-        GUST_HIP(hipStreamSynchronize(st_));
-        double d;
-        std::memcpy(&d, host_, sizeof(double));
-        return d;
-    }
+    Deferred& dq_;
     hipStream_t st_;
-    unsigned long long* dev_;
-    unsigned long long* host_;
 };
 
 void box_op(hipStream_t st, const IBox& box, const gust_array4* y, const gust_array4* x, double a, int op)
@@ -506,46 +505,67 @@ int gust_pressure_correction(gust_poisson* solver, const gust_geometry* geom, gu
         Scratch rhs = scratch(st, 30, cells, 0), phi = scratch(st, 31, cells, 1), omega = scratch(st, 32, fb[2], 0),
                 div = scratch(st, 33, cells, 0);
         Scratch beta[3], keep[3], F[3];
-        Reducer red(st);
+        DeferredUse dq(st);
+        Reducer red(*dq);
+        // the values the checks and the report need, read when the batch resolves
+        struct Vals {
+            double speed[3] = {0, 0, 0};
+            double jmin = 0;
+            gust_pressure_correction_result res{};
+        };
+        auto vals = std::make_shared<Vals>();
         // face_coefficients (Simplec.cpp:87-112)
         for (int d = 0; d < 3; ++d) {
             beta[d] = scratch(st, 34 + d, fb[d], 0);
             keep[d] = scratch(st, 37 + d, fb[d], 0);
             F[d] = scratch(st, 40 + d, fb[d], 0);
-            if (red.flag_bad_nonneg(fb[d], dd[d])) {
-                GUST_FAIL("gust: pressure_correction: the SIMPLEC denominator %s must be finite and >= 0 everywhere",
-                          dname[d]);
-            }
+            red.check_nonneg(fb[d], dd[d], std::string("gust: pressure_correction: the SIMPLEC denominator ") +
+                                               dname[d] + " must be finite and >= 0 everywhere");
             const Box3 b = box3(fb[d]);
             beta_keep_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<const double>(dd[d]),
                                                                         dview<double>(&beta[d].v),
                                                                         dview<double>(&keep[d].v));
             GUST_HIP(hipGetLastError());
         }
-        gust_pressure_correction_result res{};
         check_rc(gust_velocity_divergence(geom, u, v, w, z_nd, ax, ay, az, detJ, &rhs.v, stream));
-        res.divergence_before = red.max_abs(cells, &rhs.v);
+        red.max_abs(cells, &rhs.v, [vals](double x) { vals->res.divergence_before = x; });
         if (source) { box_op(st, cells, &rhs.v, source, 0.0, 2); }
-        const double speed = std::max({red.max_abs(fb[0], u), red.max_abs(fb[1], v), red.max_abs(fb[2], w)});
-        const double dz_min = red.min_pos(cells, detJ) * g.dx[2];
-        const double inv_h = 1.0 / g.dx[0] + 1.0 / g.dx[1] + 1.0 / dz_min;
+        red.max_abs(fb[0], u, [vals](double x) { vals->speed[0] = x; });
+        red.max_abs(fb[1], v, [vals](double x) { vals->speed[1] = x; });
+        red.max_abs(fb[2], w, [vals](double x) { vals->speed[2] = x; });
+        red.min_pos(cells, detJ, [vals](double x) { vals->jmin = x; });
         gust_face_coeffs coeffs{&beta[0].v, &beta[1].v, &beta[2].v, nullptr, 1.0};
         box_op(st, grow(cells, 1, 1, 1), &phi.v, nullptr, 0.0, 3);
-        check_rc(gust_poisson_solve(solver, &rhs.v, &phi.v, &coeffs, bc, &F[0].v, &F[1].v, &F[2].v, &res.poisson,
-                                    stream));
-        if (!project_net_source && res.poisson.singular &&
-            std::abs(res.poisson.removed_mean) > 1.0e-8 * speed * inv_h) {
-            GUST_FAIL("gust: pressure_correction: the predicted velocity has a net source the boundaries cannot "
-                      "carry: J-weighted mean divergence %s 1/s against the scale %s 1/s (a boundary condition or "
-                      "the predictor is inconsistent)",
-                      std::to_string(res.poisson.removed_mean).c_str(), std::to_string(speed * inv_h).c_str());
+        {
+            const int rc = gust_poisson_solve(solver, &rhs.v, &phi.v, &coeffs, bc, &F[0].v, &F[1].v, &F[2].v,
+                                              &vals->res.poisson, stream);
+            if (rc != 0) {
+                // an earlier check's error (a bad denominator) explains a failed solve
+                std::string err = gust_last_error();
+                dq->resolve();
+                set_error("%s", err.c_str());
+                throw Fail{rc};
+            }
         }
+        const double dx0 = g.dx[0], dx1 = g.dx[1], dz = g.dx[2];
+        dq->then([vals, project_net_source, dx0, dx1, dz] {
+            const double speed = std::max({vals->speed[0], vals->speed[1], vals->speed[2]});
+            const double dz_min = vals->jmin * dz;
+            const double inv_h = 1.0 / dx0 + 1.0 / dx1 + 1.0 / dz_min;
+            const gust_poisson_result& pr = vals->res.poisson;
+            if (!project_net_source && pr.singular && std::abs(pr.removed_mean) > 1.0e-8 * speed * inv_h) {
+                GUST_FAIL("gust: pressure_correction: the predicted velocity has a net source the boundaries cannot "
+                          "carry: J-weighted mean divergence %s 1/s against the scale %s 1/s (a boundary condition "
+                          "or the predictor is inconsistent)",
+                          std::to_string(pr.removed_mean).c_str(), std::to_string(speed * inv_h).c_str());
+            }
+        });
         // p += alpha_p phi; u, v, Omega += the masked fluxes; w follows.
         box_op(st, cells, p, &phi.v, alpha_p, 0);
-        res.max_dp = alpha_p * red.max_abs(cells, &phi.v);
-        res.max_du = red.max_abs(fb[0], &F[0].v, &keep[0].v);
-        res.max_dv = red.max_abs(fb[1], &F[1].v, &keep[1].v);
-        res.max_dw = red.max_abs(fb[2], &F[2].v, &keep[2].v);
+        red.max_abs(cells, &phi.v, [vals, alpha_p](double x) { vals->res.max_dp = alpha_p * x; });
+        red.max_abs(fb[0], &F[0].v, [vals](double x) { vals->res.max_du = x; }, &keep[0].v);
+        red.max_abs(fb[1], &F[1].v, [vals](double x) { vals->res.max_dv = x; }, &keep[1].v);
+        red.max_abs(fb[2], &F[2].v, [vals](double x) { vals->res.max_dw = x; }, &keep[2].v);
         for (int d = 0; d < 3; ++d) { box_op(st, fb[d], &F[d].v, &keep[d].v, 0.0, 1); }
         check_rc(gust_omega_from_w(geom, u, v, w, z_nd, &omega.v, 0, stream));
         box_add(st, fb[0], u, &F[0].v);
@@ -560,8 +580,11 @@ int gust_pressure_correction(gust_poisson* solver, const gust_geometry* geom, gu
         override_sync(st, g, GUST_LOC_ZFACE, w);
         fill_boundary(st, g, GUST_LOC_ZFACE, w);
         check_rc(gust_velocity_divergence(geom, u, v, w, z_nd, ax, ay, az, detJ, &div.v, stream));
-        res.divergence_after = red.max_abs(cells, &div.v);
-        if (result) { *result = res; }
+        red.max_abs(cells, &div.v, [vals, result](double x) {
+            vals->res.divergence_after = x;
+            if (result) { *result = vals->res; }
+        });
+        dq.finish();
     });
 }
 
@@ -591,14 +614,13 @@ int gust_momentum_predictor(const gust_geometry* geom, gust_faces* vel, const gu
         if (opt->sweeps < 1) { GUST_FAIL("gust: momentum_predictor: sweeps must be >= 1 (got %d)", opt->sweeps); }
         const gust_geometry& g = *geom;
         const char* dname[3] = {"dtau_x", "dtau_y", "dtau_z"};
-        Reducer red(st);
+        DeferredUse dq(st);
+        Reducer red(*dq);
         for (int c = 0; c < 3; ++c) {
             check_field(op, dname[c], dtau->c[c], g, 1 << c, 0);
             const IBox fb = loc_box(g, 1 << c);
             // finite and > 0: the non-negative flag and a zero minimum both fail
-            if (red.flag_bad_nonneg(fb, dtau->c[c]) || !(red.min_pos(fb, dtau->c[c]) > 0)) {
-                GUST_FAIL("gust: momentum_predictor: dtau_faces must be finite and > 0 everywhere");
-            }
+            red.check_positive(fb, dtau->c[c], "gust: momentum_predictor: dtau_faces must be finite and > 0 everywhere");
         }
         const bool rotation = opt->coriolis_f != 0;
         const bool buoyancy = opt->gravity > 0;
@@ -607,9 +629,17 @@ int gust_momentum_predictor(const gust_geometry* geom, gust_faces* vel, const gu
                       "conditions (the Dirichlet rows they skip); give in.bc, or gravity 0 and Coriolis off");
         }
         check_rc(gust_check_line_options(&opt->sweep));
-        gust_momentum_predictor_result out{};
-        double* dmax[3] = {&out.max_du, &out.max_dv, &out.max_dw};
-        double* last[3] = {&out.last_sweep_du, &out.last_sweep_dv, &out.last_sweep_dw};
+        double* dmax[3] = {nullptr, nullptr, nullptr};
+        double* last[3] = {nullptr, nullptr, nullptr};
+        if (result) {
+            *result = gust_momentum_predictor_result{};
+            dmax[0] = &result->max_du;
+            dmax[1] = &result->max_dv;
+            dmax[2] = &result->max_dw;
+            last[0] = &result->last_sweep_du;
+            last[1] = &result->last_sweep_dv;
+            last[2] = &result->last_sweep_dw;
+        }
         const double dxi0 = 1.0 / g.dx[0], dxi1 = 1.0 / g.dx[1];
         for (int c = 0; c < 3; ++c) {
             const int loc = 1 << c;
@@ -692,10 +722,13 @@ int gust_momentum_predictor(const gust_geometry* geom, gust_faces* vel, const gu
             }
             check_rc(gust_line_sweeps(&opt->sweep, loc, A->c[c], ap->c[c], &rhs.v, opt->sweeps, geom, &delta.v,
                                       last[c], stream));
-            *dmax[c] = red.max_abs(fb, &delta.v);
+            if (dmax[c]) {
+                double* out = dmax[c];
+                red.max_abs(fb, &delta.v, [out](double x) { *out = x; });
+            }
             box_add(st, fb, vel->c[c], &delta.v);
         }
-        if (result) { *result = out; }
+        dq.finish();
     });
 }
 
@@ -716,33 +749,29 @@ int gust_scalar_sweep(const gust_geometry* geom, int32_t eq, gust_array4* field,
         if (A->ncomp != GUST_N_COEFS) { GUST_FAIL("gust: scalar_sweep: A needs 7 components"); }
         if (std::isnan(floor) || !(floor >= 0)) { GUST_FAIL("gust: scalar_sweep: the floor must be >= 0"); }
         if (!sweep) { GUST_FAIL("gust: scalar_sweep: null line options"); }
-        Reducer red(st);
+        DeferredUse dq(st);
+        Reducer red(*dq);
         const IBox cells = loc_box(g, GUST_LOC_CELL);
-        if (red.flag_bad_nonneg(cells, dtau) || !(red.min_pos(cells, dtau) > 0)) {
-            GUST_FAIL("gust: scalar_sweep: dtau must be finite and > 0 everywhere");
-        }
+        red.check_positive(cells, dtau, "gust: scalar_sweep: dtau must be finite and > 0 everywhere");
         const Box3 b = box3(cells);
         Scratch ap = scratch(st, 51, cells, 0), rhs = scratch(st, 52, cells, 0), delta = scratch(st, 53, cells, 1);
         negate_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<const double>(R_eq), dview<double>(&rhs.v));
         scalar_ap_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<const double>(A), dview<const double>(dtau),
                                                                     dview<double>(&ap.v));
         GUST_HIP(hipGetLastError());
-        gust_scalar_sweep_result res{};
-        check_rc(gust_line_sweeps(sweep, GUST_LOC_CELL, A, &ap.v, &rhs.v, sweeps, geom, &delta.v, &res.last_sweep,
-                                  stream));
-        res.max_dq = red.max_abs(cells, &delta.v);
+        if (result) { *result = gust_scalar_sweep_result{}; }
+        check_rc(gust_line_sweeps(sweep, GUST_LOC_CELL, A, &ap.v, &rhs.v, sweeps, geom, &delta.v,
+                                  result ? &result->last_sweep : nullptr, stream));
+        if (result) { red.max_abs(cells, &delta.v, [result](double x) { result->max_dq = x; }); }
         box_add(st, cells, field, &delta.v);
         if (eq == GUST_EQ_TKE && clip) {
-            auto* cnt = static_cast<unsigned long long*>(Workspace::get(st, 54, 64));
-            auto* hc = static_cast<unsigned long long*>(Workspace::get_host_pinned(54, 64));
-            GUST_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), st));
+            unsigned long long* cnt = dq->zero_slot([result](unsigned long long v) {
+                if (result) { result->clipped = static_cast<int64_t>(v); }
+            });
             clip_kernel<<<cgrid::reduce_blocks(b.npts()), kThreads, 0, st>>>(b, dview<double>(field), floor, cnt);
             GUST_HIP(hipGetLastError());
-            GUST_HIP(hipMemcpyAsync(hc, cnt, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-            GUST_HIP(hipStreamSynchronize(st));
-            res.clipped = static_cast<int64_t>(*hc);
         }
-        if (result) { *result = res; }
+        dq.finish();
     });
 }
 

@@ -10,9 +10,11 @@
 //   4. ghosts, the pressure correction;         5. theta's convection at the corrected
 //      transport, the theta and k sweeps with the rows at the corrected transport.
 #include "cgrid.hpp"
+#include "deferred.hpp"
 
 #include <cfloat>
 #include <cmath>
+#include <exception>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -148,7 +150,7 @@ double coriolis_f(const gust_forcing& f)
 }
 
 // One iteration (Source/Simplec.cpp:452-506).
-void iterate(SimplecStage& s, gust_state* q, const gust_array4* dtau, gust_simplec_info* info, void* stream)
+void iterate_body(SimplecStage& s, gust_state* q, const gust_array4* dtau, gust_simplec_info* info, void* stream)
 {
     hipStream_t st = static_cast<hipStream_t>(stream);
     const gust_geometry* geom = &s.g;
@@ -218,6 +220,25 @@ void iterate(SimplecStage& s, gust_state* q, const gust_array4* dtau, gust_simpl
     check_rc(gust_implicit_rows(GUST_EQ_TKE, geom, &Ff, &s.mesh, &s.phys, cf, ssv, &s.Ak.v, stream));
     check_rc(gust_scalar_sweep(geom, GUST_EQ_TKE, q->f[5], &s.R[5].v, &s.Ak.v, dtau, s.opt.sweeps, s.opt.tke_floor, 1,
                                &s.opt.sweep, &inf.tke, stream));
+}
+
+// One iteration with its checks and reported values read back together at
+// its end (one copy, one wait); when a step fails, a check that failed before
+// it is the error reported, as when each check is read at once.
+void iterate(SimplecStage& s, gust_state* q, const gust_array4* dtau, gust_simplec_info* info, void* stream)
+{
+    gust_simplec_info local{};
+    gust_simplec_info& inf = info ? *info : local;
+    Deferred dq(static_cast<hipStream_t>(stream));
+    Deferred::Scope scope(dq);
+    try {
+        iterate_body(s, q, dtau, &inf, stream);
+    } catch (...) {
+        const std::exception_ptr e = std::current_exception();
+        dq.resolve();
+        std::rethrow_exception(e);
+    }
+    dq.resolve();
 }
 
 }   // namespace
