@@ -380,6 +380,7 @@ struct JacobiPlan final : JacobiPlanBase {
     T* cglob = nullptr;
     unsigned long long* chg = nullptr;
     bool owned = false;   // buffers from hipMalloc (a plan handle), not the stream workspace
+    bool chg_zeroed = false;   // chg already holds zeros (a fresh deferred slot)
 
     ~JacobiPlan() override
     {
@@ -522,8 +523,10 @@ struct JacobiPlan final : JacobiPlanBase {
     void run(int sweeps, double* changes_dev, hipStream_t st) override
     {
         if (sweeps < 1) { GUST_FAIL("gust: zline_jacobi: sweeps must be >= 1 (got %d)", sweeps); }
-        zero_u64_kernel<<<(nsys + 255) / 256, 256, 0, st>>>(chg, nsys);
-        GUST_HIP(hipGetLastError());
+        if (!chg_zeroed) {
+            zero_u64_kernel<<<(nsys + 255) / 256, 256, 0, st>>>(chg, nsys);
+            GUST_HIP(hipGetLastError());
+        }
         // delta = 0 outside the domain (check_line_system's setVal).
         for (int s = 0; s < nsys; ++s) { zero_ghosts<T>(sys[size_t(s)].delta, valid[size_t(s)], st); }
         // The sweeps ping-pong between delta and the scratch iterate, arranged
@@ -574,10 +577,16 @@ void zjacobi_run(int nsys, const gust_line_system* sys, int sweeps, double* chan
     if (sweeps < 1) { GUST_FAIL("gust: zline_jacobi: sweeps must be >= 1 (got %d)", sweeps); }
     JacobiPlan<T> plan;
     plan.init(nsys, sys, false, st);
+    Deferred* dq = change_host ? Deferred::current() : nullptr;
+    if (dq && nsys == 1) {
+        // the sweep records the change straight into a slot of the batch,
+        // read with its other values
+        plan.chg = dq->zero_slot([change_host](unsigned long long b) { *change_host = bits_to_double(b); });
+        plan.chg_zeroed = true;
+    }
     plan.run(sweeps, changes_dev, st);
-    if (!change_host) { return; }
-    if (Deferred* dq = Deferred::current()) {
-        // the change travels with the batch's other values
+    if (!change_host || plan.chg_zeroed) { return; }
+    if (dq) {
         unsigned long long* slot = dq->zero_slot([change_host](unsigned long long b) { *change_host = bits_to_double(b); });
         copy_u64_kernel<<<1, 64, 0, st>>>(plan.chg, slot, 1);
         GUST_HIP(hipGetLastError());

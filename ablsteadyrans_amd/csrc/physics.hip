@@ -367,8 +367,9 @@ __device__ __forceinline__ void gg_column(const GgMap& m, int i, int j, int& si,
 }
 
 template <int T>
-__global__ __launch_bounds__(kThreads) void fill_z_kernel(Box3 cols, V a, NodeZ z, int edge_lo, int edge_hi,
-                                                         FaceFill lo, FaceFill hi, SSV ss, SurfOpt so, GgMap gm)
+__device__ __forceinline__ void fill_z_columns(const Box3& cols, const V& a, const NodeZ& z, int edge_lo, int edge_hi,
+                                               const FaceFill& lo, const FaceFill& hi, const SSV& ss,
+                                               const SurfOpt& so, const GgMap& gm)
 {
     GUST_GRID_LOOP(cols)
     {
@@ -409,6 +410,29 @@ __global__ __launch_bounds__(kThreads) void fill_z_kernel(Box3 cols, V a, NodeZ 
     }
 }
 
+// The z-face fills of up to six fields in one launch (blockIdx.y: the field;
+// the fields are independent of one another).
+struct FzField {
+    V a;
+    Box3 cols;
+    int loc, edge_lo, edge_hi;
+    FaceFill lo, hi;
+};
+struct FzSet {
+    FzField f[6];
+};
+
+__global__ __launch_bounds__(kThreads) void fill_z_kernel(FzSet s, NodeZ z, SSV ss, SurfOpt so, GgMap gm)
+{
+    const FzField& F = s.f[blockIdx.y];
+    switch (F.loc) {
+    case sc::XF: fill_z_columns<sc::XF>(F.cols, F.a, z, F.edge_lo, F.edge_hi, F.lo, F.hi, ss, so, gm); break;
+    case sc::YF: fill_z_columns<sc::YF>(F.cols, F.a, z, F.edge_lo, F.edge_hi, F.lo, F.hi, ss, so, gm); break;
+    case sc::ZF: fill_z_columns<sc::ZF>(F.cols, F.a, z, F.edge_lo, F.edge_hi, F.lo, F.hi, ss, so, gm); break;
+    default: fill_z_columns<sc::CC>(F.cols, F.a, z, F.edge_lo, F.edge_hi, F.lo, F.hi, ss, so, gm); break;
+    }
+}
+
 // w = z_x u_f + z_y v_f on the ground and top faces (Omega = 0), ghosts
 // mirrored about it (BoundaryConditions.cpp:756-797; MassFlux_K.H:31-39).
 __global__ __launch_bounds__(kThreads) void no_flux_w_kernel(Box3 cols, V w, CV u, CV v, CV z, int edge_lo,
@@ -440,9 +464,9 @@ struct SideFill {
 };
 
 template <int T>
-__global__ __launch_bounds__(kThreads) void fill_side_kernel(Box3 plane, V a, int d, int edge_lo, int edge_hi,
-                                                            SideFill flo, SideFill fhi, NodeZ z,
-                                                            profile::ProfileView prof, int f, int klo)
+__device__ __forceinline__ void fill_side_plane(const Box3& plane, const V& a, int d, int edge_lo, int edge_hi,
+                                                const SideFill& flo, const SideFill& fhi, const NodeZ& z,
+                                                const profile::ProfileView& prof, int f, int klo)
 {
     GUST_GRID_LOOP(plane)
     {
@@ -478,12 +502,48 @@ __global__ __launch_bounds__(kThreads) void fill_side_kernel(Box3 plane, V a, in
     }
 }
 
+// The side fills of the six fields along one direction in one launch
+// (blockIdx.y: the field; the fields are independent of one another).
+struct SideField {
+    V a;
+    Box3 plane;
+    int loc, f, edge_lo, edge_hi;
+    SideFill lo, hi;
+};
+struct SideSet {
+    SideField f[6];
+};
+
+__global__ __launch_bounds__(kThreads) void fill_side_kernel(SideSet s, int d, NodeZ z, profile::ProfileView prof,
+                                                            int klo)
+{
+    const SideField& F = s.f[blockIdx.y];
+    switch (F.loc) {
+    case sc::XF: fill_side_plane<sc::XF>(F.plane, F.a, d, F.edge_lo, F.edge_hi, F.lo, F.hi, z, prof, F.f, klo); break;
+    case sc::YF: fill_side_plane<sc::YF>(F.plane, F.a, d, F.edge_lo, F.edge_hi, F.lo, F.hi, z, prof, F.f, klo); break;
+    case sc::ZF: fill_side_plane<sc::ZF>(F.plane, F.a, d, F.edge_lo, F.edge_hi, F.lo, F.hi, z, prof, F.f, klo); break;
+    default: fill_side_plane<sc::CC>(F.plane, F.a, d, F.edge_lo, F.edge_hi, F.lo, F.hi, z, prof, F.f, klo); break;
+    }
+}
+
 // FillBoundary(periodicity) on one tile: a ghost point whose periodic image
 // is a valid point takes its value.
-__global__ __launch_bounds__(kThreads) void fill_boundary_kernel(Box3 fab, V a, int vlo0, int vhi0, int vlo1,
-                                                                int vhi1, int vlo2, int vhi2, int per0, int per1,
-                                                                int nx, int ny)
+// Up to six fields per launch (blockIdx.y: the field).
+struct FbField {
+    V a;
+    Box3 fab;
+    int vlo0, vhi0, vlo1, vhi1, vlo2, vhi2;
+};
+struct FbSet {
+    FbField f[6];
+};
+
+__global__ __launch_bounds__(kThreads) void fill_boundary_kernel(FbSet s, int per0, int per1, int nx, int ny)
 {
+    const FbField& F = s.f[blockIdx.y];
+    const Box3& fab = F.fab;
+    const V& a = F.a;
+    const int vlo0 = F.vlo0, vhi0 = F.vhi0, vlo1 = F.vlo1, vhi1 = F.vhi1, vlo2 = F.vlo2, vhi2 = F.vhi2;
     GUST_GRID_LOOP(fab)
     {
         int i, j, k;
@@ -1160,14 +1220,13 @@ GgMap gg_map(const gust_geometry& g)
 // The fills along z over every lateral column of the field's fab (the ghost
 // slab of the z faces spans the lateral ghosts, BoundaryConditions.cpp:83-104).
 template <int T>
-void launch_fill_z(hipStream_t st, const gust_geometry& g, gust_array4* a, int f, const gust_physics& p,
-                   const gust_array4* z, const gust_array4* ss)
+FzField fz_field(const gust_geometry& g, gust_array4* a, int f, const gust_physics& p)
 {
-    FaceFill ff[2];
+    FzField F{};
     for (int side = 0; side < 2; ++side) {
         const bool lo = side == 0;
         const int t = lo ? p.face[2] : p.face[5];
-        FaceFill& x = ff[side];
+        FaceFill& x = lo ? F.lo : F.hi;
         x.rule = rule_for(f, t, 2, lo ? 1 : 0);
         x.V = 0;
         if (t == GUST_FACE_GEOSTROPHIC) { x.V = f == F_U ? p.forcing.ug : f == F_V ? p.forcing.vg : 0.0; }
@@ -1178,11 +1237,21 @@ void launch_fill_z(hipStream_t st, const gust_geometry& g, gust_array4* a, int f
     }
     IBox cols = view_box(a);
     cols.lo[2] = cols.hi[2] = g.lo[2];
-    const Box3 b = box3(cols);
-    const int edge_lo = g.lo[2], edge_hi = sc::nodal(T, 2) ? g.hi[2] + 1 : g.hi[2];
+    F.a = dview<double>(a);
+    F.cols = box3(cols);
+    F.loc = T;
+    F.edge_lo = g.lo[2];
+    F.edge_hi = sc::nodal(T, 2) ? g.hi[2] + 1 : g.hi[2];
+    return F;
+}
+
+void launch_fill_z(hipStream_t st, const gust_geometry& g, const FzSet& set, int n, const gust_physics& p,
+                   const gust_array4* z, const gust_array4* ss)
+{
+    int64_t most = 1;
+    for (int q = 0; q < n; ++q) { most = std::max<int64_t>(most, set.f[q].cols.npts()); }
     SSV sv = ss ? ss_view(ss) : SSV{nullptr, 0, 0, 0, 0};
-    fill_z_kernel<T><<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<double>(a), node_z(z), edge_lo, edge_hi, ff[0],
-                                                               ff[1], sv, surf_opt(p.surface), gg_map(g));
+    fill_z_kernel<<<dim3(blocks_for(most), n), kThreads, 0, st>>>(set, node_z(z), sv, surf_opt(p.surface), gg_map(g));
     GUST_HIP(hipGetLastError());
 }
 
@@ -1190,17 +1259,17 @@ void launch_fill_z(hipStream_t st, const gust_geometry& g, gust_array4* a, int f
 // whole fab along x for d = 1, the domain (or the fab where periodic) along
 // the later directions.
 template <int T>
-void launch_fill_side(hipStream_t st, const gust_geometry& g, gust_array4* a, int f, int d, const gust_physics& p,
-                      const gust_array4* z, const profile::ProfileView& prof)
+SideField side_field(const gust_geometry& g, gust_array4* a, int f, int d, const gust_physics& p)
 {
-    SideFill sf[2];
+    SideField F{};
     for (int side = 0; side < 2; ++side) {
         const bool lo = side == 0;
         const int o = lo ? d : d + 3;
         const int t = p.face[o];
-        sf[side].rule = rule_for(f, t, d, lo ? 1 : 0);
-        sf[side].V = 0;
-        if (t == GUST_FACE_NO_SLIP_WALL && f <= F_W) { sf[side].V = p.wall_velocity[o][f]; }
+        SideFill& x = lo ? F.lo : F.hi;
+        x.rule = rule_for(f, t, d, lo ? 1 : 0);
+        x.V = 0;
+        if (t == GUST_FACE_NO_SLIP_WALL && f <= F_W) { x.V = p.wall_velocity[o][f]; }
     }
     const IBox fab = view_box(a);
     IBox plane;
@@ -1215,21 +1284,47 @@ void launch_fill_side(hipStream_t st, const gust_geometry& g, gust_array4* a, in
             plane.hi[e] = g.hi[e] + (sc::nodal(T, e) ? 1 : 0);
         }
     }
-    const int edge_lo = g.lo[d], edge_hi = sc::nodal(T, d) ? g.hi[d] + 1 : g.hi[d];
-    const Box3 b = box3(plane);
-    fill_side_kernel<T><<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<double>(a), d, edge_lo, edge_hi, sf[0],
-                                                                  sf[1], node_z(z), prof, f, g.lo[2]);
+    F.a = dview<double>(a);
+    F.plane = box3(plane);
+    F.loc = T;
+    F.f = f;
+    F.edge_lo = g.lo[d];
+    F.edge_hi = sc::nodal(T, d) ? g.hi[d] + 1 : g.hi[d];
+    return F;
+}
+
+void launch_fill_sides(hipStream_t st, const gust_geometry& g, const gust_state& q, int d, const gust_physics& p,
+                       const gust_array4* z, const profile::ProfileView& prof)
+{
+    SideSet set{};
+    set.f[0] = side_field<sc::XF>(g, q.f[0], F_U, d, p);
+    set.f[1] = side_field<sc::YF>(g, q.f[1], F_V, d, p);
+    set.f[2] = side_field<sc::ZF>(g, q.f[2], F_W, d, p);
+    for (int n = 3; n < 6; ++n) { set.f[n] = side_field<sc::CC>(g, q.f[n], n, d, p); }
+    int64_t most = 1;
+    for (int n = 0; n < 6; ++n) { most = std::max<int64_t>(most, set.f[n].plane.npts()); }
+    fill_side_kernel<<<dim3(blocks_for(most), 6), kThreads, 0, st>>>(set, d, node_z(z), prof, g.lo[2]);
     GUST_HIP(hipGetLastError());
 }
 
-void launch_fill_boundary(hipStream_t st, const gust_geometry& g, gust_array4* a, int loc)
+// FillBoundary of n fields (locations locs) in one launch.
+void launch_fill_boundary(hipStream_t st, const gust_geometry& g, gust_array4* const* a, const int* locs, int n)
 {
     if (!g.periodic[0] && !g.periodic[1]) { return; }
-    const IBox vb = loc_box(g, loc);
-    const Box3 b = box3(view_box(a));
-    fill_boundary_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(
-        b, dview<double>(a), vb.lo[0], vb.hi[0], vb.lo[1], vb.hi[1], vb.lo[2], vb.hi[2], g.periodic[0] ? 1 : 0,
-        g.periodic[1] ? 1 : 0, g.hi[0] - g.lo[0] + 1, g.hi[1] - g.lo[1] + 1);
+    FbSet set{};
+    int64_t most = 1;
+    for (int q = 0; q < n; ++q) {
+        const IBox vb = loc_box(g, locs[q]);
+        FbField& F = set.f[q];
+        F.a = dview<double>(a[q]);
+        F.fab = box3(view_box(a[q]));
+        F.vlo0 = vb.lo[0]; F.vhi0 = vb.hi[0];
+        F.vlo1 = vb.lo[1]; F.vhi1 = vb.hi[1];
+        F.vlo2 = vb.lo[2]; F.vhi2 = vb.hi[2];
+        most = std::max<int64_t>(most, F.fab.npts());
+    }
+    fill_boundary_kernel<<<dim3(blocks_for(most), n), kThreads, 0, st>>>(
+        set, g.periodic[0] ? 1 : 0, g.periodic[1] ? 1 : 0, g.hi[0] - g.lo[0] + 1, g.hi[1] - g.lo[1] + 1);
     GUST_HIP(hipGetLastError());
 }
 
@@ -1252,34 +1347,34 @@ void fill_ghosts_impl(hipStream_t st, const gust_geometry& g, const gust_state& 
                       const gust_physics& p, gust_array4* ss)
 {
     const int locs[6] = {GUST_LOC_XFACE, GUST_LOC_YFACE, GUST_LOC_ZFACE, GUST_LOC_CELL, GUST_LOC_CELL, GUST_LOC_CELL};
-    for (int n = 0; n < 6; ++n) { launch_fill_boundary(st, g, q.f[n], locs[n]); }
+    launch_fill_boundary(st, g, q.f, locs, 6);
     const bool wall = wall_on(p);
     bool inflow = false;
     for (int o = 0; o < 6; ++o) { inflow = inflow || p.face[o] == GUST_FACE_INFLOW; }
     const profile::ProfileView prof = inflow ? profile::device_view(st, p.inflow) : profile::ProfileView{};
     for (int d = 0; d < 2; ++d) {
-        if (g.periodic[d]) { continue; }
-        launch_fill_side<sc::XF>(st, g, q.f[0], F_U, d, p, z, prof);
-        launch_fill_side<sc::YF>(st, g, q.f[1], F_V, d, p, z, prof);
-        launch_fill_side<sc::ZF>(st, g, q.f[2], F_W, d, p, z, prof);
-        for (int n = 3; n < 6; ++n) { launch_fill_side<sc::CC>(st, g, q.f[n], n, d, p, z, prof); }
+        if (!g.periodic[d]) { launch_fill_sides(st, g, q, d, p, z, prof); }
     }
     if (wall) { launch_surface(st, g, q.f[0], q.f[1], z, p.surface, ss); }
     const gust_array4* sv = wall ? ss : nullptr;
-    launch_fill_z<sc::XF>(st, g, q.f[0], F_U, p, z, sv);
-    launch_fill_z<sc::YF>(st, g, q.f[1], F_V, p, z, sv);
-    launch_fill_z<sc::ZF>(st, g, q.f[2], F_W, p, z, sv);
-    {
-        const Box3 b = box3(no_flux_columns(q.f[2], q.f[0], q.f[1], z, g.lo[2]));
-        no_flux_w_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(
-            b, dview<double>(q.f[2]), dview<const double>(q.f[0]), dview<const double>(q.f[1]), dview<const double>(z),
-            g.lo[2], g.hi[2] + 1, dirichlet_normal(p.face[2]) ? 1 : 0, dirichlet_normal(p.face[5]) ? 1 : 0,
-            1.0 / g.dx[0], 1.0 / g.dx[1]);
-        GUST_HIP(hipGetLastError());
-        launch_fill_boundary(st, g, q.f[2], GUST_LOC_ZFACE);
-    }
-    for (int n = 3; n < 6; ++n) { launch_fill_z<sc::CC>(st, g, q.f[n], n, p, z, sv); }
-    for (int n = 0; n < 6; ++n) { launch_fill_boundary(st, g, q.f[n], locs[n]); }
+    // The z fills of all six fields: each reads only its own field (and the
+    // surface state), so one launch does the reference's per-field passes.
+    FzSet fz{};
+    fz.f[0] = fz_field<sc::XF>(g, q.f[0], F_U, p);
+    fz.f[1] = fz_field<sc::YF>(g, q.f[1], F_V, p);
+    fz.f[2] = fz_field<sc::ZF>(g, q.f[2], F_W, p);
+    for (int n = 3; n < 6; ++n) { fz.f[n] = fz_field<sc::CC>(g, q.f[n], n, p); }
+    launch_fill_z(st, g, fz, 6, p, z, sv);
+    // w on the ground and top from u and v (their z ghosts already filled).
+    // The reference's FillBoundary of w after it is subsumed by the final one:
+    // nothing in between changes w.
+    const Box3 b = box3(no_flux_columns(q.f[2], q.f[0], q.f[1], z, g.lo[2]));
+    no_flux_w_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(
+        b, dview<double>(q.f[2]), dview<const double>(q.f[0]), dview<const double>(q.f[1]), dview<const double>(z),
+        g.lo[2], g.hi[2] + 1, dirichlet_normal(p.face[2]) ? 1 : 0, dirichlet_normal(p.face[5]) ? 1 : 0,
+        1.0 / g.dx[0], 1.0 / g.dx[1]);
+    GUST_HIP(hipGetLastError());
+    launch_fill_boundary(st, g, q.f, locs, 6);
 }
 
 Rot rotation(const gust_forcing& f)

@@ -28,6 +28,7 @@
 //    discrete operator and the converged answer are the reference's, so the
 //    iteration counts are this solver's own.
 #include "common.hpp"
+#include "deferred.hpp"
 #include "terrain_stencil.hpp"
 
 #include <algorithm>
@@ -896,6 +897,18 @@ __global__ void add_scalar_kernel_r(R* __restrict__ a, const double* __restrict_
     }
 }
 
+// *out = the bits of num / den (a reported mean).
+__global__ void ratio_bits_kernel(const double* __restrict__ num, const double* __restrict__ den,
+                                  unsigned long long* __restrict__ out)
+{
+    *out = __double_as_longlong(num[0] / den[0]);
+}
+
+__global__ void copy_bits_kernel(const double* __restrict__ src, unsigned long long* __restrict__ out)
+{
+    *out = __double_as_longlong(src[0]);
+}
+
 // Copy between a compact field and a view's valid region (dir 0: view ->
 // compact, 1: compact -> view).
 template <typename R>
@@ -1517,6 +1530,27 @@ void multi_dot(PoissonImpl& P, const double* V, int64_t ldv, int nv, const doubl
     std::memcpy(host, h, size_t(nv) * 8);
 }
 
+// out[i] = <V_i, w> for i < nv, left on the device.
+void multi_dot_dev(PoissonImpl& P, const double* V, int64_t ldv, int nv, const double* w, double* out, hipStream_t st)
+{
+    const int64_t n = P.n();
+    dot_partial_kernel<<<dim3(kRedBlocks, nv), kThreads, 0, st>>>(V, ldv, w, n, ptr<double>(P.partial));
+    sum_partial_kernel<<<nv, kThreads, 0, st>>>(ptr<double>(P.partial), kRedBlocks, out);
+    GUST_HIP(hipGetLastError());
+}
+
+// v -= sum J v / sum J on the device (a level-0 compact field); nd[0], nd[1]
+// keep the sums.
+void remove_weighted_mean(PoissonImpl& P, double* v, double* nd, hipStream_t st)
+{
+    const int64_t n = P.n();
+    multi_dot_dev(P, ptr<double>(P.L0.J), 0, 1, v, nd, st);
+    fill_kernel<double><<<blocks_for(n), kThreads, 0, st>>>(ptr<double>(P.w), 1.0, n);
+    multi_dot_dev(P, ptr<double>(P.L0.J), 0, 1, ptr<double>(P.w), nd + 1, st);
+    add_scalar_kernel_r<double><<<blocks_for(n), kThreads, 0, st>>>(v, nd, nd + 1, n);
+    GUST_HIP(hipGetLastError());
+}
+
 double norm2(PoissonImpl& P, const double* v, hipStream_t st)
 {
     double d;
@@ -1526,21 +1560,11 @@ double norm2(PoissonImpl& P, const double* v, hipStream_t st)
 
 void set_coef(PoissonImpl& P, const double* h, int nv, hipStream_t st)
 {
+    // Every upload is followed by a wait (the next residual norm) before the
+    // pinned buffer is written again.
     auto* hp = static_cast<double*>(Workspace::get_host_pinned(2, size_t(nv) * 8));
-    GUST_HIP(hipStreamSynchronize(st));   // the pinned buffer may still feed a previous copy
     std::memcpy(hp, h, size_t(nv) * 8);
     GUST_HIP(hipMemcpyAsync(P.hcoef.p, hp, size_t(nv) * 8, hipMemcpyHostToDevice, st));
-}
-
-// J-weighted mean of a level-0 compact field: sum J v / sum J.
-double weighted_mean(PoissonImpl& P, const double* v, hipStream_t st)
-{
-    const int64_t n = P.n();
-    double num, den;
-    multi_dot(P, ptr<double>(P.L0.J), 0, 1, v, &num, st);
-    fill_kernel<double><<<blocks_for(n), kThreads, 0, st>>>(ptr<double>(P.w), 1.0, n);
-    multi_dot(P, ptr<double>(P.L0.J), 0, 1, ptr<double>(P.w), &den, st);
-    return num / den;
 }
 
 void check_poisson_options(const gust_poisson_options& o)
@@ -1583,12 +1607,11 @@ void check_bc(const PoissonImpl& P, const int* bc)
 
 // Check and bring in the face coefficients; form a beta on level 0 and the
 // z_cross correction (Source/Poisson.cpp:421-515).
-void prepare(PoissonImpl& P, const gust_face_coeffs* fc, const BcInfo& bc, hipStream_t st)
+void prepare(PoissonImpl& P, const gust_face_coeffs* fc, const BcInfo& bc, Deferred& dq, hipStream_t st)
 {
     const gust_array4* b[4] = {nullptr, nullptr, nullptr, nullptr};
     if (fc) { b[0] = fc->x; b[1] = fc->y; b[2] = fc->z; b[3] = fc->z_cross; }
     const char* bname[4] = {"FaceCoeffs::x", "FaceCoeffs::y", "FaceCoeffs::z", "FaceCoeffs::z_cross"};
-    auto* flag = static_cast<unsigned long long*>(Workspace::get(st, 10, 16));
     for (int d = 0; d < 4; ++d) {
         if (!b[d]) { continue; }
         const int dir = d < 3 ? d : 2;
@@ -1598,15 +1621,16 @@ void prepare(PoissonImpl& P, const gust_face_coeffs* fc, const BcInfo& bc, hipSt
             fb.hi[q] = P.geom.hi[q] + (q == dir ? 1 : 0);
         }
         if (!covers(b[d], fb, 1)) { GUST_FAIL("gust: %s must be a face MultiFab on the mesh's box array", bname[d]); }
-        fill_kernel<unsigned long long><<<1, 1, 0, st>>>(flag, 0ull, 1);
+        // the flag is read with the batch's other values (at the latest when
+        // the solve reads its initial residual)
+        const char* name = bname[d];
+        unsigned long long* flag = dq.zero_slot([name](unsigned long long h) {
+            if (h) { GUST_FAIL("gust: %s must be finite and > 0 everywhere", name); }
+        });
         const double* p = static_cast<const double*>(b[d]->p) + offset_of(*b[d], fb.lo[0], fb.lo[1], fb.lo[2]);
         const int ex = fb.len(0), ey = fb.len(1), ez = fb.len(2);
         minmax_kernel<<<blocks_for(int64_t(ex) * ey * ez), kThreads, 0, st>>>(p, b[d]->jstride, b[d]->kstride, ex, ey,
                                                                              ez, flag);
-        unsigned long long h = 0;
-        GUST_HIP(hipMemcpyAsync(&h, flag, 8, hipMemcpyDeviceToHost, st));
-        GUST_HIP(hipStreamSynchronize(st));
-        if (h) { GUST_FAIL("gust: %s must be finite and > 0 everywhere", bname[d]); }
     }
     if (b[3] && bc.dir_ground) { GUST_FAIL("gust: FaceCoeffs::z_cross with a Dirichlet ground is not supported"); }
     if (fc && (!std::isfinite(fc->precond_ratio) || !(fc->precond_ratio > 0))) {
@@ -1699,7 +1723,7 @@ void ensure_work(PoissonImpl& P, int m)
     P.krylov_v.ensure(size_t(n) * 8 * size_t(m + 1));
     P.krylov_z.ensure(size_t(n) * 8 * size_t(m));
     P.partial.ensure(size_t(kRedBlocks) * 8 * size_t(m + 2));
-    P.dots.ensure(size_t(m + 2) * 8 + 64);
+    P.dots.ensure(size_t(2 * (m + 2) + 8) * 8);
     P.hcoef.ensure(size_t(m + 2) * 8 + 64);
 }
 
@@ -1829,7 +1853,8 @@ int gust_poisson_apply(gust_poisson* p, gust_array4* Aphi, gust_array4* phi, con
         check_cells(P, Aphi, "Aphi");
         check_bc(P, bc);
         const BcInfo b = bc_info(bc);
-        prepare(P, coeffs, b, st);
+        DeferredUse dq(st);
+        prepare(P, coeffs, b, *dq, st);
         phi_in_out(P, phi, ptr<double>(P.L0.x), 0, st);
         apply_fp64(P, b, ptr<double>(P.L0.x), ptr<double>(P.L0.r), st);
         view_copy_kernel<double><<<blocks_for(P.n()), kThreads, 0, st>>>(
@@ -1837,6 +1862,7 @@ int gust_poisson_apply(gust_poisson* p, gust_array4* Aphi, gust_array4* phi, con
             Aphi->jstride, Aphi->kstride, ptr<double>(P.L0.r), P.nx, P.ny, P.nz, 1);
         fill_phi_ghosts(P, phi, b, st);
         GUST_HIP(hipGetLastError());
+        dq.finish();
     });
 }
 
@@ -1850,7 +1876,8 @@ int gust_poisson_fluxes(gust_poisson* p, gust_array4* phi, const gust_face_coeff
         check_phi(P, phi, "phi");
         check_bc(P, bc);
         const BcInfo b = bc_info(bc);
-        prepare(P, coeffs, b, st);
+        DeferredUse dq(st);
+        prepare(P, coeffs, b, *dq, st);
         phi_in_out(P, phi, ptr<double>(P.L0.x), 0, st);
         const Ctx<double> c = P.L0.ctx(b);
         const MirrorAcc<double> m = mirror<double>(ptr<double>(P.L0.x), P.nx, P.ny, P.nz, P.per, b);
@@ -1875,6 +1902,7 @@ int gust_poisson_fluxes(gust_poisson* p, gust_array4* phi, const gust_face_coeff
             GUST_HIP(hipGetLastError());
         }
         fill_phi_ghosts(P, phi, b, st);
+        dq.finish();
     });
 }
 
@@ -1889,27 +1917,44 @@ int gust_poisson_solve(gust_poisson* p, gust_array4* rhs, gust_array4* phi, cons
         check_phi(P, phi, "phi");
         check_cells(P, rhs, "rhs");
         check_bc(P, bc);
+        if ((fx || fy || fz) && !(fx && fy && fz)) {
+            GUST_FAIL("gust: PoissonSolver::solve: give all three flux arrays or none");
+        }
+        for (int d = 0; d < 3 && fx; ++d) {
+            gust_array4* F[3] = {fx, fy, fz};
+            IBox fb;
+            for (int q = 0; q < 3; ++q) { fb.lo[q] = P.geom.lo[q]; fb.hi[q] = P.geom.hi[q] + (q == d); }
+            if (!covers(F[d], fb, 1)) {
+                GUST_FAIL("gust: PoissonSolver::solve: fluxes must be face MultiFabs on the mesh's box array, "
+                          "without ghost cells");
+            }
+        }
         const BcInfo b = bc_info(bc);
-        prepare(P, coeffs, b, st);
+        // Host reads: the coefficient checks and the reported values go with
+        // the caller's batch; the initial residual resolves it (one wait),
+        // each Arnoldi step reads its dot products with one more.
+        DeferredUse dq(st);
+        prepare(P, coeffs, b, *dq, st);
         const int64_t n = P.n();
         const int m = std::max(1, P.opt.restart);
         ensure_work(P, m);
-        gust_poisson_result res{};
+        auto acc = std::make_shared<gust_poisson_result>();
         double* bvec = ptr<double>(P.L0.b);
         double* x = ptr<double>(P.L0.x);
         double* r = ptr<double>(P.L0.r);
+        double* hv_d = ptr<double>(P.dots);
+        double* hv2_d = hv_d + (m + 2);
+        double* hn_d = hv2_d + (m + 2);
+        double* mean_d = hn_d + 2;
         phi_in_out(P, rhs, bvec, 0, st);
-        res.singular = b.singular;
+        acc->singular = b.singular;
         if (b.singular) {
             // J is the left null vector: only the mean-free part of rhs is in
             // the range of A (Source/Poisson.cpp:559-567); rhs is modified.
-            res.removed_mean = weighted_mean(P, bvec, st);
-            auto* hp = static_cast<double*>(Workspace::get_host_pinned(3, 8));
-            hp[0] = res.removed_mean;
-            GUST_HIP(hipMemcpyAsync(P.dots.p, hp, 8, hipMemcpyHostToDevice, st));
-            add_scalar_kernel<<<blocks_for(n), kThreads, 0, st>>>(bvec, ptr<double>(P.dots), -1.0, n);
+            remove_weighted_mean(P, bvec, mean_d, st);
+            unsigned long long* slot = dq->zero_slot([acc](unsigned long long v) { acc->removed_mean = bits_to_double(v); });
+            ratio_bits_kernel<<<1, 1, 0, st>>>(mean_d, mean_d + 1, slot);
             phi_in_out(P, rhs, bvec, 1, st);
-            GUST_HIP(hipStreamSynchronize(st));
         }
         if (P.opt.precision == 1) { setup_hierarchy<float>(P, P.mg32, b, st); }
         else { setup_hierarchy<double>(P, P.mg64, b, st); }
@@ -1918,35 +1963,52 @@ int gust_poisson_solve(gust_poisson* p, gust_array4* rhs, gust_array4* phi, cons
         double* V = ptr<double>(P.krylov_v);
         double* Z = ptr<double>(P.krylov_z);
         double* w = ptr<double>(P.w);
-        const double r0 = norm2(P, bvec, st);
-        res.initial_residual = r0;
+        double r0sq = 0;
+        {
+            multi_dot_dev(P, bvec, 0, 1, bvec, hn_d, st);
+            unsigned long long* slot = dq->zero_slot([&r0sq](unsigned long long v) { r0sq = bits_to_double(v); });
+            copy_bits_kernel<<<1, 1, 0, st>>>(hn_d, slot);
+            GUST_HIP(hipGetLastError());
+            dq->resolve();
+        }
+        const double r0 = std::sqrt(r0sq);
+        acc->initial_residual = r0;
         const double target = std::max(P.opt.reltol * r0, P.opt.abstol);
         int iters = 0;
         bool converged = r0 <= target;
         std::vector<double> H(size_t(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), hv(m + 2), hv2(m + 2);
         GUST_HIP(hipMemcpyAsync(r, bvec, size_t(n) * 8, hipMemcpyDeviceToDevice, st));
         double beta = r0;
+        auto* hdots = static_cast<double*>(Workspace::get_host_pinned(1, size_t(2 * (m + 2) + 1) * 8));
         while (!converged && iters < P.opt.max_iter) {
             scale_copy_kernel<<<blocks_for(n), kThreads, 0, st>>>(r, 1.0 / beta, V, n);
             std::fill(g.begin(), g.end(), 0.0);
             g[0] = beta;
             int j = 0;
+            double hprev = 0;   // |w| of the previous step: V_j = w / |w| is formed at the start of step j
             for (; j < m && iters < P.opt.max_iter; ++j) {
                 double* vj = V + int64_t(j) * n;
                 double* zj = Z + int64_t(j) * n;
+                if (j > 0 && hprev > 0) { scale_copy_kernel<<<blocks_for(n), kThreads, 0, st>>>(w, 1.0 / hprev, vj, n); }
                 precondition(P, vj, zj, b, st);
                 apply_fp64(P, b, zj, w, st);
-                // classical Gram-Schmidt, twice
-                multi_dot(P, V, n, j + 1, w, hv.data(), st);
-                set_coef(P, hv.data(), j + 1, st);
-                multi_axpy_kernel<<<blocks_for(n), kThreads, 0, st>>>(w, V, n, ptr<double>(P.hcoef), j + 1, -1.0, n);
-                multi_dot(P, V, n, j + 1, w, hv2.data(), st);
-                set_coef(P, hv2.data(), j + 1, st);
-                multi_axpy_kernel<<<blocks_for(n), kThreads, 0, st>>>(w, V, n, ptr<double>(P.hcoef), j + 1, -1.0, n);
+                // classical Gram-Schmidt, twice, the coefficients staying on
+                // the device; one read of both passes and |w|^2
+                multi_dot_dev(P, V, n, j + 1, w, hv_d, st);
+                multi_axpy_kernel<<<blocks_for(n), kThreads, 0, st>>>(w, V, n, hv_d, j + 1, -1.0, n);
+                multi_dot_dev(P, V, n, j + 1, w, hv2_d, st);
+                multi_axpy_kernel<<<blocks_for(n), kThreads, 0, st>>>(w, V, n, hv2_d, j + 1, -1.0, n);
+                multi_dot_dev(P, w, 0, 1, w, hn_d, st);
+                GUST_HIP(hipMemcpyAsync(hdots, hv_d, size_t(2 * (m + 2) + 1) * 8, hipMemcpyDeviceToHost, st));
+                GUST_HIP(hipStreamSynchronize(st));
+                for (int i = 0; i <= j; ++i) {
+                    hv[i] = hdots[i];
+                    hv2[i] = hdots[(m + 2) + i];
+                }
                 for (int i = 0; i <= j; ++i) { H[size_t(i) + size_t(j) * (m + 1)] = hv[i] + hv2[i]; }
-                const double hn = norm2(P, w, st);
+                const double hn = std::sqrt(hdots[2 * (m + 2)]);
                 H[size_t(j + 1) + size_t(j) * (m + 1)] = hn;
-                if (hn > 0) { scale_copy_kernel<<<blocks_for(n), kThreads, 0, st>>>(w, 1.0 / hn, V + int64_t(j + 1) * n, n); }
+                hprev = hn;
                 // Givens rotations
                 for (int i = 0; i < j; ++i) {
                     double& a = H[size_t(i) + size_t(j) * (m + 1)];
@@ -1984,28 +2046,31 @@ int gust_poisson_solve(gust_poisson* p, gust_array4* rhs, gust_array4* phi, cons
             beta = norm2(P, r, st);
             converged = beta <= target;
         }
-        res.iterations = iters;
-        if (b.singular) {
-            const double mean = weighted_mean(P, x, st);
-            auto* hp = static_cast<double*>(Workspace::get_host_pinned(3, 8));
-            GUST_HIP(hipStreamSynchronize(st));
-            hp[0] = mean;
-            GUST_HIP(hipMemcpyAsync(P.dots.p, hp, 8, hipMemcpyHostToDevice, st));
-            add_scalar_kernel<<<blocks_for(n), kThreads, 0, st>>>(x, ptr<double>(P.dots), -1.0, n);
-        }
+        acc->iterations = iters;
+        if (b.singular) { remove_weighted_mean(P, x, mean_d, st); }
         launch_apply<double>(P.L0, b, P.per, x, bvec, r, st);
-        res.final_residual = norm2(P, r, st);
+        {
+            multi_dot_dev(P, r, 0, 1, r, hn_d, st);
+            unsigned long long* slot = dq->zero_slot([acc](unsigned long long v) {
+                acc->final_residual = std::sqrt(bits_to_double(v));
+            });
+            copy_bits_kernel<<<1, 1, 0, st>>>(hn_d, slot);
+            GUST_HIP(hipGetLastError());
+        }
         phi_in_out(P, phi, x, 1, st);
         fill_phi_ghosts(P, phi, b, st);
-        if (result) { *result = res; }
+        if (result) {
+            dq->then([acc, result] { *result = *acc; });
+        }
         if (!converged) {
-            GUST_HIP(hipStreamSynchronize(st));
+            dq->resolve();
+            const int max_iter = P.opt.max_iter;
+            const double reltol = P.opt.reltol, abstol = P.opt.abstol;
             GUST_FAIL("gust: the Poisson solve did not converge in solver.poisson_max_iter = %d iterations (residual "
                       "%g of %g; solver.poisson_reltol = %g, solver.poisson_abstol = %g)",
-                      P.opt.max_iter, res.final_residual, res.initial_residual, P.opt.reltol, P.opt.abstol);
+                      max_iter, acc->final_residual, acc->initial_residual, reltol, abstol);
         }
-        if (fx || fy || fz) {
-            if (!(fx && fy && fz)) { GUST_FAIL("gust: PoissonSolver::solve: give all three flux arrays or none"); }
+        if (fx) {
             const Ctx<double> c = P.L0.ctx(b);
             const MirrorAcc<double> mm = mirror<double>(x, P.nx, P.ny, P.nz, P.per, b);
             gust_array4* F[3] = {fx, fy, fz};
@@ -2014,10 +2079,6 @@ int gust_poisson_solve(gust_poisson* p, gust_array4* rhs, gust_array4* phi, cons
             for (int d = 0; d < 3; ++d) {
                 IBox fb;
                 for (int q = 0; q < 3; ++q) { fb.lo[q] = P.geom.lo[q]; fb.hi[q] = P.geom.hi[q] + (q == d); }
-                if (!covers(F[d], fb, 1)) {
-                    GUST_FAIL("gust: PoissonSolver::solve: fluxes must be face MultiFabs on the mesh's box array, "
-                              "without ghost cells");
-                }
                 const double* bp = beta3[d] ? static_cast<const double*>(beta3[d]->p) +
                                                   offset_of(*beta3[d], fb.lo[0], fb.lo[1], fb.lo[2])
                                             : nullptr;
@@ -2029,7 +2090,7 @@ int gust_poisson_solve(gust_poisson* p, gust_array4* rhs, gust_array4* phi, cons
                 GUST_HIP(hipGetLastError());
             }
         }
-        GUST_HIP(hipStreamSynchronize(st));
+        dq.finish();
     });
 }
 
@@ -2042,12 +2103,14 @@ int gust_poisson_vcycle(gust_poisson* p, const double* b, double* x, const gust_
         PoissonImpl& P = p->impl;
         check_bc(P, bc);
         const BcInfo bi = bc_info(bc);
+        DeferredUse dq(st);
         if (setup) {
-            prepare(P, coeffs, bi, st);
+            prepare(P, coeffs, bi, *dq, st);
             if (P.opt.precision == 1) { setup_hierarchy<float>(P, P.mg32, bi, st); }
             else { setup_hierarchy<double>(P, P.mg64, bi, st); }
         }
         precondition(P, b, x, bi, st);
+        dq.finish();
     });
 }
 

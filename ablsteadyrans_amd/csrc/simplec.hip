@@ -34,20 +34,6 @@ __device__ __forceinline__ double sc_min(double a, double b) { return (b < a) ? 
 // ---------------------------------------------------------------------------
 // Box kernels.
 
-__global__ __launch_bounds__(kThreads) void nonneg_flag_kernel(Box3 b, CV d, unsigned int* flag)
-{
-    unsigned long long bad = 0;
-    GUST_GRID_LOOP(b)
-    {
-        int i, j, k;
-        b.at(t, i, j, k);
-        const double x = d(i, j, k);
-        if (!(x >= 0) || !(x < INFINITY)) { bad = 1; }
-    }
-    bad = dev::block_max_u64<kThreads>(bad);
-    if (threadIdx.x == 0 && bad) { atomicOr(flag, 1u); }
-}
-
 __global__ __launch_bounds__(kThreads) void beta_keep_kernel(Box3 b, CV d, V beta, V keep)
 {
     GUST_GRID_LOOP(b)
@@ -60,36 +46,63 @@ __global__ __launch_bounds__(kThreads) void beta_keep_kernel(Box3 b, CV d, V bet
     }
 }
 
-// max |a (* m)| over the box, as ordered bits into *out (atomic max).
-__global__ __launch_bounds__(kThreads) void max_abs_kernel(Box3 b, CV a, CV m, int masked,
-                                                            unsigned long long* out)
+// Several reductions in one launch (blockIdx.y: the job), each into its
+// slot: max |a|, max |a m|, min a (as ordered bits of |a|; a > 0 expected),
+// or a flag (1) where a is negative, NaN or infinite.
+enum { kMaxAbs = 0, kMaxAbsMasked = 1, kMinPos = 2, kNonNeg = 3 };
+constexpr int kJobs = 8;
+struct RJob {
+    Box3 b;
+    CV a, m;
+    int op;
+    unsigned long long* out;
+};
+struct RJobs {
+    RJob j[kJobs];
+};
+
+__global__ __launch_bounds__(kThreads) void reduce_jobs_kernel(RJobs s)
+{
+    const RJob& J = s.j[blockIdx.y];
+    const Box3& b = J.b;
+    const int op = J.op;
+    unsigned long long best = op == kMinPos ? ~0ull : 0ull;
+    GUST_GRID_LOOP(b)
+    {
+        int i, j, k;
+        b.at(t, i, j, k);
+        const double x = J.a(i, j, k);
+        if (op == kNonNeg) {
+            if (!(x >= 0) || !(x < INFINITY)) { best = 1; }
+        } else {
+            const unsigned long long bits = dev::abs_bits(op == kMaxAbsMasked ? x * J.m(i, j, k) : x);
+            if (op == kMinPos) { best = bits < best ? bits : best; } else { best = bits > best ? bits : best; }
+        }
+    }
+    if (op == kMinPos) {
+        best = dev::block_min_u64<kThreads>(best);
+        if (threadIdx.x == 0) { atomicMin(J.out, best); }
+    } else {
+        best = dev::block_max_u64<kThreads>(best);
+        if (threadIdx.x == 0 && best) { atomicMax(J.out, best); }
+    }
+}
+
+// y += x and max |x| into *out (atomic max of ordered bits).
+__global__ __launch_bounds__(kThreads) void add_max_kernel(Box3 b, V y, CV x, unsigned long long* out)
 {
     unsigned long long best = 0;
     GUST_GRID_LOOP(b)
     {
         int i, j, k;
         b.at(t, i, j, k);
-        const double x = masked ? a(i, j, k) * m(i, j, k) : a(i, j, k);
-        const unsigned long long bits = dev::abs_bits(x);
+        const double v = x(i, j, k);
+        y(i, j, k) = y(i, j, k) + v;
+        const unsigned long long bits = dev::abs_bits(v);
         best = bits > best ? bits : best;
     }
     best = dev::block_max_u64<kThreads>(best);
     if (threadIdx.x == 0 && best) { atomicMax(out, best); }
-}
-
-// min a over the box (a > 0), as ordered bits into *out (atomic min).
-__global__ __launch_bounds__(kThreads) void min_pos_kernel(Box3 b, CV a, unsigned long long* out)
-{
-    unsigned long long best = ~0ull;
-    GUST_GRID_LOOP(b)
-    {
-        int i, j, k;
-        b.at(t, i, j, k);
-        const unsigned long long bits = dev::abs_bits(a(i, j, k));
-        best = bits < best ? bits : best;
-    }
-    best = dev::block_min_u64<kThreads>(best);
-    if (threadIdx.x == 0) { atomicMin(out, best); }
 }
 
 // y op= x (op 0: y += a x, 1: y *= x, 2: y -= x, 3: y = 0).
@@ -369,37 +382,34 @@ Scratch scratch(hipStream_t st, int slot, const IBox& valid, int g)
 }
 
 // Reductions into a deferred batch: each registers a slot and the callback
-// that receives its value when the batch resolves.
+// that receives its value when the batch resolves. Requests queue up and
+// run() reduces them all in one launch (at most kJobs; a full queue runs by
+// itself): the caller runs the queue before anything overwrites a field
+// that is in it.
 class Reducer {
 public:
     explicit Reducer(Deferred& dq) : dq_(dq), st_(dq.stream()) {}
+    Reducer(const Reducer&) = delete;
+    Reducer& operator=(const Reducer&) = delete;
+
     void max_abs(const IBox& box, const gust_array4* a, std::function<void(double)> f,
                  const gust_array4* mask = nullptr)
     {
-        const Box3 b = box3(box);
         unsigned long long* slot = dq_.zero_slot([f](unsigned long long v) { f(bits_to_double(v)); });
-        max_abs_kernel<<<cgrid::reduce_blocks(b.npts()), kThreads, 0, st_>>>(b, dview<const double>(a),
-                                                                             dview<const double>(mask ? mask : a),
-                                                                             mask ? 1 : 0, slot);
-        GUST_HIP(hipGetLastError());
+        push(box, a, mask, mask ? kMaxAbsMasked : kMaxAbs, slot);
     }
     void min_pos(const IBox& box, const gust_array4* a, std::function<void(double)> f)
     {
-        const Box3 b = box3(box);
         unsigned long long* slot = dq_.ones_slot([f](unsigned long long v) { f(bits_to_double(v)); });
-        min_pos_kernel<<<cgrid::reduce_blocks(b.npts()), kThreads, 0, st_>>>(b, dview<const double>(a), slot);
-        GUST_HIP(hipGetLastError());
+        push(box, a, nullptr, kMinPos, slot);
     }
     // Fails with msg when a is negative, NaN or infinite anywhere.
     void check_nonneg(const IBox& box, const gust_array4* a, std::string msg)
     {
-        const Box3 b = box3(box);
         unsigned long long* slot = dq_.zero_slot([msg](unsigned long long v) {
-            if (v & 0xffffffffull) { GUST_FAIL("%s", msg.c_str()); }
+            if (v) { GUST_FAIL("%s", msg.c_str()); }
         });
-        nonneg_flag_kernel<<<cgrid::reduce_blocks(b.npts()), kThreads, 0, st_>>>(b, dview<const double>(a),
-                                                                                 reinterpret_cast<unsigned int*>(slot));
-        GUST_HIP(hipGetLastError());
+        push(box, a, nullptr, kNonNeg, slot);
     }
     // Fails with msg unless a is finite and > 0 everywhere.
     void check_positive(const IBox& box, const gust_array4* a, const std::string& msg)
@@ -409,10 +419,41 @@ public:
             if (!(m > 0)) { GUST_FAIL("%s", msg.c_str()); }
         });
     }
+    // y += x with max |x| reported (one pass).
+    void add_max_abs(const IBox& box, const gust_array4* y, const gust_array4* x, std::function<void(double)> f)
+    {
+        const Box3 b = box3(box);
+        unsigned long long* slot = dq_.zero_slot([f](unsigned long long v) { f(bits_to_double(v)); });
+        add_max_kernel<<<cgrid::reduce_blocks(b.npts()), kThreads, 0, st_>>>(b, dview<double>(y),
+                                                                             dview<const double>(x), slot);
+        GUST_HIP(hipGetLastError());
+    }
+    void run()
+    {
+        if (n_ == 0) { return; }
+        int64_t most = 1;
+        for (int q = 0; q < n_; ++q) { most = std::max<int64_t>(most, jobs_.j[q].b.npts()); }
+        reduce_jobs_kernel<<<dim3(cgrid::reduce_blocks(most), n_), kThreads, 0, st_>>>(jobs_);
+        GUST_HIP(hipGetLastError());
+        n_ = 0;
+    }
 
 private:
+    void push(const IBox& box, const gust_array4* a, const gust_array4* m, int op, unsigned long long* slot)
+    {
+        if (n_ == kJobs) { run(); }
+        RJob& J = jobs_.j[n_++];
+        J.b = box3(box);
+        J.a = dview<const double>(a);
+        J.m = dview<const double>(m ? m : a);
+        J.op = op;
+        J.out = slot;
+    }
+
     Deferred& dq_;
     hipStream_t st_;
+    RJobs jobs_{};
+    int n_ = 0;
 };
 
 void box_op(hipStream_t st, const IBox& box, const gust_array4* y, const gust_array4* x, double a, int op)
@@ -529,11 +570,15 @@ int gust_pressure_correction(gust_poisson* solver, const gust_geometry* geom, gu
         }
         check_rc(gust_velocity_divergence(geom, u, v, w, z_nd, ax, ay, az, detJ, &rhs.v, stream));
         red.max_abs(cells, &rhs.v, [vals](double x) { vals->res.divergence_before = x; });
-        if (source) { box_op(st, cells, &rhs.v, source, 0.0, 2); }
+        if (source) {
+            red.run();
+            box_op(st, cells, &rhs.v, source, 0.0, 2);
+        }
         red.max_abs(fb[0], u, [vals](double x) { vals->speed[0] = x; });
         red.max_abs(fb[1], v, [vals](double x) { vals->speed[1] = x; });
         red.max_abs(fb[2], w, [vals](double x) { vals->speed[2] = x; });
         red.min_pos(cells, detJ, [vals](double x) { vals->jmin = x; });
+        red.run();
         gust_face_coeffs coeffs{&beta[0].v, &beta[1].v, &beta[2].v, nullptr, 1.0};
         box_op(st, grow(cells, 1, 1, 1), &phi.v, nullptr, 0.0, 3);
         {
@@ -566,6 +611,7 @@ int gust_pressure_correction(gust_poisson* solver, const gust_geometry* geom, gu
         red.max_abs(fb[0], &F[0].v, [vals](double x) { vals->res.max_du = x; }, &keep[0].v);
         red.max_abs(fb[1], &F[1].v, [vals](double x) { vals->res.max_dv = x; }, &keep[1].v);
         red.max_abs(fb[2], &F[2].v, [vals](double x) { vals->res.max_dw = x; }, &keep[2].v);
+        red.run();
         for (int d = 0; d < 3; ++d) { box_op(st, fb[d], &F[d].v, &keep[d].v, 0.0, 1); }
         check_rc(gust_omega_from_w(geom, u, v, w, z_nd, &omega.v, 0, stream));
         box_add(st, fb[0], u, &F[0].v);
@@ -584,6 +630,7 @@ int gust_pressure_correction(gust_poisson* solver, const gust_geometry* geom, gu
             vals->res.divergence_after = x;
             if (result) { *result = vals->res; }
         });
+        red.run();
         dq.finish();
     });
 }
@@ -622,6 +669,7 @@ int gust_momentum_predictor(const gust_geometry* geom, gust_faces* vel, const gu
             // finite and > 0: the non-negative flag and a zero minimum both fail
             red.check_positive(fb, dtau->c[c], "gust: momentum_predictor: dtau_faces must be finite and > 0 everywhere");
         }
+        red.run();
         const bool rotation = opt->coriolis_f != 0;
         const bool buoyancy = opt->gravity > 0;
         if ((rotation || buoyancy) && !mask) {
@@ -724,9 +772,10 @@ int gust_momentum_predictor(const gust_geometry* geom, gust_faces* vel, const gu
                                       last[c], stream));
             if (dmax[c]) {
                 double* out = dmax[c];
-                red.max_abs(fb, &delta.v, [out](double x) { *out = x; });
+                red.add_max_abs(fb, vel->c[c], &delta.v, [out](double x) { *out = x; });
+            } else {
+                box_add(st, fb, vel->c[c], &delta.v);
             }
-            box_add(st, fb, vel->c[c], &delta.v);
         }
         dq.finish();
     });
@@ -753,6 +802,7 @@ int gust_scalar_sweep(const gust_geometry* geom, int32_t eq, gust_array4* field,
         Reducer red(*dq);
         const IBox cells = loc_box(g, GUST_LOC_CELL);
         red.check_positive(cells, dtau, "gust: scalar_sweep: dtau must be finite and > 0 everywhere");
+        red.run();
         const Box3 b = box3(cells);
         Scratch ap = scratch(st, 51, cells, 0), rhs = scratch(st, 52, cells, 0), delta = scratch(st, 53, cells, 1);
         negate_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<const double>(R_eq), dview<double>(&rhs.v));
@@ -762,8 +812,11 @@ int gust_scalar_sweep(const gust_geometry* geom, int32_t eq, gust_array4* field,
         if (result) { *result = gust_scalar_sweep_result{}; }
         check_rc(gust_line_sweeps(sweep, GUST_LOC_CELL, A, &ap.v, &rhs.v, sweeps, geom, &delta.v,
                                   result ? &result->last_sweep : nullptr, stream));
-        if (result) { red.max_abs(cells, &delta.v, [result](double x) { result->max_dq = x; }); }
-        box_add(st, cells, field, &delta.v);
+        if (result) {
+            red.add_max_abs(cells, field, &delta.v, [result](double x) { result->max_dq = x; });
+        } else {
+            box_add(st, cells, field, &delta.v);
+        }
         if (eq == GUST_EQ_TKE && clip) {
             unsigned long long* cnt = dq->zero_slot([result](unsigned long long v) {
                 if (result) { result->clipped = static_cast<int64_t>(v); }
