@@ -28,6 +28,7 @@
 #include "deferred.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -40,6 +41,12 @@ namespace gust_amd {
 namespace {
 
 constexpr int kWave = 64;
+constexpr int kStagedMaxTiles = 1024;
+
+// The z-line Jacobi schedule: 0 automatic, 1 streaming, 2 LDS-staged (where
+// it fits).
+std::atomic<int> g_jacobi_schedule{0};
+int jacobi_schedule() { return g_jacobi_schedule.load(std::memory_order_relaxed); }
 
 // One system of a Jacobi launch. Pointers are at the valid box's low corner.
 template <typename T>
@@ -256,6 +263,130 @@ __global__ __launch_bounds__(kWave) void zjacobi_sweep_kernel(const JacobiSys<T>
     }
 }
 
+// The same sweep for launches with too few tiles to hide memory latency
+// (small tiles: one wave per CU or fewer). A block of kStagedThreads stages
+// one tile's operands in LDS first -- every level of every column loaded at
+// once, the neighbour terms summed into r_k in the order of the streaming
+// kernel -- and its first wave then runs the columns' Thomas recurrences
+// (Source/LineSolve_K.H:29-44) out of LDS: the serial chain waits on LDS
+// instead of one memory latency per level. Results are bitwise those of
+// zjacobi_sweep_kernel. LDS: [array][k][W] with W >= the tile's active
+// columns; arrays d_P, lower, upper, r (c' and e' overwrite upper and r)
+// and, for the last sweep, the old iterate.
+constexpr int kStagedThreads = 256;
+
+template <typename T, bool kFirst, bool kLast>
+__global__ __launch_bounds__(kStagedThreads) void zjacobi_staged_kernel(const JacobiSys<T>* __restrict__ table,
+                                                                        int nsys, int total_tiles, int dst_scratch,
+                                                                        unsigned long long* __restrict__ changes,
+                                                                        int W)
+{
+    extern __shared__ unsigned char lds_raw[];
+    const int tile = dev::xcd_remap(blockIdx.x, total_tiles);
+    const int si = find_sys(table, nsys, tile);
+    const JacobiSys<T> S = table[si];
+    const int ncols = S.nx * S.ny;
+    const int c0 = (tile - S.tile0) * kWave;
+    const int nact = ncols - c0 < kWave ? ncols - c0 : kWave;
+    const int nz = S.nz;
+    const int64_t plane = int64_t(nz) * W;
+    T* Ldg = reinterpret_cast<T*>(lds_raw);
+    T* Llo = Ldg + plane;
+    T* Lup = Llo + plane;
+    T* Lr = Lup + plane;
+    T* Lold = Lr + plane;
+    constexpr bool kOld = kLast && !kFirst;
+    const T* src = dst_scratch ? S.d : S.s;
+    T* dst = dst_scratch ? S.s : S.d;
+    const int64_t src_j = dst_scratch ? S.d_j : S.s_j, src_k = dst_scratch ? S.d_k : S.s_k;
+    const int64_t dst_j = dst_scratch ? S.s_j : S.d_j, dst_k = dst_scratch ? S.s_k : S.d_k;
+    // 1. stage: thread t takes column t % nact of level t / nact (consecutive
+    // threads, consecutive i: coalesced rows)
+    for (int t = threadIdx.x; t < nz * nact; t += kStagedThreads) {
+        const int lane = t % nact, k = t / nact;
+        const int c = c0 + lane;
+        const int i = c % S.nx, j = c / S.nx;
+        const T* ak0 = S.A + i + int64_t(j) * S.a_j + k * S.a_k;
+        const int64_t an = S.a_n;
+        const T axm = ldg(ak0 + 1 * an), axp = ldg(ak0 + 2 * an), aym = ldg(ak0 + 3 * an), ayp = ldg(ak0 + 4 * an);
+        const T azm = ldg(ak0 + 5 * an), azp = ldg(ak0 + 6 * an);
+        const T dg = ldg(S.ap + i + int64_t(j) * S.p_j + k * S.p_k);
+        const T rr = ldg(S.r + i + int64_t(j) * S.r_j + k * S.r_k);
+        T vm = T(0), vp = T(0), vjm = T(0), vjp = T(0);
+        if (!kFirst) {
+            int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1;
+            bool okm = true, okp = true, okjm = true, okjp = true;
+            if (im < 0) { if (S.Lx) { im += S.Lx; } else { okm = false; im = i; } }
+            if (ip > S.nx - 1) { if (S.Lx) { ip -= S.Lx; } else { okp = false; ip = i; } }
+            if (jm < 0) { if (S.Ly) { jm += S.Ly; } else { okjm = false; jm = j; } }
+            if (jp > S.ny - 1) { if (S.Ly) { jp -= S.Ly; } else { okjp = false; jp = j; } }
+            const int64_t kk = k * src_k;
+            const T tm = gld(src + im + int64_t(j) * src_j + kk), tp = gld(src + ip + int64_t(j) * src_j + kk);
+            const T tjm = gld(src + i + int64_t(jm) * src_j + kk), tjp = gld(src + i + int64_t(jp) * src_j + kk);
+            vm = okm ? tm : T(0);
+            vp = okp ? tp : T(0);
+            vjm = okjm ? tjm : T(0);
+            vjp = okjp ? tjp : T(0);
+        }
+        const int64_t at = int64_t(k) * W + lane;
+        Ldg[at] = dg;
+        Llo[at] = -azm;
+        Lup[at] = -azp;
+        Lr[at] = rr + axm * vm + axp * vp + aym * vjm + ayp * vjp;
+        if (kOld) { Lold[at] = gld(src + i + int64_t(j) * src_j + k * src_k); }
+    }
+    __syncthreads();
+    // 2. the recurrences, one lane per column of the first wave
+    if (threadIdx.x >= kWave) { return; }
+    const int lane = threadIdx.x;
+    unsigned long long chg = 0;
+    if (lane < nact) {
+        const int c = c0 + lane;
+        const int i = c % S.nx, j = c / S.nx;
+        T* out = dst + i + int64_t(j) * dst_j;
+        T cprev = T(0), eprev = T(0);
+        for (int k = 0; k < nz; ++k) {
+            const int64_t at = int64_t(k) * W + lane;
+            const T lower = Llo[at], upper = Lup[at], dg = Ldg[at], rk = Lr[at];
+            T ck, ek;
+            if (k == 0) {
+                const T m = dg;
+                ck = upper / m;
+                ek = rk / m;
+            } else {
+                const T m = dg - lower * cprev;
+                ck = upper / m;
+                ek = (rk - lower * eprev) / m;
+            }
+            Lup[at] = ck;
+            Lr[at] = ek;
+            cprev = ck;
+            eprev = ek;
+        }
+        T o = eprev;
+        gst(out + (nz - 1) * dst_k, o);
+        if (kLast) {
+            const T old = kFirst ? T(0) : Lold[int64_t(nz - 1) * W + lane];
+            const unsigned long long b = dev::abs_bits(double(o - old));
+            chg = b > chg ? b : chg;
+        }
+        for (int k = nz - 2; k >= 0; --k) {
+            const int64_t at = int64_t(k) * W + lane;
+            o = Lr[at] - Lup[at] * o;
+            gst(out + k * dst_k, o);
+            if (kLast) {
+                const T old = kFirst ? T(0) : Lold[at];
+                const unsigned long long b = dev::abs_bits(double(o - old));
+                chg = b > chg ? b : chg;
+            }
+        }
+    }
+    if (kLast) {
+        chg = dev::wave_max_u64(chg);
+        if (lane == 0 && chg) { atomicMax(changes + si, chg); }
+    }
+}
+
 // The copy of a periodic seam face OverrideSync keeps (Source/Predictor.cpp:125
 // with AMReX's owner mask: the low copy owns): delta(hi seam) = delta(lo seam).
 template <typename T>
@@ -381,6 +512,7 @@ struct JacobiPlan final : JacobiPlanBase {
     unsigned long long* chg = nullptr;
     bool owned = false;   // buffers from hipMalloc (a plan handle), not the stream workspace
     bool chg_zeroed = false;   // chg already holds zeros (a fresh deferred slot)
+    int staged_w = 0;          // > 0: the LDS-staged schedule, W columns per LDS row
 
     ~JacobiPlan() override
     {
@@ -433,8 +565,24 @@ struct JacobiPlan final : JacobiPlanBase {
             nzmax = std::max(nzmax, t.nz);
         }
         const size_t lds = size_t(kWave) * size_t(nzmax) * sizeof(T);
-        // Schedule: c' in LDS (default) or in global memory (GUST_JACOBI_CPRIME
-        // = global: no LDS limit on occupancy, c' round-trips through L2).
+        // Schedule: few tiles (under kStagedMaxTiles, about four waves per
+        // CU or fewer) take the LDS-staged kernel when a tile's operands fit
+        // the LDS; many tiles the streaming kernel (loads of level k + kP in
+        // flight while level k is eliminated).
+        {
+            int wmax = 1;
+            for (const auto& t : tab) {
+                const int64_t cols = int64_t(t.nx) * t.ny;
+                wmax = int(std::max<int64_t>(wmax, std::min<int64_t>(cols, kWave)));
+            }
+            const size_t staged = 5 * size_t(nzmax) * size_t(wmax) * sizeof(T);
+            const int sched = jacobi_schedule();
+            const bool fits = staged <= device_info().lds_per_block;
+            if (fits && (sched == 2 || (sched == 0 && tiles < kStagedMaxTiles))) { staged_w = wmax; }
+        }
+        // c' of the streaming kernel: in LDS (default) or in global memory
+        // (GUST_JACOBI_CPRIME = global: no LDS limit on occupancy, c'
+        // round-trips through L2).
         const char* cp = std::getenv("GUST_JACOBI_CPRIME");
         const bool force_global = cp && std::strcmp(cp, "global") == 0;
         const bool use_lds = !force_global && lds <= device_info().lds_per_block;
@@ -497,6 +645,17 @@ struct JacobiPlan final : JacobiPlanBase {
     template <bool F, bool L>
     void launch(int dst_scratch, hipStream_t st)
     {
+        if (staged_w > 0) {
+            const size_t lds = size_t(L && !F ? 5 : 4) * size_t(nzmax) * size_t(staged_w) * sizeof(T);
+            if (lds > 65536) {
+                GUST_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&zjacobi_staged_kernel<T, F, L>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+            }
+            zjacobi_staged_kernel<T, F, L><<<tiles, kStagedThreads, lds, st>>>(dtab, nsys, tiles, dst_scratch, chg,
+                                                                              staged_w);
+            GUST_HIP(hipGetLastError());
+            return;
+        }
         if (cglob) {
             zjacobi_sweep_kernel<T, F, L, false><<<tiles, kWave, 0, st>>>(dtab, nsys, tiles, dst_scratch, chg, cglob,
                                                                        nzmax);
@@ -1024,6 +1183,16 @@ int gust_jacobi_plan_sweep(gust_jacobi_plan* plan, int first, int last, int into
     return guarded([&] {
         if (!plan || !plan->impl) { GUST_FAIL("gust: jacobi_plan_sweep: plan is null"); }
         plan->impl->sweep(first != 0, last != 0, into_scratch ? 1 : 0, (hipStream_t)stream);
+    });
+}
+
+int gust_jacobi_set_schedule(int32_t schedule)
+{
+    return guarded([&] {
+        if (schedule < 0 || schedule > 2) {
+            GUST_FAIL("gust: jacobi_set_schedule: schedule must be 0 (automatic), 1 (streaming) or 2 (staged)");
+        }
+        g_jacobi_schedule.store(schedule, std::memory_order_relaxed);
     });
 }
 

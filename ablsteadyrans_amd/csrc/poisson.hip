@@ -1956,8 +1956,9 @@ int gust_poisson_solve(gust_poisson* p, gust_array4* rhs, gust_array4* phi, cons
             ratio_bits_kernel<<<1, 1, 0, st>>>(mean_d, mean_d + 1, slot);
             phi_in_out(P, rhs, bvec, 1, st);
         }
-        if (P.opt.precision == 1) { setup_hierarchy<float>(P, P.mg32, b, st); }
-        else { setup_hierarchy<double>(P, P.mg64, b, st); }
+        // The multigrid hierarchy is set up at the first preconditioner call:
+        // a right-hand side already within tolerance needs none.
+        bool hierarchy = false;
         // GMRES(m), right preconditioned, zero initial guess.
         fill_kernel<double><<<blocks_for(n), kThreads, 0, st>>>(x, 0.0, n);
         double* V = ptr<double>(P.krylov_v);
@@ -1990,6 +1991,11 @@ int gust_poisson_solve(gust_poisson* p, gust_array4* rhs, gust_array4* phi, cons
                 double* vj = V + int64_t(j) * n;
                 double* zj = Z + int64_t(j) * n;
                 if (j > 0 && hprev > 0) { scale_copy_kernel<<<blocks_for(n), kThreads, 0, st>>>(w, 1.0 / hprev, vj, n); }
+                if (!hierarchy) {
+                    if (P.opt.precision == 1) { setup_hierarchy<float>(P, P.mg32, b, st); }
+                    else { setup_hierarchy<double>(P, P.mg64, b, st); }
+                    hierarchy = true;
+                }
                 precondition(P, vj, zj, b, st);
                 apply_fp64(P, b, zj, w, st);
                 // classical Gram-Schmidt, twice, the coefficients staying on

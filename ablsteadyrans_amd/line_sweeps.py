@@ -68,6 +68,16 @@ def check_line_options(opt: LineSweepOptions) -> list[str]:
     return [_lib.lib.gust_last_error().decode()]
 
 
+JACOBI_AUTO, JACOBI_STREAMING, JACOBI_STAGED = 0, 1, 2
+
+
+def set_jacobi_schedule(schedule: int) -> None:
+    """The z-line Jacobi kernel schedule for plans made afterwards (process
+    wide): JACOBI_AUTO (few tiles staged in LDS, many streamed),
+    JACOBI_STREAMING or JACOBI_STAGED. Every schedule gives the same bits."""
+    _lib.check(_lib.lib.gust_jacobi_set_schedule(int(schedule)))
+
+
 class LineSystem:
     """One system of a batched launch: a field at ``loc`` on a tile."""
 

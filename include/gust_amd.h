@@ -132,6 +132,13 @@ int gust_jacobi_plan_run(gust_jacobi_plan* plan, int sweeps, double* changes_dev
 int gust_jacobi_plan_sweep(gust_jacobi_plan* plan, int first, int last, int into_scratch, void* stream);
 /* Unknowns and 64-column tiles of the plan. */
 int gust_jacobi_plan_info(const gust_jacobi_plan* plan, int64_t* unknowns, int32_t* tiles);
+/* The z-line Jacobi kernel schedule for plans made after the call (process
+ * wide): 0 automatic (launches of fewer than 1024 tiles stage each tile's
+ * operands in LDS before its Thomas recurrences, more tiles stream them),
+ * 1 streaming always, 2 staged wherever a tile fits the LDS. Every schedule
+ * gives the same bits.
+ * No counterpart in the reference (a launch-shape choice of this library). */
+int gust_jacobi_set_schedule(int32_t schedule);
 void gust_jacobi_plan_destroy(gust_jacobi_plan* plan);
 
 /* Line LU-SGS sweeps of one system with the given options.

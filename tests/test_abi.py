@@ -130,3 +130,12 @@ def test_mass_flux_refusals_before_any_device_call():
     m.z_nd = G.Fab(np.zeros((1, 6, 7, 8)), (0, 0, 0))
     with pytest.raises(G.GustError, match="z_nd must be the mesh's node heights"):
         MF.w_from_omega(m, u1, v1, om, om, stream=0)
+
+
+def test_jacobi_schedule_refuses_unknown_values():
+    import ablsteadyrans_amd as LS
+    for bad in (-1, 3):
+        with pytest.raises(Exception, match="jacobi_set_schedule: schedule must be 0"):
+            LS.set_jacobi_schedule(bad)
+    for ok in (LS.JACOBI_STREAMING, LS.JACOBI_STAGED, LS.JACOBI_AUTO):
+        LS.set_jacobi_schedule(ok)

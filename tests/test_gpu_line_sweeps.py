@@ -17,6 +17,16 @@ CASES = [((6, 5, 4), (0, 0)), ((6, 5, 4), (1, 1)), ((7, 4, 5), (1, 0)), ((5, 6, 
          ((8, 1, 4), (1, 1)), ((1, 1, 9), (0, 0)), ((33, 17, 12), (1, 0)), ((64, 64, 48), (1, 0))]
 
 
+@pytest.fixture(params=[1, 2], ids=["streaming", "staged"])
+def schedule(request, cuda):
+    """Both z-line Jacobi kernel schedules (the streaming ring and the
+    LDS-staged tile) must give the oracle's bits."""
+    import ablsteadyrans_amd as LS
+    LS.set_jacobi_schedule(request.param)
+    yield request.param
+    LS.set_jacobi_schedule(LS.JACOBI_AUTO)
+
+
 def _dev(arr, begin, dtype=None):
     import torch
     import ablsteadyrans_amd as G
@@ -41,7 +51,7 @@ def _gpu_jacobi(n, loc, periodic, A, ap, rhs, sweeps, dtype=None, ngrow=1):
 
 @pytest.mark.parametrize("n,periodic", CASES)
 @pytest.mark.parametrize("loc", LOCS)
-def test_jacobi_fp64_is_bit_identical_to_the_oracle(cuda, n, periodic, loc):
+def test_jacobi_fp64_is_bit_identical_to_the_oracle(schedule, n, periodic, loc):
     A, ap, rhs = L.random_system(n, loc, periodic, seed=100 + loc)
     for sweeps in (1, 2, 3):
         ref, ref_ch, ref_full = L.oracle_jacobi(n, loc, periodic, A, ap, rhs, sweeps)
@@ -55,7 +65,7 @@ def test_jacobi_fp64_is_bit_identical_to_the_oracle(cuda, n, periodic, loc):
 
 
 @pytest.mark.parametrize("loc", LOCS)
-def test_jacobi_fp32_is_within_tolerance(cuda, loc):
+def test_jacobi_fp32_is_within_tolerance(schedule, loc):
     import torch
     n, periodic = (33, 17, 24), (1, 0)
     A, ap, rhs = L.random_system(n, loc, periodic, seed=7 + loc)
@@ -138,7 +148,7 @@ def test_line_sweeps_dispatcher_routes_the_options(cuda):
         assert np.array_equal(got[0, 1:-1, 1:-1, 1:-1], ref)
 
 
-def test_batched_jacobi_equals_the_single_calls(cuda):
+def test_batched_jacobi_equals_the_single_calls(schedule):
     """Uneven tiles and all four locations in one launch per sweep."""
     import torch
     import ablsteadyrans_amd as G
@@ -163,7 +173,7 @@ def test_batched_jacobi_equals_the_single_calls(cuda):
         assert c == ref_ch
 
 
-def test_jacobi_on_tall_columns(cuda):
+def test_jacobi_on_tall_columns(schedule):
     """nz at the config sizes (48, 64, 96) and beyond (200 levels)."""
     for nz in (96, 200):
         n, periodic = (70, 3, nz), (0, 1)
