@@ -269,6 +269,27 @@ def _time_reference_decks():
     return out
 
 
+def _profile_deck(name, iterations):
+    """iterations driver iterations of one reference deck after 10 warm-up
+    iterations (the state back at the first guess in between)."""
+    import torch
+    from ablsteadyrans_amd import decks as D
+    make = {"column": D.neutral_column, "askervein": lambda: D.askervein(_askervein_grid()),
+            "abl": D.neutral_abl_flat, "hill3d": D.neutral_hill_3d}[name]
+    deck = D.prepare(make())
+    deck.driver.max_iterations = 10
+    deck.solve()
+    D.reinitialize(deck)
+    deck.driver.max_iterations = iterations
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = deck.solve()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"profile_deck": deck.name, "iterations": int(r.iterations), "seconds": round(dt, 4),
+            "ms_per_iteration": round(dt * 1e3 / max(int(r.iterations), 1), 4)}
+
+
 def _hill_tile():
     """The Gaussian hill h = 100 exp(-(r/250 m)^2) m centred in 4 x 4 km at
     dx = 20 m, 64 levels from dz1 = 1 m growing by 1.08, periodic sides."""
@@ -366,12 +387,14 @@ def _cpu_column_baseline(target_s=15.0):
 
 
 def _traffic_from_profile():
-    p = os.path.join(ROOT, "profiles", "zjacobi_sweep_c3_fp64_pmc.json")
-    if os.path.exists(p):
-        try:
-            return json.load(open(p)).get("hbm_bytes_per_launch")
-        except Exception:
-            return None
+    """HBM bytes per launch of the FP64 sweep from the committed counter
+    summary (its 'hbm_bytes_per_launch:' line), or None."""
+    p = os.path.join(ROOT, "profiles", "zjacobi_sweep_fractal_256x256x96_pmc.md")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("hbm_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
     return None
 
 
@@ -382,6 +405,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1, help="untimed steady solves per precision")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--profile-only", action="store_true", help="sweeps only (for rocprofv3 runs)")
+    ap.add_argument("--profile-deck", choices=("column", "askervein", "abl", "hill3d"),
+                    help="only N driver iterations of one reference deck (for rocprofv3 runs)")
+    ap.add_argument("--iterations", type=int, default=100, help="driver iterations of --profile-deck")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only a plain 1 GiB device copy, repeated (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -403,6 +429,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8,
                           "launches": sweep_steps}))
+        return
+    if args.profile_deck:
+        print(json.dumps(_profile_deck(args.profile_deck, args.iterations)))
         return
     if args.profile_only:
         sys64 = _systems(torch.float64, 20264203)
