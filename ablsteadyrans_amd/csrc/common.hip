@@ -67,7 +67,19 @@ struct Buf {
 
 std::mutex g_ws_mutex;
 std::map<std::tuple<int, hipStream_t, int>, Buf> g_ws;
-std::map<int, Buf> g_pinned;
+
+// Pinned host buffers are per host thread: threads driving their own
+// streams read back concurrently without sharing one.
+struct PinnedSet {
+    std::map<int, Buf> bufs;
+    ~PinnedSet()
+    {
+        for (auto& kv : bufs) {
+            if (kv.second.p) { (void)hipHostFree(kv.second.p); }
+        }
+    }
+};
+thread_local PinnedSet t_pinned;
 
 }   // namespace
 
@@ -93,8 +105,7 @@ void* Workspace::get(hipStream_t s, int slot, size_t bytes)
 
 void* Workspace::get_host_pinned(int slot, size_t bytes)
 {
-    std::lock_guard<std::mutex> lk(g_ws_mutex);
-    Buf& b = g_pinned[slot];
+    Buf& b = t_pinned.bufs[slot];
     if (b.cap < bytes) {
         if (b.p) {
             GUST_HIP(hipDeviceSynchronize());

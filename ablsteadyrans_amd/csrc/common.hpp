@@ -113,6 +113,7 @@ void check_nodal_heights(const char* op, const gust_array4* z, const gust_geomet
 class Workspace {
 public:
     static void* get(hipStream_t s, int slot, size_t bytes);
+    // Pinned host memory, one buffer per (host thread, slot).
     static void* get_host_pinned(int slot, size_t bytes);
 };
 

@@ -110,12 +110,13 @@ private:
     std::vector<std::pair<int, Callback>> cbs_;
 };
 
-// The batch an entry point reports into: the thread's current one, or its
-// own (resolved by finish()).
+// The batch an entry point reports into: the thread's current one when it
+// is on the same stream, or its own (resolved by finish()).
 class DeferredUse {
 public:
-    explicit DeferredUse(hipStream_t st) : own_(st), cur_(Deferred::current() ? Deferred::current() : &own_),
-                                           scope_(*cur_) {}
+    explicit DeferredUse(hipStream_t st)
+        : own_(st), cur_(Deferred::current() && Deferred::current()->stream() == st ? Deferred::current() : &own_),
+          scope_(*cur_) {}
     Deferred& operator*() { return *cur_; }
     Deferred* operator->() { return cur_; }
     // Resolves the batch if this entry point opened it.

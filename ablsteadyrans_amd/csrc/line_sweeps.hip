@@ -637,7 +637,10 @@ struct JacobiPlan final : JacobiPlanBase {
         }
         JacobiSys<T>* dev = nullptr;
         GUST_HIP(hipMalloc(&dev, bytes));
-        GUST_HIP(hipMemcpy(dev, tab.data(), bytes, hipMemcpyHostToDevice));
+        // on the sweep's stream (a null-stream copy need not finish before
+        // that stream's kernels start), complete before the host returns
+        GUST_HIP(hipMemcpyAsync(dev, tab.data(), bytes, hipMemcpyHostToDevice, st));
+        GUST_HIP(hipStreamSynchronize(st));
         list.push_back(Entry{std::move(key), dev});
         return dev;
     }
@@ -736,7 +739,8 @@ void zjacobi_run(int nsys, const gust_line_system* sys, int sweeps, double* chan
     if (sweeps < 1) { GUST_FAIL("gust: zline_jacobi: sweeps must be >= 1 (got %d)", sweeps); }
     JacobiPlan<T> plan;
     plan.init(nsys, sys, false, st);
-    Deferred* dq = change_host ? Deferred::current() : nullptr;
+    Deferred* dq = change_host && Deferred::current() && Deferred::current()->stream() == st ? Deferred::current()
+                                                                                               : nullptr;
     if (dq && nsys == 1) {
         // the sweep records the change straight into a slot of the batch,
         // read with its other values

@@ -210,7 +210,8 @@ inline ProfileView device_view(hipStream_t st, const gust_inflow_profile& p)
             e.cap = std::max<size_t>(total, 16);
         }
         GUST_HIP(hipStreamSynchronize(st));   // kernels still reading the previous tables
-        GUST_HIP(hipMemcpy(e.dev, h, total * sizeof(double), hipMemcpyHostToDevice));
+        GUST_HIP(hipMemcpyAsync(e.dev, h, total * sizeof(double), hipMemcpyHostToDevice, st));
+        GUST_HIP(hipStreamSynchronize(st));
         e.key = std::move(key);
     }
     ProfileView v{};

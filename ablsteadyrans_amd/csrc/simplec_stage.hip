@@ -55,11 +55,13 @@ struct Field {
     {
         if (p) { (void)hipFree(p); }
     }
-    void alloc(const IBox& b, int ncomp)
+    // Zeroed on the stream that will use the field (stream order, not the
+    // null stream's: the library's streams need not wait for it).
+    void alloc(const IBox& b, int ncomp, hipStream_t st)
     {
         const int64_t n = b.npts();
         GUST_HIP(hipMalloc(&p, size_t(n) * ncomp * sizeof(double)));
-        GUST_HIP(hipMemset(p, 0, size_t(n) * ncomp * sizeof(double)));
+        GUST_HIP(hipMemsetAsync(p, 0, size_t(n) * ncomp * sizeof(double), st));
         v.p = p;
         v.jstride = b.len(0);
         v.kstride = int64_t(b.len(0)) * b.len(1);
@@ -299,27 +301,27 @@ int gust_simplec_create(const gust_geometry* geom, const gust_mesh* mesh, const 
         s->wall = phys->face[2] == GUST_FACE_SURFACE_LAYER;
         const gust_geometry& g = *geom;
         const IBox cells = loc_box(g, GUST_LOC_CELL);
-        for (int n = 0; n < 6; ++n) { s->R[n].alloc(loc_box(g, kLocs[n]), 1); }
-        for (int n = 0; n < 3; ++n) { s->cf[n].alloc(grow(cells, 1, 1, 1), 1); }
-        for (int n = 3; n < 7; ++n) { s->cf[n].alloc(cells, 1); }
+        for (int n = 0; n < 6; ++n) { s->R[n].alloc(loc_box(g, kLocs[n]), 1, hipStream_t(stream)); }
+        for (int n = 0; n < 3; ++n) { s->cf[n].alloc(grow(cells, 1, 1, 1), 1, hipStream_t(stream)); }
+        for (int n = 3; n < 7; ++n) { s->cf[n].alloc(cells, 1, hipStream_t(stream)); }
         s->cfv = gust_closure_fields{&s->cf[0].v, &s->cf[1].v, &s->cf[2].v, &s->cf[3].v,
                                      &s->cf[4].v, &s->cf[5].v, &s->cf[6].v};
         IBox slab = grow(cells, 1, 1, 0);
         slab.hi[2] = slab.lo[2];
-        s->ss.alloc(slab, GUST_SS_N);
+        s->ss.alloc(slab, GUST_SS_N, hipStream_t(stream));
         for (int c = 0; c < 3; ++c) {
             const IBox fb = loc_box(g, 1 << c);
-            s->F[c].alloc(grow(fb, 1, 1, 1), 1);
-            s->dtf[c].alloc(fb, 1);
-            s->A[c].alloc(fb, GUST_N_COEFS);
-            s->mask[c].alloc(fb, 1);
-            s->ap[c].alloc(fb, 1);
-            s->d[c].alloc(fb, 1);
+            s->F[c].alloc(grow(fb, 1, 1, 1), 1, hipStream_t(stream));
+            s->dtf[c].alloc(fb, 1, hipStream_t(stream));
+            s->A[c].alloc(fb, GUST_N_COEFS, hipStream_t(stream));
+            s->mask[c].alloc(fb, 1, hipStream_t(stream));
+            s->ap[c].alloc(fb, 1, hipStream_t(stream));
+            s->d[c].alloc(fb, 1, hipStream_t(stream));
         }
-        s->At.alloc(cells, GUST_N_COEFS);
-        s->Ak.alloc(cells, GUST_N_COEFS);
-        s->conv0.alloc(cells, 1);
-        s->conv1.alloc(cells, 1);
+        s->At.alloc(cells, GUST_N_COEFS, hipStream_t(stream));
+        s->Ak.alloc(cells, GUST_N_COEFS, hipStream_t(stream));
+        s->conv0.alloc(cells, 1, hipStream_t(stream));
+        s->conv1.alloc(cells, 1, hipStream_t(stream));
         // The Dirichlet rows (BoundaryConditions::dirichlet_rows,
         // Source/BoundaryConditions.cpp:852-876): each velocity's face points
         // on a face of its own direction with a Dirichlet normal value
@@ -343,7 +345,9 @@ int gust_simplec_create(const gust_geometry* geom, const gust_mesh* mesh, const 
                         }
             }
             if (any) {
-                GUST_HIP(hipMemcpy(s->mask[c].p, m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice));
+                GUST_HIP(hipMemcpyAsync(s->mask[c].p, m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice,
+                                        hipStream_t(stream)));
+                GUST_HIP(hipStreamSynchronize(hipStream_t(stream)));   // m is a local
             }
         }
         // The Poisson controls with the correction's tolerance; the types of
@@ -430,7 +434,7 @@ int gust_steady_solve(gust_simplec* s, gust_state* q, const gust_driver_options*
             ~Holder() { gust_steady_destroy(c); }
         } hold_c{crit};
         Field dt;
-        dt.alloc(loc_box(st.g, GUST_LOC_CELL), 1);
+        dt.alloc(loc_box(st.g, GUST_LOC_CELL), 1, hipStream_t(stream));
         gust_array4 nu = st.phys.use_closure ? st.cf[0].v : *st.phys.nu;   // zero before the first residual
         const int hold = opt->criterion.residual_ref_iterations;
         double cfl = opt->cfl0, r = 0;
