@@ -21,6 +21,12 @@ Gaussian-hill tile
 five variables on the fractal-terrain tile (256 x 256 x 96, 65,536 columns;
 the roofline object: algorithmic bytes per launch / launch time from HIP
 events on the launch stream, 10 reals per unknown); the V(2,2) cycle.
+Serving (extra.serving, extra.serving_p99_s): 14 requests arriving at 20 per
+second (mass-consistent winds on random terrain tiles, every seventh a
+converged steady field on a tile of the reference's 3-D hill deck) served
+with 8 concurrent streams in mixed precision, beside the same requests one
+at a time in FP64; p50/p99 latency from arrival to result. Batched
+(extra.batched): 64 random tiles at once, 8 streams vs one.
 cpu_baseline: the oracle's SIMPLEC iteration composite (FP64 C pieces and an
 exact sparse pressure solve, one host thread) on the same column deck for a
 bounded number of iterations, scaled by the deck's 2231 iterations.
@@ -290,6 +296,45 @@ def _profile_deck(name, iterations):
             "ms_per_iteration": round(dt * 1e3 / max(int(r.iterations), 1), 4)}
 
 
+def _serving_leg():
+    """The serving configuration: 14 requests arriving as a Poisson process
+    at 20 per second, every seventh a converged steady field on a tile of
+    the reference's 3-D hill deck (random wind direction and hill position;
+    the deck's criterion), the others the mass-consistent wind on a random
+    terrain tile (32..128 squared, 32..80 levels, 1-5 Gaussian hills),
+    served with 8 slots in mixed precision; then the same requests one after
+    another, each alone on one stream in FP64 (the plain per-request path).
+    Returns both summaries."""
+    import numpy as np
+    from ablsteadyrans_amd import serving as S
+    rng = np.random.default_rng(20261203 + 5000)
+    arrivals = S.poisson_arrivals(14, 20.0, rng)
+    reqs = []
+    for i in range(14):
+        spec = S.hill_tile(rng) if i % 7 == 6 else S.random_tile(rng)
+        reqs.append(S.Request(i, "steady" if i % 7 == 6 else "init", spec, arrival=float(arrivals[i]),
+                              precision=1, max_iterations=4000))
+    served, wall = S.Server(slots=8).run(reqs)
+    plain = [S.Request(r.rid, r.kind, r.spec, 0.0, 0, r.max_iterations) for r in reqs]
+    seq, wall_seq = S.Server(slots=1).run(plain)
+    return S.summary(served, wall), S.summary(seq, wall_seq)
+
+
+def _batched_leg():
+    """64 independent random terrain tiles (mass-consistent wind) submitted
+    at once: 8 concurrent streams vs one stream; tiles per second."""
+    import numpy as np
+    from ablsteadyrans_amd import serving as S
+    rng = np.random.default_rng(20261203 + 4000)
+    reqs = [S.Request(i, "init", S.random_tile(rng)) for i in range(64)]
+    res8, wall8 = S.Server(slots=8).run(reqs)
+    res1, wall1 = S.Server(slots=1).run(reqs)
+    cells = sum(r.spec.cells for r in reqs)
+    return {"tiles": 64, "cells": cells, "errors": sum(1 for r in res8 + res1 if r.error),
+            "tiles_per_s_8_streams": round(64 / wall8, 3), "tiles_per_s_1_stream": round(64 / wall1, 3),
+            "speedup": round(wall1 / wall8, 3)}
+
+
 def _hill_tile():
     """The Gaussian hill h = 100 exp(-(r/250 m)^2) m centred in 4 x 4 km at
     dx = 20 m, 64 levels from dz1 = 1 m growing by 1.08, periodic sides."""
@@ -404,6 +449,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3, help="timed steady solves per precision")
     ap.add_argument("--warmup", type=int, default=1, help="untimed steady solves per precision")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving and batched legs")
     ap.add_argument("--profile-only", action="store_true", help="sweeps only (for rocprofv3 runs)")
     ap.add_argument("--profile-deck", choices=("column", "askervein", "abl", "hill3d"),
                     help="only N driver iterations of one reference deck (for rocprofv3 runs)")
@@ -479,6 +525,12 @@ def main():
     bytes32 = REALS_PER_UNKNOWN * 4 * unknowns
     gbs64 = bytes64 / (ms_ev * 1e-3) / 1e9
     gbs32 = bytes32 / (ms32 * 1e-3) / 1e9
+    serving = batched = None
+    if not args.no_serving:
+        print("bench: serving (Poisson arrivals at 20/s) and 64 batched tiles", file=sys.stderr, flush=True)
+        served, plain = _serving_leg()
+        serving = {"served_mixed_8_slots": served, "plain_fp64_one_at_a_time": plain}
+        batched = _batched_leg()
     cpu = cpu_sweep = None
     if not args.no_cpu:
         print("bench: CPU baselines (oracle, one thread)", file=sys.stderr, flush=True)
@@ -522,8 +574,10 @@ def main():
                   "fp64_vcycle_ms": round(vc64, 4), "fp64_vcycle_gbs": round(_vcycle_bytes(8) / (vc64 * 1e-3) / 1e9, 1),
                   "fp32_vcycle_ms": round(vc32, 4), "fp32_vcycle_gbs": round(_vcycle_bytes(4) / (vc32 * 1e-3) / 1e9, 1),
                   "reference_decks": decks,
-                  "cpu_sweep_baseline": cpu_sweep,
-                  "note": "p99 needs the serving loop (not built)"},
+                  "serving": serving,
+                  "serving_p99_s": serving["served_mixed_8_slots"]["all"]["p99_s"] if serving else None,
+                  "batched": batched,
+                  "cpu_sweep_baseline": cpu_sweep},
     }
     print(json.dumps(line))
 
