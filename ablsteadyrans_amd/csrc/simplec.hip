@@ -88,6 +88,12 @@ __global__ __launch_bounds__(kThreads) void reduce_jobs_kernel(RJobs s)
     }
 }
 
+// *out = the bits of *src (a device value into a deferred slot).
+__global__ void bits_to_slot_kernel(const double* src, unsigned long long* out)
+{
+    *out = static_cast<unsigned long long>(__double_as_longlong(*src));
+}
+
 // y += x and max |x| into *out (atomic max of ordered bits).
 __global__ __launch_bounds__(kThreads) void add_max_kernel(Box3 b, V y, CV x, unsigned long long* out)
 {
@@ -689,6 +695,13 @@ int gust_momentum_predictor(const gust_geometry* geom, gust_faces* vel, const gu
             last[2] = &result->last_sweep_dw;
         }
         const double dxi0 = 1.0 / g.dx[0], dxi1 = 1.0 / g.dx[1];
+        // The three z-line Jacobi solves share one launch per sweep: their
+        // systems (a_P, d, rhs) come from R, A and dtau only, never from the
+        // updated velocities, so solving them together gives the bits of the
+        // reference's one-after-the-other loop.
+        const bool batched = opt->sweep.kind == GUST_SWEEP_ZJACOBI;
+        const int rslot[3] = {44, 55, 57}, dslot[3] = {45, 56, 58};
+        gust_line_system sys[3];
         for (int c = 0; c < 3; ++c) {
             const int loc = 1 << c;
             const IBox fb = loc_box(g, loc);
@@ -713,7 +726,7 @@ int gust_momentum_predictor(const gust_geometry* geom, gust_faces* vel, const gu
                     mk, m ? 1 : 0, dview<double>(ap->c[c]), dview<double>(d->c[c]), 1.0 / cap, margin, dxi0, dxi1,
                     margin > 0 ? 1 : 0);
             }
-            Scratch rhs = scratch(st, 44, fb, 0), delta = scratch(st, 45, fb, 1);
+            Scratch rhs = scratch(st, batched ? rslot[c] : 44, fb, 0), delta = scratch(st, batched ? dslot[c] : 45, fb, 1);
             negate_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<const double>(R->c[c]),
                                                                      dview<double>(&rhs.v));
             GUST_HIP(hipGetLastError());
@@ -768,6 +781,10 @@ int gust_momentum_predictor(const gust_geometry* geom, gust_faces* vel, const gu
                     dview<double>(d->c[c]), dview<double>(&rhs.v), gt, g.lo[2], g.hi[2] + 1);
                 GUST_HIP(hipGetLastError());
             }
+            if (batched) {
+                sys[c] = gust_line_system{loc, g, *A->c[c], *ap->c[c], rhs.v, delta.v};
+                continue;
+            }
             check_rc(gust_line_sweeps(&opt->sweep, loc, A->c[c], ap->c[c], &rhs.v, opt->sweeps, geom, &delta.v,
                                       last[c], stream));
             if (dmax[c]) {
@@ -776,6 +793,25 @@ int gust_momentum_predictor(const gust_geometry* geom, gust_faces* vel, const gu
             } else {
                 box_add(st, fb, vel->c[c], &delta.v);
             }
+        }
+        if (batched) {
+            auto* chd = static_cast<double*>(Workspace::get(st, 59, 3 * sizeof(double)));
+            check_rc(gust_zline_jacobi_batched_async(3, sys, opt->sweeps, chd, stream));
+            for (int c = 0; c < 3; ++c) {
+                if (last[c]) {
+                    double* out = last[c];
+                    unsigned long long* slot = dq->zero_slot([out](unsigned long long v) { *out = bits_to_double(v); });
+                    bits_to_slot_kernel<<<1, 1, 0, st>>>(chd + c, slot);
+                }
+                const IBox fb = loc_box(g, 1 << c);
+                if (dmax[c]) {
+                    double* out = dmax[c];
+                    red.add_max_abs(fb, vel->c[c], &sys[c].delta, [out](double x) { *out = x; });
+                } else {
+                    box_add(st, fb, vel->c[c], &sys[c].delta);
+                }
+            }
+            GUST_HIP(hipGetLastError());
         }
         dq.finish();
     });
