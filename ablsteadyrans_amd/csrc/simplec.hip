@@ -14,6 +14,7 @@
 // is reproducible; they return to the host where the reference reduces too.
 #include "cgrid.hpp"
 #include "deferred.hpp"
+#include "simplec.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -512,6 +513,78 @@ void check_rc(int rc)
 }
 
 }   // namespace
+
+// theta's and k's scalar sweeps of one SIMPLEC iteration (Source/Simplec.cpp:
+// 222-281 and 494-503: theta's, then k's with its floor clip) with the two
+// z-line Jacobi solves in one launch per sweep. Neither system depends on the
+// other field (k's rows come from the fluxes, the closure and the wall
+// state), so the bits are those of the two scalar_sweep calls in a row.
+void scalar_sweeps_theta_tke(const gust_geometry& g, gust_array4* theta, const gust_array4* R_theta,
+                             const gust_array4* A_theta, gust_array4* tke, const gust_array4* R_tke,
+                             const gust_array4* A_tke, const gust_array4* dtau, int sweeps, double floor,
+                             const gust_line_options& sweep, gust_scalar_sweep_result* res_theta,
+                             gust_scalar_sweep_result* res_tke, hipStream_t st)
+{
+    void* stream = st;
+    if (sweep.kind != GUST_SWEEP_ZJACOBI) {
+        check_rc(gust_scalar_sweep(&g, GUST_EQ_THETA, theta, R_theta, A_theta, dtau, sweeps, 0.0, 0, &sweep, res_theta,
+                                   stream));
+        check_rc(gust_scalar_sweep(&g, GUST_EQ_TKE, tke, R_tke, A_tke, dtau, sweeps, floor, 1, &sweep, res_tke, stream));
+        return;
+    }
+    const char* op = "scalar_sweep";
+    check_field(op, "field", theta, g, GUST_LOC_CELL, 1);
+    check_field(op, "field", tke, g, GUST_LOC_CELL, 1);
+    check_field(op, "R", R_theta, g, GUST_LOC_CELL, 0);
+    check_field(op, "R", R_tke, g, GUST_LOC_CELL, 0);
+    check_field(op, "dtau", dtau, g, GUST_LOC_CELL, 0);
+    check_field(op, "A", A_theta, g, GUST_LOC_CELL, 0);
+    check_field(op, "A", A_tke, g, GUST_LOC_CELL, 0);
+    if (A_theta->ncomp != GUST_N_COEFS || A_tke->ncomp != GUST_N_COEFS) {
+        GUST_FAIL("gust: scalar_sweep: A needs 7 components");
+    }
+    if (std::isnan(floor) || !(floor >= 0)) { GUST_FAIL("gust: scalar_sweep: the floor must be >= 0"); }
+    DeferredUse dq(st);
+    Reducer red(*dq);
+    const IBox cells = loc_box(g, GUST_LOC_CELL);
+    red.check_positive(cells, dtau, "gust: scalar_sweep: dtau must be finite and > 0 everywhere");
+    red.run();
+    const Box3 b = box3(cells);
+    gust_array4* field[2] = {theta, tke};
+    const gust_array4* R[2] = {R_theta, R_tke};
+    const gust_array4* A[2] = {A_theta, A_tke};
+    gust_scalar_sweep_result* res[2] = {res_theta, res_tke};
+    const int slot[2][3] = {{51, 52, 53}, {63, 64, 65}};
+    gust_line_system sys[2];
+    for (int q = 0; q < 2; ++q) {
+        Scratch ap = scratch(st, slot[q][0], cells, 0), rhs = scratch(st, slot[q][1], cells, 0),
+                delta = scratch(st, slot[q][2], cells, 1);
+        negate_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<const double>(R[q]), dview<double>(&rhs.v));
+        scalar_ap_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<const double>(A[q]),
+                                                                    dview<const double>(dtau), dview<double>(&ap.v));
+        GUST_HIP(hipGetLastError());
+        if (res[q]) { *res[q] = gust_scalar_sweep_result{}; }
+        sys[q] = gust_line_system{GUST_LOC_CELL, g, *A[q], ap.v, rhs.v, delta.v};
+    }
+    auto* chd = static_cast<double*>(Workspace::get(st, 66, 2 * sizeof(double)));
+    check_rc(gust_zline_jacobi_batched_async(2, sys, sweeps, chd, stream));
+    for (int q = 0; q < 2; ++q) {
+        gust_scalar_sweep_result* r = res[q];
+        if (r) {
+            unsigned long long* s = dq->zero_slot([r](unsigned long long v) { r->last_sweep = bits_to_double(v); });
+            bits_to_slot_kernel<<<1, 1, 0, st>>>(chd + q, s);
+            red.add_max_abs(cells, field[q], &sys[q].delta, [r](double x) { r->max_dq = x; });
+        } else {
+            box_add(st, cells, field[q], &sys[q].delta);
+        }
+    }
+    unsigned long long* cnt = dq->zero_slot([res_tke](unsigned long long v) {
+        if (res_tke) { res_tke->clipped = static_cast<int64_t>(v); }
+    });
+    clip_kernel<<<cgrid::reduce_blocks(b.npts()), kThreads, 0, st>>>(b, dview<double>(tke), floor, cnt);
+    GUST_HIP(hipGetLastError());
+    dq.finish();
+}
 
 }   // namespace gust_amd
 

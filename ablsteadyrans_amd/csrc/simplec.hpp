@@ -1,0 +1,17 @@
+// simplec.hpp -- steps of simplec.hip used by the SIMPLEC stage itself.
+#pragma once
+
+#include "common.hpp"
+
+namespace gust_amd {
+
+// theta's and k's scalar sweeps of one iteration with their z-line Jacobi
+// solves sharing one launch per sweep (the bits of two scalar_sweep calls:
+// theta without a clip, then k with the floor clip). Throws on failure.
+void scalar_sweeps_theta_tke(const gust_geometry& g, gust_array4* theta, const gust_array4* R_theta,
+                             const gust_array4* A_theta, gust_array4* tke, const gust_array4* R_tke,
+                             const gust_array4* A_tke, const gust_array4* dtau, int sweeps, double floor,
+                             const gust_line_options& sweep, gust_scalar_sweep_result* res_theta,
+                             gust_scalar_sweep_result* res_tke, hipStream_t st);
+
+}   // namespace gust_amd

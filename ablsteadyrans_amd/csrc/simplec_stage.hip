@@ -11,6 +11,7 @@
 //      transport, the theta and k sweeps with the rows at the corrected transport.
 #include "cgrid.hpp"
 #include "deferred.hpp"
+#include "simplec.hpp"
 
 #include <cfloat>
 #include <cmath>
@@ -216,12 +217,12 @@ void iterate_body(SimplecStage& s, gust_state* q, const gust_array4* dtau, gust_
                                                                       dview<double>(&s.R[4].v));
         GUST_HIP(hipGetLastError());
     }
+    // k's rows do not depend on theta, so both sets of rows come first and
+    // the two scalar sweeps share their line-solve launches
     check_rc(gust_implicit_rows(GUST_EQ_THETA, geom, &Ff, &s.mesh, &s.phys, cf, ssv, &s.At.v, stream));
-    check_rc(gust_scalar_sweep(geom, GUST_EQ_THETA, q->f[4], &s.R[4].v, &s.At.v, dtau, s.opt.sweeps, 0.0, 0,
-                               &s.opt.sweep, &inf.theta, stream));
     check_rc(gust_implicit_rows(GUST_EQ_TKE, geom, &Ff, &s.mesh, &s.phys, cf, ssv, &s.Ak.v, stream));
-    check_rc(gust_scalar_sweep(geom, GUST_EQ_TKE, q->f[5], &s.R[5].v, &s.Ak.v, dtau, s.opt.sweeps, s.opt.tke_floor, 1,
-                               &s.opt.sweep, &inf.tke, stream));
+    scalar_sweeps_theta_tke(*geom, q->f[4], &s.R[4].v, &s.At.v, q->f[5], &s.R[5].v, &s.Ak.v, dtau, s.opt.sweeps,
+                            s.opt.tke_floor, s.opt.sweep, &inf.theta, &inf.tke, st);
 }
 
 // One iteration with its checks and reported values read back together at
