@@ -313,7 +313,7 @@ def _serving_leg():
     for i in range(14):
         spec = S.hill_tile(rng) if i % 7 == 6 else S.random_tile(rng)
         reqs.append(S.Request(i, "steady" if i % 7 == 6 else "init", spec, arrival=float(arrivals[i]),
-                              precision=1, max_iterations=4000))
+                              precision=1, max_iterations=6000))
     served, wall = S.Server(slots=8).run(reqs)
     plain = [S.Request(r.rid, r.kind, r.spec, 0.0, 0, r.max_iterations) for r in reqs]
     seq, wall_seq = S.Server(slots=1).run(plain)
