@@ -549,14 +549,16 @@ __global__ __launch_bounds__(64) void zebra_kernel(Split s, int color, int perx,
 // lane composes its levels' maps, a shuffle scan across the wave composes
 // the lanes', and each lane then replays its levels from its carry-in.
 template <typename R, bool kZero, bool kFromR, int M>
-__device__ __forceinline__ void zebra_scan_column(const Split& s, int color, int perx, int pery,
-                                                  const R* __restrict__ bundle, const R* __restrict__ cpl,
-                                                  const R* __restrict__ rhs, R* __restrict__ x, int64_t t, int lane)
+__global__ __launch_bounds__(256) void zebra_scan_kernel(Split s, int color, int perx, int pery,
+                                                         const R* __restrict__ bundle, const R* __restrict__ cpl,
+                                                         const R* __restrict__ rhs, R* __restrict__ x)
 {
     using dev::gld;
     using dev::gst;
     constexpr bool kNb = !kZero && !kFromR;
+    const int lane = threadIdx.x & 63;
     const int64_t rowc = int64_t(s.H) * s.ny;
+    const int64_t t = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
     if (t >= rowc) { return; }
     const int ih = int(t % s.H), j = int(t / s.H);
     const int i = 2 * ih + ((j + color) & 1);
@@ -646,48 +648,6 @@ __device__ __forceinline__ void zebra_scan_column(const Split& s, int color, int
         if (k < nz) {
             xv = ev[u] + g[u] * xv;
             gst(x + q0 + k * rowc, xv);
-        }
-    }
-}
-
-template <typename R, bool kZero, bool kFromR, int M>
-__global__ __launch_bounds__(256) void zebra_scan_kernel(Split s, int color, int perx, int pery,
-                                                         const R* __restrict__ bundle, const R* __restrict__ cpl,
-                                                         const R* __restrict__ rhs, R* __restrict__ x)
-{
-    const int64_t t = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
-    zebra_scan_column<R, kZero, kFromR, M>(s, color, perx, pery, bundle, cpl, rhs, x, t, threadIdx.x & 63);
-}
-
-// `sweeps` whole zebra sweeps (colour 0, then colour 1) of a small level in
-// one launch: one workgroup, its waves taking the columns of a colour in
-// turn, a barrier between half sweeps (the launches of the per-half-sweep
-// form, in the same order, with the same bits). first_zero: x is zero on
-// entry to the first half sweep.
-constexpr int kSmallThreads = 1024;
-constexpr int64_t kSmallCols = 32;   // columns per colour at most (more: the per-half-sweep launches win)
-
-template <typename R, int M>
-__global__ __launch_bounds__(kSmallThreads) void zebra_small_kernel(Split s, int perx, int pery,
-                                                                    const R* __restrict__ bundle,
-                                                                    const R* __restrict__ cpl,
-                                                                    const R* __restrict__ rhs, R* __restrict__ x,
-                                                                    int sweeps, int first_zero)
-{
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    constexpr int kWaves = kSmallThreads / 64;
-    const int64_t rowc = int64_t(s.H) * s.ny;
-    for (int sw = 0; sw < sweeps; ++sw) {
-        for (int color = 0; color < 2; ++color) {
-            const bool zero = first_zero && sw == 0 && color == 0;
-            for (int64_t t = wave; t < rowc; t += kWaves) {
-                if (zero) {
-                    zebra_scan_column<R, true, false, M>(s, color, perx, pery, bundle, cpl, rhs, x, t, lane);
-                } else {
-                    zebra_scan_column<R, false, false, M>(s, color, perx, pery, bundle, cpl, rhs, x, t, lane);
-                }
-            }
-            __syncthreads();
         }
     }
 }
@@ -1395,35 +1355,6 @@ void launch_zebra(const Level<R>& L, const int* per, int color, const R* rhs, hi
     GUST_HIP(hipGetLastError());
 }
 
-// `sweeps` zebra sweeps of a level small enough for one workgroup, in one
-// launch; false when the level does not qualify (the caller then sweeps
-// half sweep by half sweep).
-template <typename R>
-bool zebra_small(Level<R>& L, const int* per, int sweeps, bool first_zero, hipStream_t st)
-{
-    if (sweeps < 1 || seam_joins_colour(L, per) || L.nz > 256) { return false; }
-    const Split s = L.split();
-    const int64_t rowc = int64_t(s.H) * s.ny;
-    if (rowc > kSmallCols || rowc >= scan_cols()) { return false; }
-    const int M = (L.nz + 63) / 64;
-    const int fz = first_zero ? 1 : 0;
-    if (M == 1) {
-        zebra_small_kernel<R, 1><<<1, kSmallThreads, 0, st>>>(s, per[0], per[1], bundle_of(L), cp_of(L),
-                                                               ptr<R>(L.b), ptr<R>(L.x), sweeps, fz);
-    } else if (M == 2) {
-        zebra_small_kernel<R, 2><<<1, kSmallThreads, 0, st>>>(s, per[0], per[1], bundle_of(L), cp_of(L),
-                                                               ptr<R>(L.b), ptr<R>(L.x), sweeps, fz);
-    } else if (M == 3) {
-        zebra_small_kernel<R, 3><<<1, kSmallThreads, 0, st>>>(s, per[0], per[1], bundle_of(L), cp_of(L),
-                                                               ptr<R>(L.b), ptr<R>(L.x), sweeps, fz);
-    } else {
-        zebra_small_kernel<R, 4><<<1, kSmallThreads, 0, st>>>(s, per[0], per[1], bundle_of(L), cp_of(L),
-                                                               ptr<R>(L.b), ptr<R>(L.x), sweeps, fz);
-    }
-    GUST_HIP(hipGetLastError());
-    return true;
-}
-
 // One zebra sweep (colour 0, then colour 1); first: x is zero on entry.
 template <typename R>
 void zebra(Level<R>& L, const int* per, bool first, hipStream_t st)
@@ -1491,18 +1422,14 @@ void vcycle(PoissonImpl& P, std::vector<std::unique_ptr<Level<R>>>& mg, size_t l
 {
     Level<R>& L = *mg[l];
     if (l + 1 == mg.size()) {
-        if (!zebra_small<R>(L, P.per, P.opt.coarse_sweeps, true, st)) {
-            for (int s = 0; s < P.opt.coarse_sweeps; ++s) { zebra<R>(L, P.per, s == 0, st); }
-        }
+        for (int s = 0; s < P.opt.coarse_sweeps; ++s) { zebra<R>(L, P.per, s == 0, st); }
         if (bc.singular) { remove_mean<R>(P, L, st); }
         return;
     }
     if (P.opt.pre_sweeps == 0) {
         fill_kernel<R><<<blocks_for(L.slots()), kThreads, 0, st>>>(ptr<R>(L.x), R(0), L.slots());
     }
-    if (!zebra_small<R>(L, P.per, P.opt.pre_sweeps, true, st)) {
-        for (int s = 0; s < P.opt.pre_sweeps; ++s) { zebra<R>(L, P.per, s == 0, st); }
-    }
+    for (int s = 0; s < P.opt.pre_sweeps; ++s) { zebra<R>(L, P.per, s == 0, st); }
     Level<R>& C = *mg[l + 1];
     const Split sf = L.split(), sc = C.split();
     resid_restrict_kernel<R><<<blocks_for(C.slots()), kThreads, 0, st>>>(sf, P.per[0], P.per[1], bundle_of(L),
@@ -1516,9 +1443,7 @@ void vcycle(PoissonImpl& P, std::vector<std::unique_ptr<Level<R>>>& mg, size_t l
     mc.p = ptr<R>(C.x);
     prolong_kernel<R><<<blocks_for(L.slots()), kThreads, 0, st>>>(sf, C.rx, C.ry, mc, ptr<R>(L.x));
     GUST_HIP(hipGetLastError());
-    if (!zebra_small<R>(L, P.per, P.opt.post_sweeps, false, st)) {
-        for (int s = 0; s < P.opt.post_sweeps; ++s) { zebra<R>(L, P.per, false, st); }
-    }
+    for (int s = 0; s < P.opt.post_sweeps; ++s) { zebra<R>(L, P.per, false, st); }
 }
 
 // Premultiplied areas and the explicit stencil (with its column Thomas
