@@ -279,6 +279,24 @@ class TileMesh:
         return a[1:-1, 1:-1, 1:-1]
 
 
+@dataclass
+class TileFrame:
+    """What build_mesh takes, without the host arrays: DeviceMesh builds the
+    mesh of a TileFrame on the device (gust_build_mesh)."""
+    geom: Geometry
+    prob_lo: tuple
+    prob_hi: tuple
+    zlevels: np.ndarray
+    terrain: Terrain
+    z_nd = None   # no host arrays
+
+
+def tile_frame(geom: Geometry, prob_lo, prob_hi, zlevels: np.ndarray, terrain: Terrain) -> TileFrame:
+    if zlevels.size != geom.n[2] + 1:
+        raise ValueError("zlevels must have nz + 1 values")
+    return TileFrame(geom, tuple(prob_lo), tuple(prob_hi), np.asarray(zlevels, dtype=np.float64).copy(), terrain)
+
+
 def build_mesh(geom: Geometry, prob_lo, prob_hi, zlevels: np.ndarray, terrain: Terrain) -> TileMesh:
     """BTF mesh of one tile (Source/Mesh.cpp:107-140 with terrain.smoothing = 0)."""
     nx, ny, nz = geom.n

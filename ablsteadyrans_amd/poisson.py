@@ -87,14 +87,19 @@ class FaceCoeffs:
 
 class DeviceMesh:
     """A tile's mesh on the device: geometry, node heights (one ghost node)
-    and the metrics detJ, ax, ay, az (one ghost), from a host TileMesh."""
+    and the metrics detJ, ax, ay, az and z_cc (one ghost). From a host
+    TileMesh (uploaded), or from a TileFrame (built on the device by
+    gust_build_mesh, bitwise the host builder's)."""
 
-    def __init__(self, tile, device="cuda", dtype=None):
+    def __init__(self, tile, device="cuda", dtype=None, stream=None):
         import torch
         dtype = dtype or torch.float64
         self.tile = tile
         self.geom: Geometry = tile.geom
         b = (-1, -1, -1)
+        if getattr(tile, "z_nd", None) is None:
+            self._build(tile, device, stream)
+            return
 
         def dev(a):
             return Fab(torch.from_numpy(np.ascontiguousarray(a)[None]).to(device=device, dtype=dtype), b)
@@ -104,6 +109,32 @@ class DeviceMesh:
         self.ax = dev(tile.ax)
         self.ay = dev(tile.ay)
         self.az = dev(tile.az)
+        self.z_cc = dev(tile.z_cc)
+
+    def _build(self, frame, device, stream):
+        import torch
+        from . import mesh as M
+        nx, ny, nz = self.geom.n
+        b = (-1, -1, -1)
+
+        def zeros(shape):
+            return Fab(torch.zeros((1,) + shape, dtype=torch.float64, device=device), b)
+
+        self.z_nd = zeros((nz + 3, ny + 3, nx + 3))
+        self.detJ = zeros((nz + 2, ny + 2, nx + 2))
+        self.ax = zeros((nz + 2, ny + 2, nx + 3))
+        self.ay = zeros((nz + 2, ny + 3, nx + 2))
+        self.az = zeros((nz + 3, ny + 2, nx + 2))
+        self.z_cc = zeros((nz + 2, ny + 2, nx + 2))
+        ground = np.ascontiguousarray(M.ground_heights(frame.terrain, self.geom, frame.prob_lo, frame.prob_hi),
+                                      dtype=np.float64)
+        zl = np.ascontiguousarray(frame.zlevels, dtype=np.float64)
+        g = self.geom.c()
+        views = [self.z_nd.array4(), self.detJ.array4(), self.ax.array4(), self.ay.array4(), self.az.array4(),
+                 self.z_cc.array4()]
+        dp = C.POINTER(C.c_double)
+        _lib.check(_lib.lib.gust_build_mesh(C.byref(g), ground.ctypes.data_as(dp), zl.ctypes.data_as(dp),
+                                            *[C.byref(v) for v in views], _stream(stream)))
 
 
 def _bc(bc):

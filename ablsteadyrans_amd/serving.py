@@ -132,7 +132,7 @@ def tile_deck(spec: TileSpec, kind: str, max_iterations: int = 4000, sweeps: int
     lo, hi = (0.0, 0.0, 0.0), (lx, ly, top)
     geom = M.geometry((spec.nx, spec.ny, spec.nz), lo, hi, (1, 1))
     terrain = M.Terrain(kind="hills", hills=list(spec.hills)) if spec.hills else M.Terrain(kind="flat")
-    tile = M.build_mesh(geom, lo, hi, zl, terrain)
+    tile = M.tile_frame(geom, lo, hi, zl, terrain)   # the mesh is built on the device
     a = math.radians(spec.direction)
     sounding = dict(theta_ref_z=(0.0, 700.0, 1000.0), theta_ref=(300.0, 300.0, 309.0))
     prof = GI.InflowProfile(kind=GI.LOG_LAW, ustar=0.35, z0=0.1, direction=spec.direction, **sounding)

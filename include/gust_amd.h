@@ -1068,6 +1068,25 @@ int gust_initialize_state(const gust_geometry* geom, const gust_mesh* mesh, cons
                           const gust_init_options* opt, const int32_t* faces, gust_poisson* solver, gust_state* q,
                           gust_array4* lambda, gust_init_result* result, void* stream);
 
+/* ---- The terrain-following mesh of a tile (Source/Mesh.H) --------------- */
+
+/* Build a tile's BTF mesh on the device from its ground node heights.
+ * Replaces: Mesh::build_from_ground with terrain.smoothing = 0
+ *   (Source/Mesh.cpp:107-140, the ghost columns 37-98) and the metrics of
+ *   extern_erf/ERF_TerrainMetrics.cpp:330-356 (BTF) and 679-807 (make_J,
+ *   make_areas, make_zcc).
+ * ground: host, (nx + 3) * (ny + 3) node heights, node (i, j) at
+ *   [(j + 1) * (nx + 3) + (i + 1)] for i in -1 .. nx + 1 (what the terrain
+ *   gives at each node, ghost nodes included; the periodic images and the
+ *   clamped edge columns are taken here). zlevels: host, nz + 1 increasing.
+ * Outputs (device, one ghost layer): z_nd on the nodes, detJ and zcc (may be
+ *   null) on the cells, ax, ay, az on the faces, bitwise the host builder's.
+ * Refuses terrain at or above the top level and inverted cells with the
+ *   reference's messages. */
+int gust_build_mesh(const gust_geometry* geom, const double* ground, const double* zlevels, gust_array4* z_nd,
+                    gust_array4* detJ, gust_array4* ax, gust_array4* ay, gust_array4* az, gust_array4* zcc,
+                    void* stream);
+
 #ifdef __cplusplus
 }
 #endif
