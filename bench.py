@@ -451,7 +451,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving and batched legs")
     ap.add_argument("--profile-only", action="store_true", help="sweeps only (for rocprofv3 runs)")
-    ap.add_argument("--profile-deck", choices=("column", "askervein", "abl", "hill3d"),
+    ap.add_argument("--profile-deck", choices=("column", "askervein", "abl", "hill3d", "hill200"),
                     help="only N driver iterations of one reference deck (for rocprofv3 runs)")
     ap.add_argument("--iterations", type=int, default=100, help="driver iterations of --profile-deck")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -475,6 +475,11 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8,
                           "launches": sweep_steps}))
+        return
+    if args.profile_deck == "hill200":
+        ms = _time_hill_iterations(0, args.iterations)
+        print(json.dumps({"profile_deck": "gaussian_hill_200x200x64", "iterations": args.iterations,
+                          "ms_per_iteration": round(ms, 4)}))
         return
     if args.profile_deck:
         print(json.dumps(_profile_deck(args.profile_deck, args.iterations)))
