@@ -426,6 +426,33 @@ __global__ void zero_ghosts_kernel(T* p, int64_t jstride, int64_t kstride, int n
     }
 }
 
+// The same for up to kGhostSets views in one launch (blockIdx.y: the view).
+constexpr int kGhostSets = 8;
+template <typename T>
+struct GhostSet {
+    T* p[kGhostSets];
+    int64_t js[kGhostSets], ks[kGhostSets];
+    int n[kGhostSets][3], vlo[kGhostSets][3], vhi[kGhostSets][3];
+};
+
+template <typename T>
+__global__ void zero_ghosts_multi_kernel(GhostSet<T> g)
+{
+    const int s = blockIdx.y;
+    const int nx = g.n[s][0], ny = g.n[s][1], nz = g.n[s][2];
+    const int64_t n = int64_t(nx) * ny * nz;
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        const int i = int(t % nx);
+        const int64_t r = t / nx;
+        const int j = int(r % ny), k = int(r / ny);
+        if (i >= g.vlo[s][0] && i <= g.vhi[s][0] && j >= g.vlo[s][1] && j <= g.vhi[s][1] && k >= g.vlo[s][2] &&
+            k <= g.vhi[s][2]) {
+            continue;
+        }
+        g.p[s][i + j * g.js[s] + k * g.ks[s]] = T(0);
+    }
+}
+
 __global__ void copy_u64_kernel(const unsigned long long* src, unsigned long long* dst, int n)
 {
     for (int t = threadIdx.x; t < n; t += blockDim.x) { dst[t] = src[t]; }
@@ -480,6 +507,33 @@ void zero_ghosts(const gust_array4& v, const IBox& V, hipStream_t st)
                                                   V.lo[0] - v.begin[0], V.hi[0] - v.begin[0], V.lo[1] - v.begin[1],
                                                   V.hi[1] - v.begin[1], V.lo[2] - v.begin[2], V.hi[2] - v.begin[2]);
     GUST_HIP(hipGetLastError());
+}
+
+// zero_ghosts of many views, kGhostSets per launch.
+template <typename T>
+void zero_ghosts_all(const std::vector<gust_line_system>& sys, const std::vector<IBox>& valid, hipStream_t st)
+{
+    for (size_t s0 = 0; s0 < sys.size(); s0 += kGhostSets) {
+        GhostSet<T> g{};
+        int64_t most = 1;
+        const int cnt = int(std::min<size_t>(kGhostSets, sys.size() - s0));
+        for (int q = 0; q < cnt; ++q) {
+            const gust_array4& v = sys[s0 + size_t(q)].delta;
+            const IBox& V = valid[s0 + size_t(q)];
+            g.p[q] = static_cast<T*>(v.p);
+            g.js[q] = v.jstride;
+            g.ks[q] = v.kstride;
+            for (int d = 0; d < 3; ++d) {
+                g.n[q][d] = v.end[d] - v.begin[d];
+                g.vlo[q][d] = V.lo[d] - v.begin[d];
+                g.vhi[q][d] = V.hi[d] - v.begin[d];
+            }
+            most = std::max<int64_t>(most, int64_t(g.n[q][0]) * g.n[q][1] * g.n[q][2]);
+        }
+        const int blocks = int(std::min<int64_t>((most + 255) / 256, 8192));
+        zero_ghosts_multi_kernel<T><<<dim3(blocks, cnt), 256, 0, st>>>(g);
+        GUST_HIP(hipGetLastError());
+    }
 }
 
 template <typename T>
@@ -690,7 +744,7 @@ struct JacobiPlan final : JacobiPlanBase {
             GUST_HIP(hipGetLastError());
         }
         // delta = 0 outside the domain (check_line_system's setVal).
-        for (int s = 0; s < nsys; ++s) { zero_ghosts<T>(sys[size_t(s)].delta, valid[size_t(s)], st); }
+        zero_ghosts_all<T>(sys, valid, st);
         // The sweeps ping-pong between delta and the scratch iterate, arranged
         // so the last sweep writes delta; sweep 0 reads the zero iterate
         // implicitly.

@@ -1162,6 +1162,10 @@ struct PoissonImpl {
     std::vector<std::unique_ptr<Level<double>>> mg64;
     std::vector<std::unique_ptr<Level<float>>> mg32;
     DevBuf krylov_v, krylov_z, w, partial, dots, hcoef, scal;
+    // sum J over the level-0 cells (the mean removal's denominator), formed
+    // once: the mesh does not change under a solver
+    DevBuf sum_j;
+    bool have_sum_j = false;
     std::vector<double> hbuf;
     int per[2] = {0, 0};
 
@@ -1544,10 +1548,14 @@ void multi_dot_dev(PoissonImpl& P, const double* V, int64_t ldv, int nv, const d
 void remove_weighted_mean(PoissonImpl& P, double* v, double* nd, hipStream_t st)
 {
     const int64_t n = P.n();
+    if (!P.have_sum_j) {
+        P.sum_j.ensure(8);
+        fill_kernel<double><<<blocks_for(n), kThreads, 0, st>>>(ptr<double>(P.w), 1.0, n);
+        multi_dot_dev(P, ptr<double>(P.L0.J), 0, 1, ptr<double>(P.w), ptr<double>(P.sum_j), st);
+        P.have_sum_j = true;
+    }
     multi_dot_dev(P, ptr<double>(P.L0.J), 0, 1, v, nd, st);
-    fill_kernel<double><<<blocks_for(n), kThreads, 0, st>>>(ptr<double>(P.w), 1.0, n);
-    multi_dot_dev(P, ptr<double>(P.L0.J), 0, 1, ptr<double>(P.w), nd + 1, st);
-    add_scalar_kernel_r<double><<<blocks_for(n), kThreads, 0, st>>>(v, nd, nd + 1, n);
+    add_scalar_kernel_r<double><<<blocks_for(n), kThreads, 0, st>>>(v, nd, ptr<double>(P.sum_j), n);
     GUST_HIP(hipGetLastError());
 }
 
@@ -1953,7 +1961,7 @@ int gust_poisson_solve(gust_poisson* p, gust_array4* rhs, gust_array4* phi, cons
             // the range of A (Source/Poisson.cpp:559-567); rhs is modified.
             remove_weighted_mean(P, bvec, mean_d, st);
             unsigned long long* slot = dq->zero_slot([acc](unsigned long long v) { acc->removed_mean = bits_to_double(v); });
-            ratio_bits_kernel<<<1, 1, 0, st>>>(mean_d, mean_d + 1, slot);
+            ratio_bits_kernel<<<1, 1, 0, st>>>(mean_d, ptr<double>(P.sum_j), slot);
             phi_in_out(P, rhs, bvec, 1, st);
         }
         // The multigrid hierarchy is set up at the first preconditioner call:
