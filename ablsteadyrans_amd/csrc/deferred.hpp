@@ -40,6 +40,18 @@ public:
         cbs_.emplace_back(s, std::move(f));
         return dev_ + s;
     }
+    // n consecutive slots that start at 0, callback i for slot i (for a
+    // kernel that writes an array of values).
+    unsigned long long* zero_run(std::vector<Callback> fs)
+    {
+        const int n = int(fs.size());
+        init();
+        if (nz_ + n > kZero) { flush(); init(); }
+        const int s0 = nz_;
+        nz_ += n;
+        for (int i = 0; i < n; ++i) { cbs_.emplace_back(s0 + i, std::move(fs[size_t(i)])); }
+        return dev_ + s0;
+    }
     // A slot that starts at all ones (atomicMin).
     unsigned long long* ones_slot(Callback f)
     {

@@ -26,6 +26,7 @@
 //    equal the CPU build bit for bit.
 #include "common.hpp"
 #include "deferred.hpp"
+#include "simplec.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -1164,6 +1165,21 @@ void lusgs_run(int loc, const gust_array4* A, const gust_array4* ap, const gust_
         GUST_HIP(hipStreamSynchronize(st));
     }
 }
+
+}   // namespace
+
+// A batch of z-line Jacobi systems whose last-sweep changes go straight
+// into `changes` (nsys slots that hold zeros, as ordered bits of |change|).
+void zline_jacobi_into(int nsys, const gust_line_system* sys, int sweeps, unsigned long long* changes, hipStream_t st)
+{
+    JacobiPlan<double> plan;
+    plan.init(nsys, sys, false, st);
+    plan.chg = changes;
+    plan.chg_zeroed = true;
+    plan.run(sweeps, nullptr, st);
+}
+
+namespace {
 
 template <typename T>
 void zjacobi_one(int loc, const gust_array4* A, const gust_array4* ap, const gust_array4* rhs, int sweeps,

@@ -89,12 +89,6 @@ __global__ __launch_bounds__(kThreads) void reduce_jobs_kernel(RJobs s)
     }
 }
 
-// *out = the bits of *src (a device value into a deferred slot).
-__global__ void bits_to_slot_kernel(const double* src, unsigned long long* out)
-{
-    *out = static_cast<unsigned long long>(__double_as_longlong(*src));
-}
-
 // y += x and max |x| into *out (atomic max of ordered bits).
 __global__ __launch_bounds__(kThreads) void add_max_kernel(Box3 b, V y, CV x, unsigned long long* out)
 {
@@ -171,13 +165,15 @@ __global__ __launch_bounds__(kThreads) void fill_periodic_kernel(Box3 grown, V f
 }
 
 // a_P = keep/dtau + A_P, d = keep (a_P - sum a_nb) (Source/Predictor.cpp:186-201).
+// With rhs = -R on the same points (the predictor's right-hand side).
 __global__ __launch_bounds__(kThreads) void ap_d_kernel(Box3 b, CV A, int64_t an, CV dt, CV mask, int has_mask, V ap,
-                                                         V d)
+                                                         V d, CV R, V rhs)
 {
     GUST_GRID_LOOP(b)
     {
         int i, j, k;
         b.at(t, i, j, k);
+        rhs(i, j, k) = R(i, j, k) * -1.0;
         const double* a = &A(i, j, k);
         const double keep = has_mask ? 1.0 - mask(i, j, k) : 1.0;
         const double p = keep / dt(i, j, k) + a[0];
@@ -220,16 +216,6 @@ __global__ __launch_bounds__(kThreads) void w_cap_kernel(Box3 b, CV A, int64_t a
 }
 
 // rhs = -R (MultiFab::Copy then mult(-1)).
-__global__ __launch_bounds__(kThreads) void negate_kernel(Box3 b, CV R, V rhs)
-{
-    GUST_GRID_LOOP(b)
-    {
-        int i, j, k;
-        b.at(t, i, j, k);
-        rhs(i, j, k) = R(i, j, k) * -1.0;
-    }
-}
-
 // Per-level sums of a face field over the faces counted once: block
 // (chunk, level) adds its contiguous share of the level (i fastest, then j)
 // and reduces it with a fixed tree; the level's partials are then added in
@@ -334,13 +320,15 @@ __global__ __launch_bounds__(kThreads) void buoyancy_kernel(Box3 b, CV dt, CV th
 }
 
 // ap = 1/dtau + A_P (Source/Simplec.cpp:247-257).
-__global__ __launch_bounds__(kThreads) void scalar_ap_kernel(Box3 b, CV A, CV dt, V ap)
+// a_P = 1/dtau + A_P and rhs = -R (Source/Simplec.cpp:240-247).
+__global__ __launch_bounds__(kThreads) void scalar_ap_kernel(Box3 b, CV A, CV dt, V ap, CV R, V rhs)
 {
     GUST_GRID_LOOP(b)
     {
         int i, j, k;
         b.at(t, i, j, k);
         ap(i, j, k) = 1.0 / dt(i, j, k) + A(i, j, k);
+        rhs(i, j, k) = R(i, j, k) * -1.0;
     }
 }
 
@@ -559,20 +547,24 @@ void scalar_sweeps_theta_tke(const gust_geometry& g, gust_array4* theta, const g
     for (int q = 0; q < 2; ++q) {
         Scratch ap = scratch(st, slot[q][0], cells, 0), rhs = scratch(st, slot[q][1], cells, 0),
                 delta = scratch(st, slot[q][2], cells, 1);
-        negate_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<const double>(R[q]), dview<double>(&rhs.v));
         scalar_ap_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<const double>(A[q]),
-                                                                    dview<const double>(dtau), dview<double>(&ap.v));
+                                                                    dview<const double>(dtau), dview<double>(&ap.v),
+                                                                    dview<const double>(R[q]), dview<double>(&rhs.v));
         GUST_HIP(hipGetLastError());
         if (res[q]) { *res[q] = gust_scalar_sweep_result{}; }
         sys[q] = gust_line_system{GUST_LOC_CELL, g, *A[q], ap.v, rhs.v, delta.v};
     }
-    auto* chd = static_cast<double*>(Workspace::get(st, 66, 2 * sizeof(double)));
-    check_rc(gust_zline_jacobi_batched_async(2, sys, sweeps, chd, stream));
+    std::vector<Deferred::Callback> fs;
+    for (int q = 0; q < 2; ++q) {
+        gust_scalar_sweep_result* r = res[q];
+        fs.emplace_back([r](unsigned long long v) {
+            if (r) { r->last_sweep = bits_to_double(v); }
+        });
+    }
+    zline_jacobi_into(2, sys, sweeps, dq->zero_run(std::move(fs)), st);
     for (int q = 0; q < 2; ++q) {
         gust_scalar_sweep_result* r = res[q];
         if (r) {
-            unsigned long long* s = dq->zero_slot([r](unsigned long long v) { r->last_sweep = bits_to_double(v); });
-            bits_to_slot_kernel<<<1, 1, 0, st>>>(chd + q, s);
             red.add_max_abs(cells, field[q], &sys[q].delta, [r](double x) { r->max_dq = x; });
         } else {
             box_add(st, cells, field[q], &sys[q].delta);
@@ -789,9 +781,11 @@ int gust_momentum_predictor(const gust_geometry* geom, gust_faces* vel, const gu
             if (m) { check_field(op, "mask", m, g, loc, 0); }
             const CV mk = dview<const double>(m ? m : A->c[c]);
             const int64_t an = A->c[c]->nstride;
+            Scratch rhs = scratch(st, batched ? rslot[c] : 44, fb, 0), delta = scratch(st, batched ? dslot[c] : 45, fb, 1);
             ap_d_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<const double>(A->c[c]), an,
                                                                    dview<const double>(dtau->c[c]), mk, m ? 1 : 0,
-                                                                   dview<double>(ap->c[c]), dview<double>(d->c[c]));
+                                                                   dview<double>(ap->c[c]), dview<double>(d->c[c]),
+                                                                   dview<const double>(R->c[c]), dview<double>(&rhs.v));
             if (c == 2 && cap > 0) {
                 check_nodal_heights(op, z_nd, g);
                 w_cap_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(
@@ -799,9 +793,6 @@ int gust_momentum_predictor(const gust_geometry* geom, gust_faces* vel, const gu
                     mk, m ? 1 : 0, dview<double>(ap->c[c]), dview<double>(d->c[c]), 1.0 / cap, margin, dxi0, dxi1,
                     margin > 0 ? 1 : 0);
             }
-            Scratch rhs = scratch(st, batched ? rslot[c] : 44, fb, 0), delta = scratch(st, batched ? dslot[c] : 45, fb, 1);
-            negate_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<const double>(R->c[c]),
-                                                                     dview<double>(&rhs.v));
             GUST_HIP(hipGetLastError());
             if (c < 2 && rotation) {
                 // the other residual with its periodic images (FillBoundary)
@@ -868,14 +859,15 @@ int gust_momentum_predictor(const gust_geometry* geom, gust_faces* vel, const gu
             }
         }
         if (batched) {
-            auto* chd = static_cast<double*>(Workspace::get(st, 59, 3 * sizeof(double)));
-            check_rc(gust_zline_jacobi_batched_async(3, sys, opt->sweeps, chd, stream));
+            std::vector<Deferred::Callback> fs;
             for (int c = 0; c < 3; ++c) {
-                if (last[c]) {
-                    double* out = last[c];
-                    unsigned long long* slot = dq->zero_slot([out](unsigned long long v) { *out = bits_to_double(v); });
-                    bits_to_slot_kernel<<<1, 1, 0, st>>>(chd + c, slot);
-                }
+                double* out = last[c];
+                fs.emplace_back([out](unsigned long long v) {
+                    if (out) { *out = bits_to_double(v); }
+                });
+            }
+            zline_jacobi_into(3, sys, opt->sweeps, dq->zero_run(std::move(fs)), st);
+            for (int c = 0; c < 3; ++c) {
                 const IBox fb = loc_box(g, 1 << c);
                 if (dmax[c]) {
                     double* out = dmax[c];
@@ -914,9 +906,9 @@ int gust_scalar_sweep(const gust_geometry* geom, int32_t eq, gust_array4* field,
         red.run();
         const Box3 b = box3(cells);
         Scratch ap = scratch(st, 51, cells, 0), rhs = scratch(st, 52, cells, 0), delta = scratch(st, 53, cells, 1);
-        negate_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<const double>(R_eq), dview<double>(&rhs.v));
         scalar_ap_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<const double>(A), dview<const double>(dtau),
-                                                                    dview<double>(&ap.v));
+                                                                    dview<double>(&ap.v), dview<const double>(R_eq),
+                                                                    dview<double>(&rhs.v));
         GUST_HIP(hipGetLastError());
         if (result) { *result = gust_scalar_sweep_result{}; }
         check_rc(gust_line_sweeps(sweep, GUST_LOC_CELL, A, &ap.v, &rhs.v, sweeps, geom, &delta.v,

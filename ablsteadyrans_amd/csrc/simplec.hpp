@@ -14,4 +14,10 @@ void scalar_sweeps_theta_tke(const gust_geometry& g, gust_array4* theta, const g
                              const gust_line_options& sweep, gust_scalar_sweep_result* res_theta,
                              gust_scalar_sweep_result* res_tke, hipStream_t st);
 
+// FP64 z-line Jacobi sweeps of a batch of systems (one launch per sweep),
+// each system's last-sweep change written as the ordered bits of |change|
+// into changes[s] (nsys slots holding zeros); defined in line_sweeps.hip.
+void zline_jacobi_into(int nsys, const gust_line_system* sys, int sweeps, unsigned long long* changes,
+                       hipStream_t st);
+
 }   // namespace gust_amd
