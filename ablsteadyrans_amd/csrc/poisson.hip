@@ -924,6 +924,55 @@ __global__ void view_copy_kernel(R* __restrict__ view, int64_t vj, int64_t vk, R
     }
 }
 
+// The three premultiplied areas in one launch (blockIdx.y: the direction):
+// out_d[t] = mesh_d[t] * beta_d(view point t), the product mul_kernel forms
+// after view_copy_kernel (the same bits, one pass). A null view: the mesh
+// areas alone.
+struct PremulSet {
+    const double* mesh[3];
+    const double* view[3];
+    int64_t vj[3], vk[3];
+    double* out[3];
+    int ex[3], ey[3], ez[3];
+};
+
+__global__ void premul_kernel(PremulSet p)
+{
+    const int d = blockIdx.y;
+    const int64_t n = int64_t(p.ex[d]) * p.ey[d] * p.ez[d];
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        if (!p.view[d]) {
+            p.out[d][t] = p.mesh[d][t];
+            continue;
+        }
+        int i, j, k;
+        cell_of(t, p.ex[d], p.ey[d], i, j, k);
+        p.out[d][t] = p.mesh[d][t] * p.view[d][i + j * p.vj[d] + int64_t(k) * p.vk[d]];
+    }
+}
+
+// The coefficient checks of up to four views in one launch (blockIdx.y).
+struct MinmaxSet {
+    const double* p[4];
+    int64_t sj[4], sk[4];
+    int nx[4], ny[4], nz[4];
+    unsigned long long* flag[4];
+};
+
+__global__ void minmax_multi_kernel(MinmaxSet s)
+{
+    const int q = blockIdx.y;
+    const int64_t n = int64_t(s.nx[q]) * s.ny[q] * s.nz[q];
+    bool bad = false;
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        int i, j, k;
+        cell_of(t, s.nx[q], s.ny[q], i, j, k);
+        const double v = s.p[q][i + j * s.sj[q] + int64_t(k) * s.sk[q]];
+        if (!(v > 0) || !(v < std::numeric_limits<double>::infinity())) { bad = true; }
+    }
+    if (bad) { atomicOr(s.flag[q], 1ull); }
+}
+
 // Fill the one-layer ghost shell of a view from its valid region with the
 // boundary conditions (Source/Poisson.cpp:84-125).
 template <typename R>
@@ -1001,21 +1050,6 @@ __global__ void face_div_kernel(int nx, int ny, int nz, double dxinv, double dyi
              + (A(az, azj, azk, i, j, k + 1) * A(fz, fzj, fzk, i, j, k + 1) - A(az, azj, azk, i, j, k) * A(fz, fzj, fzk, i, j, k)) * dzinv)
             / A(J, Jj, Jk, i, j, k);
     }
-}
-
-// min, max and NaN flag of a view's valid region (coefficient checks).
-__global__ void minmax_kernel(const double* __restrict__ p, int64_t sj, int64_t sk, int nx, int ny, int nz,
-                              unsigned long long* __restrict__ flags)
-{
-    const int64_t n = int64_t(nx) * ny * nz;
-    bool bad = false;
-    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
-        int i, j, k;
-        cell_of(t, nx, ny, i, j, k);
-        const double v = p[i + j * sj + int64_t(k) * sk];
-        if (!(v > 0) || !(v < std::numeric_limits<double>::infinity())) { bad = true; }
-    }
-    if (bad) { atomicOr(flags, 1ull); }
 }
 
 inline int blocks_for(int64_t n, int cap = 8192)
@@ -1620,6 +1654,9 @@ void prepare(PoissonImpl& P, const gust_face_coeffs* fc, const BcInfo& bc, Defer
     const gust_array4* b[4] = {nullptr, nullptr, nullptr, nullptr};
     if (fc) { b[0] = fc->x; b[1] = fc->y; b[2] = fc->z; b[3] = fc->z_cross; }
     const char* bname[4] = {"FaceCoeffs::x", "FaceCoeffs::y", "FaceCoeffs::z", "FaceCoeffs::z_cross"};
+    MinmaxSet mm{};
+    int nmm = 0;
+    int64_t most = 1;
     for (int d = 0; d < 4; ++d) {
         if (!b[d]) { continue; }
         const int dir = d < 3 ? d : 2;
@@ -1632,13 +1669,21 @@ void prepare(PoissonImpl& P, const gust_face_coeffs* fc, const BcInfo& bc, Defer
         // the flag is read with the batch's other values (at the latest when
         // the solve reads its initial residual)
         const char* name = bname[d];
-        unsigned long long* flag = dq.zero_slot([name](unsigned long long h) {
+        mm.flag[nmm] = dq.zero_slot([name](unsigned long long h) {
             if (h) { GUST_FAIL("gust: %s must be finite and > 0 everywhere", name); }
         });
-        const double* p = static_cast<const double*>(b[d]->p) + offset_of(*b[d], fb.lo[0], fb.lo[1], fb.lo[2]);
-        const int ex = fb.len(0), ey = fb.len(1), ez = fb.len(2);
-        minmax_kernel<<<blocks_for(int64_t(ex) * ey * ez), kThreads, 0, st>>>(p, b[d]->jstride, b[d]->kstride, ex, ey,
-                                                                             ez, flag);
+        mm.p[nmm] = static_cast<const double*>(b[d]->p) + offset_of(*b[d], fb.lo[0], fb.lo[1], fb.lo[2]);
+        mm.sj[nmm] = b[d]->jstride;
+        mm.sk[nmm] = b[d]->kstride;
+        mm.nx[nmm] = fb.len(0);
+        mm.ny[nmm] = fb.len(1);
+        mm.nz[nmm] = fb.len(2);
+        most = std::max<int64_t>(most, fb.npts());
+        ++nmm;
+    }
+    if (nmm > 0) {
+        minmax_multi_kernel<<<dim3(blocks_for(most), nmm), kThreads, 0, st>>>(mm);
+        GUST_HIP(hipGetLastError());
     }
     if (b[3] && bc.dir_ground) { GUST_FAIL("gust: FaceCoeffs::z_cross with a Dirichlet ground is not supported"); }
     if (fc && (!std::isfinite(fc->precond_ratio) || !(fc->precond_ratio > 0))) {
@@ -1647,20 +1692,23 @@ void prepare(PoissonImpl& P, const gust_face_coeffs* fc, const BcInfo& bc, Defer
     // a beta, as compact valid-face arrays.
     DevBuf* mesh[3] = {&P.ax_mesh, &P.ay_mesh, &P.az_mesh};
     DevBuf* out[3] = {&P.L0.ax, &P.L0.ay, &P.L0.az};
+    PremulSet pm{};
+    int64_t most_f = 1;
     for (int d = 0; d < 3; ++d) {
-        const int ex = P.nx + (d == 0), ey = P.ny + (d == 1), ez = P.nz + (d == 2);
-        const int64_t ne = int64_t(ex) * ey * ez;
-        if (!b[d]) {
-            GUST_HIP(hipMemcpyAsync(out[d]->p, mesh[d]->p, size_t(ne) * 8, hipMemcpyDeviceToDevice, st));
-            continue;
-        }
-        double* tmp = ptr<double>(P.w);
-        view_copy_kernel<double><<<blocks_for(ne), kThreads, 0, st>>>(
-            static_cast<double*>(b[d]->p) + offset_of(*b[d], P.geom.lo[0], P.geom.lo[1], P.geom.lo[2]),
-            b[d]->jstride, b[d]->kstride, tmp, ex, ey, ez, 0);
-        mul_kernel<double><<<blocks_for(ne), kThreads, 0, st>>>(ptr<double>(*mesh[d]), tmp, ptr<double>(*out[d]), ne);
-        GUST_HIP(hipGetLastError());
+        pm.ex[d] = P.nx + (d == 0);
+        pm.ey[d] = P.ny + (d == 1);
+        pm.ez[d] = P.nz + (d == 2);
+        pm.mesh[d] = ptr<double>(*mesh[d]);
+        pm.out[d] = ptr<double>(*out[d]);
+        pm.view[d] = b[d] ? static_cast<const double*>(b[d]->p) +
+                                offset_of(*b[d], P.geom.lo[0], P.geom.lo[1], P.geom.lo[2])
+                          : nullptr;
+        pm.vj[d] = b[d] ? b[d]->jstride : 0;
+        pm.vk[d] = b[d] ? b[d]->kstride : 0;
+        most_f = std::max<int64_t>(most_f, int64_t(pm.ex[d]) * pm.ey[d] * pm.ez[d]);
     }
+    premul_kernel<<<dim3(blocks_for(most_f), 3), kThreads, 0, st>>>(pm);
+    GUST_HIP(hipGetLastError());
     P.L0.has_cross = b[3] != nullptr;
     if (P.L0.has_cross) {
         const int64_t ne = int64_t(P.nx) * P.ny * (P.nz + 1);
