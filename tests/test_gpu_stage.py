@@ -249,3 +249,33 @@ def test_neutral_hill_3d_matches_the_reference_deck(cuda):
           f"(reference 148 s on four ranks), residuals {list(r.res)}, monitors {list(r.mon)[:r.nmon]}", flush=True)
     assert r.status == 1, (r.status, r.iterations, list(r.res))
     assert abs(r.iterations - 2092) <= 0.15 * 2092
+
+
+@pytest.mark.gpu
+def test_a_bad_check_in_an_iteration_reports_the_first_failure(cuda):
+    """The iteration reads its checks back together at its end (or when the
+    pressure solve fails): a NaN in dtau fails the first check on the path,
+    face_dtau's, with the reference's message, and the stage still runs a
+    good iteration afterwards."""
+    import torch
+    from ablsteadyrans_amd import GustError
+    from ablsteadyrans_amd import physics as GP
+    from ablsteadyrans_amd.fab import Fab
+    from ablsteadyrans_amd.poisson import DeviceMesh
+    tile, m, q, phys = _neutral_column(n=(6, 5, 12), top=300.0)
+    stage = GP.SimplecStage(DeviceMesh(tile), _inputs(phys), GP.SimplecOptions(sweeps=2, tke_floor=1e-6, height=300.0))
+    ds = _state(tile, q)
+    stage.init(ds)
+    g = m.geom
+    q1 = {n: a[:, 1:-1, 1:-1, 1:-1].copy() for n, a in q.items()}
+    dtau = orc.local_dtau(q1["u"], q1["v"], q1["w"], B1, m.z, B1, m.J, B1, g, 1.0)
+    bad = dtau.copy()
+    bad[2, 1, 3] = np.nan
+    with pytest.raises(GustError, match="gust: face_dtau: dtau_cc must be finite and > 0 everywhere"):
+        stage.iterate(ds, Fab(torch.from_numpy(bad[None].copy()).cuda(), tuple(g.lo)))
+    ds2 = _state(tile, q)
+    stage.init(ds2)
+    info = stage.iterate(ds2, Fab(torch.from_numpy(dtau[None].copy()).cuda(), tuple(g.lo)))
+    res, _ = stage.norms()
+    assert all(np.isfinite(r) for r in res)
+    assert np.isfinite(info.pressure.divergence_after)
