@@ -35,15 +35,48 @@ __device__ __forceinline__ double sc_min(double a, double b) { return (b < a) ? 
 // ---------------------------------------------------------------------------
 // Box kernels.
 
-__global__ __launch_bounds__(kThreads) void beta_keep_kernel(Box3 b, CV d, V beta, V keep)
+// beta = 1/d and keep of the three face directions in one launch
+// (blockIdx.y: the direction; beta_keep_kernel's arithmetic).
+struct FaceSet3 {
+    Box3 b[3];
+    CV d[3];
+    V beta[3], keep[3];
+};
+
+__global__ __launch_bounds__(kThreads) void beta_keep3_kernel(FaceSet3 s)
 {
+    const int q = blockIdx.y;
+    const Box3& b = s.b[q];
     GUST_GRID_LOOP(b)
     {
         int i, j, k;
         b.at(t, i, j, k);
-        const bool on = d(i, j, k) > 0;
-        beta(i, j, k) = on ? 1.0 / d(i, j, k) : 1.0;
-        keep(i, j, k) = on ? 1.0 : 0.0;
+        const double dv = s.d[q](i, j, k);
+        const bool on = dv > 0;
+        s.beta[q](i, j, k) = on ? 1.0 / dv : 1.0;
+        s.keep[q](i, j, k) = on ? 1.0 : 0.0;
+    }
+}
+
+// The correction of the three face fields in one launch: F *= keep, then
+// y += F (u, v and Omega; the arithmetic of the two separate passes).
+struct CorrSet3 {
+    Box3 b[3];
+    V F[3], y[3];
+    CV keep[3];
+};
+
+__global__ __launch_bounds__(kThreads) void correct3_kernel(CorrSet3 s)
+{
+    const int q = blockIdx.y;
+    const Box3& b = s.b[q];
+    GUST_GRID_LOOP(b)
+    {
+        int i, j, k;
+        b.at(t, i, j, k);
+        const double f = s.F[q](i, j, k) * s.keep[q](i, j, k);
+        s.F[q](i, j, k) = f;
+        s.y[q](i, j, k) = s.y[q](i, j, k) + f;
     }
 }
 
@@ -627,18 +660,22 @@ int gust_pressure_correction(gust_poisson* solver, const gust_geometry* geom, gu
         };
         auto vals = std::make_shared<Vals>();
         // face_coefficients (Simplec.cpp:87-112)
+        FaceSet3 bk{};
+        int64_t most = 1;
         for (int d = 0; d < 3; ++d) {
             beta[d] = scratch(st, 34 + d, fb[d], 0);
             keep[d] = scratch(st, 37 + d, fb[d], 0);
             F[d] = scratch(st, 40 + d, fb[d], 0);
             red.check_nonneg(fb[d], dd[d], std::string("gust: pressure_correction: the SIMPLEC denominator ") +
                                                dname[d] + " must be finite and >= 0 everywhere");
-            const Box3 b = box3(fb[d]);
-            beta_keep_kernel<<<blocks_for(b.npts()), kThreads, 0, st>>>(b, dview<const double>(dd[d]),
-                                                                        dview<double>(&beta[d].v),
-                                                                        dview<double>(&keep[d].v));
-            GUST_HIP(hipGetLastError());
+            bk.b[d] = box3(fb[d]);
+            bk.d[d] = dview<const double>(dd[d]);
+            bk.beta[d] = dview<double>(&beta[d].v);
+            bk.keep[d] = dview<double>(&keep[d].v);
+            most = std::max<int64_t>(most, bk.b[d].npts());
         }
+        beta_keep3_kernel<<<dim3(blocks_for(most), 3), kThreads, 0, st>>>(bk);
+        GUST_HIP(hipGetLastError());
         check_rc(gust_velocity_divergence(geom, u, v, w, z_nd, ax, ay, az, detJ, &rhs.v, stream));
         red.max_abs(cells, &rhs.v, [vals](double x) { vals->res.divergence_before = x; });
         if (source) {
@@ -683,11 +720,23 @@ int gust_pressure_correction(gust_poisson* solver, const gust_geometry* geom, gu
         red.max_abs(fb[1], &F[1].v, [vals](double x) { vals->res.max_dv = x; }, &keep[1].v);
         red.max_abs(fb[2], &F[2].v, [vals](double x) { vals->res.max_dw = x; }, &keep[2].v);
         red.run();
-        for (int d = 0; d < 3; ++d) { box_op(st, fb[d], &F[d].v, &keep[d].v, 0.0, 1); }
+        // Omega from the predicted velocities, then F *= keep and u, v, Omega
+        // += F in one launch (Omega does not read F)
         check_rc(gust_omega_from_w(geom, u, v, w, z_nd, &omega.v, 0, stream));
-        box_add(st, fb[0], u, &F[0].v);
-        box_add(st, fb[1], v, &F[1].v);
-        box_add(st, fb[2], &omega.v, &F[2].v);
+        {
+            CorrSet3 cs{};
+            gust_array4* y3[3] = {u, v, &omega.v};
+            int64_t most_c = 1;
+            for (int d = 0; d < 3; ++d) {
+                cs.b[d] = box3(fb[d]);
+                cs.F[d] = dview<double>(&F[d].v);
+                cs.y[d] = dview<double>(y3[d]);
+                cs.keep[d] = dview<const double>(&keep[d].v);
+                most_c = std::max<int64_t>(most_c, cs.b[d].npts());
+            }
+            correct3_kernel<<<dim3(blocks_for(most_c), 3), kThreads, 0, st>>>(cs);
+            GUST_HIP(hipGetLastError());
+        }
         override_sync(st, g, GUST_LOC_XFACE, u);
         override_sync(st, g, GUST_LOC_YFACE, v);
         override_sync(st, g, GUST_LOC_ZFACE, &omega.v);
