@@ -784,6 +784,27 @@ __global__ __launch_bounds__(kThreads) void accumulate_kernel(Box3 b, V y, CV x,
     }
 }
 
+// The same for up to three fields in one launch (blockIdx.y: the field).
+struct AccSet {
+    Box3 b[3];
+    V y[3];
+    CV x[3];
+    int sub;
+};
+
+__global__ __launch_bounds__(kThreads) void accumulate3_kernel(AccSet s)
+{
+    const int q = blockIdx.y;
+    const Box3& b = s.b[q];
+    GUST_GRID_LOOP(b)
+    {
+        int i, j, k;
+        b.at(t, i, j, k);
+        if (s.sub) { s.y[q](i, j, k) -= s.x[q](i, j, k); }
+        else { s.y[q](i, j, k) += s.x[q](i, j, k); }
+    }
+}
+
 // ---- implicit rows: wall model and z-face folds -------------------------------
 
 struct AV {
@@ -924,6 +945,22 @@ IBox slab_box(const gust_geometry& g)
     IBox b = grow(loc_box(g, GUST_LOC_CELL), 1, 1, 0);
     b.hi[2] = b.lo[2];
     return b;
+}
+
+// y_c -= x_c (sign < 0) or += for the three velocity locations, one launch.
+void cgrid_sub3(hipStream_t st, const gust_geometry& g, gust_array4* const* y, const gust_array4* x, double sign)
+{
+    AccSet a{};
+    int64_t most = 1;
+    for (int c = 0; c < 3; ++c) {
+        a.b[c] = box3(loc_box(g, 1 << c));
+        a.y[c] = dview<double>(y[c]);
+        a.x[c] = dview<const double>(&x[c]);
+        most = std::max<int64_t>(most, a.b[c].npts());
+    }
+    a.sub = sign < 0 ? 1 : 0;
+    accumulate3_kernel<<<dim3(blocks_for(most), 3), kThreads, 0, st>>>(a);
+    GUST_HIP(hipGetLastError());
 }
 
 void cgrid_sub(hipStream_t st, const Box3& b, gust_array4* y, const gust_array4* x, double sign)
@@ -1586,14 +1623,12 @@ int gust_residual(const gust_geometry* geom, const gust_state* q, const gust_mes
             GUST_HIP(hipGetLastError());
         }
         for (int c = 0; c < 3; ++c) {
-            const int loc = 1 << c;
-            check_rc(gust_flux_divergence(loc, geom, &G[c][0], &G[c][1], &G[c][2], mesh->detJ, &tmp[c], stream));
-            const Box3 b = box3(loc_box(g, loc));
-            cgrid_sub(st, b, Rw[c], &tmp[c], -1.0);
+            check_rc(gust_flux_divergence(1 << c, geom, &G[c][0], &G[c][1], &G[c][2], mesh->detJ, &tmp[c], stream));
         }
+        cgrid_sub3(st, g, Rw, tmp, -1.0);
         check_rc(gust_pressure_gradient(geom, q->f[3], mesh->z_nd, mesh->detJ, mesh->ax, mesh->ay, &tmp[0], &tmp[1],
                                         &tmp[2], stream));
-        for (int c = 0; c < 3; ++c) { cgrid_sub(st, box3(loc_box(g, 1 << c)), Rw[c], &tmp[c], 1.0); }
+        cgrid_sub3(st, g, Rw, tmp, 1.0);
         // forcing
         {
             gust_array4 tr = scratch_view(st, kSlotTref, grow(cells, 1, 1, 1));
