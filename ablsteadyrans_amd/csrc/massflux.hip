@@ -57,13 +57,22 @@ __global__ __launch_bounds__(kThreads) void w_kernel(Box3 b, CV u, CV v, CV om, 
     }
 }
 
-__global__ __launch_bounds__(kThreads) void area_flux_kernel(Box3 b, CV a, CV q, V f)
+// F_x = a_x u and F_y = a_y v in one launch (blockIdx.y: the direction).
+struct AreaFlux2 {
+    Box3 b[2];
+    CV a[2], q[2];
+    V f[2];
+};
+
+__global__ __launch_bounds__(kThreads) void area_flux2_kernel(AreaFlux2 s)
 {
+    const int d = blockIdx.y;
+    const Box3& b = s.b[d];
     GUST_GRID_LOOP(b)
     {
         int i, j, k;
         b.at(t, i, j, k);
-        f(i, j, k) = a(i, j, k) * q(i, j, k);
+        s.f[d](i, j, k) = s.a[d](i, j, k) * s.q[d](i, j, k);
     }
 }
 
@@ -181,10 +190,9 @@ int gust_transport_fluxes(const gust_geometry* geom, const gust_array4* u, const
         const Box3 bx = box3(grow(loc_box(*geom, GUST_LOC_XFACE), 1, 1, 1));
         const Box3 by = box3(grow(loc_box(*geom, GUST_LOC_YFACE), 1, 1, 1));
         const Box3 bz = box3(grow(loc_box(*geom, GUST_LOC_ZFACE), 1, 1, 1));
-        area_flux_kernel<<<blocks_for(bx.npts()), kThreads, 0, st>>>(bx, dview<const double>(ax),
-                                                                     dview<const double>(u), dview<double>(Fx));
-        area_flux_kernel<<<blocks_for(by.npts()), kThreads, 0, st>>>(by, dview<const double>(ay),
-                                                                     dview<const double>(v), dview<double>(Fy));
+        AreaFlux2 af{{bx, by}, {dview<const double>(ax), dview<const double>(ay)},
+                     {dview<const double>(u), dview<const double>(v)}, {dview<double>(Fx), dview<double>(Fy)}};
+        area_flux2_kernel<<<dim3(blocks_for(std::max(bx.npts(), by.npts())), 2), kThreads, 0, st>>>(af);
         omega_kernel<<<blocks_for(bz.npts()), kThreads, 0, st>>>(
             bz, dview<const double>(u), dview<const double>(v), dview<const double>(w), dview<const double>(z_nd),
             dview<double>(Fz), 1.0 / geom->dx[0], 1.0 / geom->dx[1]);
