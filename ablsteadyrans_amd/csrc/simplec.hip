@@ -122,6 +122,32 @@ __global__ __launch_bounds__(kThreads) void reduce_jobs_kernel(RJobs s)
     }
 }
 
+// The same for up to three fields in one launch (blockIdx.y: the field).
+struct AddMaxSet {
+    Box3 b[3];
+    V y[3];
+    CV x[3];
+    unsigned long long* out[3];
+};
+
+__global__ __launch_bounds__(kThreads) void add_max3_kernel(AddMaxSet s)
+{
+    const int q = blockIdx.y;
+    const Box3& b = s.b[q];
+    unsigned long long best = 0;
+    GUST_GRID_LOOP(b)
+    {
+        int i, j, k;
+        b.at(t, i, j, k);
+        const double v = s.x[q](i, j, k);
+        s.y[q](i, j, k) = s.y[q](i, j, k) + v;
+        const unsigned long long bits = dev::abs_bits(v);
+        best = bits > best ? bits : best;
+    }
+    best = dev::block_max_u64<kThreads>(best);
+    if (threadIdx.x == 0 && best) { atomicMax(s.out[q], best); }
+}
+
 // y += x and max |x| into *out (atomic max of ordered bits).
 __global__ __launch_bounds__(kThreads) void add_max_kernel(Box3 b, V y, CV x, unsigned long long* out)
 {
@@ -448,6 +474,26 @@ public:
         });
     }
     // y += x with max |x| reported (one pass).
+    // y_q += x_q with max |x_q| to f_q (f_q may be empty: no report) for
+    // n <= 3 fields in one launch.
+    void add_max_abs_n(int n, const IBox* box, gust_array4* const* y, const gust_array4* const* x,
+                       std::vector<std::function<void(double)>> f)
+    {
+        AddMaxSet a{};
+        int64_t most = 1;
+        for (int q = 0; q < n; ++q) {
+            a.b[q] = box3(box[q]);
+            a.y[q] = dview<double>(y[q]);
+            a.x[q] = dview<const double>(x[q]);
+            std::function<void(double)> fq = f[size_t(q)];
+            a.out[q] = dq_.zero_slot([fq](unsigned long long v) {
+                if (fq) { fq(bits_to_double(v)); }
+            });
+            most = std::max<int64_t>(most, a.b[q].npts());
+        }
+        add_max3_kernel<<<dim3(cgrid::reduce_blocks(most), n), kThreads, 0, st_>>>(a);
+        GUST_HIP(hipGetLastError());
+    }
     void add_max_abs(const IBox& box, const gust_array4* y, const gust_array4* x, std::function<void(double)> f)
     {
         const Box3 b = box3(box);
@@ -595,13 +641,15 @@ void scalar_sweeps_theta_tke(const gust_geometry& g, gust_array4* theta, const g
         });
     }
     zline_jacobi_into(2, sys, sweeps, dq->zero_run(std::move(fs)), st);
-    for (int q = 0; q < 2; ++q) {
-        gust_scalar_sweep_result* r = res[q];
-        if (r) {
-            red.add_max_abs(cells, field[q], &sys[q].delta, [r](double x) { r->max_dq = x; });
-        } else {
-            box_add(st, cells, field[q], &sys[q].delta);
+    {
+        const IBox boxes[2] = {cells, cells};
+        const gust_array4* xs[2] = {&sys[0].delta, &sys[1].delta};
+        std::vector<std::function<void(double)>> fs2(2);
+        for (int q = 0; q < 2; ++q) {
+            gust_scalar_sweep_result* r = res[q];
+            if (r) { fs2[size_t(q)] = [r](double x) { r->max_dq = x; }; }
         }
+        red.add_max_abs_n(2, boxes, field, xs, std::move(fs2));
     }
     unsigned long long* cnt = dq->zero_slot([res_tke](unsigned long long v) {
         if (res_tke) { res_tke->clipped = static_cast<int64_t>(v); }
@@ -916,16 +964,18 @@ int gust_momentum_predictor(const gust_geometry* geom, gust_faces* vel, const gu
                 });
             }
             zline_jacobi_into(3, sys, opt->sweeps, dq->zero_run(std::move(fs)), st);
+            IBox boxes[3];
+            const gust_array4* xs[3];
+            std::vector<std::function<void(double)>> fs3(3);
             for (int c = 0; c < 3; ++c) {
-                const IBox fb = loc_box(g, 1 << c);
+                boxes[c] = loc_box(g, 1 << c);
+                xs[c] = &sys[c].delta;
                 if (dmax[c]) {
                     double* out = dmax[c];
-                    red.add_max_abs(fb, vel->c[c], &sys[c].delta, [out](double x) { *out = x; });
-                } else {
-                    box_add(st, fb, vel->c[c], &sys[c].delta);
+                    fs3[size_t(c)] = [out](double x) { *out = x; };
                 }
             }
-            GUST_HIP(hipGetLastError());
+            red.add_max_abs_n(3, boxes, vel->c, xs, std::move(fs3));
         }
         dq.finish();
     });
