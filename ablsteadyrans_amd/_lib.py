@@ -305,7 +305,7 @@ def _load() -> C.CDLL:
         pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
-            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            f"{LIB_PATH} is missing: build it with `python ablsteadyrans_amd/build.py` "
             "(there is no non-HIP fallback)")
     return C.CDLL(LIB_PATH)
 
