@@ -165,6 +165,20 @@ __global__ __launch_bounds__(kThreads) void add_max_kernel(Box3 b, V y, CV x, un
     if (threadIdx.x == 0 && best) { atomicMax(out, best); }
 }
 
+// y = 0 on b, then y = 0 + x inside [lo, hi] (a field and its ghosts
+// cleared, the valid points set to x as an add onto zero would).
+__global__ __launch_bounds__(kThreads) void zero_add_kernel(Box3 b, V y, CV x, int lo0, int hi0, int lo1, int hi1,
+                                                             int lo2, int hi2)
+{
+    GUST_GRID_LOOP(b)
+    {
+        int i, j, k;
+        b.at(t, i, j, k);
+        const bool in = i >= lo0 && i <= hi0 && j >= lo1 && j <= hi1 && k >= lo2 && k <= hi2;
+        y(i, j, k) = in ? 0.0 + x(i, j, k) : 0.0;
+    }
+}
+
 // y op= x (op 0: y += a x, 1: y *= x, 2: y -= x, 3: y = 0).
 __global__ __launch_bounds__(kThreads) void axpy_kernel(Box3 b, V y, CV x, double a, int op)
 {
@@ -896,8 +910,15 @@ int gust_momentum_predictor(const gust_geometry* geom, gust_faces* vel, const gu
                 const int co = 1 - c, lo_other = 1 << co;
                 const IBox ob = loc_box(g, lo_other);
                 Scratch og = scratch(st, 46, ob, 1);
-                box_op(st, grow(ob, 1, 1, 1), &og.v, nullptr, 0.0, 3);
-                box_add(st, ob, &og.v, R->c[co]);
+                {
+                    // og = 0 on the grown box, then 0 + R on the valid box
+                    // (one pass; the two passes' bits)
+                    const Box3 gb = box3(grow(ob, 1, 1, 1));
+                    zero_add_kernel<<<blocks_for(gb.npts()), kThreads, 0, st>>>(
+                        gb, dview<double>(&og.v), dview<const double>(R->c[co]), ob.lo[0], ob.hi[0], ob.lo[1],
+                        ob.hi[1], ob.lo[2], ob.hi[2]);
+                    GUST_HIP(hipGetLastError());
+                }
                 fill_boundary(st, g, lo_other, &og.v);
                 // per-level mean over every face once
                 const bool per_o = g.periodic[co] != 0;
