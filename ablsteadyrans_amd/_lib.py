@@ -316,6 +316,8 @@ _P = C.POINTER
 _sig = {
     "gust_last_error": (C.c_char_p, []),
     "gust_version": (C.c_char_p, []),
+    "gust_stream_bytes_held": (C.c_int, [C.c_void_p, _P(C.c_uint64)]),
+    "gust_release_stream": (C.c_int, [C.c_void_p]),
     "gust_zline_jacobi": (C.c_int, [C.c_int, _P(Array4), _P(Array4), _P(Array4), C.c_int, _P(GeometryC),
                                     _P(Array4), _P(C.c_double), C.c_void_p]),
     "gust_zline_jacobi_f32": (C.c_int, [C.c_int, _P(Array4), _P(Array4), _P(Array4), C.c_int, _P(GeometryC),
@@ -460,3 +462,15 @@ def check(status: int) -> None:
 
 def exported_symbols() -> list[str]:
     return list(_sig)
+
+
+def stream_bytes_held(stream) -> int:
+    """Device bytes the library holds for a torch stream (its scratch and caches)."""
+    out = C.c_uint64(0)
+    check(lib.gust_stream_bytes_held(C.c_void_p(stream.cuda_stream), C.byref(out)))
+    return int(out.value)
+
+
+def release_stream(stream) -> None:
+    """Frees what the library holds for a torch stream (synchronises it first)."""
+    check(lib.gust_release_stream(C.c_void_p(stream.cuda_stream)))

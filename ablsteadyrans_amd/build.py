@@ -41,26 +41,29 @@ def _run(cmd: list[str]) -> None:
 
 
 def build_hip(verbose: bool = False) -> str:
-    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
-    objs = []
+    """Compile every source in parallel (one hipcc per file; the box and this
+    container both have several cores), then link."""
+    from concurrent.futures import ThreadPoolExecutor
+
     os.makedirs(os.path.join(PKG, "_build"), exist_ok=True)
-    for s in srcs:
+    cmds, objs = [], []
+    for s in sorted(glob.glob(os.path.join(CSRC, "*.hip"))):
         o = os.path.join(PKG, "_build", os.path.basename(s) + ".o")
-        cmd = [HIPCC] + [f for f in HIP_FLAGS if f != "-shared"] + [
-            "-c", "-I", os.path.join(ROOT, "include"), "-I", CSRC, s, "-o", o]
-        if verbose:
-            print(" ".join(cmd))
-        _run(cmd)
+        cmds.append([HIPCC] + [f for f in HIP_FLAGS if f != "-shared"] + [
+            "-c", "-I", os.path.join(ROOT, "include"), "-I", CSRC, s, "-o", o])
         objs.append(o)
     # Host-only C++ (the steady criterion): the host compiler, same rounding.
     for s in sorted(glob.glob(os.path.join(CSRC, "*.cpp"))):
         o = os.path.join(PKG, "_build", os.path.basename(s) + ".o")
-        cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-c",
-               "-I", os.path.join(ROOT, "include"), "-I", CSRC, s, "-o", o]
-        if verbose:
-            print(" ".join(cmd))
-        _run(cmd)
+        cmds.append(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-c",
+                     "-I", os.path.join(ROOT, "include"), "-I", CSRC, s, "-o", o])
         objs.append(o)
+    if verbose:
+        for c in cmds:
+            print(" ".join(c))
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", *objs, "-o", tmp])
     os.replace(tmp, LIB)

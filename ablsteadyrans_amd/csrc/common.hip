@@ -2,6 +2,7 @@
 #include "common.hpp"
 
 #include <map>
+#include <vector>
 #include <tuple>
 
 namespace gust_amd {
@@ -108,7 +109,9 @@ void* Workspace::get_host_pinned(int slot, size_t bytes)
     Buf& b = t_pinned.bufs[slot];
     if (b.cap < bytes) {
         if (b.p) {
-            GUST_HIP(hipDeviceSynchronize());
+            // No copy into this buffer can be pending: every use of it is
+            // followed by a synchronisation of its own stream before the
+            // thread goes on, so no device-wide wait is needed here.
             GUST_HIP(hipHostFree(b.p));
             b.p = nullptr;
             b.cap = 0;
@@ -118,6 +121,65 @@ void* Workspace::get_host_pinned(int slot, size_t bytes)
         b.cap = want;
     }
     return b.p;
+}
+
+namespace {
+std::mutex g_hooks_mutex;
+std::vector<StreamCacheHook>& hooks()
+{
+    static std::vector<StreamCacheHook> h;
+    return h;
+}
+}   // namespace
+
+void register_stream_cache(StreamCacheHook h)
+{
+    std::lock_guard<std::mutex> lk(g_hooks_mutex);
+    hooks().push_back(h);
+}
+
+size_t Workspace::bytes_held(hipStream_t s)
+{
+    int dev = 0;
+    GUST_HIP(hipGetDevice(&dev));
+    size_t total = 0;
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mutex);
+        for (const auto& kv : g_ws) {
+            if (std::get<0>(kv.first) == dev && std::get<1>(kv.first) == s) { total += kv.second.cap; }
+        }
+    }
+    std::vector<StreamCacheHook> hs;
+    {
+        std::lock_guard<std::mutex> lk(g_hooks_mutex);
+        hs = hooks();
+    }
+    for (const auto& h : hs) { total += h.bytes(dev, s); }
+    return total;
+}
+
+void Workspace::release(hipStream_t s)
+{
+    int dev = 0;
+    GUST_HIP(hipGetDevice(&dev));
+    GUST_HIP(hipStreamSynchronize(s));
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mutex);
+        for (auto it = g_ws.begin(); it != g_ws.end();) {
+            if (std::get<0>(it->first) == dev && std::get<1>(it->first) == s) {
+                if (it->second.p) { GUST_HIP(hipFree(it->second.p)); }
+                it = g_ws.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
+    std::vector<StreamCacheHook> hs;
+    {
+        std::lock_guard<std::mutex> lk(g_hooks_mutex);
+        hs = hooks();
+    }
+    for (const auto& h : hs) { h.drop(dev, s); }
 }
 
 const DeviceInfo& device_info()
@@ -144,5 +206,18 @@ extern "C" {
 const char* gust_last_error(void) { return gust_amd::t_err.c_str(); }
 
 const char* gust_version(void) { return "gust_amd 0.1 (gfx950)"; }
+
+int gust_stream_bytes_held(void* stream, uint64_t* bytes)
+{
+    return gust_amd::guarded([&] {
+        if (!bytes) { GUST_FAIL("gust: stream_bytes_held: bytes is null"); }
+        *bytes = gust_amd::Workspace::bytes_held((hipStream_t)stream);
+    });
+}
+
+int gust_release_stream(void* stream)
+{
+    return gust_amd::guarded([&] { gust_amd::Workspace::release((hipStream_t)stream); });
+}
 
 }

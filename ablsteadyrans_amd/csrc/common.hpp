@@ -115,7 +115,21 @@ public:
     static void* get(hipStream_t s, int slot, size_t bytes);
     // Pinned host memory, one buffer per (host thread, slot).
     static void* get_host_pinned(int slot, size_t bytes);
+    // Device bytes held for one stream of the current device (workspace
+    // buffers plus what the per-stream caches report).
+    static size_t bytes_held(hipStream_t s);
+    // Synchronises the stream and frees everything held for it on the
+    // current device; the next call on that stream allocates afresh.
+    static void release(hipStream_t s);
 };
+
+// Per-stream device caches (tables, profiles) register a hook that reports
+// and drops their entries of one (device, stream); release() calls them.
+struct StreamCacheHook {
+    size_t (*bytes)(int device, hipStream_t s);
+    void (*drop)(int device, hipStream_t s);
+};
+void register_stream_cache(StreamCacheHook h);
 
 // Device properties used for launch shapes.
 struct DeviceInfo {

@@ -22,7 +22,7 @@ namespace gust_amd {
 
 class Deferred {
 public:
-    static constexpr int kZero = 128, kOnes = 128;
+    static constexpr int kZero = 512, kOnes = 512;
     using Callback = std::function<void(unsigned long long)>;
 
     explicit Deferred(hipStream_t st) : st_(st) {}
@@ -35,7 +35,7 @@ public:
     unsigned long long* zero_slot(Callback f)
     {
         init();
-        if (nz_ >= kZero) { flush(); init(); }
+        if (nz_ >= kZero) { overflow(); }
         const int s = nz_++;
         cbs_.emplace_back(s, std::move(f));
         return dev_ + s;
@@ -46,7 +46,7 @@ public:
     {
         const int n = int(fs.size());
         init();
-        if (nz_ + n > kZero) { flush(); init(); }
+        if (nz_ + n > kZero) { overflow(); }
         const int s0 = nz_;
         nz_ += n;
         for (int i = 0; i < n; ++i) { cbs_.emplace_back(s0 + i, std::move(fs[size_t(i)])); }
@@ -56,7 +56,7 @@ public:
     unsigned long long* ones_slot(Callback f)
     {
         init();
-        if (no_ >= kOnes) { flush(); init(); }
+        if (no_ >= kOnes) { overflow(); }
         const int s = kZero + no_++;
         cbs_.emplace_back(s, std::move(f));
         return dev_ + s;
@@ -96,6 +96,14 @@ public:
 
 private:
     void init();
+    // A full batch is an error, not an early flush: queued device work (a
+    // Reducer's pending jobs) may still target slots of this batch, and a
+    // flush would hand their callbacks the initial values.
+    [[noreturn]] void overflow() const
+    {
+        GUST_FAIL("gust: internal: more than %d deferred device checks in one batch (%d at 0, %d at all ones)",
+                  kZero, nz_, no_);
+    }
     void flush()
     {
         if (!inited_) {

@@ -556,6 +556,68 @@ struct JacobiPlanBase {
     int64_t unknowns = 0;
 };
 
+// A device copy of a system table, uploaded the first time its bytes are
+// seen on the stream: an iteration that sweeps the same systems every time
+// copies nothing and waits for nothing. Entries are dropped by
+// Workspace::release of their stream.
+struct TableEntry {
+    std::vector<char> key;
+    void* dev;
+};
+std::mutex g_table_mu;
+std::map<std::pair<int, hipStream_t>, std::vector<TableEntry>> g_tables;
+
+size_t table_cache_bytes(int device, hipStream_t st)
+{
+    std::lock_guard<std::mutex> lock(g_table_mu);
+    auto it = g_tables.find({device, st});
+    size_t n = 0;
+    if (it != g_tables.end()) {
+        for (const TableEntry& e : it->second) { n += e.key.size(); }
+    }
+    return n;
+}
+
+void table_cache_drop(int device, hipStream_t st)
+{
+    std::lock_guard<std::mutex> lock(g_table_mu);
+    auto it = g_tables.find({device, st});
+    if (it == g_tables.end()) { return; }
+    for (TableEntry& e : it->second) { (void)hipFree(e.dev); }
+    g_tables.erase(it);
+}
+
+void* table_cache_get(hipStream_t st, const void* tab, size_t bytes)
+{
+    static const bool hooked = [] {
+        register_stream_cache(StreamCacheHook{&table_cache_bytes, &table_cache_drop});
+        return true;
+    }();
+    (void)hooked;
+    std::vector<char> key(bytes);
+    std::memcpy(key.data(), tab, bytes);
+    int device = 0;
+    GUST_HIP(hipGetDevice(&device));
+    std::lock_guard<std::mutex> lock(g_table_mu);
+    std::vector<TableEntry>& list = g_tables[{device, st}];
+    for (TableEntry& e : list) {
+        if (e.key == key) { return e.dev; }
+    }
+    if (list.size() >= 64) {
+        GUST_HIP(hipStreamSynchronize(st));
+        for (TableEntry& e : list) { (void)hipFree(e.dev); }
+        list.clear();
+    }
+    void* dev = nullptr;
+    GUST_HIP(hipMalloc(&dev, bytes));
+    // on the sweep's stream (a null-stream copy need not finish before
+    // that stream's kernels start), complete before the host returns
+    GUST_HIP(hipMemcpyAsync(dev, tab, bytes, hipMemcpyHostToDevice, st));
+    GUST_HIP(hipStreamSynchronize(st));
+    list.push_back(TableEntry{std::move(key), dev});
+    return dev;
+}
+
 template <typename T>
 struct JacobiPlan final : JacobiPlanBase {
     std::vector<gust_line_system> sys;
@@ -664,40 +726,10 @@ struct JacobiPlan final : JacobiPlanBase {
         }
     }
 
-    // A device copy of the table, uploaded the first time this table (its
-    // bytes) is seen on the stream: an iteration that sweeps the same
-    // systems every time copies nothing and waits for nothing.
     static JacobiSys<T>* cached_table(hipStream_t st, const std::vector<JacobiSys<T>>& tab)
     {
-        struct Entry {
-            std::vector<char> key;
-            JacobiSys<T>* dev;
-        };
-        static std::mutex mu;
-        static std::map<std::pair<int, hipStream_t>, std::vector<Entry>> cache;
-        const size_t bytes = tab.size() * sizeof(JacobiSys<T>);
-        std::vector<char> key(bytes);
-        std::memcpy(key.data(), tab.data(), bytes);
-        int device = 0;
-        GUST_HIP(hipGetDevice(&device));
-        std::lock_guard<std::mutex> lock(mu);
-        std::vector<Entry>& list = cache[{device, st}];
-        for (Entry& e : list) {
-            if (e.key == key) { return e.dev; }
-        }
-        if (list.size() >= 64) {
-            GUST_HIP(hipStreamSynchronize(st));
-            for (Entry& e : list) { (void)hipFree(e.dev); }
-            list.clear();
-        }
-        JacobiSys<T>* dev = nullptr;
-        GUST_HIP(hipMalloc(&dev, bytes));
-        // on the sweep's stream (a null-stream copy need not finish before
-        // that stream's kernels start), complete before the host returns
-        GUST_HIP(hipMemcpyAsync(dev, tab.data(), bytes, hipMemcpyHostToDevice, st));
-        GUST_HIP(hipStreamSynchronize(st));
-        list.push_back(Entry{std::move(key), dev});
-        return dev;
+        return static_cast<JacobiSys<T>*>(
+            table_cache_get(st, tab.data(), tab.size() * sizeof(JacobiSys<T>)));
     }
 
     template <bool F, bool L>

@@ -1,4 +1,6 @@
 // mesh.hip -- a tile's terrain-following mesh built on the device.
+// Derived in part from ERF (BSD 3-Clause, Copyright (c) 2022 The Regents of the University of California
+// et al.); the full notice is in THIRD_PARTY_NOTICES.md at the repository root.
 //
 // Reference behaviour (GUST):
 //   Source/Mesh.cpp:107-140        build_from_ground (terrain.smoothing = 0):

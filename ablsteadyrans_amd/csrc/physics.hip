@@ -1,4 +1,6 @@
 // physics.hip -- the residual R(q) of the steady RANS equations on a tile
+// Derived in part from ERF (BSD 3-Clause, Copyright (c) 2022 The Regents of the University of California
+// et al.); the full notice is in THIRD_PARTY_NOTICES.md at the repository root.
 // with periodic sides, and the implicit rows of its five transported
 // fields: heights above the ground, the k-L closure (Axell-Liungman, ERF's
 // relations), the neutral wall model, the ghost fills of the ground and the

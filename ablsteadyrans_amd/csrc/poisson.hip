@@ -1,4 +1,6 @@
 // poisson.hip -- the terrain-following pressure-correction solve for gfx950.
+// Derived in part from ERF (BSD 3-Clause, Copyright (c) 2022 The Regents of the University of California
+// et al.); the full notice is in THIRD_PARTY_NOTICES.md at the repository root.
 //
 // Reference behaviour (GUST):
 //   Source/Poisson.H:119-203, Source/Poisson.cpp:395-731   PoissonSolver

@@ -162,6 +162,38 @@ inline std::vector<std::string> problems(const gust_inflow_profile& p)
 // The tables are uploaded when the profile differs from the last one seen
 // on this (device, stream), with a synchronisation; otherwise the previous
 // upload is reused.
+// The device copies of the profile tables, one per (device, stream),
+// re-uploaded when a stream sees other tables.
+struct TableEntry {
+    std::vector<double> key;
+    double* dev = nullptr;
+    size_t cap = 0;
+};
+inline std::mutex& table_mutex()
+{
+    static std::mutex mu;
+    return mu;
+}
+inline std::map<std::pair<int, hipStream_t>, TableEntry>& table_map()
+{
+    static std::map<std::pair<int, hipStream_t>, TableEntry> m;
+    return m;
+}
+inline size_t table_bytes(int device, hipStream_t st)
+{
+    std::lock_guard<std::mutex> lock(table_mutex());
+    auto it = table_map().find({device, st});
+    return it == table_map().end() ? 0 : it->second.cap * sizeof(double);
+}
+inline void table_drop(int device, hipStream_t st)
+{
+    std::lock_guard<std::mutex> lock(table_mutex());
+    auto it = table_map().find({device, st});
+    if (it == table_map().end()) { return; }
+    if (it->second.dev) { (void)hipFree(it->second.dev); }
+    table_map().erase(it);
+}
+
 inline ProfileView device_view(hipStream_t st, const gust_inflow_profile& p)
 {
     const int n = p.kind == GUST_INFLOW_TABLE ? p.n : 0;
@@ -186,17 +218,17 @@ inline ProfileView device_view(hipStream_t st, const gust_inflow_profile& p)
         h[5 * n + m] = p.theta_ref_z[m];
         h[5 * n + rn + m] = p.theta_ref[m];
     }
-    struct Entry {
-        std::vector<double> key;
-        double* dev = nullptr;
-        size_t cap = 0;
-    };
-    static std::mutex mu;
-    static std::map<std::pair<int, hipStream_t>, Entry> cache;
+    static const bool hooked = [] {
+        register_stream_cache(StreamCacheHook{&table_bytes, &table_drop});
+        return true;
+    }();
+    (void)hooked;
+    std::mutex& mu = table_mutex();
+    auto& cache = table_map();
     int device = 0;
     GUST_HIP(hipGetDevice(&device));
     std::lock_guard<std::mutex> lock(mu);
-    Entry& e = cache[{device, st}];
+    TableEntry& e = cache[{device, st}];
     const size_t total = key.size() - head;
     if (e.key != key) {
         if (total > e.cap) {

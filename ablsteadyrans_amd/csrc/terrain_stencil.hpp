@@ -1,4 +1,6 @@
 // terrain_stencil.hpp -- the terrain-following Poisson stencil on the device.
+// Derived in part from ERF (BSD 3-Clause, Copyright (c) 2022 The Regents of the University of California
+// et al.); the full notice is in THIRD_PARTY_NOTICES.md at the repository root.
 //
 // The face gradients of a cell-centred field on the terrain-following mesh
 // (the physical d/dx, d/dy with the terrain cross terms, and the z-face

@@ -1,4 +1,7 @@
 """Host-side construction of a tile's terrain-following mesh (built once per tile).
+Derived in part from ERF (BSD 3-Clause, Copyright (c) 2022 The Regents of the
+University of California et al.); the full notice is in THIRD_PARTY_NOTICES.md
+at the repository root.
 
 Restates the reference's mesh pipeline operation by operation (float64 numpy,
 no contraction, so the node heights and metrics equal the CPU build's):

@@ -24,6 +24,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+from . import _lib
 from . import decks as D
 from . import initializer as GI
 from . import mesh as M
@@ -203,6 +204,13 @@ class Server:
         def worker():
             torch.cuda.set_device(device)
             stream = torch.cuda.Stream()
+            try:
+                serve(stream)
+            finally:
+                # the library's scratch and caches of this stream go with it
+                _lib.release_stream(stream)
+
+        def serve(stream):
             while True:
                 with lock:
                     while True:

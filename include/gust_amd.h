@@ -86,6 +86,19 @@ const char* gust_last_error(void);
 /* Library version string. */
 const char* gust_version(void);
 
+/* Device memory the library holds for one stream on the current device
+ * (scratch workspaces and per-stream table caches), in bytes. No counterpart
+ * in the reference (AMReX's arena owns its scratch); for callers that budget
+ * HBM per stream, such as a server admitting requests. */
+int gust_stream_bytes_held(void* stream, uint64_t* bytes);
+
+/* Synchronises the stream and frees every buffer and cache the library holds
+ * for it on the current device (the next call on that stream allocates
+ * afresh). No counterpart in the reference; for callers that retire streams
+ * (a server's worker exiting). Handles created on the stream (Poisson
+ * solvers, SIMPLEC stages) keep their own memory until destroyed. */
+int gust_release_stream(void* stream);
+
 /* z-line Jacobi sweeps of one system.
  * Replaces: Real zline_jacobi(Loc, const MultiFab& A, const MultiFab& ap,
  *   const MultiFab& rhs, int sweeps, const Mesh&, const Geometry&,

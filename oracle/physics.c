@@ -1,5 +1,7 @@
 /*
  * physics.c -- oracle of the residual's physics on a tile: heights above the ground, the k-L closure, the neutral surface
+ * Derived in part from ERF (BSD 3-Clause, Copyright (c) 2022 The Regents of the University of California
+ * et al.); the full notice is in THIRD_PARTY_NOTICES.md at the repository root.
  * layer, the ghost fills of the ground and the top, the momentum forcing,
  * and the wall and boundary rows of the residual and the implicit side.
  *

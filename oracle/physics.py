@@ -1,4 +1,7 @@
 """The residual R(q) of Source/Residual.cpp:159-256 and the implicit rows of
+Derived in part from ERF (BSD 3-Clause, Copyright (c) 2022 The Regents of the
+University of California et al.); the full notice is in THIRD_PARTY_NOTICES.md
+at the repository root.
 Source/Residual.cpp:272-304, composed from the oracle's C pieces.
 
 TEST INFRASTRUCTURE ONLY (see ``oracle.py``): used by tests/ and bench.py's

@@ -1,5 +1,7 @@
 /*
  * poisson.c -- oracle restatement of the reference's terrain Poisson operator.
+ * Derived in part from ERF (BSD 3-Clause, Copyright (c) 2022 The Regents of the University of California
+ * et al.); the full notice is in THIRD_PARTY_NOTICES.md at the repository root.
  *
  * TEST INFRASTRUCTURE ONLY (see gust_oracle.h). Compiled without
  * floating-point contraction, in the reference's operation order:
