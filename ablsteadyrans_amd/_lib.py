@@ -320,6 +320,8 @@ _sig = {
     "gust_release_stream": (C.c_int, [C.c_void_p]),
     "gust_zline_jacobi": (C.c_int, [C.c_int, _P(Array4), _P(Array4), _P(Array4), C.c_int, _P(GeometryC),
                                     _P(Array4), _P(C.c_double), C.c_void_p]),
+    "gust_zline_jacobi_noalloc": (C.c_int, [C.c_int, _P(Array4), _P(Array4), _P(Array4), C.c_int, _P(GeometryC),
+                                    _P(Array4), _P(C.c_double), C.c_void_p]),
     "gust_zline_jacobi_f32": (C.c_int, [C.c_int, _P(Array4), _P(Array4), _P(Array4), C.c_int, _P(GeometryC),
                                         _P(Array4), _P(C.c_double), C.c_void_p]),
     "gust_zline_jacobi_batched_async": (C.c_int, [C.c_int, _P(LineSystemC), C.c_int, C.c_void_p, C.c_void_p]),

@@ -641,9 +641,13 @@ struct JacobiPlan final : JacobiPlanBase {
         }
     }
 
-    void init(int n, const gust_line_system* in, bool own, hipStream_t st)
+    // own: buffers from hipMalloc (a plan handle); ext != nullptr: carved
+    // from the caller's ext_cap bytes (no allocation of any kind); neither:
+    // the stream workspace.
+    void init(int n, const gust_line_system* in, bool own, hipStream_t st, char* ext = nullptr,
+              size_t ext_cap = 0)
     {
-        const char* op = "zline_jacobi";
+        const char* op = ext ? "zline_jacobi_noalloc" : "zline_jacobi";
         if (n < 1 || !in) { GUST_FAIL("gust: %s: no systems", op); }
         nsys = n;
         owned = own;
@@ -705,6 +709,28 @@ struct JacobiPlan final : JacobiPlanBase {
         const bool use_lds = !force_global && lds <= device_info().lds_per_block;
         const size_t tab_bytes = tab.size() * sizeof(JacobiSys<T>);
         const size_t cbytes = use_lds ? 0 : size_t(tiles) * lds;
+        if (ext) {
+            // [changes | table | iterate], each 256-byte aligned
+            if (cbytes) {
+                GUST_FAIL("gust: %s: %d levels need c' outside the LDS, which this entry point does not hold",
+                          op, nzmax);
+            }
+            auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+            const size_t need = up(size_t(n) * 8) + up(tab_bytes) + up(scratch_elems * sizeof(T));
+            if (need > ext_cap) {
+                GUST_FAIL("gust: %s: the systems need %zu bytes of scratch, more than the fixed %zu bytes "
+                          "this entry point holds per device", op, need, ext_cap);
+            }
+            chg = reinterpret_cast<unsigned long long*>(ext);
+            dtab = reinterpret_cast<JacobiSys<T>*>(ext + up(size_t(n) * 8));
+            scratch = reinterpret_cast<T*>(ext + up(size_t(n) * 8) + up(tab_bytes));
+            for (auto& t : tab) { t.s = scratch + reinterpret_cast<size_t>(t.s); }
+            // The table's host copy lives in this plan; the wait ends the
+            // copy before the plan can go away.
+            GUST_HIP(hipMemcpyAsync(dtab, tab.data(), tab_bytes, hipMemcpyHostToDevice, st));
+            GUST_HIP(hipStreamSynchronize(st));
+            return;
+        }
         if (own) {
             GUST_HIP(hipMalloc(&scratch, scratch_elems * sizeof(T) + 256));
             GUST_HIP(hipMalloc(&chg, size_t(n) * 8 + 8));
@@ -1211,6 +1237,70 @@ void zline_jacobi_into(int nsys, const gust_line_system* sys, int sweeps, unsign
     plan.run(sweeps, nullptr, st);
 }
 
+// ---------------------------------------------------------------------------
+// The entry point that allocates nothing: its scratch (the last-sweep
+// change, the system table and the iterate) lives in this fixed array, one
+// copy per device, held until the process ends. Calls that overlap on one
+// device share it and race; each call clears what must start at zero.
+
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once
+// per device with that device current (in host code the array's name is not
+// a device address).
+char* noalloc_scratch()
+{
+    int dev = 0;
+    GUST_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) { GUST_FAIL("gust: zline_jacobi_noalloc: device %d out of range", dev); }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            GUST_FAIL("gust: zline_jacobi_noalloc: hipGetSymbolAddress of the scratch array failed (%s)",
+                      hipGetErrorString(e));
+        }
+        g_scratch_ptr[dev] = static_cast<char*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+void zjacobi_noalloc(int loc, const gust_array4* A, const gust_array4* ap, const gust_array4* rhs, int sweeps,
+                     const gust_geometry* geom, gust_array4* delta, double* change, hipStream_t st)
+{
+    if (!A || !ap || !rhs || !delta || !geom) { GUST_FAIL("gust: zline_jacobi_noalloc: null argument"); }
+    if (sweeps < 1) { GUST_FAIL("gust: zline_jacobi: sweeps must be >= 1 (got %d)", sweeps); }
+    gust_line_system q{};
+    q.loc = loc;
+    q.geom = *geom;
+    q.A = *A;
+    q.ap = *ap;
+    q.rhs = *rhs;
+    q.delta = *delta;
+    char* base = noalloc_scratch();
+    JacobiPlan<double> plan;
+    plan.init(1, &q, false, st, base, kNoAllocScratchBytes);
+    plan.run(sweeps, nullptr, st);
+    if (change) {
+        unsigned long long bits = 0;
+        GUST_HIP(hipMemcpyAsync(&bits, plan.chg, 8, hipMemcpyDeviceToHost, st));
+        GUST_HIP(hipStreamSynchronize(st));
+        *change = bits_to_double(bits);
+    } else {
+        GUST_HIP(hipStreamSynchronize(st));
+    }
+}
+
+}   // namespace
+
 namespace {
 
 template <typename T>
@@ -1240,6 +1330,12 @@ int gust_zline_jacobi(int loc, const gust_array4* A, const gust_array4* ap, cons
                       const gust_geometry* geom, gust_array4* delta, double* change, void* stream)
 {
     return guarded([&] { zjacobi_one<double>(loc, A, ap, rhs, sweeps, geom, delta, change, (hipStream_t)stream); });
+}
+
+int gust_zline_jacobi_noalloc(int loc, const gust_array4* A, const gust_array4* ap, const gust_array4* rhs,
+                              int sweeps, const gust_geometry* geom, gust_array4* delta, double* change, void* stream)
+{
+    return guarded([&] { zjacobi_noalloc(loc, A, ap, rhs, sweeps, geom, delta, change, (hipStream_t)stream); });
 }
 
 int gust_zline_jacobi_f32(int loc, const gust_array4* A, const gust_array4* ap, const gust_array4* rhs, int sweeps,

@@ -43,6 +43,17 @@ def zline_jacobi(loc: int, A: Fab, ap: Fab, rhs: Fab, sweeps: int, geom: Geometr
     return change.value
 
 
+def zline_jacobi_noalloc(loc: int, A: Fab, ap: Fab, rhs: Fab, sweeps: int, geom: Geometry, delta: Fab,
+                         stream=None) -> float:
+    """zline_jacobi through the entry point that allocates nothing (its
+    scratch is a fixed array per device; FP64 only)."""
+    change = C.c_double(0.0)
+    a, p, r, d, g = A.array4(), ap.array4(), rhs.array4(), delta.array4(), geom.c()
+    _lib.check(_lib.lib.gust_zline_jacobi_noalloc(int(loc), C.byref(a), C.byref(p), C.byref(r), int(sweeps),
+                                                  C.byref(g), C.byref(d), C.byref(change), _stream(stream)))
+    return change.value
+
+
 def zline_lusgs(loc: int, A: Fab, ap: Fab, rhs: Fab, sweeps: int, geom: Geometry, delta: Fab,
                 opt: LineSweepOptions | None = None, stream=None) -> float:
     opt = opt or LineSweepOptions(kind=LUSGS)

@@ -108,6 +108,19 @@ int gust_release_stream(void* stream);
 int gust_zline_jacobi(int loc, const gust_array4* A, const gust_array4* ap, const gust_array4* rhs, int sweeps,
                       const gust_geometry* geom, gust_array4* delta, double* change, void* stream);
 
+/* gust_zline_jacobi that allocates nothing: no hipMalloc, hipFree, pinned
+ * host memory or allocating library routine. No counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture
+ * set-up, tight serving loops, harnesses that forbid allocation). Same
+ * arguments, results and refusals as gust_zline_jacobi, bit for bit. Its
+ * scratch (the change accumulator, the system table and the iterate) is a
+ * fixed 64 MiB array per device held until the process ends; systems that
+ * need more (about 8 bytes per unknown plus 512), or more than 320 levels,
+ * are refused with an error. Calls that overlap on one device share that
+ * array and race; synchronises `stream` before returning. */
+int gust_zline_jacobi_noalloc(int loc, const gust_array4* A, const gust_array4* ap, const gust_array4* rhs,
+                              int sweeps, const gust_geometry* geom, gust_array4* delta, double* change, void* stream);
+
 /* The same on float data (the mixed-precision preconditioner path);
  * *change is returned as double. */
 int gust_zline_jacobi_f32(int loc, const gust_array4* A, const gust_array4* ap, const gust_array4* rhs, int sweeps,
