@@ -134,6 +134,7 @@ class PredictorOptionsC(C.Structure):
         ("gravity", C.c_double),
         ("theta0", C.c_double),
         ("sweep", LineOptionsC),
+        ("sweep_precision", C.c_int32),
     ]
 
 
@@ -269,7 +270,7 @@ class SimplecOptionsC(C.Structure):
     _fields_ = [("pressure_reltol", C.c_double), ("alpha_p", C.c_double), ("rotation_relax", C.c_double),
                 ("w_vnn_cap", C.c_double), ("w_vnn_cap_slope_margin", C.c_double), ("sweeps", C.c_int32),
                 ("sweep", LineOptionsC), ("tke_floor", C.c_double), ("scales", JvpScalesC), ("height", C.c_double),
-                ("poisson", PoissonOptionsC)]
+                ("poisson", PoissonOptionsC), ("sweep_precision", C.c_int32)]
 
 
 class SimplecInfoC(C.Structure):

@@ -1301,6 +1301,16 @@ void zjacobi_noalloc(int loc, const gust_array4* A, const gust_array4* ap, const
 
 }   // namespace
 
+void zline_jacobi_into_f32(int nsys, const gust_line_system* sys, int sweeps, unsigned long long* changes,
+                           hipStream_t st)
+{
+    JacobiPlan<float> plan;
+    plan.init(nsys, sys, false, st);
+    plan.chg = changes;
+    plan.chg_zeroed = true;
+    plan.run(sweeps, nullptr, st);
+}
+
 namespace {
 
 template <typename T>

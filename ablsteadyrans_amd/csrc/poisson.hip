@@ -277,6 +277,95 @@ __global__ void metric_kernel(stencil::ZMetric<R, NodeAcc<R>> zm, int nx, int ny
     }
 }
 
+// The mesh factors and face coefficients one cell's operator reads, loaded
+// once into registers: the 15 probes of coef_seed_kernel evaluate the same
+// expressions on the same values (hence the same bits) without reloading
+// them. Reads use offsets known at compile time once inlined.
+template <typename R>
+struct MetricCache {
+    R ryv[2][2];     // ry(i + a, j, k + c)
+    R rxv[2][2];     // rx(i, j + b, k + c)
+    R zfv[3][2];     // hzi, hxz, hez at (i, j, k + c)
+    int ci, cj, ck;
+    __device__ __forceinline__ R ry(int i, int, int k) const { return ryv[i - ci][k - ck]; }
+    __device__ __forceinline__ R rx(int, int j, int k) const { return rxv[j - cj][k - ck]; }
+    __device__ __forceinline__ R hzi(int, int, int k) const { return zfv[0][k - ck]; }
+    __device__ __forceinline__ R hxz(int, int, int k) const { return zfv[1][k - ck]; }
+    __device__ __forceinline__ R hez(int, int, int k) const { return zfv[2][k - ck]; }
+};
+
+// A face or cell array read at the cell's own point and one step up in a
+// single direction dir (0: i + 1, 1: j + 1, 2: k + 1).
+template <typename R>
+struct FaceCache {
+    const R* p;   // non-null when the array is present (z_cross)
+    R v0, v1;
+    int ci, cj, ck;
+    __device__ __forceinline__ R operator()(int i, int j, int k) const
+    {
+        return (i - ci) + (j - cj) + (k - ck) == 0 ? v0 : v1;
+    }
+};
+
+template <typename R>
+__device__ __forceinline__ FaceCache<R> face_cache(const GridAcc<R>& a, int dir, bool two, int i, int j, int k)
+{
+    FaceCache<R> f;
+    f.p = a.p;
+    f.ci = i;
+    f.cj = j;
+    f.ck = k;
+    f.v0 = R(0);
+    f.v1 = R(0);
+    if (a.p) {
+        f.v0 = a(i, j, k);
+        if (two) { f.v1 = a(i + (dir == 0), j + (dir == 1), k + (dir == 2)); }
+    }
+    return f;
+}
+
+template <typename R>
+using CachedCtx = OpCtx<R, NodeAcc<R>, FaceCache<R>, MetricCache<R>>;
+
+template <typename R>
+__device__ __forceinline__ CachedCtx<R> cache_ctx(const Ctx<R>& c, int i, int j, int k)
+{
+    CachedCtx<R> q;
+    q.zp = c.zp;
+    q.mt.ci = i;
+    q.mt.cj = j;
+    q.mt.ck = k;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            q.mt.ryv[a][d] = c.mt.ry(i + a, j, k + d);
+            q.mt.rxv[a][d] = c.mt.rx(i, j + a, k + d);
+        }
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+        q.mt.zfv[0][d] = c.mt.hzi(i, j, k + d);
+        q.mt.zfv[1][d] = c.mt.hxz(i, j, k + d);
+        q.mt.zfv[2][d] = c.mt.hez(i, j, k + d);
+    }
+    q.ax = face_cache(c.ax, 0, true, i, j, k);
+    q.ay = face_cache(c.ay, 1, true, i, j, k);
+    q.az = face_cache(c.az, 2, true, i, j, k);
+    q.J = face_cache(c.J, 2, false, i, j, k);
+    q.dcross = face_cache(c.dcross, 2, true, i, j, k);
+    q.az_mesh = face_cache(c.az_mesh, 2, true, i, j, k);
+    q.dxinv = c.dxinv;
+    q.dyinv = c.dyinv;
+    q.dzinv = c.dzinv;
+    q.nx = c.nx;
+    q.ny = c.ny;
+    q.nz = c.nz;
+    q.zf = c.zf;
+    q.dir_ground = c.dir_ground;
+    q.top_open = c.top_open;
+    return q;
+}
+
 // The explicit 15-point stencil of a level: each coefficient is the
 // operator applied to a probe that is 1 at one point (column, level) and 0
 // elsewhere, with the boundary mirrors resolved exactly as the operator
@@ -293,6 +382,7 @@ __global__ __launch_bounds__(kThreads) void coef_seed_kernel(Ctx<R> c, MirrorAcc
         int ni[4], nj[4];
         bool ok[4];
         nb_columns(i, j, c.nx, c.ny, perx, pery, ni, nj, ok);
+        const CachedCtx<R> cc = cache_ctx(c, i, j, k);
         // the resolved footprint
         int ri[5][3], rj[5][3], rk[5][3];
         R sg[5][3];
@@ -326,7 +416,7 @@ __global__ __launch_bounds__(kThreads) void coef_seed_kernel(Ctx<R> c, MirrorAcc
                         const bool hit = ri[col][e] == ti && rj[col][e] == tj && rk[col][e] == k + d;
                         pt.v[col][e] = hit ? sg[col][e] : R(0);
                     }
-                v = stencil::apply_cell<R, R>(pt, c, i, j, k);
+                v = stencil::apply_cell<R, R>(pt, cc, i, j, k);
             }
             const int e = p == 0 ? (d == -1 ? kBLo : d == 0 ? kBDg : kBUp) : kBNb + 3 * (p - 1) + d + 1;
             bundle[t * kBundle + e] = v;

@@ -148,6 +148,62 @@ __global__ __launch_bounds__(kThreads) void add_max3_kernel(AddMaxSet s)
     if (threadIdx.x == 0 && best) { atomicMax(s.out[q], best); }
 }
 
+// The mixed-precision sweeps' operands: A's six neighbour components, a_P
+// and rhs of up to three systems rounded to FP32 compact arrays
+// (blockIdx.y: the system).
+struct ToF32Set {
+    Box3 b[3];
+    CV A[3], ap[3], r[3];
+    int64_t an[3];
+    float* A32[3];
+    float* ap32[3];
+    float* r32[3];
+};
+
+__global__ __launch_bounds__(kThreads) void to_f32_kernel(ToF32Set s)
+{
+    const int q = blockIdx.y;
+    const Box3& b = s.b[q];
+    const int64_t n = b.npts();
+    GUST_GRID_LOOP(b)
+    {
+        int i, j, k;
+        b.at(t, i, j, k);
+        const double* a = &s.A[q](i, j, k);
+#pragma unroll
+        for (int c = 1; c < 7; ++c) { s.A32[q][t + c * n] = float(a[c * s.an[q]]); }
+        s.ap32[q][t] = float(s.ap[q](i, j, k));
+        s.r32[q][t] = float(s.r[q](i, j, k));
+    }
+}
+
+// y_q += x_q with x_q FP32 (the mixed-precision change, widened exactly)
+// and max |x_q| into out[q], for up to three fields (blockIdx.y).
+struct AddMaxSetF {
+    Box3 b[3];
+    V y[3];
+    DView<const float> x[3];
+    unsigned long long* out[3];
+};
+
+__global__ __launch_bounds__(kThreads) void add_max3_f32_kernel(AddMaxSetF s)
+{
+    const int q = blockIdx.y;
+    const Box3& b = s.b[q];
+    unsigned long long best = 0;
+    GUST_GRID_LOOP(b)
+    {
+        int i, j, k;
+        b.at(t, i, j, k);
+        const double v = double(s.x[q](i, j, k));
+        s.y[q](i, j, k) = s.y[q](i, j, k) + v;
+        const unsigned long long bits = dev::abs_bits(v);
+        best = bits > best ? bits : best;
+    }
+    best = dev::block_max_u64<kThreads>(best);
+    if (threadIdx.x == 0 && best) { atomicMax(s.out[q], best); }
+}
+
 // y += x and max |x| into *out (atomic max of ordered bits).
 __global__ __launch_bounds__(kThreads) void add_max_kernel(Box3 b, V y, CV x, unsigned long long* out)
 {
@@ -449,6 +505,53 @@ Scratch scratch(hipStream_t st, int slot, const IBox& valid, int g)
     return s;
 }
 
+// A compact FP32 field of ncomp components over box b from the workspace.
+gust_array4 scratch_f32(hipStream_t st, int slot, const IBox& b, int ncomp)
+{
+    const int64_t n = b.npts();
+    gust_array4 v{};
+    v.p = Workspace::get(st, slot, size_t(n) * size_t(ncomp) * sizeof(float));
+    v.jstride = b.len(0);
+    v.kstride = int64_t(b.len(0)) * b.len(1);
+    v.nstride = n;
+    for (int d = 0; d < 3; ++d) {
+        v.begin[d] = b.lo[d];
+        v.end[d] = b.hi[d] + 1;
+    }
+    v.ncomp = ncomp;
+    return v;
+}
+
+// The FP32 twins of n <= 3 FP64 z-line systems (A, a_P, rhs rounded in one
+// launch; delta an FP32 field with one ghost), in workspace slots 80-91.
+void systems_f32(int n, const gust_line_system* in, gust_line_system* out, hipStream_t st)
+{
+    ToF32Set c{};
+    int64_t most = 1;
+    for (int q = 0; q < n; ++q) {
+        const gust_line_system& a = in[q];
+        const IBox vb = loc_box(a.geom, a.loc);
+        gust_line_system& b = out[q];
+        b.loc = a.loc;
+        b.geom = a.geom;
+        b.A = scratch_f32(st, 80 + 4 * q, vb, GUST_N_COEFS);
+        b.ap = scratch_f32(st, 81 + 4 * q, vb, 1);
+        b.rhs = scratch_f32(st, 82 + 4 * q, vb, 1);
+        b.delta = scratch_f32(st, 83 + 4 * q, grow(vb, 1, 1, 1), 1);
+        c.b[q] = box3(vb);
+        c.A[q] = dview<const double>(&a.A);
+        c.an[q] = a.A.nstride;
+        c.ap[q] = dview<const double>(&a.ap);
+        c.r[q] = dview<const double>(&a.rhs);
+        c.A32[q] = static_cast<float*>(b.A.p);
+        c.ap32[q] = static_cast<float*>(b.ap.p);
+        c.r32[q] = static_cast<float*>(b.rhs.p);
+        most = std::max<int64_t>(most, c.b[q].npts());
+    }
+    to_f32_kernel<<<dim3(blocks_for(most), n), kThreads, 0, st>>>(c);
+    GUST_HIP(hipGetLastError());
+}
+
 // Reductions into a deferred batch: each registers a slot and the callback
 // that receives its value when the batch resolves. Requests queue up and
 // run() reduces them all in one launch (at most kJobs; a full queue runs by
@@ -506,6 +609,25 @@ public:
             most = std::max<int64_t>(most, a.b[q].npts());
         }
         add_max3_kernel<<<dim3(cgrid::reduce_blocks(most), n), kThreads, 0, st_>>>(a);
+        GUST_HIP(hipGetLastError());
+    }
+    // add_max_abs_n with FP32 changes x_q.
+    void add_max_abs_n_f32(int n, const IBox* box, gust_array4* const* y, const gust_array4* const* x,
+                           std::vector<std::function<void(double)>> f)
+    {
+        AddMaxSetF a{};
+        int64_t most = 1;
+        for (int q = 0; q < n; ++q) {
+            a.b[q] = box3(box[q]);
+            a.y[q] = dview<double>(y[q]);
+            a.x[q] = dview<const float>(x[q]);
+            std::function<void(double)> fq = f[size_t(q)];
+            a.out[q] = dq_.zero_slot([fq](unsigned long long v) {
+                if (fq) { fq(bits_to_double(v)); }
+            });
+            most = std::max<int64_t>(most, a.b[q].npts());
+        }
+        add_max3_f32_kernel<<<dim3(cgrid::reduce_blocks(most), n), kThreads, 0, st_>>>(a);
         GUST_HIP(hipGetLastError());
     }
     void add_max_abs(const IBox& box, const gust_array4* y, const gust_array4* x, std::function<void(double)> f)
@@ -604,7 +726,7 @@ void scalar_sweeps_theta_tke(const gust_geometry& g, gust_array4* theta, const g
                              const gust_array4* A_theta, gust_array4* tke, const gust_array4* R_tke,
                              const gust_array4* A_tke, const gust_array4* dtau, int sweeps, double floor,
                              const gust_line_options& sweep, gust_scalar_sweep_result* res_theta,
-                             gust_scalar_sweep_result* res_tke, hipStream_t st)
+                             gust_scalar_sweep_result* res_tke, hipStream_t st, int sweep_precision)
 {
     void* stream = st;
     if (sweep.kind != GUST_SWEEP_ZJACOBI) {
@@ -654,16 +776,24 @@ void scalar_sweeps_theta_tke(const gust_geometry& g, gust_array4* theta, const g
             if (r) { r->last_sweep = bits_to_double(v); }
         });
     }
-    zline_jacobi_into(2, sys, sweeps, dq->zero_run(std::move(fs)), st);
+    gust_line_system s32[2];
+    if (sweep_precision == 1) {
+        systems_f32(2, sys, s32, st);
+        zline_jacobi_into_f32(2, s32, sweeps, dq->zero_run(std::move(fs)), st);
+    } else {
+        zline_jacobi_into(2, sys, sweeps, dq->zero_run(std::move(fs)), st);
+    }
     {
         const IBox boxes[2] = {cells, cells};
-        const gust_array4* xs[2] = {&sys[0].delta, &sys[1].delta};
+        const gust_line_system* used = sweep_precision == 1 ? s32 : sys;
+        const gust_array4* xs[2] = {&used[0].delta, &used[1].delta};
         std::vector<std::function<void(double)>> fs2(2);
         for (int q = 0; q < 2; ++q) {
             gust_scalar_sweep_result* r = res[q];
             if (r) { fs2[size_t(q)] = [r](double x) { r->max_dq = x; }; }
         }
-        red.add_max_abs_n(2, boxes, field, xs, std::move(fs2));
+        if (sweep_precision == 1) { red.add_max_abs_n_f32(2, boxes, field, xs, std::move(fs2)); }
+        else { red.add_max_abs_n(2, boxes, field, xs, std::move(fs2)); }
     }
     unsigned long long* cnt = dq->zero_slot([res_tke](unsigned long long v) {
         if (res_tke) { res_tke->clipped = static_cast<int64_t>(v); }
@@ -859,6 +989,12 @@ int gust_momentum_predictor(const gust_geometry* geom, gust_faces* vel, const gu
                       "conditions (the Dirichlet rows they skip); give in.bc, or gravity 0 and Coriolis off");
         }
         check_rc(gust_check_line_options(&opt->sweep));
+        if (opt->sweep_precision != 0 && opt->sweep_precision != 1) {
+            GUST_FAIL("gust: momentum_predictor: sweep_precision must be 0 (FP64) or 1 (FP32 sweeps)");
+        }
+        if (opt->sweep_precision == 1 && opt->sweep.kind != GUST_SWEEP_ZJACOBI) {
+            GUST_FAIL("gust: momentum_predictor: FP32 sweeps are z-line Jacobi sweeps (steady.sweep = zjacobi)");
+        }
         double* dmax[3] = {nullptr, nullptr, nullptr};
         double* last[3] = {nullptr, nullptr, nullptr};
         if (result) {
@@ -984,19 +1120,27 @@ int gust_momentum_predictor(const gust_geometry* geom, gust_faces* vel, const gu
                     if (out) { *out = bits_to_double(v); }
                 });
             }
-            zline_jacobi_into(3, sys, opt->sweeps, dq->zero_run(std::move(fs)), st);
+            gust_line_system s32[3];
+            const bool f32 = opt->sweep_precision == 1;
+            if (f32) {
+                systems_f32(3, sys, s32, st);
+                zline_jacobi_into_f32(3, s32, opt->sweeps, dq->zero_run(std::move(fs)), st);
+            } else {
+                zline_jacobi_into(3, sys, opt->sweeps, dq->zero_run(std::move(fs)), st);
+            }
             IBox boxes[3];
             const gust_array4* xs[3];
             std::vector<std::function<void(double)>> fs3(3);
             for (int c = 0; c < 3; ++c) {
                 boxes[c] = loc_box(g, 1 << c);
-                xs[c] = &sys[c].delta;
+                xs[c] = f32 ? &s32[c].delta : &sys[c].delta;
                 if (dmax[c]) {
                     double* out = dmax[c];
                     fs3[size_t(c)] = [out](double x) { *out = x; };
                 }
             }
-            red.add_max_abs_n(3, boxes, vel->c, xs, std::move(fs3));
+            if (f32) { red.add_max_abs_n_f32(3, boxes, vel->c, xs, std::move(fs3)); }
+            else { red.add_max_abs_n(3, boxes, vel->c, xs, std::move(fs3)); }
         }
         dq.finish();
     });

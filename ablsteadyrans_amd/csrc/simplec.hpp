@@ -12,12 +12,15 @@ void scalar_sweeps_theta_tke(const gust_geometry& g, gust_array4* theta, const g
                              const gust_array4* A_theta, gust_array4* tke, const gust_array4* R_tke,
                              const gust_array4* A_tke, const gust_array4* dtau, int sweeps, double floor,
                              const gust_line_options& sweep, gust_scalar_sweep_result* res_theta,
-                             gust_scalar_sweep_result* res_tke, hipStream_t st);
+                             gust_scalar_sweep_result* res_tke, hipStream_t st, int sweep_precision = 0);
 
 // FP64 z-line Jacobi sweeps of a batch of systems (one launch per sweep),
 // each system's last-sweep change written as the ordered bits of |change|
 // into changes[s] (nsys slots holding zeros); defined in line_sweeps.hip.
 void zline_jacobi_into(int nsys, const gust_line_system* sys, int sweeps, unsigned long long* changes,
                        hipStream_t st);
+// The same on FP32 systems (the views' pointers are float data).
+void zline_jacobi_into_f32(int nsys, const gust_line_system* sys, int sweeps, unsigned long long* changes,
+                           hipStream_t st);
 
 }   // namespace gust_amd

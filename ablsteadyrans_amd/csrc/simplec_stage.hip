@@ -134,6 +134,12 @@ void check_options(const char* op, const gust_simplec_options& o)
         GUST_FAIL("gust: %s: simplec.rotation_relax must be in (0, 1]", op);
     }
     if (o.sweeps < 1) { GUST_FAIL("gust: %s: sweeps must be >= 1 (steady.zline_sweeps)", op); }
+    if (o.sweep_precision != 0 && o.sweep_precision != 1) {
+        GUST_FAIL("gust: %s: sweep_precision must be 0 (FP64) or 1 (FP32 sweeps)", op);
+    }
+    if (o.sweep_precision == 1 && o.sweep.kind != GUST_SWEEP_ZJACOBI) {
+        GUST_FAIL("gust: %s: FP32 sweeps are z-line Jacobi sweeps (steady.sweep = zjacobi)", op);
+    }
     if (std::isnan(o.tke_floor) || !(o.tke_floor >= 0)) {
         GUST_FAIL("gust: %s: tke_floor must be >= 0 (turb.tke_floor)", op);
     }
@@ -194,6 +200,7 @@ void iterate_body(SimplecStage& s, gust_state* q, const gust_array4* dtau, gust_
     po.gravity = s.phys.forcing.gravity;
     po.theta0 = s.phys.forcing.theta0;
     po.sweep = s.opt.sweep;
+    po.sweep_precision = s.opt.sweep_precision;
     check_rc(gust_momentum_predictor(geom, &vel, q->f[4], &s.z, &Rf, &s.R[4].v, &Af, buoy ? &s.At.v : nullptr, &Df,
                                      &Mf, &po, &Pf, &Qf, &inf.predictor, stream));
     // 4. the pressure correction on u* (its ghosts refilled; the wall state
@@ -222,7 +229,7 @@ void iterate_body(SimplecStage& s, gust_state* q, const gust_array4* dtau, gust_
     check_rc(gust_implicit_rows(GUST_EQ_THETA, geom, &Ff, &s.mesh, &s.phys, cf, ssv, &s.At.v, stream));
     check_rc(gust_implicit_rows(GUST_EQ_TKE, geom, &Ff, &s.mesh, &s.phys, cf, ssv, &s.Ak.v, stream));
     scalar_sweeps_theta_tke(*geom, q->f[4], &s.R[4].v, &s.At.v, q->f[5], &s.R[5].v, &s.Ak.v, dtau, s.opt.sweeps,
-                            s.opt.tke_floor, s.opt.sweep, &inf.theta, &inf.tke, st);
+                            s.opt.tke_floor, s.opt.sweep, &inf.theta, &inf.tke, st, s.opt.sweep_precision);
 }
 
 // One iteration with its checks and reported values read back together at

@@ -155,9 +155,66 @@ def askervein(grid: M.ErfGrid) -> Deck:
                 2.0, 2)
 
 
+# The open-channel neutral ABL of the benchmark configurations: inflow at
+# xlo (the log law along +x), outflow at xhi, y periodic, the surface layer at
+# the ground, a slip top.
+OPEN_X_FACES = (GP.INFLOW, GP.PERIODIC, GP.SURFACE_LAYER, GP.OUTFLOW, GP.PERIODIC, GP.SLIP_WALL)
+
+
+def open_abl(name: str, n, extent, dz0: float, terrain, sweeps: int = 2, cfl_max: float = 10.0,
+             drop_orders: float = 6.0, max_iterations: int = 20000, rotation_relax: float = 0.3,
+             damping_fraction: float = 0.0, ref_iterations: int = 5, pressure_reltol: float = 1e-3) -> Deck:
+    """A neutral ABL over a tile of n = (nx, ny, nz) cells and extent (Lx,
+    Ly) m: levels from dz0 growing by 1.08 (the top free), the log-law inflow
+    u* = 0.4 m/s with z0 = 0.03 m at xlo, outflow at xhi, y periodic, the
+    surface layer (z0 = 0.03 m), a slip top, Coriolis with f = 1e-4 1/s
+    (latitude 90, horizontal terms only, no geostrophic forcing), the k-L
+    closure's defaults, `sweeps` z-line sweeps, CFL 1 -> cfl_max, the
+    default residual reference (5 iterations) and `drop_orders`; the first
+    guess the inflow profile by height above the ground made divergence-free.
+    damping_fraction > 0 adds Rayleigh damping of w above that fraction of
+    the top (0.2 1/s)."""
+    nx, ny, nz = n
+    zl = M.make_zlevels(nz, 1.0, dz0=dz0, stretch_ratio=1.08)
+    top = float(zl[-1])
+    lo, hi = (0.0, 0.0, 0.0), (float(extent[0]), float(extent[1]), top)
+    geom = M.geometry((nx, ny, nz), lo, hi, (0, 1))
+    tile = M.build_mesh(geom, lo, hi, zl, terrain)
+    prof = GI.InflowProfile(kind=GI.LOG_LAW, ustar=0.4, z0=0.03, direction=0.0)
+    damping = None
+    if damping_fraction > 0:
+        damping = GP.Damping(rayleigh_zdamp=damping_fraction * top, rayleigh_dampcoef=0.2, fields=("w",),
+                             ztop=top, prob_lo=lo[:2], prob_hi=hi[:2])
+    inputs = GP.ResidualInputs(closure=GP.ClosureOptions(), surface=GP.SurfaceLayerOptions(z0=0.03),
+                               forcing=GP.ForcingOptions(coriolis=True, coriolis_3d=False, latitude=90.0,
+                                                         rotation_period=125663.7061435917),
+                               faces=OPEN_X_FACES, inflow=prof, damping=damping)
+    return Deck(name, tile, inputs, prof, GI.InitOptions(first_guess=1),
+                GP.SimplecOptions(sweeps=sweeps, height=_height(tile, 0.0, top), rotation_relax=rotation_relax,
+                                  pressure_reltol=pressure_reltol),
+                GP.DriverOptions(max_iterations=max_iterations, cfl0=1.0, cfl_max=cfl_max,
+                                 residual_drop_orders=drop_orders, residual_ref_iterations=ref_iterations), 0)
+
+
+def flat_abl_64(**kw) -> Deck:
+    """The flat neutral ABL benchmark: 64 x 64 x 48 cells over 5 x 5 km, levels
+    from dz1 = 2 m growing by 1.08 (top 980.26 m), two sweeps, CFL 1 -> 10."""
+    return open_abl("flat_abl_64x64x48", (64, 64, 48), (5000.0, 5000.0), 2.0, M.Terrain(kind="flat"), **kw)
+
+
+def gaussian_hill_200(**kw) -> Deck:
+    """The Gaussian-hill benchmark: h = 100 exp(-(r / 250 m)^2) m centred in a
+    4 x 4 km tile at dx = dy = 20 m (200 x 200 cells), 64 levels from dz1 =
+    1 m growing by 1.08 (top 1709.5 m), the open-channel ABL of open_abl."""
+    t = M.Terrain(kind="hills", hills=[(2000.0, 2000.0, 100.0, 250.0)])
+    return open_abl("gaussian_hill_200x200x64", (200, 200, 64), (4000.0, 4000.0), 1.0, t, **kw)
+
+
 def prepare(deck: Deck, precision: int = 0) -> Deck:
     """The device mesh, the state at the deck's first guess and the stage
-    (precision 1: FP32 multigrid V-cycles in every pressure correction)."""
+    (precision 1: FP32 multigrid V-cycles in every pressure correction;
+    2: the mixed-precision form, FP32 V-cycles and FP32 z-line sweeps inside
+    the FP64 defect correction)."""
     deck.mesh = PS.DeviceMesh(deck.tile)
     deck.state = GP.DeviceState(deck.tile.geom)
     faces = deck.inputs.faces
@@ -166,7 +223,8 @@ def prepare(deck: Deck, precision: int = 0) -> Deck:
         GI.initialize_state(deck.mesh, deck.profile, deck.init, faces, solver, deck.state)
     else:
         GI.first_guess_state(deck.mesh, deck.profile, deck.init, deck.state)
-    deck.stage.poisson_precision = precision
+    deck.stage.poisson_precision = 1 if precision >= 1 else 0
+    deck.stage.sweep_precision = 1 if precision >= 2 else 0
     deck.simplec = GP.SimplecStage(deck.mesh, deck.inputs, deck.stage)
     return deck
 

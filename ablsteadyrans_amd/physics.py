@@ -305,6 +305,12 @@ class SimplecOptions:
     height: float = 1.0
     poisson_precision: int = 0
     poisson_max_iter: int = 200
+    sweep_precision: int = 0
+
+    @property
+    def mixed(self) -> bool:
+        """FP32 sweeps and V-cycles inside the FP64 defect correction."""
+        return self.poisson_precision == 1 and self.sweep_precision == 1
 
     def c(self):
         o = _lib.SimplecOptionsC()
@@ -314,6 +320,7 @@ class SimplecOptions:
         o.sweeps, o.tke_floor, o.height = int(self.sweeps), self.tke_floor, self.height
         o.poisson.precision = int(self.poisson_precision)
         o.poisson.max_iter = int(self.poisson_max_iter)
+        o.sweep_precision = int(self.sweep_precision)
         return o
 
 

@@ -54,6 +54,7 @@ class PredictorOptions:
     gravity: float = 0.0
     theta0: float = 300.0
     sweep: LineSweepOptions = field(default_factory=LineSweepOptions)
+    sweep_precision: int = 0   # 1: FP32 sweeps inside the FP64 defect correction
 
     def c(self):
         o = _lib.PredictorOptionsC()
@@ -62,6 +63,7 @@ class PredictorOptions:
             self.w_vnn_slope_margin
         o.coriolis_f, o.gravity, o.theta0 = self.coriolis_f, self.gravity, self.theta0
         o.sweep = self.sweep.c()
+        o.sweep_precision = int(self.sweep_precision)
         return o
 
 

@@ -461,6 +461,11 @@ typedef struct gust_predictor_options {
     double coriolis_f;
     double gravity, theta0;
     gust_line_options sweep;
+    /* 0: the z-line Jacobi sweeps in FP64 (the reference); 1: mixed precision,
+     * the sweeps' operands (A, a_P, rhs) rounded to FP32 and solved in FP32,
+     * the change added to the FP64 field (a defect correction: R stays FP64,
+     * so the converged field is the FP64 one). Jacobi sweeps only. */
+    int32_t sweep_precision;
 } gust_predictor_options;
 
 /* Source/Predictor.H:214-220 (MomentumPredictorResult). */
@@ -839,6 +844,10 @@ typedef struct gust_simplec_options {
     gust_jvp_scales scales;
     double height;
     gust_poisson_options poisson;
+    /* gust_predictor_options.sweep_precision for every sweep of the stage
+     * (momentum and scalars); with poisson.precision = 1 the stage's
+     * mixed-precision form. */
+    int32_t sweep_precision;
 } gust_simplec_options;
 
 /* Source/Simplec.H:178-184 (SimplecIterationInfo). */
