@@ -15,7 +15,10 @@ steady driver.
 """
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass, field
+
+import numpy as np
 
 from . import initializer as GI
 from . import mesh as M
@@ -155,32 +158,46 @@ def askervein(grid: M.ErfGrid) -> Deck:
                 2.0, 2)
 
 
-# The open-channel neutral ABL of the benchmark configurations: inflow at
-# xlo (the log law along +x), outflow at xhi, y periodic, the surface layer at
-# the ground, a slip top.
-OPEN_X_FACES = (GP.INFLOW, GP.PERIODIC, GP.SURFACE_LAYER, GP.OUTFLOW, GP.PERIODIC, GP.SLIP_WALL)
+def open_faces(direction: float):
+    """The face types of an open channel for a wind from `direction` degrees
+    (counter-clockwise from +x): inflow on the side the wind enters and
+    outflow opposite, along x when |cos| >= |sin| (y periodic), else along y
+    (x periodic); the surface layer at the ground, a slip top. Returns
+    (faces in Orientation order, periodicity)."""
+    a = math.radians(direction)
+    c, sn = math.cos(a), math.sin(a)
+    if abs(c) >= abs(sn):
+        xlo, xhi = (GP.INFLOW, GP.OUTFLOW) if c > 0 else (GP.OUTFLOW, GP.INFLOW)
+        return (xlo, GP.PERIODIC, GP.SURFACE_LAYER, xhi, GP.PERIODIC, GP.SLIP_WALL), (0, 1)
+    ylo, yhi = (GP.INFLOW, GP.OUTFLOW) if sn > 0 else (GP.OUTFLOW, GP.INFLOW)
+    return (GP.PERIODIC, ylo, GP.SURFACE_LAYER, GP.PERIODIC, yhi, GP.SLIP_WALL), (1, 0)
 
 
 def open_abl(name: str, n, extent, dz0: float, terrain, sweeps: int = 2, cfl_max: float = 10.0,
              drop_orders: float = 6.0, max_iterations: int = 20000, rotation_relax: float = 0.3,
-             damping_fraction: float = 0.0, ref_iterations: int = 5, pressure_reltol: float = 1e-3) -> Deck:
+             damping_fraction: float = 0.0, ref_iterations: int = 5, pressure_reltol: float = 1e-3,
+             direction: float = 0.0, zlevels=None) -> Deck:
     """A neutral ABL over a tile of n = (nx, ny, nz) cells and extent (Lx,
-    Ly) m: levels from dz0 growing by 1.08 (the top free), the log-law inflow
-    u* = 0.4 m/s with z0 = 0.03 m at xlo, outflow at xhi, y periodic, the
-    surface layer (z0 = 0.03 m), a slip top, Coriolis with f = 1e-4 1/s
-    (latitude 90, horizontal terms only, no geostrophic forcing), the k-L
-    closure's defaults, `sweeps` z-line sweeps, CFL 1 -> cfl_max, the
-    default residual reference (5 iterations) and `drop_orders`; the first
-    guess the inflow profile by height above the ground made divergence-free.
-    damping_fraction > 0 adds Rayleigh damping of w above that fraction of
-    the top (0.2 1/s)."""
+    Ly) m: levels from dz0 growing by 1.08 (the top free; or the given
+    zlevels), the log-law inflow u* = 0.4 m/s with z0 = 0.03 m from
+    `direction` on the side the wind enters, outflow opposite, the other
+    pair periodic (open_faces; direction 0: inflow at xlo, outflow at xhi, y
+    periodic), the surface layer (z0 = 0.03 m), a slip top, Coriolis with
+    f = 1e-4 1/s (latitude 90, horizontal terms only, no geostrophic
+    forcing), the k-L closure's defaults, `sweeps` z-line sweeps, CFL 1 ->
+    cfl_max, the default residual reference (5 iterations) and
+    `drop_orders`; the first guess the inflow profile by height above the
+    ground made divergence-free. damping_fraction > 0 adds Rayleigh damping
+    of w above that fraction of the top (0.2 1/s)."""
     nx, ny, nz = n
-    zl = M.make_zlevels(nz, 1.0, dz0=dz0, stretch_ratio=1.08)
+    zl = np.asarray(zlevels, dtype=np.float64) if zlevels is not None else \
+        M.make_zlevels(nz, 1.0, dz0=dz0, stretch_ratio=1.08)
     top = float(zl[-1])
     lo, hi = (0.0, 0.0, 0.0), (float(extent[0]), float(extent[1]), top)
-    geom = M.geometry((nx, ny, nz), lo, hi, (0, 1))
+    faces, periodic = open_faces(direction)
+    geom = M.geometry((nx, ny, nz), lo, hi, periodic)
     tile = M.build_mesh(geom, lo, hi, zl, terrain)
-    prof = GI.InflowProfile(kind=GI.LOG_LAW, ustar=0.4, z0=0.03, direction=0.0)
+    prof = GI.InflowProfile(kind=GI.LOG_LAW, ustar=0.4, z0=0.03, direction=direction)
     damping = None
     if damping_fraction > 0:
         damping = GP.Damping(rayleigh_zdamp=damping_fraction * top, rayleigh_dampcoef=0.2, fields=("w",),
@@ -188,7 +205,7 @@ def open_abl(name: str, n, extent, dz0: float, terrain, sweeps: int = 2, cfl_max
     inputs = GP.ResidualInputs(closure=GP.ClosureOptions(), surface=GP.SurfaceLayerOptions(z0=0.03),
                                forcing=GP.ForcingOptions(coriolis=True, coriolis_3d=False, latitude=90.0,
                                                          rotation_period=125663.7061435917),
-                               faces=OPEN_X_FACES, inflow=prof, damping=damping)
+                               faces=faces, inflow=prof, damping=damping)
     return Deck(name, tile, inputs, prof, GI.InitOptions(first_guess=1),
                 GP.SimplecOptions(sweeps=sweeps, height=_height(tile, 0.0, top), rotation_relax=rotation_relax,
                                   pressure_reltol=pressure_reltol),
@@ -208,6 +225,20 @@ def gaussian_hill_200(**kw) -> Deck:
     1 m growing by 1.08 (top 1709.5 m), the open-channel ABL of open_abl."""
     t = M.Terrain(kind="hills", hills=[(2000.0, 2000.0, 100.0, 250.0)])
     return open_abl("gaussian_hill_200x200x64", (200, 200, 64), (4000.0, 4000.0), 1.0, t, **kw)
+
+
+def fractal_tile_256():
+    """The fractal-terrain tile of the kernel-level configuration: 256 x 256 x
+    96 cells at dx = dy = 25 m (6.4 km), an isotropic Gaussian random
+    surface with spectral slope k^-2 (seed 20264203, rms 80 m, clipped to
+    0..400 m), the first layer 2 m with the ratio solved so the top is
+    2000 m, BTF, no periodic sides."""
+    nx, ny, nz = 256, 256, 96
+    lo, hi = (0.0, 0.0, 0.0), (nx * 25.0, ny * 25.0, 2000.0)
+    geom = M.geometry((nx, ny, nz), lo, hi)
+    zl = M.make_zlevels(nz, 2000.0, dz0=2.0, stretch_ratio=0.0)
+    h = M.fractal_heights(nx, ny, seed=20264203, rms=80.0, slope=-2.0, clip=(0.0, 400.0))
+    return M.build_mesh(geom, lo, hi, zl, M.Terrain(kind="fractal", heights=h))
 
 
 def prepare(deck: Deck, precision: int = 0) -> Deck:

@@ -17,7 +17,6 @@ server with every request arriving at time zero.
 """
 from __future__ import annotations
 
-import math
 import threading
 import time
 from dataclasses import dataclass, field
@@ -31,29 +30,61 @@ from . import mesh as M
 from . import physics as GP
 from . import reductions as RD
 
-# Device bytes per cell of a served tile: the state, the stage's residual,
-# coefficient, flux and closure fields, the Poisson hierarchy and the
-# GMRES(50) Krylov basis (about 100 FP64 vectors per cell).
+# Device bytes per cell of a served tile before any measurement: the state,
+# the stage's residual, coefficient, flux and closure fields, the Poisson
+# hierarchy and the GMRES(50) Krylov basis (about 100 FP64 vectors per
+# cell). Server measures the real figure on a calibration tile
+# (measure_bytes_per_cell) and admits by that.
 BYTES_PER_CELL = 3200
+
+
+def measure_bytes_per_cell(spec: "TileSpec | None" = None, precision: int = 2) -> float:
+    """Device memory one steady request holds, per cell: the free-memory drop
+    across preparing its deck and two SIMPLEC iterations on a fresh stream
+    (the library's scratch for that stream included), on an otherwise idle
+    device. The stream's scratch is released afterwards."""
+    import torch
+    spec = spec or TileSpec(96, 96, 56, hills=[(960.0, 960.0, 60.0, 300.0)])
+    st = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    free0, _ = torch.cuda.mem_get_info()
+    with torch.cuda.stream(st):
+        deck = D.prepare(tile_deck(spec, "steady", 2), precision)
+        deck.solve()
+        st.synchronize()
+        free1, _ = torch.cuda.mem_get_info()
+        held = _lib.stream_bytes_held(st)
+    del deck
+    _lib.release_stream(st)
+    torch.cuda.synchronize()
+    used = max(free0 - free1, held)
+    return used / spec.cells
 
 
 @dataclass
 class TileSpec:
     """A periodic terrain tile: nx x ny x nz cells of dx metres, Gaussian hills
     (xc, yc, H, L) with h = sum H exp(-r^2 / L^2), the wind from `direction`
-    degrees (counter-clockwise from +x), the top at `top` metres."""
+    degrees (counter-clockwise from +x); levels from dz0 growing by
+    `stretch` (the top free), or uniform up to `top` when dz0 = 0."""
     nx: int
     ny: int
     nz: int
-    dx: float = 50.0
+    dx: float = 20.0
     hills: list = field(default_factory=list)
     direction: float = 0.0
     top: float = 1000.0
-    dz0: float = 0.0     # 0: uniform levels
+    dz0: float = 1.0
+    stretch: float = 1.08
 
     @property
     def cells(self) -> int:
         return self.nx * self.ny * self.nz
+
+    def zlevels(self):
+        if self.dz0 > 0:
+            return M.make_zlevels(self.nz, 1.0, dz0=self.dz0, stretch_ratio=self.stretch)
+        return M.make_zlevels(self.nz, self.top)
 
 
 @dataclass
@@ -62,9 +93,10 @@ class Request:
     kind: str            # "init" or "steady"
     spec: TileSpec
     arrival: float = 0.0   # seconds after the server's start
-    precision: int = 1     # 0 FP64 V-cycles, 1 FP32 V-cycles under FP64 GMRES
+    precision: int = 2     # 0 FP64; 1 FP32 V-cycles; 2 mixed (FP32 sweeps and V-cycles in FP64 defect correction)
     max_iterations: int = 4000
-    drop_orders: float = 2.0
+    drop_orders: float = 6.0
+    ref_iterations: int = 5
 
 
 @dataclass
@@ -86,23 +118,35 @@ class Result:
         return self.finish - self.arrival
 
 
-def random_tile(rng: np.random.Generator, max_slope: float = 0.3) -> TileSpec:
+# A hill's height at most this fraction of its width L (its steepest slope
+# 0.86 H / L, about 23 degrees): spikes like H = 270 m over L = 110 m leave
+# no positive friction velocity in the first guess on 20 m cells.
+MAX_ASPECT = 0.5
+
+
+def random_tile(rng: np.random.Generator) -> TileSpec:
     """The batched/served tile distribution: nx, ny uniform in 32..128, nz in
-    32..80, 1-5 Gaussian hills with H in 20-300 m and L in 100-600 m (H at
-    most max_slope L, drawn again otherwise: steeper hills make the steady
-    iteration diverge on these meshes), a random wind direction in 0-360
-    degrees."""
+    32..80, dx = dy = 20 m, levels from dz1 = 1 m growing by 1.08, 1-5
+    Gaussian hills with H uniform in 20-300 m (clipped to MAX_ASPECT L) and
+    L in 100-600 m at uniform centres, a wind direction uniform in 0-360
+    degrees. A tile's top is
+    (1.08^nz - 1) / 0.08 m (134 m at 32 levels, 5.9 km at 80): when the
+    hills' summed heights exceed half of it they are scaled down together to
+    that, so the terrain-following levels stay ordered."""
     nx, ny = int(rng.integers(32, 129)), int(rng.integers(32, 129))
     nz = int(rng.integers(32, 81))
-    dx = 50.0
+    dx = 20.0
     hills = []
     for _ in range(int(rng.integers(1, 6))):
-        while True:
-            H, L = float(rng.uniform(20.0, 300.0)), float(rng.uniform(100.0, 600.0))
-            if H <= max_slope * L:
-                break
-        hills.append((float(rng.uniform(0, nx * dx)), float(rng.uniform(0, ny * dx)), H, L))
-    return TileSpec(nx, ny, nz, dx, hills, float(rng.uniform(0.0, 360.0)))
+        H, L = float(rng.uniform(20.0, 300.0)), float(rng.uniform(100.0, 600.0))
+        H = min(H, MAX_ASPECT * L)
+        hills.append([float(rng.uniform(0, nx * dx)), float(rng.uniform(0, ny * dx)), H, L])
+    spec = TileSpec(nx, ny, nz, dx, [], float(rng.uniform(0.0, 360.0)))
+    top = float(spec.zlevels()[-1])
+    total = sum(h[2] for h in hills)
+    scale = min(1.0, 0.5 * top / total)
+    spec.hills = [(xc, yc, H * scale, L) for xc, yc, H, L in hills]
+    return spec
 
 
 def hill_tile(rng: np.random.Generator) -> TileSpec:
@@ -111,44 +155,26 @@ def hill_tile(rng: np.random.Generator) -> TileSpec:
     top, one Gaussian hill (100 m high, 500 m wide) within the middle half."""
     lx = 64 * 80.0
     xc, yc = float(rng.uniform(0.25 * lx, 0.75 * lx)), float(rng.uniform(0.25 * lx, 0.75 * lx))
-    return TileSpec(64, 64, 20, 80.0, [(xc, yc, 100.0, 500.0)], float(rng.uniform(0.0, 360.0)), top=800.0)
+    return TileSpec(64, 64, 20, 80.0, [(xc, yc, 100.0, 500.0)], float(rng.uniform(0.0, 360.0)), top=800.0, dz0=0.0)
 
 
-def tile_deck(spec: TileSpec, kind: str, max_iterations: int = 4000, sweeps: int = 16, drop_orders: float = 2.0,
-              ref_iterations: int = 100) -> D.Deck:
-    """The deck of one request: periodic sides, the surface layer (z0 = 0.1
-    m), a slip top with 0.03 K/m over the sounding (300 K to 700 m), the
-    geostrophic wind of 10 m/s from the request's direction at latitude 90,
-    Rayleigh damping of w and theta above 0.6875 of the top (550 m of 800 m
-    in the reference's 3-D hill deck), the closure with a k
-    floor of 1e-6, sixteen z-line sweeps, CFL 1 -> 4 (the settings of the
-    reference's Neutral_Hill_3D deck); the first guess the log law (u* = 0.35
-    m/s) along the wind made divergence-free; steady at a drop of
-    drop_orders below the residual reference taken over the first
-    ref_iterations iterations (the 3-D hill deck's criterion: 2 orders over
-    100; residual_tol 5e-5, Source/SteadyCriterion.H:16-32)."""
-    lx, ly = spec.nx * spec.dx, spec.ny * spec.dx
-    zl = M.make_zlevels(spec.nz, spec.top, dz0=spec.dz0)
-    top = float(zl[-1])
-    lo, hi = (0.0, 0.0, 0.0), (lx, ly, top)
-    geom = M.geometry((spec.nx, spec.ny, spec.nz), lo, hi, (1, 1))
+def tile_deck(spec: TileSpec, kind: str, max_iterations: int = 4000, sweeps: int = 16, drop_orders: float = 6.0,
+              ref_iterations: int = 5) -> D.Deck:
+    """The deck of one request: the open-channel neutral ABL of the
+    benchmark configurations (decks.open_abl) over the tile's Gaussian hills
+    -- the log-law inflow (u* = 0.4 m/s, z0 = 0.03 m) from the request's
+    direction on the side the wind enters, outflow opposite, the other pair
+    periodic, the surface layer, a slip top, Coriolis (f = 1e-4 1/s), the
+    k-L closure -- with the solver settings of the reference's 3-D hill
+    deck (sixteen z-line sweeps, CFL 1 -> 4, rotation_relax 1); steady at a
+    drop of drop_orders below the residual reference of the first
+    ref_iterations iterations (residual_tol 5e-5, Source/SteadyCriterion.H:
+    16-32). The first guess is the inflow profile made divergence-free."""
     terrain = M.Terrain(kind="hills", hills=list(spec.hills)) if spec.hills else M.Terrain(kind="flat")
-    tile = M.tile_frame(geom, lo, hi, zl, terrain)   # the mesh is built on the device
-    a = math.radians(spec.direction)
-    sounding = dict(theta_ref_z=(0.0, 700.0, 1000.0), theta_ref=(300.0, 300.0, 309.0))
-    prof = GI.InflowProfile(kind=GI.LOG_LAW, ustar=0.35, z0=0.1, direction=spec.direction, **sounding)
-    damping = GP.Damping(rayleigh_zdamp=0.6875 * top, rayleigh_dampcoef=0.2, fields=("w", "theta"), ztop=top,
-                         prob_lo=lo[:2], prob_hi=hi[:2])
-    inputs = GP.ResidualInputs(closure=GP.ClosureOptions(tke_floor=1e-6), surface=GP.SurfaceLayerOptions(z0=0.1),
-                               forcing=GP.ForcingOptions(coriolis=True, coriolis_3d=False, latitude=90.0,
-                                                         rotation_period=125663.7061435917, geostrophic=True,
-                                                         ug=10.0 * math.cos(a), vg=10.0 * math.sin(a)),
-                               zhi_theta_grad=0.03, damping=damping, **sounding)
-    height = RD.residual_scale_height(tile.zlevels, 0.0, top)
-    return D.Deck(f"tile{spec.nx}x{spec.ny}x{spec.nz}", tile, inputs, prof, GI.InitOptions(first_guess=1),
-                  GP.SimplecOptions(sweeps=sweeps, tke_floor=1e-6, height=height, rotation_relax=1.0),
-                  GP.DriverOptions(max_iterations=max_iterations, cfl0=1.0, cfl_max=4.0,
-                                   residual_drop_orders=drop_orders, residual_ref_iterations=ref_iterations), 0)
+    return D.open_abl(f"tile{spec.nx}x{spec.ny}x{spec.nz}", (spec.nx, spec.ny, spec.nz),
+                      (spec.nx * spec.dx, spec.ny * spec.dx), spec.dz0, terrain, sweeps=sweeps, cfl_max=4.0,
+                      drop_orders=drop_orders, max_iterations=max_iterations, rotation_relax=1.0,
+                      ref_iterations=ref_iterations, direction=spec.direction, zlevels=spec.zlevels())
 
 
 def run_request(req: Request, keep_fields: bool = False, stream=None) -> Result:
@@ -160,8 +186,8 @@ def run_request(req: Request, keep_fields: bool = False, stream=None) -> Result:
     res = Result(req.rid, req.kind, req.spec.cells, req.arrival, start, start)
     try:
         with torch.cuda.stream(st):
-            deck = D.prepare(tile_deck(req.spec, req.kind, req.max_iterations, drop_orders=req.drop_orders),
-                              req.precision)
+            deck = D.prepare(tile_deck(req.spec, req.kind, req.max_iterations, drop_orders=req.drop_orders,
+                                       ref_iterations=req.ref_iterations), req.precision)
             if req.kind == "steady":
                 r = deck.solve()
                 res.iterations, res.status = int(r.iterations), int(r.status)
@@ -176,15 +202,27 @@ def run_request(req: Request, keep_fields: bool = False, stream=None) -> Result:
 
 class Server:
     """Serves requests with `slots` concurrent tiles at most, each admitted
-    when its HBM estimate (cells x BYTES_PER_CELL) fits what the running
+    when its HBM estimate (cells x bytes_per_cell) fits what the running
     ones leave of `hbm_budget` bytes. Requests are taken in arrival order; a
     request that does not fit waits (later, smaller ones do not overtake
     it)."""
 
-    def __init__(self, slots: int = 8, hbm_budget: float = 64e9):
+    def __init__(self, slots: int = 8, hbm_budget: float = 64e9, bytes_per_cell: float | None = None):
         if slots < 1:
             raise ValueError("slots must be >= 1")
         self.slots, self.hbm_budget = int(slots), float(hbm_budget)
+        self.bytes_per_cell = float(bytes_per_cell) if bytes_per_cell else BYTES_PER_CELL
+
+    @classmethod
+    def measured(cls, slots: int = 8, budget_fraction: float = 0.75, margin: float = 1.15) -> "Server":
+        """A server whose budget is budget_fraction of the free HBM and whose
+        admission charges each tile the measured bytes per cell (times a
+        margin for the scratch that grows with the largest tile a stream
+        serves)."""
+        import torch
+        bpc = measure_bytes_per_cell() * margin
+        free, _ = torch.cuda.mem_get_info()
+        return cls(slots, budget_fraction * free, bpc)
 
     def run(self, requests: list[Request], keep_fields: bool = False) -> tuple[list[Result], float]:
         """Serve the requests as they arrive (Request.arrival seconds after
@@ -192,7 +230,7 @@ class Server:
         from the call to the last result."""
         import torch
         for r in requests:
-            if r.spec.cells * BYTES_PER_CELL > self.hbm_budget:
+            if r.spec.cells * self.bytes_per_cell > self.hbm_budget:
                 raise ValueError(f"request {r.rid} needs more than the whole HBM budget")
         order = sorted(range(len(requests)), key=lambda i: (requests[i].arrival, i))
         lock = threading.Condition()
@@ -218,7 +256,7 @@ class Server:
                             return
                         i = order[state["next"]]
                         req = requests[i]
-                        need = req.spec.cells * BYTES_PER_CELL
+                        need = req.spec.cells * self.bytes_per_cell
                         wait = t0 + req.arrival - time.perf_counter()
                         if wait <= 0 and state["in_use"] + need <= self.hbm_budget:
                             state["next"] += 1

@@ -1,35 +1,31 @@
 #!/usr/bin/env python3
 """Benchmark of the SIMPLEC hot path on one MI355X.
 
-Headline (value): steady solves per second at a 6-order residual drop, FP64,
-on the reference's Neutral_Column deck (Tests/Canonical_RANS/Neutral_Column/
-inputs_steady_column: 4 x 4 x 64 cells over 2560 x 2560 x 1000 m, periodic
-sides, surface layer z0 = 0.1 m, slip top, the driving force F = 1.225e-4
-m/s^2, the uncapped closure with the logarithmic face mean, two z-line
-sweeps, CFL ramp to 50, steady at residual_tol 1e-8 and a 6-order drop; the
-reference measured 2231 iterations). One step = one whole steady solve from
-the deck's first guess (the log law with u* = 0.25 m/s and its equilibrium
-k), the driver loop on the device (gust_steady_solve); the state is resident
-in HBM before the timed region. The mixed form (FP32 multigrid V-cycles
-under the FP64 GMRES of every pressure correction) is timed the same way.
+Headline (value): steady solves per second at a 6-order residual drop on the
+Gaussian-hill configuration (h = 100 exp(-(r/250 m)^2) m in 4 x 4 km,
+200 x 200 x 64 cells at dx = 20 m, 64 levels from dz1 = 1 m growing by 1.08;
+the open-channel neutral ABL: log-law inflow, outflow, y periodic, surface
+layer, slip top, Coriolis, the k-L closure) in the mixed-precision form
+(FP32 z-line sweeps and FP32 multigrid V-cycles inside the FP64 defect
+correction: the residual, GMRES and the coarsest solve stay FP64), with the
+solver settings of the reference's 3-D hill deck (16 sweeps, CFL 1 -> 4,
+rotation_relax 1). One step = one SIMPLEC iteration of that solve (the
+per-iteration hot path on the 2.56 M-cell tile): `warmup` driver iterations
+from the first guess, then `steps` iterations from the first guess timed
+between synchronisations (ms_per_step); value = 1 / the wall time of one
+whole steady solve from the first guess (the library's driver loop, state
+resident in HBM), measured in the same run. The plain FP64 path is timed the
+same way (extra.c2_fp64), and the per-field relative L2 of the mixed
+solution against the FP64 one is reported (extra.c2_field_error).
 
-Also measured (extra): the reference's own steady decks with published CPU
-times (Askervein, Neutral_ABL_Flat, Neutral_Hill_3D: iterations to the
-verdict and wall time, beside the reference's); one SIMPLEC iteration on the
-Gaussian-hill tile
-(200 x 200 x 64, dz1 = 1 m, FP64 and mixed); the z-line Jacobi sweep of the
-five variables on the fractal-terrain tile (256 x 256 x 96, 65,536 columns;
-the roofline object: algorithmic bytes per launch / launch time from HIP
-events on the launch stream, 10 reals per unknown); the V(2,2) cycle.
-Serving (extra.serving, extra.serving_p99_s): 14 requests arriving at 20 per
-second (mass-consistent winds on random terrain tiles, every seventh a
-converged steady field on a tile of the reference's 3-D hill deck) served
-with 8 concurrent streams in mixed precision, beside the same requests one
-at a time in FP64; p50/p99 latency from arrival to result. Batched
-(extra.batched): 64 random tiles at once, 8 streams vs one.
-cpu_baseline: the oracle's SIMPLEC iteration composite (FP64 C pieces and an
-exact sparse pressure solve, one host thread) on the same column deck for a
-bounded number of iterations, scaled by the deck's 2231 iterations.
+Also measured (extra): the reference's Neutral_Column deck (the previous
+headline: 4 x 4 x 64, 2231 iterations) FP64 and mixed; the reference's own
+steady decks with published CPU times (Askervein, Neutral_ABL_Flat,
+Neutral_Hill_3D); the z-line Jacobi sweep of the five variables on the
+fractal-terrain tile (256 x 256 x 96, 65,536 columns; the roofline object:
+algorithmic bytes per launch / launch time from HIP events on the launch
+stream, 10 reals per unknown); the V(2,2) cycle. Serving and batched legs
+(extra.serving, extra.batched). cpu_baseline: the oracle on the host cores.
 """
 from __future__ import annotations
 
@@ -150,15 +146,9 @@ def _cpu_baseline(target_s=12.0):
 
 
 def _fractal_tile():
-    """The fractal-terrain tile: 256 x 256 x 96 cells at dx = 25 m, a seeded
-    k^-2 random surface (rms 80 m, clipped to 0..400 m), first layer 2 m with
-    the ratio solved so the top is 2000 m, BTF."""
-    from ablsteadyrans_amd import mesh as M
-    lo, hi = (0.0, 0.0, 0.0), (NX * 25.0, NY * 25.0, 2000.0)
-    geom = M.geometry((NX, NY, NZ), lo, hi)
-    zl = M.make_zlevels(NZ, 2000.0, dz0=2.0, stretch_ratio=0.0)
-    h = M.fractal_heights(NX, NY, seed=20264203, rms=80.0, slope=-2.0, clip=(0.0, 400.0))
-    return M.build_mesh(geom, lo, hi, zl, M.Terrain(kind="fractal", heights=h))
+    """The fractal-terrain tile (decks.fractal_tile_256)."""
+    from ablsteadyrans_amd import decks as D
+    return D.fractal_tile_256()
 
 
 def _time_vcycle(tile, precision, reps):
@@ -223,6 +213,16 @@ def _vcycle_bytes(s, pre=2, post=2, coarse=16):
 COLUMN_ITERATIONS_REF = 2231   # Tests/Canonical_RANS/Neutral_Column/README.md:26
 
 
+def _vcycle_alg_bytes(s, pre=2, post=2):
+    """The V-cycle's compulsory bytes with x/y semi-coarsening (levels
+    Nc / 4^l): (4/3) Nc [(pre + post) 8 s + 8 s + 3.5 s] -- per smoothing
+    pass the iterate read and written, rhs and the five mesh arrays; the
+    residual pass; the transfers. 2.92 GB FP64 / 1.46 GB FP32 for V(2,2) on
+    256 x 256 x 96."""
+    nc = NX * NY * NZ
+    return 4.0 / 3.0 * nc * ((pre + post) * 8 * s + 8 * s + 3.5 * s)
+
+
 def _askervein_grid():
     """extern_erf/askervein.txt (the reference's terrain grid), as the test
     fixture holds it."""
@@ -249,6 +249,68 @@ def _time_solves(precision, steps, warmup):
             times.append(time.perf_counter() - t0)
         last = r
     return sum(times) / len(times), int(last.iterations), int(last.status), float(last.mon[2]), times
+
+
+# The Gaussian-hill benchmark's solver settings: those of the reference's
+# 3-D hill deck (Tests/Canonical_RANS/Neutral_Hill_3D/inputs_steady_hill3d:
+# 16 z-line sweeps, CFL 1 -> 4, rotation_relax 1). With 2 sweeps and CFL 10
+# the hill's wake stalls the relative residual near 0.4.
+C2_SETTINGS = dict(sweeps=16, cfl_max=4.0, rotation_relax=1.0)
+C2_WORKLOAD = ("steady solve to a 6-order residual drop (residual_tol 5e-5, monitors flat over 50) of the "
+               "Gaussian hill h = 100 exp(-(r/250 m)^2) m centred in 4 x 4 km, 200x200x64 cells (dx = dy = 20 m, "
+               "dz1 = 1 m growing by 1.08, top 1709.5 m, BTF); neutral ABL: log-law inflow (u* 0.4 m/s, z0 "
+               "0.03 m) at xlo, outflow at xhi, y periodic, surface layer z0 0.03 m, slip top, Coriolis f = "
+               "1e-4 1/s, k-L closure; 16 z-line Jacobi sweeps, CFL 1 -> 4, rotation_relax 1, pressure reltol "
+               "1e-3 (GMRES + x/y semi-coarsening multigrid)")
+
+
+def _c2_run(precision, steps, warmup):
+    """The Gaussian-hill steady solve at one precision (0 FP64; 2 mixed:
+    FP32 sweeps and V-cycles inside the FP64 defect correction): warmup
+    driver iterations from the first guess, then `steps` timed iterations
+    from the first guess (ms per step), then one whole steady solve from
+    the first guess timed end to end (the driver loop in the library).
+    Returns a dict and the converged fields (host arrays)."""
+    import torch
+    from ablsteadyrans_amd import decks as D
+    from ablsteadyrans_amd import physics as GP
+    deck = D.prepare(D.gaussian_hill_200(**C2_SETTINGS), precision)
+    full = deck.driver.max_iterations
+    if warmup > 0:
+        deck.driver.max_iterations = warmup
+        deck.solve()
+        D.reinitialize(deck)
+    deck.driver.max_iterations = steps
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    deck.solve()
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t0) * 1e3 / steps
+    D.reinitialize(deck)
+    deck.driver.max_iterations = full
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = deck.solve()
+    torch.cuda.synchronize()
+    solve_s = time.perf_counter() - t0
+    fields = {n: deck.state[n].numpy()[0, 2:-2, 2:-2, 2:-2].copy() for n in GP.FIELDS}
+    out = {"iterations": int(r.iterations), "status": int(r.status), "seconds_per_solve": round(solve_s, 3),
+           "steady_solves_per_s": round(1.0 / solve_s, 5), "ms_per_step": round(ms_step, 3),
+           "ms_per_iteration_whole_solve": round(solve_s * 1e3 / max(int(r.iterations), 1), 3),
+           "relative_residual": float(r.relative_residual), "mean_ustar": round(float(r.mon[2]), 6)}
+    del deck
+    torch.cuda.empty_cache()
+    return out, fields
+
+
+def _field_errors(a, b):
+    """Per-field relative L2 of a against b (valid points)."""
+    import numpy as np
+    out = {}
+    for n in a:
+        den = float(np.linalg.norm(b[n]))
+        out[n] = float(np.linalg.norm(a[n] - b[n]) / den) if den > 0 else float(np.linalg.norm(a[n]))
+    return out
 
 
 def _time_reference_decks():
@@ -446,12 +508,13 @@ def _traffic_from_profile():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3, help="timed steady solves per precision")
-    ap.add_argument("--warmup", type=int, default=1, help="untimed steady solves per precision")
+    ap.add_argument("--steps", type=int, default=20,
+                    help="timed SIMPLEC iterations of the Gaussian-hill steady solve per precision")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed iterations before them")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving and batched legs")
     ap.add_argument("--profile-only", action="store_true", help="sweeps only (for rocprofv3 runs)")
-    ap.add_argument("--profile-deck", choices=("column", "askervein", "abl", "hill3d", "hill200"),
+    ap.add_argument("--profile-deck", choices=("column", "askervein", "abl", "hill3d", "hill200", "c2", "c2mixed"),
                     help="only N driver iterations of one reference deck (for rocprofv3 runs)")
     ap.add_argument("--iterations", type=int, default=100, help="driver iterations of --profile-deck")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -481,6 +544,21 @@ def main():
         print(json.dumps({"profile_deck": "gaussian_hill_200x200x64", "iterations": args.iterations,
                           "ms_per_iteration": round(ms, 4)}))
         return
+    if args.profile_deck in ("c2", "c2mixed"):
+        from ablsteadyrans_amd import decks as D
+        deck = D.prepare(D.gaussian_hill_200(**C2_SETTINGS), 2 if args.profile_deck == "c2mixed" else 0)
+        deck.driver.max_iterations = 10
+        deck.solve()
+        D.reinitialize(deck)
+        deck.driver.max_iterations = args.iterations
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = deck.solve()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        print(json.dumps({"profile_deck": deck.name, "precision": args.profile_deck, "iterations": int(r.iterations),
+                          "ms_per_iteration": round(dt * 1e3 / max(int(r.iterations), 1), 4)}))
+        return
     if args.profile_deck:
         print(json.dumps(_profile_deck(args.profile_deck, args.iterations)))
         return
@@ -498,15 +576,17 @@ def main():
         print(json.dumps({"profile_only": True, "ms_per_sweep_fp64": ms_ev, "vcycle_ms": [v64, v32]}))
         return
 
-    # 1. the headline: whole steady solves of the column deck, FP64 and mixed
-    print("bench: steady solves of the Neutral_Column deck (FP64)", file=sys.stderr, flush=True)
-    s64, it64, st64, us64, t64 = _time_solves(0, args.steps, args.warmup)
-    print("bench: steady solves of the Neutral_Column deck (mixed)", file=sys.stderr, flush=True)
-    s32, it32, st32, us32, t32 = _time_solves(1, args.steps, args.warmup)
-    # 2. one SIMPLEC iteration on the hill tile
-    print("bench: SIMPLEC iterations on the 200x200x64 hill tile", file=sys.stderr, flush=True)
-    hill64 = _time_hill_iterations(0, 10)
-    hill32 = _time_hill_iterations(1, 10)
+    # 1. the headline: the Gaussian-hill steady solve, mixed and FP64
+    print("bench: steady solve of the 200x200x64 Gaussian hill (mixed)", file=sys.stderr, flush=True)
+    c2m, fm = _c2_run(2, args.steps, args.warmup)
+    print("bench: steady solve of the 200x200x64 Gaussian hill (FP64)", file=sys.stderr, flush=True)
+    c2d, fd = _c2_run(0, args.steps, args.warmup)
+    errs = _field_errors(fm, fd)
+    del fm, fd
+    # 2. the reference's column deck, whole solves (the previous headline)
+    print("bench: steady solves of the Neutral_Column deck (FP64, mixed)", file=sys.stderr, flush=True)
+    s64, it64, st64, us64, t64 = _time_solves(0, 3, 1)
+    s32, it32, st32, us32, t32 = _time_solves(2, 3, 1)
     # 3. the reference's own steady decks with published CPU times
     print("bench: the reference's steady decks (Askervein, Neutral_ABL_Flat, Neutral_Hill_3D)", file=sys.stderr,
           flush=True)
@@ -538,46 +618,54 @@ def main():
         batched = _batched_leg()
     cpu = cpu_sweep = None
     if not args.no_cpu:
-        print("bench: CPU baselines (oracle, one thread)", file=sys.stderr, flush=True)
+        print("bench: CPU baselines (oracle)", file=sys.stderr, flush=True)
         cpu = _cpu_column_baseline()
         cpu_sweep = _cpu_baseline()
-    value = 1.0 / s64
+    value = c2m["steady_solves_per_s"]
     line = {
         "metric": "steady solves/s at 6-order residual drop, mixed vs FP64; sweep HBM GB/s; p99",
-        "value": round(value, 4),
+        "value": value,
         "unit": "steady solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(s64 * 1e3, 2),
+        "ms_per_step": c2m["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic: the reference's Neutral_Column deck (its first guess), resident in HBM",
-        "config": {"workload": "steady solve to a 6-order residual drop (residual_tol 1e-8) of the reference's "
-                               "Neutral_Column deck: 4x4x64 cells, 2560x2560x1000 m, periodic sides, surface "
-                               "layer z0 0.1 m, F = 1.225e-4 m/s^2, 2 z-line sweeps, CFL 1 -> 50",
-                   "iterations": it64, "iterations_reference": COLUMN_ITERATIONS_REF, "status": st64,
-                   "ustar": round(us64, 8), "seconds_per_solve": [round(t, 3) for t in t64]},
+        "dtype": "f32 sweeps and V-cycles in f64 defect correction",
+        "data": "synthetic: the analytic Gaussian-hill terrain and the log-law first guess made divergence-free, "
+                "resident in HBM",
+        "config": {"workload": C2_WORKLOAD, "cells": [200, 200, 64], "precision": "mixed",
+                   "step": "one SIMPLEC iteration (residual, momentum predictor with 16 sweeps, pressure "
+                           "correction, theta and k sweeps)",
+                   "iterations_to_steady": c2m["iterations"], "status": c2m["status"],
+                   "seconds_per_solve": c2m["seconds_per_solve"],
+                   "ms_per_iteration_whole_solve": c2m["ms_per_iteration_whole_solve"]},
         "roofline": {"bound": "hbm", "achieved": round(gbs64, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": _traffic_from_profile(),
                      "kernel": "zjacobi_sweep_kernel<double,false,false> (the metric's sweep HBM GB/s: 5 "
                                "variables on 256x256x96, one launch)",
                      "algorithmic_bytes_per_launch": bytes64, "avg_launch_ms": round(ms_ev, 5)},
         "cpu_baseline": cpu,
-        "extra": {"mixed_steady_solves_per_s": round(1.0 / s32, 4), "mixed_iterations": it32, "mixed_status": st32,
-                  "mixed_ustar": round(us32, 8), "mixed_vs_fp64": round(s64 / s32, 3),
-                  "ms_per_iteration_column_fp64": round(s64 * 1e3 / max(it64, 1), 3),
-                  "hill_200x200x64_ms_per_iteration_fp64": round(hill64, 3),
-                  "hill_200x200x64_ms_per_iteration_mixed": round(hill32, 3),
+        "extra": {"c2_mixed": c2m, "c2_fp64": c2d,
+                  "mixed_vs_fp64": round(c2m["steady_solves_per_s"] / c2d["steady_solves_per_s"], 3),
+                  "c2_field_error_mixed_vs_fp64_rel_l2": {k: float(f"{v:.3e}") for k, v in errs.items()},
+                  "column_fp64_steady_solves_per_s": round(1.0 / s64, 4), "column_fp64_iterations": it64,
+                  "column_fp64_status": st64, "column_ustar": round(us64, 8),
+                  "column_mixed_steady_solves_per_s": round(1.0 / s32, 4), "column_mixed_iterations": it32,
+                  "column_ms_per_iteration_fp64": round(s64 * 1e3 / max(it64, 1), 3),
                   "sweep_fp64_gbs": round(gbs64, 2), "sweep_fp64_ms": round(ms_ev, 5),
                   "sweep_fp32_gbs": round(gbs32, 2), "sweep_fp32_ms": round(ms32, 5),
                   "sweep_fp32_frac": round(gbs32 / HBM_PEAK_GBS, 4),
                   "fp64_line_solve_2_sweeps_ms": round(run64, 4), "fp32_line_solve_2_sweeps_ms": round(run32, 4),
                   "vcycle_levels": levels,
-                  "fp64_vcycle_ms": round(vc64, 4), "fp64_vcycle_gbs": round(_vcycle_bytes(8) / (vc64 * 1e-3) / 1e9, 1),
-                  "fp32_vcycle_ms": round(vc32, 4), "fp32_vcycle_gbs": round(_vcycle_bytes(4) / (vc32 * 1e-3) / 1e9, 1),
+                  "fp64_vcycle_ms": round(vc64, 4), "fp32_vcycle_ms": round(vc32, 4),
+                  "fp64_vcycle_gbs_algorithmic": round(_vcycle_alg_bytes(8) / (vc64 * 1e-3) / 1e9, 1),
+                  "fp32_vcycle_gbs_algorithmic": round(_vcycle_alg_bytes(4) / (vc32 * 1e-3) / 1e9, 1),
+                  "fp64_vcycle_frac_algorithmic": round(_vcycle_alg_bytes(8) / (vc64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                  "fp64_vcycle_gbs_moved": round(_vcycle_bytes(8) / (vc64 * 1e-3) / 1e9, 1),
+                  "fp32_vcycle_gbs_moved": round(_vcycle_bytes(4) / (vc32 * 1e-3) / 1e9, 1),
                   "reference_decks": decks,
                   "serving": serving,
                   "serving_p99_s": serving["served_mixed_8_slots"]["all"]["p99_s"] if serving else None,

@@ -46,15 +46,23 @@ def periodic_images(a, per, nodal):
     return a
 
 
-def pressure_correction(mesh, u, v, w, p, d, bc, alpha_p, source=None):
+def pressure_correction(mesh, u, v, w, p, d, bc, alpha_p, source=None, solve=None):
     """u, v, w, p: one-ghost (1, ...) arrays (begin -1); d = (d_u, d_v, d_w)
-    valid-face arrays. Returns (u, v, w, p, phi, F, div_before, div_after)."""
+    valid-face arrays. Returns (u, v, w, p, phi, F, div_before, div_after).
+    solve(rhs, coeffs, bc) -> phi, when given, replaces the sparse direct
+    solve (for tiles too large to factor); it must return the solution of
+    the oracle operator's system with the J-weighted mean removed in the
+    singular case (tests/config_helpers.py: refinement on the oracle
+    operator)."""
     g = PH.ogeom(mesh)
     per = mesh.geom.periodic
     betas, keeps = zip(*(beta_keep(x) for x in d))
     div0 = orc.velocity_divergence(u, v, w, B, mesh.z_nd, B, mesh.ax, mesh.ay, mesh.az, B, mesh.detJ, B, g)
     rhs = div0 - (source if source is not None else 0.0)
     coeffs = (_embed(mesh.ax, betas[0]), _embed(mesh.ay, betas[1]), _embed(mesh.az, betas[2]))
+    if solve is not None:
+        phi = solve(rhs, coeffs, bc)
+        return _finish_correction(mesh, u, v, w, p, bc, alpha_p, phi, coeffs, keeps, div0, g, per)
     A = PH.sparse_operator(mesh, bc, coeffs=coeffs).tocsr()
     J = mesh.detJ[1:-1, 1:-1, 1:-1]
     singular = all(b != orc.DIRICHLET for b in bc)
@@ -69,6 +77,11 @@ def pressure_correction(mesh, u, v, w, p, d, bc, alpha_p, source=None):
     phi = spla.spsolve(A.tocsc(), r).reshape(rhs.shape)
     if singular:
         phi -= np.sum(J * phi) / np.sum(J)
+    return _finish_correction(mesh, u, v, w, p, bc, alpha_p, phi, coeffs, keeps, div0, g, per)
+
+
+def _finish_correction(mesh, u, v, w, p, bc, alpha_p, phi, coeffs, keeps, div0, g, per):
+    """The fluxes of phi and the updates of p, u, v, Omega and w (Source/Simplec.cpp:180-215)."""
     F = orc.poisson_fluxes(np.pad(phi, 1)[None].copy(), B, mesh.z_nd, B, g, bc, coeffs=coeffs, coeff_begin=B)
     p = p.copy()
     p[0, 1:-1, 1:-1, 1:-1] += alpha_p * phi

@@ -50,8 +50,9 @@ def poisson_bc(m, phys: P.Physics):
 
 
 def simplec_iteration(tile, m, q, phys: P.Physics, dtau, sweeps=2, alpha_p=1.0, tke_floor=0.0, relax=0.3,
-                      w_vnn_cap=0.0, margin=4.0):
-    """q (dict of two-ghost arrays) is updated in place; returns the residual R(q)."""
+                      w_vnn_cap=0.0, margin=4.0, solve=None):
+    """q (dict of two-ghost arrays) is updated in place; returns the residual R(q).
+    solve: the pressure step's linear solve (None: sparse direct)."""
     g = m.geom
     nz = g.hi[2] - g.lo[2] + 1
     P.fill_ghosts(q, m, phys)
@@ -73,7 +74,8 @@ def simplec_iteration(tile, m, q, phys: P.Physics, dtau, sweeps=2, alpha_p=1.0, 
         put_valid(q, n, a)
     P.fill_ghosts(q, m, phys)
     bc = poisson_bc(m, phys)
-    u, v, w, p, *_ = SH.pressure_correction(tile, one(q["u"]), one(q["v"]), one(q["w"]), one(q["p"]), ds, bc, alpha_p)
+    u, v, w, p, *_ = SH.pressure_correction(tile, one(q["u"]), one(q["v"]), one(q["w"]), one(q["p"]), ds, bc, alpha_p,
+                                            solve=solve)
     for n, a in zip(("u", "v", "w", "p"), (u, v, w, p)):
         put_valid(q, n, a)
     conv0 = orc.convection(0, q["theta"], B2, F, Fb, m.J, B1, g, phys.theta_limiter, phys.theta_eps)

@@ -1,0 +1,376 @@
+"""The benchmark configurations at their real sizes on the GPU.
+
+* flat ABL 64x64x48 (decks.flat_abl_64): the sweeps of the first iteration
+  against the oracle at 1e-12, ten whole SIMPLEC iterations against the
+  oracle's composite at 1e-8 of each field's scale (the composite's pressure
+  step solves the oracle operator's system by refinement to 1e-11, see
+  tests/config_helpers.py), and 300 iterations with the configuration's own
+  settings (2 sweeps, SER CFL 1 -> 10, pressure reltol 1e-3) whose final
+  state's residual, evaluated by the oracle, equals the stage's.
+* Gaussian hill 200x200x64 (decks.gaussian_hill_200): one iteration against
+  the oracle's composite; the steady solve to a 6-order drop in FP64 and in
+  the mixed form (FP32 sweeps and V-cycles in the FP64 defect correction),
+  each field within 1e-6 relative L2 of the FP64 one, and the mixed field a
+  steady state of the oracle's residual.
+* fractal 256x256x96 (decks.fractal_tile_256): one five-variable FP64 z-line
+  sweep bit-identical to the oracle (31.6 M unknowns); FP64 and FP32 V(2,2)
+  cycles reduce the residual by a stated factor per cycle; a GMRES solve
+  reaches its tolerance and its phi satisfies the oracle operator's system.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from oracle import physics as P
+from tests import config_helpers as CH
+from tests import linesys as L
+from tests import poisson_helpers as PH
+from tests import stage_helpers as STH
+
+pytestmark = pytest.mark.gpu
+
+B1 = (-1, -1, -1)
+
+
+def _dtau_fab(dtau, g):
+    import torch
+    from ablsteadyrans_amd.fab import Fab
+    return Fab(torch.from_numpy(dtau[None].copy()).cuda(), tuple(g.lo))
+
+
+def _close_fields(ds, q, tol, what):
+    from ablsteadyrans_amd import physics as GP
+    for n in GP.FIELDS:
+        a, b = ds[n].numpy()[0, 2:-2, 2:-2, 2:-2], q[n][0, 2:-2, 2:-2, 2:-2]
+        scale = max(float(np.abs(b).max()), 1e-30)
+        err = float(np.abs(a - b).max())
+        assert err <= tol * scale, (what, n, err, scale)
+
+
+def _run_composite(deck, iterations, cfl=2.0, check_each=True, tol=1e-8):
+    """`iterations` stage iterations on the device beside the oracle's
+    composite from the same state, compared after each."""
+    tile = deck.tile
+    m = CH.omesh(tile)
+    phys = CH.open_abl_physics(deck)
+    q = CH.host_state(deck.state)
+    stage = deck.simplec
+    stage.init(deck.state)
+    P.fill_ghosts(q, m, phys)
+    solve = CH.RefinedSolve(tile, deck.mesh)
+    g = m.geom
+    opt = deck.stage
+    for it in range(iterations):
+        q1 = {n: a[:, 1:-1, 1:-1, 1:-1].copy() for n, a in q.items()}
+        dtau = orc.local_dtau(q1["u"], q1["v"], q1["w"], B1, m.z, B1, m.J, B1, g, cfl)
+        STH.simplec_iteration(tile, m, q, phys, dtau, sweeps=opt.sweeps, tke_floor=opt.tke_floor,
+                              relax=opt.rotation_relax, solve=solve)
+        stage.iterate(deck.state, _dtau_fab(dtau, g))
+        if check_each or it == iterations - 1:
+            _close_fields(deck.state, q, tol, f"iteration {it + 1}")
+    print(f"composite pressure solves: oracle residual <= {max(solve.history):.2e} of rhs", flush=True)
+    return q
+
+
+# ---------------------------------------------------------------------------
+# flat ABL 64 x 64 x 48
+
+def test_flat_abl_64_first_iteration_sweeps_match_the_oracle(cuda):
+    """The momentum predictor's and the scalars' z-line systems of the first
+    iteration (assembled by the oracle from the configuration's first guess)
+    swept on the device: 1 and 2 sweeps bit-identical to the oracle."""
+    import torch
+    import ablsteadyrans_amd as G
+    from ablsteadyrans_amd import decks as D
+    deck = D.prepare(D.flat_abl_64())
+    tile = deck.tile
+    m = CH.omesh(tile)
+    phys = CH.open_abl_physics(deck)
+    q = CH.host_state(deck.state)
+    P.fill_ghosts(q, m, phys)
+    R, cf, ss = P.residual(q, m, phys)
+    Fx, Fy, Fz, Fb = orc.transport_fluxes(q["u"], q["v"], q["w"], (-2, -2, -2), m.z, B1, m.ax, m.ay, B1, m.geom)
+    F = [Fx[None], Fy[None], Fz[None]]
+    n = tile.geom.n
+    per = tuple(tile.geom.periodic)
+    rng = np.random.default_rng(20261203)
+    for eq, loc in ((P.EQ_U, 1), (P.EQ_V, 2), (P.EQ_W, 4), (P.EQ_THETA, 0), (P.EQ_TKE, 0)):
+        A = P.implicit_rows(eq, F, Fb, m, phys, cf, ss)
+        A = np.ascontiguousarray(A.reshape((7,) + A.shape[-3:]))
+        # a_P = 1/dtau + A_P with a positive pseudo-time step; rhs from R
+        ap = (A[0] + rng.uniform(0.5, 2.0, size=A.shape[1:]))[None].copy()
+        rhs = -rng.standard_normal(A.shape[1:])[None] * 1e-3
+        L.seam_consistent(ap, loc, per)
+        L.seam_consistent(rhs, loc, per)
+        geom = G.Geometry.of_size(*n, periodic=per)
+        fa = [G.Fab(torch.from_numpy(np.ascontiguousarray(x)).cuda(), (0, 0, 0)) for x in (A, ap, rhs)]
+        for sweeps in (1, 2):
+            d = G.Fab.at_loc(geom, loc, 1, 1)
+            ch = G.zline_jacobi(loc, *fa, sweeps, geom, d)
+            ref, ref_ch, _ = L.oracle_jacobi(n, loc, per, A, ap, rhs, sweeps)
+            got = d.numpy()[0, 1:-1, 1:-1, 1:-1]
+            assert np.array_equal(got, ref), (eq, sweeps, float(np.max(np.abs(got - ref))))
+            assert ch == ref_ch
+
+
+def test_flat_abl_64_ten_iterations_match_the_oracle_composite(cuda):
+    """Ten SIMPLEC iterations (2 sweeps, local dtau at CFL 2, the pressure
+    correction converged: reltol 1e-10 on the device, the oracle operator's
+    system to 1e-11 in the composite): every field within 1e-8 of its scale
+    after every iteration."""
+    from ablsteadyrans_amd import decks as D
+    deck = D.prepare(D.flat_abl_64(pressure_reltol=1e-10))
+    deck.stage.poisson_max_iter = 400
+    _run_composite(deck, 10)
+
+
+def test_flat_abl_64_300_iterations_with_its_settings(cuda):
+    """300 driver iterations with the configuration's own settings (2 sweeps,
+    SER CFL 1 -> 10, pressure reltol 1e-3): the relative residual falls below
+    0.5, and the oracle's residual of the final state gives the stage's norms
+    (1e-9)."""
+    from ablsteadyrans_amd import decks as D
+    from ablsteadyrans_amd import physics as GP
+    deck = D.prepare(D.flat_abl_64())
+    deck.driver.max_iterations = 300
+    r = deck.solve()
+    assert r.iterations == 300 or r.status == 1
+    print(f"flat ABL 64: r_n {r.relative_residual:.3e} after {r.iterations} iterations, res {list(r.res)}",
+          flush=True)
+    assert r.relative_residual < 0.5 and all(np.isfinite(x) for x in r.res)
+    # the state's residual by the oracle, normed as the stage norms it
+    m = CH.omesh(deck.tile)
+    phys = CH.open_abl_physics(deck)
+    q = CH.host_state(deck.state)
+    P.fill_ghosts(q, m, phys)
+    R, _, _ = P.residual(q, m, phys)
+    deck.simplec.iterate(deck.state, _local_dtau(deck))   # norms of R(q_300) come with the next iteration
+    res, _ = deck.simplec.norms()
+    ref = CH.stage_norms_of(R, q, m, deck.stage)
+    for a, b, name in zip(res, ref, GP.FIELDS):
+        assert abs(a - b) <= 1e-9 * max(abs(b), 1e-30), (name, a, b)
+
+
+def _local_dtau(deck, cfl=1.0):
+    from ablsteadyrans_amd import pseudotime as PT
+    from ablsteadyrans_amd.fab import Fab
+    g = deck.tile.geom
+    dt = Fab.at_loc(g, 0, 1, 0)
+    q = deck.state
+    PT.local_dtau(deck.mesh, q["u"], q["v"], q["w"], cfl, dt)
+    return dt
+
+
+# ---------------------------------------------------------------------------
+# Gaussian hill 200 x 200 x 64
+
+def test_gaussian_hill_200_one_iteration_matches_the_oracle_composite(cuda):
+    from ablsteadyrans_amd import decks as D
+    deck = D.prepare(D.gaussian_hill_200(sweeps=16, cfl_max=4.0, rotation_relax=1.0, pressure_reltol=1e-12))
+    _run_composite(deck, 1)
+
+
+def test_gaussian_hill_200_mixed_steady_solve_matches_fp64(cuda):
+    """The steady solve to a 6-order drop with the reference 3-D hill deck's
+    solver settings, FP64 and mixed: both steady, each field of the mixed
+    solution within 1e-6 relative L2 of the FP64 one, and the mixed solution
+    a steady state of the oracle's residual (its norms as small as the
+    criterion's last residuals)."""
+    import time
+    from ablsteadyrans_amd import decks as D
+    from ablsteadyrans_amd import physics as GP
+    out = {}
+    for prec in (0, 2):
+        deck = D.prepare(D.gaussian_hill_200(sweeps=16, cfl_max=4.0, rotation_relax=1.0), prec)
+        t0 = time.perf_counter()
+        r = deck.solve()
+        dt = time.perf_counter() - t0
+        print(f"gaussian hill 200, precision {prec}: status {r.status} after {r.iterations} iterations in "
+              f"{dt:.2f} s, r_n {r.relative_residual:.3e}", flush=True)
+        assert r.status == 1, (prec, r.iterations, list(r.res))
+        out[prec] = ({n: deck.state[n].numpy()[0, 2:-2, 2:-2, 2:-2].copy() for n in GP.FIELDS}, r, deck)
+    fd, fm = out[0][0], out[2][0]
+    for n in GP.FIELDS:
+        e = float(np.linalg.norm(fm[n] - fd[n]) / max(np.linalg.norm(fd[n]), 1e-300))
+        print(f"  {n}: mixed vs FP64 relative L2 {e:.2e}", flush=True)
+        assert e <= 1e-6, (n, e)
+    deck, r = out[2][2], out[2][1]
+    m = CH.omesh(deck.tile)
+    phys = CH.open_abl_physics(deck)
+    q = CH.host_state(deck.state)
+    P.fill_ghosts(q, m, phys)
+    R, _, _ = P.residual(q, m, phys)
+    ref = CH.stage_norms_of(R, q, m, deck.stage)
+    # the stage reports R at the start of its last iteration; the final state's
+    # residual is of the same size
+    for a, b, name in zip(r.res, ref, GP.FIELDS):
+        assert b <= 1.5 * a + 1e-14, (name, a, b)
+
+
+# ---------------------------------------------------------------------------
+# fractal 256 x 256 x 96
+
+FRACTAL_N = (256, 256, 96)
+
+
+def _fractal_systems(seed=20264203):
+    """The five variables' random M-matrix systems at u, v, w, theta, k
+    locations (the recipe of linesys.random_system)."""
+    out = []
+    for q, loc in enumerate((1, 2, 4, 0, 0)):
+        A, ap, rhs = L.random_system(FRACTAL_N, loc, (0, 0), seed=seed + q)
+        out.append((loc, A, ap, rhs))
+    return out
+
+
+def test_fractal_256_five_variable_sweep_is_bit_identical(cuda):
+    """One batched launch per sweep over the five variables (31.6 M
+    unknowns, the streaming schedule), 1 and 2 sweeps, against the oracle."""
+    import torch
+    import ablsteadyrans_amd as G
+    geom = G.Geometry.of_size(*FRACTAL_N, periodic=(0, 0))
+    systems, host = [], []
+    for loc, A, ap, rhs in _fractal_systems():
+        fa = [G.Fab(torch.from_numpy(x).cuda(), (0, 0, 0)) for x in (A, ap, rhs)]
+        systems.append(G.LineSystem(loc, geom, *fa, G.Fab.at_loc(geom, loc, 1, 1)))
+        host.append((loc, A, ap, rhs))
+    for sweeps in (1, 2):
+        changes = torch.zeros(len(systems), dtype=torch.float64, device="cuda")
+        G.zline_jacobi_batched(systems, sweeps, changes)
+        torch.cuda.synchronize()
+        for s, (loc, A, ap, rhs) in zip(systems, host):
+            ref, ref_ch, _ = L.oracle_jacobi(FRACTAL_N, loc, (0, 0), A, ap, rhs, sweeps)
+            got = s.delta.numpy()[0, 1:-1, 1:-1, 1:-1]
+            assert np.array_equal(got, ref), (loc, sweeps, float(np.max(np.abs(got - ref))))
+        assert all(float(c) > 0 for c in changes.cpu())
+
+
+def _fractal_solver(precision):
+    from ablsteadyrans_amd import decks as D
+    from ablsteadyrans_amd import poisson as PS
+    tile = D.fractal_tile_256()
+    dm = PS.DeviceMesh(tile)
+    return tile, dm, PS.PoissonSolver(dm, PS.PoissonOptions(precision=precision, reltol=1e-10, max_iter=100))
+
+
+BC_FRACTAL = (1, 1, 1, 2, 1, 2)   # Neumann sides and ground, Dirichlet (open) x-hi and top
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
+def test_fractal_256_vcycle_convergence_factor(cuda, precision):
+    """One V(2,2) cycle on a random right-hand side reduces the residual at
+    least tenfold (FP64 and FP32 levels). On this rough terrain the
+    rediscretized coarse levels misrepresent the smoothest modes: a second
+    cycle as a stationary iteration amplifies them (the flat twin of the tile
+    contracts by 0.07 per cycle), so the cycle serves as GMRES's
+    preconditioner, whose convergence the next test checks."""
+    import torch
+    from ablsteadyrans_amd.fab import Fab
+    tile, dm, S = _fractal_solver(precision)
+    nx, ny, nz = FRACTAL_N
+    n = nx * ny * nz
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(7)
+    b = torch.rand(n, generator=gen, dtype=torch.float64, device="cuda") - 0.5
+    x = torch.zeros_like(b)
+    phi = Fab(torch.zeros((1, nz + 2, ny + 2, nx + 2), dtype=torch.float64, device="cuda"), (-1, -1, -1))
+    Ax = Fab(torch.zeros((1, nz, ny, nx), dtype=torch.float64, device="cuda"), (0, 0, 0))
+
+    def resid(xv):
+        phi.data[0, 1:-1, 1:-1, 1:-1] = xv.view(nz, ny, nx)
+        S.apply(Ax, phi, None, BC_FRACTAL)
+        return b - Ax.data.view(-1)
+
+    r = b.clone()
+    norms = [float(torch.linalg.norm(r))]
+    for cyc in range(2):
+        z = torch.zeros_like(b)
+        S.vcycle(r.contiguous(), z, None, BC_FRACTAL, setup=(cyc == 0))
+        x += z
+        r = resid(x)
+        norms.append(float(torch.linalg.norm(r)))
+    factors = [norms[i + 1] / norms[i] for i in range(2)]
+    print(f"fractal V(2,2) precision {precision}: residual factors per cycle {factors}", flush=True)
+    assert factors[0] <= 0.1, factors
+    assert all(np.isfinite(factors))
+
+
+def test_fractal_256_gmres_solve_reaches_its_tolerance(cuda):
+    """GMRES with the FP32 V-cycle preconditioner to reltol 1e-10; the
+    solution satisfies the oracle operator's system to 1e-9 of the rhs."""
+    import torch
+    from ablsteadyrans_amd.fab import Fab
+    tile, dm, S = _fractal_solver(1)
+    nx, ny, nz = FRACTAL_N
+    rhs_h = np.random.default_rng(3).uniform(-1, 1, size=(nz, ny, nx))
+    rhs = Fab(torch.from_numpy(rhs_h[None].copy()).cuda(), (0, 0, 0))
+    phi = Fab(torch.zeros((1, nz + 2, ny + 2, nx + 2), dtype=torch.float64, device="cuda"), (-1, -1, -1))
+    res = S.solve(rhs, phi, None, BC_FRACTAL)
+    print(f"fractal GMRES: {res.iterations} iterations, {res.initial_residual:.3e} -> {res.final_residual:.3e}",
+          flush=True)
+    assert res.final_residual <= 1e-10 * res.initial_residual
+    assert res.iterations <= 60
+    ph = phi.numpy()[0, 1:-1, 1:-1, 1:-1]
+    r = rhs_h - PH.apply(tile, ph, BC_FRACTAL)
+    assert np.linalg.norm(r) <= 1e-9 * np.linalg.norm(rhs_h), np.linalg.norm(r) / np.linalg.norm(rhs_h)
+
+
+# ---------------------------------------------------------------------------
+# 64 batched tiles and the served stream of such tiles
+
+def _c4_specs(n, base=20264203):
+    from ablsteadyrans_amd import serving as S
+    return [S.random_tile(np.random.default_rng(base + i)) for i in range(n)]
+
+
+def test_batched_64_tiles_mixed_iterations_alone_and_together(cuda):
+    """64 tiles of the batched distribution (32..128 squared, 32..80 levels
+    from 1 m by 1.08, 1-5 Gaussian hills, random wind direction), each
+    prepared (mesh, mass-consistent first guess) and advanced three mixed-
+    precision SIMPLEC iterations, eight tiles at a time on their own streams:
+    four of them run alone give the same bits, and their FP64 runs agree
+    within 1e-6 relative L2 per field."""
+    from ablsteadyrans_amd import serving as S
+    specs = _c4_specs(64)
+    reqs = [S.Request(i, "steady", sp, max_iterations=3, precision=2) for i, sp in enumerate(specs)]
+    res, wall = S.Server(slots=8).run(reqs, keep_fields=True)
+    cells = sum(sp.cells for sp in specs)
+    print(f"64 tiles ({cells} cells): {wall:.2f} s with 8 streams", flush=True)
+    assert all(r.error is None for r in res), [r.error for r in res if r.error]
+    assert all(r.iterations == 3 for r in res)
+    for i in (0, 17, 42, 63):
+        alone = S.run_request(reqs[i], keep_fields=True)
+        for n, a in alone.fields.items():
+            assert np.array_equal(a, res[i].fields[n]), (i, n)
+        fp64 = S.run_request(S.Request(i, "steady", specs[i], max_iterations=3, precision=0), keep_fields=True)
+        for n, a in fp64.fields.items():
+            v = a[0, 2:-2, 2:-2, 2:-2]
+            e = float(np.linalg.norm(res[i].fields[n][0, 2:-2, 2:-2, 2:-2] - v) / max(np.linalg.norm(v), 1e-300))
+            assert e <= 1e-6, (i, n, e)
+
+
+def test_served_stream_of_batched_tiles_alone_equals_the_mix(cuda):
+    """Requests on tiles of the batched distribution arriving at 20 per
+    second (half mass-consistent winds, half steady solves cut at 30
+    iterations), served with 4 slots: each result is bitwise the result of
+    the same request alone."""
+    from ablsteadyrans_amd import serving as S
+    specs = _c4_specs(6, base=20265203)
+    arr = S.poisson_arrivals(6, 20.0, np.random.default_rng(20265203))
+    reqs = [S.Request(i, "init" if i % 2 else "steady", sp, arrival=float(a), max_iterations=30)
+            for i, (sp, a) in enumerate(zip(specs, arr))]
+    mixed, wall = S.Server(slots=4).run(reqs, keep_fields=True)
+    for r in reqs:
+        r.arrival = 0.0
+    for i, r in enumerate(reqs):
+        a = S.run_request(r, keep_fields=True)
+        m = mixed[i]
+        assert a.error is None and m.error is None, (a.error, m.error)
+        assert a.iterations == m.iterations
+        for n in a.fields:
+            assert np.array_equal(a.fields[n], m.fields[n]), (i, n)
+    s = S.summary(mixed, wall)
+    print(f"served: {s}", flush=True)
+    assert s["errors"] == 0

@@ -10,10 +10,10 @@ pytestmark = pytest.mark.gpu
 
 
 def _requests():
-    specs = [S.TileSpec(24, 20, 16, hills=[(600.0, 500.0, 60.0, 300.0)], direction=30.0),
-             S.TileSpec(16, 28, 20, hills=[(400.0, 700.0, 40.0, 250.0), (200.0, 300.0, 30.0, 200.0)],
-                        direction=200.0),
-             S.TileSpec(20, 20, 24, hills=[(500.0, 500.0, 80.0, 400.0)], direction=310.0)]
+    specs = [S.TileSpec(24, 20, 16, 50.0, hills=[(600.0, 500.0, 60.0, 300.0)], direction=30.0, dz0=0.0),
+             S.TileSpec(16, 28, 20, 50.0, hills=[(400.0, 700.0, 40.0, 250.0), (200.0, 300.0, 30.0, 200.0)],
+                        direction=200.0, dz0=0.0),
+             S.TileSpec(20, 20, 24, 50.0, hills=[(500.0, 500.0, 80.0, 400.0)], direction=310.0, dz0=0.0)]
     reqs = [S.Request(0, "steady", specs[0], max_iterations=25),
             S.Request(1, "init", specs[1]),
             S.Request(2, "steady", specs[2], max_iterations=25, precision=0),

@@ -7,15 +7,29 @@ from ablsteadyrans_amd import serving as S
 
 
 def test_random_tiles_follow_the_batched_distribution():
+    """nx, ny in 32..128, nz in 32..80, dx 20 m, levels from 1 m by 1.08;
+    1-5 hills, H in 20-300 m (scaled down together to half the top when
+    their sum would exceed it), L in 100-600 m; direction in 0-360."""
     rng = np.random.default_rng(1)
+    scaled = 0
     for _ in range(200):
         t = S.random_tile(rng)
         assert 32 <= t.nx <= 128 and 32 <= t.ny <= 128 and 32 <= t.nz <= 80
+        assert t.dx == 20.0 and t.dz0 == 1.0 and t.stretch == 1.08
+        zl = t.zlevels()
+        assert zl[1] == 1.0 and abs(zl[-1] - (1.08 ** t.nz - 1) / 0.08) <= 1e-9 * zl[-1]
         assert 1 <= len(t.hills) <= 5
+        total = sum(h[2] for h in t.hills)
+        assert total <= 0.5 * zl[-1] * (1 + 1e-12)
+        if total < 0.5 * zl[-1] * (1 - 1e-12):   # not scaled: the drawn heights
+            assert all(h[2] >= 20.0 for h in t.hills), t.hills
+        else:
+            scaled += 1
         for (xc, yc, H, L) in t.hills:
-            assert 20.0 <= H <= 300.0 and 100.0 <= L <= 600.0 and H <= 0.3 * L
+            assert H <= 300.0 and H <= S.MAX_ASPECT * L * (1 + 1e-12) and 100.0 <= L <= 600.0
             assert 0 <= xc <= t.nx * t.dx and 0 <= yc <= t.ny * t.dx
         assert 0.0 <= t.direction < 360.0
+    assert 0 < scaled < 200
 
 
 def test_poisson_arrivals_have_the_rate():
