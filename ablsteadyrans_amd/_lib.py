@@ -75,6 +75,7 @@ class PoissonOptionsC(C.Structure):
         ("pre_sweeps", C.c_int32),
         ("post_sweeps", C.c_int32),
         ("coarse_sweeps", C.c_int32),
+        ("max_levels", C.c_int32),
     ]
 
 

@@ -796,6 +796,7 @@ __global__ __launch_bounds__(kThreads) void resid_restrict_kernel(Split f, int p
         for (int bb = 0; bb < ry; ++bb)
             for (int a = 0; a < rx; ++a) {
                 const int i = I * rx + a, j = Jc * ry + bb;
+                if (i >= f.nx || j >= f.ny) { continue; }   // the last cell of an odd extent
                 const int64_t q = f.at(i, j, k);
                 const int64_t c0 = f.at(i, j, 0);
                 R c[kBundle];
@@ -890,26 +891,32 @@ __global__ void mul_kernel(const R* __restrict__ b, const R* __restrict__ c, R* 
 }
 
 // Coarse face values: the mean of the fine faces a coarse face covers.
+// An odd fine extent coarsens to (n + 1) / 2 cells, the last coarse cell
+// covering one fine cell: fine faces and cells past the fine extent are
+// left out of the means (the last x-face maps to the domain's face).
 template <typename R>
-__global__ void coarsen_faces_kernel(int dir, int nxc, int nyc, int nz, int rx, int ry, const R* __restrict__ f,
-                                     R* __restrict__ c)
+__global__ void coarsen_faces_kernel(int dir, int nxc, int nyc, int nz, int rx, int ry, int nxf, int nyf,
+                                     const R* __restrict__ f, R* __restrict__ c)
 {
     // face array extents (coarse): x-faces (nxc+1, nyc, nz), y (nxc, nyc+1, nz), z (nxc, nyc, nz+1)
     const int ex = nxc + (dir == 0), ey = nyc + (dir == 1), ez = nz + (dir == 2);
-    const int fx = nxc * rx + (dir == 0), fy = nyc * ry + (dir == 1);
+    const int fx = nxf + (dir == 0), fy = nyf + (dir == 1);
     const int64_t n = int64_t(ex) * ey * ez;
     for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
         int I, Jc, k;
         cell_of(t, ex, ey, I, Jc, k);
         const int na = dir == 0 ? 1 : rx, nb = dir == 1 ? 1 : ry;
         R s = R(0);
+        int cnt = 0;
         for (int b = 0; b < nb; ++b)
             for (int a = 0; a < na; ++a) {
-                const int i = dir == 0 ? I * rx : I * rx + a;
-                const int j = dir == 1 ? Jc * ry : Jc * ry + b;
+                const int i = dir == 0 ? min(I * rx, nxf) : I * rx + a;
+                const int j = dir == 1 ? min(Jc * ry, nyf) : Jc * ry + b;
+                if (i >= fx || j >= fy) { continue; }
                 s += f[i + int64_t(j) * fx + int64_t(k) * fx * fy];
+                ++cnt;
             }
-        c[t] = s / R(na * nb);
+        c[t] = s / R(cnt);
     }
 }
 
@@ -1378,11 +1385,16 @@ void build_coarse_meshes(PoissonImpl& P, std::vector<std::unique_ptr<Level<R>>>&
         }
         GUST_HIP(hipMemsetAsync(L->coef.p, 0, size_t(kBundle + 1) * size_t(ns) * sizeof(R), st));
         mg.push_back(std::move(L));
-        // Coarsen x and y by two where the size is even and at least 4.
-        const int cx = (nx % 2 == 0 && nx >= 4) ? 2 : 1;
-        const int cy = (ny % 2 == 0 && ny >= 4) ? 2 : 1;
+        // Coarsen x and y by two where the size is at least 4: an even size
+        // halves; an odd one (a non-periodic direction only) becomes
+        // (n + 1) / 2, its last coarse cell over one fine cell and the ghost
+        // node beyond the domain (the coarse grid reaches half a fine cell
+        // past the side: a preconditioner's approximation).
+        const int cx = (nx >= 4 && (nx % 2 == 0 || (!P.per[0] && nx >= 5))) ? 2 : 1;
+        const int cy = (ny >= 4 && (ny % 2 == 0 || (!P.per[1] && ny >= 5))) ? 2 : 1;
         if ((cx == 1 && cy == 1) || lev >= 12 || int64_t(nx) * ny <= 16) { break; }
-        const int ncx = nx / cx, ncy = ny / cy;
+        if (P.opt.max_levels > 0 && lev + 1 >= P.opt.max_levels) { break; }
+        const int ncx = (nx + cx - 1) / cx, ncy = (ny + cy - 1) / cy;
         std::vector<double> zc(size_t(ncx + 3) * (ncy + 3) * (nz + 3));
         const int csj = ncx + 3;
         const int64_t csk = int64_t(ncx + 3) * (ncy + 3);
@@ -1604,17 +1616,19 @@ void setup_hierarchy(PoissonImpl& P, std::vector<std::unique_ptr<Level<R>>>& mg,
                 const int64_t ne = int64_t(L.nx + (dir == 0)) * (L.ny + (dir == 1)) * (L.nz + (dir == 2));
                 const R* f = dir == 0 ? ptr<R>(F.ax) : dir == 1 ? ptr<R>(F.ay) : ptr<R>(F.az);
                 R* c = dir == 0 ? ptr<R>(L.ax) : dir == 1 ? ptr<R>(L.ay) : ptr<R>(L.az);
-                coarsen_faces_kernel<R><<<blocks_for(ne), kThreads, 0, st>>>(dir, L.nx, L.ny, L.nz, L.rx, L.ry, f, c);
+                coarsen_faces_kernel<R><<<blocks_for(ne), kThreads, 0, st>>>(dir, L.nx, L.ny, L.nz, L.rx, L.ry, F.nx,
+                                                                              F.ny, f, c);
             }
             L.has_cross = F.has_cross;
             if (L.has_cross) {
                 const int64_t nzf = int64_t(L.nx) * L.ny * (L.nz + 1);
                 L.dcross.ensure(size_t(nzf) * sizeof(R));
                 L.azm.ensure(size_t(nzf) * sizeof(R));
-                coarsen_faces_kernel<R><<<blocks_for(nzf), kThreads, 0, st>>>(2, L.nx, L.ny, L.nz, L.rx, L.ry,
-                                                                              ptr<R>(F.dcross), ptr<R>(L.dcross));
-                coarsen_faces_kernel<R><<<blocks_for(nzf), kThreads, 0, st>>>(2, L.nx, L.ny, L.nz, L.rx, L.ry,
-                                                                              ptr<R>(F.azm), ptr<R>(L.azm));
+                coarsen_faces_kernel<R><<<blocks_for(nzf), kThreads, 0, st>>>(2, L.nx, L.ny, L.nz, L.rx, L.ry, F.nx,
+                                                                              F.ny, ptr<R>(F.dcross),
+                                                                              ptr<R>(L.dcross));
+                coarsen_faces_kernel<R><<<blocks_for(nzf), kThreads, 0, st>>>(2, L.nx, L.ny, L.nz, L.rx, L.ry, F.nx,
+                                                                              F.ny, ptr<R>(F.azm), ptr<R>(L.azm));
             }
         }
         GUST_HIP(hipGetLastError());
@@ -1714,6 +1728,7 @@ void check_poisson_options(const gust_poisson_options& o)
     if (o.max_iter < 1) { err += "\n  solver.poisson_max_iter must be >= 1"; }
     if (o.restart < 1) { err += "\n  solver.poisson_restart must be >= 1"; }
     if (o.precision != 0 && o.precision != 1) { err += "\n  precision must be 0 (fp64) or 1 (fp32 V-cycle)"; }
+    if (o.max_levels < 0) { err += "\n  the multigrid's max_levels must be >= 0 (0: no limit)"; }
     if (o.pre_sweeps < 0 || o.post_sweeps < 0 || o.coarse_sweeps < 1) {
         err += "\n  the V-cycle needs pre/post sweeps >= 0 and coarse sweeps >= 1";
     }
@@ -1905,6 +1920,7 @@ int gust_poisson_default_options(gust_poisson_options* o)
         o->pre_sweeps = 2;
         o->post_sweeps = 2;
         o->coarse_sweeps = 16;
+        o->max_levels = 0;
     });
 }
 

@@ -223,12 +223,19 @@ def ground_heights(t: Terrain, geom: Geometry, prob_lo, prob_hi) -> np.ndarray:
                 out[b, a] = t.grid.bilinear(x + 1e-3, y + 1e-3)
         return out
     if t.kind == "hills":
+        # along a periodic side each hill repeats with the period (its
+        # images one period either way), so the ground is continuous across
+        # the seam instead of stepping where a hill meets it
         X = prob_lo[0] + ii * dx
         Y = prob_lo[1] + jj * dy
         XX, YY = np.meshgrid(X, Y)
         h = np.zeros_like(XX)
+        sx = (-nx * dx, 0.0, nx * dx) if perx else (0.0,)
+        sy = (-ny * dy, 0.0, ny * dy) if pery else (0.0,)
         for (xc, yc, H, L) in t.hills:
-            h = h + H * np.exp(-((XX - xc) ** 2 + (YY - yc) ** 2) / (L * L))
+            for ox in sx:
+                for oy in sy:
+                    h = h + H * np.exp(-((XX - xc - ox) ** 2 + (YY - yc - oy) ** 2) / (L * L))
         return h
     if t.kind == "fractal":
         hv = np.asarray(t.heights, dtype=np.float64)

@@ -43,6 +43,7 @@ class PoissonOptions:
     pre_sweeps: int = 2
     post_sweeps: int = 2
     coarse_sweeps: int = 16
+    max_levels: int = 0
 
     def c(self) -> _lib.PoissonOptionsC:
         o = _lib.PoissonOptionsC()

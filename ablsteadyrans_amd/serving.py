@@ -149,15 +149,6 @@ def random_tile(rng: np.random.Generator) -> TileSpec:
     return spec
 
 
-def hill_tile(rng: np.random.Generator) -> TileSpec:
-    """A tile of the reference's Neutral_Hill_3D deck with a random wind
-    direction and hill position: 64 x 64 x 20 cells of 80 m under an 800 m
-    top, one Gaussian hill (100 m high, 500 m wide) within the middle half."""
-    lx = 64 * 80.0
-    xc, yc = float(rng.uniform(0.25 * lx, 0.75 * lx)), float(rng.uniform(0.25 * lx, 0.75 * lx))
-    return TileSpec(64, 64, 20, 80.0, [(xc, yc, 100.0, 500.0)], float(rng.uniform(0.0, 360.0)), top=800.0, dz0=0.0)
-
-
 def tile_deck(spec: TileSpec, kind: str, max_iterations: int = 4000, sweeps: int = 16, drop_orders: float = 6.0,
               ref_iterations: int = 5) -> D.Deck:
     """The deck of one request: the open-channel neutral ABL of the

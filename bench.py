@@ -177,17 +177,17 @@ def _time_vcycle(tile, precision, reps):
 
 
 def _mg_levels(nx, ny, nz):
-    """Cells per multigrid level: x/y halved while even and >= 4, until
-    nx*ny <= 16 (or 12 levels)."""
+    """Cells per multigrid level of a tile with no periodic side: x/y halved
+    (odd sizes to (n + 1) / 2) while >= 4, until nx*ny <= 16 (or 12 levels)."""
     cells = []
     for lev in range(64):
         cells.append(nx * ny * nz)
-        cx = 2 if (nx % 2 == 0 and nx >= 4) else 1
-        cy = 2 if (ny % 2 == 0 and ny >= 4) else 1
+        cx = 2 if nx >= 4 else 1
+        cy = 2 if ny >= 4 else 1
         if (cx == 1 and cy == 1) or lev >= 12 or nx * ny <= 16:
             break
-        nx //= cx
-        ny //= cy
+        nx = (nx + cx - 1) // cx
+        ny = (ny + cy - 1) // cy
     return cells
 
 
@@ -358,99 +358,52 @@ def _profile_deck(name, iterations):
             "ms_per_iteration": round(dt * 1e3 / max(int(r.iterations), 1), 4)}
 
 
-def _serving_leg():
-    """The serving configuration: 14 requests arriving as a Poisson process
-    at 20 per second, every seventh a converged steady field on a tile of
-    the reference's 3-D hill deck (random wind direction and hill position;
-    the deck's criterion), the others the mass-consistent wind on a random
-    terrain tile (32..128 squared, 32..80 levels, 1-5 Gaussian hills),
-    served with 8 slots in mixed precision; then the same requests one after
-    another, each alone on one stream in FP64 (the plain per-request path).
-    Returns both summaries."""
+def _serving_leg(n=16):
+    """The serving configuration: n requests arriving as a Poisson process at
+    20 per second (exponential gaps, seed 20265203), each a tile of the
+    batched distribution (serving.random_tile: 32..128 squared, 32..80
+    levels from 1 m by 1.08, 1-5 Gaussian hills, a random wind direction),
+    half of them (alternating) the mass-consistent wind and half a converged
+    steady field at a 6-order residual drop (at most 6000 iterations),
+    served with 8 slots in the mixed-precision form, admitted by HBM budget
+    (75 % of the free HBM, each tile charged its cells times the measured
+    bytes per cell); then the same requests one after another, each alone
+    on one stream in FP64 (the plain per-request path). Returns both
+    summaries and the measured bytes per cell."""
     import numpy as np
     from ablsteadyrans_amd import serving as S
-    rng = np.random.default_rng(20261203 + 5000)
-    arrivals = S.poisson_arrivals(14, 20.0, rng)
+    rng = np.random.default_rng(20265203)
+    arrivals = S.poisson_arrivals(n, 20.0, rng)
     reqs = []
-    for i in range(14):
-        spec = S.hill_tile(rng) if i % 7 == 6 else S.random_tile(rng)
-        reqs.append(S.Request(i, "steady" if i % 7 == 6 else "init", spec, arrival=float(arrivals[i]),
-                              precision=1, max_iterations=6000))
-    served, wall = S.Server(slots=8).run(reqs)
+    for i in range(n):
+        spec = S.random_tile(np.random.default_rng(20265203 + 1 + i))
+        reqs.append(S.Request(i, "steady" if i % 2 else "init", spec, arrival=float(arrivals[i]), precision=2,
+                              max_iterations=6000))
+    server = S.Server.measured(slots=8)
+    served, wall = server.run(reqs)
     plain = [S.Request(r.rid, r.kind, r.spec, 0.0, 0, r.max_iterations) for r in reqs]
     seq, wall_seq = S.Server(slots=1).run(plain)
-    return S.summary(served, wall), S.summary(seq, wall_seq)
+    return S.summary(served, wall), S.summary(seq, wall_seq), round(server.bytes_per_cell, 1)
 
 
-def _batched_leg():
-    """64 independent random terrain tiles (mass-consistent wind) submitted
-    at once: 8 concurrent streams vs one stream; tiles per second."""
+def _batched_leg(iterations=20):
+    """64 independent tiles of the batched distribution (seeds 20264203 +
+    tile) submitted at once, each prepared (mesh on the device, the
+    mass-consistent first guess) and advanced `iterations` mixed-precision
+    SIMPLEC iterations: 8 concurrent streams vs one stream; tiles per
+    second and cell-iterations per second."""
     import numpy as np
     from ablsteadyrans_amd import serving as S
-    rng = np.random.default_rng(20261203 + 4000)
-    reqs = [S.Request(i, "init", S.random_tile(rng)) for i in range(64)]
+    reqs = [S.Request(i, "steady", S.random_tile(np.random.default_rng(20264203 + i)), precision=2,
+                      max_iterations=iterations) for i in range(64)]
     res8, wall8 = S.Server(slots=8).run(reqs)
     res1, wall1 = S.Server(slots=1).run(reqs)
     cells = sum(r.spec.cells for r in reqs)
-    return {"tiles": 64, "cells": cells, "errors": sum(1 for r in res8 + res1 if r.error),
+    return {"tiles": 64, "cells": cells, "iterations_per_tile": iterations,
+            "errors": sum(1 for r in res8 + res1 if r.error),
             "tiles_per_s_8_streams": round(64 / wall8, 3), "tiles_per_s_1_stream": round(64 / wall1, 3),
+            "cell_iterations_per_s_8_streams": round(cells * iterations / wall8, 1),
             "speedup": round(wall1 / wall8, 3)}
-
-
-def _hill_tile():
-    """The Gaussian hill h = 100 exp(-(r/250 m)^2) m centred in 4 x 4 km at
-    dx = 20 m, 64 levels from dz1 = 1 m growing by 1.08, periodic sides."""
-    from ablsteadyrans_amd import mesh as M
-    zl = M.make_zlevels(64, 1.0, dz0=1.0, stretch_ratio=1.08)
-    top = float(zl[-1])
-    geom = M.geometry((200, 200, 64), (0.0, 0.0, 0.0), (4000.0, 4000.0, top), (1, 1))
-    t = M.Terrain(kind="hills", hills=[(2000.0, 2000.0, 100.0, 250.0)])
-    return M.build_mesh(geom, (0.0, 0.0, 0.0), (4000.0, 4000.0, top), zl, t), top
-
-
-def _time_hill_iterations(precision, iters, warmup=3):
-    """ms per SIMPLEC iteration on the hill tile (neutral ABL with rotation,
-    the log-law first guess, local dtau at CFL 2), HIP events on the stream."""
-    import torch
-    from ablsteadyrans_amd import initializer as GI
-    from ablsteadyrans_amd import physics as GP
-    from ablsteadyrans_amd.fab import Fab
-    from ablsteadyrans_amd.poisson import DeviceMesh
-    tile, top = _hill_tile()
-    dm = DeviceMesh(tile)
-    inputs = GP.ResidualInputs(closure=GP.ClosureOptions(tke_floor=1e-6), surface=GP.SurfaceLayerOptions(z0=0.03),
-                               forcing=GP.ForcingOptions(coriolis=True, coriolis_3d=False, latitude=90.0,
-                                                         rotation_period=125663.7061435917, geostrophic=True,
-                                                         ug=10.0))
-    stage = GP.SimplecStage(dm, inputs, GP.SimplecOptions(sweeps=2, tke_floor=1e-6, height=top,
-                                                          poisson_precision=precision))
-    q = GP.DeviceState(tile.geom)
-    GI.first_guess_state(dm, GI.InflowProfile(kind=GI.LOG_LAW, ustar=0.4, z0=0.03), GI.InitOptions(first_guess=1), q)
-    dtau = Fab.at_loc(tile.geom, 0, 1, 0)
-    stage.init(q)
-    import ctypes as C
-    from ablsteadyrans_amd import _lib
-    g = tile.geom.c()
-
-    def step():
-        nu = stage.nu_view()
-        _lib.check(_lib.lib.gust_local_dtau(C.byref(g), C.byref(q["u"].array4()), C.byref(q["v"].array4()),
-                                            C.byref(q["w"].array4()), C.byref(dm.z_nd.array4()),
-                                            C.byref(dm.detJ.array4()), C.byref(nu), 2.0, C.byref(dtau.array4()),
-                                            C.c_void_p(torch.cuda.current_stream().cuda_stream)))
-        stage.iterate(q, dtau)
-
-    for _ in range(warmup):
-        step()
-    torch.cuda.synchronize()
-    stream = torch.cuda.current_stream()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(iters):
-        step()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / iters
 
 
 def _cpu_column_baseline(target_s=15.0):
@@ -514,7 +467,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving and batched legs")
     ap.add_argument("--profile-only", action="store_true", help="sweeps only (for rocprofv3 runs)")
-    ap.add_argument("--profile-deck", choices=("column", "askervein", "abl", "hill3d", "hill200", "c2", "c2mixed"),
+    ap.add_argument("--profile-deck", choices=("column", "askervein", "abl", "hill3d", "c2", "c2mixed"),
                     help="only N driver iterations of one reference deck (for rocprofv3 runs)")
     ap.add_argument("--iterations", type=int, default=100, help="driver iterations of --profile-deck")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -538,11 +491,6 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8,
                           "launches": sweep_steps}))
-        return
-    if args.profile_deck == "hill200":
-        ms = _time_hill_iterations(0, args.iterations)
-        print(json.dumps({"profile_deck": "gaussian_hill_200x200x64", "iterations": args.iterations,
-                          "ms_per_iteration": round(ms, 4)}))
         return
     if args.profile_deck in ("c2", "c2mixed"):
         from ablsteadyrans_amd import decks as D
@@ -613,8 +561,9 @@ def main():
     serving = batched = None
     if not args.no_serving:
         print("bench: serving (Poisson arrivals at 20/s) and 64 batched tiles", file=sys.stderr, flush=True)
-        served, plain = _serving_leg()
-        serving = {"served_mixed_8_slots": served, "plain_fp64_one_at_a_time": plain}
+        served, plain, bpc = _serving_leg()
+        serving = {"served_mixed_8_slots": served, "plain_fp64_one_at_a_time": plain,
+                   "measured_bytes_per_cell": bpc}
         batched = _batched_leg()
     cpu = cpu_sweep = None
     if not args.no_cpu:

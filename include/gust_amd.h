@@ -204,6 +204,7 @@ typedef struct gust_poisson_options {
     int32_t verbose;
     int32_t precision;
     int32_t pre_sweeps, post_sweeps, coarse_sweeps;   /* V(2,2), 16 on the coarsest level */
+    int32_t max_levels;    /* multigrid levels at most (0: coarsen while x or y halves) */
 } gust_poisson_options;
 
 /* Source/Poisson.H:96-103 (FaceCoeffs): optional face coefficients (null =
