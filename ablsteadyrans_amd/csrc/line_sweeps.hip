@@ -993,6 +993,16 @@ __global__ __launch_bounds__(256) void lusgs_kernel(const LusgsSys<T>* __restric
         }
     } else {
         W = S.d; wj = S.d_j; wk = S.d_k;
+        // delta starts at 0 (check_line_system's setVal): the passes read
+        // the columns not yet updated, so nothing a previous call left there
+        // may remain (the ghosts are zeroed before the launch).
+        const int64_t n = int64_t(S.nx) * S.ny * S.nz;
+        for (int64_t t = tid; t < n; t += nth) {
+            const int i = int(t % S.nx);
+            const int64_t rr = t / S.nx;
+            const int j = int(rr % S.ny), k = int(rr / S.ny);
+            W[i + j * wj + k * wk] = T(0);
+        }
     }
     __syncthreads();
     const int sx = S.s_hi[0] - S.s_lo[0] + 1, sy = S.s_hi[1] - S.s_lo[1] + 1;

@@ -111,7 +111,8 @@ def neutral_hill_3d() -> Deck:
     lo, hi = (-2560.0, -2560.0, 0.0), (2560.0, 2560.0, 800.0)
     geom = M.geometry((64, 64, 20), lo, hi, (1, 1))
     zl = M.make_zlevels(20, 800.0)
-    tile = M.build_mesh(geom, lo, hi, zl, M.Terrain(kind="agnesi", hill_height=100.0, hill_half_width=500.0))
+    tile = M.build_mesh(geom, lo, hi, zl, M.Terrain(kind="agnesi", hill_height=100.0, hill_half_width=500.0,
+                                                       hill_shape="round"))
     sounding = dict(theta_ref_z=(0.0, 700.0, 1000.0), theta_ref=(300.0, 300.0, 309.0))
     prof = GI.InflowProfile(kind=GI.LOG_LAW, ustar=0.35, z0=0.1, **sounding)
     inputs = GP.ResidualInputs(closure=GP.ClosureOptions(tke_floor=1e-6), surface=GP.SurfaceLayerOptions(z0=0.1),

@@ -163,19 +163,88 @@ class ErfGrid:
 
 @dataclass
 class Terrain:
-    """A ground description: ``kind`` in flat | agnesi | erf_grid | hills | fractal."""
+    """A ground description: ``kind`` in flat | agnesi | erf_grid | xyz | hills | fractal."""
 
     kind: str = "flat"
     # agnesi (Source/TerrainSurface.cpp:329-386)
     hill_height: float = 0.0
     hill_half_width: float = 1.0
-    hill_shape: str = "hill"          # hill | ridge_x | ridge_y
+    hill_shape: str = "ridge_x"       # ridge_x | ridge_y | round (the reference's default: ridge_x)
     # erf_grid
     grid: ErfGrid | None = None
     # hills: list of (xc, yc, H, L): h = sum H exp(-((x-xc)^2 + (y-yc)^2) / L^2)
     hills: list = field(default_factory=list)
     # fractal: a precomputed (ny+1, nx+1) node height array
     heights: np.ndarray | None = None
+    # xyz: scattered points (x, y, z arrays), inverse-distance weighted
+    points: tuple | None = None
+    idw_neighbors: int = 6
+    idw_power: float = 2.0
+
+
+def read_xyz_points(path: str, key: str = "terrain.file"):
+    """Source/TerrainSurface.cpp:64-99: one "x y z" per line, comma or white
+    space separated, '#' comments; lines without three numbers are skipped,
+    points with a non-finite value are dropped. Returns (x, y, z, nonfinite)."""
+    xs, ys, zs, bad = [], [], [], 0
+    try:
+        lines = open(path).read().split("\n")
+    except OSError:
+        raise ValueError(f"gust: {key} = {path}: cannot be read") from None
+    for line in lines:
+        tok = line.split("#", 1)[0].replace(",", " ").split()
+        if len(tok) != 3:
+            continue
+        try:
+            x, y, z = (float(t) for t in tok)
+        except ValueError:
+            continue
+        if np.isfinite(x) and np.isfinite(y) and np.isfinite(z):
+            xs.append(x)
+            ys.append(y)
+            zs.append(z)
+        else:
+            bad += 1
+    if not xs:
+        raise ValueError(f"gust: {key} = {path}: holds no 'x y z' point")
+    return np.array(xs), np.array(ys), np.array(zs), bad
+
+
+def idw_heights(points, xq: np.ndarray, yq: np.ndarray, k: int = 6, power: float = 2.0) -> np.ndarray:
+    """Source/TerrainSurface.cpp:136-187: the inverse-distance-weighted height
+    at each query from its k nearest points (squared distances dx^2 + dy^2,
+    ties broken by point index), weights d^-power summed nearest first; a
+    point closer than 1e-6 m gives its own height."""
+    px, py, pz = (np.asarray(a, dtype=np.float64) for a in points[:3])
+    n = px.size
+    if not 1 <= k <= n:
+        raise ValueError(f"gust: terrain.idw_neighbors = {k} must be between 1 and the number of points in "
+                         f"terrain.file ({n})")
+    xq = np.asarray(xq, dtype=np.float64).ravel()
+    yq = np.asarray(yq, dtype=np.float64).ravel()
+    out = np.empty(xq.size)
+    chunk = max(1, int(4_000_000 // max(n, 1)))
+    for s0 in range(0, xq.size, chunk):
+        qx, qy = xq[s0:s0 + chunk, None], yq[s0:s0 + chunk, None]
+        dx = px[None, :] - qx
+        dy = py[None, :] - qy
+        d2 = dx * dx + dy * dy
+        # the k smallest by (d2, index): a stable sort on d2 keeps index order within ties
+        order = np.argsort(d2, axis=1, kind="stable")[:, :k]
+        dk = np.take_along_axis(d2, order, axis=1)
+        zk = pz[order]
+        wsum = np.zeros(dk.shape[0])
+        zval = np.zeros(dk.shape[0])
+        with np.errstate(divide="ignore", invalid="ignore"):   # exact hits are replaced below
+            for i in range(k):
+                w = np.power(dk[:, i], -power / 2.0)
+                wsum = wsum + w
+                zval = zval + w * zk[:, i]
+            res = zval / wsum
+        hit = dk[:, 0] < 1.0e-12
+        res[hit] = zk[hit, 0]
+        out[s0:s0 + chunk] = res
+    return out
 
 
 def ground_heights(t: Terrain, geom: Geometry, prob_lo, prob_hi) -> np.ndarray:
@@ -209,6 +278,8 @@ def ground_heights(t: Terrain, geom: Geometry, prob_lo, prob_hi) -> np.ndarray:
         if t.hill_shape == "ridge_y":
             yl = Y / L
             return np.broadcast_to((hm / (1.0 + yl * yl))[:, None], out.shape).copy()
+        if t.hill_shape != "round":
+            raise ValueError(f"gust: terrain.hill_shape must be ridge_x, ridge_y or round (got {t.hill_shape})")
         XX, YY = np.meshgrid(X, Y)
         r = np.sqrt(XX * XX + YY * YY)
         rl = r / L
@@ -237,6 +308,13 @@ def ground_heights(t: Terrain, geom: Geometry, prob_lo, prob_hi) -> np.ndarray:
                 for oy in sy:
                     h = h + H * np.exp(-((XX - xc - ox) ** 2 + (YY - yc - oy) ** 2) / (L * L))
         return h
+    if t.kind == "xyz":
+        if t.points is None:
+            raise ValueError("gust: terrain.type = xyz needs the points (read_xyz_points)")
+        X = float(prob_lo[0]) + ii * float(dx)
+        Y = float(prob_lo[1]) + jj * float(dy)
+        XX, YY = np.meshgrid(X, Y)
+        return idw_heights(t.points, XX, YY, t.idw_neighbors, t.idw_power).reshape(XX.shape)
     if t.kind == "fractal":
         hv = np.asarray(t.heights, dtype=np.float64)
         if hv.shape != (ny + 1, nx + 1):
@@ -307,8 +385,99 @@ def tile_frame(geom: Geometry, prob_lo, prob_hi, zlevels: np.ndarray, terrain: T
     return TileFrame(geom, tuple(prob_lo), tuple(prob_hi), np.asarray(zlevels, dtype=np.float64).copy(), terrain)
 
 
-def build_mesh(geom: Geometry, prob_lo, prob_hi, zlevels: np.ndarray, terrain: Terrain) -> TileMesh:
-    """BTF mesh of one tile (Source/Mesh.cpp:107-140 with terrain.smoothing = 0)."""
+def _stf_levels(Z, zlevels, nx, ny, nz, perx, pery, gamma_m):
+    """The smoothed terrain-following levels (STF, Klemp 2011), k = 1 .. nz,
+    as extern_erf/ERF_TerrainMetrics.cpp:384-573 computes them on the node
+    slab grown by one ghost node (Z[1] holds the ground there): the
+    smoothed surface h_s of level k from level k-1's by up to 50 Laplacian
+    passes with beta_k = 0.2 min(z_k / (2 h_m), 1), each pass accepted only
+    while the minimum layer thickness ratio stays >= gamma_m (a rejected
+    first pass carries level k-1's surface up unsmoothed); z = z_k + A_k h_s
+    with A = cos^6(pi z / (2 z_H)) below z_H = 2.44 h_m / (1 - gamma_m).
+    Returns the number of levels whose smoothing was capped."""
+    # the ground of the grown slab: ghost columns take the clamped node
+    ic = np.minimum(np.maximum(np.arange(-1, nx + 2), 0), nx) + 1
+    jc = np.minimum(np.maximum(np.arange(-1, ny + 2), 0), ny) + 1
+    zs = Z[1][np.ix_(jc, ic)].copy()
+    Z[1] = zs
+    h_m = float(zs.max())
+    if h_m < np.finfo(np.float64).eps:
+        h_m = 1e-16
+
+    def fill(h):
+        # FillBoundary of a nodal slab with one ghost node
+        if perx:
+            h[:, 0] = h[:, nx]
+            h[:, nx + 2] = h[:, 2]
+        if pery:
+            h[0, :] = h[ny, :]
+            h[ny + 2, :] = h[2, :]
+
+    H = np.zeros((nz + 1, ny + 3, nx + 3))   # h_s by level
+    H[0] = zs
+    fill(H[0])
+    Hold = H.copy()
+    z_H = 2.44 * h_m / (1 - gamma_m)
+    capped = 0
+    # clipped neighbour indices of every slab node (ii +- 1, jj +- 1 read ghosts)
+    ii = np.minimum(np.maximum(np.arange(-1, nx + 2), 0), nx) + 1
+    jj = np.minimum(np.maximum(np.arange(-1, ny + 2), 0), ny) + 1
+    I, J = np.meshgrid(ii, jj)
+    for k in range(1, nz + 1):
+        zz, zzm = float(zlevels[k]), float(zlevels[k - 1])
+        foo = np.cos((np.pi / 2) * (zz / z_H))
+        A = foo * foo * foo * foo * foo * foo if zz < z_H else 0.0
+        foom = np.cos((np.pi / 2) * (zzm / z_H))
+        Am = foom * foom * foom * foom * foom * foom if zzm < z_H else 0.0
+        beta = 0.2 * min(zz / (2 * h_m), 1.0)
+        it = 0
+        while it < 50:
+            src = Hold[k - 1] if it == 0 else Hold[k]
+            lap = src[J, I + 1] + src[J, I - 1] + src[J + 1, I] + src[J - 1, I] - 4 * src[J, I]
+            H[k] = src + beta * lap
+            diff = float(np.min((zz + A * H[k] - (zzm + Am * H[k - 1])) / (zz - zzm)))
+            if diff < gamma_m:
+                capped += 1
+                if it == 0:
+                    H[k] = H[k - 1]
+                    Hold[:] = H
+                    fill(Hold[k])
+                else:
+                    H[:] = Hold
+                break
+            Hold[:] = H
+            it += 1
+            fill(Hold[k])
+        Z[k + 1] = zz + A * Hold[k]
+        if k == 1:
+            Z[0] = 2.0 * Z[1] - Z[2]
+    return capped
+
+
+def _sullivan_levels(Z, zlevels, nx, ny, nz):
+    """Sullivan et al. (2014) levels, z = z_k + (1 - z_k / z_top)^3 h
+    (extern_erf/ERF_TerrainMetrics.cpp:596-630) on the grown slab, h the
+    clamped column's ground."""
+    ic = np.minimum(np.maximum(np.arange(-1, nx + 2), 0), nx) + 1
+    jc = np.minimum(np.maximum(np.arange(-1, ny + 2), 0), ny) + 1
+    zs = Z[1][np.ix_(jc, ic)].copy()
+    z_top = float(zlevels[nz])
+    for k in range(0, nz + 1):
+        z = float(zlevels[k])
+        Z[k + 1] = z + np.power(1.0 - (z / z_top), 3.0) * zs
+    Z[1] = zs
+    Z[0] = 2.0 * zs - Z[2]
+
+
+def build_mesh(geom: Geometry, prob_lo, prob_hi, zlevels: np.ndarray, terrain: Terrain, smoothing: int = 0,
+               gamma_m: float = 0.5) -> TileMesh:
+    """The terrain-following mesh of one tile (Source/Mesh.cpp:107-140):
+    terrain.smoothing = 0 BTF, 1 STF (gamma_m: the minimum layer thickness
+    ratio), 2 Sullivan. The STF capped-level count is mesh.stf_capped_levels."""
+    if smoothing not in (0, 1, 2):
+        raise ValueError("gust: terrain.smoothing must be 0 (BTF), 1 (STF) or 2 (Sullivan)")
+    if not (np.isfinite(gamma_m) and 0 < gamma_m < 1):
+        raise ValueError("gust: terrain.gamma_m must be in (0, 1)")
     nx, ny, nz = geom.n
     if zlevels.size != nz + 1:
         raise ValueError("zlevels must have nz + 1 values")
@@ -326,20 +495,28 @@ def build_mesh(geom: Geometry, prob_lo, prob_hi, zlevels: np.ndarray, terrain: T
     if not pery:
         Z[2:nz + 2, 0, :] = BOGUS
         Z[2:nz + 2, ny + 2, :] = BOGUS
-    # BTF (extern_erf/ERF_TerrainMetrics.cpp:330-356): the clamped column's
-    # ground for every column of the grown box, k = 0 .. nz.
-    ic = np.minimum(np.maximum(np.arange(-1, nx + 2), 0), nx) + 1
-    jc = np.minimum(np.maximum(np.arange(-1, ny + 2), 0), ny) + 1
-    zs = Z[1][np.ix_(jc, ic)].copy()
     z_top = float(zlevels[nz])
-    z_lev_sfc = float(zlevels[0])
-    Z[1] = zs
-    for k in range(1, nz + 1):
-        z = float(zlevels[k])
-        Z[k + 1] = ((zs - z_lev_sfc) * z_top + (z_top - zs) * z) / (z_top - z_lev_sfc)
-    _fill_boundary_nodal(Z, nx, ny, perx, pery)
-    Z[0] = 2.0 * Z[1] - Z[2]
-    _fill_boundary_nodal(Z, nx, ny, perx, pery)
+    capped = 0
+    if smoothing == 1:
+        capped = _stf_levels(Z, zlevels, nx, ny, nz, perx, pery, gamma_m)
+        _fill_boundary_nodal(Z, nx, ny, perx, pery)
+    elif smoothing == 2:
+        _sullivan_levels(Z, zlevels, nx, ny, nz)
+        _fill_boundary_nodal(Z, nx, ny, perx, pery)
+    else:
+        # BTF (extern_erf/ERF_TerrainMetrics.cpp:330-356): the clamped column's
+        # ground for every column of the grown box, k = 0 .. nz.
+        ic = np.minimum(np.maximum(np.arange(-1, nx + 2), 0), nx) + 1
+        jc = np.minimum(np.maximum(np.arange(-1, ny + 2), 0), ny) + 1
+        zs = Z[1][np.ix_(jc, ic)].copy()
+        z_lev_sfc = float(zlevels[0])
+        Z[1] = zs
+        for k in range(1, nz + 1):
+            z = float(zlevels[k])
+            Z[k + 1] = ((zs - z_lev_sfc) * z_top + (z_top - zs) * z) / (z_top - z_lev_sfc)
+        _fill_boundary_nodal(Z, nx, ny, perx, pery)
+        Z[0] = 2.0 * Z[1] - Z[2]
+        _fill_boundary_nodal(Z, nx, ny, perx, pery)
     Z[nz + 2] = Z[nz + 1] + 1 * (Z[nz + 1] - Z[nz])
     # Source/Mesh.cpp:129-133: OverrideSync, FillBoundary, ghost columns.
     if perx:
@@ -378,6 +555,7 @@ def build_mesh(geom: Geometry, prob_lo, prob_hi, zlevels: np.ndarray, terrain: T
                       + Z[1:nz + 3, 1:ny + 3, 0:nx + 2] + Z[1:nz + 3, 1:ny + 3, 1:nx + 3])
     _fill_boundary_cell(zcc, nx, ny, perx, pery, nodal=(0, 0))
     mesh = TileMesh(geom, tuple(prob_lo), tuple(prob_hi), zlevels.copy(), Z, detJ, ax, ay, az, zcc)
+    mesh.stf_capped_levels = capped
     ground = Z[1, 1:ny + 2, 1:nx + 2]
     if not float(ground.max()) < z_top:
         raise ValueError(f"gust: the terrain reaches {float(ground.max())} m, at or above the top level {z_top} m")

@@ -13,6 +13,11 @@ tests/golden/ (data only: inputs and expected outputs, no source text).
                        as the plotfile's face means, p, theta, tke, z,
                        detJ; (nz, ny, nx), i fastest) and the gold JSON
                        numbers.
+  erf_hill2d.npz       Tests/Unit_Tests/Gold/erf_hill2d_{btf,stf,sullivan}:
+                       ERF's z_phys and detJ (cell, (nz, ny, nx)) and the
+                       node heights (Level_0/Nu_nd component 2 = z - k dz,
+                       plus k dz; (nz+1, ny+1, nx+1)) of the 32 x 1 x 16
+                       Agnesi hill for terrain.smoothing 0, 1, 2.
 
 Run from the repository root with the reference tree at /root/reference.
 """
@@ -59,6 +64,28 @@ def read_plotfile_cells(plt, nvar_expected=1):
     return names, lo, hi, out
 
 
+def read_vismf_nodal(mf_base, comp):
+    """One component of a single-FAB VisMF MultiFab (Level_0/<name>_H and its
+    data file) as (lo, array (nz', ny', nx'))."""
+    ch = open(mf_base + "_H").read()
+    fods = re.findall(r"FabOnDisk: (\S+) (\d+)", ch)
+    if len(fods) != 1:
+        raise SystemExit("expected one FAB in " + mf_base)
+    fname, off = fods[0]
+    with open(os.path.join(os.path.dirname(mf_base), fname), "rb") as f:
+        f.seek(int(off))
+        line = f.readline().decode()
+        mm = re.search(r"\(\((-?\d+),(-?\d+),(-?\d+)\) \((-?\d+),(-?\d+),(-?\d+)\) \(\d+,\d+,\d+\)\) (\d+)", line)
+        b = [int(mm.group(i)) for i in range(1, 7)]
+        nc = int(mm.group(7))
+        if "(8 7 6 5 4 3 2 1)" not in line or "(64 11 52 0 1 12 0 1023)" not in line:
+            raise SystemExit("unexpected FAB real descriptor: " + line)
+        n = (b[3] - b[0] + 1) * (b[4] - b[1] + 1) * (b[5] - b[2] + 1)
+        data = np.frombuffer(f.read(8 * n * nc), dtype="<f8").reshape(nc, b[5] - b[2] + 1, b[4] - b[1] + 1,
+                                                                        b[3] - b[0] + 1)
+    return tuple(b[:3]), data[comp].copy()
+
+
 def main():
     v = np.loadtxt(os.path.join(REF, "extern_erf", "askervein.txt"), dtype=np.float64).ravel()
     nx, ny = int(v[0]), int(v[1])
@@ -81,6 +108,19 @@ def main():
     np.savez_compressed(os.path.join(OUT, "init_askervein.npz"), **{n: st[q] for q, n in enumerate(names)},
                         **{k: gi[k] for k in ("div_before", "div_after", "iterations", "reltol", "speed_max", "wmax",
                                               "theta_min", "theta_max", "p_absmax", "k_min")})
+    hill = {}
+    for name in ("btf", "stf", "sullivan"):
+        base = os.path.join(REF, "Tests/Unit_Tests/Gold", f"erf_hill2d_{name}")
+        names, lo, hi, cells = read_plotfile_cells(base)
+        assert names == ["z_phys", "detJ"], names
+        nlo, zmk = read_vismf_nodal(os.path.join(base, "Level_0", "Nu_nd"), 2)
+        assert nlo == (0, 0, 0)
+        dz = 1000.0 / 16
+        k = np.arange(zmk.shape[0], dtype=np.float64)[:, None, None]
+        hill[f"{name}_z_phys"] = cells[0]
+        hill[f"{name}_detJ"] = cells[1]
+        hill[f"{name}_z_nd"] = zmk + k * dz
+    np.savez_compressed(os.path.join(OUT, "erf_hill2d.npz"), **hill)
     print("wrote fixtures to", OUT)
 
 

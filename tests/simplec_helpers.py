@@ -154,13 +154,39 @@ def random_state(mesh, seed, open_top=False):
 
 # --- momentum predictor and scalar sweep (Source/Predictor.cpp:129-405, Source/Simplec.cpp:222-281) ---------
 
+# The line-sweep kind of the composites: None for z-line Jacobi, or
+# (ghosts, overlap, order, alternate) for line LU-SGS (set by lusgs_sweeps).
+_LUSGS = None
+
+
+class lusgs_sweeps:
+    """Within this context the composites sweep with the oracle's line
+    LU-SGS (Source/LineSweeps.cpp:415-428) instead of z-line Jacobi."""
+
+    def __init__(self, ghosts=0, overlap=0, order=0, alternate=0):
+        self.opt = (ghosts, overlap, order, alternate)
+
+    def __enter__(self):
+        global _LUSGS
+        self.prev, _LUSGS = _LUSGS, self.opt
+        return self
+
+    def __exit__(self, *exc):
+        global _LUSGS
+        _LUSGS = self.prev
+
+
 def _zline(loc, A, ap, rhs, sweeps, geom):
-    """The oracle's z-line Jacobi on valid arrays: returns (delta valid, change)."""
+    """The oracle's z-line Jacobi (or LU-SGS, see lusgs_sweeps) on valid
+    arrays: returns (delta valid, change)."""
     sh = rhs.shape
     d = np.zeros((1, sh[0] + 2, sh[1] + 2, sh[2] + 2))
     lo = tuple(geom.lo)
-    ch = orc.zline_jacobi(loc, A, lo, ap[None].copy(), rhs[None].copy(), lo, sweeps, geom, d,
-                          (lo[0] - 1, lo[1] - 1, lo[2] - 1))
+    db = (lo[0] - 1, lo[1] - 1, lo[2] - 1)
+    if _LUSGS is not None:
+        ch = orc.zline_lusgs(loc, A, lo, ap[None].copy(), rhs[None].copy(), lo, sweeps, geom, d, db, *_LUSGS)
+    else:
+        ch = orc.zline_jacobi(loc, A, lo, ap[None].copy(), rhs[None].copy(), lo, sweeps, geom, d, db)
     return d[0, 1:-1, 1:-1, 1:-1].copy(), ch
 
 

@@ -19,7 +19,7 @@ def _askervein_grid():
 
 CASES = [
     ("flat", (6, 5, 7), (1, 1), M.Terrain(kind="flat"), 0.0, 1.08),
-    ("agnesi", (16, 12, 10), (1, 1), M.Terrain(kind="agnesi", hill_height=100.0, hill_half_width=300.0), 0.0, 0.0),
+    ("agnesi", (16, 12, 10), (1, 1), M.Terrain(kind="agnesi", hill_height=100.0, hill_half_width=300.0, hill_shape="round"), 0.0, 0.0),
     ("hills", (14, 11, 9), (0, 1), M.Terrain(kind="hills", hills=[(300.0, 200.0, 60.0, 150.0),
                                                                 (100.0, 500.0, 30.0, 90.0)]), 20.0, 1.1),
     ("hills_open", (9, 13, 8), (0, 0), M.Terrain(kind="hills", hills=[(200.0, 300.0, 80.0, 200.0)]), 5.0, 0.0),

@@ -204,6 +204,9 @@ def test_askervein_projection_matches_the_gold(cuda, precision):
     div1 = H.face_div(m, U)
     assert np.sqrt(np.sum(div1 ** 2)) <= 10 * 1e-10 * div0
     print(f"askervein precision {precision}: {res.iterations} GMRES iterations, {S.levels} levels")
+    # rt_poisson_askervein_mlmg's bound (Tests/CMakeLists.txt:605-618): at
+    # most 1.1 times the gold's 6 iterations
+    assert res.iterations <= 1.1 * float(gold["iterations_mlmg"]), res.iterations
 
 
 def test_vcycle_is_a_convergent_iteration(cuda):

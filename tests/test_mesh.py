@@ -83,3 +83,70 @@ def test_btf_hill_is_terrain_following_and_periodic_ghosts_are_images():
     depth = (Z[9, 1:-1, 1:-1] - Z[1, 1:-1, 1:-1])
     cd = 0.25 * (depth[:-1, :-1] + depth[1:, :-1] + depth[:-1, 1:] + depth[1:, 1:])
     assert np.allclose(J.sum(axis=0) * 100.0, cd, rtol=1e-12)
+
+
+def _hill2d(smoothing):
+    """Tests/Unit_Tests/test_Mesh.cpp:56-69: 32 x 1 x 16 over (-2560, 0, 0) to
+    (2560, 40, 1000), periodic x and y, the Agnesi hill 100 m / 500 m,
+    uniform levels, terrain.gamma_m 0.5."""
+    geom = M.geometry((32, 1, 16), (-2560.0, 0.0, 0.0), (2560.0, 40.0, 1000.0), (1, 1))
+    return M.build_mesh(geom, (-2560.0, 0.0, 0.0), (2560.0, 40.0, 1000.0), M.make_zlevels(16, 1000.0),
+                        M.Terrain(kind="agnesi", hill_height=100.0, hill_half_width=500.0), smoothing=smoothing)
+
+
+@pytest.mark.parametrize("smoothing,name", [(0, "btf"), (1, "stf"), (2, "sullivan")])
+def test_meshes_equal_erf(smoothing, name):
+    """Mesh.MetricsEqualErf (Tests/Unit_Tests/test_Mesh.cpp:642-673): z_phys
+    and detJ within 1e-13 and the node heights within 1e-12 of ERF's own
+    (Tests/Unit_Tests/Gold/erf_hill2d_*), for BTF, STF and Sullivan; STF
+    capped at 2 of the 16 levels, as ERF reported."""
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "erf_hill2d.npz"))
+    m = _hill2d(smoothing)
+    zc = m.z_cc[1:-1, 1:-1, 1:-1]
+    J = m.detJ[1:-1, 1:-1, 1:-1]
+    zn = m.z_nd[1:-1, 1:-1, 1:-1]
+    for got, key, tol in ((zc, "z_phys", 1e-13), (J, "detJ", 1e-13), (zn, "z_nd", 1e-12)):
+        want = gold[f"{name}_{key}"]
+        assert got.shape == want.shape, (key, got.shape, want.shape)
+        err = float(np.max(np.abs(got - want)))
+        assert err <= tol * float(np.max(np.abs(want))), (name, key, err)
+    if smoothing == 1:
+        assert m.stf_capped_levels == 2
+
+
+def test_xyz_terrain_on_a_periodic_ramp(tmp_path):
+    """Tests/Unit_Tests/test_Mesh.cpp:593-634: points z = 2 i + j on a 100 m
+    grid (13 x 11), a 12 x 10 x 4 mesh over 1200 x 1000 m periodic in x:
+    IDW returns each point exactly on its own node, and every node (ghosts
+    too) equals the node at i mod 12 (periodic x), j clamped (y)."""
+    f = tmp_path / "ramp.xyz"
+    f.write_text("# x y z\n" + "".join(f"{100.0 * i} {100.0 * j} {2.0 * i + j}\n"
+                                       for j in range(11) for i in range(13)))
+    pts = M.read_xyz_points(str(f))
+    assert pts[0].size == 143 and pts[3] == 0
+    geom = M.geometry((12, 10, 4), (0.0, 0.0, 0.0), (1200.0, 1000.0, 400.0), (1, 0))
+    h = M.ground_heights(M.Terrain(kind="xyz", points=pts), geom, (0.0, 0.0, 0.0), (1200.0, 1000.0, 400.0))
+    for b, j in enumerate(range(-1, 12)):
+        for a, i in enumerate(range(-1, 14)):
+            want = 2.0 * (i % 12) + max(0, min(10, j))
+            assert h[b, a] == want, (i, j, h[b, a], want)
+
+
+def test_idw_closed_forms():
+    """Equidistant neighbours average; ties at the k-th distance go to the
+    lower point index; a point within 1e-6 m gives its own height; the
+    weights are d^-p of the k nearest."""
+    x = np.array([0.0, 1.0, 0.0, 1.0, 5.0])
+    y = np.array([0.0, 0.0, 1.0, 1.0, 5.0])
+    z = np.array([1.0, 2.0, 3.0, 4.0, 100.0])
+    P = (x, y, z)
+    assert M.idw_heights(P, [0.5], [0.5], k=4)[0] == (1.0 + 2.0 + 3.0 + 4.0) / 4
+    assert M.idw_heights(P, [0.5], [0.5], k=1)[0] == 1.0           # four ties: the lowest index
+    assert M.idw_heights(P, [1.0 + 1e-7], [0.0], k=3)[0] == 2.0    # within 1e-6 m of point 1
+    d2 = np.array([0.25 ** 2 + 0.0, 0.75 ** 2, 0.25 ** 2 + 1.0])   # query (0.25, 0) to points 0, 1, 2
+    w = d2 ** -1.5
+    want = (w[0] * 1.0 + w[1] * 2.0 + w[2] * 3.0) / w.sum()
+    got = M.idw_heights(P, [0.25], [0.0], k=3, power=3.0)[0]
+    assert abs(got - want) <= 1e-15 * want
+    with pytest.raises(ValueError, match="idw_neighbors"):
+        M.idw_heights(P, [0.0], [0.0], k=6)
