@@ -296,6 +296,33 @@ class DriverResultC(C.Structure):
                 ("nmon", C.c_int32)]
 
 
+class SequencingOptionsC(C.Structure):
+    """``gust_sequencing_options``."""
+
+    _fields_ = [("enabled", C.c_int32), ("drop_orders", C.c_double), ("monitor_tol", C.c_double),
+                ("hold", C.c_int32), ("cfl_previous", C.c_int32), ("given", C.c_uint32)]
+
+
+class LevelControlC(C.Structure):
+    """``gust_level_control``."""
+
+    _fields_ = [("loose", C.POINTER(SequencingOptionsC)), ("seeded", C.c_int32), ("seed_ref", C.c_double * 6),
+                ("release_hold", C.c_int32), ("start_cfl", C.c_double)]
+
+
+class LevelResultC(C.Structure):
+    """``gust_level_result``."""
+
+    _fields_ = [("driver", DriverResultC), ("stop", C.c_int32), ("hold", C.c_int32), ("res_ref", C.c_double * 6),
+                ("reason", C.c_char * 256)]
+
+
+class ProlongResultC(C.Structure):
+    """``gust_prolong_result``."""
+
+    _fields_ = [("projection", PressureCorrectionResultC), ("tke_clipped", C.c_int64)]
+
+
 def _load() -> C.CDLL:
     # One HIP runtime per process: torch ships its own libamdhip64 and the
     # device tensors, streams and events handed to the library come from it.
@@ -420,6 +447,19 @@ _sig = {
     "gust_simplec_norms": (C.c_int, [C.c_void_p, _P(C.c_double), _P(C.c_double), _P(C.c_int32)]),
     "gust_simplec_nu": (C.c_int, [C.c_void_p, _P(Array4)]),
     "gust_steady_solve": (C.c_int, [C.c_void_p, _P(StateC), _P(DriverOptionsC), _P(DriverResultC), C.c_void_p]),
+    "gust_steady_solve_level": (C.c_int, [C.c_void_p, _P(StateC), _P(DriverOptionsC), _P(LevelControlC),
+                                          _P(LevelResultC), C.c_void_p]),
+    "gust_sequencing_default_options": (C.c_int, [_P(SequencingOptionsC)]),
+    "gust_check_sequencing_options": (C.c_int, [_P(SequencingOptionsC), _P(SteadyParamsC), C.c_int32, C.c_int32,
+                                                C.c_int32, _P(C.c_int32), C.c_char_p, C.c_int64]),
+    "gust_loose_stop": (C.c_int, [C.c_void_p, _P(SequencingOptionsC), C.c_int32, _P(C.c_int32), C.c_char_p,
+                                  C.c_int64]),
+    "gust_steady_references": (C.c_int, [C.c_void_p, _P(C.c_double), C.c_int32]),
+    "gust_prolong_fields": (C.c_int, [_P(GeometryC), _P(GeometryC), C.c_int32, _P(StateC), _P(StateC), C.c_void_p]),
+    "gust_restrict_state": (C.c_int, [_P(GeometryC), _P(MeshC), _P(StateC), _P(GeometryC), _P(MeshC), C.c_int32,
+                                      _P(StateC), _P(Array4), C.c_void_p]),
+    "gust_prolong_state": (C.c_int, [C.c_void_p, C.c_void_p, _P(GeometryC), C.c_int32, _P(StateC), _P(StateC),
+                                     C.c_double, _P(ProlongResultC), C.c_void_p]),
     "gust_steady_default_params": (C.c_int, [_P(SteadyParamsC)]),
     "gust_steady_validate": (C.c_int, [_P(SteadyParamsC), _P(C.c_int32), C.c_char_p, C.c_int64]),
     "gust_steady_create": (C.c_int, [_P(SteadyParamsC), C.c_int32, _P(C.c_char_p), C.c_int32, _P(C.c_char_p),

@@ -45,6 +45,46 @@ __global__ __launch_bounds__(kThreads) void conv_update_kernel(Box3 b, V c1, CV 
     }
 }
 
+// k >= floor on the valid cells, the count of raised cells
+// (Source/Transfer.cpp:368-384).
+__global__ __launch_bounds__(kThreads) void clip_floor_kernel(Box3 b, V k, double floor, unsigned long long* count)
+{
+    unsigned long long mine = 0;
+    GUST_GRID_LOOP(b)
+    {
+        int i, j, kk;
+        b.at(t, i, j, kk);
+        if (k(i, j, kk) < floor) {
+            k(i, j, kk) = floor;
+            ++mine;
+        }
+    }
+    if (mine) { atomicAdd(count, mine); }
+}
+
+// d = 1 - mask on one velocity's faces (the projection's denominators,
+// Source/Transfer.cpp:388-399: d.mult(-1), d.plus(1)).
+__global__ __launch_bounds__(kThreads) void one_minus_kernel(Box3 b, V d, CV m)
+{
+    GUST_GRID_LOOP(b)
+    {
+        int i, j, k;
+        b.at(t, i, j, k);
+        d(i, j, k) = -1.0 * m(i, j, k) + 1.0;
+    }
+}
+
+// dst = src over a box (both views cover it).
+__global__ __launch_bounds__(kThreads) void copy_box_kernel(Box3 b, V dst, CV src)
+{
+    GUST_GRID_LOOP(b)
+    {
+        int i, j, k;
+        b.at(t, i, j, k);
+        dst(i, j, k) = src(i, j, k);
+    }
+}
+
 // A device field owned by the stage.
 struct Field {
     double* p = nullptr;
@@ -420,53 +460,170 @@ int gust_simplec_nu(const gust_simplec* s, gust_array4* nu)
     });
 }
 
+// The steady loop (Source/Driver.cpp:403-468) with a level's seeding and
+// start (Source/Driver.cpp:275-313) and, on a coarse level of grid
+// sequencing, the loose stop after each step (Source/Sequencing.cpp:361-390).
+static void solve_loop(const char* op, gust_simplec* s, gust_state* q, const gust_driver_options* opt,
+                       const gust_level_control* lc, gust_level_result* out, void* stream)
+{
+    if (!s || !q || !opt || !out) { GUST_FAIL("gust: %s: null argument", op); }
+    SimplecStage& st = *as(s);
+    if (opt->max_iterations < 1) { GUST_FAIL("gust: %s: steady.max_iterations must be >= 1", op); }
+    if (!(opt->cfl0 > 0 && std::isfinite(opt->cfl0)) || !(opt->cfl_max >= opt->cfl0 && std::isfinite(opt->cfl_max))) {
+        GUST_FAIL("gust: %s: steady.cfl0 must be > 0 and steady.cfl_max >= steady.cfl0", op);
+    }
+    const char* names[6] = {"u", "v", "w", "p", "theta", "tke"};
+    const char* mnames[3] = {"max_speed", "mean_tke", "mean_ustar"};
+    const double mscales[3] = {st.opt.scales.velocity, st.opt.scales.tke, 0.1};
+    const int nmon = st.wall ? 3 : 2;
+    gust_steady* crit = nullptr;
+    check_rc(gust_steady_create(&opt->criterion, 6, names, nmon, mnames, mscales, &crit));
+    struct Holder {
+        gust_steady* c;
+        ~Holder() { gust_steady_destroy(c); }
+    } hold_c{crit};
+    int hold = opt->criterion.residual_ref_iterations;
+    double cfl = opt->cfl0, r = 0;
+    if (lc) {
+        if (lc->seeded) { check_rc(gust_steady_seed_reference(crit, lc->seed_ref, 6)); }
+        if (lc->release_hold) {
+            bool seeded = false;
+            for (int n = 0; n < 6 && lc->seeded; ++n) { seeded = seeded || lc->seed_ref[n] > 0; }
+            if (!seeded) {
+                GUST_FAIL("gust: Driver::release_reference_hold: the reference is not seeded (seed_reference first)");
+            }
+            hold = 0;
+        }
+        if (lc->start_cfl != 0) {
+            if (hold > 0) { GUST_FAIL("gust: Driver::start_cfl: the reference hold is on"); }
+            if (!std::isfinite(lc->start_cfl) || !(lc->start_cfl > 0) || lc->start_cfl > opt->cfl_max) {
+                GUST_FAIL("gust: Driver::start_cfl: %.3e is not finite, > 0 and <= steady.cfl_max = %.3e",
+                          lc->start_cfl, opt->cfl_max);
+            }
+            cfl = std::max(lc->start_cfl, opt->cfl0);
+        }
+    }
+    const gust_sequencing_options* loose = lc ? lc->loose : nullptr;
+    Field dt;
+    dt.alloc(loc_box(st.g, GUST_LOC_CELL), 1, hipStream_t(stream));
+    gust_array4 nu = st.phys.use_closure ? st.cf[0].v : *st.phys.nu;   // zero before the first residual
+    gust_steady_result v{};
+    int it = 0;
+    out->stop = GUST_LEVEL_MAX_ITERATIONS;
+    out->reason[0] = '\0';
+    check_rc(gust_simplec_init(s, q, stream));
+    while (it < opt->max_iterations) {
+        check_rc(gust_local_dtau(&st.g, q->f[0], q->f[1], q->f[2], &st.z, &st.J, &nu, cfl, &dt.v, stream));
+        iterate(st, q, &dt.v, nullptr, stream);
+        ++it;
+        check_rc(gust_steady_add(crit, st.res, 6, st.mon, nmon));
+        check_rc(gust_steady_relative_residual(crit, st.res, 6, hold, r, &r));
+        check_rc(gust_steady_verdict(crit, &v, nullptr, 0));
+        if (v.status == GUST_STEADY_STEADY) {
+            out->stop = GUST_LEVEL_STEADY;
+            break;
+        }
+        if (v.status == GUST_STEADY_DIVERGED) {
+            out->stop = GUST_LEVEL_DIVERGED;
+            break;
+        }
+        check_rc(gust_steady_next_cfl(crit, hold, cfl, r, opt->cfl0, opt->cfl_max, opt->ser_exponent,
+                                      opt->ser_max_growth, &cfl));
+        if (loose) {
+            int32_t stop = 0;
+            check_rc(gust_loose_stop(crit, loose, hold, &stop, out->reason, int64_t(sizeof out->reason)));
+            if (stop) {
+                out->stop = GUST_LEVEL_LOOSE;
+                break;
+            }
+        }
+    }
+    gust_driver_result& res = out->driver;
+    res.iterations = it;
+    res.status = v.status;
+    res.cfl = cfl;
+    res.relative_residual = r;
+    for (int n = 0; n < 6; ++n) { res.res[n] = st.res[n]; }
+    for (int n = 0; n < 3; ++n) { res.mon[n] = st.mon[n]; }
+    res.nmon = nmon;
+    out->hold = hold;
+    check_rc(gust_steady_references(crit, out->res_ref, 6));
+}
+
 int gust_steady_solve(gust_simplec* s, gust_state* q, const gust_driver_options* opt, gust_driver_result* result,
                       void* stream)
 {
     return guarded([&] {
-        const char* op = "Driver::run";
-        if (!s || !q || !opt || !result) { GUST_FAIL("gust: %s: null argument", op); }
-        SimplecStage& st = *as(s);
-        if (opt->max_iterations < 1) { GUST_FAIL("gust: %s: steady.max_iterations must be >= 1", op); }
-        if (!(opt->cfl0 > 0 && std::isfinite(opt->cfl0)) || !(opt->cfl_max >= opt->cfl0 && std::isfinite(opt->cfl_max))) {
-            GUST_FAIL("gust: %s: steady.cfl0 must be > 0 and steady.cfl_max >= steady.cfl0", op);
+        if (!result) { GUST_FAIL("gust: Driver::run: null argument"); }
+        gust_level_result out{};
+        solve_loop("Driver::run", s, q, opt, nullptr, &out, stream);
+        *result = out.driver;
+    });
+}
+
+int gust_steady_solve_level(gust_simplec* s, gust_state* q, const gust_driver_options* opt,
+                            const gust_level_control* lc, gust_level_result* result, void* stream)
+{
+    return guarded([&] {
+        if (!lc) { GUST_FAIL("gust: grid sequencing: null level control"); }
+        solve_loop("Driver::run", s, q, opt, lc, result, stream);
+    });
+}
+
+int gust_prolong_state(gust_simplec* fine_stage, gust_poisson* solver, const gust_geometry* coarse_geom,
+                       int32_t ratio, const gust_state* coarse, gust_state* fine, double tke_floor,
+                       gust_prolong_result* result, void* stream)
+{
+    return guarded([&] {
+        const char* op = "prolong_state";
+        if (!fine_stage || !solver || !coarse_geom || !coarse || !fine || !result) {
+            GUST_FAIL("gust: %s: null argument", op);
         }
-        const char* names[6] = {"u", "v", "w", "p", "theta", "tke"};
-        const char* mnames[3] = {"max_speed", "mean_tke", "mean_ustar"};
-        const double mscales[3] = {st.opt.scales.velocity, st.opt.scales.tke, 0.1};
-        const int nmon = st.wall ? 3 : 2;
-        gust_steady* crit = nullptr;
-        check_rc(gust_steady_create(&opt->criterion, 6, names, nmon, mnames, mscales, &crit));
-        struct Holder {
-            gust_steady* c;
-            ~Holder() { gust_steady_destroy(c); }
-        } hold_c{crit};
-        Field dt;
-        dt.alloc(loc_box(st.g, GUST_LOC_CELL), 1, hipStream_t(stream));
-        gust_array4 nu = st.phys.use_closure ? st.cf[0].v : *st.phys.nu;   // zero before the first residual
-        const int hold = opt->criterion.residual_ref_iterations;
-        double cfl = opt->cfl0, r = 0;
-        gust_steady_result v{};
-        int it = 0;
-        check_rc(gust_simplec_init(s, q, stream));
-        while (it < opt->max_iterations) {
-            check_rc(gust_local_dtau(&st.g, q->f[0], q->f[1], q->f[2], &st.z, &st.J, &nu, cfl, &dt.v, stream));
-            iterate(st, q, &dt.v, nullptr, stream);
-            ++it;
-            check_rc(gust_steady_add(crit, st.res, 6, st.mon, nmon));
-            check_rc(gust_steady_relative_residual(crit, st.res, 6, hold, r, &r));
-            check_rc(gust_steady_verdict(crit, &v, nullptr, 0));
-            if (v.status != GUST_STEADY_NOT_STEADY) { break; }
-            check_rc(gust_steady_next_cfl(crit, hold, cfl, r, opt->cfl0, opt->cfl_max, opt->ser_exponent,
-                                          opt->ser_max_growth, &cfl));
+        if (!std::isfinite(tke_floor) || !(tke_floor >= 0)) {
+            GUST_FAIL("gust: prolong_state: tke_floor must be finite and >= 0 (turb.tke_floor)");
         }
-        result->iterations = it;
-        result->status = v.status;
-        result->cfl = cfl;
-        result->relative_residual = r;
-        for (int n = 0; n < 6; ++n) { result->res[n] = st.res[n]; }
-        for (int n = 0; n < 3; ++n) { result->mon[n] = st.mon[n]; }
-        result->nmon = nmon;
+        SimplecStage& st = *as(fine_stage);
+        hipStream_t hs = hipStream_t(stream);
+        std::memset(result, 0, sizeof *result);
+        check_rc(gust_prolong_fields(&st.g, coarse_geom, ratio, coarse, fine, stream));
+        {
+            const Box3 b = box3(loc_box(st.g, GUST_LOC_CELL));
+            auto* cnt = static_cast<unsigned long long*>(Workspace::get(hs, 204, sizeof(unsigned long long)));
+            GUST_HIP(hipMemsetAsync(cnt, 0, sizeof *cnt, hs));
+            clip_floor_kernel<<<blocks_for(b.npts()), kThreads, 0, hs>>>(b, dview<double>(fine->f[5]), tke_floor, cnt);
+            GUST_HIP(hipGetLastError());
+            unsigned long long h = 0;
+            GUST_HIP(hipMemcpyAsync(&h, cnt, sizeof h, hipMemcpyDeviceToHost, hs));
+            GUST_HIP(hipStreamSynchronize(hs));
+            result->tke_clipped = int64_t(h);
+        }
+        check_rc(gust_fill_ghosts(&st.g, fine, &st.mesh, &st.phys, st.wall ? &st.ss.v : nullptr, stream));
+        // The projection: the correction with d = 1 off the Dirichlet rows;
+        // the stage's d fields are scratch until its next predictor.
+        for (int c = 0; c < 3; ++c) {
+            const Box3 b = box3(loc_box(st.g, 1 << c));
+            one_minus_kernel<<<blocks_for(b.npts()), kThreads, 0, hs>>>(b, dview<double>(&st.d[c].v),
+                                                                          dview<const double>(&st.mask[c].v));
+            GUST_HIP(hipGetLastError());
+        }
+        const gust_array4& pv = *fine->f[3];
+        IBox pb;
+        for (int d = 0; d < 3; ++d) {
+            pb.lo[d] = pv.begin[d];
+            pb.hi[d] = pv.end[d] - 1;
+        }
+        Field keep;
+        keep.alloc(pb, 1, hs);
+        const Box3 b = box3(pb);
+        copy_box_kernel<<<blocks_for(b.npts()), kThreads, 0, hs>>>(b, dview<double>(&keep.v), dview<const double>(&pv));
+        GUST_HIP(hipGetLastError());
+        check_rc(gust_pressure_correction(solver, &st.g, fine->f[0], fine->f[1], fine->f[2], fine->f[3], &st.z, &st.ax,
+                                          &st.ay, &st.az, &st.J, &st.d[0].v, &st.d[1].v, &st.d[2].v, st.pbc, 1.0,
+                                          nullptr, 0, &result->projection, stream));
+        copy_box_kernel<<<blocks_for(b.npts()), kThreads, 0, hs>>>(b, dview<double>(fine->f[3]),
+                                                                    dview<const double>(&keep.v));
+        GUST_HIP(hipGetLastError());
+        GUST_HIP(hipStreamSynchronize(hs));   // keep is freed on return
     });
 }
 

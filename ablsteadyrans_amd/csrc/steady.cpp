@@ -296,6 +296,113 @@ bool load(const void* buf, int64_t size, History& h)
     return r.ok && r.at == r.n;
 }
 
+// Grid sequencing's loose stop (Source/Sequencing.cpp:196-252): after the
+// last recorded iteration, (a) every residual drop_orders below its floored
+// reference or below residual_tol, once the level's hold is over (a
+// released seeded level: min(3, residual_ref_iterations) iterations); or
+// (b) with monitor_tol > 0 and after residual_ref_iterations iterations,
+// every monitor's full window within monitor_tol of max(|latest|, scale).
+// Never on an empty or non-finite history.
+std::string fmt_e(double x)
+{
+    char b[64];
+    std::snprintf(b, sizeof b, "%.3e", x);
+    return b;
+}
+
+constexpr int kReleasedMinIterations = 3;
+
+bool loose_stop(const Steady& c, const gust_sequencing_options& o, int hold, std::string& reason)
+{
+    const History& h = c.h;
+    const gust_steady_params& p = c.p;
+    if (h.count == 0 || h.nonfinite) { return false; }
+    const int first = hold > 0 ? hold : std::min(p.residual_ref_iterations, kReleasedMinIterations);
+    if (h.count >= first && !h.res_last.empty()) {
+        const double drop = std::pow(10.0, -o.drop_orders);
+        bool all = true;
+        for (size_t q = 0; q < h.res_last.size(); ++q) {
+            const double ref = std::max(h.res_ref[q], p.residual_tol);
+            const double r = h.res_last[q];
+            all = all && (r <= ref * drop || r < p.residual_tol);
+        }
+        if (all) {
+            reason = "every residual " + fmt_e(o.drop_orders) + " orders below its reference";
+            return true;
+        }
+    }
+    const size_t need = size_t(p.monitor_window) + 1;
+    if (o.monitor_tol > 0 && h.count >= p.residual_ref_iterations && !c.mon.empty() &&
+        h.mon_hist.size() == c.mon.size()) {
+        bool all = true;
+        for (size_t m = 0; m < c.mon.size() && all; ++m) {
+            const auto& w = h.mon_hist[m];
+            if (w.size() < need) {
+                all = false;
+                break;
+            }
+            const double last = w.back();
+            double change = 0;
+            for (const double x : w) { change = std::max(change, std::fabs(x - last)); }
+            all = change <= o.monitor_tol * std::max(std::fabs(last), c.mon[m].scale);
+        }
+        if (all) {
+            reason = "every monitor within " + fmt_e(o.monitor_tol) + " over " + std::to_string(p.monitor_window) +
+                     " iterations";
+            return true;
+        }
+    }
+    return false;
+}
+
+// Source/Sequencing.cpp:118-176 (check_sequencing_options): each problem
+// names its key; the ranges are checked only with sequencing on.
+std::vector<std::string> check_sequencing(const gust_sequencing_options& o, const gust_steady_params& strict,
+                                          int levels, int nx, int ny)
+{
+    std::vector<std::string> err;
+    if (o.enabled) {
+        if (!std::isfinite(o.drop_orders) || !(o.drop_orders > 0) ||
+            !(std::isfinite(strict.residual_drop_orders) && o.drop_orders <= strict.residual_drop_orders)) {
+            err.emplace_back("steady.sequencing_drop_orders = " + fmt_e(o.drop_orders) +
+                             ": must be > 0 and <= steady.residual_drop_orders = " +
+                             fmt_e(strict.residual_drop_orders) +
+                             " (the loose stop of the coarse levels is not stricter than the strict one)");
+        }
+        if (!std::isfinite(o.monitor_tol) || !(o.monitor_tol >= 0) || !(o.monitor_tol < 1)) {
+            err.emplace_back("steady.sequencing_monitor_tol = " + fmt_e(o.monitor_tol) +
+                             ": must be 0 (off) or > 0 and < 1");
+        }
+    }
+    if (o.enabled && levels < 2) {
+        err.emplace_back("steady.sequencing = 1 needs mesh.hierarchy_levels >= 2 (it is " + std::to_string(levels) +
+                         ")");
+    }
+    if (o.enabled && levels >= 2 && levels <= GUST_MAX_HIERARCHY_LEVELS) {
+        const int r = 1 << (levels - 1);
+        const int cx = nx / r, cy = ny / r;
+        if (cx < 2 || cy < 2) {
+            err.emplace_back("steady.sequencing = 1 needs at least 2 cells in x and in y on the coarsest level: "
+                             "mesh.n_cell x and y / 2^(mesh.hierarchy_levels - 1) = " +
+                             std::to_string(cx) + " " + std::to_string(cy) + "; use fewer levels");
+        }
+    }
+    if (o.enabled && o.cfl_previous && o.hold == 1) {
+        err.emplace_back("steady.sequencing_cfl_start = previous needs steady.sequencing_hold = 0 "
+                         "(the hold runs the first steady.residual_ref_iterations iterations at steady.cfl0)");
+    }
+    if (!o.enabled) {
+        const char* keys[4] = {"sequencing_drop_orders", "sequencing_monitor_tol", "sequencing_hold",
+                               "sequencing_cfl_start"};
+        for (int b = 0; b < 4; ++b) {
+            if (o.given & (1u << b)) {
+                err.emplace_back(std::string("steady.") + keys[b] + " has no effect unless steady.sequencing = 1");
+            }
+        }
+    }
+    return err;
+}
+
 Steady* as(gust_steady* c) { return reinterpret_cast<Steady*>(c); }
 const Steady* as(const gust_steady* c) { return reinterpret_cast<const Steady*>(c); }
 
@@ -430,6 +537,18 @@ int gust_steady_residual_thresholds(const gust_steady* c, double* out, int32_t n
     });
 }
 
+int gust_steady_references(const gust_steady* c, double* out, int32_t n)
+{
+    return guarded([&] {
+        const Steady& s = need(c, "SteadyCriterion::history");
+        if (!out || n != int32_t(s.h.res_ref.size())) {
+            GUST_FAIL("gust: SteadyCriterion::history: %d references asked for, the criterion has %d", int(n),
+                      int(s.h.res_ref.size()));
+        }
+        for (int q = 0; q < n; ++q) { out[q] = s.h.res_ref[size_t(q)]; }
+    });
+}
+
 int gust_steady_seed_reference(gust_steady* c, const double* ref, int32_t n)
 {
     return guarded([&] {
@@ -508,6 +627,40 @@ int gust_steady_next_cfl(const gust_steady* c, int32_t hold, double cfl, double 
         }
         const int rc = gust_ser_cfl(cfl, r, cfl0, cfl_max, ser_exponent, max_growth, out);
         if (rc != 0) { throw Fail{rc}; }
+    });
+}
+
+int gust_sequencing_default_options(gust_sequencing_options* o)
+{
+    return guarded([&] {
+        if (!o) { GUST_FAIL("gust: SequencingOptions: null output"); }
+        std::memset(o, 0, sizeof *o);
+        o->drop_orders = 2;
+    });
+}
+
+int gust_check_sequencing_options(const gust_sequencing_options* o, const gust_steady_params* strict,
+                                  int32_t levels, int32_t nx, int32_t ny, int32_t* count, char* buf, int64_t cap)
+{
+    return guarded([&] {
+        if (!o || !strict || !count) { GUST_FAIL("gust: check_sequencing_options: null argument"); }
+        const auto err = check_sequencing(*o, *strict, levels, nx, ny);
+        std::string all;
+        for (size_t n = 0; n < err.size(); ++n) { all += (n ? "\n" : "") + err[n]; }
+        *count = int32_t(err.size());
+        put_text(all, buf, cap);
+    });
+}
+
+int gust_loose_stop(const gust_steady* c, const gust_sequencing_options* o, int32_t hold_iterations, int32_t* stop,
+                    char* reason, int64_t cap)
+{
+    return guarded([&] {
+        const Steady& s = need(c, "loose_stop");
+        if (!o || !stop) { GUST_FAIL("gust: loose_stop: null argument"); }
+        std::string why;
+        *stop = loose_stop(s, *o, hold_iterations, why) ? 1 : 0;
+        put_text(why, reason, cap);
     });
 }
 

@@ -470,10 +470,13 @@ def _sullivan_levels(Z, zlevels, nx, ny, nz):
 
 
 def build_mesh(geom: Geometry, prob_lo, prob_hi, zlevels: np.ndarray, terrain: Terrain, smoothing: int = 0,
-               gamma_m: float = 0.5) -> TileMesh:
+               gamma_m: float = 0.5, ground: np.ndarray | None = None) -> TileMesh:
     """The terrain-following mesh of one tile (Source/Mesh.cpp:107-140):
     terrain.smoothing = 0 BTF, 1 STF (gamma_m: the minimum layer thickness
-    ratio), 2 Sullivan. The STF capped-level count is mesh.stf_capped_levels."""
+    ratio), 2 Sullivan. The STF capped-level count is mesh.stf_capped_levels.
+    ground: the node heights of the ground slab grown by one ghost node
+    ((ny + 3, nx + 3), index (j + 1, i + 1)) in place of the terrain's
+    (Mesh::build_from_ground, Source/Mesh.cpp:118-151)."""
     if smoothing not in (0, 1, 2):
         raise ValueError("gust: terrain.smoothing must be 0 (BTF), 1 (STF) or 2 (Sullivan)")
     if not (np.isfinite(gamma_m) and 0 < gamma_m < 1):
@@ -485,7 +488,12 @@ def build_mesh(geom: Geometry, prob_lo, prob_hi, zlevels: np.ndarray, terrain: T
     dzeta = geom.dx[2]
     # Z[k+1, j+1, i+1] = z_nd(i, j, k), i in [-1, nx+1], k in [-1, nz+1]
     Z = np.zeros((nz + 3, ny + 3, nx + 3))
-    Z[1] = ground_heights(terrain, geom, prob_lo, prob_hi)
+    if ground is not None:
+        if ground.shape != (ny + 3, nx + 3):
+            raise ValueError("the ground slab must be (ny + 3, nx + 3) nodes")
+        Z[1] = ground
+    else:
+        Z[1] = ground_heights(terrain, geom, prob_lo, prob_hi)
     # setDomainBndry(bogus): nodes outside the domain along non-periodic axes
     Z[0] = BOGUS
     Z[nz + 2] = BOGUS
@@ -606,3 +614,134 @@ def _fill_ghost_columns(Z, nx, ny, nz, perx, pery):
         Z[1:nz + 2, ny + 2, :] = Z[1:nz + 2, ny + 1, :]
     Z[0] = 2.0 * Z[1] - Z[2]
     Z[nz + 2] = Z[nz + 1] + 1 * (Z[nz + 1] - Z[nz])
+
+
+# The horizontal grid hierarchy (Source/MeshHierarchy.H/.cpp,
+# Source/SmoothSurface.H/.cpp): levels l = 0 .. L-1, level l has 2^l times
+# fewer cells in x and y and the same z-levels; its ground is level 0's at
+# its nodes, smoothed on its own grid by passes (2^l - 1) passes of
+# h += beta (sum of the four neighbours - 4 h) with beta = 1/8; its mesh is
+# built from that ground with the run's smoothing.
+
+MAX_HIERARCHY_LEVELS = 10
+MAX_HIERARCHY_PASSES = 64
+HIERARCHY_BETA = 0.125
+
+
+def hierarchy_ratio(level: int) -> int:
+    """2^level (Source/MeshHierarchy.cpp:56-63)."""
+    if level < 0 or level >= MAX_HIERARCHY_LEVELS:
+        raise ValueError(f"gust: hierarchy level {level} outside [0, {MAX_HIERARCHY_LEVELS - 1}] "
+                         "(mesh.hierarchy_levels)")
+    return 1 << level
+
+
+def ground_passes(level: int, passes_per_level: int) -> int:
+    """passes_per_level (2^l - 1) (Source/MeshHierarchy.cpp:65-68)."""
+    return passes_per_level * (hierarchy_ratio(level) - 1)
+
+
+def check_hierarchy_inputs(n_cell, levels: int, passes: int = 1, passes_given: bool = False) -> list:
+    """Source/MeshHierarchy.cpp:70-111 for one box: each problem names its key."""
+    err = []
+    L = int(levels)
+    nx, ny = int(n_cell[0]), int(n_cell[1])
+    if L < 1 or L > MAX_HIERARCHY_LEVELS:
+        err.append(f"mesh.hierarchy_levels must be an integer from 1 to {MAX_HIERARCHY_LEVELS} (got {L})")
+    elif L > 1:
+        r = 1 << (L - 1)
+        if nx >= 1 and ny >= 1 and (nx % r != 0 or ny % r != 0):
+            err.append(f"mesh.hierarchy_levels = {L} needs mesh.n_cell x and y divisible by 2^{L - 1} = {r} "
+                       f"(got {nx} {ny})")
+        if nx >= 1 and ny >= 1 and nx % r == 0 and ny % r == 0 and nx // r == 1 and ny // r == 1:
+            err.append(f"mesh.hierarchy_levels = {L} leaves the coarsest level 1 x 1 cells (mesh.n_cell {nx} {ny}); "
+                       "use fewer levels")
+    if passes < 0 or passes > MAX_HIERARCHY_PASSES:
+        err.append(f"terrain.hierarchy_passes must be an integer from 0 to {MAX_HIERARCHY_PASSES} (got {passes})")
+    if passes_given and L == 1:
+        err.append("terrain.hierarchy_passes has no effect unless mesh.hierarchy_levels > 1")
+    return err
+
+
+def fill_surface_ghosts(h: np.ndarray, nx: int, ny: int, perx: bool, pery: bool) -> None:
+    """The ghost nodes of a ground slab ((ny + 3, nx + 3), index (j + 1, i +
+    1)): periodic images, else the nearest domain node
+    (Source/SmoothSurface.cpp:50-77)."""
+    _fill_slab_images(h, nx, ny, perx, pery)
+    if not perx:
+        h[:, 0] = h[:, 1]
+        h[:, nx + 2] = h[:, nx + 1]
+    if not pery:
+        h[0, :] = h[1, :]
+        h[ny + 2, :] = h[ny + 1, :]
+
+
+def _fill_slab_images(h, nx, ny, perx, pery):
+    if perx:
+        h[:, 0] = h[:, nx]
+        h[:, nx + 2] = h[:, 2]
+    if pery:
+        h[0, :] = h[ny, :]
+        h[ny + 2, :] = h[2, :]
+
+
+def smooth_surface(h: np.ndarray, nx: int, ny: int, perx: bool, pery: bool, passes: int, beta: float) -> None:
+    """passes passes of h += beta (h(i+1) + h(i-1) + h(j+1) + h(j-1) - 4 h) on
+    the valid nodes of a ground slab, in place; across a non-periodic face
+    the neighbour is the edge node itself; the ghosts refilled after
+    (Source/SmoothSurface.cpp:9-48)."""
+    if passes < 0 or not np.isfinite(beta) or not (0 <= beta < 0.25):
+        raise ValueError("gust: smooth_surface needs passes >= 0 and 0 <= beta < 1/4")
+    if passes == 0:
+        return
+    i = np.arange(0, nx + 1)
+    j = np.arange(0, ny + 1)
+    im = np.where((~np.asarray(perx)) & (i == 0), i, i - 1) + 1
+    ip = np.where((~np.asarray(perx)) & (i == nx), i, i + 1) + 1
+    jm = np.where((~np.asarray(pery)) & (j == 0), j, j - 1) + 1
+    jp = np.where((~np.asarray(pery)) & (j == ny), j, j + 1) + 1
+    I, J = np.meshgrid(i + 1, j + 1)
+    IM, _ = np.meshgrid(im, j)
+    IP, _ = np.meshgrid(ip, j)
+    _, JM = np.meshgrid(i, jm)
+    _, JP = np.meshgrid(i, jp)
+    for _ in range(passes):
+        _fill_slab_images(h, nx, ny, perx, pery)
+        ho = h.copy()
+        h[J, I] = ho[J, I] + beta * (ho[J, IP] + ho[J, IM] + ho[JP, I] + ho[JM, I] - 4.0 * ho[J, I])
+    fill_surface_ghosts(h, nx, ny, perx, pery)
+
+
+def inject_ground(z_fine: np.ndarray, r: int, nxc: int, nyc: int, perx: bool, pery: bool) -> np.ndarray:
+    """The coarse ground slab: node (I, J) takes fine node (r I, r J), the
+    ghosts filled (Source/MeshHierarchy.cpp:128-150). z_fine: a mesh's z_nd
+    (its k = 0 slab is read)."""
+    h = np.zeros((nyc + 3, nxc + 3))
+    h[1:nyc + 2, 1:nxc + 2] = z_fine[1, 1:r * nyc + 2:r, 1:r * nxc + 2:r]
+    fill_surface_ghosts(h, nxc, nyc, perx, pery)
+    return h
+
+
+def hierarchy(tile: TileMesh, levels: int, passes: int = 1, smoothing: int = 0, gamma_m: float = 0.5,
+              passes_given: bool = False) -> list:
+    """The meshes of levels 0 .. levels-1 (level 0 is tile), each level's
+    ground level 0's at its nodes smoothed by ground_passes(l, passes) passes,
+    its mesh built from it with the run's smoothing and gamma_m
+    (MeshHierarchy::MeshHierarchy, Source/MeshHierarchy.cpp:152-242)."""
+    g0 = tile.geom
+    err = check_hierarchy_inputs(g0.n, levels, passes, passes_given)
+    if err:
+        raise ValueError("gust: invalid grid hierarchy:\n  " + "\n  ".join(err))
+    out = [tile]
+    perx, pery = bool(g0.periodic[0]), bool(g0.periodic[1])
+    for lev in range(1, levels):
+        r = hierarchy_ratio(lev)
+        nxc, nyc = g0.n[0] // r, g0.n[1] // r
+        geom = geometry((nxc, nyc, g0.n[2]), tile.prob_lo, tile.prob_hi, g0.periodic[:2])
+        h = inject_ground(tile.z_nd, r, nxc, nyc, perx, pery)
+        smooth_surface(h, nxc, nyc, perx, pery, ground_passes(lev, passes), HIERARCHY_BETA)
+        m = build_mesh(geom, tile.prob_lo, tile.prob_hi, tile.zlevels, Terrain(kind="flat"), smoothing=smoothing,
+                       gamma_m=gamma_m, ground=h)
+        m.ground_passes = ground_passes(lev, passes)
+        out.append(m)
+    return out

@@ -124,6 +124,12 @@ class SteadyCriterion:
         r, n = _doubles(ref)
         _lib.check(_lib.lib.gust_steady_seed_reference(self._h, r, n))
 
+    def references(self) -> list:
+        """The residual references (SteadyHistory::res_ref)."""
+        out = (C.c_double * len(self.residual_names))()
+        _lib.check(_lib.lib.gust_steady_references(self._h, out, len(self.residual_names)))
+        return list(out)
+
     def history(self) -> bytes:
         """The recorded history (SteadyHistory) as the library's byte image."""
         size = C.c_int64(0)

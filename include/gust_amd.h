@@ -959,6 +959,11 @@ int gust_steady_iterations(const gust_steady* c, int32_t* n);
  *   Source/SteadyCriterion.cpp:229-238. */
 int gust_steady_residual_thresholds(const gust_steady* c, double* out, int32_t n);
 
+/* out[q] = the residual references (the largest of the first
+ * residual_ref_iterations values, or the seed raised by them).
+ * Replaces: SteadyCriterion::history().res_ref -- Source/SteadyCriterion.H:69-78. */
+int gust_steady_references(const gust_steady* c, double* out, int32_t n);
+
 /* Replaces: void SteadyCriterion::seed_reference(const vector<Real>&) --
  *   Source/SteadyCriterion.cpp:85-94. */
 int gust_steady_seed_reference(gust_steady* c, const double* ref, int32_t n);
@@ -985,6 +990,44 @@ int gust_steady_relative_residual(const gust_steady* c, const double* residuals,
  * Replaces: Real Driver::next_cfl(Real r) const -- Source/Driver.cpp:325-334. */
 int gust_steady_next_cfl(const gust_steady* c, int32_t hold, double cfl, double r, double cfl0, double cfl_max,
                          double ser_exponent, double max_growth, double* out);
+
+/* ---- Grid sequencing's stop and checks (host only) ------------------------ */
+
+/* Largest mesh.hierarchy_levels (Source/MeshHierarchy.H:48). */
+#define GUST_MAX_HIERARCHY_LEVELS 10
+
+/* Source/Sequencing.H:85-92 (SequencingOptions). given: the keys given, as
+ * bits 1 sequencing_drop_orders, 2 sequencing_monitor_tol, 4 sequencing_hold,
+ * 8 sequencing_cfl_start. */
+typedef struct gust_sequencing_options {
+    int32_t enabled;        /* steady.sequencing (0 | 1; 0) */
+    double drop_orders;     /* steady.sequencing_drop_orders (2) */
+    double monitor_tol;     /* steady.sequencing_monitor_tol (0: off) */
+    int32_t hold;           /* steady.sequencing_hold (0 | 1; 0) */
+    int32_t cfl_previous;   /* steady.sequencing_cfl_start: 0 cfl0, 1 previous */
+    uint32_t given;
+} gust_sequencing_options;
+
+/* Replaces: SequencingOptions{} -- Source/Sequencing.H:85-92. */
+int gust_sequencing_default_options(gust_sequencing_options* o);
+
+/* One message per problem, each naming its key, one line each into buf
+ * (NUL-terminated, truncated to cap); *count = the number of problems.
+ * levels = mesh.hierarchy_levels, nx, ny = mesh.n_cell x and y.
+ * Replaces: vector<string> check_sequencing_options(const SequencingOptions&,
+ *   const SteadyParams&, const MeshInputs&) -- Source/Sequencing.cpp:118-176. */
+int gust_check_sequencing_options(const gust_sequencing_options* o, const gust_steady_params* strict,
+                                  int32_t levels, int32_t nx, int32_t ny, int32_t* count, char* buf, int64_t cap);
+
+/* The loose stop after the last iteration the criterion recorded: *stop = 1
+ * and the branch that held in reason (NUL-terminated, truncated to cap), or
+ * *stop = 0. hold_iterations: the level's reference hold (0 on a released
+ * seeded level).
+ * Replaces: LooseStop loose_stop(const SteadyHistory&, const SteadyParams&,
+ *   const vector<MonitorSpec>&, const SequencingOptions&, int) --
+ *   Source/Sequencing.cpp:179-234. */
+int gust_loose_stop(const gust_steady* c, const gust_sequencing_options* o, int32_t hold_iterations, int32_t* stop,
+                    char* reason, int64_t cap);
 
 /* ---- The steady driver ---------------------------------------------------- */
 
@@ -1013,6 +1056,87 @@ typedef struct gust_driver_result {
  *   Source/Driver.cpp:403-492 (step, verdict, run). */
 int gust_steady_solve(gust_simplec* s, gust_state* q, const gust_driver_options* opt, gust_driver_result* result,
                       void* stream);
+
+/* How a level of grid sequencing starts and stops (the driver's seeding,
+ * Source/Driver.cpp:275-313, and the level loop of
+ * Source/Sequencing.cpp:361-390). loose: the loose stop's options (null: the
+ * strict criterion alone, as level 0); seeded: seed_ref holds the coarsest
+ * level's residual references (seed_reference); release_hold: no reference
+ * hold (needs seeded); start_cfl > 0: the first CFL (no hold; raised to cfl0). */
+typedef struct gust_level_control {
+    const gust_sequencing_options* loose;
+    int32_t seeded;
+    double seed_ref[6];
+    int32_t release_hold;
+    double start_cfl;
+} gust_level_control;
+
+/* stop: GUST_LEVEL_* (how the loop ended); res_ref: the criterion's
+ * references at the end (the coarsest level's seed the finer ones). */
+enum { GUST_LEVEL_MAX_ITERATIONS = 0, GUST_LEVEL_STEADY = 1, GUST_LEVEL_LOOSE = 2, GUST_LEVEL_DIVERGED = 3 };
+typedef struct gust_level_result {
+    gust_driver_result driver;
+    int32_t stop;
+    int32_t hold;   /* the level's reference hold (Driver::hold_iterations) */
+    double res_ref[6];
+    char reason[256];
+} gust_level_result;
+
+/* The steady loop of gust_steady_solve on one level of grid sequencing: the
+ * seeding and start of lc, then until Steady, Diverged, the loose stop or
+ * max_iterations (the CFL of the next iteration is computed before the
+ * loose stop is tested, as the reference's step does). A diverged level
+ * returns status Diverged (the caller aborts, naming the level).
+ * Replaces: Driver::seed_reference / release_reference_hold / start_cfl /
+ *   step and the level loop of sequence_coarse_levels --
+ *   Source/Driver.cpp:275-313, 403-468, Source/Sequencing.cpp:361-390. */
+int gust_steady_solve_level(gust_simplec* s, gust_state* q, const gust_driver_options* opt,
+                            const gust_level_control* lc, gust_level_result* result, void* stream);
+
+/* ---- Grid transfers (Source/Transfer.H) -----------------------------------
+ * Level lc is coarser than level lf by ratio = 2^(lc - lf) in x and y; z is
+ * never coarsened. The geometries are the two levels' (the same lo, hi z;
+ * x, y coarsened by ratio). */
+
+/* fine's valid points of u, v, w, p, theta, k from coarse's, bilinear per
+ * zeta-level at each field's own staggered point: linear extrapolation in
+ * the half cell next to a non-periodic face, the periodic images across a
+ * seam, constant along a non-periodic cell-centred direction one coarse cell
+ * wide. coarse's ghosts are not read; fine's are not filled.
+ * Replaces: void prolong_fields(const MeshHierarchy&, int lc, const State&
+ *   coarse, int lf, State& fine) -- Source/Transfer.cpp:324-352. */
+int gust_prolong_fields(const gust_geometry* fine_geom, const gust_geometry* coarse_geom, int32_t ratio,
+                        const gust_state* coarse, gust_state* fine, void* stream);
+
+/* coarse's valid points from fine: theta, k, p the detJ-weighted means, u, v
+ * the face fluxes over the coarse areas, w from the restricted vertical flux
+ * Omega (omega, when not null, receives Omega_c on the coarse z-faces); the
+ * slope term of w reads u, v with periodic images and zero-gradient rows
+ * below the ground and above the top. fine's u and v need one ghost filled.
+ * Replaces: void restrict_state(const MeshHierarchy&, int lf, const State&
+ *   fine, int lc, State& coarse, MultiFab* omega) -- Source/Transfer.cpp:263-322. */
+int gust_restrict_state(const gust_geometry* fine_geom, const gust_mesh* fine_mesh, const gust_state* fine,
+                        const gust_geometry* coarse_geom, const gust_mesh* coarse_mesh, int32_t ratio,
+                        gust_state* coarse, gust_array4* omega, void* stream);
+
+typedef struct gust_prolong_result {
+    gust_pressure_correction_result projection;   /* divergence before and after, the Poisson solve */
+    int64_t tke_clipped;                          /* fine cells raised to the k floor */
+} gust_prolong_result;
+
+/* The prolongation into the fine level of stage: gust_prolong_fields, k
+ * clipped at tke_floor, the stage's boundary conditions (its ghost fill with
+ * the wall state), then the projection: the pressure correction with d = 1
+ * off the Dirichlet rows (0 on them), solver (a gust_poisson on the fine
+ * mesh) and the stage's pressure boundary types, alpha 1; p is restored
+ * afterwards.
+ * Replaces: ProlongResult prolong_state(const MeshHierarchy&, int lc, const
+ *   State& coarse, int lf, State& fine, const BoundaryConditions&, const
+ *   SurfaceLayerOptions*, const BuoyancyOptions&, PoissonSolver&, Real
+ *   tke_floor) -- Source/Transfer.cpp:354-403. */
+int gust_prolong_state(gust_simplec* fine_stage, gust_poisson* solver, const gust_geometry* coarse_geom,
+                       int32_t ratio, const gust_state* coarse, gust_state* fine, double tke_floor,
+                       gust_prolong_result* result, void* stream);
 
 /* ---- The mass-consistent initializer --------------------------------------
  * Source/Initializer.H, Source/InflowProfile.H: the first guess from an

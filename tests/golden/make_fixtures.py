@@ -18,6 +18,11 @@ tests/golden/ (data only: inputs and expected outputs, no source text).
                        node heights (Level_0/Nu_nd component 2 = z - k dz,
                        plus k dz; (nz+1, ny+1, nx+1)) of the 32 x 1 x 16
                        Agnesi hill for terrain.smoothing 0, 1, 2.
+  askervein_hierarchy.npz  Tests/Discretization/Grid_Hierarchy/Gold/:
+                       z and detJ (cell, (nz, ny, nx)) of levels 1 and 2 of
+                       the three-level STF Askervein hierarchy
+                       (askervein_hierarchy_level{1,2}_plt00000) and the
+                       gold JSON (askervein_hierarchy_mesh.json) as text.
 
 Run from the repository root with the reference tree at /root/reference.
 """
@@ -121,6 +126,15 @@ def main():
         hill[f"{name}_detJ"] = cells[1]
         hill[f"{name}_z_nd"] = zmk + k * dz
     np.savez_compressed(os.path.join(OUT, "erf_hill2d.npz"), **hill)
+    base = os.path.join(REF, "Tests/Discretization/Grid_Hierarchy/Gold")
+    hier = {"mesh_json": open(os.path.join(base, "askervein_hierarchy_mesh.json")).read()}
+    for lev in (1, 2):
+        names, lo, hi, cells = read_plotfile_cells(os.path.join(base, f"askervein_hierarchy_level{lev}_plt00000"),
+                                                   nvar_expected=2)
+        assert names == ["z", "detJ"], names
+        hier[f"level{lev}_z"] = cells[0]
+        hier[f"level{lev}_detJ"] = cells[1]
+    np.savez_compressed(os.path.join(OUT, "askervein_hierarchy.npz"), **hier)
     print("wrote fixtures to", OUT)
 
 
