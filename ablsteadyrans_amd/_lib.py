@@ -460,6 +460,13 @@ _sig = {
                                       _P(StateC), _P(Array4), C.c_void_p]),
     "gust_prolong_state": (C.c_int, [C.c_void_p, C.c_void_p, _P(GeometryC), C.c_int32, _P(StateC), _P(StateC),
                                      C.c_double, _P(ProlongResultC), C.c_void_p]),
+    "gust_write_plotfile": (C.c_int, [C.c_char_p, C.c_int32, _P(GeometryC), _P(C.c_double), _P(C.c_double),
+                                      _P(StateC), _P(Array4), _P(Array4), C.c_int32, C.c_char_p, C.c_int64,
+                                      C.c_void_p]),
+    "gust_write_checkpoint": (C.c_int, [C.c_char_p, C.c_int32, _P(GeometryC), _P(C.c_double), _P(C.c_double),
+                                        _P(StateC), _P(Array4), C.c_int32, C.c_char_p, C.c_int64, C.c_void_p]),
+    "gust_read_checkpoint": (C.c_int, [C.c_char_p, _P(GeometryC), _P(C.c_double), _P(C.c_double), _P(StateC),
+                                       _P(Array4), _P(C.c_int32), C.c_void_p]),
     "gust_steady_default_params": (C.c_int, [_P(SteadyParamsC)]),
     "gust_steady_validate": (C.c_int, [_P(SteadyParamsC), _P(C.c_int32), C.c_char_p, C.c_int64]),
     "gust_steady_create": (C.c_int, [_P(SteadyParamsC), C.c_int32, _P(C.c_char_p), C.c_int32, _P(C.c_char_p),

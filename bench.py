@@ -5,18 +5,23 @@ Headline (value): steady solves per second at a 6-order residual drop on the
 Gaussian-hill configuration (h = 100 exp(-(r/250 m)^2) m in 4 x 4 km,
 200 x 200 x 64 cells at dx = 20 m, 64 levels from dz1 = 1 m growing by 1.08;
 the open-channel neutral ABL: log-law inflow, outflow, y periodic, surface
-layer, slip top, Coriolis, the k-L closure) in the mixed-precision form
-(FP32 z-line sweeps and FP32 multigrid V-cycles inside the FP64 defect
-correction: the residual, GMRES and the coarsest solve stay FP64), with the
-solver settings of the reference's 3-D hill deck (16 sweeps, CFL 1 -> 4,
-rotation_relax 1). One step = one SIMPLEC iteration of that solve (the
-per-iteration hot path on the 2.56 M-cell tile): `warmup` driver iterations
-from the first guess, then `steps` iterations from the first guess timed
-between synchronisations (ms_per_step); value = 1 / the wall time of one
-whole steady solve from the first guess (the library's driver loop, state
-resident in HBM), measured in the same run. The plain FP64 path is timed the
-same way (extra.c2_fp64), and the per-field relative L2 of the mixed
-solution against the FP64 one is reported (extra.c2_field_error).
+layer, slip top, Coriolis, the k-L closure) in the fast form: mixed
+precision (FP32 z-line sweeps and FP32 multigrid V-cycles inside the FP64
+defect correction: the residual, GMRES and the coarsest solve stay FP64)
+with two levels of grid sequencing (the 100 x 100 level to the loose 2-order
+stop, then the 200 x 200 level to the 6-order criterion, seeded with the
+coarse level's references), with the solver settings of the reference's
+3-D hill deck (16 sweeps, CFL 1 -> 4, rotation_relax 1). value = 1 / the
+wall time of one whole sequenced solve from the coarse level's first guess
+(hierarchy built before the timed region, state resident in HBM). One step
+= one SIMPLEC iteration on the 2.56 M-cell tile (mixed): `warmup` driver
+iterations from the first guess, then `steps` iterations from the first
+guess timed between synchronisations (ms_per_step). The plain FP64 path
+(one level, the drop-in form) and plain mixed precision are timed as whole
+solves too (extra.c2_fp64, extra.c2_mixed); the per-field relative L2 of
+each fast form's converged fields against the plain FP64 ones is reported
+(extra.c2_field_error_*), and the ratio to the plain FP64 solves/s
+(extra.fast_vs_fp64).
 
 Also measured (extra): the reference's Neutral_Column deck (the previous
 headline: 4 x 4 x 64, 2231 iterations) FP64 and mixed; the reference's own
@@ -303,6 +308,55 @@ def _c2_run(precision, steps, warmup):
     return out, fields
 
 
+def _c2_sequenced_run(precision, levels=2):
+    """The Gaussian-hill steady solve with grid sequencing: the coarse
+    level(s) from the initializer to the loose 2-order stop, then level 0
+    (the 200x200x64 tile) from the prolonged state to the strict 6-order
+    criterion seeded with the coarsest level's references. Timed end to end
+    (hierarchy prepared before the timed region). Returns a dict and level
+    0's converged fields."""
+    import torch
+    from ablsteadyrans_amd import decks as D
+    from ablsteadyrans_amd import physics as GP
+    from ablsteadyrans_amd import sequencing as SQ
+    deck = D.gaussian_hill_200(**C2_SETTINGS)
+    sd = SQ.prepare_sequenced(deck, levels, precision=precision)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res, summ = SQ.solve_sequenced(sd, SQ.SequencingOptions(enabled=True))
+    torch.cuda.synchronize()
+    solve_s = time.perf_counter() - t0
+    fields = {n: deck.state[n].numpy()[0, 2:-2, 2:-2, 2:-2].copy() for n in GP.FIELDS}
+    out = {"levels": levels, "status": int(res.driver.status), "seconds_per_solve": round(solve_s, 3),
+           "steady_solves_per_s": round(1.0 / solve_s, 5),
+           "relative_residual": float(res.driver.relative_residual),
+           "per_level": [{"level": x.level, "cells": [x.cells_x, x.cells_y], "iterations": x.iterations,
+                          "seconds": round(x.wall_seconds, 3), "stop": x.stop} for x in summ]}
+    del sd, deck
+    torch.cuda.empty_cache()
+    return out, fields
+
+
+def _hill3d_sequenced():
+    """The reference's Neutral_Hill_3D deck with three levels of grid
+    sequencing (docs/results.rst:658-676: 54 s on four ranks, 706 level-0
+    iterations after 383 and 51)."""
+    import torch
+    from ablsteadyrans_amd import decks as D
+    from ablsteadyrans_amd import sequencing as SQ
+    deck = D.neutral_hill_3d()
+    sd = SQ.prepare_sequenced(deck, 3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res, summ = SQ.solve_sequenced(sd, SQ.SequencingOptions(enabled=True))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"deck": "Neutral_Hill_3D, 3 levels of grid sequencing", "status": int(res.driver.status),
+            "iterations_per_level": [x.iterations for x in summ], "iterations_reference": [383, 51, 706],
+            "seconds": round(dt, 3), "seconds_reference": 54.0, "reference_ranks": 4,
+            "speedup_vs_reference": round(54.0 / dt, 2)}
+
+
 def _field_errors(a, b):
     """Per-field relative L2 of a against b (valid points)."""
     import numpy as np
@@ -529,8 +583,12 @@ def main():
     c2m, fm = _c2_run(2, args.steps, args.warmup)
     print("bench: steady solve of the 200x200x64 Gaussian hill (FP64)", file=sys.stderr, flush=True)
     c2d, fd = _c2_run(0, args.steps, args.warmup)
+    print("bench: steady solve of the 200x200x64 Gaussian hill (mixed, two levels of grid sequencing)",
+          file=sys.stderr, flush=True)
+    c2s, fs = _c2_sequenced_run(2)
     errs = _field_errors(fm, fd)
-    del fm, fd
+    errs_seq = _field_errors(fs, fd)
+    del fm, fd, fs
     # 2. the reference's column deck, whole solves (the previous headline)
     print("bench: steady solves of the Neutral_Column deck (FP64, mixed)", file=sys.stderr, flush=True)
     s64, it64, st64, us64, t64 = _time_solves(0, 3, 1)
@@ -539,6 +597,7 @@ def main():
     print("bench: the reference's steady decks (Askervein, Neutral_ABL_Flat, Neutral_Hill_3D)", file=sys.stderr,
           flush=True)
     decks = _time_reference_decks()
+    decks.append(_hill3d_sequenced())
     # 4. the z-line sweep (the roofline kernel) and the V-cycle on the fractal tile
     print("bench: z-line sweeps and V-cycles on the 256x256x96 fractal tile", file=sys.stderr, flush=True)
     sys64 = _systems(torch.float64, 20264203)
@@ -570,7 +629,7 @@ def main():
         print("bench: CPU baselines (oracle)", file=sys.stderr, flush=True)
         cpu = _cpu_column_baseline()
         cpu_sweep = _cpu_baseline()
-    value = c2m["steady_solves_per_s"]
+    value = c2s["steady_solves_per_s"]
     line = {
         "metric": "steady solves/s at 6-order residual drop, mixed vs FP64; sweep HBM GB/s; p99",
         "value": value,
@@ -585,20 +644,23 @@ def main():
         "dtype": "f32 sweeps and V-cycles in f64 defect correction",
         "data": "synthetic: the analytic Gaussian-hill terrain and the log-law first guess made divergence-free, "
                 "resident in HBM",
-        "config": {"workload": C2_WORKLOAD, "cells": [200, 200, 64], "precision": "mixed",
-                   "step": "one SIMPLEC iteration (residual, momentum predictor with 16 sweeps, pressure "
-                           "correction, theta and k sweeps)",
-                   "iterations_to_steady": c2m["iterations"], "status": c2m["status"],
-                   "seconds_per_solve": c2m["seconds_per_solve"],
-                   "ms_per_iteration_whole_solve": c2m["ms_per_iteration_whole_solve"]},
+        "config": {"workload": C2_WORKLOAD, "cells": [200, 200, 64],
+                   "form": "mixed precision, two levels of grid sequencing (100x100 then 200x200)",
+                   "step": "one SIMPLEC iteration on the 200x200x64 tile (residual, momentum predictor with 16 "
+                           "sweeps, pressure correction, theta and k sweeps), mixed precision",
+                   "iterations_per_level": [x["iterations"] for x in c2s["per_level"]], "status": c2s["status"],
+                   "seconds_per_solve": c2s["seconds_per_solve"]},
         "roofline": {"bound": "hbm", "achieved": round(gbs64, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": _traffic_from_profile(),
                      "kernel": "zjacobi_sweep_kernel<double,false,false> (the metric's sweep HBM GB/s: 5 "
                                "variables on 256x256x96, one launch)",
                      "algorithmic_bytes_per_launch": bytes64, "avg_launch_ms": round(ms_ev, 5)},
         "cpu_baseline": cpu,
-        "extra": {"c2_mixed": c2m, "c2_fp64": c2d,
+        "extra": {"c2_mixed_sequenced": c2s, "c2_mixed": c2m, "c2_fp64": c2d,
+                  "fast_vs_fp64": round(c2s["steady_solves_per_s"] / c2d["steady_solves_per_s"], 3),
                   "mixed_vs_fp64": round(c2m["steady_solves_per_s"] / c2d["steady_solves_per_s"], 3),
+                  "c2_field_error_mixed_sequenced_vs_fp64_rel_l2": {k: float(f"{v:.3e}")
+                                                                    for k, v in errs_seq.items()},
                   "c2_field_error_mixed_vs_fp64_rel_l2": {k: float(f"{v:.3e}") for k, v in errs.items()},
                   "column_fp64_steady_solves_per_s": round(1.0 / s64, 4), "column_fp64_iterations": it64,
                   "column_fp64_status": st64, "column_ustar": round(us64, 8),

@@ -1138,6 +1138,41 @@ int gust_prolong_state(gust_simplec* fine_stage, gust_poisson* solver, const gus
                        int32_t ratio, const gust_state* coarse, gust_state* fine, double tke_floor,
                        gust_prolong_result* result, void* stream);
 
+/* ---- The state's files (Source/StateIO.H) ---------------------------------
+ * prefix + the step as five digits names the directory (created). prob_lo,
+ * prob_hi: the domain's corners (3 each). max_grid_size > 0 splits the
+ * domain in x and y into boxes of at most that many cells (AMReX's maxSize);
+ * 0 writes one box. name (may be null) receives the directory's name. */
+
+/* An AMReX plotfile (HyperCLaw-V1.1): the cell variables u, v, w (face
+ * means), p, theta, tke, z (the mean of the cell's eight node heights) and,
+ * with detJ, detJ; the node heights with their ghost nodes as the VisMF
+ * MultiFab Level_0/z_phys_nd.
+ * Replaces: string write_plotfile(const string& prefix, const State&, const
+ *   Geometry&, const MultiFab* detJ) -- Source/StateIO.cpp:41-85. */
+int gust_write_plotfile(const char* prefix, int32_t step, const gust_geometry* geom, const double* prob_lo,
+                        const double* prob_hi, const gust_state* q, const gust_array4* z_nd, const gust_array4* detJ,
+                        int32_t max_grid_size, char* name, int64_t cap, void* stream);
+
+/* A checkpoint: the text Header (magic and version, real bytes, step, mesh
+ * in hex floats, periodicity, BoxArray) and every field as a VisMF MultiFab
+ * with its ghosts (u, v, w, p, theta, tke: two; z_phys_nd: one).
+ * Replaces: string write_checkpoint(const string& prefix, State&, const
+ *   MeshInputs&) -- Source/StateIO.cpp:87-113. */
+int gust_write_checkpoint(const char* prefix, int32_t step, const gust_geometry* geom, const double* prob_lo,
+                          const double* prob_hi, const gust_state* q, const gust_array4* z_nd, int32_t max_grid_size,
+                          char* name, int64_t cap, void* stream);
+
+/* Reads a checkpoint into q (valid values exactly, from any box layout) and,
+ * when z_nd is not null, the node heights with their ghost nodes; *step =
+ * its step. Fails, naming io.restart, on a corrupt Header, another
+ * precision, or a mesh (n_cell, prob_lo, prob_hi, periodicity) other than
+ * geom's and prob_lo/prob_hi.
+ * Replaces: void read_checkpoint(const string& dir, State&, const MeshInputs&)
+ *   -- Source/StateIO.cpp:115-211. */
+int gust_read_checkpoint(const char* dir, const gust_geometry* geom, const double* prob_lo, const double* prob_hi,
+                         gust_state* q, gust_array4* z_nd, int32_t* step, void* stream);
+
 /* ---- The mass-consistent initializer --------------------------------------
  * Source/Initializer.H, Source/InflowProfile.H: the first guess from an
  * inflow profile by height above the ground, then one anisotropic
