@@ -500,6 +500,68 @@ def _cpu_column_baseline(target_s=15.0):
             "ms_per_iteration": round(per_it * 1e3, 3)}
 
 
+def _cpu_info():
+    """(CPU model, logical CPUs) of the host."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def _cpu_c2_baseline(n=99, fp64_iterations=None):
+    """The oracle's SIMPLEC iteration composite (the C oracle's FP64 pieces,
+    numpy in between; the pressure correction's Poisson solve by GMRES with
+    a z-line preconditioner on the oracle operator at the deck's reltol
+    1e-3, tests/cpu_poisson.py) on one host thread, one iteration from the
+    first guess on an n x n x 64 tile of the Gaussian-hill workload (the same
+    dx, levels, hill and physics, the hill centred); reported as steady
+    solves/s of the 200 x 200 x 64 solve: the iteration time scaled by the
+    cell ratio and by the plain FP64 GPU solve's iteration count."""
+    from ablsteadyrans_amd import decks as D
+    from ablsteadyrans_amd import mesh as M
+    from oracle import initializer as OI
+    from oracle import oracle as orc
+    from tests import config_helpers as CH
+    from tests import cpu_poisson as CP
+    from tests import physics_helpers as PHY
+    from tests import stage_helpers as STH
+    deck = D.open_abl(f"gaussian_hill_{n}x{n}x64", (n, n, 64), (20.0 * n, 20.0 * n), 1.0,
+                      M.Terrain(kind="hills", hills=[(10.0 * n, 10.0 * n, 100.0, 250.0)]), **C2_SETTINGS)
+    tile = deck.tile
+    m = PHY.omesh(tile)
+    phys = CH.open_abl_physics(deck)
+    q = PHY.zeros_state(PHY.P_geom(tile))
+    gi = deck.inputs.inflow
+    OI.first_guess_state(q, m, OI.InflowProfile(kind=OI.LOG_LAW, ustar=gi.ustar, z0=gi.z0, direction=gi.direction),
+                         OI.InitOptions(first_guess=1))
+    solve = CP.LineGMRES(tile, rtol=deck.stage.pressure_reltol)
+    B1 = (-1, -1, -1)
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(limits=1):   # numpy's BLAS on one thread too
+        t0 = time.perf_counter()
+        q1 = {k: a[:, 1:-1, 1:-1, 1:-1].copy() for k, a in q.items()}
+        dtau = orc.local_dtau(q1["u"], q1["v"], q1["w"], B1, m.z, B1, m.J, B1, m.geom, 1.0)
+        STH.simplec_iteration(tile, m, q, phys, dtau, sweeps=C2_SETTINGS["sweeps"], tke_floor=0.0,
+                              relax=C2_SETTINGS["rotation_relax"], solve=solve)
+        dt = time.perf_counter() - t0
+    scale = (200.0 * 200.0) / (n * n)
+    its = fp64_iterations or 868
+    per_it = dt * scale
+    model, ncpu = _cpu_info()
+    return {"value": float(f"{1.0 / (per_it * its):.4g}"), "unit": "steady solves/s", "cores": 1, "kind": "port",
+            "sample": f"one SIMPLEC iteration of the oracle's composite (C pieces, numpy between them, GMRES with "
+                      f"a z-line preconditioner for the pressure: {solve.iterations} iterations) on a {n}x{n}x64 "
+                      f"tile of the Gaussian-hill workload in {dt:.1f} s on 1 thread, scaled by the cell ratio "
+                      f"{scale:g} and the {its} iterations of the plain FP64 solve; host CPU {model}, "
+                      f"{ncpu} logical CPUs",
+            "s_per_iteration_200x200x64": round(per_it, 2), "cpu_model": model, "host_logical_cpus": ncpu}
+
+
 def _traffic_from_profile():
     """HBM bytes per launch of the FP64 sweep from the committed counter
     summary (its 'hbm_bytes_per_launch:' line), or None."""
@@ -624,10 +686,11 @@ def main():
         serving = {"served_mixed_8_slots": served, "plain_fp64_one_at_a_time": plain,
                    "measured_bytes_per_cell": bpc}
         batched = _batched_leg()
-    cpu = cpu_sweep = None
+    cpu = cpu_sweep = cpu_column = None
     if not args.no_cpu:
         print("bench: CPU baselines (oracle)", file=sys.stderr, flush=True)
-        cpu = _cpu_column_baseline()
+        cpu = _cpu_c2_baseline(fp64_iterations=c2d["iterations"])
+        cpu_column = _cpu_column_baseline()
         cpu_sweep = _cpu_baseline()
     value = c2s["steady_solves_per_s"]
     line = {
@@ -681,7 +744,8 @@ def main():
                   "serving": serving,
                   "serving_p99_s": serving["served_mixed_8_slots"]["all"]["p99_s"] if serving else None,
                   "batched": batched,
-                  "cpu_sweep_baseline": cpu_sweep},
+                  "cpu_sweep_baseline": cpu_sweep,
+                  "cpu_column_baseline": cpu_column},
     }
     print(json.dumps(line))
 
