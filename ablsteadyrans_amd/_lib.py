@@ -76,6 +76,7 @@ class PoissonOptionsC(C.Structure):
         ("post_sweeps", C.c_int32),
         ("coarse_sweeps", C.c_int32),
         ("max_levels", C.c_int32),
+        ("fixed_steps", C.c_int32),
     ]
 
 
@@ -467,6 +468,13 @@ _sig = {
                                         _P(StateC), _P(Array4), C.c_int32, C.c_char_p, C.c_int64, C.c_void_p]),
     "gust_read_checkpoint": (C.c_int, [C.c_char_p, _P(GeometryC), _P(C.c_double), _P(C.c_double), _P(StateC),
                                        _P(Array4), _P(C.c_int32), C.c_void_p]),
+    "gust_poisson_set_fixed_steps": (C.c_int, [C.c_void_p, C.c_int32]),
+    "gust_simplec_set_pressure_steps": (C.c_int, [C.c_void_p, C.c_int32]),
+    "gust_poisson_fixed_steps_misses": (C.c_int, [C.c_void_p, C.c_int32, _P(C.c_int64)]),
+    "gust_simplec_capture": (C.c_int, [C.c_void_p, _P(StateC), _P(Array4), C.c_void_p]),
+    "gust_simplec_replay": (C.c_int, [C.c_void_p, _P(SimplecInfoC), C.c_void_p]),
+    "gust_steady_solve_graphed": (C.c_int, [C.c_void_p, _P(StateC), _P(DriverOptionsC), C.c_int32,
+                                            _P(DriverResultC), C.c_void_p]),
     "gust_steady_default_params": (C.c_int, [_P(SteadyParamsC)]),
     "gust_steady_validate": (C.c_int, [_P(SteadyParamsC), _P(C.c_int32), C.c_char_p, C.c_int64]),
     "gust_steady_create": (C.c_int, [_P(SteadyParamsC), C.c_int32, _P(C.c_char_p), C.c_int32, _P(C.c_char_p),

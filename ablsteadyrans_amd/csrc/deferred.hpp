@@ -73,7 +73,41 @@ public:
     void resolve()
     {
         if (cbs_.empty()) { return; }
+        if (recording_) {
+            GUST_FAIL("gust: internal: a host read inside a captured iteration (use the device-resident forms)");
+        }
         flush();
+    }
+
+    // A batch recorded into a captured graph: the graph holds the slots'
+    // initialisation, the kernels that write them and the copy to pinned
+    // host memory; after each launch of the graph (and a wait) replay()
+    // runs the callbacks on the copied values. Every callback must refer
+    // only to objects that outlive the graph.
+    struct Recorded {
+        std::vector<std::pair<int, Callback>> cbs;
+        const unsigned long long* host = nullptr;
+        void replay() const
+        {
+            for (const auto& c : cbs) { c.second(c.first >= 0 && host ? host[c.first] : 0ull); }
+        }
+    };
+    void begin_recording() { recording_ = true; }
+    // Enqueues (records) the slots' copy to the host and hands the callbacks
+    // over; the batch is empty afterwards.
+    Recorded end_recording()
+    {
+        Recorded r;
+        if (inited_) {
+            GUST_HIP(hipMemcpyAsync(host_, dev_, sizeof(unsigned long long) * (kZero + kOnes),
+                                    hipMemcpyDeviceToHost, st_));
+            r.host = host_;
+        }
+        r.cbs.swap(cbs_);
+        inited_ = false;
+        nz_ = no_ = 0;
+        recording_ = false;
+        return r;
     }
 
     static Deferred*& current()
@@ -126,7 +160,7 @@ private:
     unsigned long long* dev_ = nullptr;
     unsigned long long* host_ = nullptr;
     int nz_ = 0, no_ = 0;
-    bool inited_ = false;
+    bool inited_ = false, recording_ = false;
     std::vector<std::pair<int, Callback>> cbs_;
 };
 

@@ -1008,6 +1008,87 @@ __global__ void copy_bits_kernel(const double* __restrict__ src, unsigned long l
     *out = __double_as_longlong(src[0]);
 }
 
+// GMRES(k) with its Hessenberg matrix, Givens rotations and right-hand side
+// on the device (a fixed number of Arnoldi steps, no host reads: the form a
+// captured graph can hold). g[] layout: [0] r0, [1] target, [2] the scale of
+// the next Arnoldi vector, [3] spare, then H ((k+1) x k, column-major), cs
+// (k), sn (k), rhs (k+1).
+struct GmDev {
+    double* g;
+    int k;
+    __device__ double& H(int i, int j) const { return g[4 + i + j * (k + 1)]; }
+    __device__ double& cs(int i) const { return g[4 + (k + 1) * k + i]; }
+    __device__ double& sn(int i) const { return g[4 + (k + 1) * k + k + i]; }
+    __device__ double& rhs(int i) const { return g[4 + (k + 1) * k + 2 * k + i]; }
+};
+
+__global__ void gm_start_kernel(GmDev m, const double* __restrict__ r0sq, double reltol, double abstol)
+{
+    const double r0 = sqrt(*r0sq);
+    m.g[0] = r0;
+    m.g[1] = fmax(reltol * r0, abstol);
+    m.g[2] = r0 > 0 ? 1.0 / r0 : 0.0;
+    for (int i = 0; i <= m.k; ++i) { m.rhs(i) = 0.0; }
+    m.rhs(0) = r0;
+}
+
+// b = a * (*s) (the Arnoldi vector from w and its scale on the device).
+__global__ void scale_copy_dev_kernel(const double* __restrict__ a, const double* __restrict__ s,
+                                      double* __restrict__ b, int64_t n)
+{
+    const double f = *s;
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        b[t] = a[t] * f;
+    }
+}
+
+// Column j of H from the two Gram-Schmidt passes and |w|, the earlier
+// rotations applied to it, its own rotation, the rotated right-hand side.
+__global__ void gm_arnoldi_kernel(GmDev m, int j, const double* __restrict__ hv, const double* __restrict__ hv2,
+                                  const double* __restrict__ hn2)
+{
+    for (int i = 0; i <= j; ++i) { m.H(i, j) = hv[i] + hv2[i]; }
+    const double hn = sqrt(*hn2);
+    m.H(j + 1, j) = hn;
+    m.g[2] = hn > 0 ? 1.0 / hn : 0.0;
+    for (int i = 0; i < j; ++i) {
+        const double a = m.H(i, j), c2 = m.H(i + 1, j);
+        m.H(i, j) = m.cs(i) * a + m.sn(i) * c2;
+        m.H(i + 1, j) = -m.sn(i) * a + m.cs(i) * c2;
+    }
+    const double a = m.H(j, j), c2 = m.H(j + 1, j);
+    const double den = hypot(a, c2);
+    m.cs(j) = den > 0 ? a / den : 1.0;
+    m.sn(j) = den > 0 ? c2 / den : 0.0;
+    m.H(j, j) = den;
+    m.H(j + 1, j) = 0.0;
+    m.rhs(j + 1) = -m.sn(j) * m.rhs(j);
+    m.rhs(j) = m.cs(j) * m.rhs(j);
+}
+
+// y = H^-1 g (upper triangular; a zero pivot, an exhausted Krylov space,
+// gives 0).
+__global__ void gm_backsolve_kernel(GmDev m, double* __restrict__ y)
+{
+    for (int i = m.k - 1; i >= 0; --i) {
+        double s = m.rhs(i);
+        for (int q = i + 1; q < m.k; ++q) { s -= m.H(i, q) * y[q]; }
+        const double d = m.H(i, i);
+        y[i] = d != 0 ? s / d : 0.0;
+    }
+}
+
+// The true residual's norm against the target: its bits, and 1 when it is
+// not within the target.
+__global__ void gm_check_kernel(GmDev m, const double* __restrict__ rr2, unsigned long long* __restrict__ final_bits,
+                                unsigned long long* __restrict__ miss, unsigned long long* __restrict__ r0_bits)
+{
+    const double f = sqrt(*rr2);
+    *final_bits = static_cast<unsigned long long>(__double_as_longlong(f));
+    *r0_bits = static_cast<unsigned long long>(__double_as_longlong(m.g[0]));
+    *miss = (f <= m.g[1]) ? 0ull : 1ull;
+}
+
 // Copy between a compact field and a view's valid region (dir 0: view ->
 // compact, 1: compact -> view).
 template <typename R>
@@ -1294,11 +1375,15 @@ struct PoissonImpl {
     std::vector<double> zp_host;
     std::vector<std::unique_ptr<Level<double>>> mg64;
     std::vector<std::unique_ptr<Level<float>>> mg32;
-    DevBuf krylov_v, krylov_z, w, partial, dots, hcoef, scal;
+    DevBuf krylov_v, krylov_z, w, partial, dots, hcoef, scal, gm;
     // sum J over the level-0 cells (the mean removal's denominator), formed
     // once: the mesh does not change under a solver
     DevBuf sum_j;
     bool have_sum_j = false;
+    // The fixed-step form's tolerance misses: an error (default), or, for a
+    // caller that adapts the step count, counted.
+    bool soft_miss = false;
+    int64_t misses = 0;
     std::vector<double> hbuf;
     int per[2] = {0, 0};
 
@@ -1729,6 +1814,9 @@ void check_poisson_options(const gust_poisson_options& o)
     if (o.restart < 1) { err += "\n  solver.poisson_restart must be >= 1"; }
     if (o.precision != 0 && o.precision != 1) { err += "\n  precision must be 0 (fp64) or 1 (fp32 V-cycle)"; }
     if (o.max_levels < 0) { err += "\n  the multigrid's max_levels must be >= 0 (0: no limit)"; }
+    if (o.fixed_steps < 0 || o.fixed_steps > 200) {
+        GUST_FAIL("gust: PoissonOptions: fixed_steps must be 0 (adaptive GMRES) or 1..200 (got %d)", o.fixed_steps);
+    }
     if (o.pre_sweeps < 0 || o.post_sweeps < 0 || o.coarse_sweeps < 1) {
         err += "\n  the V-cycle needs pre/post sweeps >= 0 and coarse sweeps >= 1";
     }
@@ -1895,6 +1983,30 @@ void apply_fp64(PoissonImpl& P, const BcInfo& bc, const double* x, double* y, hi
     launch_apply<double>(P.L0, bc, P.per, x, nullptr, y, st);
 }
 
+// The fluxes F = -beta grad phi of the solution x on the valid faces.
+void solve_fluxes(PoissonImpl& P, const BcInfo& b, const gust_face_coeffs* coeffs, const double* x, gust_array4* fx,
+                  gust_array4* fy, gust_array4* fz, hipStream_t st)
+{
+    const Ctx<double> c = P.L0.ctx(b);
+    const MirrorAcc<double> mm = mirror<double>(x, P.nx, P.ny, P.nz, P.per, b);
+    gust_array4* F[3] = {fx, fy, fz};
+    const gust_array4* beta3[3] = {coeffs ? coeffs->x : nullptr, coeffs ? coeffs->y : nullptr,
+                                   coeffs ? coeffs->z : nullptr};
+    for (int d = 0; d < 3; ++d) {
+        IBox fb;
+        for (int q = 0; q < 3; ++q) { fb.lo[q] = P.geom.lo[q]; fb.hi[q] = P.geom.hi[q] + (q == d); }
+        const double* bp = beta3[d] ? static_cast<const double*>(beta3[d]->p) +
+                                          offset_of(*beta3[d], fb.lo[0], fb.lo[1], fb.lo[2])
+                                    : nullptr;
+        const int64_t ne = int64_t(fb.len(0)) * fb.len(1) * fb.len(2);
+        flux_kernel<double><<<blocks_for(ne), kThreads, 0, st>>>(
+            d, c, mm, bp, beta3[d] ? beta3[d]->jstride : 0, beta3[d] ? beta3[d]->kstride : 0,
+            static_cast<double*>(F[d]->p) + offset_of(*F[d], fb.lo[0], fb.lo[1], fb.lo[2]), F[d]->jstride,
+            F[d]->kstride);
+        GUST_HIP(hipGetLastError());
+    }
+}
+
 }   // namespace
 
 }   // namespace gust_amd
@@ -1921,6 +2033,27 @@ int gust_poisson_default_options(gust_poisson_options* o)
         o->post_sweeps = 2;
         o->coarse_sweeps = 16;
         o->max_levels = 0;
+        o->fixed_steps = 0;
+    });
+}
+
+int gust_poisson_set_fixed_steps(gust_poisson* p, int32_t steps)
+{
+    return guarded([&] {
+        if (!p) { GUST_FAIL("gust: PoissonSolver: null solver"); }
+        gust_poisson_options o = p->impl.opt;
+        o.fixed_steps = steps;
+        check_poisson_options(o);
+        p->impl.opt.fixed_steps = steps;
+    });
+}
+
+int gust_poisson_fixed_steps_misses(gust_poisson* p, int32_t soft, int64_t* misses)
+{
+    return guarded([&] {
+        if (!p) { GUST_FAIL("gust: PoissonSolver: null solver"); }
+        p->impl.soft_miss = soft != 0;
+        if (misses) { *misses = p->impl.misses; }
     });
 }
 
@@ -2120,6 +2253,71 @@ int gust_poisson_solve(gust_poisson* p, gust_array4* rhs, gust_array4* phi, cons
             ratio_bits_kernel<<<1, 1, 0, st>>>(mean_d, ptr<double>(P.sum_j), slot);
             phi_in_out(P, rhs, bvec, 1, st);
         }
+        if (P.opt.fixed_steps > 0) {
+            // GMRES(k) on the device, k = fixed_steps Arnoldi steps whatever
+            // the residual: no host reads, so a captured graph can hold the
+            // solve; the tolerance is checked at the end (a deferred check).
+            const int k = P.opt.fixed_steps;
+            ensure_work(P, std::max(m, k));
+            P.gm.ensure(size_t(4 + (k + 1) * k + 3 * k + 1) * 8);
+            double* hv_k = ptr<double>(P.dots);
+            double* hv2_k = hv_k + (k + 2);
+            double* hn_k = hv2_k + (k + 2);
+            const GmDev gm{ptr<double>(P.gm), k};
+            double* Vk = ptr<double>(P.krylov_v);
+            double* Zk = ptr<double>(P.krylov_z);
+            double* wk = ptr<double>(P.w);
+            fill_kernel<double><<<blocks_for(n), kThreads, 0, st>>>(x, 0.0, n);
+            multi_dot_dev(P, bvec, 0, 1, bvec, hn_k, st);
+            gm_start_kernel<<<1, 1, 0, st>>>(gm, hn_k, P.opt.reltol, P.opt.abstol);
+            scale_copy_dev_kernel<<<blocks_for(n), kThreads, 0, st>>>(bvec, gm.g + 2, Vk, n);
+            GUST_HIP(hipGetLastError());
+            if (P.opt.precision == 1) { setup_hierarchy<float>(P, P.mg32, b, st); }
+            else { setup_hierarchy<double>(P, P.mg64, b, st); }
+            for (int j = 0; j < k; ++j) {
+                double* vj = Vk + int64_t(j) * n;
+                double* zj = Zk + int64_t(j) * n;
+                if (j > 0) { scale_copy_dev_kernel<<<blocks_for(n), kThreads, 0, st>>>(wk, gm.g + 2, vj, n); }
+                precondition(P, vj, zj, b, st);
+                apply_fp64(P, b, zj, wk, st);
+                multi_dot_dev(P, Vk, n, j + 1, wk, hv_k, st);
+                multi_axpy_kernel<<<blocks_for(n), kThreads, 0, st>>>(wk, Vk, n, hv_k, j + 1, -1.0, n);
+                multi_dot_dev(P, Vk, n, j + 1, wk, hv2_k, st);
+                multi_axpy_kernel<<<blocks_for(n), kThreads, 0, st>>>(wk, Vk, n, hv2_k, j + 1, -1.0, n);
+                multi_dot_dev(P, wk, 0, 1, wk, hn_k, st);
+                gm_arnoldi_kernel<<<1, 1, 0, st>>>(gm, j, hv_k, hv2_k, hn_k);
+                GUST_HIP(hipGetLastError());
+            }
+            gm_backsolve_kernel<<<1, 1, 0, st>>>(gm, ptr<double>(P.hcoef));
+            multi_axpy_kernel<<<blocks_for(n), kThreads, 0, st>>>(x, Zk, n, ptr<double>(P.hcoef), k, 1.0, n);
+            GUST_HIP(hipGetLastError());
+            acc->iterations = k;
+            if (b.singular) { remove_weighted_mean(P, x, mean_d, st); }
+            launch_apply<double>(P.L0, b, P.per, x, bvec, r, st);
+            multi_dot_dev(P, r, 0, 1, r, hn_k, st);
+            unsigned long long* fin = dq->zero_slot([acc](unsigned long long v) { acc->final_residual = bits_to_double(v); });
+            unsigned long long* r0b = dq->zero_slot([acc](unsigned long long v) { acc->initial_residual = bits_to_double(v); });
+            const double reltol = P.opt.reltol;
+            PoissonImpl* impl = &P;
+            unsigned long long* miss = dq->zero_slot([acc, k, reltol, impl](unsigned long long v) {
+                if (v && impl->soft_miss) {
+                    ++impl->misses;
+                } else if (v) {
+                    GUST_FAIL("gust: the Poisson solve did not converge in its %d fixed GMRES steps (residual %g of %g; "
+                              "solver.poisson_reltol = %g)", k, acc->final_residual, acc->initial_residual, reltol);
+                }
+            });
+            gm_check_kernel<<<1, 1, 0, st>>>(gm, hn_k, fin, miss, r0b);
+            GUST_HIP(hipGetLastError());
+            phi_in_out(P, phi, x, 1, st);
+            fill_phi_ghosts(P, phi, b, st);
+            if (result) {
+                dq->then([acc, result] { *result = *acc; });
+            }
+            if (fx) { solve_fluxes(P, b, coeffs, x, fx, fy, fz, st); }
+            dq.finish();
+            return;
+        }
         // The multigrid hierarchy is set up at the first preconditioner call:
         // a right-hand side already within tolerance needs none.
         bool hierarchy = false;
@@ -2240,26 +2438,7 @@ int gust_poisson_solve(gust_poisson* p, gust_array4* rhs, gust_array4* phi, cons
                       "%g of %g; solver.poisson_reltol = %g, solver.poisson_abstol = %g)",
                       max_iter, acc->final_residual, acc->initial_residual, reltol, abstol);
         }
-        if (fx) {
-            const Ctx<double> c = P.L0.ctx(b);
-            const MirrorAcc<double> mm = mirror<double>(x, P.nx, P.ny, P.nz, P.per, b);
-            gust_array4* F[3] = {fx, fy, fz};
-            const gust_array4* beta3[3] = {coeffs ? coeffs->x : nullptr, coeffs ? coeffs->y : nullptr,
-                                           coeffs ? coeffs->z : nullptr};
-            for (int d = 0; d < 3; ++d) {
-                IBox fb;
-                for (int q = 0; q < 3; ++q) { fb.lo[q] = P.geom.lo[q]; fb.hi[q] = P.geom.hi[q] + (q == d); }
-                const double* bp = beta3[d] ? static_cast<const double*>(beta3[d]->p) +
-                                                  offset_of(*beta3[d], fb.lo[0], fb.lo[1], fb.lo[2])
-                                            : nullptr;
-                const int64_t ne = int64_t(fb.len(0)) * fb.len(1) * fb.len(2);
-                flux_kernel<double><<<blocks_for(ne), kThreads, 0, st>>>(
-                    d, c, mm, bp, beta3[d] ? beta3[d]->jstride : 0, beta3[d] ? beta3[d]->kstride : 0,
-                    static_cast<double*>(F[d]->p) + offset_of(*F[d], fb.lo[0], fb.lo[1], fb.lo[2]), F[d]->jstride,
-                    F[d]->kstride);
-                GUST_HIP(hipGetLastError());
-            }
-        }
+        if (fx) { solve_fluxes(P, b, coeffs, x, fx, fy, fz, st); }
         dq.finish();
     });
 }

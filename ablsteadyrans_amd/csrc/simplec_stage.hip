@@ -143,9 +143,25 @@ struct SimplecStage {
     gust_closure_fields cfv{};
     double res[6]{}, mon[3]{};
     int32_t nmon = 0;
+    // One iteration captured as a HIP graph (its kernels and the copy of
+    // its checks), the checks' callbacks and the info they fill.
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    Deferred::Recorded rec;
+    gust_simplec_info ginfo{};
+
+    void drop_graph()
+    {
+        if (exec) { (void)hipGraphExecDestroy(exec); }
+        if (graph) { (void)hipGraphDestroy(graph); }
+        exec = nullptr;
+        graph = nullptr;
+        rec = Deferred::Recorded{};
+    }
 
     ~SimplecStage()
     {
+        drop_graph();
         if (solver) { gust_poisson_destroy(solver); }
     }
 
@@ -289,6 +305,49 @@ void iterate(SimplecStage& s, gust_state* q, const gust_array4* dtau, gust_simpl
         std::rethrow_exception(e);
     }
     dq.resolve();
+}
+
+// One iteration recorded into the stage's graph (not run): the stream is
+// captured, the iteration's launches and its checks' copy become the graph.
+// The caller has run an iteration on the same stream, state and dtau first,
+// so every buffer and table the iteration uses exists (nothing allocates or
+// uploads during the capture), and the pressure solver has fixed steps.
+void capture(SimplecStage& s, gust_state* q, const gust_array4* dtau, hipStream_t st)
+{
+    s.drop_graph();
+    if (st == nullptr) {
+        GUST_FAIL("gust: SimplecStage::capture: the null stream cannot be captured; use a stream of your own");
+    }
+    if (s.opt.sweep.kind != GUST_SWEEP_ZJACOBI) {
+        GUST_FAIL("gust: SimplecStage::capture: a captured iteration needs z-line Jacobi sweeps (steady.sweep = "
+                  "zjacobi); the LU-SGS sweeps upload their plan at every call");
+    }
+    Deferred dq(st);
+    Deferred::Scope scope(dq);
+    dq.begin_recording();
+    GUST_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    try {
+        iterate_body(s, q, dtau, &s.ginfo, st);
+        s.rec = dq.end_recording();
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(st, &g);
+        if (g) { (void)hipGraphDestroy(g); }
+        (void)dq.end_recording();
+        throw;
+    }
+    GUST_HIP(hipStreamEndCapture(st, &s.graph));
+    GUST_HIP(hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0));
+}
+
+// The captured iteration once: launch, wait, the checks' callbacks in order.
+void replay(SimplecStage& s, gust_simplec_info* info, hipStream_t st)
+{
+    if (!s.exec) { GUST_FAIL("gust: SimplecStage: no captured iteration (capture first)"); }
+    GUST_HIP(hipGraphLaunch(s.exec, st));
+    GUST_HIP(hipStreamSynchronize(st));
+    s.rec.replay();
+    if (info) { *info = s.ginfo; }
 }
 
 }   // namespace
@@ -548,6 +607,142 @@ static void solve_loop(const char* op, gust_simplec* s, gust_state* q, const gus
     res.nmon = nmon;
     out->hold = hold;
     check_rc(gust_steady_references(crit, out->res_ref, 6));
+}
+
+int gust_simplec_set_pressure_steps(gust_simplec* s, int32_t steps)
+{
+    return guarded([&] {
+        if (!s) { GUST_FAIL("gust: SimplecStage: null stage"); }
+        SimplecStage& st = *as(s);
+        check_rc(gust_poisson_set_fixed_steps(st.solver, steps));
+        st.drop_graph();
+    });
+}
+
+int gust_simplec_capture(gust_simplec* s, gust_state* q, const gust_array4* dtau, void* stream)
+{
+    return guarded([&] {
+        if (!s || !q || !dtau) { GUST_FAIL("gust: SimplecStage::capture: null argument"); }
+        SimplecStage& st = *as(s);
+        check_field("SimplecStage::capture", "dtau", dtau, st.g, GUST_LOC_CELL, 0);
+        capture(st, q, dtau, hipStream_t(stream));
+    });
+}
+
+int gust_simplec_replay(gust_simplec* s, gust_simplec_info* info, void* stream)
+{
+    return guarded([&] {
+        if (!s) { GUST_FAIL("gust: SimplecStage::replay: null stage"); }
+        replay(*as(s), info, hipStream_t(stream));
+    });
+}
+
+int gust_steady_solve_graphed(gust_simplec* s, gust_state* q, const gust_driver_options* opt, int32_t pressure_steps,
+                              gust_driver_result* result, void* stream)
+{
+    return guarded([&] {
+        const char* op = "Driver::run";
+        if (!s || !q || !opt || !result) { GUST_FAIL("gust: %s: null argument", op); }
+        if (pressure_steps < 0 || pressure_steps > 200) {
+            GUST_FAIL("gust: %s: pressure_steps must be 0 (from the first iteration) or 1..200", op);
+        }
+        SimplecStage& st = *as(s);
+        hipStream_t hs = hipStream_t(stream);
+        // The null stream cannot be captured: the loop then runs on a stream
+        // of its own (default priority), after the null stream's work, and
+        // the null stream waits for it at the end.
+        struct OwnStream {
+            hipStream_t s = nullptr;
+            ~OwnStream()
+            {
+                if (s) {
+                    (void)hipStreamSynchronize(s);
+                    (void)gust_release_stream(s);
+                    (void)hipStreamDestroy(s);
+                }
+            }
+        } own;
+        if (hs == nullptr) {
+            GUST_HIP(hipStreamSynchronize(nullptr));
+            GUST_HIP(hipStreamCreateWithFlags(&own.s, hipStreamNonBlocking));
+            hs = own.s;
+            stream = own.s;
+        }
+        if (opt->max_iterations < 1) { GUST_FAIL("gust: %s: steady.max_iterations must be >= 1", op); }
+        if (!(opt->cfl0 > 0 && std::isfinite(opt->cfl0)) || !(opt->cfl_max >= opt->cfl0 && std::isfinite(opt->cfl_max))) {
+            GUST_FAIL("gust: %s: steady.cfl0 must be > 0 and steady.cfl_max >= steady.cfl0", op);
+        }
+        check_rc(gust_simplec_set_pressure_steps(s, 0));
+        struct Restore {
+            gust_simplec* s;
+            gust_poisson* p;
+            ~Restore()
+            {
+                gust_simplec_set_pressure_steps(s, 0);
+                gust_poisson_fixed_steps_misses(p, 0, nullptr);
+            }
+        } restore{s, st.solver};
+        int64_t misses = 0;
+        check_rc(gust_poisson_fixed_steps_misses(st.solver, 1, &misses));
+        int steps = pressure_steps;
+        const char* names[6] = {"u", "v", "w", "p", "theta", "tke"};
+        const char* mnames[3] = {"max_speed", "mean_tke", "mean_ustar"};
+        const double mscales[3] = {st.opt.scales.velocity, st.opt.scales.tke, 0.1};
+        const int nmon = st.wall ? 3 : 2;
+        gust_steady* crit = nullptr;
+        check_rc(gust_steady_create(&opt->criterion, 6, names, nmon, mnames, mscales, &crit));
+        struct Holder {
+            gust_steady* c;
+            ~Holder() { gust_steady_destroy(c); }
+        } hold_c{crit};
+        const int hold = opt->criterion.residual_ref_iterations;
+        Field dt;
+        dt.alloc(loc_box(st.g, GUST_LOC_CELL), 1, hs);
+        gust_array4 nu = st.phys.use_closure ? st.cf[0].v : *st.phys.nu;
+        double cfl = opt->cfl0, r = 0;
+        gust_steady_result v{};
+        int it = 0;
+        check_rc(gust_simplec_init(s, q, stream));
+        while (it < opt->max_iterations) {
+            check_rc(gust_local_dtau(&st.g, q->f[0], q->f[1], q->f[2], &st.z, &st.J, &nu, cfl, &dt.v, stream));
+            if (steps == 0) {
+                // GMRES to the tolerance, its count sets the fixed steps
+                gust_simplec_info inf{};
+                iterate(st, q, &dt.v, &inf, stream);
+                steps = std::min(200, std::max(1, inf.pressure.poisson.iterations + 1));
+                check_rc(gust_simplec_set_pressure_steps(s, steps));
+            } else if (!st.exec) {
+                // with the steps fixed: the warm-up (buffers, tables, the
+                // solver's work space at this step count), then the capture
+                check_rc(gust_simplec_set_pressure_steps(s, steps));
+                iterate(st, q, &dt.v, nullptr, stream);
+                capture(st, q, &dt.v, hs);
+            } else {
+                replay(st, nullptr, hs);
+                int64_t now = 0;
+                check_rc(gust_poisson_fixed_steps_misses(st.solver, 1, &now));
+                if (now != misses) {
+                    misses = now;
+                    steps = std::min(200, steps + 2);
+                    check_rc(gust_simplec_set_pressure_steps(s, steps));   // drops the graph
+                }
+            }
+            ++it;
+            check_rc(gust_steady_add(crit, st.res, 6, st.mon, nmon));
+            check_rc(gust_steady_relative_residual(crit, st.res, 6, hold, r, &r));
+            check_rc(gust_steady_verdict(crit, &v, nullptr, 0));
+            if (v.status != GUST_STEADY_NOT_STEADY) { break; }
+            check_rc(gust_steady_next_cfl(crit, hold, cfl, r, opt->cfl0, opt->cfl_max, opt->ser_exponent,
+                                          opt->ser_max_growth, &cfl));
+        }
+        result->iterations = it;
+        result->status = v.status;
+        result->cfl = cfl;
+        result->relative_residual = r;
+        for (int n = 0; n < 6; ++n) { result->res[n] = st.res[n]; }
+        for (int n = 0; n < 3; ++n) { result->mon[n] = st.mon[n]; }
+        result->nmon = nmon;
+    });
 }
 
 int gust_steady_solve(gust_simplec* s, gust_state* q, const gust_driver_options* opt, gust_driver_result* result,

@@ -390,6 +390,37 @@ class SimplecStage:
         _lib.check(_lib.lib.gust_simplec_nu(self._h, C.byref(v)))
         return v
 
+    def set_pressure_steps(self, steps: int) -> None:
+        """k >= 1: the pressure solve runs k fixed GMRES steps on the device
+        (what a captured iteration needs); 0: GMRES to the tolerance."""
+        _lib.check(_lib.lib.gust_simplec_set_pressure_steps(self._h, int(steps)))
+
+    def capture(self, q: DeviceState, dtau: Fab, stream=None) -> None:
+        """Records one iteration on (q, dtau) as a HIP graph (not run); run
+        one iteration on them first, with the pressure steps fixed."""
+        self._capture_views = (q.c(), dtau.array4())
+        qc, dv = self._capture_views
+        _lib.check(_lib.lib.gust_simplec_capture(self._h, C.byref(qc), C.byref(dv), _stream(stream)))
+
+    def replay(self, stream=None):
+        """Runs the captured iteration (one launch, one wait, the checks);
+        returns its info."""
+        info = _lib.SimplecInfoC()
+        _lib.check(_lib.lib.gust_simplec_replay(self._h, C.byref(info), _stream(stream)))
+        return info
+
+    def solve_graphed(self, q: DeviceState, opt: DriverOptions | None = None, pressure_steps: int = 0,
+                      stream=None) -> _lib.DriverResultC:
+        """Driver::run with the iteration captured and replayed
+        (gust_steady_solve_graphed; pressure_steps 0: the first iteration's
+        GMRES count + 1)."""
+        qc = q.c()
+        oc = (opt or DriverOptions()).c()
+        r = _lib.DriverResultC()
+        _lib.check(_lib.lib.gust_steady_solve_graphed(self._h, C.byref(qc), C.byref(oc), int(pressure_steps),
+                                                      C.byref(r), _stream(stream)))
+        return r
+
     def solve(self, q: DeviceState, opt: DriverOptions | None = None, stream=None) -> _lib.DriverResultC:
         """Driver::run with this stage (Source/Driver.cpp:403-492)."""
         qc = q.c()

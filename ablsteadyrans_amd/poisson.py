@@ -44,6 +44,7 @@ class PoissonOptions:
     post_sweeps: int = 2
     coarse_sweeps: int = 16
     max_levels: int = 0
+    fixed_steps: int = 0   # k > 0: k GMRES steps on the device, no host reads (graph capture)
 
     def c(self) -> _lib.PoissonOptionsC:
         o = _lib.PoissonOptionsC()

@@ -205,6 +205,9 @@ typedef struct gust_poisson_options {
     int32_t precision;
     int32_t pre_sweeps, post_sweeps, coarse_sweeps;   /* V(2,2), 16 on the coarsest level */
     int32_t max_levels;    /* multigrid levels at most (0: coarsen while x or y halves) */
+    int32_t fixed_steps;   /* 0: GMRES(restart) to the tolerance, reading each Arnoldi step on the host;
+                              k in 1..200: k Arnoldi steps on the device, no host reads (a captured graph
+                              can hold the solve), the tolerance checked at the end */
 } gust_poisson_options;
 
 /* Source/Poisson.H:96-103 (FaceCoeffs): optional face coefficients (null =
@@ -1056,6 +1059,46 @@ typedef struct gust_driver_result {
  *   Source/Driver.cpp:403-492 (step, verdict, run). */
 int gust_steady_solve(gust_simplec* s, gust_state* q, const gust_driver_options* opt, gust_driver_result* result,
                       void* stream);
+
+/* ---- One SIMPLEC iteration as a HIP graph ----------------------------------
+ * No counterpart in the reference (its iteration is a sequence of launches
+ * and host reads): the iteration's launches, its checks and the copy of
+ * their values recorded once and replayed, one launch and one wait per
+ * iteration. The pressure solve runs a fixed number of GMRES steps on the
+ * device (gust_poisson_options.fixed_steps); its tolerance is still checked
+ * (an error after the replay when the steps were too few). */
+
+/* The stage's pressure solver runs k fixed device GMRES steps (k >= 1) or,
+ * k = 0, GMRES to the tolerance with host reads (the default). Drops any
+ * captured iteration. */
+int gust_poisson_set_fixed_steps(gust_poisson* p, int32_t steps);
+/* soft = 1: a fixed-step solve that misses its tolerance is counted, not an
+ * error (a caller that adapts the step count); *misses (may be null) = the
+ * count so far. */
+int gust_poisson_fixed_steps_misses(gust_poisson* p, int32_t soft, int64_t* misses);
+int gust_simplec_set_pressure_steps(gust_simplec* s, int32_t steps);
+
+/* Records one iteration on (q, dtau) into the stage's graph without running
+ * it. Before it, run one iteration with gust_simplec_iterate on the same
+ * stream, state and dtau and with the pressure steps fixed, so that every
+ * buffer and table exists; the graph holds q's and dtau's addresses. */
+int gust_simplec_capture(gust_simplec* s, gust_state* q, const gust_array4* dtau, void* stream);
+
+/* Runs the captured iteration: one graph launch, one wait, then the checks
+ * (the first failed one is the error, with the reference's message), the
+ * norms (gust_simplec_norms) and info (may be null). */
+int gust_simplec_replay(gust_simplec* s, gust_simplec_info* info, void* stream);
+
+/* gust_steady_solve with the iteration captured and replayed: the first
+ * iteration runs eagerly with GMRES to the tolerance and its GMRES count + 1
+ * becomes the fixed pressure steps (pressure_steps > 0 fixes them instead);
+ * the next runs eagerly with them and is captured; from then on each
+ * iteration is one replay (local dtau at the current CFL is the one launch
+ * outside the graph). A replay whose pressure solve misses its tolerance
+ * keeps its (less converged) correction, and the steps grow by 2 and the
+ * iteration is captured again. */
+int gust_steady_solve_graphed(gust_simplec* s, gust_state* q, const gust_driver_options* opt, int32_t pressure_steps,
+                              gust_driver_result* result, void* stream);
 
 /* How a level of grid sequencing starts and stops (the driver's seeding,
  * Source/Driver.cpp:275-313, and the level loop of
