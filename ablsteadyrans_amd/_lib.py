@@ -368,6 +368,9 @@ _sig = {
     "gust_jacobi_plan_info": (C.c_int, [C.c_void_p, _P(C.c_int64), _P(C.c_int32)]),
     "gust_jacobi_plan_destroy": (None, [C.c_void_p]),
     "gust_jacobi_set_schedule": (C.c_int, [C.c_int32]),
+    "gust_jacobi_tuned": (C.c_int, [C.c_int32, _P(C.c_int32), _P(C.c_int32), _P(C.c_int32), _P(C.c_int32),
+                                    _P(C.c_int32), _P(C.c_float), C.c_char_p, C.c_int64]),
+    "gust_jacobi_tune_reset": (C.c_int, []),
     "gust_build_mesh": (C.c_int, [_P(GeometryC), _P(C.c_double), _P(C.c_double), _P(Array4), _P(Array4),
                                   _P(Array4), _P(Array4), _P(Array4), _P(Array4), C.c_void_p]),
     "gust_poisson_default_options": (C.c_int, [_P(PoissonOptionsC)]),

@@ -44,10 +44,39 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kStagedMaxTiles = 1024;
 
-// The z-line Jacobi schedule: 0 automatic, 1 streaming, 2 LDS-staged (where
-// it fits).
+// The z-line Jacobi schedule: 0 autotuned, 1 streaming, 2 LDS-staged (where
+// it fits), 3 the fixed rule (staged under kStagedMaxTiles tiles).
 std::atomic<int> g_jacobi_schedule{0};
 int jacobi_schedule() { return g_jacobi_schedule.load(std::memory_order_relaxed); }
+
+// The autotuner's candidates: the LDS-staged kernel, the streaming kernel
+// with c' in LDS at prefetch depths 2, 4 and 8, the streaming kernel with c'
+// in global memory (no LDS limit on occupancy). Every one gives the same
+// bits; the first run of a plan whose shapes are new times each (all its
+// sweeps, HIP events on its stream) and keeps the fastest for those shapes.
+enum TuneChoice : int { kChoiceStaged = 0, kChoiceP4 = 1, kChoiceP2 = 2, kChoiceP8 = 3, kChoiceGlobal = 4,
+                        kNumChoices = 5 };
+const char* kChoiceNames[kNumChoices] = {"staged", "streaming-p4", "streaming-p2", "streaming-p8",
+                                         "streaming-global-cprime"};
+
+struct TuneKey {
+    int dtype = 0, sweeps = 0;
+    std::vector<int> shape;   // (nx, ny, nz) of every system
+    bool operator<(const TuneKey& o) const
+    {
+        if (dtype != o.dtype) { return dtype < o.dtype; }
+        if (sweeps != o.sweeps) { return sweeps < o.sweeps; }
+        return shape < o.shape;
+    }
+};
+
+struct TuneRecord {
+    int choice = kChoiceP4;
+    float ms[kNumChoices] = {-1, -1, -1, -1, -1};
+};
+
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneRecord> g_tuned;
 
 // One system of a Jacobi launch. Pointers are at the valid box's low corner.
 template <typename T>
@@ -630,6 +659,11 @@ struct JacobiPlan final : JacobiPlanBase {
     bool owned = false;   // buffers from hipMalloc (a plan handle), not the stream workspace
     bool chg_zeroed = false;   // chg already holds zeros (a fresh deferred slot)
     int staged_w = 0;          // > 0: the LDS-staged schedule, W columns per LDS row
+    int wmax_ = 1;             // the widest tile (columns)
+    bool staged_fits = false;  // a tile's operands fit the LDS
+    int kp = 4;                // the streaming kernel's prefetch depth
+    bool tune = false;         // the first run times the candidates (schedule 0)
+    bool no_alloc = false;     // scratch carved from the caller's bytes
 
     ~JacobiPlan() override
     {
@@ -699,13 +733,18 @@ struct JacobiPlan final : JacobiPlanBase {
             const size_t staged = 5 * size_t(nzmax) * size_t(wmax) * sizeof(T);
             const int sched = jacobi_schedule();
             const bool fits = staged <= device_info().lds_per_block;
-            if (fits && (sched == 2 || (sched == 0 && tiles < kStagedMaxTiles))) { staged_w = wmax; }
+            wmax_ = wmax;
+            staged_fits = fits;
+            if (fits && (sched == 2 || ((sched == 0 || sched == 3) && tiles < kStagedMaxTiles))) { staged_w = wmax; }
+            tune = sched == 0 && !ext;
         }
+        no_alloc = ext != nullptr;
         // c' of the streaming kernel: in LDS (default) or in global memory
         // (GUST_JACOBI_CPRIME = global: no LDS limit on occupancy, c'
         // round-trips through L2).
         const char* cp = std::getenv("GUST_JACOBI_CPRIME");
         const bool force_global = cp && std::strcmp(cp, "global") == 0;
+        if (force_global) { tune = false; }
         const bool use_lds = !force_global && lds <= device_info().lds_per_block;
         const size_t tab_bytes = tab.size() * sizeof(JacobiSys<T>);
         const size_t cbytes = use_lds ? 0 : size_t(tiles) * lds;
@@ -775,16 +814,60 @@ struct JacobiPlan final : JacobiPlanBase {
         if (cglob) {
             zjacobi_sweep_kernel<T, F, L, false><<<tiles, kWave, 0, st>>>(dtab, nsys, tiles, dst_scratch, chg, cglob,
                                                                        nzmax);
+        } else if (kp == 2) {
+            launch_lds<F, L, 2>(dst_scratch, st);
+        } else if (kp == 8) {
+            launch_lds<F, L, 8>(dst_scratch, st);
         } else {
-            const size_t lds = size_t(kWave) * size_t(nzmax) * sizeof(T);
-            if (lds > 65536) {
-                GUST_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&zjacobi_sweep_kernel<T, F, L, true>),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-            }
-            zjacobi_sweep_kernel<T, F, L, true><<<tiles, kWave, lds, st>>>(dtab, nsys, tiles, dst_scratch, chg,
-                                                                           cglob, nzmax);
+            launch_lds<F, L, 4>(dst_scratch, st);
         }
         GUST_HIP(hipGetLastError());
+    }
+
+    template <bool F, bool L, int P>
+    void launch_lds(int dst_scratch, hipStream_t st)
+    {
+        const size_t lds = size_t(kWave) * size_t(nzmax) * sizeof(T);
+        if (lds > 65536) {
+            GUST_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&zjacobi_sweep_kernel<T, F, L, true, P>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+        }
+        zjacobi_sweep_kernel<T, F, L, true, P><<<tiles, kWave, lds, st>>>(dtab, nsys, tiles, dst_scratch, chg,
+                                                                          cglob, nzmax);
+    }
+
+    // Makes candidate c this plan's schedule (false: it does not apply).
+    bool apply_choice(int c, hipStream_t st)
+    {
+        const size_t lds = size_t(kWave) * size_t(nzmax) * sizeof(T);
+        if (c == kChoiceStaged) {
+            if (!staged_fits) { return false; }
+            staged_w = wmax_;
+            return true;
+        }
+        staged_w = 0;
+        if (c == kChoiceGlobal) {
+            if (no_alloc) { return false; }
+            if (!cglob) { cglob = static_cast<T*>(Workspace::get(st, 5, size_t(tiles) * lds)); }
+            return true;
+        }
+        if (lds > device_info().lds_per_block) { return false; }
+        if (!owned) { cglob = nullptr; }
+        kp = c == kChoiceP2 ? 2 : c == kChoiceP8 ? 8 : 4;
+        return true;
+    }
+
+    TuneKey key(int sweeps) const
+    {
+        TuneKey k;
+        k.dtype = sizeof(T) == 8 ? 0 : 1;
+        k.sweeps = sweeps;
+        for (const auto& t : tab) {
+            k.shape.push_back(t.nx);
+            k.shape.push_back(t.ny);
+            k.shape.push_back(t.nz);
+        }
+        return k;
     }
 
     void sweep(bool first, bool last, int dst_scratch, hipStream_t st) override
@@ -798,6 +881,59 @@ struct JacobiPlan final : JacobiPlanBase {
     void run(int sweeps, double* changes_dev, hipStream_t st) override
     {
         if (sweeps < 1) { GUST_FAIL("gust: zline_jacobi: sweeps must be >= 1 (got %d)", sweeps); }
+        if (tune && !owned) {
+            tune = false;
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            GUST_HIP(hipStreamIsCapturing(st, &cs));
+            autotune(sweeps, st, cs == hipStreamCaptureStatusNone);
+        }
+        run_once(sweeps, changes_dev, st);
+    }
+
+    // The first plan of a set of shapes times every candidate that applies
+    // (each run gives the same bits: delta starts at 0, the change maxima are
+    // idempotent); later plans of those shapes take the recorded fastest.
+    void autotune(int sweeps, hipStream_t st, bool may_time)
+    {
+        const TuneKey k = key(sweeps);
+        {
+            std::lock_guard<std::mutex> lock(g_tune_mu);
+            auto it = g_tuned.find(k);
+            if (it != g_tuned.end()) {
+                apply_choice(it->second.choice, st);
+                return;
+            }
+        }
+        if (!may_time) { return; }   // under graph capture: the fixed rule
+        TuneRecord rec;
+        hipEvent_t e0, e1;
+        GUST_HIP(hipEventCreate(&e0));
+        GUST_HIP(hipEventCreate(&e1));
+        float best = -1;
+        for (int c = 0; c < kNumChoices; ++c) {
+            if (!apply_choice(c, st)) { continue; }
+            run_once(sweeps, nullptr, st);   // the warm-up of this candidate
+            GUST_HIP(hipEventRecord(e0, st));
+            run_once(sweeps, nullptr, st);
+            GUST_HIP(hipEventRecord(e1, st));
+            GUST_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            GUST_HIP(hipEventElapsedTime(&ms, e0, e1));
+            rec.ms[c] = ms;
+            if (best < 0 || ms < best) {
+                best = ms;
+                rec.choice = c;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        apply_choice(rec.choice, st);
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        g_tuned[k] = rec;
+    }
+
+    void run_once(int sweeps, double* changes_dev, hipStream_t st)
+    {
         if (!chg_zeroed) {
             zero_u64_kernel<<<(nsys + 255) / 256, 256, 0, st>>>(chg, nsys);
             GUST_HIP(hipGetLastError());
@@ -1411,10 +1547,39 @@ int gust_jacobi_plan_sweep(gust_jacobi_plan* plan, int first, int last, int into
 int gust_jacobi_set_schedule(int32_t schedule)
 {
     return guarded([&] {
-        if (schedule < 0 || schedule > 2) {
-            GUST_FAIL("gust: jacobi_set_schedule: schedule must be 0 (automatic), 1 (streaming) or 2 (staged)");
+        if (schedule < 0 || schedule > 3) {
+            GUST_FAIL("gust: jacobi_set_schedule: schedule must be 0 (autotuned), 1 (streaming), 2 (staged) or 3 "
+                      "(the fixed rule)");
         }
         g_jacobi_schedule.store(schedule, std::memory_order_relaxed);
+    });
+}
+
+int gust_jacobi_tuned(int32_t index, int32_t* count, int32_t* dtype, int32_t* sweeps, int32_t* nsys,
+                      int32_t* choice, float* ms, char* name, int64_t cap)
+{
+    return guarded([&] {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        if (count) { *count = int32_t(g_tuned.size()); }
+        if (index < 0 || index >= int32_t(g_tuned.size())) { return; }
+        auto it = g_tuned.begin();
+        std::advance(it, index);
+        if (dtype) { *dtype = it->first.dtype; }
+        if (sweeps) { *sweeps = it->first.sweeps; }
+        if (nsys) { *nsys = int32_t(it->first.shape.size() / 3); }
+        if (choice) { *choice = it->second.choice; }
+        if (ms) {
+            for (int c = 0; c < kNumChoices; ++c) { ms[c] = it->second.ms[c]; }
+        }
+        if (name && cap > 0) { std::snprintf(name, size_t(cap), "%s", kChoiceNames[it->second.choice]); }
+    });
+}
+
+int gust_jacobi_tune_reset(void)
+{
+    return guarded([&] {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        g_tuned.clear();
     });
 }
 

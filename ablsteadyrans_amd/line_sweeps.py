@@ -79,14 +79,41 @@ def check_line_options(opt: LineSweepOptions) -> list[str]:
     return [_lib.lib.gust_last_error().decode()]
 
 
-JACOBI_AUTO, JACOBI_STREAMING, JACOBI_STAGED = 0, 1, 2
+JACOBI_AUTO, JACOBI_STREAMING, JACOBI_STAGED, JACOBI_RULE = 0, 1, 2, 3
 
 
 def set_jacobi_schedule(schedule: int) -> None:
     """The z-line Jacobi kernel schedule for plans made afterwards (process
-    wide): JACOBI_AUTO (few tiles staged in LDS, many streamed),
-    JACOBI_STREAMING or JACOBI_STAGED. Every schedule gives the same bits."""
+    wide): JACOBI_AUTO (autotuned per set of system shapes: the first line
+    solve times the candidates and the fastest is kept), JACOBI_STREAMING,
+    JACOBI_STAGED or JACOBI_RULE (few tiles staged in LDS, many streamed).
+    Every schedule gives the same bits."""
     _lib.check(_lib.lib.gust_jacobi_set_schedule(int(schedule)))
+
+
+TUNE_CHOICES = ("staged", "streaming-p4", "streaming-p2", "streaming-p8", "streaming-global-cprime")
+
+
+def jacobi_tuned() -> list:
+    """The autotuner's records: dicts of dtype, sweeps, systems, the chosen
+    candidate and every candidate's time in ms (None: did not apply)."""
+    n = C.c_int32(0)
+    _lib.check(_lib.lib.gust_jacobi_tuned(-1, C.byref(n), None, None, None, None, None, None, 0))
+    out = []
+    for i in range(n.value):
+        dt, sw, ns, ch = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        ms = (C.c_float * 5)()
+        _lib.check(_lib.lib.gust_jacobi_tuned(i, None, C.byref(dt), C.byref(sw), C.byref(ns), C.byref(ch), ms,
+                                              None, 0))
+        out.append({"dtype": "f64" if dt.value == 0 else "f32", "sweeps": sw.value, "systems": ns.value,
+                    "choice": TUNE_CHOICES[ch.value],
+                    "ms": {TUNE_CHOICES[c]: (round(float(ms[c]), 4) if ms[c] >= 0 else None) for c in range(5)}})
+    return out
+
+
+def jacobi_tune_reset() -> None:
+    """Forgets the autotuner's records."""
+    _lib.check(_lib.lib.gust_jacobi_tune_reset())
 
 
 class LineSystem:

@@ -648,6 +648,8 @@ def main():
     print("bench: steady solve of the 200x200x64 Gaussian hill (mixed, two levels of grid sequencing)",
           file=sys.stderr, flush=True)
     c2s, fs = _c2_sequenced_run(2)
+    from ablsteadyrans_amd import line_sweeps as _LS
+    tuned = [r for r in _LS.jacobi_tuned() if r["systems"] >= 2][:8]
     errs = _field_errors(fm, fd)
     errs_seq = _field_errors(fs, fd)
     del fm, fd, fs
@@ -740,6 +742,7 @@ def main():
                   "fp64_vcycle_frac_algorithmic": round(_vcycle_alg_bytes(8) / (vc64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                   "fp64_vcycle_gbs_moved": round(_vcycle_bytes(8) / (vc64 * 1e-3) / 1e9, 1),
                   "fp32_vcycle_gbs_moved": round(_vcycle_bytes(4) / (vc32 * 1e-3) / 1e9, 1),
+                  "jacobi_autotuner_c2": tuned,
                   "reference_decks": decks,
                   "serving": serving,
                   "serving_p99_s": serving["served_mixed_8_slots"]["all"]["p99_s"] if serving else None,

@@ -159,12 +159,26 @@ int gust_jacobi_plan_sweep(gust_jacobi_plan* plan, int first, int last, int into
 /* Unknowns and 64-column tiles of the plan. */
 int gust_jacobi_plan_info(const gust_jacobi_plan* plan, int64_t* unknowns, int32_t* tiles);
 /* The z-line Jacobi kernel schedule for plans made after the call (process
- * wide): 0 automatic (launches of fewer than 1024 tiles stage each tile's
- * operands in LDS before its Thomas recurrences, more tiles stream them),
- * 1 streaming always, 2 staged wherever a tile fits the LDS. Every schedule
- * gives the same bits.
+ * wide): 0 autotuned (the default: the first line solve of a set of system
+ * shapes and sweep count times every candidate -- the LDS-staged kernel,
+ * the streaming kernel with c' in LDS at prefetch depths 2, 4, 8, the
+ * streaming kernel with c' in global memory -- over all its sweeps with HIP
+ * events and keeps the fastest for those shapes; under graph capture an
+ * untimed shape takes rule 3), 1 streaming always, 2 staged wherever a tile
+ * fits the LDS, 3 the fixed rule (fewer than 1024 tiles staged, more
+ * streamed). Plan handles (gust_jacobi_plan_create) take rule 3 under 0.
+ * Every schedule gives the same bits.
  * No counterpart in the reference (a launch-shape choice of this library). */
 int gust_jacobi_set_schedule(int32_t schedule);
+
+/* The autotuner's records: *count = their number; record index (in key
+ * order) gives the dtype (0 f64, 1 f32), sweeps, the number of systems, the
+ * chosen candidate, the five candidates' times in ms (-1: did not apply)
+ * and the choice's name. gust_jacobi_tune_reset forgets them.
+ * No counterpart in the reference. */
+int gust_jacobi_tuned(int32_t index, int32_t* count, int32_t* dtype, int32_t* sweeps, int32_t* nsys,
+                      int32_t* choice, float* ms, char* name, int64_t cap);
+int gust_jacobi_tune_reset(void);
 void gust_jacobi_plan_destroy(gust_jacobi_plan* plan);
 
 /* Line LU-SGS sweeps of one system with the given options.

@@ -17,10 +17,11 @@ CASES = [((6, 5, 4), (0, 0)), ((6, 5, 4), (1, 1)), ((7, 4, 5), (1, 0)), ((5, 6, 
          ((8, 1, 4), (1, 1)), ((1, 1, 9), (0, 0)), ((33, 17, 12), (1, 0)), ((64, 64, 48), (1, 0))]
 
 
-@pytest.fixture(params=[1, 2], ids=["streaming", "staged"])
+@pytest.fixture(params=[1, 2, 0], ids=["streaming", "staged", "autotuned"])
 def schedule(request, cuda):
-    """Both z-line Jacobi kernel schedules (the streaming ring and the
-    LDS-staged tile) must give the oracle's bits."""
+    """Every z-line Jacobi kernel schedule (the streaming ring, the LDS-staged
+    tile, and the autotuner's choice among them and the prefetch depths and
+    global c') must give the oracle's bits."""
     import ablsteadyrans_amd as LS
     LS.set_jacobi_schedule(request.param)
     yield request.param
@@ -226,3 +227,50 @@ def test_plan_and_graph_replays_match_the_one_shot_calls(cuda):
             assert np.array_equal(s.delta.numpy(), r), rep
         assert np.array_equal(plan.changes.cpu().numpy(), ch_rep)
     plan.close()
+
+
+def test_autotuner_records_and_reuses_its_choice(cuda):
+    """The first batched solve of a set of shapes times every candidate that
+    applies and keeps the fastest; the same shapes again add no record and
+    give the same bits; the staged candidate is skipped where a tile does
+    not fit the LDS."""
+    import importlib
+    import torch
+    import ablsteadyrans_amd as G
+    LSM = importlib.import_module("ablsteadyrans_amd.line_sweeps")
+    LSM.set_jacobi_schedule(LSM.JACOBI_AUTO)
+    LSM.jacobi_tune_reset()
+    rng = np.random.default_rng(21)
+    specs = [((48, 40, 64), (1, 0), 0), ((40, 48, 64), (0, 1), 1), ((48, 40, 65), (1, 0), 4)]
+    outs = []
+    for rep in range(2):
+        systems = []
+        for t, (n, periodic, loc) in enumerate(specs):
+            A, ap, rhs = L.random_system(n, loc, periodic, seed=500 + t)
+            geom = G.Geometry.of_size(*n, periodic=periodic)
+            d = G.Fab.at_loc(geom, loc, 1, 1, dtype=torch.float64)
+            systems.append(G.LineSystem(loc, geom, _dev(A, (0, 0, 0)), _dev(ap, (0, 0, 0)), _dev(rhs, (0, 0, 0)), d))
+        G.zline_jacobi_batched(systems, 3)
+        torch.cuda.synchronize()
+        outs.append([s.delta.numpy().copy() for s in systems])
+        recs = LSM.jacobi_tuned()
+        print(f"autotuner after pass {rep}: {recs}", flush=True)
+        assert len(recs) == 1
+    rec = recs[0]
+    timed = {k: v for k, v in rec["ms"].items() if v is not None}
+    assert rec["choice"] == min(timed, key=timed.get) and len(timed) >= 4
+    assert rec["sweeps"] == 3 and rec["systems"] == 3
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+    LSM.set_jacobi_schedule(1)
+    systems = []
+    for t, (n, periodic, loc) in enumerate(specs):
+        A, ap, rhs = L.random_system(n, loc, periodic, seed=500 + t)
+        geom = G.Geometry.of_size(*n, periodic=periodic)
+        d = G.Fab.at_loc(geom, loc, 1, 1, dtype=torch.float64)
+        systems.append(G.LineSystem(loc, geom, _dev(A, (0, 0, 0)), _dev(ap, (0, 0, 0)), _dev(rhs, (0, 0, 0)), d))
+    G.zline_jacobi_batched(systems, 3)
+    torch.cuda.synchronize()
+    LSM.set_jacobi_schedule(LSM.JACOBI_AUTO)
+    for a, s in zip(outs[0], systems):
+        assert np.array_equal(a, s.delta.numpy())
