@@ -77,6 +77,7 @@ class PoissonOptionsC(C.Structure):
         ("coarse_sweeps", C.c_int32),
         ("max_levels", C.c_int32),
         ("fixed_steps", C.c_int32),
+        ("seam_copy", C.c_int32),
     ]
 
 

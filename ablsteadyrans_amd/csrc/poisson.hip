@@ -218,6 +218,14 @@ __device__ __forceinline__ void nb_columns(int i, int j, int nx, int ny, int per
     }
 }
 
+// A column on a seam that joins two columns of one colour: the first or the
+// last of a periodic direction of odd length above one.
+__host__ __device__ __forceinline__ bool on_seam(int i, int j, int nx, int ny, int perx, int pery)
+{
+    return (perx && nx > 1 && (nx & 1) && (i == 0 || i == nx - 1)) ||
+           (pery && ny > 1 && (ny & 1) && (j == 0 || j == ny - 1));
+}
+
 // A level's explicit stencil, one bundle of kBundle reals per split slot:
 // the own column's lower entry, 1 / Thomas pivot, (k-1, k, k+1) of the x-,
 // x+, y-, y+ neighbour columns, the own diagonal and upper entries. The
@@ -640,30 +648,43 @@ __global__ __launch_bounds__(64) void zebra_kernel(Split s, int color, int perx,
 // and x_k = e'_k - c'_k x_{k+1}, are affine in the carried value, so each
 // lane composes its levels' maps, a shuffle scan across the wave composes
 // the lanes', and each lane then replays its levels from its carry-in.
-template <typename R, bool kZero, bool kFromR, int M>
-__global__ __launch_bounds__(256) void zebra_scan_kernel(Split s, int color, int perx, int pery,
-                                                         const R* __restrict__ bundle, const R* __restrict__ cpl,
-                                                         const R* __restrict__ rhs, R* __restrict__ x)
+template <typename R, bool kZero, bool kFromR, int M, bool kShadow>
+__device__ __forceinline__ void scan_column(const Split& s, int color, int perx, int pery,
+                                            const R* __restrict__ bundle, const R* __restrict__ cpl,
+                                            const R* __restrict__ rhs, R* __restrict__ x, R* __restrict__ xs,
+                                            int64_t t, int lane)
 {
     using dev::gld;
     using dev::gst;
     constexpr bool kNb = !kZero && !kFromR;
-    const int lane = threadIdx.x & 63;
     const int64_t rowc = int64_t(s.H) * s.ny;
-    const int64_t t = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
     if (t >= rowc) { return; }
     const int ih = int(t % s.H), j = int(t / s.H);
+    if (kShadow) {
+        // The other colour's column t, when it sits on a seam: its values as
+        // they stand (zero for a first sweep) into the copy its own half
+        // sweep reads for its same-colour neighbours.
+        const int io = 2 * ih + ((j + color + 1) & 1);
+        if (io < s.nx && on_seam(io, j, s.nx, s.ny, perx, pery)) {
+            const int64_t qo = int64_t(color ^ 1) * s.plane + t;
+            for (int k = lane; k < s.nz; k += 64) { gst(xs + qo + k * rowc, kZero ? R(0) : gld(x + qo + k * rowc)); }
+        }
+    }
     const int i = 2 * ih + ((j + color) & 1);
     if (i >= s.nx) { return; }   // uniform over the wave
     const int nz = s.nz;
     const int64_t q0 = color * s.plane + t;
     int64_t nb[4] = {0, 0, 0, 0};
     bool ok[4] = {false, false, false, false};
+    bool same[4] = {false, false, false, false};
     if (kNb) {
         int ni[4], nj[4];
         nb_columns(i, j, s.nx, s.ny, perx, pery, ni, nj, ok);
 #pragma unroll
-        for (int p = 0; p < 4; ++p) { nb[p] = s.at(ni[p], nj[p], 0); }
+        for (int p = 0; p < 4; ++p) {
+            nb[p] = s.at(ni[p], nj[p], 0);
+            same[p] = kShadow && ((ni[p] + nj[p]) & 1) == color;
+        }
     }
     // Forward: e_k = bet_k + alp_k e_{k-1} on my levels.
     R alp[M], bet[M], g[M];
@@ -684,8 +705,9 @@ __global__ __launch_bounds__(256) void zebra_scan_kernel(Split s, int color, int
                 R xm[4], x0[4], xp[4];
 #pragma unroll
                 for (int p = 0; p < 4; ++p) {
-                    const R a = gld(x + nb[p] + km * rowc), b0 = gld(x + nb[p] + k * rowc),
-                            cc = gld(x + nb[p] + kp * rowc);
+                    const R* src = same[p] ? xs : x;
+                    const R a = gld(src + nb[p] + km * rowc), b0 = gld(src + nb[p] + k * rowc),
+                            cc = gld(src + nb[p] + kp * rowc);
                     xm[p] = ok[p] ? a : R(0);
                     x0[p] = ok[p] ? b0 : R(0);
                     xp[p] = ok[p] ? cc : R(0);
@@ -744,36 +766,60 @@ __global__ __launch_bounds__(256) void zebra_scan_kernel(Split s, int color, int
     }
 }
 
+// kShadow: a level whose periodic seam joins two columns of one colour.
+// Such a column reads its same-colour neighbour from xs, a copy taken before
+// this half sweep (by the other colour's half sweep, or the prolongation),
+// so every column sees the values the half sweep started from; and the
+// launch refreshes xs for the other colour's seam columns.
+template <typename R, bool kZero, bool kFromR, int M, bool kShadow>
+__global__ __launch_bounds__(256) void zebra_scan_kernel(Split s, int color, int perx, int pery,
+                                                         const R* __restrict__ bundle, const R* __restrict__ cpl,
+                                                         const R* __restrict__ rhs, R* __restrict__ x,
+                                                         R* __restrict__ xs)
+{
+    const int64_t t = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    scan_column<R, kZero, kFromR, M, kShadow>(s, color, perx, pery, bundle, cpl, rhs, x, xs, t,
+                                              int(threadIdx.x & 63));
+}
+
 // The neighbour-column part of one colour's right-hand side,
 // r = b - (neighbour columns) x, for the levels zebra_kernel<kFromR> serves.
+template <typename R>
+__device__ __forceinline__ void offcol_cell(const Split& s, int color, int perx, int pery, const R* __restrict__ bundle,
+                                            const R* __restrict__ b, const R* __restrict__ x, R* __restrict__ r,
+                                            int64_t t)
+{
+    const int64_t rowc = int64_t(s.H) * s.ny;
+    const int ih = int(t % s.H);
+    const int64_t rr = t / s.H;
+    const int j = int(rr % s.ny), k = int(rr / s.ny);
+    const int i = 2 * ih + ((j + color) & 1);
+    if (i >= s.nx) { return; }
+    int ni[4], nj[4];
+    bool ok[4];
+    nb_columns(i, j, s.nx, s.ny, perx, pery, ni, nj, ok);
+    const int64_t q = color * s.plane + t;
+    const int km = k > 0 ? k - 1 : 0, kp = k + 1 < s.nz ? k + 1 : s.nz - 1;
+    R c[kBundle];
+    load_bundle(c, bundle + q * kBundle);
+    R xm[4], x0[4], xp[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const int64_t c0 = s.at(ni[p], nj[p], 0);
+        xm[p] = ok[p] ? x[c0 + km * rowc] : R(0);
+        x0[p] = ok[p] ? x[c0 + k * rowc] : R(0);
+        xp[p] = ok[p] ? x[c0 + kp * rowc] : R(0);
+    }
+    r[q] = b[q] - nb_terms(c, xm, x0, xp);
+}
+
 template <typename R>
 __global__ __launch_bounds__(kThreads) void offcol_kernel(Split s, int color, int perx, int pery,
                                                           const R* __restrict__ bundle, const R* __restrict__ b,
                                                           const R* __restrict__ x, R* __restrict__ r)
 {
-    const int64_t rowc = int64_t(s.H) * s.ny;
     for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < s.plane; t += int64_t(gridDim.x) * blockDim.x) {
-        const int ih = int(t % s.H);
-        const int64_t rr = t / s.H;
-        const int j = int(rr % s.ny), k = int(rr / s.ny);
-        const int i = 2 * ih + ((j + color) & 1);
-        if (i >= s.nx) { continue; }
-        int ni[4], nj[4];
-        bool ok[4];
-        nb_columns(i, j, s.nx, s.ny, perx, pery, ni, nj, ok);
-        const int64_t q = color * s.plane + t;
-        const int km = k > 0 ? k - 1 : 0, kp = k + 1 < s.nz ? k + 1 : s.nz - 1;
-        R c[kBundle];
-        load_bundle(c, bundle + q * kBundle);
-        R xm[4], x0[4], xp[4];
-#pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            const int64_t c0 = s.at(ni[p], nj[p], 0);
-            xm[p] = ok[p] ? x[c0 + km * rowc] : R(0);
-            x0[p] = ok[p] ? x[c0 + k * rowc] : R(0);
-            xp[p] = ok[p] ? x[c0 + kp * rowc] : R(0);
-        }
-        r[q] = b[q] - nb_terms(c, xm, x0, xp);
+        offcol_cell(s, color, perx, pery, bundle, b, x, r, t);
     }
 }
 
@@ -822,9 +868,12 @@ __global__ __launch_bounds__(kThreads) void resid_restrict_kernel(Split f, int p
 }
 
 // x_f += bilinear (in the coarsened directions) interpolation of x_c.
+// xs (may be null): the seam columns' copy the shadowed half sweeps read,
+// given the prolonged values too.
 template <typename R>
 __global__ __launch_bounds__(kThreads) void prolong_kernel(Split f, int rx, int ry, SplitMirror<R> xc,
-                                                           R* __restrict__ xf)
+                                                           R* __restrict__ xf, R* __restrict__ xs, int perx,
+                                                           int pery)
 {
     for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < 2 * f.plane;
          t += int64_t(gridDim.x) * blockDim.x) {
@@ -838,7 +887,9 @@ __global__ __launch_bounds__(kThreads) void prolong_kernel(Split f, int rx, int 
         if (di) { v += (R(1) - wx) * wy * xc(I + di, Jc, k); }
         if (dj) { v += wx * (R(1) - wy) * xc(I, Jc + dj, k); }
         if (di && dj) { v += (R(1) - wx) * (R(1) - wy) * xc(I + di, Jc + dj, k); }
-        xf[t] += v;
+        const R nv = xf[t] + v;
+        xf[t] = nv;
+        if (xs && on_seam(i, j, f.nx, f.ny, perx, pery)) { xs[t] = nv; }
     }
 }
 
@@ -1518,8 +1569,10 @@ void launch_apply(const Level<R>& L, const BcInfo& bc, const int* per, const R* 
 }
 
 // A periodic direction of odd length (more than one cell) puts two columns
-// of one colour side by side across the seam; such a level forms each
-// colour's right-hand side in a pass of its own before the line solves.
+// of one colour side by side across the seam. Such a level either reads
+// the seam neighbours from a copy (the wave-per-column kernels) or forms
+// each colour's right-hand side in a pass of its own before the line solves
+// (the lane-per-column kernel).
 template <typename R>
 bool seam_joins_colour(const Level<R>& L, const int* per)
 {
@@ -1545,14 +1598,38 @@ const R* bundle_of(const Level<R>& L) { return ptr<R>(L.coef); }
 template <typename R>
 const R* cp_of(const Level<R>& L) { return ptr<R>(L.coef) + size_t(kBundle) * size_t(L.slots()); }
 
-template <typename R, bool kZero, bool kFromR, int M>
+// The levels whose half sweeps run one wave per column.
+template <typename R>
+bool scan_level(const Level<R>& L)
+{
+    return int64_t(L.split().H) * L.ny < scan_cols() && L.nz <= 256;
+}
+
+// A seam level smoothed by the wave-per-column kernels keeps its seam
+// columns' copy in r (free on such a level: no separate right-hand side).
+template <typename R>
+R* seam_copy(Level<R>& L, const int* per, bool on)
+{
+    return on && seam_joins_colour(L, per) && scan_level(L) ? ptr<R>(L.r) : nullptr;
+}
+
+template <typename R, bool kZero, bool kFromR, int M, bool kShadow = false>
 void launch_scan(const Level<R>& L, const int* per, int color, const R* rhs, hipStream_t st)
 {
     const Split s = L.split();
     const int64_t rowc = int64_t(s.H) * s.ny;
-    zebra_scan_kernel<R, kZero, kFromR, M><<<int((rowc + 3) / 4), 256, 0, st>>>(s, color, per[0], per[1],
-                                                                                bundle_of(L), cp_of(L), rhs,
-                                                                                ptr<R>(L.x));
+    zebra_scan_kernel<R, kZero, kFromR, M, kShadow><<<int((rowc + 3) / 4), 256, 0, st>>>(
+        s, color, per[0], per[1], bundle_of(L), cp_of(L), rhs, ptr<R>(L.x), kShadow ? ptr<R>(L.r) : nullptr);
+}
+
+// One zebra sweep of a seam level by the wave-per-column kernels.
+template <typename R, int M>
+void shadow_sweep(const Level<R>& L, const int* per, bool first, hipStream_t st)
+{
+    if (first) { launch_scan<R, true, false, M, true>(L, per, 0, ptr<R>(L.b), st); }
+    else { launch_scan<R, false, false, M, true>(L, per, 0, ptr<R>(L.b), st); }
+    launch_scan<R, false, false, M, true>(L, per, 1, ptr<R>(L.b), st);
+    GUST_HIP(hipGetLastError());
 }
 
 template <typename R, bool kZero, bool kFromR>
@@ -1560,7 +1637,7 @@ void launch_zebra(const Level<R>& L, const int* per, int color, const R* rhs, hi
 {
     const Split s = L.split();
     const int64_t rowc = int64_t(s.H) * s.ny;
-    if (rowc < scan_cols() && L.nz <= 256) {
+    if (scan_level(L)) {
         const int M = (L.nz + 63) / 64;
         if (M == 1) { launch_scan<R, kZero, kFromR, 1>(L, per, color, rhs, st); }
         else if (M == 2) { launch_scan<R, kZero, kFromR, 2>(L, per, color, rhs, st); }
@@ -1582,14 +1659,24 @@ void launch_zebra(const Level<R>& L, const int* per, int color, const R* rhs, hi
     GUST_HIP(hipGetLastError());
 }
 
-// One zebra sweep (colour 0, then colour 1); first: x is zero on entry.
+// One zebra sweep (colour 0, then colour 1); first: x is zero on entry;
+// copy: a seam level smoothed one wave per column reads its seam neighbours
+// from their copy (else: a pass of its own forms the neighbour terms).
 template <typename R>
-void zebra(Level<R>& L, const int* per, bool first, hipStream_t st)
+void zebra(Level<R>& L, const int* per, bool first, bool copy, hipStream_t st)
 {
     if (!seam_joins_colour(L, per)) {
         if (first) { launch_zebra<R, true, false>(L, per, 0, ptr<R>(L.b), st); }
         else { launch_zebra<R, false, false>(L, per, 0, ptr<R>(L.b), st); }
         launch_zebra<R, false, false>(L, per, 1, ptr<R>(L.b), st);
+        return;
+    }
+    if (copy && scan_level(L)) {
+        const int M = (L.nz + 63) / 64;
+        if (M == 1) { shadow_sweep<R, 1>(L, per, first, st); }
+        else if (M == 2) { shadow_sweep<R, 2>(L, per, first, st); }
+        else if (M == 3) { shadow_sweep<R, 3>(L, per, first, st); }
+        else { shadow_sweep<R, 4>(L, per, first, st); }
         return;
     }
     const Split s = L.split();
@@ -1649,14 +1736,14 @@ void vcycle(PoissonImpl& P, std::vector<std::unique_ptr<Level<R>>>& mg, size_t l
 {
     Level<R>& L = *mg[l];
     if (l + 1 == mg.size()) {
-        for (int s = 0; s < P.opt.coarse_sweeps; ++s) { zebra<R>(L, P.per, s == 0, st); }
+        for (int s = 0; s < P.opt.coarse_sweeps; ++s) { zebra<R>(L, P.per, s == 0, P.opt.seam_copy != 0, st); }
         if (bc.singular) { remove_mean<R>(P, L, st); }
         return;
     }
     if (P.opt.pre_sweeps == 0) {
         fill_kernel<R><<<blocks_for(L.slots()), kThreads, 0, st>>>(ptr<R>(L.x), R(0), L.slots());
     }
-    for (int s = 0; s < P.opt.pre_sweeps; ++s) { zebra<R>(L, P.per, s == 0, st); }
+    for (int s = 0; s < P.opt.pre_sweeps; ++s) { zebra<R>(L, P.per, s == 0, P.opt.seam_copy != 0, st); }
     Level<R>& C = *mg[l + 1];
     const Split sf = L.split(), sc = C.split();
     resid_restrict_kernel<R><<<blocks_for(C.slots()), kThreads, 0, st>>>(sf, P.per[0], P.per[1], bundle_of(L),
@@ -1668,9 +1755,11 @@ void vcycle(PoissonImpl& P, std::vector<std::unique_ptr<Level<R>>>& mg, size_t l
     mc.m = mirror<R>(nullptr, C.nx, C.ny, C.nz, P.per, bc);
     mc.s = sc;
     mc.p = ptr<R>(C.x);
-    prolong_kernel<R><<<blocks_for(L.slots()), kThreads, 0, st>>>(sf, C.rx, C.ry, mc, ptr<R>(L.x));
+    prolong_kernel<R><<<blocks_for(L.slots()), kThreads, 0, st>>>(sf, C.rx, C.ry, mc, ptr<R>(L.x),
+                                                                   seam_copy(L, P.per, P.opt.seam_copy != 0), P.per[0],
+                                                                   P.per[1]);
     GUST_HIP(hipGetLastError());
-    for (int s = 0; s < P.opt.post_sweeps; ++s) { zebra<R>(L, P.per, false, st); }
+    for (int s = 0; s < P.opt.post_sweeps; ++s) { zebra<R>(L, P.per, false, P.opt.seam_copy != 0, st); }
 }
 
 // Premultiplied areas and the explicit stencil (with its column Thomas
@@ -1820,6 +1909,7 @@ void check_poisson_options(const gust_poisson_options& o)
     if (o.pre_sweeps < 0 || o.post_sweeps < 0 || o.coarse_sweeps < 1) {
         err += "\n  the V-cycle needs pre/post sweeps >= 0 and coarse sweeps >= 1";
     }
+    if (o.seam_copy != 0 && o.seam_copy != 1) { err += "\n  the multigrid's seam_copy must be 0 or 1"; }
     if (!err.empty()) { GUST_FAIL("gust: invalid inputs:%s", err.c_str()); }
 }
 
@@ -2034,6 +2124,7 @@ int gust_poisson_default_options(gust_poisson_options* o)
         o->coarse_sweeps = 16;
         o->max_levels = 0;
         o->fixed_steps = 0;
+        o->seam_copy = 1;
     });
 }
 

@@ -45,6 +45,7 @@ class PoissonOptions:
     coarse_sweeps: int = 16
     max_levels: int = 0
     fixed_steps: int = 0   # k > 0: k GMRES steps on the device, no host reads (graph capture)
+    seam_copy: int = 1     # odd periodic sizes: seam neighbours from a copy (0: a separate neighbour pass)
 
     def c(self) -> _lib.PoissonOptionsC:
         o = _lib.PoissonOptionsC()

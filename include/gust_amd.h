@@ -222,6 +222,10 @@ typedef struct gust_poisson_options {
     int32_t fixed_steps;   /* 0: GMRES(restart) to the tolerance, reading each Arnoldi step on the host;
                               k in 1..200: k Arnoldi steps on the device, no host reads (a captured graph
                               can hold the solve), the tolerance checked at the end */
+    int32_t seam_copy;     /* a level whose periodic seam joins two columns of one colour (an odd periodic
+                              size): 1 (default), its seam columns read those neighbours from a copy, one
+                              launch per half sweep; 0, a pass of its own forms each colour's neighbour
+                              terms first. The same arithmetic, the same bits. */
 } gust_poisson_options;
 
 /* Source/Poisson.H:96-103 (FaceCoeffs): optional face coefficients (null =
