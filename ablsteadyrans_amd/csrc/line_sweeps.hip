@@ -45,19 +45,24 @@ constexpr int kWave = 64;
 constexpr int kStagedMaxTiles = 1024;
 
 // The z-line Jacobi schedule: 0 autotuned, 1 streaming, 2 LDS-staged (where
-// it fits), 3 the fixed rule (staged under kStagedMaxTiles tiles).
+// it fits), 3 the fixed rule (staged under kStagedMaxTiles tiles), 4 the
+// register kernel (where the columns fit a level bucket, else streaming).
 std::atomic<int> g_jacobi_schedule{0};
 int jacobi_schedule() { return g_jacobi_schedule.load(std::memory_order_relaxed); }
 
 // The autotuner's candidates: the LDS-staged kernel, the streaming kernel
 // with c' in LDS at prefetch depths 2, 4 and 8, the streaming kernel with c'
-// in global memory (no LDS limit on occupancy). Every one gives the same
+// in global memory (no LDS limit on occupancy), the register kernel (e' in
+// registers, c' in LDS), the register kernel with c' in registers too
+// (FP32, or FP64 columns of at most 65 levels) and, for FP64 columns of up
+// to 97 levels, with c' split between LDS and registers. Every one gives the same
 // bits; the first run of a plan whose shapes are new times each (all its
 // sweeps, HIP events on its stream) and keeps the fastest for those shapes.
 enum TuneChoice : int { kChoiceStaged = 0, kChoiceP4 = 1, kChoiceP2 = 2, kChoiceP8 = 3, kChoiceGlobal = 4,
-                        kNumChoices = 5 };
+                        kChoiceRegs = 5, kChoiceRegsC = 6, kChoiceRegsSplit = 7, kNumChoices = 8 };
 const char* kChoiceNames[kNumChoices] = {"staged", "streaming-p4", "streaming-p2", "streaming-p8",
-                                         "streaming-global-cprime"};
+                                         "streaming-global-cprime", "register-e", "register-ce",
+                                         "register-split"};
 
 struct TuneKey {
     int dtype = 0, sweeps = 0;
@@ -72,7 +77,7 @@ struct TuneKey {
 
 struct TuneRecord {
     int choice = kChoiceP4;
-    float ms[kNumChoices] = {-1, -1, -1, -1, -1};
+    float ms[kNumChoices] = {-1, -1, -1, -1, -1, -1, -1, -1};
 };
 
 std::mutex g_tune_mu;
@@ -292,6 +297,209 @@ __global__ __launch_bounds__(kWave) void zjacobi_sweep_kernel(const JacobiSys<T>
         if (lane == 0 && chg) { atomicMax(changes + si, chg); }
     }
 }
+
+// The streaming sweep with e' held in registers for the whole column instead
+// of staged in the output column: the output is then written once, by the
+// back substitution, and e' never travels to L2 or HBM (in the kernel above
+// it is written, re-read and overwritten, and a wave's 64 x nz e' values
+// outlive the L2 once a few hundred waves are in flight). The level loops are
+// unrolled to kNZ levels (every register index is a constant). Loads are
+// issued for every unrolled level (a column with nz < kNZ re-reads its top
+// level) and only the arithmetic of levels past the column's top is skipped,
+// by a branch that is uniform over the wave (all lanes of a wave sweep the
+// same system): no loaded value is merged at a branch, so the prefetched
+// loads stay in flight across levels. Every array is read and written
+// through a buffer resource: the level is a scalar offset shared by the wave,
+// the column a 32-bit per-lane offset, so a load is one instruction with no
+// per-lane address arithmetic (the unrolled body stays small enough for the
+// instruction cache). c' of levels below kCL lives in LDS, of the levels
+// above in registers (kCL = kNZ: all in LDS; 0: all in registers, no LDS;
+// in between: less LDS per wave, so more waves per CU). Same operations in
+// the same order as zjacobi_sweep_kernel: the same bits.
+using Rsrc = __amdgpu_buffer_rsrc_t;
+constexpr int kRsrcWord3 = 0x00020000;   // raw (untyped) buffer access on gfx9
+constexpr int kAuxNontemporal = 2;       // the nt cache policy: streamed once
+
+__device__ __forceinline__ Rsrc rsrc_of(const void* base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, 0x7fffffff, kRsrcWord3);
+}
+
+template <typename T, int kAux>
+__device__ __forceinline__ T bload(Rsrc r, uint32_t voff, uint32_t soff)
+{
+    if constexpr (sizeof(T) == 8) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, kAux));
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, kAux));
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void bstore(T v, Rsrc r, uint32_t voff, uint32_t soff)
+{
+    if constexpr (sizeof(T) == 8) {
+        using U2 = unsigned int __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, voff, soff, 0);
+    } else {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, soff, 0);
+    }
+}
+
+template <typename T, bool kFirst, bool kLast, int kNZ, int kCL>
+__global__ __launch_bounds__(kWave, 1) void zjacobi_regs_kernel(const JacobiSys<T>* __restrict__ table, int nsys,
+                                                                int total_tiles, int dst_scratch,
+                                                                unsigned long long* __restrict__ changes)
+{
+    constexpr int kP = 4;   // prefetch depth of the forward elimination
+    constexpr int kB = 8;   // prefetch depth of the old iterate (last sweep)
+    constexpr bool kOld = kLast && !kFirst;
+    constexpr uint32_t kS = sizeof(T);
+    extern __shared__ unsigned char lds_raw[];
+    const int lane = threadIdx.x;
+    const int tile = dev::xcd_remap(blockIdx.x, total_tiles);
+    T* cls = reinterpret_cast<T*>(lds_raw);
+    const int si = find_sys(table, nsys, tile);
+    const JacobiSys<T> S = table[si];
+    const int c = (tile - S.tile0) * kWave + lane;
+    const int ncols = S.nx * S.ny;
+    unsigned long long chg = 0;
+    if (c < ncols) {
+        const int i = c % S.nx, j = c / S.nx;
+        const T* src = dst_scratch ? S.d : S.s;
+        T* dst = dst_scratch ? S.s : S.d;
+        const int64_t src_j = dst_scratch ? S.d_j : S.s_j, src_k = dst_scratch ? S.d_k : S.s_k;
+        const int64_t dst_j = dst_scratch ? S.s_j : S.d_j, dst_k = dst_scratch ? S.s_k : S.d_k;
+        int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1;
+        bool okm = true, okp = true, okjm = true, okjp = true;
+        if (im < 0) { if (S.Lx) { im += S.Lx; } else { okm = false; im = i; } }
+        if (ip > S.nx - 1) { if (S.Lx) { ip -= S.Lx; } else { okp = false; ip = i; } }
+        if (jm < 0) { if (S.Ly) { jm += S.Ly; } else { okjm = false; jm = j; } }
+        if (jp > S.ny - 1) { if (S.Ly) { jp -= S.Ly; } else { okjp = false; jp = j; } }
+        // buffer resources (the host checks every offset fits 31 bits)
+        const Rsrc rA1 = rsrc_of(S.A + 1 * S.a_n), rA2 = rsrc_of(S.A + 2 * S.a_n), rA3 = rsrc_of(S.A + 3 * S.a_n);
+        const Rsrc rA4 = rsrc_of(S.A + 4 * S.a_n), rA5 = rsrc_of(S.A + 5 * S.a_n), rA6 = rsrc_of(S.A + 6 * S.a_n);
+        const Rsrc rP = rsrc_of(S.ap), rR = rsrc_of(S.r), rS = rsrc_of(src), rD = rsrc_of(dst);
+        // per-lane byte offsets within a level, per-level byte strides
+        const uint32_t oa = uint32_t(i + int64_t(j) * S.a_j) * kS;
+        const uint32_t op = uint32_t(i + int64_t(j) * S.p_j) * kS;
+        const uint32_t orr = uint32_t(i + int64_t(j) * S.r_j) * kS;
+        const uint32_t om = uint32_t(im + int64_t(j) * src_j) * kS;
+        const uint32_t opp = uint32_t(ip + int64_t(j) * src_j) * kS;
+        const uint32_t ojm = uint32_t(i + int64_t(jm) * src_j) * kS;
+        const uint32_t ojp = uint32_t(i + int64_t(jp) * src_j) * kS;
+        const uint32_t oc = uint32_t(i + int64_t(j) * src_j) * kS;
+        const uint32_t od = uint32_t(i + int64_t(j) * dst_j) * kS;
+        const uint32_t sa = uint32_t(S.a_k) * kS, sp = uint32_t(S.p_k) * kS, sr = uint32_t(S.r_k) * kS;
+        const uint32_t ss = uint32_t(src_k) * kS, sd = uint32_t(dst_k) * kS;
+        const int nz = S.nz;
+        auto load = [&](LevelOps<T>& o, int k) {
+            const uint32_t ka = uint32_t(k) * sa;
+            o.axm = bload<T, kAuxNontemporal>(rA1, oa, ka);
+            o.axp = bload<T, kAuxNontemporal>(rA2, oa, ka);
+            o.aym = bload<T, kAuxNontemporal>(rA3, oa, ka);
+            o.ayp = bload<T, kAuxNontemporal>(rA4, oa, ka);
+            o.azm = bload<T, kAuxNontemporal>(rA5, oa, ka);
+            o.azp = bload<T, kAuxNontemporal>(rA6, oa, ka);
+            o.dg = bload<T, kAuxNontemporal>(rP, op, uint32_t(k) * sp);
+            o.rr = bload<T, kAuxNontemporal>(rR, orr, uint32_t(k) * sr);
+            if (!kFirst) {
+                const uint32_t ks = uint32_t(k) * ss;
+                const T tm = bload<T, 0>(rS, om, ks), tp = bload<T, 0>(rS, opp, ks);
+                const T tjm = bload<T, 0>(rS, ojm, ks), tjp = bload<T, 0>(rS, ojp, ks);
+                o.vm = okm ? tm : T(0);
+                o.vp = okp ? tp : T(0);
+                o.vjm = okjm ? tjm : T(0);
+                o.vjp = okjp ? tjp : T(0);
+            } else {
+                o.vm = o.vp = o.vjm = o.vjp = T(0);
+            }
+        };
+        T ev[kNZ];
+        T cv[kCL < kNZ ? kNZ - kCL : 1];
+        LevelOps<T> ring[kP];
+#pragma unroll
+        for (int u = 0; u < kP; ++u) { load(ring[u], u < nz ? u : nz - 1); }
+        // Forward elimination (Source/LineSolve_K.H:29-42).
+        T cprev = T(0), eprev = T(0);
+        auto level = [&](int k) {
+            const LevelOps<T> o = ring[k % kP];
+            if (k + kP < kNZ) { load(ring[k % kP], k + kP < nz ? k + kP : nz - 1); }
+            if (k < nz) {
+                const T rk = o.rr + o.axm * o.vm + o.axp * o.vp + o.aym * o.vjm + o.ayp * o.vjp;
+                const T lower = -o.azm, upper = -o.azp;
+                T ck, ek;
+                if (k == 0) {
+                    const T m = o.dg;
+                    ck = upper / m;
+                    ek = rk / m;
+                } else {
+                    const T m = o.dg - lower * cprev;
+                    ck = upper / m;
+                    ek = (rk - lower * eprev) / m;
+                }
+                if (k >= kCL) { cv[k >= kCL ? k - kCL : 0] = ck; } else { cls[k * kWave + lane] = ck; }
+                ev[k] = ek;
+                cprev = ck;
+                eprev = ek;
+            }
+        };
+        // two loops of at most 49 levels: each stays under the compiler's
+        // full-unroll limit (a partly unrolled loop would index ev at run
+        // time and move it to scratch memory)
+        constexpr int kHalf = (kNZ + 1) / 2;
+#pragma unroll
+        for (int k = 0; k < kHalf; ++k) { level(k); }
+#pragma unroll
+        for (int k = kHalf; k < kNZ; ++k) { level(k); }
+        // Back substitution (Source/LineSolve_K.H:43-44) from the registers.
+        T o = eprev;
+        bstore<T>(o, rD, od, uint32_t(nz - 1) * sd);
+        if (kLast) {
+            const T old = kFirst ? T(0) : bload<T, 0>(rS, oc, uint32_t(nz - 1) * ss);
+            const unsigned long long b = dev::abs_bits(double(o - old));
+            chg = b > chg ? b : chg;
+        }
+        // The old iterate of level k, read kB unrolled levels ahead (clamped
+        // to the column like the forward loads).
+        T ob[kB];
+        if (kOld) {
+#pragma unroll
+            for (int u = 0; u < kB; ++u) {
+                const int k = kNZ - 2 - u;
+                if (k >= 0) { ob[k % kB] = bload<T, 0>(rS, oc, uint32_t(k <= nz - 2 ? k : nz - 1) * ss); }
+            }
+        }
+#pragma unroll
+        for (int k = kNZ - 2; k >= 0; --k) {
+            T oldk = T(0);
+            if (kOld) {
+                oldk = ob[k % kB];
+                if (k - kB >= 0) { ob[k % kB] = bload<T, 0>(rS, oc, uint32_t(k - kB <= nz - 2 ? k - kB : nz - 1) * ss); }
+            }
+            if (k <= nz - 2) {
+                const T ck = k >= kCL ? cv[k >= kCL ? k - kCL : 0] : cls[k * kWave + lane];
+                o = ev[k] - ck * o;
+                bstore<T>(o, rD, od, uint32_t(k) * sd);
+                if (kLast) {
+                    const unsigned long long b = dev::abs_bits(double(o - oldk));
+                    chg = b > chg ? b : chg;
+                }
+            }
+        }
+    }
+    if (kLast) {
+        chg = dev::wave_max_u64(chg);
+        if (lane == 0 && chg) { atomicMax(changes + si, chg); }
+    }
+}
+
+// The register kernel's level buckets: the columns of the configurations
+// (48, 64 and 96 cells, one more level at z-faces) fit exactly.
+constexpr int kRegBuckets[3] = {49, 65, 97};
+// FP64 columns of the largest bucket with c' split: this many levels in LDS
+// (40 KB per wave: four waves per CU), the rest in registers.
+constexpr int kSplitLevels = 80;
 
 // The same sweep for launches with too few tiles to hide memory latency
 // (small tiles: one wave per CU or fewer). A block of kStagedThreads stages
@@ -664,6 +872,8 @@ struct JacobiPlan final : JacobiPlanBase {
     int kp = 4;                // the streaming kernel's prefetch depth
     bool tune = false;         // the first run times the candidates (schedule 0)
     bool no_alloc = false;     // scratch carved from the caller's bytes
+    int regs_nz = 0;           // > 0: the register kernel, unrolled to this many levels
+    int regs_cl = 0;           // the register kernel's levels of c' in LDS
 
     ~JacobiPlan() override
     {
@@ -722,8 +932,8 @@ struct JacobiPlan final : JacobiPlanBase {
         const size_t lds = size_t(kWave) * size_t(nzmax) * sizeof(T);
         // Schedule: few tiles (under kStagedMaxTiles, about four waves per
         // CU or fewer) take the LDS-staged kernel when a tile's operands fit
-        // the LDS; many tiles the streaming kernel (loads of level k + kP in
-        // flight while level k is eliminated).
+        // the LDS; many tiles the register kernel or the streaming kernel
+        // (below).
         {
             int wmax = 1;
             for (const auto& t : tab) {
@@ -736,6 +946,7 @@ struct JacobiPlan final : JacobiPlanBase {
             wmax_ = wmax;
             staged_fits = fits;
             if (fits && (sched == 2 || ((sched == 0 || sched == 3) && tiles < kStagedMaxTiles))) { staged_w = wmax; }
+            if (sched == 4 && offsets_fit32()) { regs_nz = reg_bucket(nzmax); regs_cl = regs_nz; }
             tune = sched == 0 && !ext;
         }
         no_alloc = ext != nullptr;
@@ -745,6 +956,13 @@ struct JacobiPlan final : JacobiPlanBase {
         const char* cp = std::getenv("GUST_JACOBI_CPRIME");
         const bool force_global = cp && std::strcmp(cp, "global") == 0;
         if (force_global) { tune = false; }
+        // Many tiles under the fixed rule (and the autotuner's starting
+        // point): the register kernel where the columns fit a level bucket
+        // (the fastest candidate on the configurations' tiles), else the
+        // streaming kernel.
+        if (staged_w == 0 && regs_nz == 0 && !force_global && (jacobi_schedule() == 0 || jacobi_schedule() == 3)) {
+            if (!apply_choice(sizeof(T) == 8 ? kChoiceRegsSplit : kChoiceRegsC, st)) { apply_choice(kChoiceRegs, st); }
+        }
         const bool use_lds = !force_global && lds <= device_info().lds_per_block;
         const size_t tab_bytes = tab.size() * sizeof(JacobiSys<T>);
         const size_t cbytes = use_lds ? 0 : size_t(tiles) * lds;
@@ -797,9 +1015,73 @@ struct JacobiPlan final : JacobiPlanBase {
             table_cache_get(st, tab.data(), tab.size() * sizeof(JacobiSys<T>)));
     }
 
+    // The register kernel addresses every array through a buffer resource
+    // with a 32-bit lane offset plus a 32-bit level offset: every byte it
+    // reaches (one component of A, ap, rhs, delta, the scratch iterate) must
+    // lie within 2^31 bytes of the array's base.
+    bool offsets_fit32() const
+    {
+        const int64_t lim = int64_t(1) << 31;
+        for (const auto& t : tab) {
+            const int64_t last = int64_t(t.nx - 1), top = int64_t(t.nz - 1), ny1 = int64_t(t.ny - 1);
+            const int64_t js[5] = {t.a_j, t.p_j, t.r_j, t.d_j, t.s_j};
+            const int64_t ks[5] = {t.a_k, t.p_k, t.r_k, t.d_k, t.s_k};
+            for (int q = 0; q < 5; ++q) {
+                if (js[q] < 0 || ks[q] < 0) { return false; }
+                if ((last + ny1 * js[q]) * int64_t(sizeof(T)) >= lim) { return false; }
+                if (top * ks[q] * int64_t(sizeof(T)) >= lim) { return false; }
+            }
+        }
+        return true;
+    }
+
+    static int reg_bucket(int nz)
+    {
+        for (int b : kRegBuckets) {
+            if (nz <= b) { return b; }
+        }
+        return 0;
+    }
+
+    template <bool F, bool L, int NZ, int CL>
+    void launch_regs_at(int dst_scratch, hipStream_t st)
+    {
+        const size_t lds = size_t(kWave) * size_t(std::min(nzmax, CL)) * sizeof(T);
+        if (lds > 65536) {
+            GUST_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&zjacobi_regs_kernel<T, F, L, NZ, CL>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+        }
+        zjacobi_regs_kernel<T, F, L, NZ, CL><<<tiles, kWave, lds, st>>>(dtab, nsys, tiles, dst_scratch, chg);
+    }
+
+    // regs_cl: levels of c' in LDS (regs_nz: all of them; 0: none).
+    template <bool F, bool L>
+    void launch_regs(int dst_scratch, hipStream_t st)
+    {
+        constexpr int b0 = kRegBuckets[0], b1 = kRegBuckets[1], b2 = kRegBuckets[2];
+        if (regs_nz == b0) {
+            if (regs_cl == 0) { launch_regs_at<F, L, b0, 0>(dst_scratch, st); }
+            else { launch_regs_at<F, L, b0, b0>(dst_scratch, st); }
+        } else if (regs_nz == b1) {
+            if (regs_cl == 0) { launch_regs_at<F, L, b1, 0>(dst_scratch, st); }
+            else { launch_regs_at<F, L, b1, b1>(dst_scratch, st); }
+        } else if (regs_cl == b2) {
+            launch_regs_at<F, L, b2, b2>(dst_scratch, st);
+        } else if constexpr (sizeof(T) == 4) {
+            launch_regs_at<F, L, b2, 0>(dst_scratch, st);
+        } else {
+            launch_regs_at<F, L, b2, kSplitLevels>(dst_scratch, st);
+        }
+    }
+
     template <bool F, bool L>
     void launch(int dst_scratch, hipStream_t st)
     {
+        if (regs_nz > 0) {
+            launch_regs<F, L>(dst_scratch, st);
+            GUST_HIP(hipGetLastError());
+            return;
+        }
         if (staged_w > 0) {
             const size_t lds = size_t(L && !F ? 5 : 4) * size_t(nzmax) * size_t(staged_w) * sizeof(T);
             if (lds > 65536) {
@@ -840,15 +1122,40 @@ struct JacobiPlan final : JacobiPlanBase {
     bool apply_choice(int c, hipStream_t st)
     {
         const size_t lds = size_t(kWave) * size_t(nzmax) * sizeof(T);
+        regs_nz = 0;
+        regs_cl = 0;
         if (c == kChoiceStaged) {
             if (!staged_fits) { return false; }
             staged_w = wmax_;
             return true;
         }
         staged_w = 0;
+        if (c == kChoiceRegs || c == kChoiceRegsC || c == kChoiceRegsSplit) {
+            const int b = offsets_fit32() ? reg_bucket(nzmax) : 0;
+            if (b == 0) { return false; }
+            int cl = b;
+            if (c == kChoiceRegsC) {
+                // c' in registers too: FP32 columns, or FP64 columns of the
+                // two smaller buckets (twice the registers per level)
+                if (sizeof(T) == 8 && b == kRegBuckets[2]) { return false; }
+                cl = 0;
+            } else if (c == kChoiceRegsSplit) {
+                // FP64 columns of the largest bucket: the top levels' c' in
+                // registers, four waves per CU by LDS instead of three
+                if (sizeof(T) != 8 || b != kRegBuckets[2] || nzmax <= kSplitLevels) { return false; }
+                cl = kSplitLevels;
+            }
+            if (size_t(kWave) * size_t(std::min(nzmax, cl)) * sizeof(T) > device_info().lds_per_block) { return false; }
+            regs_nz = b;
+            regs_cl = cl;
+            return true;
+        }
         if (c == kChoiceGlobal) {
             if (no_alloc) { return false; }
-            if (!cglob) { cglob = static_cast<T*>(Workspace::get(st, 5, size_t(tiles) * lds)); }
+            if (!cglob) {
+                if (owned) { return false; }   // a plan handle holds c' in global memory only when LDS cannot
+                cglob = static_cast<T*>(Workspace::get(st, 5, size_t(tiles) * lds));
+            }
             return true;
         }
         if (lds > device_info().lds_per_block) { return false; }
@@ -881,7 +1188,7 @@ struct JacobiPlan final : JacobiPlanBase {
     void run(int sweeps, double* changes_dev, hipStream_t st) override
     {
         if (sweeps < 1) { GUST_FAIL("gust: zline_jacobi: sweeps must be >= 1 (got %d)", sweeps); }
-        if (tune && !owned) {
+        if (tune) {
             tune = false;
             hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
             GUST_HIP(hipStreamIsCapturing(st, &cs));
@@ -1547,9 +1854,9 @@ int gust_jacobi_plan_sweep(gust_jacobi_plan* plan, int first, int last, int into
 int gust_jacobi_set_schedule(int32_t schedule)
 {
     return guarded([&] {
-        if (schedule < 0 || schedule > 3) {
-            GUST_FAIL("gust: jacobi_set_schedule: schedule must be 0 (autotuned), 1 (streaming), 2 (staged) or 3 "
-                      "(the fixed rule)");
+        if (schedule < 0 || schedule > 4) {
+            GUST_FAIL("gust: jacobi_set_schedule: schedule must be 0 (autotuned), 1 (streaming), 2 (staged), 3 "
+                      "(the fixed rule) or 4 (register)");
         }
         g_jacobi_schedule.store(schedule, std::memory_order_relaxed);
     });

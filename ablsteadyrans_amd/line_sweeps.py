@@ -79,19 +79,21 @@ def check_line_options(opt: LineSweepOptions) -> list[str]:
     return [_lib.lib.gust_last_error().decode()]
 
 
-JACOBI_AUTO, JACOBI_STREAMING, JACOBI_STAGED, JACOBI_RULE = 0, 1, 2, 3
+JACOBI_AUTO, JACOBI_STREAMING, JACOBI_STAGED, JACOBI_RULE, JACOBI_REGISTER = 0, 1, 2, 3, 4
 
 
 def set_jacobi_schedule(schedule: int) -> None:
     """The z-line Jacobi kernel schedule for plans made afterwards (process
     wide): JACOBI_AUTO (autotuned per set of system shapes: the first line
     solve times the candidates and the fastest is kept), JACOBI_STREAMING,
-    JACOBI_STAGED or JACOBI_RULE (few tiles staged in LDS, many streamed).
-    Every schedule gives the same bits."""
+    JACOBI_STAGED, JACOBI_RULE (few tiles staged in LDS, many streamed) or
+    JACOBI_REGISTER (e' held in registers, columns of up to 97 levels; taller
+    ones stream). Every schedule gives the same bits."""
     _lib.check(_lib.lib.gust_jacobi_set_schedule(int(schedule)))
 
 
-TUNE_CHOICES = ("staged", "streaming-p4", "streaming-p2", "streaming-p8", "streaming-global-cprime")
+TUNE_CHOICES = ("staged", "streaming-p4", "streaming-p2", "streaming-p8", "streaming-global-cprime", "register-e",
+                "register-ce", "register-split")
 
 
 def jacobi_tuned() -> list:
@@ -102,12 +104,12 @@ def jacobi_tuned() -> list:
     out = []
     for i in range(n.value):
         dt, sw, ns, ch = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
-        ms = (C.c_float * 5)()
+        ms = (C.c_float * len(TUNE_CHOICES))()
         _lib.check(_lib.lib.gust_jacobi_tuned(i, None, C.byref(dt), C.byref(sw), C.byref(ns), C.byref(ch), ms,
                                               None, 0))
         out.append({"dtype": "f64" if dt.value == 0 else "f32", "sweeps": sw.value, "systems": ns.value,
                     "choice": TUNE_CHOICES[ch.value],
-                    "ms": {TUNE_CHOICES[c]: (round(float(ms[c]), 4) if ms[c] >= 0 else None) for c in range(5)}})
+                    "ms": {TUNE_CHOICES[c]: (round(float(ms[c]), 4) if ms[c] >= 0 else None) for c in range(len(TUNE_CHOICES))}})
     return out
 
 

@@ -96,6 +96,30 @@ def _time_sweeps(plan, steps, warmup):
     return e0.elapsed_time(e1) / steps, (t1 - t0) * 1e3 / steps
 
 
+_CHOICE_KERNELS = {
+    "staged": "zjacobi_staged_kernel<{T},false,false>",
+    "streaming-p4": "zjacobi_sweep_kernel<{T},false,false,true,4>",
+    "streaming-p2": "zjacobi_sweep_kernel<{T},false,false,true,2>",
+    "streaming-p8": "zjacobi_sweep_kernel<{T},false,false,true,8>",
+    "streaming-global-cprime": "zjacobi_sweep_kernel<{T},false,false,false,4>",
+    "register-e": "zjacobi_regs_kernel<{T},false,false,97,97>",
+    "register-ce": "zjacobi_regs_kernel<{T},false,false,97,0>",
+    "register-split": "zjacobi_regs_kernel<{T},false,false,97,80>",
+}
+
+
+def _sweep_choice(dtype_name):
+    """The autotuner's choice for the five fractal-tile systems (the plan's
+    first run, 2 sweeps) and the kernel it launches for a middle sweep."""
+    import importlib
+    LSM = importlib.import_module("ablsteadyrans_amd.line_sweeps")
+    for r in LSM.jacobi_tuned():
+        if r["systems"] == len(VARS) and r["dtype"] == dtype_name and r["sweeps"] == 2:
+            T = "double" if dtype_name == "f64" else "float"
+            return r, _CHOICE_KERNELS[r["choice"]].format(T=T)
+    return None, "zjacobi_sweep_kernel (rule 3: streaming, prefetch depth 4)"
+
+
 def _time_run(plan, sweeps, reps):
     import torch
     stream = torch.cuda.current_stream()
@@ -648,7 +672,8 @@ def main():
     print("bench: steady solve of the 200x200x64 Gaussian hill (mixed, two levels of grid sequencing)",
           file=sys.stderr, flush=True)
     c2s, fs = _c2_sequenced_run(2)
-    from ablsteadyrans_amd import line_sweeps as _LS
+    import importlib
+    _LS = importlib.import_module("ablsteadyrans_amd.line_sweeps")   # the package exports a function of that name
     tuned = [r for r in _LS.jacobi_tuned() if r["systems"] >= 2][:8]
     errs = _field_errors(fm, fd)
     errs_seq = _field_errors(fs, fd)
@@ -669,6 +694,7 @@ def main():
     unknowns = plan64.unknowns
     bytes64 = REALS_PER_UNKNOWN * 8 * unknowns
     ms_ev, _ = _time_sweeps(plan64, sweep_steps, sweep_warmup)
+    tune64, kernel64 = _sweep_choice("f64")
     run64 = _time_run(plan64, 2, 20)
     del plan64, sys64
     tile = _fractal_tile()
@@ -677,6 +703,7 @@ def main():
     sys32 = _systems(torch.float32, 20264203)
     plan32 = G.JacobiPlan(sys32)
     ms32, _ = _time_sweeps(plan32, sweep_steps, sweep_warmup)
+    tune32, kernel32 = _sweep_choice("f32")
     run32 = _time_run(plan32, 2, 20)
     bytes32 = REALS_PER_UNKNOWN * 4 * unknowns
     gbs64 = bytes64 / (ms_ev * 1e-3) / 1e9
@@ -717,8 +744,8 @@ def main():
                    "seconds_per_solve": c2s["seconds_per_solve"]},
         "roofline": {"bound": "hbm", "achieved": round(gbs64, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": _traffic_from_profile(),
-                     "kernel": "zjacobi_sweep_kernel<double,false,false> (the metric's sweep HBM GB/s: 5 "
-                               "variables on 256x256x96, one launch)",
+                     "kernel": kernel64 + " (the metric's sweep HBM GB/s: 5 variables on 256x256x96, one "
+                                          "launch; the autotuner's choice)",
                      "algorithmic_bytes_per_launch": bytes64, "avg_launch_ms": round(ms_ev, 5)},
         "cpu_baseline": cpu,
         "extra": {"c2_mixed_sequenced": c2s, "c2_mixed": c2m, "c2_fp64": c2d,
@@ -734,6 +761,7 @@ def main():
                   "sweep_fp64_gbs": round(gbs64, 2), "sweep_fp64_ms": round(ms_ev, 5),
                   "sweep_fp32_gbs": round(gbs32, 2), "sweep_fp32_ms": round(ms32, 5),
                   "sweep_fp32_frac": round(gbs32 / HBM_PEAK_GBS, 4),
+                  "sweep_fp64_autotuner": tune64, "sweep_fp32_autotuner": tune32, "sweep_fp32_kernel": kernel32,
                   "fp64_line_solve_2_sweeps_ms": round(run64, 4), "fp32_line_solve_2_sweeps_ms": round(run32, 4),
                   "vcycle_levels": levels,
                   "fp64_vcycle_ms": round(vc64, 4), "fp32_vcycle_ms": round(vc32, 4),

@@ -162,18 +162,22 @@ int gust_jacobi_plan_info(const gust_jacobi_plan* plan, int64_t* unknowns, int32
  * wide): 0 autotuned (the default: the first line solve of a set of system
  * shapes and sweep count times every candidate -- the LDS-staged kernel,
  * the streaming kernel with c' in LDS at prefetch depths 2, 4, 8, the
- * streaming kernel with c' in global memory -- over all its sweeps with HIP
- * events and keeps the fastest for those shapes; under graph capture an
- * untimed shape takes rule 3), 1 streaming always, 2 staged wherever a tile
- * fits the LDS, 3 the fixed rule (fewer than 1024 tiles staged, more
- * streamed). Plan handles (gust_jacobi_plan_create) take rule 3 under 0.
+ * streaming kernel with c' in global memory, the register kernel with e'
+ * held in registers and c' in LDS, in registers (FP32 or short FP64
+ * columns) or split between them (FP64 columns of 66-97 levels) -- over all
+ * its sweeps with HIP events and keeps the fastest for those shapes; under
+ * graph capture an untimed shape takes rule 3), 1 streaming always, 2
+ * staged wherever a tile fits the LDS, 3 the fixed rule (fewer than 1024
+ * tiles staged, more streamed), 4 the register kernel (columns of up to 97
+ * levels; taller ones stream). Plan handles (gust_jacobi_plan_create) tune
+ * under 0 at their first run outside graph capture.
  * Every schedule gives the same bits.
  * No counterpart in the reference (a launch-shape choice of this library). */
 int gust_jacobi_set_schedule(int32_t schedule);
 
 /* The autotuner's records: *count = their number; record index (in key
  * order) gives the dtype (0 f64, 1 f32), sweeps, the number of systems, the
- * chosen candidate, the five candidates' times in ms (-1: did not apply)
+ * chosen candidate, the eight candidates' times in ms (-1: did not apply)
  * and the choice's name. gust_jacobi_tune_reset forgets them.
  * No counterpart in the reference. */
 int gust_jacobi_tuned(int32_t index, int32_t* count, int32_t* dtype, int32_t* sweeps, int32_t* nsys,

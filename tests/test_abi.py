@@ -136,10 +136,10 @@ def test_jacobi_schedule_refuses_unknown_values():
     import ablsteadyrans_amd as LS
     import importlib
     L = importlib.import_module("ablsteadyrans_amd.line_sweeps")
-    for bad in (-1, 4):
+    for bad in (-1, 5):
         with pytest.raises(Exception, match="jacobi_set_schedule: schedule must be 0"):
             LS.set_jacobi_schedule(bad)
-    for ok in (LS.JACOBI_STREAMING, LS.JACOBI_STAGED, L.JACOBI_RULE, LS.JACOBI_AUTO):
+    for ok in (LS.JACOBI_STREAMING, LS.JACOBI_STAGED, L.JACOBI_RULE, L.JACOBI_REGISTER, LS.JACOBI_AUTO):
         LS.set_jacobi_schedule(ok)
     L.jacobi_tune_reset()
     assert L.jacobi_tuned() == []

@@ -17,11 +17,12 @@ CASES = [((6, 5, 4), (0, 0)), ((6, 5, 4), (1, 1)), ((7, 4, 5), (1, 0)), ((5, 6, 
          ((8, 1, 4), (1, 1)), ((1, 1, 9), (0, 0)), ((33, 17, 12), (1, 0)), ((64, 64, 48), (1, 0))]
 
 
-@pytest.fixture(params=[1, 2, 0], ids=["streaming", "staged", "autotuned"])
+@pytest.fixture(params=[1, 2, 4, 0], ids=["streaming", "staged", "register", "autotuned"])
 def schedule(request, cuda):
     """Every z-line Jacobi kernel schedule (the streaming ring, the LDS-staged
-    tile, and the autotuner's choice among them and the prefetch depths and
-    global c') must give the oracle's bits."""
+    tile, the register kernel, and the autotuner's choice among them, the
+    prefetch depths, global c' and c' in registers) must give the oracle's
+    bits."""
     import ablsteadyrans_amd as LS
     LS.set_jacobi_schedule(request.param)
     yield request.param
