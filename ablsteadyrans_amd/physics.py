@@ -294,7 +294,9 @@ class SimplecOptions:
     """Source/Simplec.H:71-83 (simplec.*) and the stage's settings:
     steady.zline_sweeps, the sweep options, turb.tke_floor, the residual
     scales, the scale height (residual_scale_height) and the Poisson
-    controls (precision 1: FP32 V-cycles under FP64 GMRES)."""
+    controls (precision 1: FP32 V-cycles under FP64 GMRES; the coarsest
+    level's zebra sweeps; the most multigrid levels, 0 = coarsen while x or
+    y halves)."""
     pressure_reltol: float = 1.0e-3
     alpha_p: float = 1.0
     rotation_relax: float = 0.3
@@ -305,6 +307,8 @@ class SimplecOptions:
     height: float = 1.0
     poisson_precision: int = 0
     poisson_max_iter: int = 200
+    poisson_coarse_sweeps: int = 16
+    poisson_max_levels: int = 0
     sweep_precision: int = 0
 
     @property
@@ -320,6 +324,8 @@ class SimplecOptions:
         o.sweeps, o.tke_floor, o.height = int(self.sweeps), self.tke_floor, self.height
         o.poisson.precision = int(self.poisson_precision)
         o.poisson.max_iter = int(self.poisson_max_iter)
+        o.poisson.coarse_sweeps = int(self.poisson_coarse_sweeps)
+        o.poisson.max_levels = int(self.poisson_max_levels)
         o.sweep_precision = int(self.sweep_precision)
         return o
 

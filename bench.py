@@ -108,16 +108,13 @@ _CHOICE_KERNELS = {
 }
 
 
-def _sweep_choice(dtype_name):
-    """The autotuner's choice for the five fractal-tile systems (the plan's
-    first run, 2 sweeps) and the kernel it launches for a middle sweep."""
-    import importlib
-    LSM = importlib.import_module("ablsteadyrans_amd.line_sweeps")
-    for r in LSM.jacobi_tuned():
-        if r["systems"] == len(VARS) and r["dtype"] == dtype_name and r["sweeps"] == 2:
-            T = "double" if dtype_name == "f64" else "float"
-            return r, _CHOICE_KERNELS[r["choice"]].format(T=T)
-    return None, "zjacobi_sweep_kernel (rule 3: streaming, prefetch depth 4)"
+def _rule_kernel(dtype_name):
+    """The kernel the fixed schedule rule launches for a middle sweep of the
+    five fractal-tile systems (columns of 96 and 97 levels): the register
+    kernel, c' split between LDS and registers for FP64, in registers for
+    FP32 (the profiles under profiles/ are of the same kernels)."""
+    T = "double" if dtype_name == "f64" else "float"
+    return _CHOICE_KERNELS["register-split" if dtype_name == "f64" else "register-ce"].format(T=T)
 
 
 def _time_run(plan, sweeps, reps):
@@ -651,6 +648,11 @@ def main():
         print(json.dumps(_profile_deck(args.profile_deck, args.iterations)))
         return
     if args.profile_only:
+        # the fixed rule (no autotuning): every profiling pass launches the
+        # same kernels (the register kernel, c' split for FP64, in registers
+        # for FP32)
+        import importlib
+        importlib.import_module("ablsteadyrans_amd.line_sweeps").set_jacobi_schedule(3)
         sys64 = _systems(torch.float64, 20264203)
         plan64 = G.JacobiPlan(sys64)
         ms_ev, _ = _time_sweeps(plan64, sweep_steps, sweep_warmup)
@@ -689,12 +691,15 @@ def main():
     decks.append(_hill3d_sequenced())
     # 4. the z-line sweep (the roofline kernel) and the V-cycle on the fractal tile
     print("bench: z-line sweeps and V-cycles on the 256x256x96 fractal tile", file=sys.stderr, flush=True)
+    # the fixed schedule rule for the roofline legs: the kernels the committed
+    # profiles measured (the autotuner picks among candidates within ~1 %)
+    _LS.set_jacobi_schedule(3)
     sys64 = _systems(torch.float64, 20264203)
     plan64 = G.JacobiPlan(sys64)
     unknowns = plan64.unknowns
     bytes64 = REALS_PER_UNKNOWN * 8 * unknowns
     ms_ev, _ = _time_sweeps(plan64, sweep_steps, sweep_warmup)
-    tune64, kernel64 = _sweep_choice("f64")
+    kernel64 = _rule_kernel("f64")
     run64 = _time_run(plan64, 2, 20)
     del plan64, sys64
     tile = _fractal_tile()
@@ -703,8 +708,9 @@ def main():
     sys32 = _systems(torch.float32, 20264203)
     plan32 = G.JacobiPlan(sys32)
     ms32, _ = _time_sweeps(plan32, sweep_steps, sweep_warmup)
-    tune32, kernel32 = _sweep_choice("f32")
+    kernel32 = _rule_kernel("f32")
     run32 = _time_run(plan32, 2, 20)
+    _LS.set_jacobi_schedule(0)
     bytes32 = REALS_PER_UNKNOWN * 4 * unknowns
     gbs64 = bytes64 / (ms_ev * 1e-3) / 1e9
     gbs32 = bytes32 / (ms32 * 1e-3) / 1e9
@@ -745,7 +751,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(gbs64, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": _traffic_from_profile(),
                      "kernel": kernel64 + " (the metric's sweep HBM GB/s: 5 variables on 256x256x96, one "
-                                          "launch; the autotuner's choice)",
+                                          "launch)",
                      "algorithmic_bytes_per_launch": bytes64, "avg_launch_ms": round(ms_ev, 5)},
         "cpu_baseline": cpu,
         "extra": {"c2_mixed_sequenced": c2s, "c2_mixed": c2m, "c2_fp64": c2d,
@@ -761,7 +767,7 @@ def main():
                   "sweep_fp64_gbs": round(gbs64, 2), "sweep_fp64_ms": round(ms_ev, 5),
                   "sweep_fp32_gbs": round(gbs32, 2), "sweep_fp32_ms": round(ms32, 5),
                   "sweep_fp32_frac": round(gbs32 / HBM_PEAK_GBS, 4),
-                  "sweep_fp64_autotuner": tune64, "sweep_fp32_autotuner": tune32, "sweep_fp32_kernel": kernel32,
+                  "sweep_fp32_kernel": kernel32,
                   "fp64_line_solve_2_sweeps_ms": round(run64, 4), "fp32_line_solve_2_sweeps_ms": round(run32, 4),
                   "vcycle_levels": levels,
                   "fp64_vcycle_ms": round(vc64, 4), "fp32_vcycle_ms": round(vc32, 4),
