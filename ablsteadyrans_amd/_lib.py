@@ -325,6 +325,57 @@ class ProlongResultC(C.Structure):
     _fields_ = [("projection", PressureCorrectionResultC), ("tke_clipped", C.c_int64)]
 
 
+class NewtonOptionsC(C.Structure):
+    """``gust_newton_options`` (Source/Newton.H:78-99, Continuation.H:63-79)."""
+
+    _fields_ = [("gmres_restart", C.c_int32), ("gmres_max_iters", C.c_int32), ("fd_eps_scale", C.c_double),
+                ("krylov_rtol", C.c_double), ("preconditioner", C.c_int32), ("cfl_max", C.c_double),
+                ("start_cfl", C.c_double), ("ser_exponent", C.c_double), ("ser_max_growth", C.c_double),
+                ("armijo_c", C.c_double), ("backtrack", C.c_double), ("lambda_min", C.c_double),
+                ("cfl_cut", C.c_double), ("max_retries", C.c_int32), ("dtau_inf_at_cap", C.c_int32),
+                ("ptc_growth", C.c_double)]
+
+
+class NewtonStepInfoC(C.Structure):
+    """``gust_newton_step_info`` (Source/Newton.H:236-252)."""
+
+    _fields_ = [("gmres_iterations", C.c_int32), ("gmres_status", C.c_int32), ("residual_norm", C.c_double),
+                ("krylov_residual", C.c_double), ("krylov_relative", C.c_double), ("eps_min", C.c_double),
+                ("eps_max", C.c_double), ("jvp_calls", C.c_int32), ("applied", C.c_int32)]
+
+
+class ContinuationStateC(C.Structure):
+    """``gust_continuation_state`` (Source/Continuation.H:118-126)."""
+
+    _fields_ = [("steps", C.c_int32), ("cfl", C.c_double), ("infinite", C.c_int32), ("r0", C.c_double),
+                ("r_last", C.c_double)]
+
+
+class ContinuationRecordC(C.Structure):
+    """``gust_continuation_record`` (Source/Continuation.H:133-151)."""
+
+    _fields_ = [("step", C.c_int32), ("attempt", C.c_int32), ("residual", C.c_double),
+                ("residual_new", C.c_double), ("cfl", C.c_double), ("infinite", C.c_int32), ("lambda_", C.c_double),
+                ("backtracks", C.c_int32), ("gmres_iterations", C.c_int32), ("gmres_status", C.c_int32),
+                ("krylov_relative", C.c_double), ("outcome", C.c_int32), ("by_growth", C.c_int32)]
+
+
+class SwitchOptionsC(C.Structure):
+    """``gust_switch_options`` (Source/Driver.H:67-97)."""
+
+    _fields_ = [("switch_window", C.c_int32), ("switch_residual", C.c_double),
+                ("switch_back_growth", C.c_double), ("switch_monitor_tol", C.c_double),
+                ("newton_fallback_failures", C.c_int32)]
+
+
+class NewtonRunResultC(C.Structure):
+    """``gust_newton_run_result``."""
+
+    _fields_ = [("driver", DriverResultC), ("simplec_iterations", C.c_int32), ("newton_iterations", C.c_int32),
+                ("switches", C.c_int32), ("first_switch", C.c_int32), ("newton_steps", C.c_int32),
+                ("stage", C.c_int32), ("handed_back", C.c_int32)]
+
+
 def _load() -> C.CDLL:
     # One HIP runtime per process: torch ships its own libamdhip64 and the
     # device tensors, streams and events handed to the library come from it.
@@ -507,6 +558,22 @@ _sig = {
     "gust_initialize_state": (C.c_int, [_P(GeometryC), _P(MeshC), _P(InflowProfileC), _P(InitOptionsC),
                                         _P(C.c_int32), C.c_void_p, _P(StateC), _P(Array4), _P(InitResultC),
                                         C.c_void_p]),
+    "gust_fill_ghosts_homogeneous": (C.c_int, [_P(GeometryC), _P(StateC), _P(MeshC), _P(PhysicsC), C.c_void_p]),
+    "gust_newton_default_options": (C.c_int, [_P(NewtonOptionsC)]),
+    "gust_check_newton_options": (C.c_int, [_P(NewtonOptionsC), C.c_char_p, C.c_int64, _P(C.c_int32)]),
+    "gust_jvp": (C.c_int, [C.c_void_p, _P(StateC), _P(StateC), _P(ResidualsC), _P(ResidualsC), _P(C.c_double),
+                           C.c_void_p]),
+    "gust_newton_create": (C.c_int, [C.c_void_p, _P(NewtonOptionsC), _P(C.c_void_p), C.c_void_p]),
+    "gust_newton_destroy": (None, [C.c_void_p]),
+    "gust_newton_step": (C.c_int, [C.c_void_p, _P(StateC), _P(Array4), _P(NewtonStepInfoC), C.c_void_p]),
+    "gust_newton_residual_norm": (C.c_int, [C.c_void_p, _P(StateC), _P(C.c_double), C.c_void_p]),
+    "gust_continuation_step": (C.c_int, [C.c_void_p, _P(StateC), _P(C.c_int32), _P(ContinuationRecordC),
+                                         C.c_void_p]),
+    "gust_continuation_get": (C.c_int, [C.c_void_p, _P(ContinuationStateC)]),
+    "gust_continuation_restore": (C.c_int, [C.c_void_p, _P(ContinuationStateC)]),
+    "gust_switch_default_options": (C.c_int, [_P(SwitchOptionsC)]),
+    "gust_steady_solve_newton": (C.c_int, [C.c_void_p, C.c_void_p, _P(StateC), _P(DriverOptionsC),
+                                           _P(SwitchOptionsC), _P(NewtonRunResultC), C.c_void_p]),
 }
 for _name, (_res, _args) in _sig.items():
     _f = getattr(lib, _name)

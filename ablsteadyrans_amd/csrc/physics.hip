@@ -272,7 +272,9 @@ __global__ __launch_bounds__(kThreads) void surface_kernel(Box3 cols, CV u, CV v
 
 // ---- ghost fills --------------------------------------------------------------
 
-enum : int { R_EVEN = 0, R_ODD, R_VALUE, R_PROFILE, R_GRADIENT };
+// R_ZERO: the homogeneous fill's profile rule (the face point and every
+// ghost layer 0, BoundaryConditions.cpp:438-457 with homogeneous set).
+enum : int { R_EVEN = 0, R_ODD, R_VALUE, R_PROFILE, R_GRADIENT, R_ZERO };
 enum : int { F_U = 0, F_V, F_W, F_P, F_THETA, F_TKE };
 
 int rule_for(int f, int t, int d, int lo)
@@ -481,10 +483,14 @@ __device__ __forceinline__ void fill_side_plane(const Box3& plane, const V& a, i
             const int edge = is_lo ? edge_lo : edge_hi;
             const int mirror = nodal_d ? 2 * edge : (is_lo ? 2 * edge - 1 : 2 * edge + 1);
             const double sgn = s.rule == R_ODD ? -1.0 : 1.0;
-            const int first = (nodal_d && (s.rule == R_VALUE || s.rule == R_PROFILE)) ? 0 : 1;
+            const int first = (nodal_d && (s.rule == R_VALUE || s.rule == R_PROFILE || s.rule == R_ZERO)) ? 0 : 1;
             for (int layer = first; layer <= 2; ++layer) {
                 const int pd = is_lo ? edge - layer : edge + layer;
                 const int pi = d == 0 ? pd : i, pj = d == 1 ? pd : j;
+                if (s.rule == R_ZERO) {
+                    a(pi, pj, k) = 0.0;
+                    continue;
+                }
                 if (s.rule == R_PROFILE) {
                     double u, v, th, kk, nut;
                     profile::profile_eval(prof, heights::height_above_ground<T>(z, pi, pj, k, klo), u, v, th, kk,
@@ -1259,7 +1265,9 @@ GgMap gg_map(const gust_geometry& g)
 // The fills along z over every lateral column of the field's fab (the ghost
 // slab of the z faces spans the lateral ghosts, BoundaryConditions.cpp:83-104).
 template <int T>
-FzField fz_field(const gust_geometry& g, gust_array4* a, int f, const gust_physics& p)
+// homogeneous: the fill of a perturbation (BoundaryConditions.cpp:427, 510):
+// every prescribed value and gradient 0, the wall law's gradient too.
+FzField fz_field(const gust_geometry& g, gust_array4* a, int f, const gust_physics& p, bool homogeneous = false)
 {
     FzField F{};
     for (int side = 0; side < 2; ++side) {
@@ -1273,6 +1281,11 @@ FzField fz_field(const gust_geometry& g, gust_array4* a, int f, const gust_physi
         x.G = p.zhi_theta_grad;
         x.wall = (t == GUST_FACE_SURFACE_LAYER) ? 1 : 0;
         x.comp = f == F_U ? 0 : f == F_V ? 1 : 2;
+        if (homogeneous) {
+            x.V = 0;
+            x.G = 0;
+            x.wall = 0;
+        }
     }
     IBox cols = view_box(a);
     cols.lo[2] = cols.hi[2] = g.lo[2];
@@ -1298,7 +1311,8 @@ void launch_fill_z(hipStream_t st, const gust_geometry& g, const FzSet& set, int
 // whole fab along x for d = 1, the domain (or the fab where periodic) along
 // the later directions.
 template <int T>
-SideField side_field(const gust_geometry& g, gust_array4* a, int f, int d, const gust_physics& p)
+SideField side_field(const gust_geometry& g, gust_array4* a, int f, int d, const gust_physics& p,
+                     bool homogeneous = false)
 {
     SideField F{};
     for (int side = 0; side < 2; ++side) {
@@ -1309,6 +1323,10 @@ SideField side_field(const gust_geometry& g, gust_array4* a, int f, int d, const
         x.rule = rule_for(f, t, d, lo ? 1 : 0);
         x.V = 0;
         if (t == GUST_FACE_NO_SLIP_WALL && f <= F_W) { x.V = p.wall_velocity[o][f]; }
+        if (homogeneous) {
+            x.V = 0;
+            if (x.rule == R_PROFILE) { x.rule = R_ZERO; }
+        }
     }
     const IBox fab = view_box(a);
     IBox plane;
@@ -1333,13 +1351,13 @@ SideField side_field(const gust_geometry& g, gust_array4* a, int f, int d, const
 }
 
 void launch_fill_sides(hipStream_t st, const gust_geometry& g, const gust_state& q, int d, const gust_physics& p,
-                       const gust_array4* z, const profile::ProfileView& prof)
+                       const gust_array4* z, const profile::ProfileView& prof, bool homogeneous = false)
 {
     SideSet set{};
-    set.f[0] = side_field<sc::XF>(g, q.f[0], F_U, d, p);
-    set.f[1] = side_field<sc::YF>(g, q.f[1], F_V, d, p);
-    set.f[2] = side_field<sc::ZF>(g, q.f[2], F_W, d, p);
-    for (int n = 3; n < 6; ++n) { set.f[n] = side_field<sc::CC>(g, q.f[n], n, d, p); }
+    set.f[0] = side_field<sc::XF>(g, q.f[0], F_U, d, p, homogeneous);
+    set.f[1] = side_field<sc::YF>(g, q.f[1], F_V, d, p, homogeneous);
+    set.f[2] = side_field<sc::ZF>(g, q.f[2], F_W, d, p, homogeneous);
+    for (int n = 3; n < 6; ++n) { set.f[n] = side_field<sc::CC>(g, q.f[n], n, d, p, homogeneous); }
     int64_t most = 1;
     for (int n = 0; n < 6; ++n) { most = std::max<int64_t>(most, set.f[n].plane.npts()); }
     fill_side_kernel<<<dim3(blocks_for(most), 6), kThreads, 0, st>>>(set, d, node_z(z), prof, g.lo[2]);
@@ -1381,28 +1399,31 @@ IBox no_flux_columns(const gust_array4* w, const gust_array4* u, const gust_arra
 }
 
 // fill_ghosts on validated arguments (BoundaryConditions.cpp:597-666); ss
-// receives the surface state.
+// receives the surface state. homogeneous: the fill of a perturbation
+// (every prescribed value, profile and gradient 0, no surface state; the
+// ground w's no-flux value is linear and kept).
 void fill_ghosts_impl(hipStream_t st, const gust_geometry& g, const gust_state& q, const gust_array4* z,
-                      const gust_physics& p, gust_array4* ss)
+                      const gust_physics& p, gust_array4* ss, bool homogeneous = false)
 {
     const int locs[6] = {GUST_LOC_XFACE, GUST_LOC_YFACE, GUST_LOC_ZFACE, GUST_LOC_CELL, GUST_LOC_CELL, GUST_LOC_CELL};
     launch_fill_boundary(st, g, q.f, locs, 6);
-    const bool wall = wall_on(p);
+    const bool wall = wall_on(p) && !homogeneous;
     bool inflow = false;
     for (int o = 0; o < 6; ++o) { inflow = inflow || p.face[o] == GUST_FACE_INFLOW; }
+    inflow = inflow && !homogeneous;
     const profile::ProfileView prof = inflow ? profile::device_view(st, p.inflow) : profile::ProfileView{};
     for (int d = 0; d < 2; ++d) {
-        if (!g.periodic[d]) { launch_fill_sides(st, g, q, d, p, z, prof); }
+        if (!g.periodic[d]) { launch_fill_sides(st, g, q, d, p, z, prof, homogeneous); }
     }
     if (wall) { launch_surface(st, g, q.f[0], q.f[1], z, p.surface, ss); }
     const gust_array4* sv = wall ? ss : nullptr;
     // The z fills of all six fields: each reads only its own field (and the
     // surface state), so one launch does the reference's per-field passes.
     FzSet fz{};
-    fz.f[0] = fz_field<sc::XF>(g, q.f[0], F_U, p);
-    fz.f[1] = fz_field<sc::YF>(g, q.f[1], F_V, p);
-    fz.f[2] = fz_field<sc::ZF>(g, q.f[2], F_W, p);
-    for (int n = 3; n < 6; ++n) { fz.f[n] = fz_field<sc::CC>(g, q.f[n], n, p); }
+    fz.f[0] = fz_field<sc::XF>(g, q.f[0], F_U, p, homogeneous);
+    fz.f[1] = fz_field<sc::YF>(g, q.f[1], F_V, p, homogeneous);
+    fz.f[2] = fz_field<sc::ZF>(g, q.f[2], F_W, p, homogeneous);
+    for (int n = 3; n < 6; ++n) { fz.f[n] = fz_field<sc::CC>(g, q.f[n], n, p, homogeneous); }
     launch_fill_z(st, g, fz, 6, p, z, sv);
     // w on the ground and top from u and v (their z ghosts already filled).
     // The reference's FillBoundary of w after it is subsumed by the final one:
@@ -1521,6 +1542,20 @@ int gust_fill_ghosts(const gust_geometry* geom, gust_state* q, const gust_mesh* 
             }
         }
         fill_ghosts_impl(st, *geom, *q, mesh->z_nd, *phys, ss);
+    });
+}
+
+int gust_fill_ghosts_homogeneous(const gust_geometry* geom, gust_state* q, const gust_mesh* mesh,
+                                 const gust_physics* phys, void* stream)
+{
+    return guarded([&] {
+        const char* op = "fill_ghosts";
+        check_geom(op, geom);
+        check_state(op, *geom, q);
+        check_mesh(op, *geom, mesh);
+        check_physics(op, *geom, phys);
+        if (!phys->use_bc) { GUST_FAIL("gust: %s: needs the boundary conditions", op); }
+        fill_ghosts_impl(static_cast<hipStream_t>(stream), *geom, *q, mesh->z_nd, *phys, nullptr, true);
     });
 }
 

@@ -12,6 +12,7 @@
 #include "cgrid.hpp"
 #include "deferred.hpp"
 #include "simplec.hpp"
+#include "stage.hpp"
 
 #include <cfloat>
 #include <cmath>
@@ -85,41 +86,6 @@ __global__ __launch_bounds__(kThreads) void copy_box_kernel(Box3 b, V dst, CV sr
     }
 }
 
-// A device field owned by the stage.
-struct Field {
-    double* p = nullptr;
-    gust_array4 v{};
-    Field() = default;
-    Field(const Field&) = delete;
-    Field& operator=(const Field&) = delete;
-    ~Field()
-    {
-        if (p) { (void)hipFree(p); }
-    }
-    // Zeroed on the stream that will use the field (stream order, not the
-    // null stream's: the library's streams need not wait for it).
-    void alloc(const IBox& b, int ncomp, hipStream_t st)
-    {
-        const int64_t n = b.npts();
-        GUST_HIP(hipMalloc(&p, size_t(n) * ncomp * sizeof(double)));
-        GUST_HIP(hipMemsetAsync(p, 0, size_t(n) * ncomp * sizeof(double), st));
-        v.p = p;
-        v.jstride = b.len(0);
-        v.kstride = int64_t(b.len(0)) * b.len(1);
-        v.nstride = n;
-        for (int d = 0; d < 3; ++d) {
-            v.begin[d] = b.lo[d];
-            v.end[d] = b.hi[d] + 1;
-        }
-        v.ncomp = ncomp;
-    }
-};
-
-void check_rc(int rc)
-{
-    if (rc != 0) { throw Fail{rc}; }
-}
-
 const int kLocs[6] = {GUST_LOC_XFACE, GUST_LOC_YFACE, GUST_LOC_ZFACE, GUST_LOC_CELL, GUST_LOC_CELL, GUST_LOC_CELL};
 
 bool dirichlet_normal(int t)
@@ -129,49 +95,6 @@ bool dirichlet_normal(int t)
 }
 
 }   // namespace
-
-struct SimplecStage {
-    gust_geometry g{};
-    gust_array4 z{}, J{}, ax{}, ay{}, az{};
-    gust_mesh mesh{};
-    gust_physics phys{};
-    gust_simplec_options opt{};
-    gust_poisson* solver = nullptr;
-    int32_t pbc[6]{};
-    bool wall = false;
-    Field R[6], cf[7], ss, F[3], dtf[3], A[3], At, Ak, mask[3], ap[3], d[3], conv0, conv1, dtau;
-    gust_closure_fields cfv{};
-    double res[6]{}, mon[3]{};
-    int32_t nmon = 0;
-    // One iteration captured as a HIP graph (its kernels and the copy of
-    // its checks), the checks' callbacks and the info they fill.
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    Deferred::Recorded rec;
-    gust_simplec_info ginfo{};
-
-    void drop_graph()
-    {
-        if (exec) { (void)hipGraphExecDestroy(exec); }
-        if (graph) { (void)hipGraphDestroy(graph); }
-        exec = nullptr;
-        graph = nullptr;
-        rec = Deferred::Recorded{};
-    }
-
-    ~SimplecStage()
-    {
-        drop_graph();
-        if (solver) { gust_poisson_destroy(solver); }
-    }
-
-    gust_residuals residuals() const
-    {
-        gust_residuals r;
-        for (int n = 0; n < 6; ++n) { r.R[n] = &R[n].v; }
-        return r;
-    }
-};
 
 namespace {
 
@@ -351,6 +274,16 @@ void replay(SimplecStage& s, gust_simplec_info* info, hipStream_t st)
 }
 
 }   // namespace
+
+void check_rc(int rc)
+{
+    if (rc != 0) { throw Fail{rc}; }
+}
+
+void simplec_iterate(SimplecStage& s, gust_state* q, const gust_array4* dtau, gust_simplec_info* info, void* stream)
+{
+    iterate(s, q, dtau, info, stream);
+}
 
 }   // namespace gust_amd
 
@@ -563,7 +496,7 @@ static void solve_loop(const char* op, gust_simplec* s, gust_state* q, const gus
         }
     }
     const gust_sequencing_options* loose = lc ? lc->loose : nullptr;
-    Field dt;
+    DevField dt;
     dt.alloc(loc_box(st.g, GUST_LOC_CELL), 1, hipStream_t(stream));
     gust_array4 nu = st.phys.use_closure ? st.cf[0].v : *st.phys.nu;   // zero before the first residual
     gust_steady_result v{};
@@ -696,7 +629,7 @@ int gust_steady_solve_graphed(gust_simplec* s, gust_state* q, const gust_driver_
             ~Holder() { gust_steady_destroy(c); }
         } hold_c{crit};
         const int hold = opt->criterion.residual_ref_iterations;
-        Field dt;
+        DevField dt;
         dt.alloc(loc_box(st.g, GUST_LOC_CELL), 1, hs);
         gust_array4 nu = st.phys.use_closure ? st.cf[0].v : *st.phys.nu;
         double cfl = opt->cfl0, r = 0;
@@ -807,7 +740,7 @@ int gust_prolong_state(gust_simplec* fine_stage, gust_poisson* solver, const gus
             pb.lo[d] = pv.begin[d];
             pb.hi[d] = pv.end[d] - 1;
         }
-        Field keep;
+        DevField keep;
         keep.alloc(pb, 1, hs);
         const Box3 b = box3(pb);
         copy_box_kernel<<<blocks_for(b.npts()), kThreads, 0, hs>>>(b, dview<double>(&keep.v), dview<const double>(&pv));

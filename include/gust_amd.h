@@ -1347,6 +1347,162 @@ int gust_build_mesh(const gust_geometry* geom, const double* ground, const doubl
                     gust_array4* detJ, gust_array4* ax, gust_array4* ay, gust_array4* az, gust_array4* zcc,
                     void* stream);
 
+/* ---- The Newton-Krylov finish (Source/Newton.H, Source/Continuation.H,
+ * Source/NewtonPreconditioner.H, Source/NewtonStage.H) ------------------- */
+
+/* The fill of a perturbation: every ghost of the state as gust_fill_ghosts,
+ * with each prescribed value, inflow profile and z gradient (the wall law's
+ * too) replaced by 0; no surface state.
+ * Replaces: BoundaryConditions::fill_ghosts(q, surface, buoyancy,
+ *   homogeneous = true) -- Source/BoundaryConditions.H:163-164,
+ *   Source/BoundaryConditions.cpp:418-540, 597-666. */
+int gust_fill_ghosts_homogeneous(const gust_geometry* geom, gust_state* q, const gust_mesh* mesh,
+                                 const gust_physics* phys, void* stream);
+
+/* The Newton stage's inputs. Replaces: NewtonOptions -- Source/Newton.H:78-99;
+ * ContinuationOptions -- Source/Continuation.H:63-79; the driver's
+ * steady.newton_cfl_max, steady.newton_preconditioner, steady.cfl_max (the
+ * preconditioner's CFL without a step, the continuation's first CFL) --
+ * Source/Driver.H:78-97. */
+typedef struct gust_newton_options {
+    int32_t gmres_restart;     /* steady.newton_gmres_restart, 30 (5..200) */
+    int32_t gmres_max_iters;   /* steady.newton_gmres_max_iters, 2000 (>= 1) */
+    double fd_eps_scale;       /* steady.newton_fd_eps_scale, sqrt(machine epsilon) */
+    double krylov_rtol;        /* steady.newton_krylov_rtol, 1e-2 (0 < rtol < 1) */
+    int32_t preconditioner;    /* steady.newton_preconditioner: 0 identity, 1 simplec (default) */
+    double cfl_max;            /* steady.newton_cfl_max, 128: the continuation's cap and SER cfl0 */
+    double start_cfl;          /* steady.cfl_max: the continuation's first CFL and the preconditioner's
+                                  CFL when a step has none */
+    double ser_exponent, ser_max_growth;   /* steady.ser_exponent (1), steady.ser_max_growth (2) */
+    double armijo_c;           /* steady.newton_armijo_c, 1e-4 */
+    double backtrack;          /* steady.newton_backtrack, 0.5 */
+    double lambda_min;         /* steady.newton_lambda_min, 1e-2 */
+    double cfl_cut;            /* steady.newton_cfl_cut, 0.25 */
+    int32_t max_retries;       /* steady.newton_max_retries, 4 */
+    int32_t dtau_inf_at_cap;   /* steady.newton_dtau_inf_at_cap, 1 */
+    double ptc_growth;         /* steady.newton_ptc_growth, 1 (off) */
+} gust_newton_options;
+
+/* Source/Newton.H:236-252 (NewtonStepInfo). */
+typedef struct gust_newton_step_info {
+    int32_t gmres_iterations;
+    int32_t gmres_status;      /* 0 converged, 1 the iteration cap */
+    double residual_norm;      /* |T^-1 R(q)| before the step */
+    double krylov_residual, krylov_relative;
+    double eps_min, eps_max;
+    int32_t jvp_calls;
+    int32_t applied;
+} gust_newton_step_info;
+
+/* Source/Continuation.H:118-126 (ContinuationState). */
+typedef struct gust_continuation_state {
+    int32_t steps;
+    double cfl;
+    int32_t infinite;
+    double r0, r_last;
+} gust_continuation_state;
+
+/* Source/Continuation.H:133-151 (ContinuationRecord); outcome 0 accepted,
+ * 1 skipped, 2 rejected. */
+typedef struct gust_continuation_record {
+    int32_t step, attempt;
+    double residual, residual_new, cfl;
+    int32_t infinite;
+    double lambda;
+    int32_t backtracks, gmres_iterations, gmres_status;
+    double krylov_relative;
+    int32_t outcome, by_growth;
+} gust_continuation_record;
+
+typedef struct gust_newton gust_newton;
+
+/* The documented defaults. */
+int gust_newton_default_options(gust_newton_options* opt);
+/* One message per violated range (naming steady.<key>), one line each, into
+ * buf (NUL-terminated, truncated to cap); *count = their number.
+ * Replaces: check_newton_options -- Source/Newton.cpp:161-193;
+ *   check_continuation_options -- Source/Continuation.cpp. */
+int gust_check_newton_options(const gust_newton_options* opt, char* buf, int64_t cap, int32_t* count);
+
+/* J v by the forward difference of the full residual (Source/Residual.cpp:
+ * 324-396): eps = step (1 + mean|q/s|) / rms(v/s) over the six fields
+ * (theta about the stage's theta_ref), q + eps v on every point, its ghost
+ * cells refilled, Jv = (R(q + eps v) - R0) / eps on the valid points. q:
+ * ghosts filled, R0 = R(q); v: ghosts filled (homogeneously). *eps = 0 and
+ * Jv = 0 when v = 0. The stage gives the mesh, the residual inputs and the
+ * scales (step: scales.step; a Newton stage uses its fd_eps_scale).
+ * Replaces: Real jvp(const State& q, const State& v, const ResidualInputs&,
+ *   const Mesh&, const Geometry&, ResidualFields& Jv, const ResidualFields*
+ *   R0, bool refill = true) -- Source/Residual.H (jvp), Residual.cpp:324-396. */
+int gust_jvp(gust_simplec* stage, const gust_state* q, const gust_state* v, const gust_residuals* R0,
+             gust_residuals* Jv, double* eps, void* stream);
+
+/* The Newton-Krylov core, the continuation and the stage on a SIMPLEC
+ * stage's tile: its mesh, residual inputs, scales, sweeps, pressure solver
+ * and Dirichlet rows (the stage must outlive it). Allocates FGMRES(restart)'s
+ * 2 restart + 1 vectors of the six fields and the work states.
+ * Replaces: NewtonKrylov, Continuation, SimplecPreconditioner and NewtonStage's
+ *   constructors -- Source/Newton.H:277-285, Continuation.H:163-168,
+ *   NewtonPreconditioner.H:151-155, NewtonStage.H:99-102. */
+int gust_newton_create(gust_simplec* stage, const gust_newton_options* opt, gust_newton** out, void* stream);
+void gust_newton_destroy(gust_newton* n);
+
+/* q += dq from (D + J) dq = -R(q) (D = 1/dtau on the rows with a pseudo-time
+ * term; dtau null: plain Newton), FGMRES in the scaled variables with the
+ * options' preconditioner; q's ghosts filled before and after; the update
+ * skipped when |R| or the Krylov residual is not finite or the Krylov
+ * residual exceeds |R|. Synchronises.
+ * Replaces: NewtonStepInfo NewtonKrylov::step(State&, const MultiFab* dtau,
+ *   NewtonPreconditioner*) -- Source/Newton.H:287-292, Newton.cpp:531-539. */
+int gust_newton_step(gust_newton* n, gust_state* q, const gust_array4* dtau, gust_newton_step_info* info,
+                     void* stream);
+/* |T^-1 R(fill(q))|_2 (q's ghosts filled).
+ * Replaces: NewtonKrylov::residual_norm -- Source/Newton.cpp:553-565. */
+int gust_newton_residual_norm(gust_newton* n, gust_state* q, double* r, void* stream);
+
+/* One continuation step with its retries: local dtau at the state's CFL (or
+ * none at infinite dtau), the Newton direction, the Armijo line search (k
+ * clipped at the stage's floor), the SER update; *status 0 accepted, 1
+ * failed (q unchanged). *last: the last attempt's record (may be null).
+ * Replaces: ContinuationStatus Continuation::step(State&, NewtonPreconditioner*)
+ *   -- Source/Continuation.H:171-174, Continuation.cpp:339-427. */
+int gust_continuation_step(gust_newton* n, gust_state* q, int32_t* status, gust_continuation_record* last,
+                           void* stream);
+/* The continuation state: read, or restore (the CFL brought to at most the
+ * cap). Replaces: Continuation::state / restore -- Continuation.H:180-188. */
+int gust_continuation_get(const gust_newton* n, gust_continuation_state* st);
+int gust_continuation_restore(gust_newton* n, const gust_continuation_state* st);
+
+/* The driver's switch rules between the SIMPLEC and Newton stages
+ * (Source/Driver.H:67-97). */
+typedef struct gust_switch_options {
+    int32_t switch_window;        /* steady.switch_window, 20 */
+    double switch_residual;       /* steady.switch_residual, 1e-4 */
+    double switch_back_growth;    /* steady.switch_back_growth, 10 */
+    double switch_monitor_tol;    /* steady.switch_monitor_tol, 1e-2 (0: off) */
+    int32_t newton_fallback_failures;   /* steady.newton_fallback_failures, 2 */
+} gust_switch_options;
+
+/* What a run with the Newton stage did. */
+typedef struct gust_newton_run_result {
+    gust_driver_result driver;
+    int32_t simplec_iterations, newton_iterations;
+    int32_t switches, first_switch;    /* the iteration of the first switch to Newton, -1 if none */
+    int32_t newton_steps;              /* accepted continuation steps */
+    int32_t stage;                     /* the stage at the end: 0 SIMPLEC, 1 Newton */
+    int32_t handed_back;
+} gust_newton_run_result;
+
+/* steady.stage = simplec_newton: the driver loop over the SIMPLEC stage and
+ * the Newton stage with the switch rules, the hysteresis and the fallbacks,
+ * until Steady, Diverged or max_iterations. Synchronises every iteration.
+ * Replaces: Driver::run with the stages SIMPLEC and Newton --
+ *   Source/Driver.cpp:335-400, 403-492; NewtonStage::init / iterate /
+ *   leave -- Source/NewtonStage.cpp:91-207. */
+int gust_steady_solve_newton(gust_simplec* s, gust_newton* n, gust_state* q, const gust_driver_options* opt,
+                             const gust_switch_options* sw, gust_newton_run_result* result, void* stream);
+int gust_switch_default_options(gust_switch_options* sw);
+
 #ifdef __cplusplus
 }
 #endif

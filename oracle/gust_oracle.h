@@ -218,6 +218,7 @@ int oracle_closure_evaluate(const oa4* u, const oa4* v, const oa4* w, const oa4*
 int oracle_surface_state(const oa4* u, const oa4* v, const oa4* z, const osurface* o, const ogeom* g, oa4* ss);
 /* q: u, v, w, p, theta, k with two ghosts; -3 if an inflow face has no profile. */
 int oracle_fill_ghosts(oa4* const q[6], const oa4* z, const obc* bc, const osurface* so, const ogeom* g, oa4* ss);
+int oracle_fill_ghosts_homogeneous(oa4* const q[6], const oa4* z, const obc* bc, const ogeom* g);
 double oracle_coriolis_parameter(int enabled, double latitude, double rotation_period);
 void oracle_fill_theta_ref(int n, const double* zs, const double* ts, const oa4* z, oa4* theta_ref);
 void oracle_forcing(const oa4* u, const oa4* v, const oa4* w, const oa4* theta, const oa4* theta_ref,

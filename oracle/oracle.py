@@ -138,6 +138,7 @@ def lib():
                                               V]
         L.oracle_surface_state.argtypes = [V, V, V, P(OSurface), P(OGeom), V]
         L.oracle_fill_ghosts.argtypes = [P(OA4) * 6, V, P(OBc), P(OSurface), P(OGeom), V]
+        L.oracle_fill_ghosts_homogeneous.argtypes = [P(OA4) * 6, V, P(OBc), P(OGeom)]
         L.oracle_coriolis_parameter.argtypes = [C.c_int, D, D]
         L.oracle_coriolis_parameter.restype = D
         L.oracle_fill_theta_ref.argtypes = [C.c_int, P(D), P(D), V, V]

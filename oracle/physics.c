@@ -504,10 +504,12 @@ static void slab_ranges(const oa4* a, int t, const ogeom* g, int d, int rlo[3], 
 }
 
 /* fill_face (BoundaryConditions.cpp:418-470) for the even, odd, value and
- * profile rules. */
+ * profile rules; homogeneous: V = 0 and the profile's points 0 (427, 442,
+ * 455). */
 static void fill_face(oa4* a, int t, int f, int d, int lo, int rule, double V, const oprofile* prof, const oa4* z,
-                      const ogeom* g)
+                      const ogeom* g, int homogeneous)
 {
+    if (homogeneous) { V = 0.0; }
     const int nodal_d = os_nodal(t, d);
     const int edge = lo ? g->lo[d] : (nodal_d ? g->hi[d] + 1 : g->hi[d]);
     const int klo = g->lo[2];
@@ -523,7 +525,7 @@ static void fill_face(oa4* a, int t, int f, int d, int lo, int rule, double V, c
             for (int j = lo3[1]; j <= hi3[1]; ++j)
                 for (int i = lo3[0]; i <= hi3[0]; ++i) {
                     if (rule == R_PROFILE) {
-                        AT(a, i, j, k) = profile_value(prof, f, hag(z, t, i, j, k, klo));
+                        AT(a, i, j, k) = homogeneous ? 0.0 : profile_value(prof, f, hag(z, t, i, j, k, klo));
                         continue;
                     }
                     if (layer == 0) { AT(a, i, j, k) = V; continue; }
@@ -536,9 +538,9 @@ static void fill_face(oa4* a, int t, int f, int d, int lo, int rule, double V, c
 }
 
 /* fill_gradient_z (BoundaryConditions.cpp:472-514): the ground (wall law,
- * per column) or the top (a constant). */
+ * per column) or the top (a constant); homogeneous: G = 0 (510). */
 static void fill_gradient_z(oa4* a, int t, int f, int lo, int wall, double Gc, const oa4* z, const oa4* ss,
-                            const osurface* so, const ogeom* g)
+                            const osurface* so, const ogeom* g, int homogeneous)
 {
     const int edge = lo ? g->lo[2] : g->hi[2];
     const int c1 = lo ? edge + 1 : edge - 1;
@@ -557,6 +559,7 @@ static void fill_gradient_z(oa4* a, int t, int f, int lo, int wall, double Gc, c
                 ground_gradient(ss, so, bi, bj, Gb);
                 G = 0.5 * (Gb[comp] + Ga[comp]);
             }
+            if (homogeneous) { G = 0.0; }
             const double zc1 = point_z(z, t, i, j, c1), zg1 = point_z(z, t, i, j, g1), zg2 = point_z(z, t, i, j, g2);
             AT(a, i, j, g1) = AT(a, i, j, c1) + (zg1 - zc1) * G;
             AT(a, i, j, g2) = AT(a, i, j, g1) + (zg2 - zg1) * G;
@@ -564,7 +567,7 @@ static void fill_gradient_z(oa4* a, int t, int f, int lo, int wall, double Gc, c
 }
 
 static void fill_field(oa4* a, int t, int f, int d, const obc* bc, const oa4* z, const oa4* ss, const osurface* so,
-                       const ogeom* g)
+                       const ogeom* g, int homogeneous)
 {
     for (int lo = 1; lo >= 0; --lo) {
         const int ft = bc->face[face_of(d, lo)];
@@ -574,12 +577,12 @@ static void fill_field(oa4* a, int t, int f, int d, const obc* bc, const oa4* z,
         if (ft == OFACE_NO_SLIP_WALL && f <= F_W) { V = bc->wall_velocity[face_of(d, lo)][f]; }
         if (rule == R_GRADIENT) {
             if (ft == OFACE_SURFACE_LAYER) {
-                fill_gradient_z(a, t, f, lo, 1, 0.0, z, ss, so, g);
+                fill_gradient_z(a, t, f, lo, !homogeneous, 0.0, z, ss, so, g, homogeneous);
             } else {
-                fill_gradient_z(a, t, f, lo, 0, bc->zhi_theta_grad, z, ss, so, g);
+                fill_gradient_z(a, t, f, lo, 0, bc->zhi_theta_grad, z, ss, so, g, homogeneous);
             }
         } else {
-            fill_face(a, t, f, d, lo, rule, V, bc->profile, z, g);
+            fill_face(a, t, f, d, lo, rule, V, bc->profile, z, g, homogeneous);
         }
     }
 }
@@ -611,27 +614,41 @@ static void fill_no_flux_w(oa4* w, const oa4* u, const oa4* v, const oa4* z, con
     fill_boundary(w, OS_ZF, g);
 }
 
-int oracle_fill_ghosts(oa4* const q[6], const oa4* z, const obc* bc, const osurface* so, const ogeom* g, oa4* ss)
+static int fill_ghosts_impl(oa4* const q[6], const oa4* z, const obc* bc, const osurface* so, const ogeom* g,
+                            oa4* ss, int homogeneous)
 {
     const int t[6] = {OS_XF, OS_YF, OS_ZF, OS_CC, OS_CC, OS_CC};
     for (int o = 0; o < 6; ++o) {
-        if (bc->face[o] == OFACE_INFLOW && !bc->profile) { return -3; }
+        if (bc->face[o] == OFACE_INFLOW && !bc->profile && !homogeneous) { return -3; }
     }
     for (int n = 0; n < 6; ++n) { fill_boundary(q[n], t[n], g); }
-    const int wall = bc->face[2] == OFACE_SURFACE_LAYER;
+    const int wall = bc->face[2] == OFACE_SURFACE_LAYER && !homogeneous;
     for (int d = 0; d < 3; ++d) {
         if (d == 2 && wall) {
             if (oracle_surface_state(q[0], q[1], z, so, g, ss) != 0) { return -1; }
         }
         if (d < 2 && g->periodic[d]) { continue; }
-        fill_field(q[0], t[0], F_U, d, bc, z, ss, so, g);
-        fill_field(q[1], t[1], F_V, d, bc, z, ss, so, g);
-        fill_field(q[2], t[2], F_W, d, bc, z, ss, so, g);
+        fill_field(q[0], t[0], F_U, d, bc, z, ss, so, g, homogeneous);
+        fill_field(q[1], t[1], F_V, d, bc, z, ss, so, g, homogeneous);
+        fill_field(q[2], t[2], F_W, d, bc, z, ss, so, g, homogeneous);
         if (d == 2) { fill_no_flux_w(q[2], q[0], q[1], z, bc, g); }
-        for (int n = 3; n < 6; ++n) { fill_field(q[n], t[n], n, d, bc, z, ss, so, g); }
+        for (int n = 3; n < 6; ++n) { fill_field(q[n], t[n], n, d, bc, z, ss, so, g, homogeneous); }
     }
     for (int n = 0; n < 6; ++n) { fill_boundary(q[n], t[n], g); }
     return 0;
+}
+
+int oracle_fill_ghosts(oa4* const q[6], const oa4* z, const obc* bc, const osurface* so, const ogeom* g, oa4* ss)
+{
+    return fill_ghosts_impl(q, z, bc, so, g, ss, 0);
+}
+
+/* The fill of a perturbation (BoundaryConditions::fill_ghosts with
+ * homogeneous = true): every prescribed value, profile and z gradient 0, no
+ * surface state. */
+int oracle_fill_ghosts_homogeneous(oa4* const q[6], const oa4* z, const obc* bc, const ogeom* g)
+{
+    return fill_ghosts_impl(q, z, bc, NULL, g, NULL, 1);
 }
 
 /* ---- forcing ---------------------------------------------------------- */

@@ -233,6 +233,16 @@ def fill_ghosts(q, mesh: Mesh, phys: Physics):
     return ss if phys.bc.zlo == SURFACE_LAYER else None
 
 
+def fill_ghosts_homogeneous(q, mesh: Mesh, phys: Physics):
+    """Source/BoundaryConditions.cpp:418-540, 597-666 with homogeneous = true
+    (the fill of a perturbation: every prescribed value, inflow profile and
+    z gradient, the wall law's too, replaced by 0), in place."""
+    g = mesh.geom
+    views = [orc.a4(q[n], B2) for n in ("u", "v", "w", "p", "theta", "tke")]
+    arr = (C.POINTER(orc.OA4) * 6)(*[C.pointer(x) for x in views])
+    orc.lib().oracle_fill_ghosts_homogeneous(arr, _v(mesh.z, B1), phys.bc.c(phys.forcing), g)
+
+
 def theta_ref_field(mesh: Mesh, phys: Physics):
     """Source/Forcing.cpp:88-125 on the cells and one ghost (begin -1)."""
     t = np.zeros(_shape(mesh.geom, 0, 1))
