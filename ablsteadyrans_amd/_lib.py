@@ -333,7 +333,7 @@ class NewtonOptionsC(C.Structure):
                 ("start_cfl", C.c_double), ("ser_exponent", C.c_double), ("ser_max_growth", C.c_double),
                 ("armijo_c", C.c_double), ("backtrack", C.c_double), ("lambda_min", C.c_double),
                 ("cfl_cut", C.c_double), ("max_retries", C.c_int32), ("dtau_inf_at_cap", C.c_int32),
-                ("ptc_growth", C.c_double)]
+                ("ptc_growth", C.c_double), ("pc_denominator", C.c_int32)]
 
 
 class NewtonStepInfoC(C.Structure):
@@ -565,6 +565,8 @@ _sig = {
                            C.c_void_p]),
     "gust_newton_create": (C.c_int, [C.c_void_p, _P(NewtonOptionsC), _P(C.c_void_p), C.c_void_p]),
     "gust_newton_destroy": (None, [C.c_void_p]),
+    "gust_newton_pc_setup": (C.c_int, [C.c_void_p, _P(StateC), _P(Array4), C.c_void_p]),
+    "gust_newton_pc_apply": (C.c_int, [C.c_void_p, _P(ResidualsC), _P(ResidualsC), C.c_void_p]),
     "gust_newton_step": (C.c_int, [C.c_void_p, _P(StateC), _P(Array4), _P(NewtonStepInfoC), C.c_void_p]),
     "gust_newton_residual_norm": (C.c_int, [C.c_void_p, _P(StateC), _P(C.c_double), C.c_void_p]),
     "gust_continuation_step": (C.c_int, [C.c_void_p, _P(StateC), _P(C.c_int32), _P(ContinuationRecordC),

@@ -540,6 +540,7 @@ struct NewtonSolver {
     DevState q0, d, qn;
     DevRes Rp;
     DevField pcf[7], pss, F0[3], F1[3], dtf[3], A[3], At, Ak, ap[3], dd[3], conv0, conv1, pdtau;
+    DevField At1, Ak1;   // stage denominators: theta's and k's rows at the corrected transport
     gust_closure_fields pcfv{};
     double speed = 1;
     bool pc_ready = false;
@@ -915,22 +916,31 @@ struct NewtonSolver {
             GUST_HIP(hipGetLastError());
         }
         fill_h(&d.s, st);
-        for (int c = 0; c < 3; ++c) {
-            diag_denominator_kernel<<<grid_for(pk.box[c].npts()), kThreads, 0, st>>>(
-                pk.box[c], dview<double>(&dd[c].v), dview<const double>(&ap[c].v));
+        // the denominators: diagonal, d = a_P where the stage's d > 0; stage,
+        // the predictor's d = a_P - sum a_nb as it left them
+        const bool linear = opt.pc_denominator == 1;
+        if (linear) {
+            for (int c = 0; c < 3; ++c) {
+                diag_denominator_kernel<<<grid_for(pk.box[c].npts()), kThreads, 0, st>>>(
+                    pk.box[c], dview<double>(&dd[c].v), dview<const double>(&ap[c].v));
+            }
+            GUST_HIP(hipGetLastError());
         }
-        GUST_HIP(hipGetLastError());
         // the source of the correction: r_p (a net source is projected out)
         scale_copy_field_from_packed(r, 3, 1.0, st);
         gust_pressure_correction_result cres{};
         check_rc(gust_pressure_correction(S.solver, &g, d.s.f[0], d.s.f[1], d.s.f[2], d.s.f[3], &S.z, &S.ax, &S.ay,
                                           &S.az, &S.J, &dd[0].v, &dd[1].v, &dd[2].v, S.pbc, S.opt.alpha_p,
                                           &Rp.f[3].v, 1, &cres, st));
-        // 3. theta's coupling: the directional derivative of theta's
-        // convection along du with the step sigma du (sigma = sqrt(eps) U /
-        // max|du|), divided by sigma (Rp's theta still holds -r_theta)
-        const double dmax = velocity_max(d.s, st);
-        const double sigma = dmax > 0 ? std::sqrt(std::numeric_limits<double>::epsilon()) * speed / dmax : 1.0;
+        // 3. theta's coupling (Rp's theta still holds -r_theta). diagonal:
+        // the directional derivative of theta's convection along du with the
+        // step sigma du (sigma = sqrt(eps) U / max|du|), divided by sigma;
+        // stage: the difference at u + du itself (sigma = 1)
+        double sigma = 1.0;
+        if (linear) {
+            const double dmax = velocity_max(d.s, st);
+            sigma = dmax > 0 ? std::sqrt(std::numeric_limits<double>::epsilon()) * speed / dmax : 1.0;
+        }
         copy_state(qn.s, q0.s, st);
         saxpy_valid(qn.s, sigma, d.s, 0, 2, st);
         fill(&qn.s, nullptr, st);
@@ -942,11 +952,22 @@ struct NewtonSolver {
             pk.box[4], dview<double>(&Rp.f[4].v), 1.0 / sigma, dview<const double>(&conv1.v),
             dview<const double>(&conv0.v));
         GUST_HIP(hipGetLastError());
-        // 4. the theta and k steps on the increments, q's transport, no floor
+        // 4. the theta and k steps on the increments, no floor. diagonal:
+        // with q's transport (the rows of setup); stage: with the corrected
+        // transport F1 (rows rebuilt from it)
+        gust_array4* rows_t = &At.v;
+        gust_array4* rows_k = &Ak.v;
+        if (!linear) {
+            gust_faces F1f{{&F1[0].v, &F1[1].v, &F1[2].v}};
+            check_rc(gust_implicit_rows(GUST_EQ_THETA, &g, &F1f, &S.mesh, &S.phys, cf, ssv, &At1.v, st));
+            check_rc(gust_implicit_rows(GUST_EQ_TKE, &g, &F1f, &S.mesh, &S.phys, cf, ssv, &Ak1.v, st));
+            rows_t = &At1.v;
+            rows_k = &Ak1.v;
+        }
         gust_scalar_sweep_result rt{}, rk{};
-        check_rc(gust_scalar_sweep(&g, GUST_EQ_THETA, d.s.f[4], &Rp.f[4].v, &At.v, &pdtau.v, S.opt.sweeps, 0.0, 0,
+        check_rc(gust_scalar_sweep(&g, GUST_EQ_THETA, d.s.f[4], &Rp.f[4].v, rows_t, &pdtau.v, S.opt.sweeps, 0.0, 0,
                                    &S.opt.sweep, &rt, st));
-        check_rc(gust_scalar_sweep(&g, GUST_EQ_TKE, d.s.f[5], &Rp.f[5].v, &Ak.v, &pdtau.v, S.opt.sweeps, 0.0, 0,
+        check_rc(gust_scalar_sweep(&g, GUST_EQ_TKE, d.s.f[5], &Rp.f[5].v, rows_k, &pdtau.v, S.opt.sweeps, 0.0, 0,
                                    &S.opt.sweep, &rk, st));
         // 5. dq: the increment's valid points; the Dirichlet rows of u, v, w
         // take r; the periodic seam copies take their owner's
@@ -1361,6 +1382,9 @@ std::vector<std::string> newton_option_errors(const gust_newton_options& o)
     if (o.preconditioner != 0 && o.preconditioner != 1) {
         err.push_back("steady.newton_preconditioner must be identity or simplec");
     }
+    if (o.pc_denominator != 0 && o.pc_denominator != 1) {
+        err.push_back("the preconditioner's pressure denominator must be stage (0) or diagonal (1)");
+    }
     if (std::isnan(o.cfl_max) || !(o.cfl_max > 0) || !std::isfinite(o.cfl_max)) {
         err.push_back("steady.newton_cfl_max = " + num(o.cfl_max) + ": must be finite and > 0");
     }
@@ -1430,6 +1454,7 @@ int gust_newton_default_options(gust_newton_options* o)
         o->max_retries = 4;
         o->dtau_inf_at_cap = 1;
         o->ptc_growth = 1;
+        o->pc_denominator = 1;
     });
 }
 
@@ -1551,6 +1576,10 @@ int gust_newton_create(gust_simplec* stage, const gust_newton_options* opt, gust
             }
             nw->At.alloc(cells, GUST_N_COEFS, st);
             nw->Ak.alloc(cells, GUST_N_COEFS, st);
+            if (opt->pc_denominator == 0) {
+                nw->At1.alloc(cells, GUST_N_COEFS, st);
+                nw->Ak1.alloc(cells, GUST_N_COEFS, st);
+            }
             nw->conv0.alloc(cells, 1, st);
             nw->conv1.alloc(cells, 1, st);
             nw->pdtau.alloc(cells, 1, st);
@@ -1619,6 +1648,49 @@ int gust_newton_residual_norm(gust_newton* n, gust_state* q, double* r, void* st
         if (!n || !q || !r) { GUST_FAIL("gust: NewtonKrylov::residual_norm: null argument"); }
         check_state2("NewtonKrylov::residual_norm", as_n(n)->g, q, "q");
         *r = as_n(n)->residual_norm(q, static_cast<hipStream_t>(stream));
+    });
+}
+
+int gust_newton_pc_setup(gust_newton* n, const gust_state* q, const gust_array4* dtau, void* stream)
+{
+    return guarded([&] {
+        if (!n || !q) { GUST_FAIL("gust: SimplecPreconditioner::setup: null argument"); }
+        NewtonSolver& N = *as_n(n);
+        if (N.opt.preconditioner != 1) {
+            GUST_FAIL("gust: SimplecPreconditioner::setup: the Newton stage has the identity preconditioner");
+        }
+        check_state2("SimplecPreconditioner::setup", N.g, q, "q");
+        if (dtau && !covers(dtau, loc_box(N.g, GUST_LOC_CELL), 1)) {
+            GUST_FAIL("gust: SimplecPreconditioner::setup: dtau must cover the valid cells");
+        }
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        N.pc_setup(*q, dtau, st);
+        GUST_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int gust_newton_pc_apply(gust_newton* n, const gust_residuals* r, gust_residuals* dq, void* stream)
+{
+    return guarded([&] {
+        if (!n || !r || !dq) { GUST_FAIL("gust: SimplecPreconditioner::apply: null argument"); }
+        NewtonSolver& N = *as_n(n);
+        if (N.opt.preconditioner != 1) {
+            GUST_FAIL("gust: SimplecPreconditioner::apply: the Newton stage has the identity preconditioner");
+        }
+        for (int f = 0; f < kNF; ++f) {
+            const IBox vb = loc_box(N.g, kStageLocs[f]);
+            if (!covers(r->R[f], vb, 1) || !covers(dq->R[f], vb, 1)) {
+                GUST_FAIL("gust: SimplecPreconditioner::apply: r and dq must cover every field's valid points");
+            }
+        }
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        const double ones[kNF] = {1, 1, 1, 1, 1, 1};
+        N.pack(r->R, ones, N.b.p, st);
+        N.pc_apply(N.b.p, N.x.p, st);
+        gust_array4* dst[kNF];
+        for (int f = 0; f < kNF; ++f) { dst[f] = const_cast<gust_array4*>(dq->R[f]); }
+        N.unpack(dst, ones, N.x.p, st);
+        GUST_HIP(hipStreamSynchronize(st));
     });
 }
 

@@ -22,6 +22,9 @@ from .physics import FIELDS, LOCS, DeviceState, DriverOptions, ResidualInputs, S
 from .poisson import DeviceMesh, _stream
 
 IDENTITY, SIMPLEC = 0, 1
+# the SIMPLEC preconditioner's pressure denominators (PressureDenominator,
+# Source/NewtonPreconditioner.H:116-118)
+STAGE_DENOMINATOR, DIAGONAL_DENOMINATOR = 0, 1
 
 
 @dataclass
@@ -46,6 +49,7 @@ class NewtonOptions:
     max_retries: int = 4
     dtau_inf_at_cap: int = 1
     ptc_growth: float = 1.0
+    pc_denominator: int = DIAGONAL_DENOMINATOR
 
     def c(self) -> _lib.NewtonOptionsC:
         o = _lib.NewtonOptionsC()
@@ -153,6 +157,22 @@ class NewtonStage:
         r = C.c_double(0)
         _lib.check(_lib.lib.gust_newton_residual_norm(self._h, C.byref(qc), C.byref(r), _stream(stream)))
         return float(r.value)
+
+    def pc_setup(self, q: DeviceState, dtau: Fab | None = None, stream=None) -> None:
+        """SimplecPreconditioner::setup (Source/NewtonPreconditioner.cpp:
+        114-148): the linearisation point q (ghosts filled) and the cell step
+        (None: the local step at start_cfl)."""
+        qc = q.c()
+        _lib.check(_lib.lib.gust_newton_pc_setup(self._h, C.byref(qc), C.byref(dtau.array4()) if dtau is not None
+                                                 else None, _stream(stream)))
+
+    def pc_apply(self, r: dict, dq: dict, stream=None) -> None:
+        """SimplecPreconditioner::apply (Source/NewtonPreconditioner.cpp:
+        150-224): dq = one SIMPLEC iteration on (D + J) dq = r; r and dq as
+        :func:`residual_fields` gives them."""
+        rc, k0 = _res_c(r)
+        dc, k1 = _res_c(dq)
+        _lib.check(_lib.lib.gust_newton_pc_apply(self._h, C.byref(rc), C.byref(dc), _stream(stream)))
 
     def continuation_step(self, q: DeviceState, stream=None):
         """Continuation::step (Source/Continuation.cpp:339-427): returns

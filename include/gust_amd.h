@@ -1381,6 +1381,11 @@ typedef struct gust_newton_options {
     int32_t max_retries;       /* steady.newton_max_retries, 4 */
     int32_t dtau_inf_at_cap;   /* steady.newton_dtau_inf_at_cap, 1 */
     double ptc_growth;         /* steady.newton_ptc_growth, 1 (off) */
+    int32_t pc_denominator;    /* the SIMPLEC preconditioner's pressure denominators
+                                  (PressureDenominator, Source/NewtonPreconditioner.H:116-118):
+                                  1 diagonal, d = a_P (default); 0 stage, the SIMPLEC
+                                  d = a_P - sum a_nb, with which apply(-R(q)) is one
+                                  iteration of the stage */
 } gust_newton_options;
 
 /* Source/Newton.H:236-252 (NewtonStepInfo). */
@@ -1459,6 +1464,18 @@ int gust_newton_step(gust_newton* n, gust_state* q, const gust_array4* dtau, gus
 /* |T^-1 R(fill(q))|_2 (q's ghosts filled).
  * Replaces: NewtonKrylov::residual_norm -- Source/Newton.cpp:553-565. */
 int gust_newton_residual_norm(gust_newton* n, gust_state* q, double* r, void* stream);
+
+/* The SIMPLEC preconditioner of the stage (created with preconditioner 1)
+ * on its own: setup at q (ghosts filled) with the cell step dtau (null: the
+ * local step at start_cfl), then dq = apply(r) on the six fields' valid
+ * points (r's periodic seam copies equal; dq's take the owner's, the low
+ * face). The Newton step calls both itself; these are for callers that
+ * precondition their own Krylov solves.
+ * Replaces: SimplecPreconditioner::setup(const State&, const MultiFab* dtau),
+ *   SimplecPreconditioner::apply(NewtonVector& dq, const NewtonVector& r) --
+ *   Source/NewtonPreconditioner.H:141-144, NewtonPreconditioner.cpp:114-224. */
+int gust_newton_pc_setup(gust_newton* n, const gust_state* q, const gust_array4* dtau, void* stream);
+int gust_newton_pc_apply(gust_newton* n, const gust_residuals* r, gust_residuals* dq, void* stream);
 
 /* One continuation step with its retries: local dtau at the state's CFL (or
  * none at infinite dtau), the Newton direction, the Armijo line search (k

@@ -143,3 +143,22 @@ def test_jacobi_schedule_refuses_unknown_values():
         LS.set_jacobi_schedule(ok)
     L.jacobi_tune_reset()
     assert L.jacobi_tuned() == []
+
+
+def test_newton_option_refusals_name_their_keys():
+    """check_newton_options / check_continuation_options (Source/Newton.cpp:
+    161-193): the defaults pass; each bad value is refused with the key it
+    came from, before any device call."""
+    from ablsteadyrans_amd import newton as NW
+    assert NW.check_newton_options(NW.NewtonOptions()) == []
+    cases = [("gmres_restart", 4, "steady.newton_gmres_restart"), ("gmres_restart", 201, "steady.newton_gmres_restart"),
+             ("krylov_rtol", 1.0, "steady.newton_krylov_rtol"), ("preconditioner", 2, "steady.newton_preconditioner"),
+             ("start_cfl", 200.0, "steady.cfl_max"), ("armijo_c", 0.5, "steady.newton_armijo_c"),
+             ("fd_eps_scale", 1e-13, "steady.newton_fd_eps_scale"), ("ptc_growth", 0.5, "steady.newton_ptc_growth"),
+             ("pc_denominator", 2, "pressure denominator")]
+    for name, bad, key in cases:
+        opt = NW.NewtonOptions(**{name: bad})
+        msgs = NW.check_newton_options(opt)
+        assert len(msgs) == 1 and key in msgs[0], (name, msgs)
+    both = NW.check_newton_options(NW.NewtonOptions(gmres_restart=1, lambda_min=2.0))
+    assert len(both) == 2

@@ -170,3 +170,121 @@ def test_simplec_newton_run_of_the_column_deck(cuda):
     errs = _rel_l2(q, np.load(GOLD))
     print("  vs the composite's converged state:", {k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
     assert max(errs.values()) <= 1e-6, errs
+
+
+def _pc_case(case):
+    """The two tiles of the stage's oracle tests: a periodic hill with
+    stratification, rotation and the wall model; an inflow/outflow pair along
+    x with slip and no-slip walls along y."""
+    from ablsteadyrans_amd import physics as GP
+    from tests import physics_helpers as PHY
+    from tests import test_gpu_physics as TGP
+    from tests.test_gpu_stage import _inputs
+    if case == "hill":
+        tile = PHY.hill_mesh(n=(16, 12, 10), height=60.0, half=220.0)
+        q = PHY.random_state(tile, 11, U=6.0, amp=0.05)
+        phys = P.Physics(closure=P.ClosureOptions(tke_floor=1e-6),
+                         bc=P.Boundary(zlo=P.SURFACE_LAYER, zhi=P.SLIP_WALL, zhi_theta_grad=0.003),
+                         surface=P.SurfaceOptions(z0=0.1),
+                         forcing=P.Forcing(coriolis=True, latitude=45.0, geostrophic=True, ug=6.0, vg=1.0),
+                         theta_ref_z=(0.0, 1000.0), theta_ref=(300.0, 303.0))
+        return tile, q, phys, _inputs(phys), 600.0
+    tile, _m, q, phys = TGP._open_setup(0, seed=11)
+    phys.closure = P.ClosureOptions(tke_floor=1e-6)
+    q["w"] *= 0.05
+    q["tke"] += 0.5   # clear of the stage's k floor (apply has none)
+    gi = TGP._open_inputs(phys)
+    gi.closure = GP.ClosureOptions(tke_floor=1e-6)
+    return tile, q, phys, gi, 500.0
+
+
+def _seam_sync(a, n, geom):
+    """OverrideSync of a valid-point array: the hi copies of a periodic
+    direction's faces take the lo copies' values."""
+    if n == "u" and geom.periodic[0]:
+        a[..., -1] = a[..., 0]
+    if n == "v" and geom.periodic[1]:
+        a[..., -1, :] = a[..., 0, :]
+    return a
+
+
+@pytest.mark.parametrize("case", ["hill", "open"])
+def test_stage_denominator_apply_is_one_simplec_iteration(cuda, case):
+    """NewtonPreconditioner.ApplyIsOneSimplecIteration
+    (Tests/Unit_Tests/test_NewtonPreconditioner.cpp:316-411): with the stage's
+    denominators, apply(-R(q)) at a non-uniform step (local dtau at CFL 5)
+    equals the increment of one iteration of the SIMPLEC stage from q at that
+    step, within 1e-9 of each field's largest increment (the pressure modulo
+    its J-weighted mean, the gauge); on the Dirichlet rows of u, v and w
+    (identity rows of the Newton system) dq = r exactly."""
+    import torch
+    from ablsteadyrans_amd import newton as NW
+    from ablsteadyrans_amd import physics as GP
+    from ablsteadyrans_amd.fab import Fab
+    from ablsteadyrans_amd.poisson import DeviceMesh
+    from oracle import oracle as orc
+    from tests import physics_helpers as PHY
+    tile, q, phys, gi, height = _pc_case(case)
+    m = PHY.omesh(tile)
+    g = m.geom
+    dm = DeviceMesh(tile)
+    stage = GP.SimplecStage(dm, gi, GP.SimplecOptions(pressure_reltol=1e-12, sweeps=2, tke_floor=1e-6,
+                                                      height=height))
+    ds = _dstate(tile, q)
+    stage.init(ds)
+    GP.fill_ghosts(dm, ds, gi)
+    B1 = (-1, -1, -1)
+    q1 = {n: ds[n].numpy()[:, 1:-1, 1:-1, 1:-1].copy() for n in FIELDS}
+    dtau = orc.local_dtau(q1["u"], q1["v"], q1["w"], B1, m.z, B1, m.J, B1, g, 5.0)
+    assert dtau.max() > 1.2 * dtau.min()
+    dt = Fab(torch.from_numpy(dtau[None].copy()).cuda(), tuple(g.lo))
+    # r = -R(q), its seam copies equal
+    R = NW.residual_fields(tile.geom)
+    GP.residual(dm, ds, gi, R)
+    torch.cuda.synchronize()
+    r = NW.residual_fields(tile.geom)
+    rnp = {}
+    for n in FIELDS:
+        rnp[n] = _seam_sync(-R[n].numpy(), n, tile.geom)
+        r[n].data.copy_(torch.from_numpy(rnp[n]))
+    nk = NW.NewtonStage(stage, NW.NewtonOptions(pc_denominator=NW.STAGE_DENOMINATOR))
+    nk.pc_setup(ds, dt)
+    dq = NW.residual_fields(tile.geom)
+    nk.pc_apply(r, dq)
+    torch.cuda.synchronize()
+    # one stage iteration from q at the same step
+    qs = GP.DeviceState(tile.geom)
+    for n in FIELDS:
+        qs[n].data.copy_(ds[n].data)
+    info = stage.iterate(qs, dt)
+    torch.cuda.synchronize()
+    assert info.tke.clipped == 0
+    Jv = tile.detJ[1:-1, 1:-1, 1:-1]
+    eqs = (P.EQ_U, P.EQ_V, P.EQ_W)
+    for f, n in enumerate(FIELDS):
+        d = dq[n].numpy()[0]
+        inc = _seam_sync(qs[n].numpy()[0, 2:-2, 2:-2, 2:-2] - ds[n].numpy()[0, 2:-2, 2:-2, 2:-2], n, tile.geom)
+        if n == "p":
+            assert Jv.shape == d.shape
+            d = d - (Jv * d).sum() / Jv.sum()
+            inc = inc - (Jv * inc).sum() / Jv.sum()
+        if f < 3:
+            mask = P.dirichlet_rows(eqs[f], m, phys)[0]
+            assert mask.shape == d.shape
+            assert np.array_equal(d[mask > 0], rnp[n][0][mask > 0]), n
+            if n == "w":
+                assert mask.sum() > 0
+            d, inc = d * (1 - mask), inc * (1 - mask)
+        scale = float(np.abs(inc).max())
+        err = float(np.abs(d - inc).max())
+        print(f"{case} {n}: max |increment| {scale:.3e}, max |apply - increment| {err:.3e}", flush=True)
+        assert scale > 0, n
+        assert err <= 1e-9 * scale, (n, err, scale)
+    # the diagonal denominators differ (the pressure increment no longer
+    # scales with 1/dtau)
+    nd = NW.NewtonStage(stage, NW.NewtonOptions())
+    nd.pc_setup(ds, dt)
+    dq2 = NW.residual_fields(tile.geom)
+    nd.pc_apply(r, dq2)
+    torch.cuda.synchronize()
+    assert not np.allclose(dq2["p"].numpy(), dq["p"].numpy(), rtol=1e-3, atol=0)
