@@ -288,3 +288,145 @@ def test_stage_denominator_apply_is_one_simplec_iteration(cuda, case):
     nd.pc_apply(r, dq2)
     torch.cuda.synchronize()
     assert not np.allclose(dq2["p"].numpy(), dq["p"].numpy(), rtol=1e-3, atol=0)
+
+
+def _base_value(f, x, y, z, Lx=900.0, Ly=600.0, H=400.0):
+    """A smooth non-uniform state (Tests/Unit_Tests/test_NewtonPreconditioner.cpp:96-107)."""
+    ax, ay, az = 2 * np.pi * x / Lx, 2 * np.pi * y / Ly, np.pi * z / H
+    return (8 + 2 * np.sin(ax + 0.3) + z / H, 3 + np.cos(ay) + 0.5 * np.sin(az),
+            0.5 * np.sin(az) * (1 + 0.2 * np.cos(ax)), 5 * np.sin(ax) * np.cos(ay), 300 + 0.01 * z + 0.5 * np.sin(ax),
+            0.4 + 0.1 * np.cos(ay))[f]
+
+
+def _open_hill(hill=30.0):
+    """The open hill of NewtonPreconditioner.IterationsBoundedOverCfl
+    (test_NewtonPreconditioner.cpp:163-211): 9 x 6 x 8 cells on 900 x 600 x
+    400 m over a 30 m hill, inflow (log law, unbounded length) and outflow
+    along x, periodic y, the wall law on the ground, a slip top, upwind
+    convection, no buoyancy, fixed diffusivities of 2 m^2/s. The hill is
+    round here (the reference's is a ridge along y)."""
+    import torch
+    from ablsteadyrans_amd import initializer as GI
+    from ablsteadyrans_amd import mesh as M
+    from ablsteadyrans_amd import physics as GP
+    from ablsteadyrans_amd.fab import Fab
+    from tests import physics_helpers as PHY
+    n, hi = (9, 6, 8), (900.0, 600.0, 400.0)
+    geom = M.geometry(n, (0.0, 0.0, 0.0), hi, (0, 1))
+    zl = M.make_zlevels(n[2], hi[2])
+    tile = M.build_mesh(geom, (0.0, 0.0, 0.0), hi, zl, M.Terrain(kind="hills", hills=[(450.0, 300.0, hill, 150.0)]))
+    q = GP.DeviceState(tile.geom)
+    for f, name in enumerate(FIELDS):
+        X, Y, Z = PHY.positions(tile, GP.LOCS[f])
+        q[name].data.copy_(torch.from_numpy(_base_value(f, X, Y, Z)[None].copy()))
+    diff = []
+    for _ in range(3):
+        d = Fab.at_loc(tile.geom, 0, 1, 1)
+        d.data.fill_(2.0)
+        diff.append(d)
+    gi = GP.ResidualInputs(closure=None, nu=diff[0], gamma_theta=diff[1], gamma_tke=diff[2],
+                           surface=GP.SurfaceLayerOptions(z0=0.1), forcing=GP.ForcingOptions(gravity=0.0),
+                           faces=(GP.INFLOW, GP.PERIODIC, GP.SURFACE_LAYER, GP.OUTFLOW, GP.PERIODIC, GP.SLIP_WALL),
+                           theta_ref_z=(-1000.0, 5000.0), theta_ref=(300.0, 300.0),
+                           momentum_limiter=GP.UPWIND, theta_limiter=GP.UPWIND,
+                           inflow=GI.InflowProfile(kind=GI.LOG_LAW, unbounded=True))
+    return tile, q, gi, diff[0]
+
+
+def _open_hill_steps(cfl, pc, denominator=1):
+    """Four Newton steps at a fixed CFL (dtau from each step's state), GMRES(200)
+    at Krylov tolerance 1e-3; the GMRES iterations of each step."""
+    import torch
+    from ablsteadyrans_amd import newton as NW
+    from ablsteadyrans_amd import physics as GP
+    from ablsteadyrans_amd import pseudotime as PT
+    from ablsteadyrans_amd.fab import Fab
+    from ablsteadyrans_amd.poisson import DeviceMesh
+    tile, q, gi, nu = _open_hill()
+    dm = DeviceMesh(tile)
+    stage = GP.SimplecStage(dm, gi, GP.SimplecOptions(sweeps=2, height=400.0))
+    GP.fill_ghosts(dm, q, gi)
+    nk = NW.NewtonStage(stage, NW.NewtonOptions(gmres_restart=200, krylov_rtol=1e-3, preconditioner=pc,
+                                                pc_denominator=denominator, cfl_max=1e5, start_cfl=10.0))
+    dtau = Fab.at_loc(tile.geom, 0, 1, 0)
+    its = []
+    for _ in range(4):
+        PT.local_dtau(dm, q["u"], q["v"], q["w"], cfl, dtau, nu=nu)
+        info = nk.step(q, dtau)
+        assert info["applied"], (cfl, pc, denominator, info)
+        its.append(info["gmres_iterations"])
+    torch.cuda.synchronize()
+    return its
+
+
+def test_preconditioner_iterations_bounded_over_cfl(cuda):
+    """NewtonPreconditioner.IterationsBoundedOverCfl
+    (test_NewtonPreconditioner.cpp:558-575): with the diagonal denominators
+    the mean GMRES iterations per Newton step stay within 1.5 x their CFL-10
+    value up to CFL 1e4; with the stage's denominators they grow at least
+    1.5-fold to CFL 1e4 and exceed the diagonal form's five-fold there; the
+    identity needs at least five times the diagonal form's at CFL 10 and
+    1e4."""
+    from ablsteadyrans_amd import newton as NW
+    cfls = (1e1, 1e2, 1e3, 1e4)
+    sc, st, ids = [], [], []
+    for cfl in cfls:
+        ends = cfl in (cfls[0], cfls[-1])
+        sc.append(float(np.mean(_open_hill_steps(cfl, NW.SIMPLEC))))
+        st.append(float(np.mean(_open_hill_steps(cfl, NW.SIMPLEC, NW.STAGE_DENOMINATOR))) if ends else 0.0)
+        ids.append(float(np.mean(_open_hill_steps(cfl, NW.IDENTITY))) if ends else 0.0)
+        print(f"CFL {cfl:g}: mean GMRES iterations simplec {sc[-1]}, stage's d {st[-1]}, identity {ids[-1]}",
+              flush=True)
+    for n in range(1, len(cfls)):
+        assert sc[n] < 1.5 * sc[0], (cfls[n], sc)
+    assert st[3] >= 1.5 * st[0], st
+    assert st[3] >= 5 * sc[3], (st, sc)
+    assert ids[0] >= 5 * sc[0] and ids[3] >= 5 * sc[3], (ids, sc)
+
+
+def _continuation_run(pc, max_steps=30, tol=1e-8):
+    from ablsteadyrans_amd import newton as NW
+    from ablsteadyrans_amd import physics as GP
+    from ablsteadyrans_amd.poisson import DeviceMesh
+    tile, q, gi, _nu = _open_hill()
+    dm = DeviceMesh(tile)
+    stage = GP.SimplecStage(dm, gi, GP.SimplecOptions(sweeps=2, height=400.0))
+    GP.fill_ghosts(dm, q, gi)
+    nk = NW.NewtonStage(stage, NW.NewtonOptions(gmres_restart=200, preconditioner=pc))
+    r0 = nk.residual_norm(q)
+    steps = gmres = 0
+    r = r0
+    for _ in range(max_steps):
+        ok, rec = nk.continuation_step(q)
+        assert ok, rec
+        steps += 1
+        gmres += rec["gmres_iterations"]
+        r = nk.residual_norm(q)
+        if r <= tol * r0:
+            break
+    return q, r0, r, steps, gmres
+
+
+def test_continuation_identity_versus_simplec(cuda):
+    """NewtonPreconditioner.ContinuationIdentityVersusSimplec
+    (test_NewtonPreconditioner.cpp:586-620): the pseudo-transient
+    continuation on the open hill with GMRES(200) converges to 1e-8 of its
+    first residual with either preconditioner, to the same state (1e-6 of
+    the speed scale), the SIMPLEC preconditioner in at most as many steps
+    with at most a fifth of the GMRES iterations (those of each step's
+    accepted attempt)."""
+    from ablsteadyrans_amd import newton as NW
+    qa, r0a, ra, sa, ga = _continuation_run(NW.IDENTITY)
+    qb, r0b, rb, sb, gb = _continuation_run(NW.SIMPLEC)
+    print(f"identity: |R0| {r0a:.4e}, {sa} steps, {ga} GMRES iterations, |R| {ra:.3e}", flush=True)
+    print(f"simplec:  |R0| {r0b:.4e}, {sb} steps, {gb} GMRES iterations, |R| {rb:.3e}", flush=True)
+    assert r0a == r0b
+    assert ra <= 1e-8 * r0a and rb <= 1e-8 * r0b
+    assert sb <= sa and 5 * gb <= ga, (sa, sb, ga, gb)
+    speed = float(max(np.abs(qa[n].numpy()).max() for n in ("u", "v", "w")))
+    for n in FIELDS:
+        a, b = qa[n].numpy()[0, 2:-2, 2:-2, 2:-2], qb[n].numpy()[0, 2:-2, 2:-2, 2:-2]
+        err = float(np.abs(a - b).max())
+        print(f"  {n}: max |identity - simplec| {err:.2e}", flush=True)
+        if n in ("u", "v", "w"):
+            assert err <= 1e-6 * speed, (n, err, speed)
