@@ -277,6 +277,34 @@ def _time_solves(precision, steps, warmup):
     return sum(times) / len(times), int(last.iterations), int(last.status), float(last.mon[2]), times
 
 
+def _time_newton_solves(steps, warmup):
+    """Whole steady solves of the column deck with steady.stage =
+    simplec_newton (SIMPLEC, the switch at r <= 1e-4 after 20 iterations with
+    the monitors settled, then pseudo-transient Newton-Krylov with the
+    SIMPLEC preconditioner, GMRES(60)): seconds per solve and the counts."""
+    import torch
+    from ablsteadyrans_amd import decks as D
+    from ablsteadyrans_amd import newton as NW
+    deck = D.prepare(D.neutral_column(), 0)
+    nk = NW.NewtonStage(deck.simplec, NW.NewtonOptions(gmres_restart=60, start_cfl=deck.driver.cfl_max))
+    times, last = [], None
+    for rep in range(warmup + steps):
+        D.reinitialize(deck)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = nk.solve(deck.state, deck.driver, NW.SwitchOptions())
+        torch.cuda.synchronize()
+        if rep >= warmup:
+            times.append(time.perf_counter() - t0)
+        last = r
+    sec = sum(times) / len(times)
+    return {"steady_solves_per_s": round(1.0 / sec, 4), "seconds_per_solve": round(sec, 4),
+            "iterations": last["iterations"], "simplec_iterations": last["simplec_iterations"],
+            "newton_iterations": last["newton_iterations"], "newton_steps": last["newton_steps"],
+            "first_switch": last["first_switch"], "status": last["status"],
+            "relative_residual": last["relative_residual"]}
+
+
 # The Gaussian-hill benchmark's solver settings: those of the reference's
 # 3-D hill deck (Tests/Canonical_RANS/Neutral_Hill_3D/inputs_steady_hill3d:
 # 16 z-line sweeps, CFL 1 -> 4, rotation_relax 1). With 2 sweeps and CFL 10
@@ -684,6 +712,7 @@ def main():
     print("bench: steady solves of the Neutral_Column deck (FP64, mixed)", file=sys.stderr, flush=True)
     s64, it64, st64, us64, t64 = _time_solves(0, 3, 1)
     s32, it32, st32, us32, t32 = _time_solves(2, 3, 1)
+    newton = _time_newton_solves(3, 1)
     # 3. the reference's own steady decks with published CPU times
     print("bench: the reference's steady decks (Askervein, Neutral_ABL_Flat, Neutral_Hill_3D)", file=sys.stderr,
           flush=True)
@@ -764,6 +793,8 @@ def main():
                   "column_fp64_status": st64, "column_ustar": round(us64, 8),
                   "column_mixed_steady_solves_per_s": round(1.0 / s32, 4), "column_mixed_iterations": it32,
                   "column_ms_per_iteration_fp64": round(s64 * 1e3 / max(it64, 1), 3),
+                  "column_simplec_newton": newton,
+                  "column_simplec_newton_vs_simplec": round(newton["steady_solves_per_s"] * s64, 3),
                   "sweep_fp64_gbs": round(gbs64, 2), "sweep_fp64_ms": round(ms_ev, 5),
                   "sweep_fp32_gbs": round(gbs32, 2), "sweep_fp32_ms": round(ms32, 5),
                   "sweep_fp32_frac": round(gbs32 / HBM_PEAK_GBS, 4),
