@@ -1315,7 +1315,23 @@ void zjacobi_run(int nsys, const gust_line_system* sys, int sweeps, double* chan
 }
 
 // ---------------------------------------------------------------------------
-// Line LU-SGS: one workgroup per system.
+// Line LU-SGS. The Thomas denominators m_k and ratios c'_k of a column
+// depend on its coefficients only, not on the right-hand side, so one
+// parallel pre-pass over every column of the sweep region computes them for
+// all the sweeps of the call. A sweep then walks the (i, j) diagonals in
+// order (a diagonal's columns are independent): all threads of the
+// workgroup form the diagonal's right-hand sides (every column, every level)
+// from the neighbours the previous diagonal left, then one thread per column
+// runs the forward recurrence e_k = (r_k - l_k e_{k-1}) / m_k and the back
+// substitution with their operands loaded ahead in chunks. Each operation
+// is the column Thomas solve's own, in its order (the bits of the sequential
+// column-by-column pass). One workgroup per system; a batch of systems (u,
+// v, w of the momentum predictor) shares the launch.
+
+constexpr int kLusgsMaxBatch = 3;
+constexpr int kLusgsChunk = 8;
+constexpr int kLusgsRhsBatch = 6;
+constexpr size_t kLusgsMaxLds = 144 * 1024;
 
 template <typename T>
 struct LusgsSys {
@@ -1330,8 +1346,9 @@ struct LusgsSys {
     T* w;                // zero ghosts: work array at valid lo (ng + 1 ghosts)
     int64_t w_j, w_k;
     int w_gx, w_gy;      // work ghost widths
-    T* c;                // Thomas scratch over the sweep region
-    T* e;
+    T* c;                // c'_k over the sweep region
+    T* e;                // right-hand sides, then e'_k
+    T* m;                // the denominators m_k
     int64_t c_j, c_k;
     int nx, ny, nz;      // valid extents at loc (local index 0 = valid lo)
     int Lx, Ly;          // periodic shifts (cells), 0 if not periodic
@@ -1340,6 +1357,13 @@ struct LusgsSys {
     int fold[2];         // one-cell periodic axis folded into the diagonal
     int zero_mode;       // 1: zero ghosts (work array), 0: lagged (in delta)
     int loc;
+    T* sk[10];           // the skewed copies (lagged, no alternation): kSk*
+    int sP, sJ;          // their extents: sP = nx + ny + 3 diagonals, sJ = ny + 2
+};
+
+template <typename T>
+struct LusgsBatch {
+    LusgsSys<T> s[kLusgsMaxBatch];
 };
 
 // Coefficient/ap/rhs index (local) of a point of the grown region: the
@@ -1353,72 +1377,44 @@ __device__ __forceinline__ int wrap_idx(int i, int n, int L)
     return i;
 }
 
+// m_k and c'_k of every column of the sweep region (blockIdx.y: the
+// system). The fold of a periodic one-cell axis (Source/LineSweeps.cpp:
+// 242-256) moves its two neighbour coefficients into the diagonal.
 template <typename T>
-__device__ void lusgs_column(const LusgsSys<T>& S, T* W, int64_t wj, int64_t wk, int i, int j, bool track,
-                             unsigned long long& chg)
+__global__ __launch_bounds__(256) void lusgs_factor_kernel(const LusgsBatch<T> B)
 {
-    const int ii = wrap_idx(i, S.nx, S.Lx), jj = wrap_idx(j, S.ny, S.Ly);
-    const T* a = S.A + ii + int64_t(jj) * S.a_j;
-    const T* pap = S.ap + ii + int64_t(jj) * S.p_j;
-    const T* pr = S.r + ii + int64_t(jj) * S.r_j;
-    T* col = W + i + int64_t(j) * wj;
-    const int64_t cb = int64_t(i - S.s_lo[0]) + int64_t(j - S.s_lo[1]) * S.c_j;
-    T* cc = S.c + cb;
-    T* ee = S.e + cb;
-    const int nz = S.nz;
-    const bool owned = i >= S.o_lo[0] && i <= S.o_hi[0] && j >= S.o_lo[1] && j <= S.o_hi[1];
-    T cprev = T(0), eprev = T(0);
-    for (int k = 0; k < nz; ++k) {
-        const T* ak = a + k * S.a_k;
-        T axm = ak[1 * S.a_n], axp = ak[2 * S.a_n], aym = ak[3 * S.a_n], ayp = ak[4 * S.a_n];
-        const T azm = ak[5 * S.a_n], azp = ak[6 * S.a_n];
-        T dg = pap[k * S.p_k];
-        // The fold of a periodic one-cell axis (Source/LineSweeps.cpp:242-256).
-        if (S.fold[0]) { dg -= axm + axp; axm = T(0); axp = T(0); }
-        if (S.fold[1]) { dg -= aym + ayp; aym = T(0); ayp = T(0); }
-        const T* wk0 = col + k * wk;
-        const T rk = pr[k * S.r_k] + axm * wk0[-1] + axp * wk0[1] + aym * wk0[-wj] + ayp * wk0[wj];
-        const T lower = -azm, upper = -azp;
-        T ck, ek;
-        if (k == 0) {
-            const T m = dg;
-            ck = upper / m;
-            ek = rk / m;
-        } else {
-            const T m = dg - lower * cprev;
-            ck = upper / m;
-            ek = (rk - lower * eprev) / m;
+    const LusgsSys<T>& S = B.s[blockIdx.y];
+    const int sx = S.s_hi[0] - S.s_lo[0] + 1, sy = S.s_hi[1] - S.s_lo[1] + 1;
+    const int64_t ncol = int64_t(sx) * sy;
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < ncol;
+         t += int64_t(gridDim.x) * blockDim.x) {
+        const int i = S.s_lo[0] + int(t % sx), j = S.s_lo[1] + int(t / sx);
+        const int ii = wrap_idx(i, S.nx, S.Lx), jj = wrap_idx(j, S.ny, S.Ly);
+        const T* a = S.A + ii + int64_t(jj) * S.a_j;
+        const T* pap = S.ap + ii + int64_t(jj) * S.p_j;
+        T* cc = S.c + t;
+        T* mm = S.m + t;
+        T cprev = T(0);
+        for (int k = 0; k < S.nz; ++k) {
+            const T* ak = a + k * S.a_k;
+            T dg = pap[k * S.p_k];
+            if (S.fold[0]) { dg -= ak[1 * S.a_n] + ak[2 * S.a_n]; }
+            if (S.fold[1]) { dg -= ak[3 * S.a_n] + ak[4 * S.a_n]; }
+            const T lower = -ak[5 * S.a_n], upper = -ak[6 * S.a_n];
+            const T mk = k == 0 ? dg : dg - lower * cprev;
+            const T ck = upper / mk;
+            cc[k * S.c_k] = ck;
+            mm[k * S.c_k] = mk;
+            cprev = ck;
         }
-        cc[k * S.c_k] = ck;
-        ee[k * S.c_k] = ek;
-        cprev = ck;
-        eprev = ek;
-    }
-    T o = eprev;
-    {
-        T* p = col + (nz - 1) * wk;
-        if (track && owned) {
-            const unsigned long long b = dev::abs_bits(double(*p - o));
-            chg = b > chg ? b : chg;
-        }
-        *p = o;
-    }
-    for (int k = nz - 2; k >= 0; --k) {
-        o = ee[k * S.c_k] - cc[k * S.c_k] * o;
-        T* p = col + k * wk;
-        if (track && owned) {
-            const unsigned long long b = dev::abs_bits(double(*p - o));
-            chg = b > chg ? b : chg;
-        }
-        *p = o;
     }
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void lusgs_kernel(const LusgsSys<T>* __restrict__ table, int sweeps, int alternate,
+__global__ __launch_bounds__(256) void lusgs_kernel(const LusgsBatch<T> B, int sweeps, int alternate,
                                                     unsigned long long* __restrict__ changes)
 {
-    const LusgsSys<T> S = table[blockIdx.x];
+    const LusgsSys<T>& S = B.s[blockIdx.x];
     const int tid = threadIdx.x, nth = blockDim.x;
     T* W;
     int64_t wj, wk;
@@ -1448,6 +1444,7 @@ __global__ __launch_bounds__(256) void lusgs_kernel(const LusgsSys<T>* __restric
         }
     }
     __syncthreads();
+    const int nz = S.nz;
     const int sx = S.s_hi[0] - S.s_lo[0] + 1, sy = S.s_hi[1] - S.s_lo[1] + 1;
     const int last = sx + sy - 2;
     unsigned long long chg = 0;
@@ -1481,11 +1478,88 @@ __global__ __launch_bounds__(256) void lusgs_kernel(const LusgsSys<T>* __restric
                 const int p = f ? tt : last - tt;
                 const int amin = p - (sy - 1) > 0 ? p - (sy - 1) : 0;
                 const int amax = p < sx - 1 ? p : sx - 1;
-                for (int m = tid; m <= amax - amin; m += nth) {
-                    const int ia = amin + m;
+                const int ncd = amax - amin + 1;
+                // 1. the right-hand sides of the diagonal's columns
+                const int nq = ncd * nz;
+                for (int q = tid; q < nq; q += nth) {
+                    const int ia = amin + q % ncd, k = q / ncd;
                     const int i = S.s_lo[0] + ia;
                     const int j = jrev ? S.s_hi[1] - (p - ia) : S.s_lo[1] + p - ia;
-                    lusgs_column(S, W, wj, wk, i, j, track, chg);
+                    const int ii = wrap_idx(i, S.nx, S.Lx), jj = wrap_idx(j, S.ny, S.Ly);
+                    const T* ak = S.A + ii + int64_t(jj) * S.a_j + k * S.a_k;
+                    T axm = ak[1 * S.a_n], axp = ak[2 * S.a_n], aym = ak[3 * S.a_n], ayp = ak[4 * S.a_n];
+                    if (S.fold[0]) { axm = T(0); axp = T(0); }
+                    if (S.fold[1]) { aym = T(0); ayp = T(0); }
+                    const T* wk0 = W + i + int64_t(j) * wj + k * wk;
+                    const T rk = S.r[ii + int64_t(jj) * S.r_j + k * S.r_k] + axm * wk0[-1] + axp * wk0[1] +
+                                 aym * wk0[-wj] + ayp * wk0[wj];
+                    S.e[int64_t(i - S.s_lo[0]) + int64_t(j - S.s_lo[1]) * S.c_j + k * S.c_k] = rk;
+                }
+                __syncthreads();
+                // 2. the recurrences, one thread per column
+                for (int mi = tid; mi < ncd; mi += nth) {
+                    const int ia = amin + mi;
+                    const int i = S.s_lo[0] + ia;
+                    const int j = jrev ? S.s_hi[1] - (p - ia) : S.s_lo[1] + p - ia;
+                    const int ii = wrap_idx(i, S.nx, S.Lx), jj = wrap_idx(j, S.ny, S.Ly);
+                    const T* az = S.A + ii + int64_t(jj) * S.a_j + 5 * S.a_n;
+                    const int64_t cb = int64_t(i - S.s_lo[0]) + int64_t(j - S.s_lo[1]) * S.c_j;
+                    T* ee = S.e + cb;
+                    const T* cc = S.c + cb;
+                    const T* mm = S.m + cb;
+                    T* col = W + i + int64_t(j) * wj;
+                    const bool tr = track && i >= S.o_lo[0] && i <= S.o_hi[0] && j >= S.o_lo[1] && j <= S.o_hi[1];
+                    T eprev = T(0);
+                    for (int k0 = 0; k0 < nz; k0 += kLusgsChunk) {
+                        T rr[kLusgsChunk], ll[kLusgsChunk], dd[kLusgsChunk];
+#pragma unroll
+                        for (int u = 0; u < kLusgsChunk; ++u) {
+                            const int k = k0 + u < nz ? k0 + u : nz - 1;
+                            rr[u] = ee[k * S.c_k];
+                            ll[u] = -az[k * S.a_k];
+                            dd[u] = mm[k * S.c_k];
+                        }
+#pragma unroll
+                        for (int u = 0; u < kLusgsChunk; ++u) {
+                            const int k = k0 + u;
+                            if (k < nz) {
+                                const T ek = k == 0 ? rr[u] / dd[u] : (rr[u] - ll[u] * eprev) / dd[u];
+                                ee[k * S.c_k] = ek;
+                                eprev = ek;
+                            }
+                        }
+                    }
+                    T o = eprev;
+                    {
+                        T* pz = col + (nz - 1) * wk;
+                        if (tr) {
+                            const unsigned long long b = dev::abs_bits(double(*pz - o));
+                            chg = b > chg ? b : chg;
+                        }
+                        *pz = o;
+                    }
+                    for (int k0 = nz - 2; k0 >= 0; k0 -= kLusgsChunk) {
+                        T ev[kLusgsChunk], cv[kLusgsChunk], ov[kLusgsChunk];
+#pragma unroll
+                        for (int u = 0; u < kLusgsChunk; ++u) {
+                            const int k = k0 - u >= 0 ? k0 - u : 0;
+                            ev[u] = ee[k * S.c_k];
+                            cv[u] = cc[k * S.c_k];
+                            ov[u] = tr ? col[k * wk] : T(0);
+                        }
+#pragma unroll
+                        for (int u = 0; u < kLusgsChunk; ++u) {
+                            const int k = k0 - u;
+                            if (k >= 0) {
+                                o = ev[u] - cv[u] * o;
+                                if (tr) {
+                                    const unsigned long long b = dev::abs_bits(double(ov[u] - o));
+                                    chg = b > chg ? b : chg;
+                                }
+                                col[k * wk] = o;
+                            }
+                        }
+                    }
                 }
                 __syncthreads();
             }
@@ -1497,9 +1571,9 @@ __global__ __launch_bounds__(256) void lusgs_kernel(const LusgsSys<T>* __restric
     if ((tid & 63) == 0) { red[tid >> 6] = chg; }
     __syncthreads();
     if (tid == 0) {
-        unsigned long long m = 0;
-        for (int w = 0; w < (nth + 63) / 64; ++w) { m = red[w] > m ? red[w] : m; }
-        changes[blockIdx.x] = m;
+        unsigned long long mx = 0;
+        for (int w = 0; w < (nth + 63) / 64; ++w) { mx = red[w] > mx ? red[w] : mx; }
+        changes[blockIdx.x] = mx;
     }
     // The result into delta (zero ghosts) and the seam copies.
     if (S.zero_mode) {
@@ -1556,6 +1630,308 @@ __global__ __launch_bounds__(256) void lusgs_kernel(const LusgsSys<T>* __restric
     }
 }
 
+
+// The lagged, non-alternating sweeps (the default) on skewed copies. Point
+// (i, j, k), i in [-1, nx], j in [-1, ny], sits at
+// (((i + j + 2) sJ + j + 1) nz + k): one (i + j) diagonal is one contiguous
+// run (its columns consecutive in j, each column's levels consecutive), so
+// the diagonal's loads coalesce and touch few pages (in the caller's layout
+// each column of a diagonal is its own cache line and each level its own
+// page range); its four neighbours sit at -sJ nz, +sJ nz, -(sJ + 1) nz and
+// +(sJ + 1) nz. The arithmetic is the column Thomas solve's, in its order
+// (the same bits).
+enum { kSkW = 0, kSkAxm, kSkAxp, kSkAym, kSkAyp, kSkR, kSkLz, kSkM, kSkC, kSkN };
+
+template <typename T>
+__device__ __forceinline__ int64_t skew_at(const LusgsSys<T>& S, int i, int j, int k)
+{
+    return (int64_t(i + j + 2) * S.sJ + (j + 1)) * S.nz + k;
+}
+
+// The skewed copies of each system (blockIdx.y): the iterate zeroed, ghosts
+// included; per valid column its neighbour coefficients, rhs, lower
+// diagonal -a_zm and the Thomas factors m_k, c'_k.
+template <typename T>
+__global__ __launch_bounds__(256) void lusgs_skew_layout_kernel(const LusgsBatch<T> B)
+{
+    const LusgsSys<T>& S = B.s[blockIdx.y];
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    const int64_t t0 = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    const int64_t nw = int64_t(S.sP) * S.sJ * S.nz;
+    for (int64_t t = t0; t < nw; t += stride) { S.sk[kSkW][t] = T(0); }
+    const int64_t ncol = int64_t(S.nx) * S.ny;
+    for (int64_t t = t0; t < ncol; t += stride) {
+        const int i = int(t % S.nx), j = int(t / S.nx);
+        const T* a = S.A + i + int64_t(j) * S.a_j;
+        const T* pap = S.ap + i + int64_t(j) * S.p_j;
+        const T* pr = S.r + i + int64_t(j) * S.r_j;
+        int64_t x = skew_at(S, i, j, 0);
+        T cprev = T(0);
+        for (int k = 0; k < S.nz; ++k, ++x) {
+            const T* ak = a + k * S.a_k;
+            const T dg = pap[k * S.p_k];
+            const T lower = -ak[5 * S.a_n], upper = -ak[6 * S.a_n];
+            const T mk = k == 0 ? dg : dg - lower * cprev;
+            const T ck = upper / mk;
+            S.sk[kSkAxm][x] = ak[1 * S.a_n];
+            S.sk[kSkAxp][x] = ak[2 * S.a_n];
+            S.sk[kSkAym][x] = ak[3 * S.a_n];
+            S.sk[kSkAyp][x] = ak[4 * S.a_n];
+            S.sk[kSkR][x] = pr[k * S.r_k];
+            S.sk[kSkLz][x] = lower;
+            S.sk[kSkM][x] = mk;
+            S.sk[kSkC][x] = ck;
+            cprev = ck;
+        }
+    }
+}
+
+// NZB > 0: columns of at most NZB levels hold their operands in registers
+// (every diagonal fits the workgroup: at most 256 columns); else chunks of
+// kLusgsChunk levels. Dynamic LDS: min(nx, ny) * nz reals.
+template <typename T, int NZB>
+__global__ __launch_bounds__(256) void lusgs_skew_kernel(const LusgsBatch<T> B, int sweeps,
+                                                         unsigned long long* __restrict__ changes)
+{
+    // register path: the diagonal's right-hand sides, -a_zm, m_k, c'_k and
+    // (last pass) the old iterate, each [column][level] with an odd column
+    // stride cs (coalesced copies in, few bank conflicts out); chunked path:
+    // the right-hand sides, then e'_k, [level][column]
+    extern __shared__ unsigned char lds_raw[];
+    T* Es = reinterpret_cast<T*>(lds_raw);
+    const LusgsSys<T>& S = B.s[blockIdx.x];
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int nx = S.nx, ny = S.ny, nz = S.nz, J = S.sJ;
+    const int ldj = nx < ny ? nx : ny;   // the longest diagonal
+    const int cs = nz | 1;
+    const int64_t lda = int64_t(ldj) * cs;
+    T* Ls = Es + lda;
+    T* Ms = Ls + lda;
+    T* Cs = Ms + lda;
+    T* Os = Cs + lda;
+    const int64_t dst = int64_t(J) * nz;   // one diagonal
+    T* __restrict__ W = S.sk[kSkW];
+    const T* __restrict__ Axm = S.sk[kSkAxm];
+    const T* __restrict__ Axp = S.sk[kSkAxp];
+    const T* __restrict__ Aym = S.sk[kSkAym];
+    const T* __restrict__ Ayp = S.sk[kSkAyp];
+    const T* __restrict__ Rr = S.sk[kSkR];
+    const T* __restrict__ Lz = S.sk[kSkLz];
+    const T* __restrict__ Mm = S.sk[kSkM];
+    const T* __restrict__ Cc = S.sk[kSkC];
+    const Rsrc rL = rsrc_of(Lz), rM = rsrc_of(Mm), rC = rsrc_of(Cc), rW = rsrc_of(W);
+    const int last = nx + ny - 2;
+    unsigned long long chg = 0;
+    for (int s = 0; s < sweeps; ++s) {
+        for (int f = 1; f >= 0; --f) {
+            // FillBoundary: the periodic images into the iterate's ghosts
+            if (S.Lx) {
+                const int64_t n = int64_t(ny) * nz;
+                for (int64_t t = tid; t < n; t += nth) {
+                    const int j = int(t % ny), k = int(t / ny);
+                    W[skew_at(S, -1, j, k)] = W[skew_at(S, S.Lx - 1, j, k)];
+                    W[skew_at(S, nx, j, k)] = W[skew_at(S, nx - S.Lx, j, k)];
+                }
+            }
+            if (S.Ly) {
+                const int64_t n = int64_t(nx) * nz;
+                for (int64_t t = tid; t < n; t += nth) {
+                    const int i = int(t % nx), k = int(t / nx);
+                    W[skew_at(S, i, -1, k)] = W[skew_at(S, i, S.Ly - 1, k)];
+                    W[skew_at(S, i, ny, k)] = W[skew_at(S, i, ny - S.Ly, k)];
+                }
+            }
+            __syncthreads();
+            const bool track = (s == sweeps - 1) && f == 0;
+            for (int tt = 0; tt <= last; ++tt) {
+                const int p = f ? tt : last - tt;
+                const int jmin = p - (nx - 1) > 0 ? p - (nx - 1) : 0;
+                const int jmax = p < ny - 1 ? p : ny - 1;
+                const int ncd = jmax - jmin + 1;
+                const int64_t x0 = (int64_t(p + 2) * J + (jmin + 1)) * nz;
+                // b. the diagonal's right-hand sides into LDS (and, register
+                // path, its recurrence operands), each thread's share loaded
+                // before any of it is stored
+                const int nq = ncd * nz;
+                for (int q0 = 0; q0 < nq; q0 += nth * kLusgsRhsBatch) {
+                    T rv[kLusgsRhsBatch], lv[kLusgsRhsBatch], mv[kLusgsRhsBatch], cv2[kLusgsRhsBatch],
+                        wv[kLusgsRhsBatch];
+#pragma unroll
+                    for (int u = 0; u < kLusgsRhsBatch; ++u) {
+                        const int qu = q0 + tid + u * nth;
+                        const int q = qu < nq ? qu : nq - 1;
+                        const int64_t x = x0 + q;   // column q / nz, level q % nz
+                        rv[u] = Rr[x] + Axm[x] * W[x - dst] + Axp[x] * W[x + dst] + Aym[x] * W[x - dst - nz] +
+                                Ayp[x] * W[x + dst + nz];
+                        if (NZB > 0) {
+                            lv[u] = Lz[x];
+                            mv[u] = Mm[x];
+                            cv2[u] = Cc[x];
+                            wv[u] = track ? W[x] : T(0);
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < kLusgsRhsBatch; ++u) {
+                        const int q = q0 + tid + u * nth;
+                        if (q < nq) {
+                            if (NZB > 0) {
+                                const int at = (q / nz) * cs + q % nz;
+                                Es[at] = rv[u];
+                                Ls[at] = lv[u];
+                                Ms[at] = mv[u];
+                                Cs[at] = cv2[u];
+                                Os[at] = wv[u];
+                            } else {
+                                Es[(q % nz) * ldj + q / nz] = rv[u];
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+                // c. the recurrences, one thread per column
+                if (NZB > 0) {
+                    if (tid < ncd) {
+                        constexpr int NB = NZB > 0 ? NZB : 1;
+                        const uint32_t vo = uint32_t(x0 + int64_t(tid) * nz) * uint32_t(sizeof(T));
+                        const int cb = tid * cs;
+                        T rr[NB], ll[NB], dd[NB], cv[NB], ov[NB];
+#pragma unroll
+                        for (int u = 0; u < NB; ++u) {
+                            if (u < nz) {
+                                rr[u] = Es[cb + u];
+                                ll[u] = Ls[cb + u];
+                                dd[u] = Ms[cb + u];
+                                cv[u] = Cs[cb + u];
+                                ov[u] = Os[cb + u];
+                            }
+                        }
+                        T ep = T(0);
+#pragma unroll
+                        for (int u = 0; u < NB; ++u) {
+                            if (u < nz) {
+                                const T ek = u == 0 ? rr[u] / dd[u] : (rr[u] - ll[u] * ep) / dd[u];
+                                rr[u] = ek;
+                                ep = ek;
+                            }
+                        }
+                        T o = ep;
+#pragma unroll
+                        for (int u = NB - 1; u >= 0; --u) {
+                            if (u < nz) {
+                                if (u < nz - 1) { o = rr[u] - cv[u] * o; }
+                                if (track) {
+                                    const unsigned long long b = dev::abs_bits(double(ov[u] - o));
+                                    chg = b > chg ? b : chg;
+                                }
+                                bstore<T>(o, rW, vo, uint32_t(u) * uint32_t(sizeof(T)));
+                            }
+                        }
+                    }
+                } else {
+                    for (int m = tid; m < ncd; m += nth) {
+                        const int64_t xc = x0 + int64_t(m) * nz;
+                        T eprev = T(0);
+                        for (int k0 = 0; k0 < nz; k0 += kLusgsChunk) {
+                            T rr[kLusgsChunk], l2[kLusgsChunk], d2[kLusgsChunk];
+#pragma unroll
+                            for (int u = 0; u < kLusgsChunk; ++u) {
+                                if (k0 + u < nz) {
+                                    const int64_t x = xc + (k0 + u);
+                                    rr[u] = Es[(k0 + u) * ldj + m];
+                                    l2[u] = Lz[x];
+                                    d2[u] = Mm[x];
+                                }
+                            }
+#pragma unroll
+                            for (int u = 0; u < kLusgsChunk; ++u) {
+                                const int k = k0 + u;
+                                if (k < nz) {
+                                    const T ek = k == 0 ? rr[u] / d2[u] : (rr[u] - l2[u] * eprev) / d2[u];
+                                    Es[k * ldj + m] = ek;
+                                    eprev = ek;
+                                }
+                            }
+                        }
+                        T o = eprev;
+                        {
+                            const int64_t x = xc + (nz - 1);
+                            if (track) {
+                                const unsigned long long b = dev::abs_bits(double(W[x] - o));
+                                chg = b > chg ? b : chg;
+                            }
+                            W[x] = o;
+                        }
+                        for (int k0 = nz - 2; k0 >= 0; k0 -= kLusgsChunk) {
+                            T ev[kLusgsChunk], c2[kLusgsChunk], o2[kLusgsChunk];
+#pragma unroll
+                            for (int u = 0; u < kLusgsChunk; ++u) {
+                                if (k0 - u >= 0) {
+                                    const int64_t x = xc + (k0 - u);
+                                    ev[u] = Es[(k0 - u) * ldj + m];
+                                    c2[u] = Cc[x];
+                                    o2[u] = track ? W[x] : T(0);
+                                }
+                            }
+#pragma unroll
+                            for (int u = 0; u < kLusgsChunk; ++u) {
+                                const int k = k0 - u;
+                                if (k >= 0) {
+                                    o = ev[u] - c2[u] * o;
+                                    if (track) {
+                                        const unsigned long long b = dev::abs_bits(double(o2[u] - o));
+                                        chg = b > chg ? b : chg;
+                                    }
+                                    W[xc + k] = o;
+                                }
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+    // max |change| over the block.
+    __shared__ unsigned long long red[256 / 64];
+    chg = dev::wave_max_u64(chg);
+    if ((tid & 63) == 0) { red[tid >> 6] = chg; }
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long mx = 0;
+        for (int w = 0; w < (nth + 63) / 64; ++w) { mx = red[w] > mx ? red[w] : mx; }
+        changes[blockIdx.x] = mx;
+    }
+    // The iterate into delta (whose ghosts were zeroed before the launch and
+    // stay zero), then the seam copies (the low copy owns).
+    {
+        const int64_t n = int64_t(nx) * ny * nz;
+        for (int64_t t = tid; t < n; t += nth) {
+            const int i = int(t % nx);
+            const int64_t rr = t / nx;
+            const int j = int(rr % ny), k = int(rr / ny);
+            S.d[i + j * S.d_j + k * S.d_k] = W[skew_at(S, i, j, k)];
+        }
+    }
+    __syncthreads();
+    for (int d = 0; d < 2; ++d) {
+        if (!((S.loc >> d) & 1) || !(d == 0 ? S.Lx : S.Ly)) { continue; }
+        const int64_t n = d == 0 ? int64_t(ny) * nz : int64_t(nx) * nz;
+        for (int64_t t = tid; t < n; t += nth) {
+            int64_t lo, hi;
+            if (d == 0) {
+                const int j = int(t % ny), k = int(t / ny);
+                lo = j * S.d_j + k * S.d_k;
+                hi = lo + (nx - 1);
+            } else {
+                const int i = int(t % nx), k = int(t / nx);
+                lo = i + k * S.d_k;
+                hi = lo + (ny - 1) * S.d_j;
+            }
+            S.d[hi] = S.d[lo];
+        }
+    }
+}
+
 int check_line_options_impl(const gust_line_options* opt)
 {
     if (!opt) { GUST_FAIL("gust: zline_lusgs: options are null"); }
@@ -1580,13 +1956,15 @@ int check_line_options_impl(const gust_line_options* opt)
     return 0;
 }
 
+// One system of a batch: checked, its views at the valid lo corner, its
+// sweep and owned regions; *scratch_bytes: its share of the workspace (c',
+// e', m over the sweep region and, zero ghosts, the work array).
 template <typename T>
-void lusgs_run(int loc, const gust_array4* A, const gust_array4* ap, const gust_array4* rhs, int sweeps,
-               const gust_geometry* g, gust_array4* delta, const gust_line_options* opt, double* change,
-               hipStream_t st)
+LusgsSys<T> lusgs_system(int loc, const gust_array4* A, const gust_array4* ap, const gust_array4* rhs, int sweeps,
+                         const gust_geometry* g, gust_array4* delta, const gust_line_options* opt,
+                         size_t* scratch_bytes, IBox& V)
 {
     const char* op = "zline_lusgs";
-    IBox V;
     check_line_system(op, loc, A, ap, rhs, sweeps, g, delta, V);
     check_line_options_impl(opt);
     // check_line_overlap_layout (Source/LineSweeps.cpp:343-359), one box.
@@ -1643,7 +2021,7 @@ void lusgs_run(int loc, const gust_array4* A, const gust_array4* ap, const gust_
     const int sx = shi[0] - slo[0] + 1, sy = shi[1] - slo[1] + 1;
     t.c_j = sx;
     t.c_k = int64_t(sx) * sy;
-    const size_t cbytes = size_t(sx) * sy * t.nz * sizeof(T);
+    const size_t cbytes = (size_t(sx) * sy * t.nz * sizeof(T) + 255) / 256 * 256;
     size_t wbytes = 0;
     if (zero) {
         t.w_gx = ngx + 1;
@@ -1651,33 +2029,161 @@ void lusgs_run(int loc, const gust_array4* A, const gust_array4* ap, const gust_
         const int64_t tx = t.nx + 2 * t.w_gx, ty = t.ny + 2 * t.w_gy;
         t.w_j = tx;
         t.w_k = tx * ty;
-        wbytes = size_t(tx * ty * t.nz) * sizeof(T);
+        wbytes = (size_t(tx * ty * t.nz) * sizeof(T) + 255) / 256 * 256;
     }
-    char* ws = static_cast<char*>(Workspace::get(st, 3, 2 * cbytes + wbytes + 1024));
+    *scratch_bytes = 3 * cbytes + wbytes;
+    return t;
+}
+
+// Points a system's scratch pointers into ws (its share, as lusgs_system
+// sized it).
+template <typename T>
+void lusgs_place(LusgsSys<T>& t, char* ws)
+{
+    const int sx = int(t.c_j), sy = int(t.c_k / t.c_j);
+    const size_t cbytes = (size_t(sx) * sy * t.nz * sizeof(T) + 255) / 256 * 256;
     t.c = reinterpret_cast<T*>(ws);
     t.e = reinterpret_cast<T*>(ws + cbytes);
-    if (zero) { t.w = reinterpret_cast<T*>(ws + 2 * cbytes) + t.w_gx + t.w_gy * t.w_j; }
-    auto* chg = static_cast<unsigned long long*>(Workspace::get(st, 1, 16));
-    auto* dtab = static_cast<LusgsSys<T>*>(Workspace::get(st, 4, sizeof(LusgsSys<T>)));
-    GUST_HIP(hipMemcpyAsync(dtab, &t, sizeof t, hipMemcpyHostToDevice, st));
-    zero_ghosts<T>(*delta, V, st);
-    lusgs_kernel<T><<<1, 256, 0, st>>>(dtab, sweeps, opt->alternate, chg);
+    t.m = reinterpret_cast<T*>(ws + 2 * cbytes);
+    if (t.zero_mode) { t.w = reinterpret_cast<T*>(ws + 3 * cbytes) + t.w_gx + t.w_gy * t.w_j; }
+}
+
+// LU-SGS sweeps of a batch of systems (one workgroup each, one launch);
+// each system's last-sweep change as the ordered bits of |change| into
+// changes[s]. The systems go to the kernels by value: nothing waits.
+template <typename T>
+void lusgs_batch(int nsys, const gust_line_system* sys, const gust_line_options* opt, int sweeps,
+                 unsigned long long* changes, hipStream_t st)
+{
+    if (nsys < 1 || nsys > kLusgsMaxBatch) {
+        GUST_FAIL("gust: zline_lusgs: a batch holds 1 to %d systems (got %d)", kLusgsMaxBatch, nsys);
+    }
+    LusgsBatch<T> B{};
+    size_t bytes[kLusgsMaxBatch] = {}, total = 0;
+    IBox V[kLusgsMaxBatch];
+    gust_array4 dv[kLusgsMaxBatch];
+    int64_t most = 1;
+    for (int s = 0; s < nsys; ++s) {
+        dv[s] = sys[s].delta;
+        B.s[s] = lusgs_system<T>(sys[s].loc, &sys[s].A, &sys[s].ap, &sys[s].rhs, sweeps, &sys[s].geom, &dv[s], opt,
+                                 &bytes[s], V[s]);
+        total += bytes[s];
+        most = std::max<int64_t>(most, B.s[s].c_k);
+    }
+    // lagged ghosts without alternation: the skewed copies
+    bool skew = opt->ghosts == GUST_GHOSTS_LAGGED && opt->alternate == 0;
+    for (int s = 0; s < nsys && skew; ++s) {
+        const LusgsSys<T>& t = B.s[s];
+        // the buffer offsets are 32-bit; the diagonal's right-hand sides sit
+        // in LDS
+        skew = double(t.nx + t.ny + 3) * double(t.ny + 2) * double(t.nz) * sizeof(T) < 2.0e9 &&
+               size_t(std::min(t.nx, t.ny)) * t.nz * sizeof(T) <= kLusgsMaxLds;
+    }
+    if (skew) {
+        total = 0;
+        for (int s = 0; s < nsys; ++s) {
+            LusgsSys<T>& t = B.s[s];
+            t.sP = t.nx + t.ny + 3;
+            t.sJ = t.ny + 2;
+            bytes[s] = kSkN * ((size_t(t.sP) * t.sJ * t.nz * sizeof(T) + 255) / 256 * 256);
+            total += bytes[s];
+            most = std::max<int64_t>(most, int64_t(t.sP) * t.sJ * t.nz);
+        }
+    }
+    char* ws = static_cast<char*>(Workspace::get(st, 3, total + 1024));
+    for (int s = 0; s < nsys; ++s) {
+        if (skew) {
+            LusgsSys<T>& t = B.s[s];
+            const size_t one = bytes[s] / kSkN;
+            for (int a = 0; a < kSkN; ++a) { t.sk[a] = reinterpret_cast<T*>(ws + a * one); }
+        } else {
+            lusgs_place(B.s[s], ws);
+        }
+        ws += bytes[s];
+        zero_ghosts<T>(dv[s], V[s], st);
+    }
+    if (skew) {
+        int nzmax = 0, diag = 0;
+        size_t lds = 0;
+        size_t lds_regs = 0;
+        for (int s = 0; s < nsys; ++s) {
+            const LusgsSys<T>& t = B.s[s];
+            nzmax = std::max(nzmax, t.nz);
+            diag = std::max(diag, std::min(t.nx, t.ny));
+            lds = std::max(lds, size_t(std::min(t.nx, t.ny)) * t.nz * sizeof(T));
+            lds_regs = std::max(lds_regs, 5 * size_t(std::min(t.nx, t.ny)) * (t.nz | 1) * sizeof(T));
+        }
+        const bool regs = diag <= 256 && nzmax <= 32 && lds_regs <= kLusgsMaxLds;
+        if (regs) { lds = lds_regs; }
+        if (lds > 64 * 1024) {
+            // above the default dynamic LDS limit (gfx950 has 160 KiB per CU)
+            for (const void* fn : {reinterpret_cast<const void*>(&lusgs_skew_kernel<T, 16>),
+                                   reinterpret_cast<const void*>(&lusgs_skew_kernel<T, 32>),
+                                   reinterpret_cast<const void*>(&lusgs_skew_kernel<T, 0>)}) {
+                GUST_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(kLusgsMaxLds)));
+            }
+        }
+        const unsigned lx = unsigned(std::min<int64_t>((most + 255) / 256, 2048));
+        lusgs_skew_layout_kernel<T><<<dim3(lx, unsigned(nsys)), 256, 0, st>>>(B);
+        GUST_HIP(hipGetLastError());
+        if (regs && nzmax <= 16) {
+            lusgs_skew_kernel<T, 16><<<unsigned(nsys), 256, lds, st>>>(B, sweeps, changes);
+        } else if (regs) {
+            lusgs_skew_kernel<T, 32><<<unsigned(nsys), 256, lds, st>>>(B, sweeps, changes);
+        } else {
+            lusgs_skew_kernel<T, 0><<<unsigned(nsys), 256, lds, st>>>(B, sweeps, changes);
+        }
+        GUST_HIP(hipGetLastError());
+        return;
+    }
+    const unsigned gx = unsigned(std::min<int64_t>((most + 255) / 256, 1024));
+    lusgs_factor_kernel<T><<<dim3(gx, unsigned(nsys)), 256, 0, st>>>(B);
     GUST_HIP(hipGetLastError());
+    lusgs_kernel<T><<<unsigned(nsys), 256, 0, st>>>(B, sweeps, opt->alternate, changes);
+    GUST_HIP(hipGetLastError());
+}
+
+// One system: its change recorded in the stream's deferred batch when one is
+// open (read with its other values), else read back here.
+template <typename T>
+void lusgs_run(int loc, const gust_array4* A, const gust_array4* ap, const gust_array4* rhs, int sweeps,
+               const gust_geometry* g, gust_array4* delta, const gust_line_options* opt, double* change,
+               hipStream_t st)
+{
+    {
+        // every refusal before the first device call
+        size_t bytes = 0;
+        IBox V;
+        gust_array4 dv = *delta;
+        (void)lusgs_system<T>(loc, A, ap, rhs, sweeps, g, &dv, opt, &bytes, V);
+    }
+    gust_line_system sy1{loc, *g, *A, *ap, *rhs, *delta};
+    Deferred* dq = change && Deferred::current() && Deferred::current()->stream() == st ? Deferred::current()
+                                                                                          : nullptr;
+    if (dq) {
+        unsigned long long* slot = dq->zero_slot([change](unsigned long long b) { *change = bits_to_double(b); });
+        lusgs_batch<T>(1, &sy1, opt, sweeps, slot, st);
+        return;
+    }
+    auto* chg = static_cast<unsigned long long*>(Workspace::get(st, 1, 16));
+    lusgs_batch<T>(1, &sy1, opt, sweeps, chg, st);
     if (change) {
         auto* h = static_cast<unsigned long long*>(Workspace::get_host_pinned(0, 8));
         GUST_HIP(hipMemcpyAsync(h, chg, 8, hipMemcpyDeviceToHost, st));
         GUST_HIP(hipStreamSynchronize(st));
-        unsigned long long b = h[0];
-        double v;
-        std::memcpy(&v, &b, 8);
-        *change = v;
-    } else {
-        // The table copy source lives on this stack frame.
-        GUST_HIP(hipStreamSynchronize(st));
+        *change = bits_to_double(h[0]);
     }
 }
 
 }   // namespace
+
+// A batch of FP64 LU-SGS systems (up to three) in one launch, their
+// last-sweep changes into `changes` as ordered bits of |change|.
+void zline_lusgs_into(int nsys, const gust_line_system* sys, const gust_line_options* opt, int sweeps,
+                      unsigned long long* changes, hipStream_t st)
+{
+    lusgs_batch<double>(nsys, sys, opt, sweeps, changes, st);
+}
 
 // A batch of z-line Jacobi systems whose last-sweep changes go straight
 // into `changes` (nsys slots that hold zeros, as ordered bits of |change|).

@@ -728,13 +728,11 @@ void scalar_sweeps_theta_tke(const gust_geometry& g, gust_array4* theta, const g
                              const gust_line_options& sweep, gust_scalar_sweep_result* res_theta,
                              gust_scalar_sweep_result* res_tke, hipStream_t st, int sweep_precision)
 {
-    void* stream = st;
-    if (sweep.kind != GUST_SWEEP_ZJACOBI) {
-        check_rc(gust_scalar_sweep(&g, GUST_EQ_THETA, theta, R_theta, A_theta, dtau, sweeps, 0.0, 0, &sweep, res_theta,
-                                   stream));
-        check_rc(gust_scalar_sweep(&g, GUST_EQ_TKE, tke, R_tke, A_tke, dtau, sweeps, floor, 1, &sweep, res_tke, stream));
-        return;
-    }
+    check_rc(gust_check_line_options(&sweep));
+    // LU-SGS runs in FP64 whatever the sweep precision; both kinds solve
+    // theta and k in shared launches (their systems come from R, A and dtau)
+    const bool lusgs = sweep.kind == GUST_SWEEP_LUSGS;
+    if (lusgs) { sweep_precision = 0; }
     const char* op = "scalar_sweep";
     check_field(op, "field", theta, g, GUST_LOC_CELL, 1);
     check_field(op, "field", tke, g, GUST_LOC_CELL, 1);
@@ -777,7 +775,9 @@ void scalar_sweeps_theta_tke(const gust_geometry& g, gust_array4* theta, const g
         });
     }
     gust_line_system s32[2];
-    if (sweep_precision == 1) {
+    if (lusgs) {
+        zline_lusgs_into(2, sys, &sweep, sweeps, dq->zero_run(std::move(fs)), st);
+    } else if (sweep_precision == 1) {
         systems_f32(2, sys, s32, st);
         zline_jacobi_into_f32(2, s32, sweeps, dq->zero_run(std::move(fs)), st);
     } else {
@@ -1007,11 +1007,13 @@ int gust_momentum_predictor(const gust_geometry* geom, gust_faces* vel, const gu
             last[2] = &result->last_sweep_dw;
         }
         const double dxi0 = 1.0 / g.dx[0], dxi1 = 1.0 / g.dx[1];
-        // The three z-line Jacobi solves share one launch per sweep: their
-        // systems (a_P, d, rhs) come from R, A and dtau only, never from the
-        // updated velocities, so solving them together gives the bits of the
-        // reference's one-after-the-other loop.
-        const bool batched = opt->sweep.kind == GUST_SWEEP_ZJACOBI;
+        // The three line solves share launches (z-line Jacobi: one per
+        // sweep; LU-SGS: one for all sweeps): their systems (a_P, d, rhs)
+        // come from R, A and dtau only, never from the updated velocities,
+        // so solving them together gives the bits of the reference's
+        // one-after-the-other loop.
+        const bool batched = true;
+        const bool lusgs = opt->sweep.kind == GUST_SWEEP_LUSGS;
         const int rslot[3] = {44, 55, 57}, dslot[3] = {45, 56, 58};
         gust_line_system sys[3];
         for (int c = 0; c < 3; ++c) {
@@ -1121,8 +1123,11 @@ int gust_momentum_predictor(const gust_geometry* geom, gust_faces* vel, const gu
                 });
             }
             gust_line_system s32[3];
-            const bool f32 = opt->sweep_precision == 1;
-            if (f32) {
+            // LU-SGS runs in FP64 whatever the sweep precision
+            const bool f32 = opt->sweep_precision == 1 && !lusgs;
+            if (lusgs) {
+                zline_lusgs_into(3, sys, &opt->sweep, opt->sweeps, dq->zero_run(std::move(fs)), st);
+            } else if (f32) {
                 systems_f32(3, sys, s32, st);
                 zline_jacobi_into_f32(3, s32, opt->sweeps, dq->zero_run(std::move(fs)), st);
             } else {

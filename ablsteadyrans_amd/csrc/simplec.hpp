@@ -22,5 +22,10 @@ void zline_jacobi_into(int nsys, const gust_line_system* sys, int sweeps, unsign
 // The same on FP32 systems (the views' pointers are float data).
 void zline_jacobi_into_f32(int nsys, const gust_line_system* sys, int sweeps, unsigned long long* changes,
                            hipStream_t st);
+// FP64 line LU-SGS of up to three systems in one launch (one workgroup
+// each), with the options' ghosts, overlap and alternation; the changes as
+// zline_jacobi_into's.
+void zline_lusgs_into(int nsys, const gust_line_system* sys, const gust_line_options* opt, int sweeps,
+                      unsigned long long* changes, hipStream_t st);
 
 }   // namespace gust_amd

@@ -12,7 +12,7 @@ import ctypes as C
 from dataclasses import dataclass, field
 
 from . import _lib
-from .fab import CELL, XFACE, YFACE, ZFACE, Fab, Geometry
+from .fab import CELL, XFACE, YFACE, ZFACE, Fab, Geometry, LineSweepOptions
 from .poisson import DeviceMesh, _stream
 
 PERIODIC, INFLOW, OUTFLOW, SLIP_WALL, NO_SLIP_WALL, SURFACE_LAYER, GEOSTROPHIC = range(7)
@@ -310,6 +310,7 @@ class SimplecOptions:
     poisson_coarse_sweeps: int = 16
     poisson_max_levels: int = 0
     sweep_precision: int = 0
+    sweep: LineSweepOptions | None = None   # steady.sweep and steady.line_* (None: z-line Jacobi)
 
     @property
     def mixed(self) -> bool:
@@ -327,6 +328,8 @@ class SimplecOptions:
         o.poisson.coarse_sweeps = int(self.poisson_coarse_sweeps)
         o.poisson.max_levels = int(self.poisson_max_levels)
         o.sweep_precision = int(self.sweep_precision)
+        if self.sweep is not None:
+            o.sweep = self.sweep.c()
         return o
 
 

@@ -130,6 +130,25 @@ def test_lusgs_is_bit_identical_to_the_oracle(cuda, n, periodic, loc):
                 assert ch == ref_ch
 
 
+@pytest.mark.parametrize("n,periodic", [((20, 12, 24), (1, 1)), ((9, 7, 40), (0, 1)), ((300, 260, 3), (1, 0))])
+@pytest.mark.parametrize("loc", LOCS)
+def test_lusgs_deep_and_wide_tiles_are_bit_identical(cuda, n, periodic, loc):
+    """Lagged ghosts without alternation on columns of 17-32 levels (held in
+    registers), of more than 32 (in chunks) and on diagonals of more than 256
+    columns: the oracle's bits and change."""
+    import torch
+    import ablsteadyrans_amd as G
+    A, ap, rhs = L.random_system(n, loc, periodic, seed=300 + loc)
+    geom = G.Geometry.of_size(*n, periodic=periodic)
+    dA, dap, drhs = _dev(A, (0, 0, 0)), _dev(ap, (0, 0, 0)), _dev(rhs, (0, 0, 0))
+    for sweeps in (1, 2):
+        ref, ref_ch, _ = L.oracle_lusgs(n, loc, periodic, A, ap, rhs, sweeps)
+        d = G.Fab.at_loc(geom, loc, 1, 1, dtype=torch.float64)
+        ch = G.zline_lusgs(loc, dA, dap, drhs, sweeps, geom, d, G.LineSweepOptions(kind=1))
+        assert np.array_equal(d.numpy()[0, 1:-1, 1:-1, 1:-1], ref), (n, loc, sweeps)
+        assert ch == ref_ch
+
+
 def test_line_sweeps_dispatcher_routes_the_options(cuda):
     import torch
     import ablsteadyrans_amd as G
