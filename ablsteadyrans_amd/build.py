@@ -73,8 +73,10 @@ def build_hip(verbose: bool = False) -> str:
 def build_oracle(verbose: bool = False) -> str:
     srcs = sorted(glob.glob(os.path.join(ORACLE_DIR, "*.c")))
     tmp = ORACLE_LIB + ".tmp"
-    cmd = ["gcc", "-O3", "-std=c11", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared",
-           "-Wall", "-Wno-unused-function", *srcs, "-o", tmp, "-lm"]
+    # x86-64-v3 (AVX2): the build host and the GPU hosts both have it; the
+    # separate roundings (-ffp-contract=off) keep every result IEEE's
+    cmd = ["gcc", "-O3", "-std=c11", "-march=x86-64-v3", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC",
+           "-shared", "-Wall", "-Wno-unused-function", *srcs, "-o", tmp, "-lm"]
     if verbose:
         print(" ".join(cmd))
     _run(cmd)

@@ -25,6 +25,7 @@ from . import mesh as M
 from . import physics as GP
 from . import poisson as PS
 from . import reductions as RD
+from .fab import LUSGS, LineSweepOptions
 
 
 @dataclass
@@ -71,6 +72,23 @@ def neutral_column() -> Deck:
                 GI.InitOptions(first_guess=1, projection=False), GP.SimplecOptions(sweeps=2, height=1000.0),
                 GP.DriverOptions(max_iterations=20000, cfl0=1.0, cfl_max=50.0, residual_tol=1.0e-8,
                                  residual_drop_orders=6.0), 2231)
+
+
+def neutral_hill_3d_lusgs() -> Deck:
+    """Neutral_Hill_3D with the reference's recommended CPU line solver
+    (steady.sweep = lusgs, lagged ghosts, 2 sweeps, CFL cap 16;
+    docs/results.rst:10-23, 253-303): the reference measured 823 iterations
+    in 51 s on four ranks (four boxes) and 860 on one box
+    (docs/results.rst:357-380)."""
+    d = neutral_hill_3d()
+    d.name = "Neutral_Hill_3D (LU-SGS)"
+    d.stage.sweeps = 2
+    d.stage.sweep = LineSweepOptions(kind=LUSGS)
+    d.driver.cfl_max = 16.0
+    d.iterations_reference = 860
+    d.seconds_reference = 51.0
+    d.ranks_reference = 4
+    return d
 
 
 def neutral_abl_flat() -> Deck:

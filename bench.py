@@ -131,19 +131,24 @@ def _time_run(plan, sweeps, reps):
     return e0.elapsed_time(e1) / reps
 
 
-def _cpu_baseline(target_s=12.0):
-    """The oracle's zline_jacobi (2 sweeps per call, one thread) over the five
-    variables of a 128 x 128 x 96 sample; reported per sweep in the metric's
-    unit (algorithmic GB/s)."""
+def _cpu_threads():
+    """The host threads the CPU baselines use: OMP_NUM_THREADS when set (the
+    GPU hosts set their share), else the logical CPUs."""
+    try:
+        return max(1, int(os.environ.get("OMP_NUM_THREADS", "")))
+    except ValueError:
+        return max(1, os.cpu_count() or 1)
+
+
+def _sweep_sample(nx, ny, rng):
+    """The five variables' z-line systems on an nx x ny x NZ tile (diagonally
+    dominant random rows), host arrays for the oracle."""
     import numpy as np
     from oracle import oracle as orc
-    nx, ny, nz = 128, 128, NZ
-    g = orc.geom_c((0, 0, 0), (nx - 1, ny - 1, nz - 1), (0, 0, 0))
-    rng = np.random.default_rng(20264203)
-    data = []
-    unknowns = 0
+    g = orc.geom_c((0, 0, 0), (nx - 1, ny - 1, NZ - 1), (0, 0, 0))
+    data, unknowns = [], 0
     for _name, loc in VARS:
-        sx, sy, sz = nx + (loc & 1), ny + ((loc >> 1) & 1), nz + ((loc >> 2) & 1)
+        sx, sy, sz = nx + (loc & 1), ny + ((loc >> 1) & 1), NZ + ((loc >> 2) & 1)
         A = np.zeros((7, sz, sy, sx))
         nb = rng.uniform(0.1, 1.0, size=(6, sz, sy, sx))
         nb[0:2] *= 3.0
@@ -154,21 +159,48 @@ def _cpu_baseline(target_s=12.0):
         d = np.zeros((1, sz + 2, sy + 2, sx + 2))
         data.append((loc, A, ap, rhs, d))
         unknowns += sx * sy * sz
+    return g, data, unknowns
+
+
+def _time_sweep_sample(g, data, unknowns, threads, target_s):
+    from oracle import oracle as orc
     calls = 0
     t0 = time.perf_counter()
     while True:
         for loc, A, ap, rhs, d in data:
-            orc.zline_jacobi(loc, A, (0, 0, 0), ap, rhs, (0, 0, 0), 2, g, d, (-1, -1, -1))
+            orc.zline_jacobi(loc, A, (0, 0, 0), ap, rhs, (0, 0, 0), 2, g, d, (-1, -1, -1), threads=threads)
         calls += 1
         if time.perf_counter() - t0 >= target_s:
             break
     dt = time.perf_counter() - t0
     per_sweep = dt / (2 * calls)
-    gbs = REALS_PER_UNKNOWN * 8 * unknowns / per_sweep / 1e9
-    return {"value": round(gbs, 3), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"oracle zline_jacobi FP64, 5 variables on 128x128x96 (same nz, {unknowns} unknowns per sweep), "
-                      f"{2 * calls} sweeps in {dt:.1f} s = {per_sweep * 1e3:.1f} ms per sweep, 1 thread",
-            "ms_per_sweep": round(per_sweep * 1e3, 3)}
+    return REALS_PER_UNKNOWN * 8 * unknowns / per_sweep / 1e9, per_sweep, 2 * calls, dt
+
+
+def _cpu_baseline(target_s=8.0):
+    """The oracle's z-line Jacobi (compiled C, -O3 -march=x86-64-v3, two
+    sweeps per call; the Python loop only issues one call per variable) on
+    the host: one thread on a 128 x 128 x 96 sample, and the host's threads
+    (OpenMP over columns) on the fractal tile's own 256 x 256 x 96 systems;
+    reported per sweep in the metric's unit (algorithmic GB/s)."""
+    import numpy as np
+    rng = np.random.default_rng(20264203)
+    g1, d1, u1 = _sweep_sample(128, 128, rng)
+    gbs1, ps1, n1, dt1 = _time_sweep_sample(g1, d1, u1, 1, target_s)
+    del d1
+    threads = _cpu_threads()
+    gN, dN, uN = _sweep_sample(NX, NY, rng)
+    gbsN, psN, nN, dtN = _time_sweep_sample(gN, dN, uN, threads, target_s)
+    del dN
+    model, ncpu = _cpu_info()
+    return {"value": round(gbsN, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "logical_cpus": ncpu,
+            "sample": f"oracle zline_jacobi FP64 (compiled C, OpenMP over columns), 5 variables on {NX}x{NY}x{NZ} "
+                      f"({uN} unknowns per sweep), {nN} sweeps in {dtN:.1f} s = {psN * 1e3:.1f} ms per sweep on "
+                      f"{threads} threads; one thread on 128x128x{NZ} ({u1} unknowns): {n1} sweeps in {dt1:.1f} s = "
+                      f"{ps1 * 1e3:.1f} ms per sweep, {gbs1:.3f} GB/s",
+            "ms_per_sweep": round(psN * 1e3, 3), "single_thread_gbs": round(gbs1, 3),
+            "single_thread_ms_per_sweep_128x128": round(ps1 * 1e3, 3)}
 
 
 def _fractal_tile():
@@ -420,11 +452,13 @@ def _time_reference_decks():
     """The reference's own steady decks with published CPU times (Askervein:
     175 iterations, 2 s on two ranks; Neutral_ABL_Flat: about 2850
     iterations, 13 s on two ranks; Neutral_Hill_3D: 2092 iterations, 148 s
-    on four ranks), each from its initializer state."""
+    on four ranks; the same with line LU-SGS: 823 iterations, 51 s on four
+    ranks, 860 on one box), each from its initializer state."""
     import torch
     from ablsteadyrans_amd import decks as D
     out = []
-    for deck in (D.askervein(_askervein_grid()), D.neutral_abl_flat(), D.neutral_hill_3d()):
+    for deck in (D.askervein(_askervein_grid()), D.neutral_abl_flat(), D.neutral_hill_3d(),
+                 D.neutral_hill_3d_lusgs()):
         D.prepare(deck)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -608,7 +642,23 @@ def _cpu_c2_baseline(n=99, fp64_iterations=None):
                       f"tile of the Gaussian-hill workload in {dt:.1f} s on 1 thread, scaled by the cell ratio "
                       f"{scale:g} and the {its} iterations of the plain FP64 solve; host CPU {model}, "
                       f"{ncpu} logical CPUs",
-            "s_per_iteration_200x200x64": round(per_it, 2), "cpu_model": model, "host_logical_cpus": ncpu}
+            "s_per_iteration_200x200x64": round(per_it, 2), "cpu_model": model, "host_logical_cpus": ncpu,
+            "reference_published": _reference_cpu_rate(its)}
+
+
+def _reference_cpu_rate(its):
+    """The reference's own CPU throughput as its documentation publishes it
+    (docs/results.rst:200-201: Neutral_Hill_3D, 64 x 64 x 20 cells, 16
+    z-line Jacobi sweeps, 2092 iterations in 148 s on four ranks), in cell
+    iterations per second per rank, and what it implies for the 200 x 200 x
+    64 solve of `its` iterations on one rank and on this host's threads
+    (perfect scaling assumed: an upper bound for the reference)."""
+    rate = 2092 * 64 * 64 * 20 / (148.0 * 4)
+    cells = 200 * 200 * 64
+    threads = _cpu_threads()
+    return {"cell_iterations_per_s_per_rank": round(rate), "source": "docs/results.rst:200-201",
+            "c2_steady_solves_per_s_1_rank": float(f"{rate / (cells * its):.4g}"),
+            f"c2_steady_solves_per_s_{threads}_ranks": float(f"{rate * threads / (cells * its):.4g}")}
 
 
 def _traffic_from_profile():

@@ -64,6 +64,9 @@ void oracle_solve_column(const oa4* a, const oa4* dg, const oa4* r, const oa4* d
  * Returns 0 and the last sweep's max |change| in *change, or -1. */
 int oracle_zline_jacobi(int loc, const oa4* A, const oa4* ap, const oa4* rhs, int sweeps, const ogeom* g,
                         oa4* delta, double* change);
+/* The same on `threads` OpenMP threads (the columns split; the same bits). */
+int oracle_zline_jacobi_threads(int loc, const oa4* A, const oa4* ap, const oa4* rhs, int sweeps, const ogeom* g,
+                                oa4* delta, double* change, int threads);
 
 /* Line LU-SGS (Source/LineSweeps.cpp:181-305, 415-428): lagged or zero
  * ghosts, overlap, sequential or diagonal order, alternate 0/1.

@@ -225,27 +225,38 @@ static int check_line_system(const char* op, int loc, const oa4* A, const oa4* a
     return 0;
 }
 
-int oracle_zline_jacobi(int loc, const oa4* A, const oa4* ap, const oa4* rhs, int sweeps, const ogeom* g,
-                        oa4* delta, double* change)
+/* The z-line Jacobi sweeps on `threads` host threads: within a sweep every
+ * column reads the previous iterate only, so the columns split across the
+ * threads (OpenMP) with the serial order's bits; the change is a maximum
+ * (exact in any order). */
+static int zline_jacobi_impl(int loc, const oa4* A, const oa4* ap, const oa4* rhs, int sweeps, const ogeom* g,
+                             oa4* delta, double* change, int threads)
 {
     obox V;
     if (check_line_system("zline_jacobi", loc, A, ap, rhs, sweeps, g, delta, &V)) { return -1; }
+    if (threads < 1) { threads = 1; }
     oa4 c = fab_new(V, 1), e = fab_new(V, 1), next = fab_new(V, 1);
     double ch = 0;
+    const int ilo = V.lo[0], jlo = V.lo[1], nxv = V.hi[0] - V.lo[0] + 1, nyv = V.hi[1] - V.lo[1] + 1;
+    const long ncol = (long)nxv * nyv;
     for (int s = 0; s < sweeps; ++s) {
         fill_boundary(delta, &V, g);
-        for (int j = V.lo[1]; j <= V.hi[1]; ++j)
-            for (int i = V.lo[0]; i <= V.hi[0]; ++i) {
-                oracle_solve_column(A, ap, rhs, delta, &c, &e, &next, i, j, V.lo[2], V.hi[2]);
-            }
+#pragma omp parallel for num_threads(threads) schedule(static) if (threads > 1)
+        for (long t = 0; t < ncol; ++t) {
+            const int i = ilo + (int)(t % nxv), j = jlo + (int)(t / nxv);
+            oracle_solve_column(A, ap, rhs, delta, &c, &e, &next, i, j, V.lo[2], V.hi[2]);
+        }
         if (s == sweeps - 1) {
-            ch = 0;
-            for (int k = V.lo[2]; k <= V.hi[2]; ++k)
-                for (int j = V.lo[1]; j <= V.hi[1]; ++j)
-                    for (int i = V.lo[0]; i <= V.hi[0]; ++i) {
-                        const double dlt = fabs(AT(&next, i, j, k, 0) - AT(delta, i, j, k, 0));
-                        if (dlt > ch) { ch = dlt; }
-                    }
+            double m = 0;
+#pragma omp parallel for num_threads(threads) schedule(static) reduction(max : m) if (threads > 1)
+            for (long t = 0; t < ncol; ++t) {
+                const int i = ilo + (int)(t % nxv), j = jlo + (int)(t / nxv);
+                for (int k = V.lo[2]; k <= V.hi[2]; ++k) {
+                    const double dlt = fabs(AT(&next, i, j, k, 0) - AT(delta, i, j, k, 0));
+                    if (dlt > m) { m = dlt; }
+                }
+            }
+            ch = m;
         }
         copy_box(delta, &next, &V, 1);
     }
@@ -255,6 +266,18 @@ int oracle_zline_jacobi(int loc, const oa4* A, const oa4* ap, const oa4* rhs, in
     fab_free(&next);
     if (change) { *change = ch; }
     return 0;
+}
+
+int oracle_zline_jacobi(int loc, const oa4* A, const oa4* ap, const oa4* rhs, int sweeps, const ogeom* g,
+                        oa4* delta, double* change)
+{
+    return zline_jacobi_impl(loc, A, ap, rhs, sweeps, g, delta, change, 1);
+}
+
+int oracle_zline_jacobi_threads(int loc, const oa4* A, const oa4* ap, const oa4* rhs, int sweeps, const ogeom* g,
+                                oa4* delta, double* change, int threads)
+{
+    return zline_jacobi_impl(loc, A, ap, rhs, sweeps, g, delta, change, threads);
 }
 
 /* One pass over the columns of S, in place in d (Source/LineSweeps.cpp:38-155).

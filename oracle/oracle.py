@@ -89,6 +89,8 @@ def lib():
         L.oracle_last_lusgs_launches.restype = C.c_uint64
         L.oracle_zline_jacobi.argtypes = [C.c_int, P(OA4), P(OA4), P(OA4), C.c_int, P(OGeom), P(OA4),
                                           P(C.c_double)]
+        L.oracle_zline_jacobi_threads.argtypes = [C.c_int, P(OA4), P(OA4), P(OA4), C.c_int, P(OGeom), P(OA4),
+                                                  P(C.c_double), C.c_int]
         L.oracle_zline_lusgs.argtypes = [C.c_int, P(OA4), P(OA4), P(OA4), C.c_int, P(OGeom), P(OA4), C.c_int,
                                          C.c_int, C.c_int, C.c_int, P(C.c_double)]
         L.oracle_poisson_error.restype = C.c_char_p
@@ -199,11 +201,17 @@ def _check(rc):
         raise OracleError(lib().oracle_last_error().decode())
 
 
-def zline_jacobi(loc, A, A_begin, ap, rhs, v_begin, sweeps, geom, delta, d_begin):
-    """Source/Predictor.cpp:87-127. Arrays are modified in place (delta)."""
+def zline_jacobi(loc, A, A_begin, ap, rhs, v_begin, sweeps, geom, delta, d_begin, threads=1):
+    """Source/Predictor.cpp:87-127. Arrays are modified in place (delta).
+    threads > 1: the columns split over that many OpenMP threads (the same
+    bits)."""
     ch = C.c_double(0)
-    _check(lib().oracle_zline_jacobi(loc, a4(A, A_begin), a4(ap, v_begin), a4(rhs, v_begin), sweeps, geom,
-                                     a4(delta, d_begin), C.byref(ch)))
+    if threads > 1:
+        _check(lib().oracle_zline_jacobi_threads(loc, a4(A, A_begin), a4(ap, v_begin), a4(rhs, v_begin), sweeps,
+                                                 geom, a4(delta, d_begin), C.byref(ch), int(threads)))
+    else:
+        _check(lib().oracle_zline_jacobi(loc, a4(A, A_begin), a4(ap, v_begin), a4(rhs, v_begin), sweeps, geom,
+                                         a4(delta, d_begin), C.byref(ch)))
     return ch.value
 
 

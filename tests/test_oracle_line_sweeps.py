@@ -311,3 +311,16 @@ def test_case_grid_is_covered():
         A, ap, rhs = L.random_system(n, loc, periodic, seed=1)
         x, ch, _ = L.oracle_jacobi(n, loc, periodic, A, ap, rhs, 2)
         assert ch > 0 and np.max(np.abs(x)) > 0.1
+
+
+@pytest.mark.parametrize("n,periodic", [((6, 5, 4), (1, 1)), ((33, 17, 12), (1, 0)), ((20, 21, 9), (0, 0))])
+@pytest.mark.parametrize("loc", LOCS)
+def test_threaded_jacobi_is_the_serial_jacobi_bitwise(n, periodic, loc):
+    """The CPU baseline's threaded sweeps (the columns split over OpenMP
+    threads) give the serial oracle's bits and change."""
+    A, ap, rhs = L.random_system(n, loc, periodic, seed=40 + loc)
+    ref, ref_ch, ref_d = L.oracle_jacobi(n, loc, periodic, A, ap, rhs, 3)
+    for threads in (2, 3, 8):
+        d, db = L.new_delta(n, loc)
+        ch = orc.zline_jacobi(loc, A, (0, 0, 0), ap, rhs, (0, 0, 0), 3, L.geom(n, periodic), d, db, threads=threads)
+        assert np.array_equal(d, ref_d) and ch == ref_ch, threads
