@@ -480,7 +480,7 @@ def _profile_deck(name, iterations):
     import torch
     from ablsteadyrans_amd import decks as D
     make = {"column": D.neutral_column, "askervein": lambda: D.askervein(_askervein_grid()),
-            "abl": D.neutral_abl_flat, "hill3d": D.neutral_hill_3d}[name]
+            "abl": D.neutral_abl_flat, "hill3d": D.neutral_hill_3d, "hill3d_lusgs": D.neutral_hill_3d_lusgs}[name]
     deck = D.prepare(make())
     deck.driver.max_iterations = 10
     deck.solve()
@@ -682,7 +682,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving and batched legs")
     ap.add_argument("--profile-only", action="store_true", help="sweeps only (for rocprofv3 runs)")
-    ap.add_argument("--profile-deck", choices=("column", "askervein", "abl", "hill3d", "c2", "c2mixed"),
+    ap.add_argument("--profile-deck", choices=("column", "askervein", "abl", "hill3d", "hill3d_lusgs", "c2", "c2mixed"),
                     help="only N driver iterations of one reference deck (for rocprofv3 runs)")
     ap.add_argument("--iterations", type=int, default=100, help="driver iterations of --profile-deck")
     ap.add_argument("--copy-calibration", action="store_true",
