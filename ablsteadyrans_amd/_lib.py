@@ -499,6 +499,7 @@ _sig = {
                                       C.c_void_p]),
     "gust_simplec_destroy": (C.c_int, [C.c_void_p]),
     "gust_simplec_init": (C.c_int, [C.c_void_p, _P(StateC), C.c_void_p]),
+    "gust_simplec_reset": (C.c_int, [C.c_void_p, C.c_void_p]),
     "gust_simplec_iterate": (C.c_int, [C.c_void_p, _P(StateC), _P(Array4), _P(SimplecInfoC), C.c_void_p]),
     "gust_simplec_norms": (C.c_int, [C.c_void_p, _P(C.c_double), _P(C.c_double), _P(C.c_int32)]),
     "gust_simplec_nu": (C.c_int, [C.c_void_p, _P(Array4)]),

@@ -421,6 +421,19 @@ int gust_simplec_init(gust_simplec* s, gust_state* q, void* stream)
     });
 }
 
+int gust_simplec_reset(gust_simplec* s, void* stream)
+{
+    return guarded([&] {
+        if (!s) { GUST_FAIL("gust: SimplecStage::reset: null stage"); }
+        SimplecStage& st = *as(s);
+        hipStream_t h = static_cast<hipStream_t>(stream);
+        for (DevField& f : st.cf) {
+            if (f.p) { GUST_HIP(hipMemsetAsync(f.p, 0, size_t(f.v.nstride) * f.v.ncomp * sizeof(double), h)); }
+        }
+        GUST_HIP(hipStreamSynchronize(h));
+    });
+}
+
 int gust_simplec_iterate(gust_simplec* s, gust_state* q, const gust_array4* dtau, gust_simplec_info* info,
                          void* stream)
 {

@@ -280,7 +280,10 @@ def prepare(deck: Deck, precision: int = 0) -> Deck:
 
 
 def reinitialize(deck: Deck) -> None:
-    """The state back at the deck's first guess (the stage and mesh kept)."""
+    """The state back at the deck's first guess (the stage and mesh kept,
+    the closure's history cleared: the next solve is a fresh run's)."""
+    if deck.simplec is not None:
+        deck.simplec.reset()
     if deck.init.projection:
         solver = PS.PoissonSolver(deck.mesh, PS.PoissonOptions(reltol=deck.init_reltol))
         GI.initialize_state(deck.mesh, deck.profile, deck.init, deck.inputs.faces, solver, deck.state)

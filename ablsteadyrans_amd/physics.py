@@ -380,6 +380,11 @@ class SimplecStage:
         qc = q.c()
         _lib.check(_lib.lib.gust_simplec_init(self._h, C.byref(qc), _stream(stream)))
 
+    def reset(self, stream=None) -> None:
+        """Forget the closure's history: the next run starts as on a fresh
+        stage."""
+        _lib.check(_lib.lib.gust_simplec_reset(self._h, _stream(stream)))
+
     def iterate(self, q: DeviceState, dtau: Fab, stream=None):
         """Source/Simplec.cpp:452-506; returns the iteration's info."""
         qc = q.c()

@@ -905,6 +905,13 @@ int gust_simplec_destroy(gust_simplec* s);
  * Replaces: void SimplecStage::init(State&) -- Source/Simplec.cpp:338-359. */
 int gust_simplec_init(gust_simplec* s, gust_state* q, void* stream);
 
+/* The closure's history forgotten (its fields zero, as on a fresh stage), so
+ * that a new run on this stage starts as a fresh one: the driver's first step
+ * limit reads the diffusivity of the last closure fields. Synchronises.
+ * Replaces: constructing a new SimplecStage for the next run (the reference's
+ *   application builds one per run, Source/Simplec.cpp:293-336). */
+int gust_simplec_reset(gust_simplec* s, void* stream);
+
 /* One SIMPLEC iteration with the cell pseudo-time step dtau: ghosts and
  * surface state, R(q) and the norms, the momentum predictor, the ghosts,
  * the pressure correction, theta's convection at the corrected transport,

@@ -319,3 +319,24 @@ def test_a_bad_check_in_an_iteration_reports_the_first_failure(cuda):
     res, _ = stage.norms()
     assert all(np.isfinite(r) for r in res)
     assert np.isfinite(info.pressure.divergence_after)
+
+
+@pytest.mark.gpu
+def test_a_reinitialized_deck_runs_as_a_fresh_one(cuda):
+    """decks.reinitialize clears the closure's history (gust_simplec_reset):
+    a second run on the same stage from the first guess gives the fresh
+    stage's bits (the driver's first step limit reads the closure's nu)."""
+    from ablsteadyrans_amd import decks as D
+    from ablsteadyrans_amd.physics import FIELDS
+    runs = []
+    for reuse in (False, True):
+        deck = D.prepare(D.neutral_column())
+        deck.driver.max_iterations = 60
+        if reuse:
+            deck.solve()
+            D.reinitialize(deck)
+        r = deck.solve()
+        runs.append((int(r.iterations), {n: deck.state[n].numpy().copy() for n in FIELDS}))
+    assert runs[0][0] == runs[1][0] == 60
+    for n in FIELDS:
+        assert np.array_equal(runs[0][1][n], runs[1][1][n]), n
