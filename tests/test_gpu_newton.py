@@ -3,10 +3,16 @@ NewtonPreconditioner.*, NewtonStage.*): the homogeneous ghost fill bit for
 bit against the oracle's; J v against the oracle's forward difference of
 the same residual (the same eps, bit for bit) and exactly linear in powers of
 two; continuation steps on the reference's Neutral_Column deck from a
-SIMPLEC state converging ten orders at the Krylov tolerance's rate; and the steady.stage = simplec_newton run of
-that deck reaching the deck's criterion in fewer iterations than SIMPLEC
-alone, at the state the oracle's composite converges to
-(tests/golden/column_steady.npz) within 1e-6 relative L2."""
+SIMPLEC state converging ten orders at the Krylov tolerance's rate; the
+steady.stage = simplec_newton run of that deck reaching the deck's criterion
+in fewer iterations than SIMPLEC alone, at the state the oracle's composite
+converges to (tests/golden/column_steady.npz) within 1e-6 relative L2; the
+preconditioner with the stage's denominators equal to one SIMPLEC iteration
+(NewtonPreconditioner.ApplyIsOneSimplecIteration); its GMRES iterations
+bounded over CFL with the diagonal denominators
+(NewtonPreconditioner.IterationsBoundedOverCfl); and the continuation with
+either preconditioner reaching the same state
+(NewtonPreconditioner.ContinuationIdentityVersusSimplec)."""
 from __future__ import annotations
 
 import os
