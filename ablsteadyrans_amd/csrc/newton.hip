@@ -207,15 +207,6 @@ __global__ __launch_bounds__(kThreads) void scale_copy_kernel(int64_t n, double 
     }
 }
 
-// y += a x
-__global__ __launch_bounds__(kThreads) void axpy_kernel(int64_t n, double a, const double* __restrict__ x,
-                                                        double* __restrict__ y)
-{
-    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
-        y[t] += a * x[t];
-    }
-}
-
 // w += sign * sum_j c[j] V_j (c on the device)
 __global__ __launch_bounds__(kThreads) void multi_axpy_kernel(int64_t n, int nv, const double* __restrict__ Vb,
                                                               int64_t ldv, const double* __restrict__ c, double sign,
@@ -245,7 +236,7 @@ __global__ __launch_bounds__(kThreads) void mdot_kernel(int64_t n, const double*
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int h = kThreads / 2; h > 0; h >>= 1) {
-        if (threadIdx.x < h) { sh[threadIdx.x] += sh[threadIdx.x + h]; }
+        if (int(threadIdx.x) < h) { sh[threadIdx.x] += sh[threadIdx.x + h]; }
         __syncthreads();
     }
     if (threadIdx.x == 0) { partial[j * kDotBlocks + blockIdx.x] = sh[0]; }
@@ -261,7 +252,7 @@ __global__ __launch_bounds__(kThreads) void mdot_final_kernel(const double* __re
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int h = kThreads / 2; h > 0; h >>= 1) {
-        if (threadIdx.x < h) { sh[threadIdx.x] += sh[threadIdx.x + h]; }
+        if (int(threadIdx.x) < h) { sh[threadIdx.x] += sh[threadIdx.x + h]; }
         __syncthreads();
     }
     if (threadIdx.x == 0) { out[j] = sh[0]; }
@@ -288,7 +279,7 @@ __global__ __launch_bounds__(kThreads) void range_kernel(int64_t n, const double
     snan[threadIdx.x] = nan;
     __syncthreads();
     for (int h = kThreads / 2; h > 0; h >>= 1) {
-        if (threadIdx.x < h) {
+        if (int(threadIdx.x) < h) {
             smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + h]);
             smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + h]);
             snan[threadIdx.x] = fmax(snan[threadIdx.x], snan[threadIdx.x + h]);
@@ -500,13 +491,6 @@ double ser(double cfl, double r, double cfl0, double cfl_max, double p, double g
     double out = 0;
     check_rc(gust_ser_cfl(cfl, r, cfl0, cfl_max, p, growth, &out));
     return out;
-}
-
-std::string sci(double x)
-{
-    char b[32];
-    std::snprintf(b, sizeof(b), "%.3e", x);
-    return b;
 }
 
 }   // namespace
@@ -976,7 +960,6 @@ struct NewtonSolver {
         dirichlet_take_kernel<<<grid_for(pk.off[3]), kThreads, 0, st>>>(pk.off[3], dir.p, r, out);
         GUST_HIP(hipGetLastError());
         seam_override(out, st);
-        (void)cf;
     }
 
     // Rp_f = a * (packed r's field f) on the valid points
