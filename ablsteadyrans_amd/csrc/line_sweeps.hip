@@ -1719,7 +1719,7 @@ __global__ __launch_bounds__(256) void lusgs_skew_kernel(const LusgsBatch<T> B, 
     const T* __restrict__ Lz = S.sk[kSkLz];
     const T* __restrict__ Mm = S.sk[kSkM];
     const T* __restrict__ Cc = S.sk[kSkC];
-    const Rsrc rL = rsrc_of(Lz), rM = rsrc_of(Mm), rC = rsrc_of(Cc), rW = rsrc_of(W);
+    const Rsrc rW = rsrc_of(W);
     const int last = nx + ny - 2;
     unsigned long long chg = 0;
     for (int s = 0; s < sweeps; ++s) {
