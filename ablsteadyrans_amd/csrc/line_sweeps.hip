@@ -2185,6 +2185,13 @@ void zline_lusgs_into(int nsys, const gust_line_system* sys, const gust_line_opt
     lusgs_batch<double>(nsys, sys, opt, sweeps, changes, st);
 }
 
+// The same on FP32 systems (the views' pointers are float data).
+void zline_lusgs_into_f32(int nsys, const gust_line_system* sys, const gust_line_options* opt, int sweeps,
+                          unsigned long long* changes, hipStream_t st)
+{
+    lusgs_batch<float>(nsys, sys, opt, sweeps, changes, st);
+}
+
 // A batch of z-line Jacobi systems whose last-sweep changes go straight
 // into `changes` (nsys slots that hold zeros, as ordered bits of |change|).
 void zline_jacobi_into(int nsys, const gust_line_system* sys, int sweeps, unsigned long long* changes, hipStream_t st)

@@ -729,10 +729,9 @@ void scalar_sweeps_theta_tke(const gust_geometry& g, gust_array4* theta, const g
                              gust_scalar_sweep_result* res_tke, hipStream_t st, int sweep_precision)
 {
     check_rc(gust_check_line_options(&sweep));
-    // LU-SGS runs in FP64 whatever the sweep precision; both kinds solve
-    // theta and k in shared launches (their systems come from R, A and dtau)
+    // both kinds solve theta and k in shared launches (their systems come
+    // from R, A and dtau), in FP64 or (sweep_precision 1) FP32
     const bool lusgs = sweep.kind == GUST_SWEEP_LUSGS;
-    if (lusgs) { sweep_precision = 0; }
     const char* op = "scalar_sweep";
     check_field(op, "field", theta, g, GUST_LOC_CELL, 1);
     check_field(op, "field", tke, g, GUST_LOC_CELL, 1);
@@ -775,7 +774,10 @@ void scalar_sweeps_theta_tke(const gust_geometry& g, gust_array4* theta, const g
         });
     }
     gust_line_system s32[2];
-    if (lusgs) {
+    if (lusgs && sweep_precision == 1) {
+        systems_f32(2, sys, s32, st);
+        zline_lusgs_into_f32(2, s32, &sweep, sweeps, dq->zero_run(std::move(fs)), st);
+    } else if (lusgs) {
         zline_lusgs_into(2, sys, &sweep, sweeps, dq->zero_run(std::move(fs)), st);
     } else if (sweep_precision == 1) {
         systems_f32(2, sys, s32, st);
@@ -992,9 +994,6 @@ int gust_momentum_predictor(const gust_geometry* geom, gust_faces* vel, const gu
         if (opt->sweep_precision != 0 && opt->sweep_precision != 1) {
             GUST_FAIL("gust: momentum_predictor: sweep_precision must be 0 (FP64) or 1 (FP32 sweeps)");
         }
-        if (opt->sweep_precision == 1 && opt->sweep.kind != GUST_SWEEP_ZJACOBI) {
-            GUST_FAIL("gust: momentum_predictor: FP32 sweeps are z-line Jacobi sweeps (steady.sweep = zjacobi)");
-        }
         double* dmax[3] = {nullptr, nullptr, nullptr};
         double* last[3] = {nullptr, nullptr, nullptr};
         if (result) {
@@ -1123,9 +1122,11 @@ int gust_momentum_predictor(const gust_geometry* geom, gust_faces* vel, const gu
                 });
             }
             gust_line_system s32[3];
-            // LU-SGS runs in FP64 whatever the sweep precision
-            const bool f32 = opt->sweep_precision == 1 && !lusgs;
-            if (lusgs) {
+            const bool f32 = opt->sweep_precision == 1;
+            if (lusgs && f32) {
+                systems_f32(3, sys, s32, st);
+                zline_lusgs_into_f32(3, s32, &opt->sweep, opt->sweeps, dq->zero_run(std::move(fs)), st);
+            } else if (lusgs) {
                 zline_lusgs_into(3, sys, &opt->sweep, opt->sweeps, dq->zero_run(std::move(fs)), st);
             } else if (f32) {
                 systems_f32(3, sys, s32, st);

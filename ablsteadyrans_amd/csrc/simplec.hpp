@@ -27,5 +27,8 @@ void zline_jacobi_into_f32(int nsys, const gust_line_system* sys, int sweeps, un
 // zline_jacobi_into's.
 void zline_lusgs_into(int nsys, const gust_line_system* sys, const gust_line_options* opt, int sweeps,
                       unsigned long long* changes, hipStream_t st);
+// The same on FP32 systems.
+void zline_lusgs_into_f32(int nsys, const gust_line_system* sys, const gust_line_options* opt, int sweeps,
+                          unsigned long long* changes, hipStream_t st);
 
 }   // namespace gust_amd

@@ -116,9 +116,6 @@ void check_options(const char* op, const gust_simplec_options& o)
     if (o.sweep_precision != 0 && o.sweep_precision != 1) {
         GUST_FAIL("gust: %s: sweep_precision must be 0 (FP64) or 1 (FP32 sweeps)", op);
     }
-    if (o.sweep_precision == 1 && o.sweep.kind != GUST_SWEEP_ZJACOBI) {
-        GUST_FAIL("gust: %s: FP32 sweeps are z-line Jacobi sweeps (steady.sweep = zjacobi)", op);
-    }
     if (std::isnan(o.tke_floor) || !(o.tke_floor >= 0)) {
         GUST_FAIL("gust: %s: tke_floor must be >= 0 (turb.tke_floor)", op);
     }

@@ -487,10 +487,10 @@ typedef struct gust_predictor_options {
     double coriolis_f;
     double gravity, theta0;
     gust_line_options sweep;
-    /* 0: the z-line Jacobi sweeps in FP64 (the reference); 1: mixed precision,
-     * the sweeps' operands (A, a_P, rhs) rounded to FP32 and solved in FP32,
-     * the change added to the FP64 field (a defect correction: R stays FP64,
-     * so the converged field is the FP64 one). Jacobi sweeps only. */
+    /* 0: the line sweeps in FP64 (the reference); 1: mixed precision, the
+     * sweeps' operands (A, a_P, rhs) rounded to FP32 and solved in FP32 (z-line
+     * Jacobi or line LU-SGS), the change added to the FP64 field (a defect
+     * correction: R stays FP64, so the converged field is the FP64 one). */
     int32_t sweep_precision;
 } gust_predictor_options;
 

@@ -340,3 +340,49 @@ def test_a_reinitialized_deck_runs_as_a_fresh_one(cuda):
     assert runs[0][0] == runs[1][0] == 60
     for n in FIELDS:
         assert np.array_equal(runs[0][1][n], runs[1][1][n]), n
+
+
+@pytest.mark.gpu
+def test_lusgs_in_mixed_precision_follows_fp64(cuda):
+    """steady.sweep = lusgs in the mixed form: the line LU-SGS systems rounded
+    to FP32 and solved in FP32 inside the FP64 defect correction (predictor,
+    theta and k). Three iterations on the stratified, rotating hill stay
+    within 1e-5 of each field's scale of the FP64 LU-SGS iterations, and
+    differ from them (the FP32 path ran)."""
+    import torch
+    from ablsteadyrans_amd import physics as GP
+    from ablsteadyrans_amd.fab import Fab, LineSweepOptions, LUSGS
+    from ablsteadyrans_amd.poisson import DeviceMesh
+    tile = PHY.hill_mesh(n=(16, 12, 10), height=60.0, half=220.0)
+    m = PHY.omesh(tile)
+    q = PHY.random_state(tile, 11, U=6.0, amp=0.05)
+    phys = P.Physics(closure=P.ClosureOptions(tke_floor=1e-6),
+                     bc=P.Boundary(zlo=P.SURFACE_LAYER, zhi=P.SLIP_WALL, zhi_theta_grad=0.003),
+                     surface=P.SurfaceOptions(z0=0.1),
+                     forcing=P.Forcing(coriolis=True, latitude=45.0, geostrophic=True, ug=6.0, vg=1.0),
+                     theta_ref_z=(0.0, 1000.0), theta_ref=(300.0, 303.0))
+    P.fill_ghosts(q, m, phys)
+    g = m.geom
+    out = []
+    for prec in (0, 1):
+        dm = DeviceMesh(tile)
+        stage = GP.SimplecStage(dm, _inputs(phys), GP.SimplecOptions(pressure_reltol=1e-12, sweeps=2,
+                                                                     tke_floor=1e-6, height=600.0,
+                                                                     sweep=LineSweepOptions(kind=LUSGS),
+                                                                     sweep_precision=prec))
+        ds = _state(tile, q)
+        stage.init(ds)
+        for it in range(3):
+            q1 = {n: ds[n].numpy()[:, 1:-1, 1:-1, 1:-1].copy() for n in GP.FIELDS}
+            dtau = orc.local_dtau(q1["u"], q1["v"], q1["w"], B1, m.z, B1, m.J, B1, g, 1.0 + it)
+            stage.iterate(ds, Fab(torch.from_numpy(dtau[None].copy()).cuda(), tuple(g.lo)))
+        out.append({n: ds[n].numpy()[0, 2:-2, 2:-2, 2:-2].copy() for n in GP.FIELDS})
+    differs = False
+    for n in GP.FIELDS:
+        a, b = out[1][n], out[0][n]
+        scale = max(np.abs(b).max(), 1e-30)
+        err = np.abs(a - b).max() / scale
+        print(f"  {n}: max |FP32 LU-SGS - FP64 LU-SGS| / scale = {err:.2e}", flush=True)
+        assert err <= 1e-5, (n, err)
+        differs = differs or err > 0
+    assert differs
