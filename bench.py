@@ -24,13 +24,18 @@ each fast form's converged fields against the plain FP64 ones is reported
 (extra.fast_vs_fp64).
 
 Also measured (extra): the reference's Neutral_Column deck (the previous
-headline: 4 x 4 x 64, 2231 iterations) FP64 and mixed; the reference's own
+headline: 4 x 4 x 64, 2231 iterations) FP64 and mixed, and with the
+Newton-Krylov finish (steady.stage = simplec_newton); the reference's own
 steady decks with published CPU times (Askervein, Neutral_ABL_Flat,
-Neutral_Hill_3D); the z-line Jacobi sweep of the five variables on the
+Neutral_Hill_3D with z-line Jacobi, with line LU-SGS and with three levels of
+grid sequencing); the z-line Jacobi sweep of the five variables on the
 fractal-terrain tile (256 x 256 x 96, 65,536 columns; the roofline object:
 algorithmic bytes per launch / launch time from HIP events on the launch
 stream, 10 reals per unknown); the V(2,2) cycle. Serving and batched legs
-(extra.serving, extra.batched). cpu_baseline: the oracle on the host cores.
+(extra.serving, extra.batched). cpu_baseline: the oracle on the host cores
+(the composite iteration on one thread, with the reference's published CPU
+rate beside it; extra.cpu_sweep_baseline: the compiled oracle sweep on one
+thread and on the host's threads).
 """
 from __future__ import annotations
 
